@@ -1,0 +1,257 @@
+"""Benchmark: rows/s and HBM-roofline fraction of the fused query pipelines on
+one MI355X, with the CPU oracle timed beside it.
+
+Headline (the JSON line's `value`): the three-way hash join + group-by +
+top-10 workload (A 1.5M, B 15M, F ~60M rows) in its plain per-query form:
+semi-join bitmap, exact-count sizing pre-pass, hash build, probe with
+per-key revenue, top-10 — every phase of one query is one step. Inputs are
+resident in HBM before the timed region. Other pipelines are reported under
+"pipelines" with the same fields.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--skip-extra] [--no-cpu]
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from gpudbmetalcodegen_amd import engine as E  # noqa: E402
+from gpudbmetalcodegen_amd import queries as Q  # noqa: E402
+from gpudbmetalcodegen_amd import workloads as W  # noqa: E402
+
+PEAK_HBM_GBS = 8000.0
+METRIC = "rows/s + % HBM roofline per pipeline; p50/p99 latency; bit-exact vs plain form"
+COCACHE = ROOT / "gpudbmetalcodegen_amd" / "_cocache"
+
+
+def log(msg: str) -> None:
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def host_threads() -> int:
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def traffic_for(kernel: str, workload: str):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE/WRITE_SIZE
+    (profiles/traffic.json, scaled by the copy-kernel calibration)."""
+    p = ROOT / "profiles" / "traffic.json"
+    if not p.exists():
+        return None
+    try:
+        d = json.loads(p.read_text())
+        return d.get(workload, {}).get(kernel)
+    except Exception:
+        return None
+
+
+def roofline(phase: str, phase_ms: float, alg_bytes: float, workload: str) -> dict:
+    achieved = alg_bytes / (phase_ms / 1e3) / 1e9 if phase_ms > 0 else 0.0
+    return {"kernel": phase, "bound": "hbm", "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": round(achieved / PEAK_HBM_GBS, 4), "traffic": traffic_for(phase, workload),
+            "algorithmic_bytes": int(alg_bytes), "kernel_ms": round(phase_ms, 4)}
+
+
+# ---------------------------------------------------------------------------
+# Workload 1 — three-way join (headline)
+# ---------------------------------------------------------------------------
+def q3_phase_bytes(T: dict, seg: int, d: int, b_slots: int) -> dict:
+    A, B, F = T["A"], T["B"], T["F"]
+    a_sel = A["a_seg"] == seg
+    aset = np.zeros(int(A["a_key"].max()) + 2, bool)
+    aset[A["a_key"][a_sel]] = True
+    b_date_ok = B["b_date"] < d
+    b_ok = b_date_ok & aset[B["b_fkey"]]
+    bset = np.zeros(int(B["b_key"].max()) + 2, bool)
+    bset[B["b_key"][b_ok]] = True
+    f_date_ok = F["f_date"] > d
+    f_hit = f_date_ok & bset[F["f_fkey"]]
+    n_a, n_b, n_f = A["a_key"].shape[0], B["b_key"].shape[0], F["f_fkey"].shape[0]
+    return {
+        "Q3_build_a_bitmap": 5 * n_a + 4 * int(a_sel.sum()),
+        "Q3_count_b": 8 * n_b + 4 * int(b_date_ok.sum()),
+        "Q3_build_b_table": 12 * n_b + 4 * int(b_date_ok.sum()) + (16 + 4) * int(b_ok.sum()),
+        "Q3_probe_f": 20 * n_f + 4 * int(f_date_ok.sum()) + (16 + 8) * int(f_hit.sum()),
+        "Q3_topk": 16 * b_slots,
+    }
+
+
+def bench_join(ctx, steps: int, warmup: int, cpu: bool) -> dict:
+    log("generating workload 1 (A 1.5M, B 15M, F ~60M)")
+    T = W.workload1()
+    dev = Q.upload_tables(ctx, {k: T[k] for k in ("A", "B", "F")})
+    rows = sum(dev[k].rows for k in ("A", "B", "F"))
+    run = Q.run_q3(ctx, dev, W.W1_SEG, W.W1_DATE)  # compile + first run
+    ex = run.executor
+    plan = run.plan
+    log(f"workload 1: {rows} rows; timing {steps} steps after {warmup} warmups")
+    if warmup > 0:
+        ex.execute(plan, warmup - 1, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    t = ex.execute(plan, 0, steps)
+    ctx.sync()
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    res = ex.collect(plan)
+    b_slots = ex.get_symbol("b_slots")
+    pb = q3_phase_bytes(T, W.W1_SEG, W.W1_DATE, b_slots)
+    phases = {name: ms for name, ms, *_ in t.phases}
+    dom = max(phases, key=phases.get)
+    out = {
+        "workload": "three-way hash join + group-by + top-10 (A 1.5M, B 15M, F %d rows)" % dev["F"].rows,
+        "rows": rows,
+        "ms_per_step": wall_ms,
+        "gpu_ms_per_step": t.total_ms,
+        "rows_per_s": rows / (wall_ms / 1e3),
+        "phase_ms": {k: round(v, 4) for k, v in phases.items()},
+        "phase_roofline_frac": {k: round(pb[k] / (phases[k] / 1e3) / 1e9 / PEAK_HBM_GBS, 4) for k in phases if phases[k] > 0},
+        "roofline": roofline(dom, phases[dom], pb[dom], "w1"),
+        "b_slots": b_slots,
+    }
+    gpu_top = {k: [int(x) for x in v] for k, v in res.columns.items()}
+    if cpu:
+        out["cpu_baseline"], out["bit_exact"] = cpu_leg_join(T, rows, gpu_top)
+    for tb in dev.values():
+        tb.free(ctx)
+    ex.release()
+    return out
+
+
+def cpu_leg_join(T, rows, gpu_top):
+    import oracle
+
+    th = host_threads()
+    A, B, F = T["A"], T["B"], T["F"]
+    times = []
+    exp = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        exp = oracle.q3_topk(A, B, F, W.W1_SEG, W.W1_DATE, 10, threads=th)
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[1]
+    exact = gpu_top == {k: v.tolist() for k, v in exp.items()}
+    return ({"value": rows / med, "unit": "rows/s", "cores": th, "kind": "port",
+             "sample": "full workload 1 (all A, B, F rows), median of 3 runs of the oracle's threaded join + top-10"},
+            exact)
+
+
+# ---------------------------------------------------------------------------
+# Workload 0 — fused filter + group-by (6M and 120M rows)
+# ---------------------------------------------------------------------------
+def bench_q1(ctx, n: int, steps: int, warmup: int, cpu: bool) -> dict:
+    log(f"generating workload 0 at {n} rows")
+    F = W.fact_f(n, W.seed_for(0))
+    dev = Q.upload_tables(ctx, {"F": F})
+    run = Q.run_q1(ctx, dev, W.q1_cutoff(), warmup=warmup, measured=1)
+    ex, plan = run.executor, run.plan
+    ctx.sync()
+    t0 = time.perf_counter()
+    t = ex.execute(plan, 0, steps)
+    wall_ms = (time.perf_counter() - t0) * 1e3 / steps
+    res = ex.collect(plan)
+    kernel_ms = t.phases[0][1]
+    out = {"workload": f"fused filter + group-by aggregate, {n} rows", "rows": n, "ms_per_step": wall_ms,
+           "gpu_ms_per_step": t.total_ms, "rows_per_s": n / (wall_ms / 1e3),
+           "kernel_rows_per_s": n / (kernel_ms / 1e3),
+           "roofline": roofline("Q1_reduce", kernel_ms, 26.0 * n, f"w0_{n}")}
+    if cpu:
+        out["cpu_baseline"], out["bit_exact"] = cpu_leg_q1(F, n, res)
+    dev["F"].free(ctx)
+    ex.release()
+    return out
+
+
+def cpu_leg_q1(F, n, res):
+    import oracle
+
+    th = host_threads()
+    times = []
+    words = None
+    for _ in range(3):
+        t0 = time.perf_counter()
+        words = oracle.q1(F, W.q1_cutoff(), threads=th)
+        times.append(time.perf_counter() - t0)
+    med = sorted(times)[1]
+    cnt = [int(w[6]) for w in words if w[6] != 0]
+    charge = [oracle.i128(w[3], w[4]) for w in words if w[6] != 0]
+    sums = [[int(w[0]), int(w[1]), int(w[2]), int(w[5])] for w in words if w[6] != 0]
+    c = res.columns
+    got_sums = [[int(a), int(b), int(x), int(y)] for a, b, x, y in
+                zip(c["sum_qty"], c["sum_base_price"], c["sum_disc_price"], c["sum_disc"])]
+    exact = (cnt == [int(x) for x in c["count_order"]] and charge == [int(x) for x in c["sum_charge"]]
+             and sums == got_sums)
+    return ({"value": n / med, "unit": "rows/s", "cores": th, "kind": "port",
+             "sample": f"all {n} rows, median of 3 runs of the oracle's threaded Q1"}, exact)
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--skip-extra", action="store_true", help="headline workload only")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a design "
+              "note only).", file=sys.stderr)
+        return 2
+    if E.device_count() < 1:
+        print("bench.py: no GPU visible", file=sys.stderr)
+        return 1
+    ctx = E.Context(0)
+    if COCACHE.exists():
+        ctx.set_cache_dir(str(COCACHE))
+    head = bench_join(ctx, args.steps, args.warmup, not args.no_cpu)
+    pipelines = {}
+    if not args.skip_extra:
+        for name, fn in [("w0_6M", lambda: bench_q1(ctx, 6_000_000, args.steps, args.warmup, not args.no_cpu)),
+                         ("w0_120M", lambda: bench_q1(ctx, 120_000_000, args.steps, args.warmup, not args.no_cpu))]:
+            try:
+                pipelines[name] = fn()
+            except Exception as e:  # report, keep the headline line
+                pipelines[name] = {"error": str(e)[:400]}
+            log(f"{name}: {json.dumps(pipelines[name])[:300]}")
+    line = {
+        "metric": METRIC,
+        "value": head["rows_per_s"],
+        "unit": "rows/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": head["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "int64",
+        "data": "synthetic (seeded counter-based generators)",
+        "config": {"workload": head["workload"], "rows_per_step": head["rows"], "form": "plain per-query",
+                   "device": ctx.info.name.decode(), "cus": ctx.cu_count},
+        "roofline": head["roofline"],
+        "cpu_baseline": head.get("cpu_baseline"),
+        "bit_exact": head.get("bit_exact"),
+        "phase_ms": head["phase_ms"],
+        "phase_roofline_frac": head["phase_roofline_frac"],
+        "pipelines": pipelines,
+    }
+    print(json.dumps(line), flush=True)
+    ctx.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -1,0 +1,407 @@
+#pragma once
+// Device runtime library prepended to every generated HIP source.
+//
+// Counterpart of codegen/core/metal_common_header.h:39-204, written for
+// gfx950 (CDNA4): 64-lane waves, DPP row operations and readlane instead of
+// 32-wide simd_shuffle trees, native 64-bit global/LDS atomics instead of
+// emulated lo/hi 32-bit pairs, 16-byte nontemporal column loads, and the hash
+// table, bitmap, LIKE and top-k helpers the operators emit calls to.
+
+namespace gdb {
+namespace codegen {
+
+inline const char* kGdbDeviceHeader = R"GDBHIP(
+// ===== gdb device runtime (gfx950, wave64) =====
+typedef signed char i8;
+typedef unsigned char u8;
+typedef short i16;
+typedef unsigned short u16;
+typedef int i32;
+typedef unsigned int u32;
+typedef long long i64;
+typedef unsigned long long u64;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+
+#define GDB_DEV __device__ __forceinline__
+#define GDB_I64_MAX 0x7FFFFFFFFFFFFFFFll
+#define GDB_I64_MIN (-0x7FFFFFFFFFFFFFFFll - 1ll)
+#define GDB_EMPTY_KEY 0xFFFFFFFFu
+#define GDB_EMPTY_WORD 0x00000000FFFFFFFFull
+#define GDB_KNUTH 2654435769u
+
+// ---------------------------------------------------------------------------
+// Column loads. A thread owns ITEMS consecutive rows of a tile, so a column of
+// T is read as sizeof(T)*ITEMS contiguous bytes with the widest aligned vector
+// load; consecutive lanes read consecutive chunks (coalesced). Loads are
+// nontemporal: a scan touches each column byte once, and keeping it out of the
+// caches leaves L2/MALL to hash tables and bitmaps.
+// ---------------------------------------------------------------------------
+template <typename V>
+GDB_DEV V gdb_ld_stream(const V* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+template <typename T, int N>
+GDB_DEV void gdb_load_items(const T* __restrict__ p, T (&r)[N]) {
+    constexpr int B = (int)sizeof(T) * N;
+    if constexpr (B % 16 == 0) {
+#pragma unroll
+        for (int k = 0; k < B / 16; ++k) {
+            u32x4 v = gdb_ld_stream(reinterpret_cast<const u32x4*>(p) + k);
+            __builtin_memcpy(reinterpret_cast<char*>(&r[0]) + 16 * k, &v, 16);
+        }
+    } else if constexpr (B % 8 == 0) {
+#pragma unroll
+        for (int k = 0; k < B / 8; ++k) {
+            u32x2 v = gdb_ld_stream(reinterpret_cast<const u32x2*>(p) + k);
+            __builtin_memcpy(reinterpret_cast<char*>(&r[0]) + 8 * k, &v, 8);
+        }
+    } else if constexpr (B % 4 == 0) {
+#pragma unroll
+        for (int k = 0; k < B / 4; ++k) {
+            u32 v = gdb_ld_stream(reinterpret_cast<const u32*>(p) + k);
+            __builtin_memcpy(reinterpret_cast<char*>(&r[0]) + 4 * k, &v, 4);
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < N; ++k) r[k] = p[k];
+    }
+}
+
+template <typename T, int N>
+GDB_DEV void gdb_load_tail(const T* __restrict__ base, u64 r0, u64 n, T (&r)[N]) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) r[k] = (r0 + (u64)k < n) ? base[r0 + k] : (T)0;
+}
+
+// ---------------------------------------------------------------------------
+// Wave64 reductions: DPP quad_perm [1,0,3,2], [2,3,0,1], row_ror:4, row_ror:8
+// leave every lane with its 16-lane row total; four readlanes combine the
+// rows. The wave must be fully active (block sizes are multiples of 64).
+// ---------------------------------------------------------------------------
+template <int CTRL>
+GDB_DEV u32 gdb_dpp32(u32 v) {
+    return (u32)__builtin_amdgcn_update_dpp((int)0, (int)v, CTRL, 0xF, 0xF, false);
+}
+template <int CTRL>
+GDB_DEV u64 gdb_dpp64(u64 v) {
+    const u32 lo = gdb_dpp32<CTRL>((u32)v);
+    const u32 hi = gdb_dpp32<CTRL>((u32)(v >> 32));
+    return ((u64)hi << 32) | (u64)lo;
+}
+GDB_DEV u64 gdb_readlane64(u64 v, int lane) {
+    const u32 lo = (u32)__builtin_amdgcn_readlane((int)(u32)v, lane);
+    const u32 hi = (u32)__builtin_amdgcn_readlane((int)(u32)(v >> 32), lane);
+    return ((u64)hi << 32) | (u64)lo;
+}
+GDB_DEV u64 gdb_wave_sum_u64(u64 v) {
+    v += gdb_dpp64<0xB1>(v);
+    v += gdb_dpp64<0x4E>(v);
+    v += gdb_dpp64<0x124>(v);
+    v += gdb_dpp64<0x128>(v);
+    return gdb_readlane64(v, 0) + gdb_readlane64(v, 16) + gdb_readlane64(v, 32) + gdb_readlane64(v, 48);
+}
+GDB_DEV i64 gdb_min64(i64 a, i64 b) { return a < b ? a : b; }
+GDB_DEV i64 gdb_max64(i64 a, i64 b) { return a > b ? a : b; }
+GDB_DEV i64 gdb_wave_min_i64(i64 v) {
+    v = gdb_min64(v, (i64)gdb_dpp64<0xB1>((u64)v));
+    v = gdb_min64(v, (i64)gdb_dpp64<0x4E>((u64)v));
+    v = gdb_min64(v, (i64)gdb_dpp64<0x124>((u64)v));
+    v = gdb_min64(v, (i64)gdb_dpp64<0x128>((u64)v));
+    return gdb_min64(gdb_min64((i64)gdb_readlane64((u64)v, 0), (i64)gdb_readlane64((u64)v, 16)),
+                     gdb_min64((i64)gdb_readlane64((u64)v, 32), (i64)gdb_readlane64((u64)v, 48)));
+}
+GDB_DEV i64 gdb_wave_max_i64(i64 v) {
+    v = gdb_max64(v, (i64)gdb_dpp64<0xB1>((u64)v));
+    v = gdb_max64(v, (i64)gdb_dpp64<0x4E>((u64)v));
+    v = gdb_max64(v, (i64)gdb_dpp64<0x124>((u64)v));
+    v = gdb_max64(v, (i64)gdb_dpp64<0x128>((u64)v));
+    return gdb_max64(gdb_max64((i64)gdb_readlane64((u64)v, 0), (i64)gdb_readlane64((u64)v, 16)),
+                     gdb_max64((i64)gdb_readlane64((u64)v, 32), (i64)gdb_readlane64((u64)v, 48)));
+}
+GDB_DEV int gdb_lane() { return (int)(threadIdx.x & 63u); }
+GDB_DEV int gdb_wave() { return (int)(threadIdx.x >> 6); }
+
+// ---------------------------------------------------------------------------
+// Atomics (relaxed). Global ones are device (agent) scope, LDS ones are
+// workgroup scope; results unused compile to no-return atomics.
+// ---------------------------------------------------------------------------
+GDB_DEV void gdb_atomic_add_u64(u64* p, u64 v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV void gdb_atomic_add_u32(u32* p, u32 v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV u32 gdb_atomic_inc_u32(u32* p) {
+    return __hip_atomic_fetch_add(p, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV void gdb_atomic_min_i64(i64* p, i64 v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV void gdb_atomic_max_i64(i64* p, i64 v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV void gdb_atomic_max_u64(u64* p, u64 v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV void gdb_atomic_or_u32(u32* p, u32 v) {
+    __hip_atomic_fetch_or(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV bool gdb_cas_u64(u64* p, u64& expected, u64 desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV u64 gdb_load_relaxed_u64(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+GDB_DEV void gdb_lds_add_u64(u64* p, u64 v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+GDB_DEV void gdb_lds_add_u32(u32* p, u32 v) {
+    __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+GDB_DEV void gdb_lds_min_i64(i64* p, i64 v) {
+    __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+GDB_DEV void gdb_lds_max_i64(i64* p, i64 v) {
+    __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+GDB_DEV bool gdb_lds_cas_u64(u64* p, u64& expected, u64 desired) {
+    return __hip_atomic_compare_exchange_strong(p, &expected, desired, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+GDB_DEV u64 gdb_lds_load_u64(const u64* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// ---------------------------------------------------------------------------
+// Bitmaps: bit k of word k>>5 (codegen/core/metal_common_header.h:103-111).
+// Keys outside [0, nbits) are absent and are never set.
+// ---------------------------------------------------------------------------
+GDB_DEV bool gdb_bitmap_test(const u32* __restrict__ bm, u64 nbits, i64 key) {
+    return (u64)key < nbits && ((bm[(u64)key >> 5] >> ((u32)key & 31u)) & 1u);
+}
+GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
+    if ((u64)key < nbits) gdb_atomic_or_u32(bm + ((u64)key >> 5), 1u << ((u32)key & 31u));
+}
+
+// ---------------------------------------------------------------------------
+// Open-addressing hash tables over 64-bit words. Word 0 of a slot packs the
+// 32-bit key (low half, empty = 0xFFFFFFFF) with a 32-bit companion (high
+// half: a build payload, or a group's row count). Linear probing from the high
+// bits of key * 2654435769 (Fibonacci hashing); probes are bounded so a full
+// table reports overflow instead of spinning.
+// ---------------------------------------------------------------------------
+GDB_DEV u64 gdb_hash_slot(u32 key, u32 bits) {
+    return (u64)((key * GDB_KNUTH) >> (32u - bits));
+}
+
+// Insert a unique key with a 32-bit payload. Returns the slot, -1 when the
+// probe bound is reached (overflow), -2 when the key is already present.
+GDB_DEV i64 gdb_ht_insert_unique(u64* __restrict__ tab, u32 slot_words, u32 bits, u32 key, u32 payload,
+                                 u32 max_probe) {
+    const u64 mask = (1ull << bits) - 1ull;
+    u64 s = gdb_hash_slot(key, bits);
+    const u64 mine = (u64)key | ((u64)payload << 32);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        u64* w = tab + s * slot_words;
+        u64 cur = gdb_load_relaxed_u64(w);
+        if ((u32)cur == GDB_EMPTY_KEY) {
+            u64 exp = cur;
+            if (gdb_cas_u64(w, exp, mine)) return (i64)s;
+            cur = exp;
+        }
+        if ((u32)cur == key) return -2;
+    }
+    return -1;
+}
+
+// Find a key; returns the slot or -1. The table must be complete (built by an
+// earlier launch), so plain loads are coherent.
+GDB_DEV i64 gdb_ht_find(const u64* __restrict__ tab, u32 slot_words, u32 bits, u32 key, u32 max_probe) {
+    const u64 mask = (1ull << bits) - 1ull;
+    u64 s = gdb_hash_slot(key, bits);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        const u32 k = (u32)tab[s * slot_words];
+        if (k == key) return (i64)s;
+        if (k == GDB_EMPTY_KEY) return -1;
+    }
+    return -1;
+}
+
+// Group-by slot: [key | count<<32, sum, min, max]. Merge one partial group
+// (cnt rows with the given sum/min/max) into a global table. Returns 1 when a
+// new key was inserted, 0 when merged, -1 on overflow.
+GDB_DEV int gdb_gb_merge(u64* __restrict__ tab, u32 bits, u32 key, u32 cnt, i64 sum, i64 mn, i64 mx,
+                         u32 max_probe) {
+    const u64 mask = (1ull << bits) - 1ull;
+    u64 s = gdb_hash_slot(key, bits);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        u64* w = tab + s * 4;
+        u64 cur = gdb_load_relaxed_u64(w);
+        int inserted = 0;
+        if ((u32)cur == GDB_EMPTY_KEY) {
+            u64 exp = cur;
+            if (gdb_cas_u64(w, exp, (u64)key | ((u64)cnt << 32))) {
+                inserted = 1;
+            } else {
+                cur = exp;
+            }
+        }
+        if (inserted || (u32)cur == key) {
+            if (!inserted) gdb_atomic_add_u64(w, (u64)cnt << 32);
+            gdb_atomic_add_u64(w + 1, (u64)sum);
+            gdb_atomic_min_i64((i64*)(w + 2), mn);
+            gdb_atomic_max_i64((i64*)(w + 3), mx);
+            return inserted;
+        }
+    }
+    return -1;
+}
+
+// Block-local (LDS) group-by table with the same slot layout. Returns the
+// slot index, or -1 when the probe window is full (caller spills to global).
+GDB_DEV int gdb_lds_gb_update(u64* tab, u32 bits, u32 key, i64 v, u32 max_probe) {
+    const u32 mask = (1u << bits) - 1u;
+    u32 s = (u32)gdb_hash_slot(key, bits);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        u64* w = tab + (u64)s * 4;
+        u64 cur = gdb_lds_load_u64(w);
+        bool inserted = false;
+        if ((u32)cur == GDB_EMPTY_KEY) {
+            u64 exp = cur;
+            if (gdb_lds_cas_u64(w, exp, (u64)key | (1ull << 32))) {
+                inserted = true;
+            } else {
+                cur = exp;
+            }
+        }
+        if (inserted || (u32)cur == key) {
+            if (!inserted) gdb_lds_add_u64(w, 1ull << 32);
+            gdb_lds_add_u64(w + 1, (u64)v);
+            gdb_lds_min_i64((i64*)(w + 2), v);
+            gdb_lds_max_i64((i64*)(w + 3), v);
+            return (int)s;
+        }
+    }
+    return -1;
+}
+
+// ---------------------------------------------------------------------------
+// SQL LIKE '%w1%w2%' on a fixed-width NUL-padded field: the value ends at the
+// first NUL (or at width); it matches when w1 occurs and w2 occurs at or after
+// the end of the leftmost w1 (the semantics of q13_comment_match,
+// codegen/planning/metal_plan_builder.cpp:1249-1266, for any two words).
+// ---------------------------------------------------------------------------
+GDB_DEV bool gdb_match_at(const char* s, int at, const char* w, int lw) {
+    for (int j = 0; j < lw; ++j)
+        if (s[at + j] != w[j]) return false;
+    return true;
+}
+GDB_DEV bool gdb_like2(const char* s, int width, const char* w1, int l1, const char* w2, int l2) {
+    int len = width;
+    for (int i = 0; i < width; ++i) {
+        if (s[i] == 0) {
+            len = i;
+            break;
+        }
+    }
+    int p = -1;
+    for (int i = 0; i + l1 <= len; ++i) {
+        if (gdb_match_at(s, i, w1, l1)) {
+            p = i;
+            break;
+        }
+    }
+    if (p < 0) return false;
+    for (int i = p + l1; i + l2 <= len; ++i)
+        if (gdb_match_at(s, i, w2, l2)) return true;
+    return false;
+}
+
+// ---------------------------------------------------------------------------
+// Top-k over 128-bit ranks (hi, lo), larger is better; rank (0,0) is reserved
+// for "empty". Registers hold a sorted list; insertion bubbles one entry.
+// ---------------------------------------------------------------------------
+GDB_DEV bool gdb_rank_gt(u64 ah, u64 al, u64 bh, u64 bl) { return ah > bh || (ah == bh && al > bl); }
+
+template <int K>
+struct GdbTopK {
+    u64 h[K];
+    u64 l[K];
+};
+template <int K>
+GDB_DEV void gdb_topk_init(GdbTopK<K>& t) {
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        t.h[j] = 0;
+        t.l[j] = 0;
+    }
+}
+template <int K>
+GDB_DEV void gdb_topk_push(GdbTopK<K>& t, u64 h, u64 l) {
+    if (gdb_rank_gt(h, l, t.h[K - 1], t.l[K - 1])) {
+        t.h[K - 1] = h;
+        t.l[K - 1] = l;
+#pragma unroll
+        for (int j = K - 1; j > 0; --j) {
+            if (gdb_rank_gt(t.h[j], t.l[j], t.h[j - 1], t.l[j - 1])) {
+                const u64 th = t.h[j], tl = t.l[j];
+                t.h[j] = t.h[j - 1];
+                t.l[j] = t.l[j - 1];
+                t.h[j - 1] = th;
+                t.l[j - 1] = tl;
+            }
+        }
+    }
+}
+// Merge every thread's list into thread 0's list. lds must hold
+// blockDim.x * K * 2 words.
+template <int K>
+GDB_DEV void gdb_topk_block_merge(GdbTopK<K>& t, u64* lds) {
+    const int tid = (int)threadIdx.x;
+#pragma unroll
+    for (int j = 0; j < K; ++j) {
+        lds[(tid * K + j) * 2] = t.h[j];
+        lds[(tid * K + j) * 2 + 1] = t.l[j];
+    }
+    __syncthreads();
+    if (gdb_lane() == 0) {
+        const int base = gdb_wave() * 64;
+        for (int i = 1; i < 64 && base + i < (int)blockDim.x; ++i) {
+            for (int j = 0; j < K; ++j) {
+                const u64 h = lds[((base + i) * K + j) * 2];
+                const u64 l = lds[((base + i) * K + j) * 2 + 1];
+                if (h == 0 && l == 0) break;
+                gdb_topk_push<K>(t, h, l);
+            }
+        }
+    }
+    __syncthreads();
+    if (gdb_lane() == 0) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) {
+            lds[(gdb_wave() * K + j) * 2] = t.h[j];
+            lds[(gdb_wave() * K + j) * 2 + 1] = t.l[j];
+        }
+    }
+    __syncthreads();
+    if (tid == 0) {
+        const int nw = (int)(blockDim.x >> 6);
+        for (int w = 1; w < nw; ++w) {
+            for (int j = 0; j < K; ++j) {
+                const u64 h = lds[(w * K + j) * 2];
+                const u64 l = lds[(w * K + j) * 2 + 1];
+                if (h == 0 && l == 0) break;
+                gdb_topk_push<K>(t, h, l);
+            }
+        }
+    }
+    __syncthreads();
+}
+// ===== end gdb device runtime =====
+)GDBHIP";
+
+}  // namespace codegen
+}  // namespace gdb

@@ -1,0 +1,500 @@
+// engine_capi.cpp — C-ABI wrappers of the C++ engine (include/gdb/gdb_engine.h).
+// Exceptions are caught here and turned into gdb_status + a per-thread message.
+
+#include "gdb/gdb_engine.h"
+
+#include "execution/hip_generic_executor.h"
+#include "execution/runtime_compiler.h"
+#include "infra/generators.h"
+#include "planning/hip_plan_builder.h"
+
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+using namespace gdb;
+using namespace gdb::codegen;
+
+namespace {
+
+thread_local std::string g_engineErr;
+
+gdb_status setErr(gdb_status code, const std::string& msg) {
+    g_engineErr = msg;
+    return code;
+}
+
+template <typename F>
+gdb_status guard(F&& f) {
+    try {
+        return f();
+    } catch (const GdbOverflowError& e) {
+        return setErr(GDB_ERR_OVERFLOW, e.what());
+    } catch (const std::exception& e) {
+        const std::string m = e.what();
+        if (m.find("hipRTC compile failed") != std::string::npos) return setErr(GDB_ERR_COMPILE, m);
+        if (m.find("not registered") != std::string::npos || m.find("unknown") != std::string::npos)
+            return setErr(GDB_ERR_NOT_FOUND, m);
+        if (m.find("hipErrorOutOfMemory") != std::string::npos) return setErr(GDB_ERR_OOM, m);
+        return setErr(GDB_ERR_INVALID, m);
+    } catch (...) {
+        return setErr(GDB_ERR_INVALID, "unknown C++ exception");
+    }
+}
+
+gdb_status copyOut(const std::string& s, char* buf, size_t cap, size_t* need) {
+    if (need) *need = s.size() + 1;
+    if (buf && cap > 0) {
+        const size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return GDB_OK;
+}
+
+// One RuntimeCompiler per context (module cache shared by all plans).
+std::mutex g_rcMu;
+std::map<gdb_ctx, std::unique_ptr<RuntimeCompiler>> g_compilers;
+
+RuntimeCompiler& compilerFor(gdb_ctx ctx) {
+    std::lock_guard<std::mutex> lk(g_rcMu);
+    auto& rc = g_compilers[ctx];
+    if (!rc) rc = std::make_unique<RuntimeCompiler>(ctx);
+    return *rc;
+}
+
+}  // namespace
+
+struct gdb_plan_s {
+    HipQueryPlan plan;
+    HipCodegen cg;
+    std::string source;
+    gdb_ctx ctx = nullptr;
+    RuntimeCompiler::CompiledQuery cq;
+    bool compiled = false;
+    void regenerate() {
+        cg = generateFromPlan(plan);
+        source = cg.print();
+        compiled = false;
+    }
+};
+
+struct gdb_executor_s {
+    std::unique_ptr<HipGenericExecutor> ex;
+    gdb_ctx ctx = nullptr;
+};
+
+struct gdb_result_s {
+    GenericResult r;
+};
+
+extern "C" {
+
+const char* gdb_engine_last_error(void) { return g_engineErr.c_str(); }
+
+// ---- generators ----
+gdb_status gdb_gen_uniform_i32(uint64_t seed, uint64_t stream, uint64_t n, int64_t lo, int64_t hi, int32_t* out) {
+    return guard([&]() -> gdb_status {
+        infra::genUniformI32(seed, stream, n, lo, hi, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_uniform_i64(uint64_t seed, uint64_t stream, uint64_t n, int64_t lo, int64_t hi, int64_t* out) {
+    return guard([&]() -> gdb_status {
+        infra::genUniformI64(seed, stream, n, lo, hi, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_categorical_u8(uint64_t seed, uint64_t stream, uint64_t n, const uint8_t* values,
+                                  const uint32_t* weights, int nvalues, uint8_t* out) {
+    return guard([&]() -> gdb_status {
+        infra::genCategoricalU8(seed, stream, n, values, weights, nvalues, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_permutation_i32(uint64_t seed, uint64_t stream, uint64_t n, int64_t base, int32_t* out) {
+    return guard([&]() -> gdb_status {
+        infra::genPermutationI32(seed, stream, n, base, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_fanout_total(uint64_t seed, uint64_t stream, uint64_t nparent, int cmin, int cmax,
+                                uint64_t* total) {
+    return guard([&]() -> gdb_status {
+        *total = infra::fanoutTotal(seed, stream, nparent, cmin, cmax);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_fanout_fkey_i32(uint64_t seed, uint64_t stream, uint64_t nparent, int cmin, int cmax,
+                                   uint64_t nchild, int32_t* out) {
+    return guard([&]() -> gdb_status {
+        infra::genFanoutFkeyI32(seed, stream, nparent, cmin, cmax, nchild, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_zipf_i32(uint64_t seed, uint64_t stream, uint64_t n, double s, int64_t nkeys, int32_t* out) {
+    return guard([&]() -> gdb_status {
+        infra::genZipfI32(seed, stream, n, s, nkeys, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_vocabulary(uint64_t seed, int nwords, int minlen, int maxlen, char* out) {
+    return guard([&]() -> gdb_status {
+        infra::genVocabulary(seed, nwords, minlen, maxlen, out);
+        return GDB_OK;
+    });
+}
+gdb_status gdb_gen_text(uint64_t seed, uint64_t stream, uint64_t n, int width, const char* vocab, int nwords,
+                        char* out) {
+    return guard([&]() -> gdb_status {
+        infra::genText(seed, stream, n, width, vocab, nwords, out);
+        return GDB_OK;
+    });
+}
+
+// ---- plans ----
+gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out) {
+    return guard([&]() -> gdb_status {
+        if (!name || !out) return setErr(GDB_ERR_INVALID, "gdb_plan_build: null argument");
+        auto plan = buildHipPlan(name, parsePlanOptions(options ? options : ""));
+        if (!plan) return setErr(GDB_ERR_NOT_FOUND, std::string("gdb_plan_build: unknown plan ") + name);
+        auto* p = new gdb_plan_s();
+        p->plan = std::move(*plan);
+        p->regenerate();
+        *out = p;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_plan_describe(gdb_plan p, char* buf, size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status { return copyOut(p->plan.describe(), buf, cap, need); });
+}
+
+gdb_status gdb_plan_source(gdb_plan p, char* buf, size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status { return copyOut(p->source, buf, cap, need); });
+}
+
+gdb_status gdb_plan_phase_count(gdb_plan p, int* n) {
+    *n = static_cast<int>(p->plan.phases.size());
+    return GDB_OK;
+}
+
+gdb_status gdb_plan_phase_name(gdb_plan p, int idx, char* buf, size_t cap) {
+    return guard([&]() -> gdb_status {
+        if (idx < 0 || idx >= static_cast<int>(p->plan.phases.size()))
+            return setErr(GDB_ERR_INVALID, "gdb_plan_phase_name: index out of range");
+        return copyOut(p->plan.phases[static_cast<size_t>(idx)].name, buf, cap, nullptr);
+    });
+}
+
+gdb_status gdb_plan_set_launch(gdb_plan p, const char* phase, int block, int items) {
+    return guard([&]() -> gdb_status {
+        for (auto& ph : p->cg.getPhasesMutable()) {
+            if (ph.name != phase) continue;
+            bool known = false;
+            for (int v : ph.itemsVariants) known |= (v == items);
+            if (!known) return setErr(GDB_ERR_INVALID, "gdb_plan_set_launch: no ITEMS variant " + std::to_string(items));
+            if (block < 64 || block > ph.launchBound || block % 64)
+                return setErr(GDB_ERR_INVALID, "gdb_plan_set_launch: block size outside [64, launch bound]");
+            ph.threadgroupSize = block;
+            ph.items = items;
+            for (auto& pp : p->plan.phases)
+                if (pp.name == phase) {
+                    pp.threadgroupSize = block;
+                    pp.items = items;
+                }
+            return GDB_OK;
+        }
+        return setErr(GDB_ERR_NOT_FOUND, std::string("gdb_plan_set_launch: no phase ") + phase);
+    });
+}
+
+gdb_status gdb_plan_get_launch(gdb_plan p, const char* phase, int* block, int* items) {
+    for (const auto& ph : p->cg.getPhases())
+        if (ph.name == phase) {
+            *block = ph.threadgroupSize;
+            *items = ph.items;
+            return GDB_OK;
+        }
+    return setErr(GDB_ERR_NOT_FOUND, std::string("gdb_plan_get_launch: no phase ") + phase);
+}
+
+gdb_status gdb_plan_compile(gdb_ctx ctx, gdb_plan p, double* ms, int* cache_hit, uint64_t* plan_hash) {
+    return guard([&]() -> gdb_status {
+        auto& rc = compilerFor(ctx);
+        p->cq = rc.compile(p->source);
+        p->ctx = ctx;
+        p->compiled = true;
+        if (ms) *ms = p->cq.compileMs;
+        if (cache_hit) *cache_hit = p->cq.cacheHit ? 1 : 0;
+        if (plan_hash) *plan_hash = p->cq.planHash;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_plan_precompile(gdb_plan p, const char* cache_dir, const char* arch, int* was_cached,
+                               uint64_t* plan_hash) {
+    gdb_status st = gdb_module_precompile(cache_dir, arch, p->source.data(), p->source.size(), nullptr, 0, plan_hash,
+                                          was_cached);
+    if (st != GDB_OK) return setErr(st, gdb_last_error());
+    return GDB_OK;
+}
+
+gdb_status gdb_plan_free(gdb_plan p) {
+    delete p;
+    return GDB_OK;
+}
+
+// ---- executor ----
+gdb_status gdb_executor_create(gdb_ctx ctx, gdb_stream s, gdb_executor* out) {
+    return guard([&]() -> gdb_status {
+        auto* e = new gdb_executor_s();
+        e->ctx = ctx;
+        e->ex = std::make_unique<HipGenericExecutor>(ctx, s);
+        *out = e;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_destroy(gdb_executor ex) {
+    return guard([&]() -> gdb_status {
+        delete ex;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_register_table(gdb_executor ex, const char* table, uint64_t rows) {
+    return guard([&]() -> gdb_status {
+        ex->ex->registerTable(table, rows);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_register_column(gdb_executor ex, const char* table, const char* column, const void* dptr) {
+    return guard([&]() -> gdb_status {
+        ex->ex->registerTableColumn(table, column, dptr);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_register_buffer(gdb_executor ex, const char* name, void* dptr, size_t bytes) {
+    return guard([&]() -> gdb_status {
+        ex->ex->registerBuffer(name, dptr, bytes);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_register_symbol(gdb_executor ex, const char* name, uint64_t value) {
+    return guard([&]() -> gdb_status {
+        ex->ex->registerSymbol(name, value);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_get_symbol(gdb_executor ex, const char* name, uint64_t* value, int* found) {
+    return guard([&]() -> gdb_status {
+        uint64_t v = 0;
+        const bool f = ex->ex->tryGetSymbol(name, v);
+        if (value) *value = v;
+        if (found) *found = f ? 1 : 0;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_register_scalar(gdb_executor ex, const char* name, int64_t value) {
+    return guard([&]() -> gdb_status {
+        ex->ex->registerScalarInt(name, value);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_set_skip_init(gdb_executor ex, int skip) {
+    ex->ex->setSkipInit(skip != 0);
+    return GDB_OK;
+}
+
+static void registerPlanTables(gdb_executor ex, gdb_plan p) {
+    for (const auto& bt : p->plan.bufferTables) ex->ex->registerBufferTable(bt.table, bt.column, bt.buffer, bt.rowsExpr);
+}
+
+// Install the plan's sizing pre-pass rules as phase hooks on the runner.
+static void installSizingRules(gdb_executor ex, gdb_plan p) {
+    ex->ex->clearPhaseHooks();
+    for (const auto& rule : p->plan.sizingRules) {
+        ex->ex->setPhaseHook(rule.afterPhase, [rule](HipGenericExecutor& e) {
+            uint64_t count = 0;
+            e.readBuffer(rule.buffer, static_cast<size_t>(rule.word) * 8, &count, 8);
+            uint64_t v = std::max<uint64_t>(rule.minimum, count * rule.multiplier);
+            if (rule.pow2) {
+                uint64_t q = 1;
+                while (q < v) q <<= 1;
+                v = q;
+            }
+            e.registerSymbol(rule.symbol, v);
+            e.registerSymbol(rule.symbol + "_count", count);
+        });
+    }
+}
+
+static void fillTiming(const HipExecutionResult& r, gdb_exec_timing* out) {
+    if (!out) return;
+    std::memset(out, 0, sizeof(*out));
+    out->total_ms = r.totalKernelTimeMs;
+    out->alloc_ms = r.bufferAllocTimeMs;
+    out->nphases = static_cast<int>(std::min<size_t>(r.phaseTimesMs.size(), GDB_MAX_PHASES));
+    for (int i = 0; i < out->nphases; ++i) {
+        out->phase_ms[i] = r.phaseTimesMs[static_cast<size_t>(i)];
+        if (static_cast<size_t>(i) < r.phaseGrid.size()) {
+            out->grid[i] = r.phaseGrid[static_cast<size_t>(i)];
+            out->block[i] = r.phaseBlock[static_cast<size_t>(i)];
+            out->items[i] = r.phaseItems[static_cast<size_t>(i)];
+        }
+        if (static_cast<size_t>(i) < r.phaseStatus.size())
+            for (int w = 0; w < 4; ++w) out->status[i][w] = r.phaseStatus[static_cast<size_t>(i)][static_cast<size_t>(w)];
+    }
+}
+
+gdb_status gdb_executor_execute_range(gdb_executor ex, gdb_plan p, int first, int last, int warmup, int measured,
+                                      gdb_exec_timing* out) {
+    return guard([&]() -> gdb_status {
+        if (!p->compiled || p->ctx != ex->ctx) return setErr(GDB_ERR_INVALID, "gdb_executor_execute: plan not compiled for this context");
+        registerPlanTables(ex, p);
+        installSizingRules(ex, p);
+        auto& rc = compilerFor(ex->ctx);
+        HipExecutionResult r;
+        try {
+            r = ex->ex->execute(rc, p->cq, p->cg, warmup, measured, first, last);
+        } catch (const GdbOverflowError& e) {
+            if (out) {
+                std::memset(out, 0, sizeof(*out));
+                out->status[0][0] = 1;
+                out->status[0][2] = e.inserted;
+            }
+            throw;
+        }
+        fillTiming(r, out);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_execute(gdb_executor ex, gdb_plan p, int warmup, int measured, gdb_exec_timing* out) {
+    return gdb_executor_execute_range(ex, p, 0, -1, warmup, measured, out);
+}
+
+gdb_status gdb_executor_prepare(gdb_executor ex, gdb_plan p) {
+    return guard([&]() -> gdb_status {
+        registerPlanTables(ex, p);
+        ex->ex->prepare(p->cg);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_enqueue(gdb_executor ex, gdb_plan p) {
+    return guard([&]() -> gdb_status {
+        if (!p->compiled || p->ctx != ex->ctx) return setErr(GDB_ERR_INVALID, "gdb_executor_enqueue: plan not compiled for this context");
+        registerPlanTables(ex, p);
+        ex->ex->enqueue(compilerFor(ex->ctx), p->cq, p->cg);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_check_status(gdb_executor ex, gdb_plan p) {
+    return guard([&]() -> gdb_status {
+        ex->ex->checkStatus(p->cg);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_read_buffer(gdb_executor ex, const char* name, size_t offset, void* host, size_t bytes) {
+    return guard([&]() -> gdb_status {
+        ex->ex->readBuffer(name, offset, host, bytes);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_executor_buffer_ptr(gdb_executor ex, const char* name, void** dptr, size_t* bytes) {
+    void* ptr = ex->ex->getAllocatedBuffer(name);
+    if (!ptr) return setErr(GDB_ERR_NOT_FOUND, std::string("no buffer ") + name);
+    if (dptr) *dptr = ptr;
+    if (bytes) *bytes = ex->ex->getAllocatedBytes(name);
+    return GDB_OK;
+}
+
+gdb_status gdb_executor_owned_bytes(gdb_executor ex, size_t* bytes) {
+    *bytes = ex->ex->ownedBytes();
+    return GDB_OK;
+}
+
+gdb_status gdb_executor_release(gdb_executor ex) {
+    return guard([&]() -> gdb_status {
+        ex->ex->releaseAllocatedBuffers();
+        return GDB_OK;
+    });
+}
+
+// ---- results ----
+gdb_status gdb_executor_collect(gdb_executor ex, gdb_plan p, gdb_result* out) {
+    return guard([&]() -> gdb_status {
+        auto* r = new gdb_result_s();
+        r->r = ex->ex->collectResult(p->cg);
+        *out = r;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_result_shape(gdb_result r, uint64_t* rows, int* cols) {
+    if (rows) *rows = r->r.rows();
+    if (cols) *cols = static_cast<int>(r->r.columns.size());
+    return GDB_OK;
+}
+
+gdb_status gdb_result_column(gdb_result r, int idx, char* name, size_t cap, int* type) {
+    if (idx < 0 || idx >= static_cast<int>(r->r.columns.size()))
+        return setErr(GDB_ERR_INVALID, "gdb_result_column: index out of range");
+    const auto& c = r->r.columns[static_cast<size_t>(idx)];
+    copyOut(c.name, name, cap, nullptr);
+    if (type) *type = static_cast<int>(c.type);
+    return GDB_OK;
+}
+
+gdb_status gdb_result_i64(gdb_result r, int idx, int64_t* out) {
+    const auto& c = r->r.columns.at(static_cast<size_t>(idx));
+    if (c.type != ResultColumn::I64) return setErr(GDB_ERR_INVALID, "gdb_result_i64: column is not int64");
+    std::memcpy(out, c.i64.data(), c.i64.size() * 8);
+    return GDB_OK;
+}
+
+gdb_status gdb_result_i128(gdb_result r, int idx, int64_t* out) {
+    const auto& c = r->r.columns.at(static_cast<size_t>(idx));
+    if (c.type != ResultColumn::I128) return setErr(GDB_ERR_INVALID, "gdb_result_i128: column is not int128");
+    for (size_t i = 0; i < c.i128.size(); ++i) {
+        const unsigned __int128 u = static_cast<unsigned __int128>(c.i128[i]);
+        out[2 * i] = static_cast<int64_t>(static_cast<uint64_t>(u));
+        out[2 * i + 1] = static_cast<int64_t>(static_cast<uint64_t>(u >> 64));
+    }
+    return GDB_OK;
+}
+
+gdb_status gdb_result_str(gdb_result r, int idx, char* out, size_t width) {
+    const auto& c = r->r.columns.at(static_cast<size_t>(idx));
+    if (c.type != ResultColumn::STR) return setErr(GDB_ERR_INVALID, "gdb_result_str: column is not a string");
+    for (size_t i = 0; i < c.str.size(); ++i) {
+        std::memset(out + i * width, 0, width);
+        std::memcpy(out + i * width, c.str[i].data(), std::min(width, c.str[i].size()));
+    }
+    return GDB_OK;
+}
+
+gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status { return copyOut(r->r.toCanonical(), buf, cap, need); });
+}
+
+gdb_status gdb_result_free(gdb_result r) {
+    delete r;
+    return GDB_OK;
+}
+
+}  // extern "C"

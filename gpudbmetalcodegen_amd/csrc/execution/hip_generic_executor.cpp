@@ -1,0 +1,355 @@
+#include "hip_generic_executor.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cstring>
+
+namespace gdb {
+namespace codegen {
+
+namespace {
+
+void check(gdb_status st, const std::string& what) {
+    if (st != GDB_OK) throw std::runtime_error(what + ": " + gdb_last_error());
+}
+
+}  // namespace
+
+HipGenericExecutor::HipGenericExecutor(gdb_ctx ctx, gdb_stream stream)
+    : ctx_(ctx), stream_(stream ? stream : gdb_default_stream(ctx)) {
+    gdb_device_info_t info;
+    check(gdb_device_info(ctx_, &info), "gdb_device_info");
+    cuCount_ = info.cu_count;
+}
+
+HipGenericExecutor::~HipGenericExecutor() {
+    releaseAllocatedBuffers();
+    for (auto e : events_) gdb_event_destroy(e);
+}
+
+void HipGenericExecutor::registerTable(const std::string& table, uint64_t rows) {
+    tables_[table].rows = rows;
+    sizes_.registerSymbol(tableSizeName(table), rows);
+}
+
+void HipGenericExecutor::registerTableColumn(const std::string& table, const std::string& column, const void* devptr) {
+    if (reinterpret_cast<uintptr_t>(devptr) % 16 != 0)
+        throw std::runtime_error("registerTableColumn: column " + column + " must be 16-byte aligned");
+    tables_[table].columns[column] = devptr;
+}
+
+void HipGenericExecutor::registerBuffer(const std::string& name, void* devptr, size_t bytes) {
+    auto it = buffers_.find(name);
+    if (it != buffers_.end() && it->second.owned) gdb_free(ctx_, it->second.ptr);
+    buffers_[name] = Buffer{devptr, bytes, false};
+}
+
+void HipGenericExecutor::registerBufferTable(const std::string& table, const std::string& column,
+                                             const std::string& buffer, const std::string& rowsExpr) {
+    for (auto& bt : bufferTables_)
+        if (bt.table == table && bt.column == column) {
+            bt.buffer = buffer;
+            bt.rowsExpr = rowsExpr;
+            return;
+        }
+    bufferTables_.push_back({table, column, buffer, rowsExpr});
+}
+
+void HipGenericExecutor::registerSymbol(const std::string& name, uint64_t value) { sizes_.registerSymbol(name, value); }
+
+bool HipGenericExecutor::tryGetSymbol(const std::string& name, uint64_t& out) const {
+    if (!sizes_.hasSymbol(name)) return false;
+    out = sizes_.getSymbol(name);
+    return true;
+}
+
+void HipGenericExecutor::registerScalarInt(const std::string& name, int64_t value) { scalars_[name] = value; }
+
+void HipGenericExecutor::setPhaseHook(const std::string& phase, std::function<void(HipGenericExecutor&)> hook) {
+    hooks_[phase] = std::move(hook);
+}
+
+HipGenericExecutor::Buffer& HipGenericExecutor::ensureBuffer(const HipParamBinding& b) {
+    auto it = buffers_.find(b.name);
+    if (b.sizeExpr.empty()) {
+        if (it == buffers_.end())
+            throw std::runtime_error("executor: buffer '" + b.name + "' is neither registered nor sized by the plan");
+        return it->second;
+    }
+    const uint64_t elems = sizes_.resolve(b.sizeExpr);
+    const size_t bytes = std::max<size_t>(static_cast<size_t>(elems) * b.elemBytes, 16);
+    if (it != buffers_.end()) {
+        if (!it->second.owned || it->second.bytes >= bytes) return it->second;
+        check(gdb_free(ctx_, it->second.ptr), "gdb_free");
+        buffers_.erase(it);
+    }
+    Buffer nb;
+    nb.bytes = ((bytes + 255) / 256) * 256;
+    check(gdb_alloc(ctx_, nb.bytes, &nb.ptr), "gdb_alloc(" + b.name + ", " + std::to_string(nb.bytes) + " bytes)");
+    nb.owned = true;
+    return buffers_[b.name] = nb;
+}
+
+void HipGenericExecutor::resolveBufferTables() {
+    for (const auto& bt : bufferTables_) {
+        auto it = buffers_.find(bt.buffer);
+        if (it == buffers_.end()) continue;
+        tables_[bt.table].columns[bt.column] = it->second.ptr;
+        const uint64_t rows = sizes_.resolve(bt.rowsExpr);
+        tables_[bt.table].rows = rows;
+        sizes_.registerSymbol(tableSizeName(bt.table), rows);
+    }
+}
+
+std::unordered_map<std::string, const HipParamBinding*> HipGenericExecutor::initSpecs(const HipCodegen& cg) const {
+    std::unordered_map<std::string, const HipParamBinding*> spec;
+    for (const auto& p : cg.getPhases())
+        for (const auto& b : p.bindings)
+            if (b.kind == HipParamKind::DeviceBuffer && b.init != InitKind::None && !spec.count(b.name))
+                spec[b.name] = &b;
+    return spec;
+}
+
+int HipGenericExecutor::occupancy(gdb_kernel k, int block) {
+    auto key = std::make_pair(k, block);
+    auto it = occupancy_.find(key);
+    if (it != occupancy_.end()) return it->second;
+    int nb = 1;
+    if (gdb_kernel_occupancy(k, block, 0, &nb) != GDB_OK || nb < 1) nb = 1;
+    occupancy_[key] = nb;
+    return nb;
+}
+
+void HipGenericExecutor::prepare(const HipCodegen& cg, int firstPhase, int lastPhase) {
+    const auto& phases = cg.getPhases();
+    const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
+    for (int pi = firstPhase; pi < last; ++pi) {
+        for (const auto& b : phases[static_cast<size_t>(pi)].bindings) {
+            if (b.kind != HipParamKind::DeviceBuffer) continue;
+            try {
+                ensureBuffer(b);
+            } catch (const std::runtime_error&) {
+                // Sized by a symbol a hook registers later; allocated at launch.
+            }
+        }
+        resolveBufferTables();
+    }
+}
+
+void HipGenericExecutor::launchPhase(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq,
+                                     const HipCodegen::PhaseInfo& p, std::unordered_map<std::string, bool>& initDone,
+                                     const std::unordered_map<std::string, const HipParamBinding*>& initSpec,
+                                     unsigned* gridOut, gdb_event beforeKernel) {
+    std::vector<uint64_t> args;
+    args.reserve(p.bindings.size());
+    resolveBufferTables();
+    for (const auto& b : p.bindings) {
+        switch (b.kind) {
+            case HipParamKind::TableColumn: {
+                auto t = tables_.find(b.tableName);
+                if (t == tables_.end()) throw std::runtime_error("executor: table '" + b.tableName + "' not registered");
+                auto c = t->second.columns.find(b.columnName);
+                if (c == t->second.columns.end())
+                    throw std::runtime_error("executor: column '" + b.columnName + "' of table '" + b.tableName +
+                                             "' not registered");
+                args.push_back(reinterpret_cast<uint64_t>(c->second));
+                break;
+            }
+            case HipParamKind::TableSize: {
+                auto t = tables_.find(b.tableName);
+                if (t == tables_.end()) throw std::runtime_error("executor: table '" + b.tableName + "' not registered");
+                args.push_back(t->second.rows);
+                break;
+            }
+            case HipParamKind::DeviceBuffer: {
+                Buffer& buf = ensureBuffer(b);
+                if (!skipInit_ && !initDone[b.name]) {
+                    auto sp = initSpec.find(b.name);
+                    if (sp != initSpec.end()) {
+                        const HipParamBinding* ib = sp->second;
+                        const size_t bytes = static_cast<size_t>(sizes_.resolve(ib->sizeExpr)) * ib->elemBytes;
+                        if (ib->init == InitKind::Zero) {
+                            const uint8_t z = 0;
+                            check(gdb_fill_async(buf.ptr, bytes, &z, 1, stream_), "fill " + b.name);
+                        } else {
+                            check(gdb_fill_async(buf.ptr, bytes, ib->pattern.data(), static_cast<int>(ib->pattern.size()),
+                                                 stream_),
+                                  "fill " + b.name);
+                        }
+                    }
+                    initDone[b.name] = true;
+                }
+                args.push_back(reinterpret_cast<uint64_t>(buf.ptr));
+                break;
+            }
+            case HipParamKind::ScalarInt: {
+                auto s = scalars_.find(b.name);
+                if (s == scalars_.end()) throw std::runtime_error("executor: scalar '" + b.name + "' not registered");
+                args.push_back(static_cast<uint64_t>(s->second));
+                break;
+            }
+            case HipParamKind::SymbolValue: args.push_back(sizes_.resolve(b.sizeExpr)); break;
+        }
+    }
+    const int items = p.isSingleThread ? 1 : p.items;
+    bool known = false;
+    for (int v : (p.isSingleThread ? std::vector<int>{1} : p.itemsVariants)) known |= (v == items);
+    if (!known) throw std::runtime_error("executor: phase " + p.name + " has no ITEMS=" + std::to_string(items) + " variant");
+    const int block = p.isSingleThread ? 64 : p.threadgroupSize;
+    if (block > p.launchBound && !p.isSingleThread)
+        throw std::runtime_error("executor: block " + std::to_string(block) + " exceeds launch bound of " + p.name);
+    gdb_kernel k = rc.getPipeline(cq, HipCodegen::kernelName(p.name, items));
+    unsigned grid = 1;
+    if (!p.isSingleThread) {
+        const uint64_t rows = p.rowsExpr.empty() ? 1 : sizes_.resolve(p.rowsExpr);
+        const uint64_t tile = static_cast<uint64_t>(block) * static_cast<uint64_t>(items);
+        uint64_t need = std::max<uint64_t>(1, (rows + tile - 1) / tile);
+        uint64_t cap = static_cast<uint64_t>(cuCount_) * static_cast<uint64_t>(occupancy(k, block));
+        if (p.maxThreadgroups > 0) cap = std::min<uint64_t>(cap, static_cast<uint64_t>(p.maxThreadgroups));
+        cap = std::max<uint64_t>(cap, 1);
+        grid = static_cast<unsigned>(std::min(need, cap));
+    }
+    if (gridOut) *gridOut = grid;
+    if (beforeKernel) check(gdb_event_record(beforeKernel, stream_), "gdb_event_record");
+    check(gdb_launch(k, grid, static_cast<unsigned>(block), 0, args.data(), args.size() * 8, stream_),
+          "launch " + p.name);
+}
+
+void HipGenericExecutor::enqueue(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq, const HipCodegen& cg,
+                                 int firstPhase, int lastPhase) {
+    const auto& phases = cg.getPhases();
+    const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
+    auto spec = initSpecs(cg);
+    std::unordered_map<std::string, bool> initDone;
+    for (int pi = firstPhase; pi < last; ++pi)
+        launchPhase(rc, cq, phases[static_cast<size_t>(pi)], initDone, spec, nullptr);
+}
+
+HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq,
+                                               const HipCodegen& cg, int warmupRuns, int measuredRuns, int firstPhase,
+                                               int lastPhase) {
+    const auto& phases = cg.getPhases();
+    const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
+    const int np = last - firstPhase;
+    HipExecutionResult res;
+    auto t0 = std::chrono::steady_clock::now();
+    prepare(cg, firstPhase, last);
+    res.bufferAllocTimeMs =
+        std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    while (static_cast<int>(events_.size()) < 2 * np + 1) {
+        gdb_event e;
+        check(gdb_event_create(&e), "gdb_event_create");
+        events_.push_back(e);
+    }
+    auto spec = initSpecs(cg);
+    const int runs = warmupRuns + std::max(1, measuredRuns);
+    measuredPhase_.assign(static_cast<size_t>(np), 0.0f);
+    measuredFill_.assign(static_cast<size_t>(np), 0.0f);
+    for (int r = 0; r < runs; ++r) {
+        const bool measured = r >= warmupRuns;
+        std::unordered_map<std::string, bool> initDone;
+        res.phaseTimesMs.assign(static_cast<size_t>(np), 0.0f);
+        res.phaseFillMs.assign(static_cast<size_t>(np), 0.0f);
+        res.phaseNames.clear();
+        res.phaseGrid.clear();
+        res.phaseBlock.clear();
+        res.phaseItems.clear();
+        check(gdb_event_record(events_[0], stream_), "gdb_event_record");
+        for (int pi = firstPhase; pi < last; ++pi) {
+            const auto& p = phases[static_cast<size_t>(pi)];
+            unsigned grid = 0;
+            const size_t e = static_cast<size_t>(2 * (pi - firstPhase));
+            launchPhase(rc, cq, p, initDone, spec, &grid, events_[e + 1]);
+            check(gdb_event_record(events_[e + 2], stream_), "gdb_event_record");
+            res.phaseNames.push_back(p.name);
+            res.phaseGrid.push_back(grid);
+            res.phaseBlock.push_back(static_cast<unsigned>(p.isSingleThread ? 64 : p.threadgroupSize));
+            res.phaseItems.push_back(static_cast<unsigned>(p.isSingleThread ? 1 : p.items));
+            auto h = hooks_.find(p.name);
+            if (h != hooks_.end()) {
+                check(gdb_stream_sync(stream_), "gdb_stream_sync");
+                h->second(*this);
+            }
+        }
+        check(gdb_event_sync(events_[static_cast<size_t>(2 * np)]), "gdb_event_sync");
+        if (measured) {
+            const float w = 1.0f / static_cast<float>(std::max(1, measuredRuns));
+            for (int i = 0; i < np; ++i) {
+                float ms = 0.0f, fill = 0.0f;
+                const size_t e = static_cast<size_t>(2 * i);
+                check(gdb_event_elapsed_ms(events_[e + 1], events_[e + 2], &ms), "gdb_event_elapsed_ms");
+                check(gdb_event_elapsed_ms(events_[e], events_[e + 1], &fill), "gdb_event_elapsed_ms");
+                measuredPhase_[static_cast<size_t>(i)] += ms * w;
+                measuredFill_[static_cast<size_t>(i)] += fill * w;
+            }
+            float total = 0.0f;
+            check(gdb_event_elapsed_ms(events_[0], events_[static_cast<size_t>(2 * np)], &total), "gdb_event_elapsed_ms");
+            res.totalKernelTimeMs += total * w;
+        }
+    }
+    res.phaseTimesMs = measuredPhase_;
+    res.phaseFillMs = measuredFill_;
+    res.phaseStatus = checkStatus(cg, firstPhase, last);
+    return res;
+}
+
+std::vector<std::array<uint64_t, 4>> HipGenericExecutor::checkStatus(const HipCodegen& cg, int firstPhase,
+                                                                     int lastPhase) {
+    const auto& phases = cg.getPhases();
+    const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
+    std::vector<std::array<uint64_t, 4>> out;
+    for (int pi = firstPhase; pi < last; ++pi) {
+        const auto& p = phases[static_cast<size_t>(pi)];
+        std::array<uint64_t, 4> w{0, 0, 0, 0};
+        if (!p.statusBuffer.empty() && buffers_.count(p.statusBuffer)) {
+            readBuffer(p.statusBuffer, 0, w.data(), 32);
+            if (w[0] != 0) throw GdbOverflowError(p.name, w[2]);
+        }
+        out.push_back(w);
+    }
+    return out;
+}
+
+GenericResult HipGenericExecutor::collectResult(const HipCodegen& cg) const {
+    check(gdb_stream_sync(stream_), "gdb_stream_sync");
+    return HipResultCollector::collect(
+        cg.getResultSchema(),
+        [this](const std::string& name, size_t off, void* dst, size_t bytes) { readBuffer(name, off, dst, bytes); },
+        [this](const std::string& expr) { return sizes_.resolve(expr); });
+}
+
+void HipGenericExecutor::releaseAllocatedBuffers() {
+    for (auto& kv : buffers_)
+        if (kv.second.owned && kv.second.ptr) gdb_free(ctx_, kv.second.ptr);
+    buffers_.clear();
+}
+
+void* HipGenericExecutor::getAllocatedBuffer(const std::string& name) const {
+    auto it = buffers_.find(name);
+    return it == buffers_.end() ? nullptr : it->second.ptr;
+}
+
+size_t HipGenericExecutor::getAllocatedBytes(const std::string& name) const {
+    auto it = buffers_.find(name);
+    return it == buffers_.end() ? 0 : it->second.bytes;
+}
+
+size_t HipGenericExecutor::ownedBytes() const {
+    size_t n = 0;
+    for (const auto& kv : buffers_)
+        if (kv.second.owned) n += kv.second.bytes;
+    return n;
+}
+
+void HipGenericExecutor::readBuffer(const std::string& name, size_t offset, void* host, size_t bytes) const {
+    auto it = buffers_.find(name);
+    if (it == buffers_.end()) throw std::runtime_error("executor: no buffer " + name);
+    if (offset + bytes > it->second.bytes)
+        throw std::runtime_error("executor: read past the end of buffer " + name);
+    check(gdb_memcpy_d2h_async(host, static_cast<const char*>(it->second.ptr) + offset, bytes, stream_),
+          "read " + name);
+    check(gdb_stream_sync(stream_), "gdb_stream_sync");
+}
+
+}  // namespace codegen
+}  // namespace gdb

@@ -1,0 +1,138 @@
+#pragma once
+// HipGenericExecutor — binding-driven runner over the C-ABI.
+//
+// Counterpart of MetalGenericExecutor (codegen/execution/metal_generic_executor.h:46-164):
+// resolves symbolic sizes, allocates and initialises runner-owned buffers,
+// packs kernel arguments from the phase bindings, chooses the grid, launches
+// every phase on one stream and times phases with events. Differences:
+//   - buffer initialisation is a fill kernel on the stream (graph-safe), not
+//     a host memset of shared memory;
+//   - buffers are allocated lazily per phase, so a phase hook (a sizing
+//     pre-pass that registers a symbol) can size later phases;
+//   - enqueue() only launches (no sync, no events) so a whole plan can be
+//     captured in a graph or overlapped with other queries;
+//   - kernels report capacity overflow through status words that execute()
+//     checks and raises as GdbOverflowError.
+
+#include "../core/hip_codegen_base.h"
+#include "gdb/gdb_hip.h"
+#include "hip_result_collector.h"
+#include "runtime_compiler.h"
+
+#include <array>
+#include <functional>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace gdb {
+namespace codegen {
+
+struct GdbOverflowError : public std::runtime_error {
+    std::string phase;
+    uint64_t inserted = 0;
+    GdbOverflowError(const std::string& ph, uint64_t ins)
+        : std::runtime_error("hash table overflow in phase " + ph), phase(ph), inserted(ins) {}
+};
+
+struct HipExecutionResult {
+    float totalKernelTimeMs = 0.0f;          // measured run: first fill -> last kernel end (incl. host hooks)
+    std::vector<float> phaseTimesMs;         // measured run, per phase: the phase kernel alone
+    std::vector<float> phaseFillMs;          // measured run, per phase: buffer-init fills before the kernel
+    std::vector<std::string> phaseNames;
+    std::vector<std::array<uint64_t, 4>> phaseStatus;  // status words of phases that have them
+    float bufferAllocTimeMs = 0.0f;
+    std::vector<unsigned> phaseGrid, phaseBlock, phaseItems;
+};
+
+class HipGenericExecutor {
+public:
+    HipGenericExecutor(gdb_ctx ctx, gdb_stream stream = nullptr);
+    ~HipGenericExecutor();
+    HipGenericExecutor(const HipGenericExecutor&) = delete;
+    HipGenericExecutor& operator=(const HipGenericExecutor&) = delete;
+
+    // Tables: row count (also symbol n_<table>) and device column pointers
+    // (not owned). Replaces registerTableBuffer/registerTableRowCount.
+    void registerTable(const std::string& table, uint64_t rows);
+    void registerTableColumn(const std::string& table, const std::string& column, const void* devptr);
+    // External device buffer referenced by name (not owned).
+    void registerBuffer(const std::string& name, void* devptr, size_t bytes);
+    // Let a phase scan a runner-owned buffer as a table column.
+    void registerBufferTable(const std::string& table, const std::string& column, const std::string& buffer,
+                             const std::string& rowsExpr);
+    void registerSymbol(const std::string& name, uint64_t value);
+    bool tryGetSymbol(const std::string& name, uint64_t& out) const;
+    void registerScalarInt(const std::string& name, int64_t value);
+    // Host callback after a phase completes (the runner synchronises first).
+    void setPhaseHook(const std::string& phase, std::function<void(HipGenericExecutor&)> hook);
+    void clearPhaseHooks() { hooks_.clear(); }
+    // When true, buffers are not re-initialised (accumulate across runs).
+    void setSkipInit(bool skip) { skipInit_ = skip; }
+
+    // Allocate every buffer of phases [first, last) that can be sized now.
+    void prepare(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
+    // Launch phases [first, last) on the stream: init fills + kernels only.
+    // Hooks are ignored; no synchronisation.
+    void enqueue(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq, const HipCodegen& cg,
+                 int firstPhase = 0, int lastPhase = -1);
+    // Warm up, then time `measured` runs (results of the last run remain).
+    HipExecutionResult execute(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq, const HipCodegen& cg,
+                               int warmupRuns = 0, int measuredRuns = 1, int firstPhase = 0, int lastPhase = -1);
+    // Read the status words of the plan's phases; throws GdbOverflowError.
+    std::vector<std::array<uint64_t, 4>> checkStatus(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
+
+    GenericResult collectResult(const HipCodegen& cg) const;
+    void releaseAllocatedBuffers();
+    void* getAllocatedBuffer(const std::string& name) const;
+    size_t getAllocatedBytes(const std::string& name) const;
+    void readBuffer(const std::string& name, size_t offset, void* host, size_t bytes) const;
+    uint64_t resolve(const std::string& expr) const { return sizes_.resolve(expr); }
+    size_t ownedBytes() const;
+
+    gdb_stream stream() const { return stream_; }
+    gdb_ctx context() const { return ctx_; }
+    int cuCount() const { return cuCount_; }
+
+private:
+    struct Buffer {
+        void* ptr = nullptr;
+        size_t bytes = 0;
+        bool owned = true;
+    };
+    struct TableInfo {
+        uint64_t rows = 0;
+        std::unordered_map<std::string, const void*> columns;
+    };
+    struct BufferTable {
+        std::string table, column, buffer, rowsExpr;
+    };
+
+    gdb_ctx ctx_;
+    gdb_stream stream_;
+    int cuCount_ = 0;
+    HipSizeResolver sizes_;
+    std::unordered_map<std::string, TableInfo> tables_;
+    std::unordered_map<std::string, Buffer> buffers_;
+    std::vector<BufferTable> bufferTables_;
+    std::unordered_map<std::string, int64_t> scalars_;
+    std::unordered_map<std::string, std::function<void(HipGenericExecutor&)>> hooks_;
+    std::map<std::pair<gdb_kernel, int>, int> occupancy_;
+    bool skipInit_ = false;
+    gdb_event evA_ = nullptr;
+    std::vector<gdb_event> events_;
+    std::vector<float> measuredPhase_, measuredFill_;
+
+    Buffer& ensureBuffer(const HipParamBinding& b);
+    void resolveBufferTables();
+    void launchPhase(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq, const HipCodegen::PhaseInfo& p,
+                     std::unordered_map<std::string, bool>& initDone, const std::unordered_map<std::string, const HipParamBinding*>& initSpec,
+                     unsigned* gridOut, gdb_event beforeKernel = nullptr);
+    std::unordered_map<std::string, const HipParamBinding*> initSpecs(const HipCodegen& cg) const;
+    int occupancy(gdb_kernel k, int block);
+};
+
+}  // namespace codegen
+}  // namespace gdb

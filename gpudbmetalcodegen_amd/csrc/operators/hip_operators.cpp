@@ -1,0 +1,868 @@
+#include "hip_operators.h"
+
+#include <sstream>
+#include <stdexcept>
+
+namespace gdb {
+namespace codegen {
+
+namespace {
+
+std::vector<uint8_t> wordsPattern(const std::vector<uint64_t>& words) {
+    std::vector<uint8_t> p(words.size() * 8);
+    for (size_t i = 0; i < words.size(); ++i)
+        for (int b = 0; b < 8; ++b) p[i * 8 + b] = static_cast<uint8_t>(words[i] >> (8 * b));
+    return p;
+}
+
+constexpr uint64_t kEmptyWord = 0x00000000FFFFFFFFull;
+constexpr uint64_t kI64Max = 0x7FFFFFFFFFFFFFFFull;
+constexpr uint64_t kI64Min = 0x8000000000000000ull;
+
+std::string s(long long v) { return std::to_string(v); }
+
+}  // namespace
+
+void HipOperator::describeTree(std::string& out, int depth) const {
+    out += std::string(depth * 2, ' ') + describe() + "\n";
+}
+
+void HipUnaryOperator::describeTree(std::string& out, int depth) const {
+    out += std::string(depth * 2, ' ') + describe() + "\n";
+    if (child_) child_->describeTree(out, depth + 1);
+}
+
+// ===========================================================================
+// HipGridStrideScan
+// ===========================================================================
+
+HipGridStrideScan::HipGridStrideScan(const std::string& table, const std::string& idxVar)
+    : table_(table), idxVar_(idxVar) {}
+
+void HipGridStrideScan::addColumn(const std::string& column, ColumnType type) {
+    for (const auto& c : columns_)
+        if (c.name == column) return;
+    columns_.push_back({column, type});
+}
+
+void HipGridStrideScan::produce(HipCodegen& cg, ConsumerFn consume) {
+    if (columns_.empty())
+        throw std::runtime_error("HipGridStrideScan(" + table_ + "): no columns registered; call addColumn() first");
+    cg.setPhaseScannedTable(table_);
+    std::vector<std::string> ptr;
+    for (const auto& c : columns_) ptr.push_back(cg.addColumnParam(table_, c.name, c.type));
+    const std::string n = cg.addTableSizeParam(table_);
+    const int lb = cg.getPhases().back().launchBound;
+
+    cg.addComment("scan " + table_ + ": tiles of blockDim.x*ITEMS rows, ITEMS consecutive rows per thread");
+    cg.addBlock("", [&]() {
+        cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
+        cg.addLine("const u64 _step = (u64)gridDim.x * _tile;");
+        for (size_t k = 0; k < columns_.size(); ++k) {
+            const auto& c = columns_[k];
+            if (c.type.type == ColType::CHAR) {
+                cg.addLine("__shared__ __attribute__((aligned(16))) char _s_" + c.name + "[" + s(lb) + " * ITEMS * " +
+                           s(c.type.width) + "];");
+            }
+        }
+        cg.addBlock("for (u64 _t0 = (u64)blockIdx.x * _tile; _t0 < " + n + "; _t0 += _step)", [&]() {
+            cg.addLine("const u64 _r0 = _t0 + (u64)threadIdx.x * ITEMS;");
+            cg.addLine("const bool _full = _t0 + _tile <= " + n + ";");
+            bool anyChar = false;
+            for (size_t k = 0; k < columns_.size(); ++k) {
+                const auto& c = columns_[k];
+                if (c.type.type == ColType::CHAR) {
+                    anyChar = true;
+                    const std::string W = s(c.type.width);
+                    cg.addComment("stage the tile's " + c.name + " bytes in LDS with 16-byte loads");
+                    cg.addBlock("", [&]() {
+                        cg.addLine("const u64 _nrows = (_full ? _tile : " + n + " - _t0);");
+                        cg.addLine("const u64 _nb = _nrows * " + W + ";");
+                        cg.addLine("const char* _src = (const char*)" + ptr[k] + " + _t0 * " + W + ";");
+                        cg.addBlock("if (_full && (_nb % 16) == 0)", [&]() {
+                            cg.addLine("for (u64 _k = threadIdx.x; _k < _nb / 16; _k += blockDim.x)");
+                            cg.addLine("    reinterpret_cast<u32x4*>(_s_" + c.name +
+                                       ")[_k] = gdb_ld_stream(reinterpret_cast<const u32x4*>(_src) + _k);");
+                        }, " else {");
+                        {
+                            IndentGuard g(cg);
+                            cg.addLine("for (u64 _k = threadIdx.x; _k < _nb; _k += blockDim.x) _s_" + c.name +
+                                       "[_k] = _src[_k];");
+                        }
+                        cg.addLine("}");
+                    });
+                    continue;
+                }
+                const std::string T = c.type.cxx();
+                cg.addLine(T + " _r_" + c.name + "[ITEMS];");
+            }
+            if (anyChar) cg.addLine("__syncthreads();");
+            cg.addBlock("if (_full)", [&]() {
+                for (size_t k = 0; k < columns_.size(); ++k) {
+                    const auto& c = columns_[k];
+                    if (c.type.type == ColType::CHAR) continue;
+                    cg.addLine("gdb_load_items<" + c.type.cxx() + ", ITEMS>(" + ptr[k] + " + _r0, _r_" + c.name + ");");
+                }
+            }, " else {");
+            {
+                IndentGuard g(cg);
+                for (size_t k = 0; k < columns_.size(); ++k) {
+                    const auto& c = columns_[k];
+                    if (c.type.type == ColType::CHAR) continue;
+                    cg.addLine("gdb_load_tail<" + c.type.cxx() + ", ITEMS>(" + ptr[k] + ", _r0, " + n + ", _r_" +
+                               c.name + ");");
+                }
+            }
+            cg.addLine("}");
+            cg.addLine("#pragma unroll");
+            cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
+                cg.addLine("const u64 " + idxVar_ + " = _r0 + (u64)_u;");
+                cg.addBlock("if (_full || " + idxVar_ + " < " + n + ")", [&]() {
+                    for (const auto& c : columns_) {
+                        if (c.type.type == ColType::CHAR) {
+                            cg.addLine("const char* " + c.name + " = _s_" + c.name + " + ((u64)threadIdx.x * ITEMS + _u) * " +
+                                       s(c.type.width) + ";");
+                        } else {
+                            cg.addLine("const " + c.type.cxx() + " " + c.name + " = _r_" + c.name + "[_u];");
+                        }
+                    }
+                    consume();
+                });
+            });
+            if (anyChar) cg.addLine("__syncthreads();");
+        });
+    });
+}
+
+std::string HipGridStrideScan::describe() const {
+    std::string d = "GridStrideScan(" + table_ + ":";
+    for (const auto& c : columns_) d += " " + c.name + "/" + c.type.tag();
+    return d + ")";
+}
+
+// ===========================================================================
+// HipHashTableScan
+// ===========================================================================
+
+HipHashTableScan::HipHashTableScan(const std::string& table, const std::string& slotsExpr, int slotWords,
+                                   const std::string& prefix)
+    : table_(table), slotsExpr_(slotsExpr), prefix_(prefix), slotWords_(slotWords) {}
+
+void HipHashTableScan::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.setPhaseRowsExpr(slotsExpr_);
+    cg.addBufferParam(table_, "u64", 8, slotsExpr_ + " * " + s(slotWords_), InitKind::None, {}, true, true);
+    const std::string n = "nslots_" + table_;
+    cg.addSymbolParam(n, slotsExpr_);
+    cg.addComment("scan the occupied slots of hash table " + table_);
+    cg.addBlock("", [&]() {
+        cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
+        cg.addBlock("for (u64 _t0 = (u64)blockIdx.x * _tile; _t0 < " + n + "; _t0 += (u64)gridDim.x * _tile)", [&]() {
+            cg.addLine("#pragma unroll");
+            cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
+                cg.addLine("const u64 _slot = _t0 + (u64)threadIdx.x * ITEMS + (u64)_u;");
+                cg.addBlock("if (_slot < " + n + ")", [&]() {
+                    cg.addLine("const u64* _sw = " + table_ + " + _slot * " + s(slotWords_) + ";");
+                    cg.addLine("const u64 _w0 = _sw[0];");
+                    cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+                        cg.addLine("const u32 " + prefix_ + "_key = (u32)_w0;");
+                        cg.addLine("const u32 " + prefix_ + "_payload = (u32)(_w0 >> 32);");
+                        for (int w = 1; w < slotWords_; ++w)
+                            cg.addLine("const u64 " + prefix_ + "_w" + s(w) + " = _sw[" + s(w) + "];");
+                        consume();
+                    });
+                });
+            });
+        });
+    });
+}
+
+std::string HipHashTableScan::describe() const { return "HashTableScan(" + table_ + ", slots=" + slotsExpr_ + ")"; }
+
+// ===========================================================================
+// HipSelection / HipComputeExpr
+// ===========================================================================
+
+HipSelection::HipSelection(std::unique_ptr<HipOperator> child, const std::string& predicate)
+    : HipUnaryOperator(std::move(child)), predicate_(predicate) {}
+
+void HipSelection::produce(HipCodegen& cg, ConsumerFn consume) {
+    child_->produce(cg, [&]() { cg.addIf(predicate_, [&]() { consume(); }); });
+}
+
+std::string HipSelection::describe() const { return "Selection(" + predicate_ + ")"; }
+
+HipComputeExpr::HipComputeExpr(std::unique_ptr<HipOperator> child, const std::string& varName,
+                               const std::string& varType, const std::string& expression)
+    : HipUnaryOperator(std::move(child)), varName_(varName), varType_(varType), expression_(expression) {}
+
+void HipComputeExpr::produce(HipCodegen& cg, ConsumerFn consume) {
+    child_->produce(cg, [&]() {
+        cg.addLine("const " + varType_ + " " + varName_ + " = " + expression_ + ";");
+        consume();
+    });
+}
+
+std::string HipComputeExpr::describe() const { return "ComputeExpr(" + varName_ + " = " + expression_ + ")"; }
+
+// ===========================================================================
+// Bitmaps
+// ===========================================================================
+
+HipBitmapBuild::HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::string& bitmapName,
+                               const std::string& keyExpr, const std::string& bitsExpr)
+    : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr) {}
+
+void HipBitmapBuild::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBitmapWriteParam(bitmap_, bits_);
+    child_->produce(cg, [&]() {
+        cg.addLine("gdb_bitmap_set(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" + key_ + "));");
+        consume();
+    });
+}
+
+std::string HipBitmapBuild::describe() const { return "BitmapBuild(" + bitmap_ + ", key=" + key_ + ")"; }
+
+HipBitmapProbe::HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::string& bitmapName,
+                               const std::string& keyExpr, const std::string& bitsExpr, bool anti)
+    : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr), anti_(anti) {}
+
+void HipBitmapProbe::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBitmapReadParam(bitmap_, bits_);
+    child_->produce(cg, [&]() {
+        cg.addIf(std::string(anti_ ? "!" : "") + "gdb_bitmap_test(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" +
+                     key_ + "))",
+                 [&]() { consume(); });
+    });
+}
+
+std::string HipBitmapProbe::describe() const {
+    return std::string(anti_ ? "AntiBitmapProbe(" : "BitmapProbe(") + bitmap_ + ", key=" + key_ + ")";
+}
+
+// ===========================================================================
+// Hash build / probe
+// ===========================================================================
+
+HipHashBuild::HipHashBuild(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                           const std::string& payloadExpr, const std::string& slotsExpr, int slotWords)
+    : HipUnaryOperator(std::move(child)),
+      table_(table),
+      key_(keyExpr),
+      payload_(payloadExpr),
+      slots_(slotsExpr),
+      slotWords_(slotWords) {}
+
+void HipHashBuild::produce(HipCodegen& cg, ConsumerFn consume) {
+    std::vector<uint64_t> pat(static_cast<size_t>(slotWords_), 0);
+    pat[0] = kEmptyWord;
+    cg.addBufferParam(table_, "u64", 8, slots_ + " * " + s(slotWords_), InitKind::Pattern, wordsPattern(pat), false,
+                      true);
+    cg.addSymbolParam("nslots_" + table_, slots_);
+    const std::string st = table_ + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    cg.addLine("const u32 _bits_" + table_ + " = (u32)__builtin_ctzll(nslots_" + table_ + ");");
+    cg.addLine("const u32 _maxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
+               " : 65536ull);");
+    cg.addLine("u32 _ins_" + table_ + " = 0, _ovf_" + table_ + " = 0, _dup_" + table_ + " = 0;");
+    child_->produce(cg, [&]() {
+        cg.addLine("const i64 _hs = gdb_ht_insert_unique(" + table_ + ", " + s(slotWords_) + ", _bits_" + table_ +
+                   ", (u32)(" + key_ + "), (u32)(" + payload_ + "), _maxp_" + table_ + ");");
+        cg.addLine("if (_hs >= 0) ++_ins_" + table_ + "; else if (_hs == -1) ++_ovf_" + table_ + "; else ++_dup_" +
+                   table_ + ";");
+        consume();
+    });
+    cg.addComment("build status: overflow / duplicate flags and inserted-key count");
+    cg.addBlock("", [&]() {
+        cg.addLine("const u64 _a = gdb_wave_sum_u64(_ins_" + table_ + ");");
+        cg.addLine("const u64 _b = gdb_wave_sum_u64(_ovf_" + table_ + ");");
+        cg.addLine("const u64 _c = gdb_wave_sum_u64(_dup_" + table_ + ");");
+        cg.addBlock("if (gdb_lane() == 0)", [&]() {
+            cg.addLine("if (_a) gdb_atomic_add_u64(" + st + " + 2, _a);");
+            cg.addLine("if (_b) gdb_atomic_max_u64(" + st + " + 0, 1ull);");
+            cg.addLine("if (_c) gdb_atomic_max_u64(" + st + " + 1, 1ull);");
+        });
+    });
+}
+
+std::string HipHashBuild::describe() const {
+    return "HashBuild(" + table_ + ", key=" + key_ + ", payload=" + payload_ + ", slots=" + slots_ + ")";
+}
+
+HipHashProbe::HipHashProbe(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                           const std::string& slotsExpr, int slotWords, const std::string& slotVar)
+    : HipUnaryOperator(std::move(child)),
+      table_(table),
+      key_(keyExpr),
+      slots_(slotsExpr),
+      slotVar_(slotVar),
+      slotWords_(slotWords) {}
+
+void HipHashProbe::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(table_, "u64", 8, slots_ + " * " + s(slotWords_), InitKind::None, {}, true, true);
+    cg.addSymbolParam("nslots_" + table_, slots_);
+    cg.addLine("const u32 _bits_" + table_ + " = (u32)__builtin_ctzll(nslots_" + table_ + ");");
+    cg.addLine("const u32 _maxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
+               " : 65536ull);");
+    child_->produce(cg, [&]() {
+        cg.addLine("const i64 " + slotVar_ + " = gdb_ht_find(" + table_ + ", " + s(slotWords_) + ", _bits_" + table_ +
+                   ", (u32)(" + key_ + "), _maxp_" + table_ + ");");
+        cg.addIf(slotVar_ + " >= 0", [&]() {
+            cg.addLine("const u32 " + slotVar_ + "_payload = (u32)(" + table_ + "[" + slotVar_ + " * " +
+                       s(slotWords_) + "] >> 32);");
+            consume();
+        });
+    });
+}
+
+std::string HipHashProbe::describe() const { return "HashProbe(" + table_ + ", key=" + key_ + ")"; }
+
+HipSlotAtomicAdd::HipSlotAtomicAdd(std::unique_ptr<HipOperator> child, const std::string& table, int slotWords,
+                                   const std::string& slotVar, int word, const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)),
+      table_(table),
+      slotVar_(slotVar),
+      value_(valueExpr),
+      slotWords_(slotWords),
+      word_(word) {}
+
+void HipSlotAtomicAdd::produce(HipCodegen& cg, ConsumerFn consume) {
+    // The probe registered the table read-only; the atomic makes it writable.
+    cg.addBufferParam(table_, "u64", 8, "", InitKind::None, {}, false, true);
+    child_->produce(cg, [&]() {
+        cg.addLine("gdb_atomic_add_u64(" + table_ + " + " + slotVar_ + " * " + s(slotWords_) + " + " + s(word_) +
+                   ", (u64)(i64)(" + value_ + "));");
+        consume();
+    });
+}
+
+std::string HipSlotAtomicAdd::describe() const {
+    return "SlotAtomicAdd(" + table_ + "[" + slotVar_ + "].w" + s(word_) + " += " + value_ + ")";
+}
+
+HipArrayLookup::HipArrayLookup(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                               const std::string& keyExpr, const std::string& sizeExpr, const std::string& resultVar,
+                               long long sentinel)
+    : HipUnaryOperator(std::move(child)),
+      array_(arrayName),
+      key_(keyExpr),
+      size_(sizeExpr),
+      var_(resultVar),
+      sentinel_(sentinel) {}
+
+void HipArrayLookup::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(array_, "u32", 4, size_, InitKind::None, {}, true, true);
+    cg.addSymbolParam("nelem_" + array_, size_);
+    child_->produce(cg, [&]() {
+        cg.addLine("const u64 _ak_" + var_ + " = (u64)(i64)(" + key_ + ");");
+        cg.addLine("const i64 " + var_ + " = _ak_" + var_ + " < nelem_" + array_ + " ? (i64)" + array_ + "[_ak_" +
+                   var_ + "] : (i64)(" + s(sentinel_) + ");");
+        cg.addIf(var_ + " != (i64)(" + s(sentinel_) + ")", [&]() { consume(); });
+    });
+}
+
+std::string HipArrayLookup::describe() const { return "ArrayLookup(" + var_ + " = " + array_ + "[" + key_ + "])"; }
+
+// ===========================================================================
+// HipKeyedAgg
+// ===========================================================================
+
+HipKeyedAgg::HipKeyedAgg(std::unique_ptr<HipOperator> child, const std::string& outputName,
+                         const std::string& bucketExpr, int numBuckets)
+    : HipUnaryOperator(std::move(child)), out_(outputName), bucket_(bucketExpr), numBuckets_(numBuckets) {
+    if (numBuckets_ <= 0) throw std::runtime_error("HipKeyedAgg: numBuckets must be positive");
+}
+
+void HipKeyedAgg::addAggregate(const std::string& name, AggKind kind, const std::string& valueExpr,
+                               int scaleDigits) {
+    aggs_.push_back({name, kind, valueExpr, scaleDigits, 0});
+}
+
+void HipKeyedAgg::setBucketLabels(const std::vector<std::string>& labels, const std::string& keyColumn) {
+    labels_ = labels;
+    keyColumn_ = keyColumn;
+}
+
+void HipKeyedAgg::addAverage(const std::string& name, const std::string& numerator, const std::string& denominator) {
+    avgs_.push_back({name, {numerator, denominator}});
+}
+
+void HipKeyedAgg::produce(HipCodegen& cg, ConsumerFn consume) {
+    if (aggs_.empty()) throw std::runtime_error("HipKeyedAgg(" + out_ + "): no aggregates");
+    // Word layout per bucket; buckets are padded to 128 bytes so the per-block
+    // atomics of different buckets land on different cache lines.
+    int words = 0;
+    for (auto& a : aggs_) {
+        a.wordOffset = words;
+        words += (a.kind == AggKind::SumI128) ? 2 : 1;
+    }
+    const int wpb = ((words + 15) / 16) * 16;
+    std::vector<uint64_t> pat(static_cast<size_t>(wpb), 0);
+    for (const auto& a : aggs_) {
+        if (a.kind == AggKind::MinI64) pat[a.wordOffset] = kI64Max;
+        if (a.kind == AggKind::MaxI64) pat[a.wordOffset] = kI64Min;
+    }
+    const std::string NB = s(numBuckets_);
+    cg.addBufferParam(out_, "u64", 8, s((long long)numBuckets_ * wpb), InitKind::Pattern, wordsPattern(pat));
+
+    auto neutral = [](AggKind k) -> std::string {
+        if (k == AggKind::MinI64) return "GDB_I64_MAX";
+        if (k == AggKind::MaxI64) return "GDB_I64_MIN";
+        return "0";
+    };
+
+    const bool regs = numBuckets_ <= 8;
+    const long long ldsBytes = (long long)numBuckets_ * words * 8;
+    const bool lds = !regs && ldsBytes <= 48 * 1024;
+
+    if (regs) {
+        cg.addComment("KeyedAgg " + out_ + ": register accumulators, " + NB + " bucket(s)");
+        for (int b = 0; b < numBuckets_; ++b) {
+            for (const auto& a : aggs_) {
+                const std::string base = "_ka_" + out_ + "_" + a.name + "_b" + s(b);
+                switch (a.kind) {
+                    case AggKind::SumI64: cg.addLine("i64 " + base + " = 0;"); break;
+                    case AggKind::SumI128:
+                        cg.addLine("u64 " + base + "_lo = 0;");
+                        cg.addLine("i64 " + base + "_hi = 0;");
+                        break;
+                    case AggKind::Count: cg.addLine("u32 " + base + " = 0;"); break;
+                    case AggKind::MinI64: cg.addLine("i64 " + base + " = GDB_I64_MAX;"); break;
+                    case AggKind::MaxI64: cg.addLine("i64 " + base + " = GDB_I64_MIN;"); break;
+                }
+            }
+        }
+        child_->produce(cg, [&]() {
+            if (numBuckets_ > 1) cg.addLine("const int _bk = (int)(" + bucket_ + ");");
+            for (const auto& a : aggs_)
+                if (a.kind != AggKind::Count)
+                    cg.addLine("const i64 _v_" + a.name + " = (i64)(" + a.valueExpr + ");");
+            for (int b = 0; b < numBuckets_; ++b) {
+                auto body = [&]() {
+                    for (const auto& a : aggs_) {
+                        const std::string base = "_ka_" + out_ + "_" + a.name + "_b" + s(b);
+                        const std::string v = "_v_" + a.name;
+                        switch (a.kind) {
+                            case AggKind::SumI64: cg.addLine(base + " += " + v + ";"); break;
+                            case AggKind::SumI128:
+                                cg.addLine(base + "_lo += (u64)(u32)" + v + ";");
+                                cg.addLine(base + "_hi += (" + v + " >> 32);");
+                                break;
+                            case AggKind::Count: cg.addLine(base + " += 1u;"); break;
+                            case AggKind::MinI64: cg.addLine(base + " = gdb_min64(" + base + ", " + v + ");"); break;
+                            case AggKind::MaxI64: cg.addLine(base + " = gdb_max64(" + base + ", " + v + ");"); break;
+                        }
+                    }
+                };
+                if (numBuckets_ > 1)
+                    cg.addIf("_bk == " + s(b), body);
+                else
+                    body();
+            }
+            consume();
+        });
+        // Slot table: (bucket, word) -> op; op 0 add, 1 min, 2 max.
+        std::ostringstream ops, wordIdx;
+        int ns = 0;
+        std::vector<std::pair<std::string, int>> slotExprs;  // (value expr, op)
+        for (int b = 0; b < numBuckets_; ++b) {
+            for (const auto& a : aggs_) {
+                const std::string base = "_ka_" + out_ + "_" + a.name + "_b" + s(b);
+                auto add = [&](const std::string& expr, int op, int w) {
+                    if (ns) {
+                        ops << ",";
+                        wordIdx << ",";
+                    }
+                    ops << op;
+                    wordIdx << (b * wpb + w);
+                    slotExprs.push_back({expr, op});
+                    ++ns;
+                };
+                switch (a.kind) {
+                    case AggKind::SumI64: add("(u64)" + base, 0, a.wordOffset); break;
+                    case AggKind::SumI128:
+                        add(base + "_lo", 0, a.wordOffset);
+                        add("(u64)" + base + "_hi", 0, a.wordOffset + 1);
+                        break;
+                    case AggKind::Count: add("(u64)" + base, 0, a.wordOffset); break;
+                    case AggKind::MinI64: add(base, 1, a.wordOffset); break;
+                    case AggKind::MaxI64: add(base, 2, a.wordOffset); break;
+                }
+            }
+        }
+        const std::string NS = s(ns);
+        cg.addHelper("__constant__ int " + out_ + "_ops[" + NS + "] = {" + ops.str() + "};\n__constant__ int " + out_ +
+                     "_words[" + NS + "] = {" + wordIdx.str() + "};");
+        cg.addComment("block reduction: wave DPP sums -> LDS -> one global atomic per slot and block");
+        cg.addBlock("", [&]() {
+            cg.addLine("__shared__ u64 _red[" + NS + " * 16];");
+            cg.addLine("const int _lane = gdb_lane(), _wave = gdb_wave(), _nw = (int)(blockDim.x >> 6);");
+            for (int k = 0; k < ns; ++k) {
+                const auto& se = slotExprs[static_cast<size_t>(k)];
+                std::string red;
+                if (se.second == 0)
+                    red = "gdb_wave_sum_u64(" + se.first + ")";
+                else if (se.second == 1)
+                    red = "(u64)gdb_wave_min_i64(" + se.first + ")";
+                else
+                    red = "(u64)gdb_wave_max_i64(" + se.first + ")";
+                cg.addLine("{ const u64 _w = " + red + "; if (_lane == 0) _red[_wave * " + NS + " + " + s(k) +
+                           "] = _w; }");
+            }
+            cg.addLine("__syncthreads();");
+            cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + NS + "; _s += (int)blockDim.x)", [&]() {
+                cg.addLine("const int _op = " + out_ + "_ops[_s];");
+                cg.addLine("u64 _acc = _red[_s];");
+                cg.addBlock("for (int _w2 = 1; _w2 < _nw; ++_w2)", [&]() {
+                    cg.addLine("const u64 _x = _red[_w2 * " + NS + " + _s];");
+                    cg.addLine("_acc = _op == 0 ? _acc + _x : (_op == 1 ? (u64)gdb_min64((i64)_acc, (i64)_x) : "
+                               "(u64)gdb_max64((i64)_acc, (i64)_x));");
+                });
+                cg.addLine("u64* _dst = " + out_ + " + " + out_ + "_words[_s];");
+                cg.addLine("if (_op == 0) { if (_acc != 0) gdb_atomic_add_u64(_dst, _acc); }");
+                cg.addLine("else if (_op == 1) { if ((i64)_acc != GDB_I64_MAX) gdb_atomic_min_i64((i64*)_dst, (i64)_acc); }");
+                cg.addLine("else { if ((i64)_acc != GDB_I64_MIN) gdb_atomic_max_i64((i64*)_dst, (i64)_acc); }");
+            });
+        });
+    } else if (lds) {
+        const std::string W = s(words);
+        cg.addComment("KeyedAgg " + out_ + ": block-shared LDS table, " + NB + " buckets x " + W + " words");
+        cg.addLine("__shared__ u64 _kl_" + out_ + "[" + s((long long)numBuckets_ * words) + "];");
+        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + s((long long)numBuckets_ * words) +
+                        "; _s += (int)blockDim.x)",
+                    [&]() {
+                        cg.addLine("const int _w = _s % " + W + ";");
+                        std::string init = "0";
+                        for (const auto& a : aggs_) {
+                            if (a.kind == AggKind::MinI64)
+                                init = "(_w == " + s(a.wordOffset) + " ? (u64)GDB_I64_MAX : " + init + ")";
+                            if (a.kind == AggKind::MaxI64)
+                                init = "(_w == " + s(a.wordOffset) + " ? (u64)GDB_I64_MIN : " + init + ")";
+                        }
+                        cg.addLine("_kl_" + out_ + "[_s] = " + init + ";");
+                    });
+        cg.addLine("__syncthreads();");
+        child_->produce(cg, [&]() {
+            cg.addLine("const int _bk = (int)(" + bucket_ + ");");
+            cg.addBlock("if ((unsigned)_bk < " + NB + "u)", [&]() {
+                cg.addLine("u64* _kb = _kl_" + out_ + " + _bk * " + W + ";");
+                for (const auto& a : aggs_) {
+                    const std::string v = "(i64)(" + a.valueExpr + ")";
+                    const std::string w = s(a.wordOffset);
+                    switch (a.kind) {
+                        case AggKind::SumI64: cg.addLine("gdb_lds_add_u64(_kb + " + w + ", (u64)" + v + ");"); break;
+                        case AggKind::SumI128:
+                            cg.addBlock("", [&]() {
+                                cg.addLine("const i64 _v = " + v + ";");
+                                cg.addLine("gdb_lds_add_u64(_kb + " + w + ", (u64)(u32)_v);");
+                                cg.addLine("gdb_lds_add_u64(_kb + " + s(a.wordOffset + 1) + ", (u64)(_v >> 32));");
+                            });
+                            break;
+                        case AggKind::Count: cg.addLine("gdb_lds_add_u64(_kb + " + w + ", 1ull);"); break;
+                        case AggKind::MinI64: cg.addLine("gdb_lds_min_i64((i64*)(_kb + " + w + "), " + v + ");"); break;
+                        case AggKind::MaxI64: cg.addLine("gdb_lds_max_i64((i64*)(_kb + " + w + "), " + v + ");"); break;
+                    }
+                }
+            });
+            consume();
+        });
+        cg.addLine("__syncthreads();");
+        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + s((long long)numBuckets_ * words) +
+                        "; _s += (int)blockDim.x)",
+                    [&]() {
+                        cg.addLine("const int _b = _s / " + W + ", _w = _s % " + W + ";");
+                        cg.addLine("const u64 _x = _kl_" + out_ + "[_s];");
+                        cg.addLine("u64* _dst = " + out_ + " + _b * " + s(wpb) + " + _w;");
+                        std::string minCond = "false", maxCond = "false";
+                        for (const auto& a : aggs_) {
+                            if (a.kind == AggKind::MinI64) minCond += " || _w == " + s(a.wordOffset);
+                            if (a.kind == AggKind::MaxI64) maxCond += " || _w == " + s(a.wordOffset);
+                        }
+                        cg.addLine("if (" + minCond + ") { if ((i64)_x != GDB_I64_MAX) gdb_atomic_min_i64((i64*)_dst, (i64)_x); }");
+                        cg.addLine("else if (" + maxCond + ") { if ((i64)_x != GDB_I64_MIN) gdb_atomic_max_i64((i64*)_dst, (i64)_x); }");
+                        cg.addLine("else if (_x != 0) gdb_atomic_add_u64(_dst, _x);");
+                    });
+    } else {
+        cg.addComment("KeyedAgg " + out_ + ": per-row global atomics, " + NB + " buckets");
+        child_->produce(cg, [&]() {
+            cg.addLine("const int _bk = (int)(" + bucket_ + ");");
+            cg.addBlock("if ((unsigned)_bk < " + NB + "u)", [&]() {
+                cg.addLine("u64* _kb = " + out_ + " + (u64)_bk * " + s(wpb) + ";");
+                for (const auto& a : aggs_) {
+                    const std::string v = "(i64)(" + a.valueExpr + ")";
+                    const std::string w = s(a.wordOffset);
+                    switch (a.kind) {
+                        case AggKind::SumI64: cg.addLine("gdb_atomic_add_u64(_kb + " + w + ", (u64)" + v + ");"); break;
+                        case AggKind::SumI128:
+                            cg.addBlock("", [&]() {
+                                cg.addLine("const i64 _v = " + v + ";");
+                                cg.addLine("gdb_atomic_add_u64(_kb + " + w + ", (u64)(u32)_v);");
+                                cg.addLine("gdb_atomic_add_u64(_kb + " + s(a.wordOffset + 1) + ", (u64)(_v >> 32));");
+                            });
+                            break;
+                        case AggKind::Count: cg.addLine("gdb_atomic_add_u64(_kb + " + w + ", 1ull);"); break;
+                        case AggKind::MinI64: cg.addLine("gdb_atomic_min_i64((i64*)(_kb + " + w + "), " + v + ");"); break;
+                        case AggKind::MaxI64: cg.addLine("gdb_atomic_max_i64((i64*)(_kb + " + w + "), " + v + ");"); break;
+                    }
+                }
+            });
+            consume();
+        });
+    }
+
+    auto& rs = cg.resultSchema();
+    rs.kind = HipResultSchema::KEYED_AGG;
+    rs.keyed.bufferName = out_;
+    rs.keyed.numBuckets = numBuckets_;
+    rs.keyed.wordsPerBucket = wpb;
+    rs.keyed.slots.clear();
+    for (const auto& a : aggs_) rs.keyed.slots.push_back({a.name, a.kind, a.wordOffset, a.scaleDigits});
+    rs.keyed.bucketLabels = labels_;
+    rs.keyed.keyColumn = keyColumn_;
+    rs.keyed.averages = avgs_;
+}
+
+std::string HipKeyedAgg::describe() const {
+    return "KeyedAgg(" + out_ + ", " + s(numBuckets_) + " buckets, " + s((long long)aggs_.size()) + " aggs, bucket=" +
+           bucket_ + ")";
+}
+
+// ===========================================================================
+// HipAtomicCount
+// ===========================================================================
+
+HipAtomicCount::HipAtomicCount(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                               const std::string& bucketExpr, const std::string& sizeExpr, int staticSize)
+    : HipUnaryOperator(std::move(child)), array_(arrayName), bucket_(bucketExpr), size_(sizeExpr), static_(staticSize) {}
+
+void HipAtomicCount::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(array_, "u32", 4, size_, InitKind::Zero, {}, false, true);
+    cg.addSymbolParam("nelem_" + array_, size_);
+    if (static_ > 0 && static_ <= 8192) {
+        const std::string S = s(static_);
+        cg.addComment("AtomicCount " + array_ + ": LDS histogram of " + S + " bins");
+        cg.addLine("__shared__ u32 _hist_" + array_ + "[" + S + "];");
+        cg.addLine("for (int _h = (int)threadIdx.x; _h < " + S + "; _h += (int)blockDim.x) _hist_" + array_ + "[_h] = 0;");
+        cg.addLine("__syncthreads();");
+        child_->produce(cg, [&]() {
+            cg.addLine("const u64 _hb = (u64)(i64)(" + bucket_ + ");");
+            cg.addLine("if (_hb < " + S + "ull && _hb < nelem_" + array_ + ") gdb_lds_add_u32(&_hist_" + array_ +
+                       "[_hb], 1u);");
+            consume();
+        });
+        cg.addLine("__syncthreads();");
+        cg.addLine("for (int _h = (int)threadIdx.x; _h < " + S + "; _h += (int)blockDim.x) if ((u64)_h < nelem_" +
+                   array_ + " && _hist_" + array_ + "[_h]) gdb_atomic_add_u32(" + array_ + " + _h, _hist_" + array_ +
+                   "[_h]);");
+    } else {
+        cg.addComment("AtomicCount " + array_ + ": per-row global atomics");
+        child_->produce(cg, [&]() {
+            cg.addLine("const u64 _hb = (u64)(i64)(" + bucket_ + ");");
+            cg.addLine("if (_hb < nelem_" + array_ + ") gdb_atomic_add_u32(" + array_ + " + _hb, 1u);");
+            consume();
+        });
+    }
+    if (isResult_) {
+        auto& rs = cg.resultSchema();
+        rs.kind = HipResultSchema::COUNTS;
+        rs.counts.bufferName = array_;
+        rs.counts.sizeExpr = size_;
+        rs.counts.keyName = keyName_;
+        rs.counts.valueName = valueName_;
+        rs.counts.sortByCountDesc = sortDesc_;
+    }
+}
+
+void HipAtomicCount::setResult(const std::string& keyName, const std::string& valueName, bool sortByCountDesc) {
+    isResult_ = true;
+    keyName_ = keyName;
+    valueName_ = valueName;
+    sortDesc_ = sortByCountDesc;
+}
+
+std::string HipAtomicCount::describe() const { return "AtomicCount(" + array_ + "[" + bucket_ + "])"; }
+
+// ===========================================================================
+// HipHashGroupBy
+// ===========================================================================
+
+HipHashGroupBy::HipHashGroupBy(std::unique_ptr<HipOperator> child, const std::string& table,
+                               const std::string& keyExpr, const std::string& valueExpr, const std::string& slotsExpr,
+                               Variant variant, int ldsSlots)
+    : HipUnaryOperator(std::move(child)),
+      table_(table),
+      key_(keyExpr),
+      value_(valueExpr),
+      slots_(slotsExpr),
+      variant_(variant),
+      ldsSlots_(ldsSlots) {
+    if (variant_ == LDS && (ldsSlots_ < 64 || (ldsSlots_ & (ldsSlots_ - 1)) != 0 || ldsSlots_ > 4096))
+        throw std::runtime_error("HipHashGroupBy: LDS slots must be a power of two in [64, 4096]");
+}
+
+void HipHashGroupBy::produce(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(table_, "u64", 8, slots_ + " * 4", InitKind::Pattern,
+                      wordsPattern({kEmptyWord, 0, kI64Max, kI64Min}), false, true);
+    cg.addSymbolParam("nslots_" + table_, slots_);
+    const std::string st = table_ + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    const std::string T = table_;
+    cg.addLine("const u32 _bits_" + T + " = (u32)__builtin_ctzll(nslots_" + T + ");");
+    cg.addLine("const u32 _maxp_" + T + " = (u32)(nslots_" + T + " < 65536ull ? nslots_" + T + " : 65536ull);");
+    cg.addLine("u32 _ins_" + T + " = 0, _ovf_" + T + " = 0, _spill_" + T + " = 0;");
+    const std::string merge1 = "gdb_gb_merge(" + T + ", _bits_" + T + ", _gk, 1u, _gv, _gv, _gv, _maxp_" + T + ")";
+    if (variant_ == LDS) {
+        int lbits = 0;
+        while ((1 << lbits) < ldsSlots_) ++lbits;
+        const std::string LS = s(ldsSlots_);
+        cg.addComment("HashGroupBy " + T + ": LDS table of " + LS + " slots absorbs hot keys; spills go global");
+        cg.addLine("__shared__ u64 _gl_" + T + "[" + LS + " * 4];");
+        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
+            cg.addLine("_gl_" + T + "[4 * _s] = GDB_EMPTY_WORD; _gl_" + T + "[4 * _s + 1] = 0;");
+            cg.addLine("_gl_" + T + "[4 * _s + 2] = (u64)GDB_I64_MAX; _gl_" + T + "[4 * _s + 3] = (u64)GDB_I64_MIN;");
+        });
+        cg.addLine("__syncthreads();");
+        child_->produce(cg, [&]() {
+            cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
+            cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
+            cg.addBlock("if (gdb_lds_gb_update(_gl_" + T + ", " + s(lbits) + "u, _gk, _gv, 16u) < 0)", [&]() {
+                cg.addLine("const int _r = " + merge1 + ";");
+                cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
+                cg.addLine("++_spill_" + T + ";");
+            });
+            consume();
+        });
+        cg.addLine("__syncthreads();");
+        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
+            cg.addLine("const u64 _w0 = _gl_" + T + "[4 * _s];");
+            cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+                cg.addLine("const int _r = gdb_gb_merge(" + T + ", _bits_" + T + ", (u32)_w0, (u32)(_w0 >> 32), (i64)_gl_" +
+                           T + "[4 * _s + 1], (i64)_gl_" + T + "[4 * _s + 2], (i64)_gl_" + T + "[4 * _s + 3], _maxp_" +
+                           T + ");");
+                cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
+            });
+        });
+    } else {
+        cg.addComment("HashGroupBy " + T + ": every row merges into the global table");
+        child_->produce(cg, [&]() {
+            cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
+            cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
+            cg.addLine("const int _r = " + merge1 + ";");
+            cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
+            consume();
+        });
+    }
+    cg.addComment("group-by status: [0] overflow flag, [2] keys inserted, [3] rows spilled from LDS");
+    cg.addBlock("", [&]() {
+        cg.addLine("const u64 _a = gdb_wave_sum_u64(_ins_" + T + ");");
+        cg.addLine("const u64 _b = gdb_wave_sum_u64(_ovf_" + T + ");");
+        cg.addLine("const u64 _c = gdb_wave_sum_u64(_spill_" + T + ");");
+        cg.addBlock("if (gdb_lane() == 0)", [&]() {
+            cg.addLine("if (_a) gdb_atomic_add_u64(" + st + " + 2, _a);");
+            cg.addLine("if (_b) gdb_atomic_max_u64(" + st + " + 0, 1ull);");
+            cg.addLine("if (_c) gdb_atomic_add_u64(" + st + " + 3, _c);");
+        });
+    });
+
+    auto& rs = cg.resultSchema();
+    rs.kind = HipResultSchema::HASH_AGG;
+    rs.hash.tableName = T;
+    rs.hash.slotsSymbol = slots_;
+    rs.hash.slotWords = 4;
+    rs.hash.slots = {{"count", AggKind::Count, 0, 0},
+                     {"sum", AggKind::SumI64, 1, 0},
+                     {"min", AggKind::MinI64, 2, 0},
+                     {"max", AggKind::MaxI64, 3, 0}};
+}
+
+std::string HipHashGroupBy::describe() const {
+    return std::string("HashGroupBy(") + table_ + ", key=" + key_ + ", value=" + value_ + ", slots=" + slots_ +
+           ", variant=" + (variant_ == LDS ? "lds" + s(ldsSlots_) : "global") + ")";
+}
+
+// ===========================================================================
+// HipTopK
+// ===========================================================================
+
+HipTopK::HipTopK(std::unique_ptr<HipOperator> child, const std::string& outputName, int k, const std::string& rankHiExpr,
+                 const std::string& rankLoExpr)
+    : HipUnaryOperator(std::move(child)), out_(outputName), hi_(rankHiExpr), lo_(rankLoExpr), k_(k) {
+    if (k_ < 1 || k_ > 32) throw std::runtime_error("HipTopK: k must be in [1, 32]");
+}
+
+void HipTopK::addColumn(const std::string& name, int word, int shift, int bits, long long complementOf) {
+    cols_.push_back({name, word, shift, bits, complementOf});
+}
+
+void HipTopK::produce(HipCodegen& cg, ConsumerFn consume) {
+    const std::string K = s(k_);
+    const std::string ctl = out_ + "_ctl";
+    const std::string cand = out_ + "_cand";
+    const int maxBlocks = 4096;
+    cg.setPhaseMaxThreadgroups(2048);
+    cg.addBufferParam(out_, "u64", 8, s(2LL * k_), InitKind::Zero, {}, false, true);
+    cg.addBufferParam(ctl, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.addBufferParam(cand, "u64", 8, s(2LL * k_ * maxBlocks), InitKind::None, {}, false, true);
+    const int lb = cg.getPhases().back().launchBound;
+    cg.addLine("GdbTopK<" + K + "> _tk;");
+    cg.addLine("gdb_topk_init<" + K + ">(_tk);");
+    child_->produce(cg, [&]() {
+        cg.addLine("gdb_topk_push<" + K + ">(_tk, (u64)(" + hi_ + "), (u64)(" + lo_ + "));");
+        consume();
+    });
+    cg.addComment("TopK " + out_ + ": block merge, threshold-pruned candidates, last block merges");
+    cg.addBlock("", [&]() {
+        cg.addLine("__shared__ u64 _tk_lds[" + s(lb) + " * " + K + " * 2];");
+        cg.addLine("__shared__ int _tk_last;");
+        cg.addLine("gdb_topk_block_merge<" + K + ">(_tk, _tk_lds);");
+        cg.addBlock("if (threadIdx.x == 0)", [&]() {
+            cg.addLine("const u64 _thr = gdb_load_relaxed_u64(" + ctl + " + 2);");
+            cg.addLine("int _cnt = 0;");
+            cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) if ((_tk.h[_j] | _tk.l[_j]) != 0 && _tk.h[_j] >= _thr) _cnt = _j + 1;");
+            cg.addBlock("if (_cnt > 0)", [&]() {
+                cg.addLine("const u64 _base = __hip_atomic_fetch_add(" + ctl +
+                           " + 1, (u64)_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+                cg.addLine("for (int _j = 0; _j < _cnt; ++_j) { " + cand + "[(_base + _j) * 2] = _tk.h[_j]; " + cand +
+                           "[(_base + _j) * 2 + 1] = _tk.l[_j]; }");
+            });
+            cg.addLine("if ((_tk.h[" + s(k_ - 1) + "] | _tk.l[" + s(k_ - 1) + "]) != 0) gdb_atomic_max_u64(" + ctl +
+                       " + 2, _tk.h[" + s(k_ - 1) + "]);");
+            cg.addLine("__threadfence();");
+            cg.addLine("const u64 _t = __hip_atomic_fetch_add(" + ctl +
+                       " + 0, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+            cg.addLine("_tk_last = (_t == (u64)gridDim.x - 1ull) ? 1 : 0;");
+        });
+        cg.addLine("__syncthreads();");
+        cg.addBlock("if (_tk_last)", [&]() {
+            cg.addLine("__threadfence();");
+            cg.addLine("const u64 _nc = gdb_load_relaxed_u64(" + ctl + " + 1);");
+            cg.addLine("GdbTopK<" + K + "> _f;");
+            cg.addLine("gdb_topk_init<" + K + ">(_f);");
+            cg.addLine("for (u64 _c = threadIdx.x; _c < _nc; _c += blockDim.x) gdb_topk_push<" + K +
+                       ">(_f, gdb_load_relaxed_u64(" + cand + " + 2 * _c), gdb_load_relaxed_u64(" + cand +
+                       " + 2 * _c + 1));");
+            cg.addLine("gdb_topk_block_merge<" + K + ">(_f, _tk_lds);");
+            cg.addBlock("if (threadIdx.x == 0)", [&]() {
+                cg.addLine("u64 _valid = 0;");
+                cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) { " + out_ + "[2 * _j] = _f.h[_j]; " + out_ +
+                           "[2 * _j + 1] = _f.l[_j]; if ((_f.h[_j] | _f.l[_j]) != 0) ++_valid; }");
+                cg.addLine(ctl + "[3] = _valid;");
+                cg.addLine(ctl + "[0] = 0; " + ctl + "[1] = 0; " + ctl + "[2] = 0;");
+            });
+        });
+    });
+
+    auto& rs = cg.resultSchema();
+    rs.kind = HipResultSchema::TOPK;
+    rs.topk.bufferName = out_;
+    rs.topk.ctlBuffer = ctl;
+    rs.topk.k = k_;
+    rs.topk.columns.clear();
+    for (const auto& c : cols_) rs.topk.columns.push_back({c.name, c.word, c.shift, c.bits, c.complementOf});
+}
+
+std::string HipTopK::describe() const { return "TopK(" + out_ + ", k=" + s(k_) + ", rank=(" + hi_ + ", " + lo_ + "))"; }
+
+}  // namespace codegen
+}  // namespace gdb

@@ -1,0 +1,302 @@
+#pragma once
+// HIP operator emitters — producer/consumer code generation for gfx950.
+//
+// Counterpart of codegen/operators/metal_operators.h:26-353. Each operator's
+// produce() emits HIP C++ into a HipCodegen and calls its child's produce()
+// with a consumer callback, so a tree of operators becomes one fused loop.
+// The leaf scan owns the loop; unary operators wrap the per-row consumer;
+// terminal (aggregating) operators emit declarations before the loop and a
+// block/grid reduction after it.
+//
+// Row values: inside the loop every scanned column is a local named after the
+// column (e.g. `f_qty`), loaded by the scan with 16-byte vector loads; CHAR
+// columns are `const char*` into an LDS-staged copy of the tile. Expressions
+// passed to operators are HIP C++ over those names (and over scalar
+// parameters registered on the phase).
+
+#include "../core/hip_codegen_base.h"
+
+#include <memory>
+#include <string>
+#include <vector>
+
+namespace gdb {
+namespace codegen {
+
+class HipOperator {
+public:
+    virtual ~HipOperator() = default;
+    virtual void produce(HipCodegen& cg, ConsumerFn consume) = 0;
+    virtual std::string describe() const = 0;
+    virtual void describeTree(std::string& out, int depth) const;
+};
+
+class HipUnaryOperator : public HipOperator {
+protected:
+    std::unique_ptr<HipOperator> child_;
+
+public:
+    explicit HipUnaryOperator(std::unique_ptr<HipOperator> child) : child_(std::move(child)) {}
+    void describeTree(std::string& out, int depth) const override;
+};
+
+// ---------------------------------------------------------------------------
+// Leaves
+// ---------------------------------------------------------------------------
+
+// Tiled grid-stride scan over a columnar table (metal_operators.cpp:28-49).
+// A tile is blockDim.x * ITEMS rows; each thread owns ITEMS consecutive rows
+// and loads them per column with one vector load; full tiles skip bounds
+// checks. The index variable (`i` by default) is the absolute row number.
+class HipGridStrideScan : public HipOperator {
+public:
+    struct ColumnDesc {
+        std::string name;
+        ColumnType type;
+    };
+    explicit HipGridStrideScan(const std::string& table, const std::string& idxVar = "i");
+    void addColumn(const std::string& column, ColumnType type);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+    const std::string& tableName() const { return table_; }
+
+private:
+    std::string table_;
+    std::string idxVar_;
+    std::vector<ColumnDesc> columns_;
+};
+
+// Scan over the slots of a hash table built by an earlier phase; consumes
+// occupied slots only. Exposes <prefix>_key (u32), <prefix>_payload (u32,
+// high half of word 0) and <prefix>_w<k> (u64 words k >= 1) plus the slot
+// index `_slot`.
+class HipHashTableScan : public HipOperator {
+public:
+    HipHashTableScan(const std::string& table, const std::string& slotsExpr, int slotWords,
+                     const std::string& prefix);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, slotsExpr_, prefix_;
+    int slotWords_;
+};
+
+// ---------------------------------------------------------------------------
+// Pipeline operators
+// ---------------------------------------------------------------------------
+
+// WHERE filter: if (predicate) { consume(); } (metal_operators.cpp:63-69).
+class HipSelection : public HipUnaryOperator {
+public:
+    HipSelection(std::unique_ptr<HipOperator> child, const std::string& predicate);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string predicate_;
+};
+
+// const type var = expr; consume(); (metal_operators.cpp:81-95).
+class HipComputeExpr : public HipUnaryOperator {
+public:
+    HipComputeExpr(std::unique_ptr<HipOperator> child, const std::string& varName, const std::string& varType,
+                   const std::string& expression);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string varName_, varType_, expression_;
+};
+
+// Semi-join bitmap build: set bit `key` (metal_operators.cpp:108-116).
+class HipBitmapBuild : public HipUnaryOperator {
+public:
+    HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
+                   const std::string& bitsExpr);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string bitmap_, key_, bits_;
+};
+
+// Semi-join / anti-join probe (metal_operators.cpp:132-165).
+class HipBitmapProbe : public HipUnaryOperator {
+public:
+    HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
+                   const std::string& bitsExpr, bool anti = false);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string bitmap_, key_, bits_;
+    bool anti_;
+};
+
+// Open-addressing hash build of unique keys with a 32-bit payload. Replaces
+// the reference's direct-address ArrayStore join map (metal_operators.cpp:185-200)
+// and its host-built tables (query_preprocessing.cpp:352-378).
+class HipHashBuild : public HipUnaryOperator {
+public:
+    HipHashBuild(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                 const std::string& payloadExpr, const std::string& slotsExpr, int slotWords);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, key_, payload_, slots_;
+    int slotWords_;
+};
+
+// Probe a table built by HipHashBuild; consumes rows whose key is present with
+// `slotVar` (i64) and `<slotVar>_payload` (u32) in scope. Replaces
+// ArrayLookup (metal_operators.cpp:213-225) and q9_ht_probe
+// (metal_plan_builder.cpp:1598-1610).
+class HipHashProbe : public HipUnaryOperator {
+public:
+    HipHashProbe(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                 const std::string& slotsExpr, int slotWords, const std::string& slotVar);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, key_, slots_, slotVar_;
+    int slotWords_;
+};
+
+// Atomic add of an i64 value into word `word` of a hash-table slot
+// (the group-by-join-key aggregate; counterpart of MetalAtomicAgg,
+// metal_operators.cpp:525-533, with the slot as the bucket).
+class HipSlotAtomicAdd : public HipUnaryOperator {
+public:
+    HipSlotAtomicAdd(std::unique_ptr<HipOperator> child, const std::string& table, int slotWords,
+                     const std::string& slotVar, int word, const std::string& valueExpr);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, slotVar_, value_;
+    int slotWords_, word_;
+};
+
+// Direct-address lookup with a sentinel guard (metal_operators.cpp:203-225).
+class HipArrayLookup : public HipUnaryOperator {
+public:
+    HipArrayLookup(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& keyExpr,
+                   const std::string& sizeExpr, const std::string& resultVar, long long sentinel);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string array_, key_, size_, var_;
+    long long sentinel_;
+};
+
+// ---------------------------------------------------------------------------
+// Aggregation (terminal) operators
+// ---------------------------------------------------------------------------
+
+// Fused group-by over a small dense bucket domain (MetalKeyedAgg,
+// metal_operators.cpp:375-502, which also covers MetalTGReduce with one
+// bucket). Exact integer aggregates: SumI64, SumI128 (two 64-bit limbs:
+// sum of the low 32 bits and sum of the arithmetic high 32 bits), Count,
+// MinI64, MaxI64. Strategy by bucket count:
+//   <= 8 buckets: per-thread register accumulators, wave DPP reduction, LDS
+//                 block reduction, one global atomic per block and slot;
+//   <= LDS budget: block-shared LDS table with LDS atomics, flushed once;
+//   otherwise:   one global atomic per row.
+class HipKeyedAgg : public HipUnaryOperator {
+public:
+    struct Aggregate {
+        std::string name;
+        AggKind kind;
+        std::string valueExpr;
+        int scaleDigits;
+        int wordOffset = 0;
+    };
+    HipKeyedAgg(std::unique_ptr<HipOperator> child, const std::string& outputName, const std::string& bucketExpr,
+                int numBuckets);
+    void addAggregate(const std::string& name, AggKind kind, const std::string& valueExpr, int scaleDigits = 0);
+    void setBucketLabels(const std::vector<std::string>& labels, const std::string& keyColumn);
+    void addAverage(const std::string& name, const std::string& numerator, const std::string& denominator);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string out_, bucket_;
+    int numBuckets_;
+    std::vector<Aggregate> aggs_;
+    std::vector<std::string> labels_;
+    std::string keyColumn_ = "bucket";
+    std::vector<std::pair<std::string, std::pair<std::string, std::string>>> avgs_;
+};
+
+// Count per bucket into u32 counters (MetalAtomicCount,
+// metal_operators.cpp:550-594): LDS histogram when the static size fits,
+// global atomics otherwise.
+class HipAtomicCount : public HipUnaryOperator {
+public:
+    HipAtomicCount(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& bucketExpr,
+                   const std::string& sizeExpr, int staticSize);
+    // Make the counters the plan's result: rows (key, count) for non-zero
+    // counters, optionally ordered count desc then key desc (Q13).
+    void setResult(const std::string& keyName, const std::string& valueName, bool sortByCountDesc);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string array_, bucket_, size_;
+    int static_;
+    bool isResult_ = false;
+    std::string keyName_, valueName_;
+    bool sortDesc_ = false;
+};
+
+// Hash group-by with sum/count/min/max of an i64 value per 32-bit key, into
+// a global open-addressing table of 32-byte slots. Variants:
+//   GLOBAL — every row merges into the global table;
+//   LDS    — a block-local LDS table absorbs repeated (hot) keys; rows whose
+//            LDS probe window is full spill to the global table; the LDS
+//            table is flushed once at the end of the block.
+class HipHashGroupBy : public HipUnaryOperator {
+public:
+    enum Variant { GLOBAL = 0, LDS = 1 };
+    HipHashGroupBy(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                   const std::string& valueExpr, const std::string& slotsExpr, Variant variant, int ldsSlots);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, key_, value_, slots_;
+    Variant variant_;
+    int ldsSlots_;
+};
+
+// Top-k by a 128-bit rank (hi, lo), larger first, rank (0,0) reserved.
+// Per-thread register lists, block merge in LDS, candidates pruned by a
+// running global threshold, and a final merge by the last block to finish.
+// Replaces the reference's CPU full sort (codegen/codegen_main.cpp:1211-1248).
+class HipTopK : public HipUnaryOperator {
+public:
+    struct Column {
+        std::string name;
+        int word;            // 0 = rank hi, 1 = rank lo
+        int shift;
+        int bits;
+        long long complementOf;  // value = complementOf - field when != 0
+    };
+    HipTopK(std::unique_ptr<HipOperator> child, const std::string& outputName, int k, const std::string& rankHiExpr,
+            const std::string& rankLoExpr);
+    void addColumn(const std::string& name, int word, int shift, int bits, long long complementOf = 0);
+    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string out_, hi_, lo_;
+    int k_;
+    std::vector<Column> cols_;
+};
+
+}  // namespace codegen
+}  // namespace gdb

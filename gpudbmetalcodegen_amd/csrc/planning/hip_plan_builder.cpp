@@ -1,0 +1,444 @@
+#include "hip_plan_builder.h"
+
+#include <sstream>
+#include <stdexcept>
+
+namespace gdb {
+namespace codegen {
+
+// ---------------------------------------------------------------------------
+// Catalog. Column names carry a table prefix so they are unique inside a
+// kernel; the reference's TPC-H catalog (codegen/planning/tpch_schema.h:47-166)
+// has no int64 or uint8 types, which these workloads need.
+// ---------------------------------------------------------------------------
+
+namespace {
+
+ColumnType I32{ColType::I32, 0};
+ColumnType I64{ColType::I64, 0};
+ColumnType U8{ColType::U8, 0};
+ColumnType CHAR25{ColType::CHAR, 25};
+
+const std::vector<CatalogTable>& catalog() {
+    static const std::vector<CatalogTable> tables = {
+        {"F",
+         {{"f_qty", I32},
+          {"f_price", I64},
+          {"f_disc", I32},
+          {"f_tax", I32},
+          {"f_flag", U8},
+          {"f_status", U8},
+          {"f_date", I32},
+          {"f_fkey", I32}}},
+        {"A", {{"a_key", I32}, {"a_seg", U8}}},
+        {"B", {{"b_key", I32}, {"b_fkey", I32}, {"b_date", I32}}},
+        {"S", {{"s_fkey", I32}, {"s_text", CHAR25}}},
+        {"G", {{"g_key", I32}, {"g_val", I64}}},
+        {"GBBM", {{"gbbm_word", I32}}},
+    };
+    return tables;
+}
+
+std::string opt(const PlanOptions& o, const std::string& k, const std::string& def) {
+    auto it = o.find(k);
+    return it == o.end() ? def : it->second;
+}
+int optInt(const PlanOptions& o, const std::string& k, int def) {
+    auto it = o.find(k);
+    return it == o.end() ? def : std::stoi(it->second);
+}
+
+std::unique_ptr<HipGridStrideScan> makeScan(const std::string& table, std::initializer_list<const char*> cols) {
+    auto scan = std::make_unique<HipGridStrideScan>(table, "i");
+    const auto& t = catalogTable(table);
+    for (const char* c : cols) scan->addColumn(c, t.type(c));
+    return scan;
+}
+
+HipQueryPlan::Phase& appendPhase(HipQueryPlan& plan, const std::string& name, std::unique_ptr<HipOperator> root,
+                                 int block, int items, std::vector<int> variants = {}) {
+    HipQueryPlan::Phase p;
+    p.name = name;
+    p.root = std::move(root);
+    p.threadgroupSize = block;
+    p.items = items;
+    p.itemsVariants = variants.empty() ? std::vector<int>{items} : variants;
+    bool has = false;
+    for (int v : p.itemsVariants) has |= (v == items);
+    if (!has) p.itemsVariants.push_back(items);
+    plan.phases.push_back(std::move(p));
+    return plan.phases.back();
+}
+
+// ---------------------------------------------------------------------------
+// Q1 — filter + group-by (flag, status) with exact fixed-point aggregates.
+// Bucket encoding follows codegen/planning/metal_plan_builder.cpp:335
+// (flag A->0, N->2, R->4; status F->0, else 1). Aggregates follow the
+// reference's Q1 slot list (:342-349) in exact integers: price in cents,
+// discount/tax in hundredths, so disc_price carries 4 fixed-point digits and
+// charge 6 (int128).
+// ---------------------------------------------------------------------------
+HipQueryPlan buildQ1(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q1";
+    auto scan = makeScan("F", {"f_qty", "f_price", "f_disc", "f_tax", "f_flag", "f_status", "f_date"});
+    auto sel = std::make_unique<HipSelection>(std::move(scan), "f_date <= q1_cutoff");
+    const std::string bucket =
+        "((f_flag == 'A' ? 0 : (f_flag == 'N' ? 2 : 4)) + (f_status == 'F' ? 0 : 1))";
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q1_aggs", bucket, 6);
+    agg->addAggregate("sum_qty", AggKind::SumI64, "f_qty", 0);
+    agg->addAggregate("sum_base_price", AggKind::SumI64, "f_price", 2);
+    agg->addAggregate("sum_disc_price", AggKind::SumI64, "f_price * (i64)(100 - f_disc)", 4);
+    agg->addAggregate("sum_charge", AggKind::SumI128, "f_price * (i64)(100 - f_disc) * (i64)(100 + f_tax)", 6);
+    agg->addAggregate("sum_disc", AggKind::SumI64, "f_disc", 2);
+    agg->addAggregate("count_order", AggKind::Count, "1", 0);
+    agg->setBucketLabels({"A|F", "A|O", "N|F", "N|O", "R|F", "R|O"}, "f_flag,f_status");
+    agg->addAverage("avg_qty", "sum_qty", "count_order");
+    agg->addAverage("avg_price", "sum_base_price", "count_order");
+    agg->addAverage("avg_disc", "sum_disc", "count_order");
+    auto& ph = appendPhase(plan, "Q1_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 8),
+                           {4, 8});
+    ph.launchBound = 256;
+    ph.scalarParams = {"q1_cutoff"};
+    plan.requiredScalars = {"q1_cutoff"};
+    return plan;
+}
+
+// Q6 — scalar revenue over a date range, discount band and quantity bound
+// (metal_plan_builder.cpp:253-303, sql/q6.sql), exact: sum(price * disc).
+HipQueryPlan buildQ6(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q6";
+    auto scan = makeScan("F", {"f_qty", "f_price", "f_disc", "f_date"});
+    auto sel = std::make_unique<HipSelection>(
+        std::move(scan),
+        "f_date >= q6_date_lo && f_date < q6_date_hi && f_disc >= q6_disc_lo && f_disc <= q6_disc_hi && f_qty < q6_qty_hi");
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q6_sum", "0", 1);
+    agg->addAggregate("revenue", AggKind::SumI64, "f_price * (i64)f_disc", 4);
+    agg->addAggregate("count", AggKind::Count, "1", 0);
+    auto& ph = appendPhase(plan, "Q6_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 8), {4, 8});
+    ph.launchBound = 256;
+    ph.scalarParams = {"q6_date_lo", "q6_date_hi", "q6_disc_lo", "q6_disc_hi", "q6_qty_hi"};
+    plan.requiredScalars = ph.scalarParams;
+    return plan;
+}
+
+// Q3 analog — A(seg) semi-join bitmap, hash build of qualifying B keys with
+// b_date as payload, probe from F with the per-B-key revenue summed in the
+// slot, then top-10 by (revenue desc, b_date asc, b_key asc).
+// Reference: buildQ3Plan_byName (metal_plan_builder.cpp:1170-1239) and the
+// CPU top-10 (codegen_main.cpp:1211-1248).
+void q3BitmapPhase(HipQueryPlan& plan, const PlanOptions& o) {
+    auto scan = makeScan("A", {"a_key", "a_seg"});
+    auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)a_seg == q3_seg");
+    auto bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q3_a_bitmap", "a_key", "a_bits");
+    auto& ph = appendPhase(plan, "Q3_build_a_bitmap", std::move(bm), 256, optInt(o, "items_a", 4));
+    ph.scalarParams = {"q3_seg"};
+}
+
+// Exact count of the B rows the hash table will hold (the plain form's
+// sizing pre-pass).
+void q3CountPhase(HipQueryPlan& plan) {
+    auto scan = makeScan("B", {"b_fkey", "b_date"});
+    auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)b_date < q3_date");
+    auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
+    auto cnt = std::make_unique<HipKeyedAgg>(std::move(probe), "d_q3_bcount", "0", 1);
+    cnt->addAggregate("b_rows", AggKind::Count, "1", 0);
+    auto& ph = appendPhase(plan, "Q3_count_b", std::move(cnt), 256, 8);
+    ph.scalarParams = {"q3_date"};
+}
+
+HipQueryPlan buildQ3Size(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q3_SIZE";
+    q3BitmapPhase(plan, o);
+    q3CountPhase(plan);
+    plan.requiredScalars = {"q3_seg", "q3_date"};
+    plan.requiredSymbols = {"a_bits"};
+    return plan;
+}
+
+HipQueryPlan buildQ3(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q3";
+    const bool probeBitmap = optInt(o, "probe_bitmap", 1) != 0;
+    const bool prepass = opt(o, "sizing", "prepass") == "prepass";
+    q3BitmapPhase(plan, o);
+    if (prepass) {
+        q3CountPhase(plan);
+        plan.sizingRules.push_back({"Q3_count_b", "b_slots", "d_q3_bcount", 0, 2, 64, true});
+    }
+    {
+        auto scan = makeScan("B", {"b_key", "b_fkey", "b_date"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)b_date < q3_date");
+        auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
+        std::unique_ptr<HipOperator> top =
+            std::make_unique<HipHashBuild>(std::move(probe), "d_q3_b_ht", "b_key", "b_date", "b_slots", 2);
+        if (probeBitmap)
+            top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
+        auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
+        ph.scalarParams = {"q3_date"};
+    }
+    {
+        auto scan = makeScan("F", {"f_fkey", "f_price", "f_disc", "f_date"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipSelection>(std::move(scan), "(i64)f_date > q3_date");
+        if (probeBitmap)
+            cur = std::make_unique<HipBitmapProbe>(std::move(cur), "d_q3_b_bitmap", "f_fkey", "b_bits");
+        cur = std::make_unique<HipHashProbe>(std::move(cur), "d_q3_b_ht", "f_fkey", "b_slots", 2, "_bs");
+        cur = std::make_unique<HipSlotAtomicAdd>(std::move(cur), "d_q3_b_ht", 2, "_bs", 1,
+                                                 "f_price * (i64)(100 - f_disc)");
+        auto& ph = appendPhase(plan, "Q3_probe_f", std::move(cur), optInt(o, "block", 256), optInt(o, "items", 8),
+                               {4, 8});
+        ph.scalarParams = {"q3_date"};
+    }
+    {
+        auto scan = std::make_unique<HipHashTableScan>("d_q3_b_ht", "b_slots", 2, "ht");
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)ht_w1 > 0");
+        auto top = std::make_unique<HipTopK>(std::move(sel), "d_q3_top", 10, "ht_w1",
+                                             "((u64)(0x7FFFFFFFu - ht_payload) << 32) | (u64)(0xFFFFFFFFu - ht_key)");
+        top->addColumn("b_key", 1, 0, 32, 0xFFFFFFFFll);
+        top->addColumn("revenue", 0, 0, 64, 0);
+        top->addColumn("b_date", 1, 32, 32, 0x7FFFFFFFll);
+        auto& ph = appendPhase(plan, "Q3_topk", std::move(top), 256, 4);
+        ph.launchBound = 256;
+    }
+    plan.requiredScalars = {"q3_seg", "q3_date"};
+    plan.requiredSymbols = {"a_bits", "b_bits"};
+    if (!prepass) plan.requiredSymbols.push_back("b_slots");
+    return plan;
+}
+
+// GROUPBY — per-key sum/count/min/max of g_val (cfg 2; the reference's
+// per-key aggregation is Q18, metal_plan_builder.cpp:1529-1545, direct
+// addressed; here a hash table). Options: variant=lds|global, lds_slots,
+// filter=1 adds g_val < gb_val_cut (selectivity control).
+void gbCountPhases(HipQueryPlan& plan, bool filter);
+
+HipQueryPlan buildGroupBy(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "GROUPBY";
+    const bool filter = optInt(o, "filter", 0) != 0;
+    const bool prepass = opt(o, "sizing", "given") == "prepass";
+    if (prepass) {
+        gbCountPhases(plan, filter);
+        plan.sizingRules.push_back({"GBC_popcount", "gb_slots", "d_gb_distinct", 0, 2, 64, true});
+    }
+    auto scan = makeScan("G", {"g_key", "g_val"});
+    std::unique_ptr<HipOperator> cur = std::move(scan);
+    if (filter) cur = std::make_unique<HipSelection>(std::move(cur), "g_val < gb_val_cut");
+    const std::string variant = opt(o, "variant", "lds");
+    const int ldsSlots = optInt(o, "lds_slots", 2048);
+    auto gb = std::make_unique<HipHashGroupBy>(std::move(cur), "d_gb_ht", "g_key", "g_val", "gb_slots",
+                                               variant == "global" ? HipHashGroupBy::GLOBAL : HipHashGroupBy::LDS,
+                                               ldsSlots);
+    auto& ph = appendPhase(plan, "GB_agg", std::move(gb), optInt(o, "block", 512), optInt(o, "items", 4), {4});
+    ph.launchBound = 1024;
+    if (filter) {
+        ph.scalarParams = {"gb_val_cut"};
+        plan.requiredScalars = {"gb_val_cut"};
+    }
+    if (prepass) {
+        plan.requiredScalars.push_back("gb_key_min");
+        plan.requiredSymbols = {"gb_key_bits"};
+    } else {
+        plan.requiredSymbols = {"gb_slots"};
+    }
+    return plan;
+}
+
+// GB_COUNT — exact distinct-key count (the plain form's pre-pass): set bit
+// (key - gb_key_min) in a key-range bitmap, then popcount the bitmap words
+// (the bitmap is scanned as table GBBM). Counterpart of the exact max-key
+// pre-pass (codegen/execution/max_key_symbols.cpp:188-235).
+HipQueryPlan buildGbCount(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "GB_COUNT";
+    gbCountPhases(plan, optInt(o, "filter", 0) != 0);
+    plan.requiredScalars = {"gb_key_min"};
+    plan.requiredSymbols = {"gb_key_bits"};
+    return plan;
+}
+
+void gbCountPhases(HipQueryPlan& plan, bool filter) {
+    {
+        std::unique_ptr<HipOperator> cur = filter ? makeScan("G", {"g_key", "g_val"}) : makeScan("G", {"g_key"});
+        if (filter) cur = std::make_unique<HipSelection>(std::move(cur), "g_val < gb_val_cut");
+        auto bm = std::make_unique<HipBitmapBuild>(std::move(cur), "d_gb_key_bitmap", "(i64)g_key - gb_key_min",
+                                                   "gb_key_bits");
+        auto& ph = appendPhase(plan, "GBC_bitmap", std::move(bm), 256, 8);
+        ph.scalarParams = {"gb_key_min"};
+        if (filter) ph.scalarParams.push_back("gb_val_cut");
+    }
+    {
+        auto scan = makeScan("GBBM", {"gbbm_word"});
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(scan), "d_gb_distinct", "0", 1);
+        agg->addAggregate("distinct_keys", AggKind::SumI64, "__popc((u32)gbbm_word)", 0);
+        appendPhase(plan, "GBC_popcount", std::move(agg), 256, 8);
+    }
+    plan.bufferTables.push_back({"GBBM", "gbbm_word", "d_gb_key_bitmap", "(gb_key_bits + 31) / 32 + 1"});
+}
+
+// MAXKEY — min and max of one int32 column (table=..., column=...).
+HipQueryPlan buildMaxKey(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "MAXKEY";
+    const std::string table = opt(o, "table", "G");
+    const std::string col = opt(o, "column", "g_key");
+    auto scan = std::make_unique<HipGridStrideScan>(table, "i");
+    scan->addColumn(col, catalogTable(table).type(col));
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(scan), "d_maxkey", "0", 1);
+    agg->addAggregate("min_key", AggKind::MinI64, col, 0);
+    agg->addAggregate("max_key", AggKind::MaxI64, col, 0);
+    agg->addAggregate("rows", AggKind::Count, "1", 0);
+    appendPhase(plan, "MAXKEY_reduce", std::move(agg), 256, 8);
+    return plan;
+}
+
+// Q13L — byte-exact LIKE '%w1%w2%' over S.s_text (char[25]), count matching
+// rows per A key, then the histogram of those counts over all A keys
+// (buildQ13Plan_byName, metal_plan_builder.cpp:1244-1299, with LIKE in place
+// of NOT LIKE and any two words; negate=1 restores NOT LIKE).
+HipQueryPlan buildQ13L(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q13L";
+    const bool negate = optInt(o, "negate", 0) != 0;
+    const int histBins = optInt(o, "hist_bins", 1024);
+    {
+        auto scan = makeScan("S", {"s_fkey", "s_text"});
+        std::string like =
+            "gdb_like2(s_text, 25, (const char*)d_like_words, (int)like_l1, (const char*)d_like_words + 32, "
+            "(int)like_l2)";
+        auto sel = std::make_unique<HipSelection>(std::move(scan), negate ? "!" + like : like);
+        auto cnt = std::make_unique<HipAtomicCount>(std::move(sel), "d_q13_counts", "s_fkey", "a_bits", 0);
+        auto& ph = appendPhase(plan, "Q13_count", std::move(cnt), 256, optInt(o, "items", 4), {4});
+        ph.launchBound = 256;
+        ph.scalarParams = {"like_l1", "like_l2"};
+        ph.externalBuffers = {{"d_like_words", "u8"}};
+    }
+    {
+        auto scan = makeScan("A", {"a_key"});
+        auto look = std::make_unique<HipArrayLookup>(std::move(scan), "d_q13_counts", "a_key", "a_bits", "_cnt",
+                                                     -1ll);
+        auto hist = std::make_unique<HipAtomicCount>(std::move(look), "d_q13_hist", "_cnt", std::to_string(histBins),
+                                                     histBins);
+        hist->setResult("c_count", "custdist", true);
+        appendPhase(plan, "Q13_histogram", std::move(hist), 256, 4);
+    }
+    plan.requiredScalars = {"like_l1", "like_l2"};
+    plan.requiredSymbols = {"a_bits"};
+    return plan;
+}
+
+// Q22A — anti-join: A rows whose key is referenced by no B row with
+// b_date in [q22_date_lo, q22_date_hi); count and sum(a_key) per segment
+// (buildQ22Plan_byName, metal_plan_builder.cpp:1304-1357).
+HipQueryPlan buildQ22A(const PlanOptions& o) {
+    (void)o;
+    HipQueryPlan plan;
+    plan.name = "Q22A";
+    {
+        auto scan = makeScan("B", {"b_fkey", "b_date"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)b_date >= q22_date_lo && (i64)b_date < q22_date_hi");
+        auto bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q22_bitmap", "b_fkey", "a_bits");
+        auto& ph = appendPhase(plan, "Q22_build_bitmap", std::move(bm), 256, 8);
+        ph.scalarParams = {"q22_date_lo", "q22_date_hi"};
+    }
+    {
+        auto scan = makeScan("A", {"a_key", "a_seg"});
+        auto anti = std::make_unique<HipBitmapProbe>(std::move(scan), "d_q22_bitmap", "a_key", "a_bits", true);
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(anti), "d_q22_aggs", "(int)a_seg", 5);
+        agg->addAggregate("num_keys", AggKind::Count, "1", 0);
+        agg->addAggregate("sum_keys", AggKind::SumI64, "a_key", 0);
+        agg->setBucketLabels({"0", "1", "2", "3", "4"}, "a_seg");
+        appendPhase(plan, "Q22_anti_aggregate", std::move(agg), 256, 8);
+    }
+    plan.requiredScalars = {"q22_date_lo", "q22_date_hi"};
+    plan.requiredSymbols = {"a_bits"};
+    return plan;
+}
+
+}  // namespace
+
+const ColumnType& CatalogTable::type(const std::string& col) const {
+    for (const auto& c : columns)
+        if (c.first == col) return c.second;
+    throw std::runtime_error("catalog: table " + name + " has no column " + col);
+}
+
+const CatalogTable& catalogTable(const std::string& name) {
+    for (const auto& t : catalog())
+        if (t.name == name) return t;
+    throw std::runtime_error("catalog: unknown table " + name);
+}
+
+std::vector<std::string> catalogTableNames() {
+    std::vector<std::string> n;
+    for (const auto& t : catalog()) n.push_back(t.name);
+    return n;
+}
+
+PlanOptions parsePlanOptions(const std::string& text) {
+    PlanOptions o;
+    std::stringstream ss(text);
+    std::string item;
+    while (std::getline(ss, item, ',')) {
+        if (item.empty()) continue;
+        auto eq = item.find('=');
+        if (eq == std::string::npos) throw std::runtime_error("plan options: expected key=value, got '" + item + "'");
+        o[item.substr(0, eq)] = item.substr(eq + 1);
+    }
+    return o;
+}
+
+std::vector<std::string> knownPlanNames() {
+    return {"Q1", "Q6", "Q3", "Q3_SIZE", "GROUPBY", "GB_COUNT", "MAXKEY", "Q13L", "Q22A"};
+}
+
+std::optional<HipQueryPlan> buildHipPlan(const std::string& name, const PlanOptions& opts) {
+    if (name == "Q1") return buildQ1(opts);
+    if (name == "Q6") return buildQ6(opts);
+    if (name == "Q3") return buildQ3(opts);
+    if (name == "Q3_SIZE") return buildQ3Size(opts);
+    if (name == "GROUPBY") return buildGroupBy(opts);
+    if (name == "GB_COUNT") return buildGbCount(opts);
+    if (name == "MAXKEY") return buildMaxKey(opts);
+    if (name == "Q13L") return buildQ13L(opts);
+    if (name == "Q22A") return buildQ22A(opts);
+    return std::nullopt;
+}
+
+std::string HipQueryPlan::describe() const {
+    std::string out = "Plan " + name + " (" + std::to_string(phases.size()) + " phase(s))\n";
+    for (const auto& p : phases) {
+        out += "  Phase " + p.name + " [block " + std::to_string(p.threadgroupSize) + ", items " +
+               std::to_string(p.items) + (p.singleThread ? ", single" : "") + "]\n";
+        std::string tree;
+        if (p.root) p.root->describeTree(tree, 2);
+        out += tree;
+    }
+    return out;
+}
+
+HipCodegen generateFromPlan(const HipQueryPlan& plan) {
+    HipCodegen cg;
+    for (const auto& h : plan.helpers) cg.addHelper(h);
+    for (const auto& phase : plan.phases) {
+        cg.beginPhase(phase.name);
+        cg.setPhaseThreadgroupSize(phase.threadgroupSize);
+        cg.setPhaseSingleThread(phase.singleThread);
+        cg.setPhaseItemsVariants(phase.itemsVariants);
+        cg.setPhaseItems(phase.items);
+        cg.setPhaseLaunchBound(phase.launchBound);
+        if (phase.maxThreadgroups) cg.setPhaseMaxThreadgroups(phase.maxThreadgroups);
+        for (const auto& sp : phase.scalarParams) cg.addScalarParam(sp);
+        for (const auto& eb : phase.externalBuffers)
+            cg.addBufferParam(eb.first, eb.second, 1, "", InitKind::None, {}, true, true);
+        if (phase.root) phase.root->produce(cg, []() {});
+        cg.endPhase();
+    }
+    return cg;
+}
+
+}  // namespace codegen
+}  // namespace gdb

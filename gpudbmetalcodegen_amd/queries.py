@@ -1,0 +1,226 @@
+"""Query drivers: plan + runner + sizing for each workload, over the C-ABI.
+
+Each driver mirrors one step of the reference's ``runCodegenQuery``
+(codegen/codegen_main.cpp:271-1776): build the plan, compile it, register
+tables / scalars / size symbols, execute, collect. Sizing follows the
+reference's exact pre-pass idea (codegen/execution/max_key_symbols.cpp:188-235)
+generalised to hash tables.
+"""
+
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import engine as E
+from . import workloads as W
+
+Q1_LABELS = [("A", "F"), ("A", "O"), ("N", "F"), ("N", "O"), ("R", "F"), ("R", "O")]
+
+
+def next_pow2(x: int) -> int:
+    p = 1
+    while p < x:
+        p <<= 1
+    return p
+
+
+@dataclass
+class QueryRun:
+    result: E.GenericResult
+    timing: E.Timing
+    plan: E.HipQueryPlan
+    executor: E.HipGenericExecutor
+
+
+_plan_cache: dict = {}
+
+
+def get_plan(ctx: E.Context, name: str, options: str = "") -> E.HipQueryPlan:
+    key = (id(ctx), name, options)
+    p = _plan_cache.get(key)
+    if p is None:
+        p = E.build_hip_plan(name, options).compile(ctx)
+        _plan_cache[key] = p
+    return p
+
+
+def upload_tables(ctx: E.Context, tables: dict) -> dict:
+    return {name: E.DeviceTable.from_host(ctx, name, cols) for name, cols in tables.items() if isinstance(cols, dict)}
+
+
+# ---------------------------------------------------------------------------
+# Workload 0: Q1
+# ---------------------------------------------------------------------------
+def run_q1(ctx: E.Context, dev: dict, cutoff: int, warmup: int = 0, measured: int = 1,
+           options: str = "", executor: E.HipGenericExecutor | None = None) -> QueryRun:
+    plan = get_plan(ctx, "Q1", options)
+    ex = executor or E.HipGenericExecutor(ctx)
+    ex.register_table(dev["F"])
+    ex.register_scalar("q1_cutoff", cutoff)
+    t = ex.execute(plan, warmup, measured)
+    return QueryRun(ex.collect(plan), t, plan, ex)
+
+
+def _executor(ctx, executor):
+    return executor or E.HipGenericExecutor(ctx)
+
+
+# ---------------------------------------------------------------------------
+# Workload 1: Q3 analog (semi-join bitmap -> hash build -> probe -> top-10)
+# ---------------------------------------------------------------------------
+def run_q3(ctx: E.Context, dev: dict, seg: int = W.W1_SEG, d: int = W.W1_DATE, options: str = "",
+           max_a: int | None = None, max_b: int | None = None, b_slots: int | None = None, warmup: int = 0,
+           measured: int = 1, executor: E.HipGenericExecutor | None = None) -> QueryRun:
+    """Plain form: the plan's own exact-count pre-pass sizes the B hash table
+    (options ``sizing=given`` takes ``b_slots`` instead). Key-range symbols
+    come from table statistics (max key), as the reference's cached max-key
+    mode does (codegen/execution/max_key_symbols.cpp:28-41)."""
+    plan = get_plan(ctx, "Q3", options)
+    ex = _executor(ctx, executor)
+    for t in ("A", "B", "F"):
+        ex.register_table(dev[t])
+    ex.register_scalar("q3_seg", seg)
+    ex.register_scalar("q3_date", d)
+    ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
+    ex.register_symbol("b_bits", (max_b if max_b is not None else dev["B"].rows) + 1)
+    if "sizing=given" in options:
+        if b_slots is None:
+            raise ValueError("sizing=given needs b_slots")
+        ex.register_symbol("b_slots", b_slots)
+    t = ex.execute(plan, warmup, measured)
+    return QueryRun(ex.collect(plan), t, plan, ex)
+
+
+def q3_hash_groups(run: QueryRun) -> dict:
+    """Every group held in the B hash table after a Q3 run (key asc):
+    key, rev, date — used to check the join beyond the top-10."""
+    slots = run.executor.get_symbol("b_slots")
+    tab = run.executor.read_buffer("d_q3_b_ht", slots * 16, np.uint64).reshape(slots, 2)
+    w0 = tab[:, 0]
+    keys = (w0 & np.uint64(0xFFFFFFFF)).astype(np.int64)
+    occ = (keys != 0xFFFFFFFF) & (tab[:, 1].view(np.int64) > 0)
+    k = keys[occ]
+    order = np.argsort(k)
+    return {"key": k[order], "rev": tab[occ, 1].view(np.int64)[order],
+            "date": (w0[occ] >> np.uint64(32)).astype(np.int64)[order]}
+
+
+# ---------------------------------------------------------------------------
+# Workload 2: hash group-by with sizing by pre-pass, by estimate, or resize
+# ---------------------------------------------------------------------------
+@dataclass
+class GroupByRun(QueryRun):
+    slots: int = 0
+    resizes: int = 0
+    distinct: int = 0
+    times_ms: list = None
+
+
+def groupby_options(variant: str = "lds", lds_slots: int = 2048, filtered: bool = False, prepass: bool = False,
+                    block: int | None = None) -> str:
+    o = [f"variant={variant}", f"lds_slots={lds_slots}"]
+    if filtered:
+        o.append("filter=1")
+    if prepass:
+        o.append("sizing=prepass")
+    if block:
+        o.append(f"block={block}")
+    return ",".join(o)
+
+
+def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass: bool = False,
+                variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
+                key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
+                warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None) -> GroupByRun:
+    """prepass=True: the plan counts distinct keys exactly first (key-range
+    bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
+    `slots` (e.g. from the learned estimator) sizes it, and a reported
+    overflow doubles the table and reruns (resize-on-overflow)."""
+    opts = groupby_options(variant, lds_slots, val_cut is not None, prepass, block)
+    plan = get_plan(ctx, "GROUPBY", opts)
+    ex = _executor(ctx, executor)
+    ex.register_table(dev["G"])
+    if val_cut is not None:
+        ex.register_scalar("gb_val_cut", val_cut)
+    times = []
+    resizes = 0
+    if prepass:
+        if key_max is None:
+            raise ValueError("prepass sizing needs the key range (key_min, key_max)")
+        ex.register_scalar("gb_key_min", key_min)
+        ex.register_symbol("gb_key_bits", key_max - key_min + 1)
+        t = ex.execute(plan, warmup, measured)
+        times.append(t.total_ms)
+        slots = ex.get_symbol("gb_slots")
+    else:
+        if slots is None:
+            raise ValueError("give slots or prepass=True")
+        slots = max(64, next_pow2(int(slots)))
+        while True:
+            ex.register_symbol("gb_slots", slots)
+            try:
+                t = ex.execute(plan, warmup, measured)
+                times.append(t.total_ms)
+                break
+            except E.GdbOverflow:
+                st = ex.read_buffer("d_gb_ht_status", 32, np.uint64)
+                times.append(float("nan"))
+                resizes += 1
+                if resizes > max_resizes:
+                    raise
+                slots = next_pow2(max(2 * slots, 2 * int(st[2])))
+    res = ex.collect(plan)
+    return GroupByRun(res, t, plan, ex, slots=slots, resizes=resizes, distinct=res.rows(), times_ms=times)
+
+
+# ---------------------------------------------------------------------------
+# Workload 4 plan types: LIKE + per-key count + histogram (Q13), anti-join (Q22)
+# ---------------------------------------------------------------------------
+def like_words_buffer(ctx: E.Context, w1: bytes, w2: bytes) -> int:
+    if len(w1) > 31 or len(w2) > 31:
+        raise ValueError("LIKE words are limited to 31 bytes")
+    buf = np.zeros(64, np.uint8)
+    buf[:len(w1)] = np.frombuffer(w1, np.uint8)
+    buf[32:32 + len(w2)] = np.frombuffer(w2, np.uint8)
+    return ctx.upload(buf)
+
+
+def run_q13(ctx: E.Context, dev: dict, w1: bytes, w2: bytes, negate: bool = False, max_a: int | None = None,
+            words_ptr: int | None = None, warmup: int = 0, measured: int = 1,
+            executor: E.HipGenericExecutor | None = None) -> QueryRun:
+    plan = get_plan(ctx, "Q13L", "negate=1" if negate else "")
+    ex = _executor(ctx, executor)
+    ex.register_table(dev["S"])
+    ex.register_table(dev["A"])
+    own = words_ptr is None
+    wp = like_words_buffer(ctx, w1, w2) if own else words_ptr
+    try:
+        ex.register_buffer("d_like_words", wp, 64)
+        ex.register_scalar("like_l1", len(w1))
+        ex.register_scalar("like_l2", len(w2))
+        ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
+        t = ex.execute(plan, warmup, measured)
+        res = ex.collect(plan)
+    finally:
+        if own:
+            ctx.free(wp)
+    return QueryRun(res, t, plan, ex)
+
+
+def run_q22(ctx: E.Context, dev: dict, date_lo: int, date_hi: int, max_a: int | None = None, warmup: int = 0,
+            measured: int = 1, executor: E.HipGenericExecutor | None = None) -> QueryRun:
+    plan = get_plan(ctx, "Q22A")
+    ex = _executor(ctx, executor)
+    ex.register_table(dev["B"])
+    ex.register_table(dev["A"])
+    ex.register_scalar("q22_date_lo", date_lo)
+    ex.register_scalar("q22_date_hi", date_hi)
+    ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
+    t = ex.execute(plan, warmup, measured)
+    return QueryRun(ex.collect(plan), t, plan, ex)
+
+
+def result_as_lists(res: E.GenericResult) -> dict:
+    return {k: [x if isinstance(x, str) else int(x) for x in v] for k, v in res.columns.items()}

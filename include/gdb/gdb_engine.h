@@ -1,0 +1,131 @@
+/*
+ * gdb_engine.h — C-ABI over the engine's host surfaces (plan builder,
+ * emitters, runtime compiler, runner, result collector) and the seeded
+ * column generators.
+ *
+ * The engine itself is C++ (csrc/planning, csrc/operators, csrc/core,
+ * csrc/execution) and reaches the device only through gdb_hip.h. These entry
+ * points expose the reference's four host surfaces to callers that cannot
+ * link C++ (tests, benchmarks, a Python/ctypes or Go/cgo front end):
+ *   Plan API     buildMetalPlan / generateFromPlan   (codegen/planning/metal_plan_builder.h:73-78)
+ *   Runner API   RuntimeCompiler + MetalGenericExecutor
+ *                (codegen/execution/runtime_compiler.h:9-41, metal_generic_executor.h:46-164)
+ *   Result API   MetalResultCollector / GenericResult (codegen/execution/metal_result_collector.h:19-60)
+ *   Column API   loadQueryColumns (src/infra.h:930-1062) -> seeded generators + device columns
+ * Same conventions as gdb_hip.h: gdb_status return, gdb_last_error() text,
+ * no exception crosses the boundary.
+ */
+#ifndef GDB_ENGINE_H
+#define GDB_ENGINE_H
+
+#include "gdb_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gdb_plan_s* gdb_plan;
+typedef struct gdb_executor_s* gdb_executor;
+typedef struct gdb_result_s* gdb_result;
+
+/* Message of the last engine failure on this thread. */
+const char* gdb_engine_last_error(void);
+
+/* ---- Column generators (replace the .tbl/.colbin loaders, src/infra.h:141-782) ---- */
+gdb_status gdb_gen_uniform_i32(uint64_t seed, uint64_t stream, uint64_t n, int64_t lo, int64_t hi, int32_t* out);
+gdb_status gdb_gen_uniform_i64(uint64_t seed, uint64_t stream, uint64_t n, int64_t lo, int64_t hi, int64_t* out);
+gdb_status gdb_gen_categorical_u8(uint64_t seed, uint64_t stream, uint64_t n, const uint8_t* values,
+                                  const uint32_t* weights, int nvalues, uint8_t* out);
+gdb_status gdb_gen_permutation_i32(uint64_t seed, uint64_t stream, uint64_t n, int64_t base, int32_t* out);
+gdb_status gdb_gen_fanout_total(uint64_t seed, uint64_t stream, uint64_t nparent, int cmin, int cmax,
+                                uint64_t* total);
+gdb_status gdb_gen_fanout_fkey_i32(uint64_t seed, uint64_t stream, uint64_t nparent, int cmin, int cmax,
+                                   uint64_t nchild, int32_t* out);
+gdb_status gdb_gen_zipf_i32(uint64_t seed, uint64_t stream, uint64_t n, double s, int64_t nkeys, int32_t* out);
+gdb_status gdb_gen_vocabulary(uint64_t seed, int nwords, int minlen, int maxlen, char* out /* nwords*16 */);
+gdb_status gdb_gen_text(uint64_t seed, uint64_t stream, uint64_t n, int width, const char* vocab, int nwords,
+                        char* out /* n*width */);
+
+/* ---- Plan API ---- */
+/* Build a plan by name ("Q1", "Q3", "GROUPBY", ...) with shape options
+ * "key=value,..." and run the emitters. Replaces buildMetalPlan +
+ * generateFromPlan (codegen/planning/metal_plan_builder.h:73-78). */
+gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out);
+/* Plan tree (the reference's --print-plan) and generated HIP source
+ * (--dump-msl). Two-call pattern: need receives the size incl. NUL. */
+gdb_status gdb_plan_describe(gdb_plan p, char* buf, size_t cap, size_t* need);
+gdb_status gdb_plan_source(gdb_plan p, char* buf, size_t cap, size_t* need);
+gdb_status gdb_plan_phase_count(gdb_plan p, int* n);
+gdb_status gdb_plan_phase_name(gdb_plan p, int idx, char* buf, size_t cap);
+/* Launch configuration of one phase (block size, ITEMS variant); the
+ * reference's getPhasesMutable() used by --autotune-tg
+ * (codegen/codegen_main.cpp:837-929). */
+gdb_status gdb_plan_set_launch(gdb_plan p, const char* phase, int block, int items);
+gdb_status gdb_plan_get_launch(gdb_plan p, const char* phase, int* block, int* items);
+/* Compile through the context's RuntimeCompiler (hipRTC + plan-hash cache).
+ * Replaces RuntimeCompiler::compile + getPipeline
+ * (codegen/execution/runtime_compiler.cpp:20-67). */
+gdb_status gdb_plan_compile(gdb_ctx ctx, gdb_plan p, double* ms, int* cache_hit, uint64_t* plan_hash);
+/* Warm the on-disk code-object cache without a device. */
+gdb_status gdb_plan_precompile(gdb_plan p, const char* cache_dir, const char* arch, int* was_cached,
+                               uint64_t* plan_hash);
+gdb_status gdb_plan_free(gdb_plan p);
+
+/* ---- Runner API (MetalGenericExecutor, metal_generic_executor.h:46-164) ---- */
+gdb_status gdb_executor_create(gdb_ctx ctx, gdb_stream s, gdb_executor* out);
+gdb_status gdb_executor_destroy(gdb_executor ex);
+gdb_status gdb_executor_register_table(gdb_executor ex, const char* table, uint64_t rows);
+gdb_status gdb_executor_register_column(gdb_executor ex, const char* table, const char* column,
+                                        const void* dptr);
+gdb_status gdb_executor_register_buffer(gdb_executor ex, const char* name, void* dptr, size_t bytes);
+gdb_status gdb_executor_register_symbol(gdb_executor ex, const char* name, uint64_t value);
+gdb_status gdb_executor_get_symbol(gdb_executor ex, const char* name, uint64_t* value, int* found);
+gdb_status gdb_executor_register_scalar(gdb_executor ex, const char* name, int64_t value);
+gdb_status gdb_executor_set_skip_init(gdb_executor ex, int skip);
+
+#define GDB_MAX_PHASES 16
+typedef struct {
+    float total_ms;                    /* measured run, all phases */
+    float alloc_ms;                    /* buffer allocation before the runs */
+    int nphases;
+    float phase_ms[GDB_MAX_PHASES];
+    unsigned grid[GDB_MAX_PHASES];
+    unsigned block[GDB_MAX_PHASES];
+    unsigned items[GDB_MAX_PHASES];
+    uint64_t status[GDB_MAX_PHASES][4]; /* [0] overflow flag, [2] keys inserted, [3] LDS spills */
+} gdb_exec_timing;
+
+/* Warm up, run `measured` times, time phases with events; then check the
+ * phases' status words. A hash-table overflow returns GDB_ERR_OVERFLOW with
+ * the timing still filled. Replaces execute(compiled, cg, warmup, measured)
+ * (codegen/execution/metal_generic_executor.cpp:289-399). */
+gdb_status gdb_executor_execute(gdb_executor ex, gdb_plan p, int warmup, int measured, gdb_exec_timing* out);
+/* Phases [first, last) only (last = -1: to the end). Replaces the
+ * phase-range execute (metal_generic_executor.h:86-93). */
+gdb_status gdb_executor_execute_range(gdb_executor ex, gdb_plan p, int first, int last, int warmup, int measured,
+                                      gdb_exec_timing* out);
+/* Allocate, then launch without synchronising (graph capture, overlap). */
+gdb_status gdb_executor_prepare(gdb_executor ex, gdb_plan p);
+gdb_status gdb_executor_enqueue(gdb_executor ex, gdb_plan p);
+gdb_status gdb_executor_check_status(gdb_executor ex, gdb_plan p);
+gdb_status gdb_executor_read_buffer(gdb_executor ex, const char* name, size_t offset, void* host, size_t bytes);
+gdb_status gdb_executor_buffer_ptr(gdb_executor ex, const char* name, void** dptr, size_t* bytes);
+gdb_status gdb_executor_owned_bytes(gdb_executor ex, size_t* bytes);
+gdb_status gdb_executor_release(gdb_executor ex);
+
+/* ---- Result API (GenericResult, metal_result_collector.h:19-60) ---- */
+gdb_status gdb_executor_collect(gdb_executor ex, gdb_plan p, gdb_result* out);
+gdb_status gdb_result_shape(gdb_result r, uint64_t* rows, int* cols);
+/* type: 0 = int64, 1 = int128 (two int64 per row: low word, high word), 2 = string */
+gdb_status gdb_result_column(gdb_result r, int idx, char* name, size_t cap, int* type);
+gdb_status gdb_result_i64(gdb_result r, int idx, int64_t* out);
+gdb_status gdb_result_i128(gdb_result r, int idx, int64_t* out);
+gdb_status gdb_result_str(gdb_result r, int idx, char* out, size_t width);
+gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* need);
+gdb_status gdb_result_free(gdb_result r);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GDB_ENGINE_H */

@@ -1,0 +1,120 @@
+"""C-ABI boundary checks that need no GPU: the libraries load, export every
+function the headers declare, build and emit every plan, compile generated
+sources with hipRTC into the code-object cache, and report errors through
+status codes."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+from gpudbmetalcodegen_amd import _native as N
+from gpudbmetalcodegen_amd import engine as E
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADERS = sorted((ROOT / "include" / "gdb").glob("*.h"))
+
+PLANS = [("Q1", ""), ("Q6", ""), ("Q3", ""), ("Q3", "probe_bitmap=0"), ("Q3_SIZE", ""), ("GROUPBY", ""),
+         ("GROUPBY", "variant=global,filter=1"), ("GB_COUNT", ""), ("MAXKEY", "table=G,column=g_key"),
+         ("Q13L", ""), ("Q13L", "negate=1"), ("Q22A", "")]
+
+
+def declared_functions():
+    names = set()
+    for h in HEADERS:
+        text = h.read_text()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^\s*(?:[A-Za-z_][\w\s\*]*?)\b(gdb_\w+)\s*\(", text, flags=re.M):
+            names.add(m.group(1))
+    return sorted(names)
+
+
+def test_headers_declare_functions():
+    names = declared_functions()
+    assert len(names) > 60
+    assert "gdb_launch" in names and "gdb_plan_build" in names
+
+
+def test_every_declared_symbol_is_exported():
+    hip, eng = N.load()
+    missing = [n for n in declared_functions() if not (hasattr(hip, n) or hasattr(eng, n))]
+    assert missing == []
+
+
+def test_no_gpu_paths_report_status():
+    n = E.device_count()
+    if n > 0:
+        pytest.skip("a GPU is visible")
+    hip, _ = N.load()
+    h = C.c_void_p()
+    assert hip.gdb_init(0, C.byref(h)) == N.GDB_ERR_NO_DEVICE
+    assert b"no GPU" in hip.gdb_last_error()
+
+
+@pytest.mark.parametrize("name,opts", PLANS)
+def test_plan_build_describe_source(name, opts):
+    p = E.build_hip_plan(name, opts)
+    d = p.describe()
+    src = p.source()
+    assert f"Plan {name}" in d
+    for ph in p.phases():
+        assert f'extern "C" __global__' in src and f"{ph}_i" in src
+    assert "gdb device runtime" in src
+
+
+def test_unknown_plan_and_bad_options():
+    with pytest.raises(E.GdbError) as ei:
+        E.build_hip_plan("Q99")
+    assert ei.value.code == N.GDB_ERR_NOT_FOUND
+    with pytest.raises(E.GdbError):
+        E.build_hip_plan("Q1", "items")  # not key=value
+
+
+def test_set_launch_validates_variants():
+    p = E.build_hip_plan("Q1")
+    p.set_launch("Q1_reduce", 128, 4)
+    assert p.get_launch("Q1_reduce") == (128, 4)
+    with pytest.raises(E.GdbError):
+        p.set_launch("Q1_reduce", 256, 3)   # no ITEMS=3 variant compiled
+    with pytest.raises(E.GdbError):
+        p.set_launch("Q1_reduce", 512, 4)   # above the launch bound
+    with pytest.raises(E.GdbError):
+        p.set_launch("nope", 256, 4)
+
+
+def test_hiprtc_precompile_cache_hit_and_corruption(tmp_path):
+    p = E.build_hip_plan("MAXKEY", "table=G,column=g_key")
+    assert p.precompile(str(tmp_path)) is False        # compiled
+    assert p.precompile(str(tmp_path)) is True         # cache hit
+    files = list(tmp_path.glob("*.co"))
+    assert len(files) == 1
+    raw = bytearray(files[0].read_bytes())
+    raw[-1] ^= 0xFF                                    # corrupt the payload
+    files[0].write_bytes(bytes(raw))
+    assert p.precompile(str(tmp_path)) is False        # checksum mismatch -> recompiled
+    assert p.precompile(str(tmp_path)) is True
+
+
+def test_hiprtc_compile_error_surfaces_log(tmp_path):
+    hip, _ = N.load()
+    src = b'extern "C" __global__ void broken(int* p) { p[0] = undefined_name; }'
+    h, wc = C.c_uint64(), C.c_int()
+    st = hip.gdb_module_precompile(str(tmp_path).encode(), b"gfx950", src, len(src), None, 0, C.byref(h),
+                                   C.byref(wc))
+    assert st == N.GDB_ERR_COMPILE
+    msg = hip.gdb_last_error().decode()
+    assert "undefined_name" in msg and "1: extern" in msg   # numbered source dump
+
+
+def test_plan_hash_depends_on_source_and_options(tmp_path):
+    hip, _ = N.load()
+    src = b'extern "C" __global__ void k(int* p) { p[0] = 1; }'
+    keys = set()
+    for opts in ([], [b"-DX=1"]):
+        arr = (C.c_char_p * max(1, len(opts)))(*opts) if opts else None
+        h, wc = C.c_uint64(), C.c_int()
+        assert hip.gdb_module_precompile(str(tmp_path).encode(), b"gfx950", src, len(src), arr, len(opts),
+                                         C.byref(h), C.byref(wc)) == 0
+        keys.add(h.value)
+    assert len(keys) == 2

@@ -1,0 +1,91 @@
+"""Workload 2 (hash group-by of sum/count/min/max per key) on the GPU vs the
+oracle, for both table variants and all three sizing methods (exact pre-pass,
+given size, resize-on-overflow). Results are compared group by group."""
+
+import numpy as np
+import pytest
+
+import oracle
+from gpudbmetalcodegen_amd import queries as Q
+from gpudbmetalcodegen_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def _cmp(got, exp):
+    cols = got.columns
+    assert np.array_equal(cols["key"], exp["key"])
+    for c in ("count", "sum", "min", "max"):
+        assert np.array_equal(cols[c], exp[c]), c
+
+
+def _gen(n, s, nkeys, seed):
+    return W.workload2(n=n, s=s, nkeys=nkeys, seed=seed)["G"]
+
+
+@pytest.mark.parametrize("variant", ["lds", "global"])
+@pytest.mark.parametrize("n,s,nkeys", [(1, 1.1, 10), (5000, 1.1, 1000), (300_000, 1.1, 200_000),
+                                       (300_000, 0.0, 50_000), (200_000, 1.5, 20)])
+def test_groupby_prepass(ctx, variant, n, s, nkeys):
+    G = _gen(n, s, nkeys, 100 + n)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, prepass=True, variant=variant, key_min=1, key_max=nkeys)
+        exp = oracle.groupby(G)
+        _cmp(run.result, exp)
+        assert run.resizes == 0
+        assert run.slots >= 2 * len(exp["key"])
+    finally:
+        dev["G"].free(ctx)
+
+
+def test_groupby_filter_selectivity(ctx):
+    G = _gen(200_000, 1.1, 100_000, 7)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        for cut in (0, 10_000, 500_000, 1_000_001):
+            run = Q.run_groupby(ctx, dev, prepass=True, val_cut=cut, key_min=1, key_max=100_000)
+            _cmp(run.result, oracle.groupby(G, cut))
+    finally:
+        dev["G"].free(ctx)
+
+
+def test_groupby_resize_on_overflow_stays_exact(ctx):
+    G = _gen(400_000, 0.5, 300_000, 9)
+    exp = oracle.groupby(G)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, slots=1024, variant="lds")   # far too small on purpose
+        assert run.resizes >= 1
+        _cmp(run.result, exp)
+        run2 = Q.run_groupby(ctx, dev, slots=4 * len(exp["key"]), variant="global")
+        assert run2.resizes == 0
+        _cmp(run2.result, exp)
+    finally:
+        dev["G"].free(ctx)
+
+
+def test_groupby_lds_slot_sizes(ctx):
+    G = _gen(250_000, 1.2, 30_000, 13)
+    exp = oracle.groupby(G)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        for ls in (256, 1024, 4096):
+            run = Q.run_groupby(ctx, dev, prepass=True, lds_slots=ls, key_min=1, key_max=30_000)
+            _cmp(run.result, exp)
+    finally:
+        dev["G"].free(ctx)
+
+
+@pytest.mark.slow
+def test_groupby_full_size(ctx):
+    G = W.workload2()["G"]
+    exp = oracle.groupby(G, threads=16)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=20_000_000)
+        _cmp(run.result, exp)
+        assert int(run.result.columns["count"].sum()) == G["g_key"].shape[0]
+        print("GROUPBY full", run.timing.total_ms, run.timing.phases, run.timing.status)
+    finally:
+        dev["G"].free(ctx)
