@@ -198,6 +198,85 @@ def cpu_leg_q1(F, n, res):
              "sample": f"all {n} rows, median of 3 runs of the oracle's threaded Q1"}, exact)
 
 
+# ---------------------------------------------------------------------------
+# Workload 2 — skewed hash group-by: plain (exact pre-pass) vs learned sizing
+# ---------------------------------------------------------------------------
+def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
+    from gpudbmetalcodegen_amd import estimator as EST
+
+    n, nkeys = 120_000_000, 20_000_000
+    log(f"generating workload 2 ({n} Zipf(1.1) keys over 1..{nkeys})")
+    G = W.workload2()["G"]
+    dev = Q.upload_tables(ctx, {"G": G})
+    out = {"workload": f"hash group-by sum/count/min/max, {n} rows, Zipf(1.1) keys over 1..{nkeys}", "rows": n}
+    # plain form: exact distinct-count pre-pass sizes the table (2 extra kernels + a host sync)
+    run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=nkeys)
+    ex, plan = run.executor, run.plan
+    ex.execute(plan, warmup, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    t = ex.execute(plan, 0, steps)
+    plain_ms = (time.perf_counter() - t0) * 1e3 / steps
+    groups = run.result.rows()
+    phases = {name: ms for name, ms, *_ in t.phases}
+    out["plain"] = {"ms_per_step": plain_ms, "rows_per_s": n / (plain_ms / 1e3), "slots": run.slots,
+                    "phase_ms": {k: round(v, 4) for k, v in phases.items()}}
+    gb_bytes = 12.0 * n + 32.0 * groups
+    out["roofline"] = roofline("GB_agg", phases["GB_agg"], gb_bytes, "w2")
+    plain_res = run.result
+    ex.release()
+    # learned form: sample + MLP inference size the table and pick the variant; one pass
+    learned = None
+    if EST.WEIGHTS.exists():
+        est = EST.CardinalityEstimator(ctx).load()
+        keys_ptr, vals_ptr = dev["G"].columns["g_key"], dev["G"].columns["g_val"]
+
+        def learned_step():
+            hk, _ = EST.device_sample(ctx, keys_ptr, None, n)
+            x, ub = EST.features(n, 1, nkeys, hk)
+            return EST.size_for(est, x, ub)
+
+        sz = learned_step()
+        r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots)
+        for _ in range(warmup):
+            sz = learned_step()
+            r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
+                               executor=r2.executor)
+        ctx.sync()
+        t0 = time.perf_counter()
+        resizes = 0
+        for _ in range(steps):
+            sz = learned_step()
+            r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
+                               executor=r2.executor)
+            resizes += r2.resizes
+        learned_ms = (time.perf_counter() - t0) * 1e3 / steps
+        same = all(np.array_equal(plain_res.columns[c], r2.result.columns[c]) for c in plain_res.columns)
+        learned = {"ms_per_step": learned_ms, "rows_per_s": n / (learned_ms / 1e3), "slots": r2.slots,
+                   "variant": sz.variant, "predicted_groups": round(sz.groups_pred), "true_groups": groups,
+                   "resizes": resizes, "identical_to_plain": bool(same),
+                   "gb_kernel_ms": round(r2.timing.phases[-1][1], 4)}
+        r2.executor.release()
+    out["learned"] = learned
+    out["rows_per_s"] = (learned or out["plain"])["rows_per_s"]
+    if cpu:
+        import oracle
+
+        th = host_threads()
+        sample = 40_000_000
+        Gs = {k: v[:sample] for k, v in G.items()}
+        t0 = time.perf_counter()
+        oracle.groupby(Gs, threads=th)
+        dt = time.perf_counter() - t0
+        full = oracle.groupby(G, threads=th)
+        exact = all(np.array_equal(plain_res.columns[c], full[c]) for c in full)
+        out["cpu_baseline"] = {"value": sample / dt, "unit": "rows/s", "cores": th, "kind": "port",
+                               "sample": f"first {sample} rows, one run of the oracle's threaded group-by"}
+        out["bit_exact"] = exact
+    dev["G"].free(ctx)
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -220,7 +299,8 @@ def main() -> int:
     pipelines = {}
     if not args.skip_extra:
         for name, fn in [("w0_6M", lambda: bench_q1(ctx, 6_000_000, args.steps, args.warmup, not args.no_cpu)),
-                         ("w0_120M", lambda: bench_q1(ctx, 120_000_000, args.steps, args.warmup, not args.no_cpu))]:
+                         ("w0_120M", lambda: bench_q1(ctx, 120_000_000, args.steps, args.warmup, not args.no_cpu)),
+                         ("w2_groupby", lambda: bench_groupby(ctx, max(3, args.steps // 4), 1, not args.no_cpu))]:
             try:
                 pipelines[name] = fn()
             except Exception as e:  # report, keep the headline line

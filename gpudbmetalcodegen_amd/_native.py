@@ -172,6 +172,27 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_result_str", [vp, i32, vp, sz])
     _sig(eng, "gdb_result_canonical", [vp, vp, sz, P(sz)])
     _sig(eng, "gdb_result_free", [vp])
+    # gdb_kernels.h (in libgdbhip.so)
+    _sig(hip, "gdb_kernels_last_error", [], C.c_char_p)
+    _sig(hip, "gdb_mlp_forward", [vp, vp, vp, i32, vp])
+    _sig(hip, "gdb_mlp_backward", [vp, vp, vp, vp, vp, i32, C.c_float, vp])
+    _sig(hip, "gdb_mlp_reduce_adam", [vp, vp, vp, vp, vp, i32, i32, C.c_float, C.c_float, C.c_float, C.c_float, vp])
+    _sig(hip, "gdb_gen_device_zipf_i32", [vp, u64, C.c_double, i64, u64, vp])
+    _sig(hip, "gdb_gen_device_uniform_i64", [vp, u64, i64, i64, u64, vp])
+    _sig(hip, "gdb_copy_async", [vp, vp, sz, vp])
+    # estimator (gdb_engine.h)
+    _sig(eng, "gdb_est_features", [u64, i64, i64, i64, vp, vp, u64, vp, P(C.c_double)])
+    _sig(eng, "gdb_est_sample", [vp, vp, vp, u64, u64, vp, vp])
+    _sig(eng, "gdb_est_create", [vp, P(vp)])
+    _sig(eng, "gdb_est_destroy", [vp])
+    _sig(eng, "gdb_est_init", [vp, u64])
+    _sig(eng, "gdb_est_train", [vp, vp, vp, i32, i32, i32, C.c_float, u64, vp])
+    _sig(eng, "gdb_est_predict", [vp, vp, i32, vp])
+    _sig(eng, "gdb_est_save", [vp, cstr])
+    _sig(eng, "gdb_est_load", [vp, cstr])
+    _sig(eng, "gdb_est_get_state", [vp, vp, vp, vp])
+    _sig(eng, "gdb_est_set_state", [vp, vp, vp, vp])
+    _sig(eng, "gdb_est_collect_runs", [vp, i32, u64, u64, u64, i64, i64, C.c_double, C.c_double, vp, vp, vp, vp])
     _hip, _eng = hip, eng
     return hip, eng
 

@@ -7,6 +7,7 @@
 #include "execution/runtime_compiler.h"
 #include "infra/generators.h"
 #include "planning/hip_plan_builder.h"
+#include "runtime/estimator.h"
 
 #include <algorithm>
 #include <cstring>
@@ -495,6 +496,126 @@ gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* nee
 gdb_status gdb_result_free(gdb_result r) {
     delete r;
     return GDB_OK;
+}
+
+// ---- estimator ----
+gdb_status gdb_est_features(uint64_t n_rows, int64_t key_min, int64_t key_max, int64_t val_cut,
+                            const int32_t* sample_keys, const int64_t* sample_vals, uint64_t sample_n, float* out16,
+                            double* upper_bound) {
+    return guard([&]() -> gdb_status {
+        runtime::FeatureInput fi;
+        fi.nRows = n_rows;
+        fi.keyMin = key_min;
+        fi.keyMax = key_max;
+        fi.valCut = val_cut;
+        fi.sampleKeys = sample_keys;
+        fi.sampleVals = sample_vals;
+        fi.sampleSize = sample_n;
+        auto f = runtime::computeFeatures(fi);
+        std::memcpy(out16, f.x.data(), sizeof(float) * runtime::kFeatures);
+        if (upper_bound) *upper_bound = f.upperBound;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_sample(gdb_ctx ctx, const void* dkeys, const void* dvals, uint64_t n, uint64_t sample_n,
+                          int32_t* hkeys, int64_t* hvals) {
+    return guard([&]() -> gdb_status {
+        runtime::sampleDeviceColumns(ctx, gdb_default_stream(ctx), static_cast<const int32_t*>(dkeys),
+                                     static_cast<const int64_t*>(dvals), n, sample_n, hkeys, hvals);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_create(gdb_ctx ctx, gdb_estimator* out) {
+    return guard([&]() -> gdb_status {
+        *out = reinterpret_cast<gdb_estimator>(new runtime::CardinalityEstimator(ctx));
+        return GDB_OK;
+    });
+}
+
+static runtime::CardinalityEstimator& est(gdb_estimator e) { return *reinterpret_cast<runtime::CardinalityEstimator*>(e); }
+
+gdb_status gdb_est_destroy(gdb_estimator e) {
+    delete reinterpret_cast<runtime::CardinalityEstimator*>(e);
+    return GDB_OK;
+}
+
+gdb_status gdb_est_init(gdb_estimator e, uint64_t seed) {
+    return guard([&]() -> gdb_status {
+        est(e).initWeights(seed);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_train(gdb_estimator e, const float* X, const float* y, int n, int epochs, int batch, float lr,
+                         uint64_t seed, float* losses) {
+    return guard([&]() -> gdb_status {
+        runtime::TrainConfig cfg;
+        cfg.epochs = epochs;
+        cfg.batch = batch;
+        cfg.lr = lr;
+        cfg.seed = seed;
+        std::vector<float> l;
+        est(e).train(X, y, n, cfg, losses ? &l : nullptr);
+        if (losses) std::memcpy(losses, l.data(), l.size() * sizeof(float));
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_predict(gdb_estimator e, const float* X, int n, float* out) {
+    return guard([&]() -> gdb_status {
+        est(e).predict(X, n, out);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_save(gdb_estimator e, const char* path) {
+    return guard([&]() -> gdb_status {
+        est(e).save(path);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_load(gdb_estimator e, const char* path) {
+    return guard([&]() -> gdb_status {
+        est(e).load(path);
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_est_get_state(gdb_estimator e, float* params, float* mean16, float* std16) {
+    auto& x = est(e);
+    if (params) std::memcpy(params, x.params().data(), x.params().size() * sizeof(float));
+    if (mean16) std::memcpy(mean16, x.mean().data(), sizeof(float) * runtime::kFeatures);
+    if (std16) std::memcpy(std16, x.stdev().data(), sizeof(float) * runtime::kFeatures);
+    return GDB_OK;
+}
+
+gdb_status gdb_est_set_state(gdb_estimator e, const float* params, const float* mean16, const float* std16) {
+    auto& x = est(e);
+    if (params) std::memcpy(x.params().data(), params, x.params().size() * sizeof(float));
+    if (mean16) std::memcpy(x.mean().data(), mean16, sizeof(float) * runtime::kFeatures);
+    if (std16) std::memcpy(x.stdev().data(), std16, sizeof(float) * runtime::kFeatures);
+    return GDB_OK;
+}
+
+gdb_status gdb_est_collect_runs(gdb_ctx ctx, int nruns, uint64_t seed, uint64_t nmin, uint64_t nmax, int64_t kmin,
+                                int64_t kmax, double smax, double selmin, float* X, double* ub, double* groups,
+                                double* spec) {
+    return guard([&]() -> gdb_status {
+        auto logs = runtime::collectRuns(ctx, nruns, seed, nmin, nmax, kmin, kmax, smax, selmin);
+        for (size_t r = 0; r < logs.size(); ++r) {
+            std::memcpy(X + r * runtime::kFeatures, logs[r].f.x.data(), sizeof(float) * runtime::kFeatures);
+            ub[r] = logs[r].f.upperBound;
+            groups[r] = logs[r].groups;
+            spec[r * 4 + 0] = static_cast<double>(logs[r].spec.n);
+            spec[r * 4 + 1] = logs[r].spec.skew;
+            spec[r * 4 + 2] = static_cast<double>(logs[r].spec.nkeys);
+            spec[r * 4 + 3] = logs[r].spec.sel;
+        }
+        return GDB_OK;
+    });
 }
 
 }  // extern "C"

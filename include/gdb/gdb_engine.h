@@ -124,6 +124,40 @@ gdb_status gdb_result_str(gdb_result r, int idx, char* out, size_t width);
 gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* need);
 gdb_status gdb_result_free(gdb_result r);
 
+/* ---- Learned group-count estimator ----
+ * No counterpart in the reference, which sizes tables only by an exact
+ * max-key pre-pass (codegen/execution/max_key_symbols.cpp:188-235); callers
+ * are the group-by runner (table size, LDS vs global variant) and its tests.
+ * Features: 16 floats from plan statistics and a row sample; target:
+ * log2(groups) - log2(upper_bound), upper_bound = min(qualifying rows, key range). */
+typedef struct gdb_estimator_s* gdb_estimator;
+#define GDB_EST_FEATURES 16
+
+gdb_status gdb_est_features(uint64_t n_rows, int64_t key_min, int64_t key_max, int64_t val_cut,
+                            const int32_t* sample_keys, const int64_t* sample_vals, uint64_t sample_n, float* out16,
+                            double* upper_bound);
+/* Copy a sample of sample_n rows (8 evenly spaced chunks) of device columns to the host. */
+gdb_status gdb_est_sample(gdb_ctx ctx, const void* dkeys, const void* dvals, uint64_t n, uint64_t sample_n,
+                          int32_t* hkeys, int64_t* hvals);
+gdb_status gdb_est_create(gdb_ctx ctx, gdb_estimator* out);
+gdb_status gdb_est_destroy(gdb_estimator e);
+gdb_status gdb_est_init(gdb_estimator e, uint64_t seed);
+/* Train on the GPU (kernels of gdb_kernels.h); losses[epochs] optional. */
+gdb_status gdb_est_train(gdb_estimator e, const float* X, const float* y, int n, int epochs, int batch, float lr,
+                         uint64_t seed, float* losses);
+gdb_status gdb_est_predict(gdb_estimator e, const float* X, int n, float* out);
+gdb_status gdb_est_save(gdb_estimator e, const char* path);
+gdb_status gdb_est_load(gdb_estimator e, const char* path);
+gdb_status gdb_est_get_state(gdb_estimator e, float* params, float* mean16, float* std16);
+gdb_status gdb_est_set_state(gdb_estimator e, const float* params, const float* mean16, const float* std16);
+/* Log `nruns` training runs: device-generated Zipf keys (n, key range, skew,
+ * selectivity drawn from the given ranges), features from a sample, and the
+ * exact group count from the engine's GB_COUNT plan. spec[r*4..] = n, skew,
+ * key range, selectivity. */
+gdb_status gdb_est_collect_runs(gdb_ctx ctx, int nruns, uint64_t seed, uint64_t nmin, uint64_t nmax, int64_t kmin,
+                                int64_t kmax, double smax, double selmin, float* X, double* ub, double* groups,
+                                double* spec);
+
 #ifdef __cplusplus
 }
 #endif

@@ -166,3 +166,27 @@ def q22(B: dict, A: dict, max_key: int, date_lo: int, date_hi: int):
     lib.oracle_q22(_u64(b_fkey.shape[0]), ptr(b_fkey), ptr(b_date), _u64(a_key.shape[0]), ptr(a_key), ptr(a_seg),
                    _i64(max_key), _i64(date_lo), _i64(date_hi), ptr(out))
     return out[:5], out[5:]
+
+
+MLP_NPARAMS = 5313
+
+
+def mlp_forward(params: np.ndarray, X: np.ndarray) -> np.ndarray:
+    lib = load()
+    P = np.ascontiguousarray(params, np.float32)
+    Xc = np.ascontiguousarray(X, np.float32)
+    y = np.zeros(Xc.shape[0], np.float64)
+    lib.oracle_mlp_forward.restype = None
+    lib.oracle_mlp_forward(ptr(P), ptr(Xc), C.c_int(Xc.shape[0]), ptr(y))
+    return y
+
+
+def mlp_grad(params: np.ndarray, X: np.ndarray, T: np.ndarray, scale: float):
+    lib = load()
+    P = np.ascontiguousarray(params, np.float32)
+    Xc = np.ascontiguousarray(X, np.float32)
+    Tc = np.ascontiguousarray(T, np.float32)
+    g = np.zeros(MLP_NPARAMS, np.float64)
+    lib.oracle_mlp_grad.restype = C.c_double
+    loss = lib.oracle_mlp_grad(ptr(P), ptr(Xc), ptr(Tc), C.c_int(Xc.shape[0]), C.c_double(scale), ptr(g))
+    return g, loss

@@ -492,3 +492,60 @@ void oracle_q22(uint64_t n_b, const int32_t* b_fkey, const int32_t* b_date, uint
     }
     free(bset);
 }
+
+/* ------------------------------------------------------------------------ */
+/* Estimator MLP 16 -> 64 -> 64 -> 1 (ReLU), double precision                */
+/* ------------------------------------------------------------------------ */
+
+enum { MI = 16, MH = 64, OW1 = 0, OB1 = 1024, OW2 = 1088, OB2 = 5184, OW3 = 5248, OB3 = 5312, MNP = 5313 };
+
+static double mlp_one(const float* P, const float* x, double* a1, double* h1, double* a2, double* h2) {
+    for (int j = 0; j < MH; ++j) {
+        double a = P[OB1 + j];
+        for (int i = 0; i < MI; ++i) a += (double)P[OW1 + j * MI + i] * x[i];
+        a1[j] = a;
+        h1[j] = a > 0 ? a : 0;
+    }
+    double y = P[OB3];
+    for (int j = 0; j < MH; ++j) {
+        double a = P[OB2 + j];
+        for (int k = 0; k < MH; ++k) a += (double)P[OW2 + j * MH + k] * h1[k];
+        a2[j] = a;
+        h2[j] = a > 0 ? a : 0;
+        y += (double)P[OW3 + j] * h2[j];
+    }
+    return y;
+}
+
+void oracle_mlp_forward(const float* params, const float* X, int n, double* y) {
+    double a1[MH], h1[MH], a2[MH], h2[MH];
+    for (int s = 0; s < n; ++s) y[s] = mlp_one(params, X + (size_t)s * MI, a1, h1, a2, h2);
+}
+
+double oracle_mlp_grad(const float* params, const float* X, const float* T, int n, double scale, double* grad) {
+    double a1[MH], h1[MH], a2[MH], h2[MH], d2[MH];
+    double loss = 0;
+    for (int p = 0; p < MNP; ++p) grad[p] = 0;
+    for (int s = 0; s < n; ++s) {
+        const float* x = X + (size_t)s * MI;
+        const double y = mlp_one(params, x, a1, h1, a2, h2);
+        const double diff = y - T[s];
+        loss += scale * diff * diff;
+        const double d3 = 2.0 * scale * diff;
+        grad[OB3] += d3;
+        for (int j = 0; j < MH; ++j) {
+            grad[OW3 + j] += d3 * h2[j];
+            d2[j] = a2[j] > 0 ? d3 * params[OW3 + j] : 0;
+            grad[OB2 + j] += d2[j];
+            for (int k = 0; k < MH; ++k) grad[OW2 + j * MH + k] += d2[j] * h1[k];
+        }
+        for (int k = 0; k < MH; ++k) {
+            double acc = 0;
+            for (int j = 0; j < MH; ++j) acc += (double)params[OW2 + j * MH + k] * d2[j];
+            const double d1 = a1[k] > 0 ? acc : 0;
+            grad[OB1 + k] += d1;
+            for (int i = 0; i < MI; ++i) grad[OW1 + k * MI + i] += d1 * x[i];
+        }
+    }
+    return loss;
+}

@@ -100,6 +100,15 @@ int64_t oracle_q13(uint64_t n_s, const int32_t* s_fkey, const char* s_text, int 
 void oracle_q22(uint64_t n_b, const int32_t* b_fkey, const int32_t* b_date, uint64_t n_a, const int32_t* a_key,
                 const uint8_t* a_seg, int64_t max_key, int64_t date_lo, int64_t date_hi, int64_t* out10);
 
+/* ---- Group-count estimator MLP (16 -> 64 -> 64 -> 1, ReLU), double precision ----
+ * No counterpart in the reference. Parameter layout (5313 floats):
+ * W1[64][16] @0, b1 @1024, W2[64][64] @1088, b2 @5184, W3[64] @5248, b3 @5312.
+ * oracle_mlp_forward: y[s] for s < n.
+ * oracle_mlp_grad: gradient of L = scale * sum_s (y_s - t_s)^2 w.r.t. every
+ * parameter (grad[5313]); returns L. */
+void oracle_mlp_forward(const float* params, const float* X, int n, double* y);
+double oracle_mlp_grad(const float* params, const float* X, const float* T, int n, double scale, double* grad);
+
 #ifdef __cplusplus
 }
 #endif
