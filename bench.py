@@ -277,6 +277,86 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     return out
 
 
+# ---------------------------------------------------------------------------
+# Workload 3 — 64 uneven pipelines: plain per-pipeline vs persistent batched
+# ---------------------------------------------------------------------------
+def bench_batch(ctx, steps: int) -> dict:
+    from gpudbmetalcodegen_amd import batch as B
+
+    log("generating workload 3 (64 pipelines, 10^4..10^7 rows each)")
+    specs = B.workload3()
+    w = B.Workload3(ctx, specs, lds_cap=4096)
+    rows = w.total_rows
+    t0 = time.perf_counter()
+    tuned = w.autotune(trials=3)
+    tune_s = time.perf_counter() - t0
+    w.run_plain()
+    plain_ms = sorted(w.run_plain() for _ in range(max(3, steps // 4)))
+    plain = w.results()
+    best = None
+    for block in (256, 512, 1024):
+        b = w.build_batch(block, 65536)
+        b.run(graph=True)
+        ms = sorted(b.run(graph=True) for _ in range(max(3, steps // 4)))
+        ok = all(all(np.array_equal(np.asarray(g[k]), np.asarray(e[k])) for k in e) for g, e in zip(w.results(), plain))
+        if best is None or ms[len(ms) // 2] < best[1]:
+            best = (block, ms[len(ms) // 2], ok, b.grid, b.work_items)
+        b.close()
+    pm = plain_ms[len(plain_ms) // 2]
+    col_bytes = sum(s.n * (17 if s.kind == "agg" else 16) for s in specs)   # join rows: 12 B + a 4 B bitmap word
+    out = {"workload": f"64 pipelines (32 filter+group-by, 32 semi/anti-join), {rows} rows", "rows": rows,
+           "plain": {"ms_per_step": pm, "rows_per_s": rows / (pm / 1e3)},
+           "batched": {"ms_per_step": best[1], "rows_per_s": rows / (best[1] / 1e3), "block": best[0],
+                       "grid": best[3], "work_items": best[4], "graph": True},
+           "speedup": pm / best[1], "bit_exact_vs_plain": bool(best[2]), "autotune_s": round(tune_s, 2),
+           "tuned": {k: {kk: vv for kk, vv in v.items() if kk != "candidates"} for k, v in tuned.items()},
+           "rows_per_s": rows / (best[1] / 1e3),
+           "roofline": roofline("gdb_batch_kernel", best[1], col_bytes, "w3")}
+    w.close()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# Workload 4 — served stream over resident tables: plain vs continuous batching
+# ---------------------------------------------------------------------------
+def bench_serve(ctx) -> dict:
+    from gpudbmetalcodegen_amd import server as S
+
+    log("generating workload 4 (F 120M, B 30M, A 3M, S 4M x char[25])")
+    T = W.workload4()
+    db = S.ResidentDB(ctx, T)
+    del T
+    pool = [S.Worker(db) for _ in range(4)]
+    warm = S.make_stream(db, rate=400, duration=0.05, seed=1, backlog=True)
+    S.serve(db, warm, workers=4, budget_bytes=8 << 30, pool=pool)
+    out = {"workload": "served stream: scan-aggregate / 3-way join / anti-join / LIKE+group-by over "
+                       "F 120M, B 30M, A 3M, S 4M (resident %.2f GB)" % (db.resident_bytes / 1e9),
+           "rates": {}}
+    exact = True
+    for rate, dur in ((50, 2.0), (200, 2.0), (500, 1.5)):
+        a = S.make_stream(db, rate, dur, seed=rate)
+        b = S.make_stream(db, rate, dur, seed=rate)
+        plain = S.serve(db, a, workers=1, budget_bytes=1 << 40, pool=pool)
+        served = S.serve(db, b, workers=4, budget_bytes=8 << 30, pool=pool)
+        exact &= all(x.result == y.result for x, y in zip(a, b))
+        out["rates"][str(rate)] = {"plain": plain, "served": served}
+        log(f"rate {rate}: plain p99 {plain['p99_ms']:.2f} ms, served p99 {served['p99_ms']:.2f} ms")
+    a = S.make_stream(db, 400, 0.5, seed=99, backlog=True)
+    b = S.make_stream(db, 400, 0.5, seed=99, backlog=True)
+    plain = S.serve(db, a, workers=1, budget_bytes=1 << 40, pool=pool)
+    served = S.serve(db, b, workers=4, budget_bytes=8 << 30, pool=pool)
+    exact &= all(x.result == y.result for x, y in zip(a, b))
+    out["saturated"] = {"plain": plain, "served": served}
+    out["bit_exact_alone_vs_mix"] = bool(exact)
+    r500 = out["rates"]["500"]
+    out["rows_per_s"] = served["rows_per_s"]
+    out["p50_ms"], out["p99_ms"] = r500["served"]["p50_ms"], r500["served"]["p99_ms"]
+    for w in pool:
+        w.close()
+    db.close()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -300,7 +380,9 @@ def main() -> int:
     if not args.skip_extra:
         for name, fn in [("w0_6M", lambda: bench_q1(ctx, 6_000_000, args.steps, args.warmup, not args.no_cpu)),
                          ("w0_120M", lambda: bench_q1(ctx, 120_000_000, args.steps, args.warmup, not args.no_cpu)),
-                         ("w2_groupby", lambda: bench_groupby(ctx, max(3, args.steps // 4), 1, not args.no_cpu))]:
+                         ("w2_groupby", lambda: bench_groupby(ctx, max(3, args.steps // 4), 1, not args.no_cpu)),
+                         ("w3_batched", lambda: bench_batch(ctx, args.steps)),
+                         ("w4_served", lambda: bench_serve(ctx))]:
             try:
                 pipelines[name] = fn()
             except Exception as e:  # report, keep the headline line

@@ -180,6 +180,16 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(hip, "gdb_gen_device_zipf_i32", [vp, u64, C.c_double, i64, u64, vp])
     _sig(hip, "gdb_gen_device_uniform_i64", [vp, u64, i64, i64, u64, vp])
     _sig(hip, "gdb_copy_async", [vp, vp, sz, vp])
+    # autotuner + batch (gdb_engine.h)
+    _sig(eng, "gdb_autotune_phase", [vp, vp, cstr, vp, i32, vp, i32, cstr, vp, i32, i32, P(i32), P(i32), P(i64),
+                                     P(C.c_float), vp, P(i32)])
+    _sig(eng, "gdb_batch_create", [vp, P(vp)])
+    _sig(eng, "gdb_batch_add", [vp, vp, vp, i32, P(i32)])
+    _sig(eng, "gdb_batch_build", [vp, i32, u64, i32, P(C.c_uint), P(u64)])
+    _sig(eng, "gdb_batch_source", [vp, vp, sz, P(sz)])
+    _sig(eng, "gdb_batch_capture", [vp])
+    _sig(eng, "gdb_batch_run", [vp, i32, P(C.c_float)])
+    _sig(eng, "gdb_batch_destroy", [vp])
     # estimator (gdb_engine.h)
     _sig(eng, "gdb_est_features", [u64, i64, i64, i64, vp, vp, u64, vp, P(C.c_double)])
     _sig(eng, "gdb_est_sample", [vp, vp, vp, u64, u64, vp, vp])

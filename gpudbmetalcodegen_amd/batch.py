@@ -1,0 +1,231 @@
+"""Workload 3: 64 independent pipelines of uneven size, run either one query
+at a time (plain form) or all together in one persistent fused kernel
+captured in a graph (batched form), with launch shapes from the autotuner.
+
+Pipelines 0..31 filter + hash group-by (PIPE_AGG: p_date range, p_flag
+equality, group by p_key, sum/count/min/max of p_val); 32..63 alternate
+semi-join and anti-join against a 2,000,000-bit key bitmap with 30% of bits
+set (PIPE_SEMI / PIPE_ANTI: count and sum of p_val). Sizes are
+round(10^(4 + 3u)), u uniform, from the workload seed.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native as N
+from . import engine as E
+from . import workloads as W
+
+BITMAP_BITS = 2_000_000
+SIZE_CLASSES = [(0, 100_000, "small"), (100_000, 1_000_000, "medium"), (1_000_000, 1 << 62, "large")]
+
+
+def _e():
+    return N.load()[1]
+
+
+@dataclass
+class PipeSpec:
+    index: int
+    kind: str                 # "agg" | "semi" | "anti"
+    n: int
+    cols: dict
+    params: dict = field(default_factory=dict)
+    bitmap: np.ndarray | None = None
+
+    @property
+    def size_class(self) -> str:
+        for lo, hi, name in SIZE_CLASSES:
+            if lo <= self.n < hi:
+                return name
+        return "large"
+
+    @property
+    def shape(self) -> str:
+        return f"{self.kind}/{self.size_class}"
+
+
+def workload3(n_pipes: int = 64, scale: float = 1.0, seed: int | None = None) -> list[PipeSpec]:
+    sd = W.seed_for(3) if seed is None else seed
+    r = np.random.default_rng(sd)
+    sizes = np.round(10 ** (4 + 3 * r.random(n_pipes)) * scale).astype(np.int64)
+    specs = []
+    half = n_pipes // 2
+    for i in range(n_pipes):
+        n = int(max(1, sizes[i]))
+        s = sd * 1000 + i
+        if i < half:
+            K = int(round(2 ** r.uniform(4, 12)))           # 16 .. 4096 groups
+            k = W.uniform_i64(s, 1, n, 0, K - 1)
+            keys = ((k * 2654435761) % (1 << 31)).astype(np.int32)   # sparse int32 keys
+            cols = {"p_key": keys, "p_date": W.uniform_i32(s, 2, n, 0, 2405), "p_val": W.uniform_i64(s, 3, n, 0, 10**7),
+                    "p_flag": W.uniform_i32(s, 4, n, 0, 3).astype(np.uint8)}
+            lo = int(r.integers(0, 1200))
+            params = {"groups_bound": K, "pipe_date_lo": lo, "pipe_date_hi": lo + 1200, "pipe_flag": i % 4}
+            specs.append(PipeSpec(i, "agg", n, cols, params))
+        else:
+            cols = {"p_key": W.uniform_i32(s, 1, n, 0, BITMAP_BITS - 1), "p_val": W.uniform_i64(s, 3, n, 0, 10**7)}
+            bits = W.uniform_i32(s, 9, BITMAP_BITS, 0, 99) < 30
+            words = np.packbits(bits, bitorder="little")
+            words = np.concatenate([words, np.zeros((-words.size) % 4 + 4, np.uint8)]).view(np.uint32)
+            specs.append(PipeSpec(i, "semi" if (i - half) % 2 == 0 else "anti", n, cols, {}, words))
+    return specs
+
+
+def plan_name(kind: str) -> str:
+    return {"agg": "PIPE_AGG", "semi": "PIPE_SEMI", "anti": "PIPE_ANTI"}[kind]
+
+
+def phase_name(kind: str) -> str:
+    return {"agg": "PIPE_agg", "semi": "PIPE_semi", "anti": "PIPE_anti"}[kind]
+
+
+def next_pow2(x: int) -> int:
+    p = 1
+    while p < x:
+        p <<= 1
+    return p
+
+
+class Workload3:
+    """Device-resident pipelines, each with its own runner."""
+
+    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096):
+        self.ctx = ctx
+        self.specs = specs
+        self.lds_cap = lds_cap
+        self.plans = {k: E.build_hip_plan(plan_name(k), f"lds_cap={lds_cap}" if k == "agg" else "").compile(ctx)
+                      for k in {s.kind for s in specs}}
+        self.tables, self.execs, self.bitmaps = [], [], []
+        self.launch = {}   # pipe index -> (block, items, lds_bits)
+        for s in specs:
+            t = E.DeviceTable.from_host(ctx, "P", s.cols)
+            ex = E.HipGenericExecutor(ctx)
+            ex.register_table(t)
+            if s.kind == "agg":
+                for k in ("pipe_date_lo", "pipe_date_hi", "pipe_flag"):
+                    ex.register_scalar(k, s.params[k])
+                ex.register_scalar("pipe_lds_bits", 10)
+                ex.register_symbol("pipe_slots", max(64, next_pow2(2 * s.params["groups_bound"])))
+                bm = None
+            else:
+                bm = ctx.upload(s.bitmap)
+                ex.register_buffer("d_pipe_bitmap", bm, s.bitmap.nbytes)
+                ex.register_symbol("pipe_bits", BITMAP_BITS)
+            self.tables.append(t)
+            self.execs.append(ex)
+            self.bitmaps.append(bm)
+            self.launch[s.index] = (256, 4, 10)
+
+    @property
+    def total_rows(self) -> int:
+        return sum(s.n for s in self.specs)
+
+    def _apply(self, i: int) -> E.HipQueryPlan:
+        s = self.specs[i]
+        plan = self.plans[s.kind]
+        b, it, lb = self.launch[s.index]
+        plan.set_launch(phase_name(s.kind), b, it)
+        if s.kind == "agg":
+            self.execs[i].register_scalar("pipe_lds_bits", lb)
+        return plan
+
+    # --- autotuner (per pipeline shape) ----------------------------------
+    def autotune(self, blocks=(64, 128, 256, 512, 1024), items=(1, 2, 4, 8, 16), lds_bits=(8, 10, 12),
+                 trials=5) -> dict:
+        reps = {}
+        for i, s in enumerate(self.specs):
+            reps.setdefault(s.shape, i)
+        chosen = {}
+        for shape, i in sorted(reps.items()):
+            s = self.specs[i]
+            plan = self.plans[s.kind]
+            bl = (C.c_int * len(blocks))(*blocks)
+            it = (C.c_int * len(items))(*items)
+            scal = "pipe_lds_bits" if s.kind == "agg" else None
+            sv = (C.c_int64 * len(lds_bits))(*lds_bits) if scal else None
+            nsv = len(lds_bits) if scal else 0
+            ncap = len(blocks) * len(items) * max(1, nsv)
+            table = (C.c_double * (4 * ncap))()
+            bb, bi, bs, ms, nc = C.c_int(), C.c_int(), C.c_int64(), C.c_float(), C.c_int()
+            N.eng_check(_e().gdb_autotune_phase(self.execs[i].handle, plan.handle, phase_name(s.kind).encode(), bl,
+                                                len(blocks), it, len(items), scal.encode() if scal else None, sv, nsv,
+                                                trials, C.byref(bb), C.byref(bi), C.byref(bs), C.byref(ms), table,
+                                                C.byref(nc)))
+            chosen[shape] = {"block": bb.value, "items": bi.value, "lds_slots": (1 << bs.value) if scal else None,
+                             "lds_bits": int(bs.value) if scal else None, "p50_ms": ms.value,
+                             "candidates": nc.value, "tuned_on_rows": s.n}
+        for s in self.specs:
+            c = chosen[s.shape]
+            self.launch[s.index] = (c["block"], c["items"], c["lds_bits"] if c["lds_bits"] is not None else 10)
+        return chosen
+
+    # --- plain form: one query at a time ----------------------------------
+    def run_plain(self) -> float:
+        self.ctx.sync()
+        t0 = time.perf_counter()
+        for i in range(len(self.specs)):
+            self.execs[i].execute(self._apply(i), 0, 1)
+        return (time.perf_counter() - t0) * 1e3
+
+    def results(self) -> list[dict]:
+        out = []
+        for i, s in enumerate(self.specs):
+            res = self.execs[i].collect(self.plans[s.kind])
+            out.append({k: np.asarray(v) for k, v in res.columns.items()})
+        return out
+
+    # --- batched form ----------------------------------------------------
+    def build_batch(self, block: int, chunk_rows: int = 32768, grid: int = 0) -> "Batch":
+        return Batch(self, block, chunk_rows, grid)
+
+    def close(self):
+        for t in self.tables:
+            t.free(self.ctx)
+        for bm in self.bitmaps:
+            if bm:
+                self.ctx.free(bm)
+        for ex in self.execs:
+            ex.close()
+
+
+class Batch:
+    def __init__(self, w: Workload3, block: int, chunk_rows: int, grid: int):
+        self.w = w
+        self._h = C.c_void_p()
+        N.eng_check(_e().gdb_batch_create(w.ctx.handle, C.byref(self._h)))
+        for i, s in enumerate(w.specs):
+            plan = w._apply(i)
+            idx = C.c_int()
+            N.eng_check(_e().gdb_batch_add(self._h, plan.handle, w.execs[i].handle, w.launch[s.index][1],
+                                           C.byref(idx)))
+        g, nitems = C.c_uint(), C.c_uint64()
+        N.eng_check(_e().gdb_batch_build(self._h, block, chunk_rows, grid, C.byref(g), C.byref(nitems)))
+        self.grid, self.work_items, self.block = g.value, nitems.value, block
+
+    def source(self) -> str:
+        return N.two_call_string(_e().gdb_batch_source, self._h)
+
+    def capture(self) -> None:
+        N.eng_check(_e().gdb_batch_capture(self._h))
+
+    def run(self, graph: bool = True) -> float:
+        ms = C.c_float()
+        N.eng_check(_e().gdb_batch_run(self._h, 1 if graph else 0, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if self._h:
+            _e().gdb_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -194,6 +194,21 @@ std::string HipCodegen::paramList(const PhaseInfo& p, bool withTypes) {
     return os.str();
 }
 
+std::string HipCodegen::printHelpers() const {
+    std::ostringstream out;
+    for (const auto& h : helpers_) out << h << "\n";
+    return out.str();
+}
+
+std::string HipCodegen::printPhaseBody(int phase, const std::string& bodyName) const {
+    const auto& p = phases_.at(static_cast<size_t>(phase));
+    std::ostringstream out;
+    out << "template <int ITEMS>\n__device__ __forceinline__ void " << bodyName << "(" << paramList(p, true);
+    if (rangeMode_) out << (p.bindings.empty() ? "" : ", ") << "const u64 _rb, const u64 _re";
+    out << ") {\n" << p.code << "}\n";
+    return out.str();
+}
+
 std::string HipCodegen::print() const {
     std::ostringstream out;
     out << kGdbDeviceHeader;

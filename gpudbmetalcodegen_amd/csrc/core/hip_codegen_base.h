@@ -110,6 +110,16 @@ public:
     // Fresh identifier with the given prefix (unique within this codegen).
     std::string uniqueName(const std::string& prefix);
 
+    // Range mode: scans cover rows [_rb, _re) with one block (a work item of
+    // a persistent batched kernel) instead of the whole table with the grid.
+    // The phase body then takes (..., const u64 _rb, const u64 _re).
+    void setRangeMode(bool on) { rangeMode_ = on; }
+    bool rangeMode() const { return rangeMode_; }
+    // The phase body alone (template <int ITEMS> __device__ function named
+    // <bodyName>), without helpers, runtime header or kernel wrappers.
+    std::string printPhaseBody(int phase, const std::string& bodyName) const;
+    std::string printHelpers() const;
+
     // Size expressions of runner-owned buffers (first registration wins).
     const std::unordered_map<std::string, std::string>& getGlobalBufferSizes() const { return bufferSizes_; }
 
@@ -152,6 +162,7 @@ private:
     std::unordered_map<std::string, int> nameCounters_;
     HipResultSchema resultSchema_;
     unsigned indentLevel_ = 1;
+    bool rangeMode_ = false;
 
     PhaseInfo& cur(const char* op);
     void pushBinding(const char* op, HipParamBinding b);

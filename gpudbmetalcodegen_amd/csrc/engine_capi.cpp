@@ -7,6 +7,8 @@
 #include "execution/runtime_compiler.h"
 #include "infra/generators.h"
 #include "planning/hip_plan_builder.h"
+#include "runtime/autotuner.h"
+#include "runtime/batcher.h"
 #include "runtime/estimator.h"
 
 #include <algorithm>
@@ -495,6 +497,95 @@ gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* nee
 
 gdb_status gdb_result_free(gdb_result r) {
     delete r;
+    return GDB_OK;
+}
+
+// ---- autotuner ----
+gdb_status gdb_autotune_phase(gdb_executor ex, gdb_plan p, const char* phase, const int* blocks, int nblocks,
+                              const int* items, int nitems, const char* scalar_name, const int64_t* scalar_values,
+                              int nscalars, int trials, int* best_block, int* best_items, int64_t* best_scalar,
+                              float* best_ms, double* table, int* ncand) {
+    return guard([&]() -> gdb_status {
+        if (!p->compiled || p->ctx != ex->ctx) return setErr(GDB_ERR_INVALID, "gdb_autotune_phase: plan not compiled");
+        int idx = -1;
+        const auto& phs = p->cg.getPhases();
+        for (size_t i = 0; i < phs.size(); ++i)
+            if (phs[i].name == phase) idx = static_cast<int>(i);
+        if (idx < 0) return setErr(GDB_ERR_NOT_FOUND, std::string("gdb_autotune_phase: no phase ") + phase);
+        registerPlanTables(ex, p);
+        ex->ex->clearPhaseHooks();
+        auto r = runtime::autotunePhase(*ex->ex, compilerFor(ex->ctx), p->cq, p->cg, idx,
+                                        std::vector<int>(blocks, blocks + nblocks), std::vector<int>(items, items + nitems),
+                                        scalar_name ? scalar_name : "",
+                                        std::vector<int64_t>(scalar_values, scalar_values + (scalar_values ? nscalars : 0)),
+                                        trials);
+        if (r.block == 0) return setErr(GDB_ERR_INVALID, "gdb_autotune_phase: no candidate ran");
+        for (auto& pp : p->plan.phases)
+            if (pp.name == phase) {
+                pp.threadgroupSize = r.block;
+                pp.items = r.items;
+            }
+        if (best_block) *best_block = r.block;
+        if (best_items) *best_items = r.items;
+        if (best_scalar) *best_scalar = r.scalar;
+        if (best_ms) *best_ms = r.ms;
+        if (table)
+            for (size_t i = 0; i < r.table.size(); ++i)
+                for (int k = 0; k < 4; ++k) table[i * 4 + static_cast<size_t>(k)] = r.table[i][static_cast<size_t>(k)];
+        if (ncand) *ncand = static_cast<int>(r.table.size());
+        return GDB_OK;
+    });
+}
+
+// ---- batch ----
+gdb_status gdb_batch_create(gdb_ctx ctx, gdb_batch* out) {
+    return guard([&]() -> gdb_status {
+        *out = reinterpret_cast<gdb_batch>(new runtime::PipelineBatch(ctx));
+        return GDB_OK;
+    });
+}
+
+static runtime::PipelineBatch& bat(gdb_batch b) { return *reinterpret_cast<runtime::PipelineBatch*>(b); }
+
+gdb_status gdb_batch_add(gdb_batch b, gdb_plan p, gdb_executor ex, int items, int* index) {
+    return guard([&]() -> gdb_status {
+        const int i = bat(b).add(&p->plan, &p->cg, ex->ex.get(), items);
+        if (index) *index = i;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_batch_build(gdb_batch b, int block, uint64_t chunk_rows, int grid, unsigned* grid_out,
+                           uint64_t* work_items) {
+    return guard([&]() -> gdb_status {
+        bat(b).build(block, chunk_rows, grid);
+        if (grid_out) *grid_out = bat(b).grid();
+        if (work_items) *work_items = bat(b).workItems();
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_batch_source(gdb_batch b, char* buf, size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status { return copyOut(bat(b).source(), buf, cap, need); });
+}
+
+gdb_status gdb_batch_capture(gdb_batch b) {
+    return guard([&]() -> gdb_status {
+        bat(b).capture();
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_batch_run(gdb_batch b, int use_graph, float* ms) {
+    return guard([&]() -> gdb_status {
+        const float t = bat(b).run(use_graph != 0);
+        if (ms) *ms = t;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_batch_destroy(gdb_batch b) {
+    delete reinterpret_cast<runtime::PipelineBatch*>(b);
     return GDB_OK;
 }
 

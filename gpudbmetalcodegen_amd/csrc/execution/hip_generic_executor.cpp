@@ -215,6 +215,55 @@ void HipGenericExecutor::launchPhase(RuntimeCompiler& rc, const RuntimeCompiler:
           "launch " + p.name);
 }
 
+std::vector<uint64_t> HipGenericExecutor::packPhaseArgs(const HipCodegen& cg, int phase) {
+    const auto& p = cg.getPhases().at(static_cast<size_t>(phase));
+    resolveBufferTables();
+    std::vector<uint64_t> args;
+    for (const auto& b : p.bindings) {
+        switch (b.kind) {
+            case HipParamKind::TableColumn: {
+                auto t = tables_.find(b.tableName);
+                if (t == tables_.end()) throw std::runtime_error("executor: table '" + b.tableName + "' not registered");
+                auto c = t->second.columns.find(b.columnName);
+                if (c == t->second.columns.end())
+                    throw std::runtime_error("executor: column '" + b.columnName + "' not registered");
+                args.push_back(reinterpret_cast<uint64_t>(c->second));
+                break;
+            }
+            case HipParamKind::TableSize: {
+                auto t = tables_.find(b.tableName);
+                if (t == tables_.end()) throw std::runtime_error("executor: table '" + b.tableName + "' not registered");
+                args.push_back(t->second.rows);
+                break;
+            }
+            case HipParamKind::DeviceBuffer: args.push_back(reinterpret_cast<uint64_t>(ensureBuffer(b).ptr)); break;
+            case HipParamKind::ScalarInt: {
+                auto s = scalars_.find(b.name);
+                if (s == scalars_.end()) throw std::runtime_error("executor: scalar '" + b.name + "' not registered");
+                args.push_back(static_cast<uint64_t>(s->second));
+                break;
+            }
+            case HipParamKind::SymbolValue: args.push_back(sizes_.resolve(b.sizeExpr)); break;
+        }
+    }
+    return args;
+}
+
+void HipGenericExecutor::enqueueInits(const HipCodegen& cg) {
+    for (const auto& kv : initSpecs(cg)) {
+        const HipParamBinding* ib = kv.second;
+        Buffer& buf = ensureBuffer(*ib);
+        const size_t bytes = static_cast<size_t>(sizes_.resolve(ib->sizeExpr)) * ib->elemBytes;
+        if (ib->init == InitKind::Zero) {
+            const uint8_t z = 0;
+            check(gdb_fill_async(buf.ptr, bytes, &z, 1, stream_), "fill " + ib->name);
+        } else {
+            check(gdb_fill_async(buf.ptr, bytes, ib->pattern.data(), static_cast<int>(ib->pattern.size()), stream_),
+                  "fill " + ib->name);
+        }
+    }
+}
+
 void HipGenericExecutor::enqueue(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq, const HipCodegen& cg,
                                  int firstPhase, int lastPhase) {
     const auto& phases = cg.getPhases();

@@ -81,6 +81,12 @@ public:
     // Warm up, then time `measured` runs (results of the last run remain).
     HipExecutionResult execute(RuntimeCompiler& rc, const RuntimeCompiler::CompiledQuery& cq, const HipCodegen& cg,
                                int warmupRuns = 0, int measuredRuns = 1, int firstPhase = 0, int lastPhase = -1);
+    // Pieces of a launch, for callers that fuse phases of several plans into
+    // one kernel: allocate the phase's buffers and return its packed
+    // arguments (8 bytes each, binding order) without launching; and enqueue
+    // the initialisation fills of every buffer of the plan.
+    std::vector<uint64_t> packPhaseArgs(const HipCodegen& cg, int phase);
+    void enqueueInits(const HipCodegen& cg);
     // Read the status words of the plan's phases; throws GdbOverflowError.
     std::vector<std::array<uint64_t, 4>> checkStatus(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
 

@@ -51,13 +51,17 @@ void HipGridStrideScan::produce(HipCodegen& cg, ConsumerFn consume) {
     cg.setPhaseScannedTable(table_);
     std::vector<std::string> ptr;
     for (const auto& c : columns_) ptr.push_back(cg.addColumnParam(table_, c.name, c.type));
-    const std::string n = cg.addTableSizeParam(table_);
+    const std::string nParam = cg.addTableSizeParam(table_);
+    const bool range = cg.rangeMode();
+    // Grid mode: the grid strides over the whole table. Range mode: this
+    // block walks the work item [_rb, _re) (_rb is a multiple of the tile).
+    const std::string n = range ? "_re" : nParam;
     const int lb = cg.getPhases().back().launchBound;
 
     cg.addComment("scan " + table_ + ": tiles of blockDim.x*ITEMS rows, ITEMS consecutive rows per thread");
     cg.addBlock("", [&]() {
         cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
-        cg.addLine("const u64 _step = (u64)gridDim.x * _tile;");
+        cg.addLine(range ? "const u64 _step = _tile;" : "const u64 _step = (u64)gridDim.x * _tile;");
         for (size_t k = 0; k < columns_.size(); ++k) {
             const auto& c = columns_[k];
             if (c.type.type == ColType::CHAR) {
@@ -65,7 +69,8 @@ void HipGridStrideScan::produce(HipCodegen& cg, ConsumerFn consume) {
                            s(c.type.width) + "];");
             }
         }
-        cg.addBlock("for (u64 _t0 = (u64)blockIdx.x * _tile; _t0 < " + n + "; _t0 += _step)", [&]() {
+        const std::string start = range ? "_rb" : "(u64)blockIdx.x * _tile";
+        cg.addBlock("for (u64 _t0 = " + start + "; _t0 < " + n + "; _t0 += _step)", [&]() {
             cg.addLine("const u64 _r0 = _t0 + (u64)threadIdx.x * ITEMS;");
             cg.addLine("const bool _full = _t0 + _tile <= " + n + ";");
             bool anyChar = false;
@@ -715,9 +720,18 @@ void HipHashGroupBy::produce(HipCodegen& cg, ConsumerFn consume) {
     if (variant_ == LDS) {
         int lbits = 0;
         while ((1 << lbits) < ldsSlots_) ++lbits;
-        const std::string LS = s(ldsSlots_);
-        cg.addComment("HashGroupBy " + T + ": LDS table of " + LS + " slots absorbs hot keys; spills go global");
-        cg.addLine("__shared__ u64 _gl_" + T + "[" + LS + " * 4];");
+        const std::string CAP = s(ldsSlots_);
+        std::string LS = CAP, LB = s(lbits) + "u";
+        cg.addComment("HashGroupBy " + T + ": LDS table of " + CAP + " slots absorbs hot keys; spills go global");
+        cg.addLine("__shared__ u64 _gl_" + T + "[" + CAP + " * 4];");
+        if (!ldsBitsScalar_.empty()) {
+            cg.addScalarParam(ldsBitsScalar_);
+            cg.addLine("const u32 _lb_" + T + " = (u32)(" + ldsBitsScalar_ + " < 1 ? 1 : (" + ldsBitsScalar_ + " > " +
+                       s(lbits) + " ? " + s(lbits) + " : " + ldsBitsScalar_ + "));");
+            cg.addLine("const int _ls_" + T + " = 1 << _lb_" + T + ";");
+            LS = "_ls_" + T;
+            LB = "_lb_" + T;
+        }
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
             cg.addLine("_gl_" + T + "[4 * _s] = GDB_EMPTY_WORD; _gl_" + T + "[4 * _s + 1] = 0;");
             cg.addLine("_gl_" + T + "[4 * _s + 2] = (u64)GDB_I64_MAX; _gl_" + T + "[4 * _s + 3] = (u64)GDB_I64_MIN;");
@@ -726,7 +740,7 @@ void HipHashGroupBy::produce(HipCodegen& cg, ConsumerFn consume) {
         child_->produce(cg, [&]() {
             cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
             cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
-            cg.addBlock("if (gdb_lds_gb_update(_gl_" + T + ", " + s(lbits) + "u, _gk, _gv, 16u) < 0)", [&]() {
+            cg.addBlock("if (gdb_lds_gb_update(_gl_" + T + ", " + LB + ", _gk, _gv, 16u) < 0)", [&]() {
                 cg.addLine("const int _r = " + merge1 + ";");
                 cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
                 cg.addLine("++_spill_" + T + ";");

@@ -264,6 +264,10 @@ public:
     enum Variant { GLOBAL = 0, LDS = 1 };
     HipHashGroupBy(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
                    const std::string& valueExpr, const std::string& slotsExpr, Variant variant, int ldsSlots);
+    // LDS variant: take the number of LDS slots in use (log2) from a scalar
+    // at run time, up to the compiled capacity ldsSlots (autotuned per
+    // pipeline without recompiling).
+    void setRuntimeLdsBits(const std::string& scalarName) { ldsBitsScalar_ = scalarName; }
     void produce(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
@@ -271,6 +275,7 @@ private:
     std::string table_, key_, value_, slots_;
     Variant variant_;
     int ldsSlots_;
+    std::string ldsBitsScalar_;
 };
 
 // Top-k by a 128-bit rank (hi, lo), larger first, rank (0,0) reserved.

@@ -35,6 +35,7 @@ const std::vector<CatalogTable>& catalog() {
         {"S", {{"s_fkey", I32}, {"s_text", CHAR25}}},
         {"G", {{"g_key", I32}, {"g_val", I64}}},
         {"GBBM", {{"gbbm_word", I32}}},
+        {"P", {{"p_key", I32}, {"p_date", I32}, {"p_val", I64}, {"p_flag", U8}}},
     };
     return tables;
 }
@@ -358,6 +359,46 @@ HipQueryPlan buildQ22A(const PlanOptions& o) {
     return plan;
 }
 
+// PIPE_AGG — one filter + hash group-by pipeline of the batched workload:
+// rows with p_date in [lo, hi) and p_flag == f, grouped by p_key with
+// sum/count/min/max(p_val). The LDS table's capacity is compiled
+// (lds_cap) and the slots in use come from the scalar pipe_lds_bits, so the
+// autotuner can vary it per pipeline without recompiling.
+HipQueryPlan buildPipeAgg(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "PIPE_AGG";
+    auto scan = makeScan("P", {"p_key", "p_date", "p_val", "p_flag"});
+    auto sel = std::make_unique<HipSelection>(
+        std::move(scan), "(i64)p_date >= pipe_date_lo && (i64)p_date < pipe_date_hi && (i64)p_flag == pipe_flag");
+    auto gb = std::make_unique<HipHashGroupBy>(std::move(sel), "d_pipe_ht", "p_key", "p_val", "pipe_slots",
+                                               HipHashGroupBy::LDS, optInt(o, "lds_cap", 2048));
+    gb->setRuntimeLdsBits("pipe_lds_bits");
+    auto& ph = appendPhase(plan, "PIPE_agg", std::move(gb), optInt(o, "block", 256), optInt(o, "items", 4),
+                           {1, 2, 4, 8, 16});
+    ph.launchBound = 1024;
+    ph.scalarParams = {"pipe_date_lo", "pipe_date_hi", "pipe_flag"};
+    plan.requiredScalars = {"pipe_date_lo", "pipe_date_hi", "pipe_flag", "pipe_lds_bits"};
+    plan.requiredSymbols = {"pipe_slots"};
+    return plan;
+}
+
+// PIPE_SEMI / PIPE_ANTI — semi-/anti-join of P against a prebuilt key bitmap
+// (registered buffer d_pipe_bitmap of pipe_bits bits): count and sum(p_val).
+HipQueryPlan buildPipeJoin(const PlanOptions& o, bool anti) {
+    HipQueryPlan plan;
+    plan.name = anti ? "PIPE_ANTI" : "PIPE_SEMI";
+    auto scan = makeScan("P", {"p_key", "p_val"});
+    auto probe = std::make_unique<HipBitmapProbe>(std::move(scan), "d_pipe_bitmap", "p_key", "pipe_bits", anti);
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(probe), anti ? "d_pipe_anti" : "d_pipe_semi", "0", 1);
+    agg->addAggregate("rows", AggKind::Count, "1", 0);
+    agg->addAggregate("sum_val", AggKind::SumI64, "p_val", 0);
+    auto& ph = appendPhase(plan, anti ? "PIPE_anti" : "PIPE_semi", std::move(agg), optInt(o, "block", 256),
+                           optInt(o, "items", 4), {1, 2, 4, 8, 16});
+    ph.launchBound = 1024;
+    plan.requiredSymbols = {"pipe_bits"};
+    return plan;
+}
+
 }  // namespace
 
 const ColumnType& CatalogTable::type(const std::string& col) const {
@@ -392,7 +433,8 @@ PlanOptions parsePlanOptions(const std::string& text) {
 }
 
 std::vector<std::string> knownPlanNames() {
-    return {"Q1", "Q6", "Q3", "Q3_SIZE", "GROUPBY", "GB_COUNT", "MAXKEY", "Q13L", "Q22A"};
+    return {"Q1", "Q6", "Q3", "Q3_SIZE", "GROUPBY", "GB_COUNT", "MAXKEY", "Q13L", "Q22A", "PIPE_AGG", "PIPE_SEMI",
+            "PIPE_ANTI"};
 }
 
 std::optional<HipQueryPlan> buildHipPlan(const std::string& name, const PlanOptions& opts) {
@@ -405,6 +447,9 @@ std::optional<HipQueryPlan> buildHipPlan(const std::string& name, const PlanOpti
     if (name == "MAXKEY") return buildMaxKey(opts);
     if (name == "Q13L") return buildQ13L(opts);
     if (name == "Q22A") return buildQ22A(opts);
+    if (name == "PIPE_AGG") return buildPipeAgg(opts);
+    if (name == "PIPE_SEMI") return buildPipeJoin(opts, false);
+    if (name == "PIPE_ANTI") return buildPipeJoin(opts, true);
     return std::nullopt;
 }
 
@@ -420,8 +465,9 @@ std::string HipQueryPlan::describe() const {
     return out;
 }
 
-HipCodegen generateFromPlan(const HipQueryPlan& plan) {
+HipCodegen generateFromPlan(const HipQueryPlan& plan, bool rangeMode) {
     HipCodegen cg;
+    cg.setRangeMode(rangeMode);
     for (const auto& h : plan.helpers) cg.addHelper(h);
     for (const auto& phase : plan.phases) {
         cg.beginPhase(phase.name);

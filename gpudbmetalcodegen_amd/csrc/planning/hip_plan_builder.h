@@ -97,8 +97,9 @@ std::vector<std::string> catalogTableNames();
 std::optional<HipQueryPlan> buildHipPlan(const std::string& name, const PlanOptions& opts = {});
 std::vector<std::string> knownPlanNames();
 
-// Run the emitters: helpers, then one phase per plan phase.
-HipCodegen generateFromPlan(const HipQueryPlan& plan);
+// Run the emitters: helpers, then one phase per plan phase. rangeMode emits
+// scans over a work item [_rb, _re) (for the persistent batched kernel).
+HipCodegen generateFromPlan(const HipQueryPlan& plan, bool rangeMode = false);
 
 }  // namespace codegen
 }  // namespace gdb

@@ -1,0 +1,260 @@
+#include "batcher.h"
+
+#include "../core/gdb_device_header.h"
+
+#include <algorithm>
+#include <memory>
+#include <set>
+#include <sstream>
+#include <stdexcept>
+#include <unordered_set>
+
+namespace gdb {
+namespace runtime {
+
+using namespace codegen;
+
+namespace {
+
+void check(gdb_status st, const std::string& what) {
+    if (st != GDB_OK) throw std::runtime_error(what + ": " + gdb_last_error());
+}
+
+std::string stripRestrict(std::string t) {
+    const std::string r = " __restrict__";
+    auto p = t.find(r);
+    if (p != std::string::npos) t.erase(p, r.size());
+    return t;
+}
+
+}  // namespace
+
+PipelineBatch::PipelineBatch(gdb_ctx ctx) : ctx_(ctx), stream_(gdb_default_stream(ctx)), rc_(ctx) {
+    check(gdb_event_create(&ev0_), "gdb_event_create");
+    check(gdb_event_create(&ev1_), "gdb_event_create");
+}
+
+PipelineBatch::~PipelineBatch() {
+    if (graph_) gdb_graph_destroy(graph_);
+    freeDevice();
+    gdb_event_destroy(ev0_);
+    gdb_event_destroy(ev1_);
+}
+
+void PipelineBatch::freeDevice() {
+    for (void** p : {&dArgs_, &dKindItems_, &dItemPipe_, &dItemBegin_, &dItemEnd_, &dTicket_}) {
+        if (*p) gdb_free(ctx_, *p);
+        *p = nullptr;
+    }
+}
+
+int PipelineBatch::add(const HipQueryPlan* plan, const HipCodegen* cg, HipGenericExecutor* ex, int items) {
+    if (plan->phases.size() != 1) throw std::runtime_error("PipelineBatch: pipelines must be single-phase plans");
+    const auto& ph = cg->getPhases()[0];
+    bool known = false;
+    for (int v : ph.itemsVariants) known |= (v == items);
+    if (!known) throw std::runtime_error("PipelineBatch: plan " + plan->name + " has no ITEMS=" + std::to_string(items));
+    const std::string key = plan->name + "\n" + ph.code;
+    int kind = -1;
+    for (size_t k = 0; k < kinds_.size(); ++k)
+        if (kinds_[k].key == key) kind = static_cast<int>(k);
+    if (kind < 0) {
+        Kind k;
+        k.key = key;
+        k.body = "k" + std::to_string(kinds_.size()) + "_item";
+        auto rcg = std::make_unique<HipCodegen>(generateFromPlan(*plan, true));
+        for (const auto& b : rcg->getPhases()[0].bindings) k.argTypes.push_back(stripRestrict(b.declType));
+        kindCg_.push_back(std::move(rcg));
+        kinds_.push_back(k);
+        kind = static_cast<int>(kinds_.size()) - 1;
+    }
+    auto& ki = kinds_[static_cast<size_t>(kind)].items;
+    if (std::find(ki.begin(), ki.end(), items) == ki.end()) ki.push_back(items);
+    uint64_t rows = 0;
+    const std::string tbl = ph.scannedTable;
+    if (!ex->tryGetSymbol(tableSizeName(tbl), rows))
+        throw std::runtime_error("PipelineBatch: table " + tbl + " of pipeline not registered");
+    pipes_.push_back({plan, cg, ex, items, kind, rows});
+    return static_cast<int>(pipes_.size()) - 1;
+}
+
+void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
+    if (pipes_.empty()) throw std::runtime_error("PipelineBatch: no pipelines");
+    if (block < 64 || block > 1024 || block % 64) throw std::runtime_error("PipelineBatch: bad block size");
+    block_ = block;
+    for (const auto& p : pipes_)
+        if (block > p.cg->getPhases()[0].launchBound)
+            throw std::runtime_error("PipelineBatch: block size above a pipeline's launch bound");
+    // --- source -------------------------------------------------------------
+    std::ostringstream src;
+    src << kGdbDeviceHeader << "\n// ===== helpers =====\n";
+    std::unordered_set<std::string> seen;
+    for (auto& kc : kindCg_) {
+        std::string h = kc->printHelpers();
+        if (seen.insert(h).second) src << h;
+    }
+    for (size_t k = 0; k < kinds_.size(); ++k)
+        src << "\n// ===== pipeline kind " << k << " =====\n" << kindCg_[k]->printPhaseBody(0, kinds_[k].body);
+    src << R"(
+template <typename T>
+GDB_DEV T gdb_arg(u64 w) {
+    if constexpr (__is_pointer(T)) return reinterpret_cast<T>(w);
+    else return (T)w;
+}
+)";
+    int lb = 1024;
+    for (auto& kc : kindCg_) lb = std::min(lb, kc->getPhases()[0].launchBound);
+    src << "extern \"C\" __global__ void __launch_bounds__(" << lb << ") gdb_batch_kernel(const u64* __restrict__ args, "
+        << "const u32 args_stride, const u32* __restrict__ kind_items, const u32* __restrict__ item_pipe, "
+        << "const u64* __restrict__ item_begin, const u64* __restrict__ item_end, const u32 n_items, "
+        << "u32* __restrict__ ticket) {\n"
+        << "    __shared__ u32 _it;\n"
+        << "    for (;;) {\n"
+        << "        __syncthreads();\n"
+        << "        if (threadIdx.x == 0) _it = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, "
+           "__HIP_MEMORY_SCOPE_AGENT);\n"
+        << "        __syncthreads();\n"
+        << "        const u32 it = _it;\n"
+        << "        if (it >= n_items) break;\n"
+        << "        const u32 p = item_pipe[it];\n"
+        << "        const u64* a = args + (u64)p * args_stride;\n"
+        << "        const u64 rb = item_begin[it], re = item_end[it];\n"
+        << "        switch (kind_items[p]) {\n";
+    for (size_t k = 0; k < kinds_.size(); ++k) {
+        for (int it : kinds_[k].items) {
+            src << "        case " << (k * 32 + static_cast<size_t>(it)) << ": " << kinds_[k].body << "<" << it << ">(";
+            for (size_t a = 0; a < kinds_[k].argTypes.size(); ++a)
+                src << (a ? ", " : "") << "gdb_arg<" << kinds_[k].argTypes[a] << ">(a[" << a << "])";
+            src << (kinds_[k].argTypes.empty() ? "" : ", ") << "rb, re); break;\n";
+        }
+    }
+    src << "        default: break;\n        }\n    }\n"
+        << "    if (threadIdx.x == 0) {\n"
+        << "        const u32 done = __hip_atomic_fetch_add(ticket + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
+        << "        if (done == gridDim.x - 1) { ticket[0] = 0; ticket[1] = 0; }\n"
+        << "    }\n}\n";
+    source_ = src.str();
+    cq_ = rc_.compile(source_);
+    kernel_ = rc_.getPipeline(cq_, "gdb_batch_kernel");
+    // --- arguments ------------------------------------------------------------
+    argStride_ = 0;
+    for (const auto& k : kinds_) argStride_ = std::max<uint32_t>(argStride_, static_cast<uint32_t>(k.argTypes.size()));
+    argStride_ = std::max<uint32_t>(argStride_, 1);
+    std::vector<uint64_t> args(static_cast<size_t>(argStride_) * pipes_.size(), 0);
+    std::vector<uint32_t> kindItems(pipes_.size());
+    for (size_t i = 0; i < pipes_.size(); ++i) {
+        auto& p = pipes_[i];
+        auto a = p.ex->packPhaseArgs(*kindCg_[static_cast<size_t>(p.kind)], 0);
+        std::copy(a.begin(), a.end(), args.begin() + static_cast<long>(i * argStride_));
+        kindItems[i] = static_cast<uint32_t>(p.kind * 32 + p.items);
+    }
+    // --- work items: whole tiles, about chunkRows rows each -------------------
+    itemPipe_.clear();
+    itemBegin_.clear();
+    itemEnd_.clear();
+    std::vector<size_t> order(pipes_.size());
+    for (size_t i = 0; i < order.size(); ++i) order[i] = i;
+    std::sort(order.begin(), order.end(), [&](size_t a, size_t b) { return pipes_[a].rows > pipes_[b].rows; });
+    for (size_t i : order) {
+        const auto& p = pipes_[i];
+        const uint64_t tile = static_cast<uint64_t>(block) * static_cast<uint64_t>(p.items);
+        const uint64_t chunk = tile * std::max<uint64_t>(1, (chunkRows + tile - 1) / tile);
+        for (uint64_t b = 0; b < p.rows; b += chunk) {
+            itemPipe_.push_back(static_cast<uint32_t>(i));
+            itemBegin_.push_back(b);
+            itemEnd_.push_back(std::min(p.rows, b + chunk));
+        }
+    }
+    freeDevice();
+    auto up = [&](void*& d, const void* h, size_t bytes) {
+        check(gdb_alloc(ctx_, std::max<size_t>(bytes, 16), &d), "gdb_alloc");
+        if (bytes) check(gdb_memcpy_h2d_async(d, h, bytes, stream_), "upload");
+    };
+    up(dArgs_, args.data(), args.size() * 8);
+    up(dKindItems_, kindItems.data(), kindItems.size() * 4);
+    up(dItemPipe_, itemPipe_.data(), itemPipe_.size() * 4);
+    up(dItemBegin_, itemBegin_.data(), itemBegin_.size() * 8);
+    up(dItemEnd_, itemEnd_.data(), itemEnd_.size() * 8);
+    const uint64_t zero = 0;
+    up(dTicket_, &zero, 8);
+    check(gdb_stream_sync(stream_), "sync");
+    int occ = 1;
+    if (gdb_kernel_occupancy(kernel_, block, 0, &occ) != GDB_OK || occ < 1) occ = 1;
+    gdb_device_info_t info;
+    check(gdb_device_info(ctx_, &info), "gdb_device_info");
+    const unsigned full = static_cast<unsigned>(info.cu_count * occ);
+    grid_ = grid > 0 ? static_cast<unsigned>(grid) : full;
+    grid_ = std::max(1u, std::min<unsigned>(grid_, static_cast<unsigned>(std::max<size_t>(1, itemPipe_.size()))));
+    if (graph_) {
+        gdb_graph_destroy(graph_);
+        graph_ = nullptr;
+    }
+}
+
+void PipelineBatch::enqueue() {
+    if (!kernel_) throw std::runtime_error("PipelineBatch: build() first");
+    for (auto& p : pipes_) p.ex->enqueueInits(*p.cg);
+    const uint8_t z = 0;
+    check(gdb_fill_async(dTicket_, 8, &z, 1, stream_), "fill ticket");
+    // Kernel arguments: pointers 8 bytes; the two u32 arguments are packed at
+    // their natural 4-byte alignment by the layout below.
+    std::vector<uint8_t> buf;
+    auto put = [&](const void* p, size_t n, size_t align) {
+        while (buf.size() % align) buf.push_back(0);
+        const uint8_t* b = static_cast<const uint8_t*>(p);
+        buf.insert(buf.end(), b, b + n);
+    };
+    const uint64_t pa = reinterpret_cast<uint64_t>(dArgs_);
+    const uint32_t st = argStride_;
+    const uint64_t pk = reinterpret_cast<uint64_t>(dKindItems_), pp = reinterpret_cast<uint64_t>(dItemPipe_);
+    const uint64_t pb = reinterpret_cast<uint64_t>(dItemBegin_), pe = reinterpret_cast<uint64_t>(dItemEnd_);
+    const uint32_t ni = static_cast<uint32_t>(itemPipe_.size());
+    const uint64_t pt = reinterpret_cast<uint64_t>(dTicket_);
+    put(&pa, 8, 8);
+    put(&st, 4, 4);
+    put(&pk, 8, 8);
+    put(&pp, 8, 8);
+    put(&pb, 8, 8);
+    put(&pe, 8, 8);
+    put(&ni, 4, 4);
+    put(&pt, 8, 8);
+    while (buf.size() % 8) buf.push_back(0);
+    check(gdb_launch(kernel_, grid_, static_cast<unsigned>(block_), 0, buf.data(), buf.size(), stream_),
+          "launch gdb_batch_kernel");
+}
+
+void PipelineBatch::capture() {
+    if (graph_) {
+        gdb_graph_destroy(graph_);
+        graph_ = nullptr;
+    }
+    check(gdb_graph_begin_capture(stream_), "begin capture");
+    try {
+        enqueue();
+    } catch (...) {
+        gdb_graph g = nullptr;
+        gdb_graph_end_capture(stream_, &g);
+        if (g) gdb_graph_destroy(g);
+        throw;
+    }
+    check(gdb_graph_end_capture(stream_, &graph_), "end capture");
+}
+
+float PipelineBatch::run(bool useGraph) {
+    check(gdb_event_record(ev0_, stream_), "event");
+    if (useGraph) {
+        if (!graph_) capture();
+        check(gdb_graph_launch(graph_, stream_), "graph launch");
+    } else {
+        enqueue();
+    }
+    check(gdb_event_record(ev1_, stream_), "event");
+    check(gdb_event_sync(ev1_), "event sync");
+    float ms = 0.0f;
+    check(gdb_event_elapsed_ms(ev0_, ev1_, &ms), "elapsed");
+    for (auto& p : pipes_) p.ex->checkStatus(*p.cg);
+    return ms;
+}
+
+}  // namespace runtime
+}  // namespace gdb

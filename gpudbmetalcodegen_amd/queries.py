@@ -9,6 +9,7 @@ generalised to hash tables.
 
 from __future__ import annotations
 
+import threading
 from dataclasses import dataclass
 
 import numpy as np
@@ -35,14 +36,18 @@ class QueryRun:
 
 
 _plan_cache: dict = {}
+_plan_lock = threading.Lock()
 
 
 def get_plan(ctx: E.Context, name: str, options: str = "") -> E.HipQueryPlan:
+    """Built and compiled plans, one per (context, name, options); shared by
+    every runner and thread (plans are read-only once compiled)."""
     key = (id(ctx), name, options)
-    p = _plan_cache.get(key)
-    if p is None:
-        p = E.build_hip_plan(name, options).compile(ctx)
-        _plan_cache[key] = p
+    with _plan_lock:
+        p = _plan_cache.get(key)
+        if p is None:
+            p = E.build_hip_plan(name, options).compile(ctx)
+            _plan_cache[key] = p
     return p
 
 

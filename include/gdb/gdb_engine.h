@@ -124,6 +124,35 @@ gdb_status gdb_result_str(gdb_result r, int idx, char* out, size_t width);
 gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* need);
 gdb_status gdb_result_free(gdb_result r);
 
+/* ---- Autotuner: sweep block x ITEMS x (optional) runtime scalar for one
+ * phase; 1 untimed + `trials` timed runs per candidate, slowest dropped, p50
+ * wins (the reference's --autotune-tg loop, codegen/codegen_main.cpp:837-929).
+ * The plan's phase is left at the winner; `table` (may be NULL) receives
+ * ncand rows of (block, items, scalar, p50 ms). */
+gdb_status gdb_autotune_phase(gdb_executor ex, gdb_plan p, const char* phase, const int* blocks, int nblocks,
+                              const int* items, int nitems, const char* scalar_name, const int64_t* scalar_values,
+                              int nscalars, int trials, int* best_block, int* best_items, int64_t* best_scalar,
+                              float* best_ms, double* table, int* ncand);
+
+/* ---- Batched persistent launch of many pipelines ----
+ * No counterpart in the reference, which runs one command buffer per phase
+ * (codegen/execution/metal_generic_executor.cpp:338-390); callers are the
+ * batched workload and the request loop. Each pipeline is a single-phase
+ * plan with its own runner holding its registrations; build() emits one
+ * persistent kernel over all of them (work items from an atomic ticket),
+ * capture() records fills + kernel in a graph, run() launches (graph replay
+ * when use_graph) and checks every pipeline's status. Results are collected
+ * per pipeline with gdb_executor_collect on that pipeline's runner. */
+typedef struct gdb_batch_s* gdb_batch;
+gdb_status gdb_batch_create(gdb_ctx ctx, gdb_batch* out);
+gdb_status gdb_batch_add(gdb_batch b, gdb_plan p, gdb_executor ex, int items, int* index);
+gdb_status gdb_batch_build(gdb_batch b, int block, uint64_t chunk_rows, int grid, unsigned* grid_out,
+                           uint64_t* work_items);
+gdb_status gdb_batch_source(gdb_batch b, char* buf, size_t cap, size_t* need);
+gdb_status gdb_batch_capture(gdb_batch b);
+gdb_status gdb_batch_run(gdb_batch b, int use_graph, float* ms);
+gdb_status gdb_batch_destroy(gdb_batch b);
+
 /* ---- Learned group-count estimator ----
  * No counterpart in the reference, which sizes tables only by an exact
  * max-key pre-pass (codegen/execution/max_key_symbols.cpp:188-235); callers
