@@ -237,21 +237,23 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
             return EST.size_for(est, x, ub)
 
         sz = learned_step()
-        r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots)
+        r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots, collect=False)
         for _ in range(warmup):
             sz = learned_step()
             r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
-                               executor=r2.executor)
+                               executor=r2.executor, collect=False)
         ctx.sync()
+        # timed like the plain form: sizing + kernels, results left in HBM
         t0 = time.perf_counter()
         resizes = 0
         for _ in range(steps):
             sz = learned_step()
             r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
-                               executor=r2.executor)
+                               executor=r2.executor, collect=False)
             resizes += r2.resizes
         learned_ms = (time.perf_counter() - t0) * 1e3 / steps
-        same = all(np.array_equal(plain_res.columns[c], r2.result.columns[c]) for c in plain_res.columns)
+        learned_res = r2.executor.collect(r2.plan)
+        same = all(np.array_equal(plain_res.columns[c], learned_res.columns[c]) for c in plain_res.columns)
         learned = {"ms_per_step": learned_ms, "rows_per_s": n / (learned_ms / 1e3), "slots": r2.slots,
                    "variant": sz.variant, "predicted_groups": round(sz.groups_pred), "true_groups": groups,
                    "resizes": resizes, "identical_to_plain": bool(same),

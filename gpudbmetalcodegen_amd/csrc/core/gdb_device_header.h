@@ -120,6 +120,8 @@ GDB_DEV i64 gdb_wave_max_i64(i64 v) {
     return gdb_max64(gdb_max64((i64)gdb_readlane64((u64)v, 0), (i64)gdb_readlane64((u64)v, 16)),
                      gdb_max64((i64)gdb_readlane64((u64)v, 32), (i64)gdb_readlane64((u64)v, 48)));
 }
+__host__ __device__ constexpr u32 gdb_lds_align16(u32 x) { return (x + 15u) & ~15u; }
+__host__ __device__ constexpr u32 gdb_cmax(u32 a, u32 b) { return a > b ? a : b; }
 GDB_DEV int gdb_lane() { return (int)(threadIdx.x & 63u); }
 GDB_DEV int gdb_wave() { return (int)(threadIdx.x >> 6); }
 
@@ -182,8 +184,14 @@ GDB_DEV u64 gdb_lds_load_u64(const u64* p) {
 GDB_DEV bool gdb_bitmap_test(const u32* __restrict__ bm, u64 nbits, i64 key) {
     return (u64)key < nbits && ((bm[(u64)key >> 5] >> ((u32)key & 31u)) & 1u);
 }
+// Bits only go from 0 to 1 during a build, so a plain read that already sees
+// the bit skips the atomic (hot keys of a skewed column would otherwise
+// serialise on one word); a stale read only costs a redundant atomic.
 GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
-    if ((u64)key < nbits) gdb_atomic_or_u32(bm + ((u64)key >> 5), 1u << ((u32)key & 31u));
+    if ((u64)key >= nbits) return;
+    u32* w = bm + ((u64)key >> 5);
+    const u32 m = 1u << ((u32)key & 31u);
+    if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(w, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -356,47 +364,69 @@ GDB_DEV void gdb_topk_push(GdbTopK<K>& t, u64 h, u64 l) {
         }
     }
 }
-// Merge every thread's list into thread 0's list. lds must hold
-// blockDim.x * K * 2 words.
+GDB_DEV u64 gdb_shfl_xor_u64(u64 v, int mask) {
+    const int a = (gdb_lane() ^ mask) << 2;
+    const u32 lo = (u32)__builtin_amdgcn_ds_bpermute(a, (int)(u32)v);
+    const u32 hi = (u32)__builtin_amdgcn_ds_bpermute(a, (int)(u32)(v >> 32));
+    return ((u64)hi << 32) | (u64)lo;
+}
+template <int K>
+GDB_DEV void gdb_topk_pop(GdbTopK<K>& t) {
+#pragma unroll
+    for (int j = 0; j < K - 1; ++j) {
+        t.h[j] = t.h[j + 1];
+        t.l[j] = t.l[j + 1];
+    }
+    t.h[K - 1] = 0;
+    t.l[K - 1] = 0;
+}
+// Words of LDS gdb_topk_block_merge needs (blocks of up to 1024 threads).
+#define GDB_TOPK_LDS_WORDS(K) (64 + 2 * (K))
+// Block-wide top K over every thread's sorted list: K rounds of a block
+// argmax over the list heads (wave butterfly, then the per-wave maxima in
+// LDS); the thread that owns a round's maximum pops it. Entries are unique
+// (the low word carries the group key), so exactly one thread pops. On
+// return every thread's list holds the block's top K, and res (LDS, 2K
+// words) holds it as (h, l) pairs. All threads of the block must call it.
 template <int K>
 GDB_DEV void gdb_topk_block_merge(GdbTopK<K>& t, u64* lds) {
-    const int tid = (int)threadIdx.x;
+    const int nw = (int)(blockDim.x >> 6);
+    u64* res = lds + 64;
+    for (int r = 0; r < K; ++r) {
+        u64 h = t.h[0], l = t.l[0];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) {
+            const u64 oh = gdb_shfl_xor_u64(h, m), ol = gdb_shfl_xor_u64(l, m);
+            if (gdb_rank_gt(oh, ol, h, l)) {
+                h = oh;
+                l = ol;
+            }
+        }
+        u64* buf = lds + (r & 1) * 32;  // double-buffered: one barrier per round
+        if (gdb_lane() == 0) {
+            buf[2 * gdb_wave()] = h;
+            buf[2 * gdb_wave() + 1] = l;
+        }
+        __syncthreads();
+        u64 bh = buf[0], bl = buf[1];
+        for (int w = 1; w < nw; ++w) {
+            const u64 xh = buf[2 * w], xl = buf[2 * w + 1];
+            if (gdb_rank_gt(xh, xl, bh, bl)) {
+                bh = xh;
+                bl = xl;
+            }
+        }
+        if ((bh | bl) != 0 && t.h[0] == bh && t.l[0] == bl) gdb_topk_pop<K>(t);
+        if (threadIdx.x == 0) {
+            res[2 * r] = bh;
+            res[2 * r + 1] = bl;
+        }
+    }
+    __syncthreads();
 #pragma unroll
     for (int j = 0; j < K; ++j) {
-        lds[(tid * K + j) * 2] = t.h[j];
-        lds[(tid * K + j) * 2 + 1] = t.l[j];
-    }
-    __syncthreads();
-    if (gdb_lane() == 0) {
-        const int base = gdb_wave() * 64;
-        for (int i = 1; i < 64 && base + i < (int)blockDim.x; ++i) {
-            for (int j = 0; j < K; ++j) {
-                const u64 h = lds[((base + i) * K + j) * 2];
-                const u64 l = lds[((base + i) * K + j) * 2 + 1];
-                if (h == 0 && l == 0) break;
-                gdb_topk_push<K>(t, h, l);
-            }
-        }
-    }
-    __syncthreads();
-    if (gdb_lane() == 0) {
-#pragma unroll
-        for (int j = 0; j < K; ++j) {
-            lds[(gdb_wave() * K + j) * 2] = t.h[j];
-            lds[(gdb_wave() * K + j) * 2 + 1] = t.l[j];
-        }
-    }
-    __syncthreads();
-    if (tid == 0) {
-        const int nw = (int)(blockDim.x >> 6);
-        for (int w = 1; w < nw; ++w) {
-            for (int j = 0; j < K; ++j) {
-                const u64 h = lds[(w * K + j) * 2];
-                const u64 l = lds[(w * K + j) * 2 + 1];
-                if (h == 0 && l == 0) break;
-                gdb_topk_push<K>(t, h, l);
-            }
-        }
+        t.h[j] = res[2 * j];
+        t.l[j] = res[2 * j + 1];
     }
     __syncthreads();
 }

@@ -49,6 +49,9 @@ void HipCodegen::beginPhase(const std::string& phaseName) {
     phases_.push_back(p);
     current_ = static_cast<int>(phases_.size()) - 1;
     indentLevel_ = 1;
+    staged_ = false;
+    nextConsumerStageable_ = false;
+    rowBindings_.clear();
 }
 
 void HipCodegen::endPhase() {
@@ -200,11 +203,44 @@ std::string HipCodegen::printHelpers() const {
     return out.str();
 }
 
+void HipCodegen::setPhaseStaged(bool staged) { cur("setPhaseStaged").staged = staged; }
+
+bool HipCodegen::phaseStaged() const { return !phases_.empty() && current_ >= 0 && phases_.back().staged; }
+
+void HipCodegen::rowLoop(const std::function<void()>& body, const std::string& mask) {
+    const bool was = staged_;
+    addLine("#pragma unroll");
+    addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
+        addBlock("if (" + mask + "[_u])", [&]() {
+            const std::vector<std::string> binds = rowBindings_;
+            for (const auto& b : binds) addLine(b);
+            staged_ = false;
+            body();
+            staged_ = was;
+        });
+    });
+}
+
+void HipCodegen::declareLds(const std::string& type, const std::string& name, const std::string& countExpr) {
+    auto& p = cur("declareLds");
+    if (!rangeMode_) {
+        addLine("__shared__ __attribute__((aligned(16))) " + type + " " + name + "[" + countExpr + "];");
+        return;
+    }
+    const std::string off = "gdb_lds_align16(" + p.ldsBytesExpr + ")";
+    addLine(type + "* const " + name + " = reinterpret_cast<" + type + "*>(_lds + " + off + ");");
+    p.ldsBytesExpr = "(" + off + " + (u32)sizeof(" + type + ") * (u32)(" + countExpr + "))";
+}
+
 std::string HipCodegen::printPhaseBody(int phase, const std::string& bodyName) const {
     const auto& p = phases_.at(static_cast<size_t>(phase));
     std::ostringstream out;
+    if (rangeMode_)
+        out << "template <int ITEMS>\n__host__ __device__ constexpr u32 " << bodyName << "_lds_bytes() { return "
+            << p.ldsBytesExpr << "; }\n";
     out << "template <int ITEMS>\n__device__ __forceinline__ void " << bodyName << "(" << paramList(p, true);
-    if (rangeMode_) out << (p.bindings.empty() ? "" : ", ") << "const u64 _rb, const u64 _re";
+    if (rangeMode_)
+        out << (p.bindings.empty() ? "" : ", ") << "const u64 _rb, const u64 _re, unsigned char* __restrict__ _lds";
     out << ") {\n" << p.code << "}\n";
     return out.str();
 }

@@ -107,6 +107,31 @@ public:
     void addBitmapWriteParam(const std::string& name, const std::string& bitsExpr);
     void addBitmapReadParam(const std::string& name, const std::string& bitsExpr);
 
+    // Block-shared (LDS) array `name` of countExpr elements of `type`
+    // (countExpr is a constant expression and may use ITEMS). Grid mode: a
+    // __shared__ array. Range mode: a 16-byte aligned slice of the batched
+    // kernel's single LDS buffer (_lds), so the LDS of the pipeline bodies of
+    // one kernel overlap instead of adding up.
+    void declareLds(const std::string& type, const std::string& name, const std::string& countExpr);
+
+    // --- staged (item-batched) emission; see hip_operators.h ---
+    void setPhaseStaged(bool staged);
+    bool phaseStaged() const;
+    // True while emitting for a batch of ITEMS rows with active mask _act.
+    bool staged() const { return staged_; }
+    void setStaged(bool on) { staged_ = on; }
+    // Lines that bind a row's locals for item _u (columns, computed values,
+    // probe results); emitted at the top of every loop over the items.
+    void addRowBinding(const std::string& line) { rowBindings_.push_back(line); }
+    std::vector<std::string>& rowBindings() { return rowBindings_; }
+    // One unrolled loop over the active items with the row bindings in
+    // scope; body() runs in per-row context.
+    void rowLoop(const std::function<void()>& body, const std::string& mask = "_act");
+    // Whether the operator about to be produced consumes staged batches (set
+    // by the parent operator before it produces its child).
+    bool nextConsumerStageable() const { return nextConsumerStageable_; }
+    void setNextConsumerStageable(bool v) { nextConsumerStageable_ = v; }
+
     // Fresh identifier with the given prefix (unique within this codegen).
     std::string uniqueName(const std::string& prefix);
 
@@ -116,7 +141,9 @@ public:
     void setRangeMode(bool on) { rangeMode_ = on; }
     bool rangeMode() const { return rangeMode_; }
     // The phase body alone (template <int ITEMS> __device__ function named
-    // <bodyName>), without helpers, runtime header or kernel wrappers.
+    // <bodyName>), without helpers, runtime header or kernel wrappers. In
+    // range mode the body also takes (unsigned char* _lds) last, and a
+    // constexpr <bodyName>_lds_bytes<ITEMS>() gives the LDS bytes it uses.
     std::string printPhaseBody(int phase, const std::string& bodyName) const;
     std::string printHelpers() const;
 
@@ -142,6 +169,8 @@ public:
         std::vector<int> itemsVariants{4};
         int launchBound = 1024;
         std::string statusBuffer;  // optional overflow/status words checked after the phase
+        std::string ldsBytesExpr = "0u";  // range mode: LDS bytes of the body (constant expression)
+        bool staged = false;              // item-batched emission (hip_operators.h)
         std::vector<HipParamBinding> bindings;
     };
     const std::vector<PhaseInfo>& getPhases() const { return phases_; }
@@ -163,6 +192,9 @@ private:
     HipResultSchema resultSchema_;
     unsigned indentLevel_ = 1;
     bool rangeMode_ = false;
+    bool staged_ = false;
+    bool nextConsumerStageable_ = false;
+    std::vector<std::string> rowBindings_;
 
     PhaseInfo& cur(const char* op);
     void pushBinding(const char* op, HipParamBinding b);

@@ -569,6 +569,16 @@ gdb_status gdb_batch_source(gdb_batch b, char* buf, size_t cap, size_t* need) {
     return guard([&]() -> gdb_status { return copyOut(bat(b).source(), buf, cap, need); });
 }
 
+gdb_status gdb_batch_kernel_source(const gdb_plan* plans, const int* items, int n, char* buf, size_t cap,
+                                   size_t* need) {
+    return guard([&]() -> gdb_status {
+        if (n < 1 || !plans || !items) return setErr(GDB_ERR_INVALID, "gdb_batch_kernel_source: no pipelines");
+        runtime::BatchKernelSource bs;
+        for (int i = 0; i < n; ++i) bs.add(&plans[i]->plan, &plans[i]->cg, items[i]);
+        return copyOut(bs.print(), buf, cap, need);
+    });
+}
+
 gdb_status gdb_batch_capture(gdb_batch b) {
     return guard([&]() -> gdb_status {
         bat(b).capture();

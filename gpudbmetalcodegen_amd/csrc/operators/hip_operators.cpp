@@ -23,6 +23,17 @@ std::string s(long long v) { return std::to_string(v); }
 
 }  // namespace
 
+void HipOperator::produce(HipCodegen& cg, ConsumerFn consume) {
+    const bool consumerStageable = cg.nextConsumerStageable();
+    cg.setNextConsumerStageable(stageable());
+    produceImpl(cg, [&cg, consume, consumerStageable]() {
+        if (cg.staged() && !consumerStageable)
+            cg.rowLoop(consume);
+        else
+            consume();
+    });
+}
+
 void HipOperator::describeTree(std::string& out, int depth) const {
     out += std::string(depth * 2, ' ') + describe() + "\n";
 }
@@ -45,7 +56,7 @@ void HipGridStrideScan::addColumn(const std::string& column, ColumnType type) {
     columns_.push_back({column, type});
 }
 
-void HipGridStrideScan::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     if (columns_.empty())
         throw std::runtime_error("HipGridStrideScan(" + table_ + "): no columns registered; call addColumn() first");
     cg.setPhaseScannedTable(table_);
@@ -65,8 +76,7 @@ void HipGridStrideScan::produce(HipCodegen& cg, ConsumerFn consume) {
         for (size_t k = 0; k < columns_.size(); ++k) {
             const auto& c = columns_[k];
             if (c.type.type == ColType::CHAR) {
-                cg.addLine("__shared__ __attribute__((aligned(16))) char _s_" + c.name + "[" + s(lb) + " * ITEMS * " +
-                           s(c.type.width) + "];");
+                cg.declareLds("char", "_s_" + c.name, s(lb) + " * ITEMS * " + s(c.type.width));
             }
         }
         const std::string start = range ? "_rb" : "(u64)blockIdx.x * _tile";
@@ -119,6 +129,26 @@ void HipGridStrideScan::produce(HipCodegen& cg, ConsumerFn consume) {
                 }
             }
             cg.addLine("}");
+            if (cg.phaseStaged()) {
+                cg.addLine("bool _act[ITEMS];");
+                cg.addLine("#pragma unroll");
+                cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) _act[_u] = _full || _r0 + (u64)_u < " + n + ";");
+                const size_t mark = cg.rowBindings().size();
+                cg.addRowBinding("const u64 " + idxVar_ + " = _r0 + (u64)_u;");
+                for (const auto& c : columns_) {
+                    if (c.type.type == ColType::CHAR)
+                        cg.addRowBinding("const char* " + c.name + " = _s_" + c.name +
+                                         " + ((u64)threadIdx.x * ITEMS + _u) * " + s(c.type.width) + ";");
+                    else
+                        cg.addRowBinding("const " + c.type.cxx() + " " + c.name + " = _r_" + c.name + "[_u];");
+                }
+                cg.setStaged(true);
+                consume();
+                cg.setStaged(false);
+                cg.rowBindings().resize(mark);
+                if (anyChar) cg.addLine("__syncthreads();");
+                return;
+            }
             cg.addLine("#pragma unroll");
             cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
                 cg.addLine("const u64 " + idxVar_ + " = _r0 + (u64)_u;");
@@ -153,7 +183,7 @@ HipHashTableScan::HipHashTableScan(const std::string& table, const std::string& 
                                    const std::string& prefix)
     : table_(table), slotsExpr_(slotsExpr), prefix_(prefix), slotWords_(slotWords) {}
 
-void HipHashTableScan::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipHashTableScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.setPhaseRowsExpr(slotsExpr_);
     cg.addBufferParam(table_, "u64", 8, slotsExpr_ + " * " + s(slotWords_), InitKind::None, {}, true, true);
     const std::string n = "nslots_" + table_;
@@ -190,8 +220,15 @@ std::string HipHashTableScan::describe() const { return "HashTableScan(" + table
 HipSelection::HipSelection(std::unique_ptr<HipOperator> child, const std::string& predicate)
     : HipUnaryOperator(std::move(child)), predicate_(predicate) {}
 
-void HipSelection::produce(HipCodegen& cg, ConsumerFn consume) {
-    child_->produce(cg, [&]() { cg.addIf(predicate_, [&]() { consume(); }); });
+void HipSelection::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    child_->produce(cg, [&]() {
+        if (cg.staged()) {
+            cg.rowLoop([&]() { cg.addLine("_act[_u] = (" + predicate_ + ");"); });
+            consume();
+            return;
+        }
+        cg.addIf(predicate_, [&]() { consume(); });
+    });
 }
 
 std::string HipSelection::describe() const { return "Selection(" + predicate_ + ")"; }
@@ -200,9 +237,16 @@ HipComputeExpr::HipComputeExpr(std::unique_ptr<HipOperator> child, const std::st
                                const std::string& varType, const std::string& expression)
     : HipUnaryOperator(std::move(child)), varName_(varName), varType_(varType), expression_(expression) {}
 
-void HipComputeExpr::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipComputeExpr::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     child_->produce(cg, [&]() {
-        cg.addLine("const " + varType_ + " " + varName_ + " = " + expression_ + ";");
+        const std::string decl = "const " + varType_ + " " + varName_ + " = " + expression_ + ";";
+        if (cg.staged()) {
+            cg.addRowBinding(decl);
+            consume();
+            cg.rowBindings().pop_back();
+            return;
+        }
+        cg.addLine(decl);
         consume();
     });
 }
@@ -217,7 +261,7 @@ HipBitmapBuild::HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::st
                                const std::string& keyExpr, const std::string& bitsExpr)
     : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr) {}
 
-void HipBitmapBuild::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipBitmapBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapWriteParam(bitmap_, bits_);
     child_->produce(cg, [&]() {
         cg.addLine("gdb_bitmap_set(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" + key_ + "));");
@@ -231,9 +275,24 @@ HipBitmapProbe::HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::st
                                const std::string& keyExpr, const std::string& bitsExpr, bool anti)
     : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr), anti_(anti) {}
 
-void HipBitmapProbe::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipBitmapProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapReadParam(bitmap_, bits_);
     child_->produce(cg, [&]() {
+        if (cg.staged()) {
+            // all bitmap words of the batch first, then the bit tests
+            const std::string A = cg.uniqueName("_bw");
+            const std::string k = "((u64)(i64)(" + key_ + "))";
+            cg.addLine("u32 " + A + "[ITEMS];");
+            cg.rowLoop([&]() {
+                cg.addLine(A + "[_u] = " + k + " < nbits_" + bitmap_ + " ? " + bitmap_ + "[" + k + " >> 5] : 0u;");
+            });
+            cg.rowLoop([&]() {
+                cg.addLine("_act[_u] = ((" + A + "[_u] >> ((u32)" + k + " & 31u)) & 1u) " + (anti_ ? "==" : "!=") +
+                           " 0u;");
+            });
+            consume();
+            return;
+        }
         cg.addIf(std::string(anti_ ? "!" : "") + "gdb_bitmap_test(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" +
                      key_ + "))",
                  [&]() { consume(); });
@@ -257,7 +316,7 @@ HipHashBuild::HipHashBuild(std::unique_ptr<HipOperator> child, const std::string
       slots_(slotsExpr),
       slotWords_(slotWords) {}
 
-void HipHashBuild::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipHashBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     std::vector<uint64_t> pat(static_cast<size_t>(slotWords_), 0);
     pat[0] = kEmptyWord;
     cg.addBufferParam(table_, "u64", 8, slots_ + " * " + s(slotWords_), InitKind::Pattern, wordsPattern(pat), false,
@@ -303,13 +362,62 @@ HipHashProbe::HipHashProbe(std::unique_ptr<HipOperator> child, const std::string
       slotVar_(slotVar),
       slotWords_(slotWords) {}
 
-void HipHashProbe::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipHashProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(table_, "u64", 8, slots_ + " * " + s(slotWords_), InitKind::None, {}, true, true);
     cg.addSymbolParam("nslots_" + table_, slots_);
     cg.addLine("const u32 _bits_" + table_ + " = (u32)__builtin_ctzll(nslots_" + table_ + ");");
     cg.addLine("const u32 _maxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
                " : 65536ull);");
     child_->produce(cg, [&]() {
+        if (cg.staged()) {
+            // First slots of the whole batch in flight together; rows whose
+            // first slot holds another key continue probing in lock-step
+            // rounds (one load per pending row per round). Every array is
+            // indexed only inside unrolled loops, so all stay in registers.
+            const std::string T = table_, W = s(slotWords_);
+            const std::string HW = cg.uniqueName("_hw"), HS = cg.uniqueName("_hs"), HP = cg.uniqueName("_hp");
+            const std::string PD = cg.uniqueName("_pd"), ANY = cg.uniqueName("_any");
+            const std::string K = "(u32)(" + key_ + ")";
+            cg.addLine("u64 " + HW + "[ITEMS];");
+            cg.addLine("i64 " + HS + "[ITEMS];");
+            cg.addLine("u32 " + HP + "[ITEMS];");
+            cg.addLine("bool " + PD + "[ITEMS];");
+            cg.addLine("bool " + ANY + " = false;");
+            cg.rowLoop([&]() {
+                cg.addLine(HS + "[_u] = (i64)gdb_hash_slot(" + K + ", _bits_" + T + ");");
+                cg.addLine(HW + "[_u] = " + T + "[(u64)" + HS + "[_u] * " + W + "];");
+            });
+            cg.addLine("#pragma unroll");
+            cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) " + PD + "[_u] = false;");
+            cg.rowLoop([&]() {
+                cg.addLine("const u64 _w = " + HW + "[_u];");
+                cg.addLine(HP + "[_u] = (u32)(_w >> 32);");
+                cg.addBlock("if ((u32)_w != " + K + ")", [&]() {
+                    cg.addLine("_act[_u] = false;");
+                    cg.addLine(PD + "[_u] = (u32)_w != GDB_EMPTY_KEY;");
+                    cg.addLine(ANY + " = " + ANY + " || " + PD + "[_u];");
+                });
+            });
+            cg.addBlock("for (u32 _p = 1; " + ANY + " && _p < _maxp_" + T + "; ++_p)", [&]() {
+                cg.addLine(ANY + " = false;");
+                cg.rowLoop([&]() {
+                    cg.addLine(HS + "[_u] = (i64)(((u64)" + HS + "[_u] + 1ull) & ((1ull << _bits_" + T + ") - 1ull));");
+                    cg.addLine(HW + "[_u] = " + T + "[(u64)" + HS + "[_u] * " + W + "];");
+                }, PD);
+                cg.rowLoop([&]() {
+                    cg.addLine("const u64 _w = " + HW + "[_u];");
+                    cg.addLine("if ((u32)_w == " + K + ") { _act[_u] = true; " + PD + "[_u] = false; " + HP +
+                               "[_u] = (u32)(_w >> 32); }");
+                    cg.addLine("else if ((u32)_w == GDB_EMPTY_KEY) " + PD + "[_u] = false;");
+                    cg.addLine("else " + ANY + " = true;");
+                }, PD);
+            });
+            cg.addRowBinding("const i64 " + slotVar_ + " = " + HS + "[_u];");
+            cg.addRowBinding("const u32 " + slotVar_ + "_payload = " + HP + "[_u];");
+            consume();
+            cg.rowBindings().resize(cg.rowBindings().size() - 2);
+            return;
+        }
         cg.addLine("const i64 " + slotVar_ + " = gdb_ht_find(" + table_ + ", " + s(slotWords_) + ", _bits_" + table_ +
                    ", (u32)(" + key_ + "), _maxp_" + table_ + ");");
         cg.addIf(slotVar_ + " >= 0", [&]() {
@@ -331,7 +439,7 @@ HipSlotAtomicAdd::HipSlotAtomicAdd(std::unique_ptr<HipOperator> child, const std
       slotWords_(slotWords),
       word_(word) {}
 
-void HipSlotAtomicAdd::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipSlotAtomicAdd::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     // The probe registered the table read-only; the atomic makes it writable.
     cg.addBufferParam(table_, "u64", 8, "", InitKind::None, {}, false, true);
     child_->produce(cg, [&]() {
@@ -355,7 +463,7 @@ HipArrayLookup::HipArrayLookup(std::unique_ptr<HipOperator> child, const std::st
       var_(resultVar),
       sentinel_(sentinel) {}
 
-void HipArrayLookup::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipArrayLookup::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(array_, "u32", 4, size_, InitKind::None, {}, true, true);
     cg.addSymbolParam("nelem_" + array_, size_);
     child_->produce(cg, [&]() {
@@ -392,7 +500,7 @@ void HipKeyedAgg::addAverage(const std::string& name, const std::string& numerat
     avgs_.push_back({name, {numerator, denominator}});
 }
 
-void HipKeyedAgg::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipKeyedAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     if (aggs_.empty()) throw std::runtime_error("HipKeyedAgg(" + out_ + "): no aggregates");
     // Word layout per bucket; buckets are padded to 128 bytes so the per-block
     // atomics of different buckets land on different cache lines.
@@ -500,7 +608,7 @@ void HipKeyedAgg::produce(HipCodegen& cg, ConsumerFn consume) {
                      "_words[" + NS + "] = {" + wordIdx.str() + "};");
         cg.addComment("block reduction: wave DPP sums -> LDS -> one global atomic per slot and block");
         cg.addBlock("", [&]() {
-            cg.addLine("__shared__ u64 _red[" + NS + " * 16];");
+            cg.declareLds("u64", "_red", NS + " * 16");
             cg.addLine("const int _lane = gdb_lane(), _wave = gdb_wave(), _nw = (int)(blockDim.x >> 6);");
             for (int k = 0; k < ns; ++k) {
                 const auto& se = slotExprs[static_cast<size_t>(k)];
@@ -532,7 +640,7 @@ void HipKeyedAgg::produce(HipCodegen& cg, ConsumerFn consume) {
     } else if (lds) {
         const std::string W = s(words);
         cg.addComment("KeyedAgg " + out_ + ": block-shared LDS table, " + NB + " buckets x " + W + " words");
-        cg.addLine("__shared__ u64 _kl_" + out_ + "[" + s((long long)numBuckets_ * words) + "];");
+        cg.declareLds("u64", "_kl_" + out_, s((long long)numBuckets_ * words));
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + s((long long)numBuckets_ * words) +
                         "; _s += (int)blockDim.x)",
                     [&]() {
@@ -640,13 +748,13 @@ HipAtomicCount::HipAtomicCount(std::unique_ptr<HipOperator> child, const std::st
                                const std::string& bucketExpr, const std::string& sizeExpr, int staticSize)
     : HipUnaryOperator(std::move(child)), array_(arrayName), bucket_(bucketExpr), size_(sizeExpr), static_(staticSize) {}
 
-void HipAtomicCount::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipAtomicCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(array_, "u32", 4, size_, InitKind::Zero, {}, false, true);
     cg.addSymbolParam("nelem_" + array_, size_);
     if (static_ > 0 && static_ <= 8192) {
         const std::string S = s(static_);
         cg.addComment("AtomicCount " + array_ + ": LDS histogram of " + S + " bins");
-        cg.addLine("__shared__ u32 _hist_" + array_ + "[" + S + "];");
+        cg.declareLds("u32", "_hist_" + array_, S);
         cg.addLine("for (int _h = (int)threadIdx.x; _h < " + S + "; _h += (int)blockDim.x) _hist_" + array_ + "[_h] = 0;");
         cg.addLine("__syncthreads();");
         child_->produce(cg, [&]() {
@@ -705,7 +813,7 @@ HipHashGroupBy::HipHashGroupBy(std::unique_ptr<HipOperator> child, const std::st
         throw std::runtime_error("HipHashGroupBy: LDS slots must be a power of two in [64, 4096]");
 }
 
-void HipHashGroupBy::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(table_, "u64", 8, slots_ + " * 4", InitKind::Pattern,
                       wordsPattern({kEmptyWord, 0, kI64Max, kI64Min}), false, true);
     cg.addSymbolParam("nslots_" + table_, slots_);
@@ -723,7 +831,7 @@ void HipHashGroupBy::produce(HipCodegen& cg, ConsumerFn consume) {
         const std::string CAP = s(ldsSlots_);
         std::string LS = CAP, LB = s(lbits) + "u";
         cg.addComment("HashGroupBy " + T + ": LDS table of " + CAP + " slots absorbs hot keys; spills go global");
-        cg.addLine("__shared__ u64 _gl_" + T + "[" + CAP + " * 4];");
+        cg.declareLds("u64", "_gl_" + T, CAP + " * 4");
         if (!ldsBitsScalar_.empty()) {
             cg.addScalarParam(ldsBitsScalar_);
             cg.addLine("const u32 _lb_" + T + " = (u32)(" + ldsBitsScalar_ + " < 1 ? 1 : (" + ldsBitsScalar_ + " > " +
@@ -809,7 +917,7 @@ void HipTopK::addColumn(const std::string& name, int word, int shift, int bits, 
     cols_.push_back({name, word, shift, bits, complementOf});
 }
 
-void HipTopK::produce(HipCodegen& cg, ConsumerFn consume) {
+void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const std::string K = s(k_);
     const std::string ctl = out_ + "_ctl";
     const std::string cand = out_ + "_cand";
@@ -818,7 +926,6 @@ void HipTopK::produce(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(out_, "u64", 8, s(2LL * k_), InitKind::Zero, {}, false, true);
     cg.addBufferParam(ctl, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.addBufferParam(cand, "u64", 8, s(2LL * k_ * maxBlocks), InitKind::None, {}, false, true);
-    const int lb = cg.getPhases().back().launchBound;
     cg.addLine("GdbTopK<" + K + "> _tk;");
     cg.addLine("gdb_topk_init<" + K + ">(_tk);");
     child_->produce(cg, [&]() {
@@ -827,28 +934,30 @@ void HipTopK::produce(HipCodegen& cg, ConsumerFn consume) {
     });
     cg.addComment("TopK " + out_ + ": block merge, threshold-pruned candidates, last block merges");
     cg.addBlock("", [&]() {
-        cg.addLine("__shared__ u64 _tk_lds[" + s(lb) + " * " + K + " * 2];");
-        cg.addLine("__shared__ int _tk_last;");
+        cg.declareLds("u64", "_tk_lds", "GDB_TOPK_LDS_WORDS(" + K + ")");
+        cg.declareLds("u64", "_tk_sh", "2");
         cg.addLine("gdb_topk_block_merge<" + K + ">(_tk, _tk_lds);");
+        cg.addLine("const u64* _tk_res = _tk_lds + 64;");
         cg.addBlock("if (threadIdx.x == 0)", [&]() {
             cg.addLine("const u64 _thr = gdb_load_relaxed_u64(" + ctl + " + 2);");
-            cg.addLine("int _cnt = 0;");
-            cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) if ((_tk.h[_j] | _tk.l[_j]) != 0 && _tk.h[_j] >= _thr) _cnt = _j + 1;");
+            cg.addLine("u64 _cnt = 0;");
+            cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) if ((_tk_res[2 * _j] | _tk_res[2 * _j + 1]) != 0 && "
+                       "_tk_res[2 * _j] >= _thr) _cnt = (u64)_j + 1;");
             cg.addBlock("if (_cnt > 0)", [&]() {
                 cg.addLine("const u64 _base = __hip_atomic_fetch_add(" + ctl +
-                           " + 1, (u64)_cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
-                cg.addLine("for (int _j = 0; _j < _cnt; ++_j) { " + cand + "[(_base + _j) * 2] = _tk.h[_j]; " + cand +
-                           "[(_base + _j) * 2 + 1] = _tk.l[_j]; }");
+                           " + 1, _cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+                cg.addLine("for (u64 _j = 0; _j < _cnt; ++_j) { " + cand + "[(_base + _j) * 2] = _tk_res[2 * _j]; " +
+                           cand + "[(_base + _j) * 2 + 1] = _tk_res[2 * _j + 1]; }");
             });
-            cg.addLine("if ((_tk.h[" + s(k_ - 1) + "] | _tk.l[" + s(k_ - 1) + "]) != 0) gdb_atomic_max_u64(" + ctl +
-                       " + 2, _tk.h[" + s(k_ - 1) + "]);");
+            cg.addLine("if ((_tk_res[" + s(2 * k_ - 2) + "] | _tk_res[" + s(2 * k_ - 1) + "]) != 0) gdb_atomic_max_u64(" +
+                       ctl + " + 2, _tk_res[" + s(2 * k_ - 2) + "]);");
             cg.addLine("__threadfence();");
             cg.addLine("const u64 _t = __hip_atomic_fetch_add(" + ctl +
                        " + 0, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
-            cg.addLine("_tk_last = (_t == (u64)gridDim.x - 1ull) ? 1 : 0;");
+            cg.addLine("_tk_sh[0] = (_t == (u64)gridDim.x - 1ull) ? 1ull : 0ull;");
         });
         cg.addLine("__syncthreads();");
-        cg.addBlock("if (_tk_last)", [&]() {
+        cg.addBlock("if (_tk_sh[0])", [&]() {
             cg.addLine("__threadfence();");
             cg.addLine("const u64 _nc = gdb_load_relaxed_u64(" + ctl + " + 1);");
             cg.addLine("GdbTopK<" + K + "> _f;");
@@ -859,8 +968,8 @@ void HipTopK::produce(HipCodegen& cg, ConsumerFn consume) {
             cg.addLine("gdb_topk_block_merge<" + K + ">(_f, _tk_lds);");
             cg.addBlock("if (threadIdx.x == 0)", [&]() {
                 cg.addLine("u64 _valid = 0;");
-                cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) { " + out_ + "[2 * _j] = _f.h[_j]; " + out_ +
-                           "[2 * _j + 1] = _f.l[_j]; if ((_f.h[_j] | _f.l[_j]) != 0) ++_valid; }");
+                cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) { " + out_ + "[2 * _j] = _tk_res[2 * _j]; " + out_ +
+                           "[2 * _j + 1] = _tk_res[2 * _j + 1]; if ((_tk_res[2 * _j] | _tk_res[2 * _j + 1]) != 0) ++_valid; }");
                 cg.addLine(ctl + "[3] = _valid;");
                 cg.addLine(ctl + "[0] = 0; " + ctl + "[1] = 0; " + ctl + "[2] = 0;");
             });

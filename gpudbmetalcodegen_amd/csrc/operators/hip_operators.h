@@ -13,6 +13,15 @@
 // columns are `const char*` into an LDS-staged copy of the tile. Expressions
 // passed to operators are HIP C++ over those names (and over scalar
 // parameters registered on the phase).
+//
+// Staged phases (item-batched emission): the scan hands its consumer all
+// ITEMS rows of a thread at once, with an active mask `_act[ITEMS]`. A
+// stageable operator (selection, compute, bitmap probe, hash probe) emits
+// each step as its own unrolled loop over the ITEMS rows, so the random loads
+// of ITEMS rows (bitmap words, first hash slots) are in flight together
+// instead of one dependent chain per row; the first operator that is not
+// stageable gets the usual per-row scope, wrapped in one loop over the active
+// rows. Results are identical to per-row emission.
 
 #include "../core/hip_codegen_base.h"
 
@@ -26,9 +35,17 @@ namespace codegen {
 class HipOperator {
 public:
     virtual ~HipOperator() = default;
-    virtual void produce(HipCodegen& cg, ConsumerFn consume) = 0;
+    // Emit this operator and its subtree; consume() emits the consumer's code
+    // for each row (or, in a staged phase, for the batch of active rows when
+    // the consumer is stageable).
+    void produce(HipCodegen& cg, ConsumerFn consume);
     virtual std::string describe() const = 0;
     virtual void describeTree(std::string& out, int depth) const;
+    // True when the operator can consume a staged batch of rows.
+    virtual bool stageable() const { return false; }
+
+protected:
+    virtual void produceImpl(HipCodegen& cg, ConsumerFn consume) = 0;
 };
 
 class HipUnaryOperator : public HipOperator {
@@ -56,7 +73,7 @@ public:
     };
     explicit HipGridStrideScan(const std::string& table, const std::string& idxVar = "i");
     void addColumn(const std::string& column, ColumnType type);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
     const std::string& tableName() const { return table_; }
 
@@ -74,7 +91,7 @@ class HipHashTableScan : public HipOperator {
 public:
     HipHashTableScan(const std::string& table, const std::string& slotsExpr, int slotWords,
                      const std::string& prefix);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -89,8 +106,9 @@ private:
 // WHERE filter: if (predicate) { consume(); } (metal_operators.cpp:63-69).
 class HipSelection : public HipUnaryOperator {
 public:
+    bool stageable() const override { return true; }
     HipSelection(std::unique_ptr<HipOperator> child, const std::string& predicate);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -100,9 +118,10 @@ private:
 // const type var = expr; consume(); (metal_operators.cpp:81-95).
 class HipComputeExpr : public HipUnaryOperator {
 public:
+    bool stageable() const override { return true; }
     HipComputeExpr(std::unique_ptr<HipOperator> child, const std::string& varName, const std::string& varType,
                    const std::string& expression);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -114,7 +133,7 @@ class HipBitmapBuild : public HipUnaryOperator {
 public:
     HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
                    const std::string& bitsExpr);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -124,9 +143,10 @@ private:
 // Semi-join / anti-join probe (metal_operators.cpp:132-165).
 class HipBitmapProbe : public HipUnaryOperator {
 public:
+    bool stageable() const override { return true; }
     HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
                    const std::string& bitsExpr, bool anti = false);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -141,7 +161,7 @@ class HipHashBuild : public HipUnaryOperator {
 public:
     HipHashBuild(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
                  const std::string& payloadExpr, const std::string& slotsExpr, int slotWords);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -155,9 +175,10 @@ private:
 // (metal_plan_builder.cpp:1598-1610).
 class HipHashProbe : public HipUnaryOperator {
 public:
+    bool stageable() const override { return true; }
     HipHashProbe(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
                  const std::string& slotsExpr, int slotWords, const std::string& slotVar);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -172,7 +193,7 @@ class HipSlotAtomicAdd : public HipUnaryOperator {
 public:
     HipSlotAtomicAdd(std::unique_ptr<HipOperator> child, const std::string& table, int slotWords,
                      const std::string& slotVar, int word, const std::string& valueExpr);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -185,7 +206,7 @@ class HipArrayLookup : public HipUnaryOperator {
 public:
     HipArrayLookup(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& keyExpr,
                    const std::string& sizeExpr, const std::string& resultVar, long long sentinel);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -220,7 +241,7 @@ public:
     void addAggregate(const std::string& name, AggKind kind, const std::string& valueExpr, int scaleDigits = 0);
     void setBucketLabels(const std::vector<std::string>& labels, const std::string& keyColumn);
     void addAverage(const std::string& name, const std::string& numerator, const std::string& denominator);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -242,7 +263,7 @@ public:
     // Make the counters the plan's result: rows (key, count) for non-zero
     // counters, optionally ordered count desc then key desc (Q13).
     void setResult(const std::string& keyName, const std::string& valueName, bool sortByCountDesc);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -268,7 +289,7 @@ public:
     // at run time, up to the compiled capacity ldsSlots (autotuned per
     // pipeline without recompiling).
     void setRuntimeLdsBits(const std::string& scalarName) { ldsBitsScalar_ = scalarName; }
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
@@ -294,7 +315,7 @@ public:
     HipTopK(std::unique_ptr<HipOperator> child, const std::string& outputName, int k, const std::string& rankHiExpr,
             const std::string& rankLoExpr);
     void addColumn(const std::string& name, int word, int shift, int bits, long long complementOf = 0);
-    void produce(HipCodegen& cg, ConsumerFn consume) override;
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:

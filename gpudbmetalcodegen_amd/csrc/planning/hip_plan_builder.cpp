@@ -139,7 +139,7 @@ void q3BitmapPhase(HipQueryPlan& plan, const PlanOptions& o) {
 
 // Exact count of the B rows the hash table will hold (the plain form's
 // sizing pre-pass).
-void q3CountPhase(HipQueryPlan& plan) {
+void q3CountPhase(HipQueryPlan& plan, const PlanOptions& o) {
     auto scan = makeScan("B", {"b_fkey", "b_date"});
     auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)b_date < q3_date");
     auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
@@ -147,13 +147,14 @@ void q3CountPhase(HipQueryPlan& plan) {
     cnt->addAggregate("b_rows", AggKind::Count, "1", 0);
     auto& ph = appendPhase(plan, "Q3_count_b", std::move(cnt), 256, 8);
     ph.scalarParams = {"q3_date"};
+    ph.staged = optInt(o, "staged", 1) != 0;
 }
 
 HipQueryPlan buildQ3Size(const PlanOptions& o) {
     HipQueryPlan plan;
     plan.name = "Q3_SIZE";
     q3BitmapPhase(plan, o);
-    q3CountPhase(plan);
+    q3CountPhase(plan, o);
     plan.requiredScalars = {"q3_seg", "q3_date"};
     plan.requiredSymbols = {"a_bits"};
     return plan;
@@ -166,7 +167,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
     const bool prepass = opt(o, "sizing", "prepass") == "prepass";
     q3BitmapPhase(plan, o);
     if (prepass) {
-        q3CountPhase(plan);
+        q3CountPhase(plan, o);
         plan.sizingRules.push_back({"Q3_count_b", "b_slots", "d_q3_bcount", 0, 2, 64, true});
     }
     {
@@ -179,6 +180,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
             top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
         auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
+        ph.staged = optInt(o, "staged", 1) != 0;
     }
     {
         auto scan = makeScan("F", {"f_fkey", "f_price", "f_disc", "f_date"});
@@ -191,6 +193,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         auto& ph = appendPhase(plan, "Q3_probe_f", std::move(cur), optInt(o, "block", 256), optInt(o, "items", 8),
                                {4, 8});
         ph.scalarParams = {"q3_date"};
+        ph.staged = optInt(o, "staged", 1) != 0;
     }
     {
         auto scan = std::make_unique<HipHashTableScan>("d_q3_b_ht", "b_slots", 2, "ht");
@@ -334,7 +337,6 @@ HipQueryPlan buildQ13L(const PlanOptions& o) {
 // b_date in [q22_date_lo, q22_date_hi); count and sum(a_key) per segment
 // (buildQ22Plan_byName, metal_plan_builder.cpp:1304-1357).
 HipQueryPlan buildQ22A(const PlanOptions& o) {
-    (void)o;
     HipQueryPlan plan;
     plan.name = "Q22A";
     {
@@ -352,7 +354,8 @@ HipQueryPlan buildQ22A(const PlanOptions& o) {
         agg->addAggregate("num_keys", AggKind::Count, "1", 0);
         agg->addAggregate("sum_keys", AggKind::SumI64, "a_key", 0);
         agg->setBucketLabels({"0", "1", "2", "3", "4"}, "a_seg");
-        appendPhase(plan, "Q22_anti_aggregate", std::move(agg), 256, 8);
+        auto& ph = appendPhase(plan, "Q22_anti_aggregate", std::move(agg), 256, 8);
+        ph.staged = optInt(o, "staged", 1) != 0;
     }
     plan.requiredScalars = {"q22_date_lo", "q22_date_hi"};
     plan.requiredSymbols = {"a_bits"};
@@ -395,6 +398,7 @@ HipQueryPlan buildPipeJoin(const PlanOptions& o, bool anti) {
     auto& ph = appendPhase(plan, anti ? "PIPE_anti" : "PIPE_semi", std::move(agg), optInt(o, "block", 256),
                            optInt(o, "items", 4), {1, 2, 4, 8, 16});
     ph.launchBound = 1024;
+    ph.staged = optInt(o, "staged", 1) != 0;
     plan.requiredSymbols = {"pipe_bits"};
     return plan;
 }
@@ -476,6 +480,7 @@ HipCodegen generateFromPlan(const HipQueryPlan& plan, bool rangeMode) {
         cg.setPhaseItemsVariants(phase.itemsVariants);
         cg.setPhaseItems(phase.items);
         cg.setPhaseLaunchBound(phase.launchBound);
+        cg.setPhaseStaged(phase.staged);
         if (phase.maxThreadgroups) cg.setPhaseMaxThreadgroups(phase.maxThreadgroups);
         for (const auto& sp : phase.scalarParams) cg.addScalarParam(sp);
         for (const auto& eb : phase.externalBuffers)

@@ -33,6 +33,9 @@ struct HipQueryPlan {
         int launchBound = 1024;
         int maxThreadgroups = 0;
         bool singleThread = false;
+        // Item-batched emission: the random loads (bitmap words, first hash
+        // slots) of a thread's ITEMS rows are issued together.
+        bool staged = false;
         std::vector<std::string> scalarParams;  // registered before the operators run
         // Buffers the caller registers (e.g. LIKE words, prebuilt bitmaps):
         // (name, element type) read-only parameters.

@@ -48,7 +48,7 @@ void PipelineBatch::freeDevice() {
     }
 }
 
-int PipelineBatch::add(const HipQueryPlan* plan, const HipCodegen* cg, HipGenericExecutor* ex, int items) {
+int BatchKernelSource::add(const HipQueryPlan* plan, const HipCodegen* cg, int items) {
     if (plan->phases.size() != 1) throw std::runtime_error("PipelineBatch: pipelines must be single-phase plans");
     const auto& ph = cg->getPhases()[0];
     bool known = false;
@@ -70,22 +70,17 @@ int PipelineBatch::add(const HipQueryPlan* plan, const HipCodegen* cg, HipGeneri
     }
     auto& ki = kinds_[static_cast<size_t>(kind)].items;
     if (std::find(ki.begin(), ki.end(), items) == ki.end()) ki.push_back(items);
-    uint64_t rows = 0;
-    const std::string tbl = ph.scannedTable;
-    if (!ex->tryGetSymbol(tableSizeName(tbl), rows))
-        throw std::runtime_error("PipelineBatch: table " + tbl + " of pipeline not registered");
-    pipes_.push_back({plan, cg, ex, items, kind, rows});
-    return static_cast<int>(pipes_.size()) - 1;
+    return kind;
 }
 
-void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
-    if (pipes_.empty()) throw std::runtime_error("PipelineBatch: no pipelines");
-    if (block < 64 || block > 1024 || block % 64) throw std::runtime_error("PipelineBatch: bad block size");
-    block_ = block;
-    for (const auto& p : pipes_)
-        if (block > p.cg->getPhases()[0].launchBound)
-            throw std::runtime_error("PipelineBatch: block size above a pipeline's launch bound");
-    // --- source -------------------------------------------------------------
+int BatchKernelSource::launchBound() const {
+    int lb = 1024;
+    for (auto& kc : kindCg_) lb = std::min(lb, kc->getPhases()[0].launchBound);
+    return lb;
+}
+
+std::string BatchKernelSource::print() const {
+    if (kinds_.empty()) throw std::runtime_error("PipelineBatch: no pipelines");
     std::ostringstream src;
     src << kGdbDeviceHeader << "\n// ===== helpers =====\n";
     std::unordered_set<std::string> seen;
@@ -102,13 +97,18 @@ GDB_DEV T gdb_arg(u64 w) {
     else return (T)w;
 }
 )";
-    int lb = 1024;
-    for (auto& kc : kindCg_) lb = std::min(lb, kc->getPhases()[0].launchBound);
-    src << "extern \"C\" __global__ void __launch_bounds__(" << lb << ") gdb_batch_kernel(const u64* __restrict__ args, "
+    // One LDS buffer shared by every body: the largest body's need, not the sum.
+    std::string ldsMax = "16u";
+    for (size_t k = 0; k < kinds_.size(); ++k)
+        for (int it : kinds_[k].items)
+            ldsMax = "gdb_cmax(" + ldsMax + ", " + kinds_[k].body + "_lds_bytes<" + std::to_string(it) + ">())";
+    src << "extern \"C\" __global__ void __launch_bounds__(" << launchBound()
+        << ") gdb_batch_kernel(const u64* __restrict__ args, "
         << "const u32 args_stride, const u32* __restrict__ kind_items, const u32* __restrict__ item_pipe, "
         << "const u64* __restrict__ item_begin, const u64* __restrict__ item_end, const u32 n_items, "
         << "u32* __restrict__ ticket) {\n"
         << "    __shared__ u32 _it;\n"
+        << "    __shared__ __attribute__((aligned(16))) unsigned char _lds[" << ldsMax << "];\n"
         << "    for (;;) {\n"
         << "        __syncthreads();\n"
         << "        if (threadIdx.x == 0) _it = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, "
@@ -125,7 +125,7 @@ GDB_DEV T gdb_arg(u64 w) {
             src << "        case " << (k * 32 + static_cast<size_t>(it)) << ": " << kinds_[k].body << "<" << it << ">(";
             for (size_t a = 0; a < kinds_[k].argTypes.size(); ++a)
                 src << (a ? ", " : "") << "gdb_arg<" << kinds_[k].argTypes[a] << ">(a[" << a << "])";
-            src << (kinds_[k].argTypes.empty() ? "" : ", ") << "rb, re); break;\n";
+            src << (kinds_[k].argTypes.empty() ? "" : ", ") << "rb, re, _lds); break;\n";
         }
     }
     src << "        default: break;\n        }\n    }\n"
@@ -133,18 +133,39 @@ GDB_DEV T gdb_arg(u64 w) {
         << "        const u32 done = __hip_atomic_fetch_add(ticket + 1, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);\n"
         << "        if (done == gridDim.x - 1) { ticket[0] = 0; ticket[1] = 0; }\n"
         << "    }\n}\n";
-    source_ = src.str();
+    return src.str();
+}
+
+int PipelineBatch::add(const HipQueryPlan* plan, const HipCodegen* cg, HipGenericExecutor* ex, int items) {
+    const int kind = src_.add(plan, cg, items);
+    uint64_t rows = 0;
+    const std::string tbl = cg->getPhases()[0].scannedTable;
+    if (!ex->tryGetSymbol(tableSizeName(tbl), rows))
+        throw std::runtime_error("PipelineBatch: table " + tbl + " of pipeline not registered");
+    pipes_.push_back({plan, cg, ex, items, kind, rows});
+    return static_cast<int>(pipes_.size()) - 1;
+}
+
+void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
+    if (pipes_.empty()) throw std::runtime_error("PipelineBatch: no pipelines");
+    if (block < 64 || block > 1024 || block % 64) throw std::runtime_error("PipelineBatch: bad block size");
+    block_ = block;
+    for (const auto& p : pipes_)
+        if (block > p.cg->getPhases()[0].launchBound)
+            throw std::runtime_error("PipelineBatch: block size above a pipeline's launch bound");
+    source_ = src_.print();
     cq_ = rc_.compile(source_);
     kernel_ = rc_.getPipeline(cq_, "gdb_batch_kernel");
     // --- arguments ------------------------------------------------------------
     argStride_ = 0;
-    for (const auto& k : kinds_) argStride_ = std::max<uint32_t>(argStride_, static_cast<uint32_t>(k.argTypes.size()));
+    for (size_t k = 0; k < src_.kinds(); ++k)
+        argStride_ = std::max<uint32_t>(argStride_, static_cast<uint32_t>(src_.argTypes(static_cast<int>(k)).size()));
     argStride_ = std::max<uint32_t>(argStride_, 1);
     std::vector<uint64_t> args(static_cast<size_t>(argStride_) * pipes_.size(), 0);
     std::vector<uint32_t> kindItems(pipes_.size());
     for (size_t i = 0; i < pipes_.size(); ++i) {
         auto& p = pipes_[i];
-        auto a = p.ex->packPhaseArgs(*kindCg_[static_cast<size_t>(p.kind)], 0);
+        auto a = p.ex->packPhaseArgs(src_.kindCodegen(p.kind), 0);
         std::copy(a.begin(), a.end(), args.begin() + static_cast<long>(i * argStride_));
         kindItems[i] = static_cast<uint32_t>(p.kind * 32 + p.items);
     }

@@ -24,6 +24,35 @@
 namespace gdb {
 namespace runtime {
 
+// Source of the persistent batched kernel for a set of pipeline kinds (a
+// kind = one plan's phase body, instantiated for each ITEMS it is used with).
+// Needs no device, so the kernel can be emitted and compiled offline.
+class BatchKernelSource {
+public:
+    // Register a pipeline's plan and ITEMS; returns the kind index.
+    int add(const codegen::HipQueryPlan* plan, const codegen::HipCodegen* cg, int items);
+    std::string print() const;
+    size_t kinds() const { return kinds_.size(); }
+    // Argument types of a kind's body, in binding order.
+    const std::vector<std::string>& argTypes(int kind) const { return kinds_[static_cast<size_t>(kind)].argTypes; }
+    const codegen::HipCodegen& kindCodegen(int kind) const { return *kindCg_[static_cast<size_t>(kind)]; }
+    int launchBound() const;
+    void clear() {
+        kinds_.clear();
+        kindCg_.clear();
+    }
+
+private:
+    struct Kind {
+        std::string key;   // plan source identity
+        std::string body;  // range-mode body name
+        std::vector<std::string> argTypes;
+        std::vector<int> items;
+    };
+    std::vector<Kind> kinds_;
+    std::vector<std::unique_ptr<codegen::HipCodegen>> kindCg_;  // range-mode codegen per kind
+};
+
 class PipelineBatch {
 public:
     explicit PipelineBatch(gdb_ctx ctx);
@@ -61,20 +90,13 @@ private:
         int kind;
         uint64_t rows;
     };
-    struct Kind {
-        std::string key;        // plan source identity
-        std::string body;       // range-mode body name
-        std::vector<std::string> argTypes;
-        std::vector<int> items;
-    };
     gdb_ctx ctx_;
     gdb_stream stream_;
     codegen::RuntimeCompiler rc_;
     codegen::RuntimeCompiler::CompiledQuery cq_;
     gdb_kernel kernel_ = nullptr;
     std::vector<Pipe> pipes_;
-    std::vector<Kind> kinds_;
-    std::vector<std::unique_ptr<codegen::HipCodegen>> kindCg_;  // range-mode codegen per kind
+    BatchKernelSource src_;
     std::string source_;
     int block_ = 256;
     unsigned grid_ = 0;

@@ -189,6 +189,15 @@ class HipQueryPlan:
         self.compile_ms, self.cache_hit, self.plan_hash = ms.value, bool(hit.value), h.value
         return self
 
+    @staticmethod
+    def batch_kernel_source(plans: list, items: list[int]) -> str:
+        """Source of the persistent batched kernel for pipelines plans[i] at
+        items[i]; needs no device."""
+        n = len(plans)
+        hs = (C.c_void_p * n)(*[p._h.value for p in plans])
+        its = (C.c_int * n)(*items)
+        return N.two_call_string(_eng().gdb_batch_kernel_source, hs, its, n)
+
     def precompile(self, cache_dir: str, arch: str = "gfx950") -> bool:
         wc, h = C.c_int(), C.c_uint64()
         N.eng_check(_eng().gdb_plan_precompile(self._h, cache_dir.encode(), arch.encode(), C.byref(wc), C.byref(h)))

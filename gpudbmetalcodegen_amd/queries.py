@@ -138,7 +138,8 @@ def groupby_options(variant: str = "lds", lds_slots: int = 2048, filtered: bool 
 def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass: bool = False,
                 variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
                 key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
-                warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None) -> GroupByRun:
+                warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None,
+                collect: bool = True) -> GroupByRun:
     """prepass=True: the plan counts distinct keys exactly first (key-range
     bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
     `slots` (e.g. from the learned estimator) sizes it, and a reported
@@ -176,8 +177,9 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
                 if resizes > max_resizes:
                     raise
                 slots = next_pow2(max(2 * slots, 2 * int(st[2])))
-    res = ex.collect(plan)
-    return GroupByRun(res, t, plan, ex, slots=slots, resizes=resizes, distinct=res.rows(), times_ms=times)
+    res = ex.collect(plan) if collect else None
+    return GroupByRun(res, t, plan, ex, slots=slots, resizes=resizes, distinct=res.rows() if res else 0,
+                      times_ms=times)
 
 
 # ---------------------------------------------------------------------------

@@ -149,6 +149,10 @@ gdb_status gdb_batch_add(gdb_batch b, gdb_plan p, gdb_executor ex, int items, in
 gdb_status gdb_batch_build(gdb_batch b, int block, uint64_t chunk_rows, int grid, unsigned* grid_out,
                            uint64_t* work_items);
 gdb_status gdb_batch_source(gdb_batch b, char* buf, size_t cap, size_t* need);
+// Source of the batched kernel for pipelines plans[i] at items[i], without a
+// device (offline compilation with gdb_module_precompile, inspection).
+gdb_status gdb_batch_kernel_source(const gdb_plan* plans, const int* items, int n, char* buf, size_t cap,
+                                   size_t* need);
 gdb_status gdb_batch_capture(gdb_batch b);
 gdb_status gdb_batch_run(gdb_batch b, int use_graph, float* ms);
 gdb_status gdb_batch_destroy(gdb_batch b);

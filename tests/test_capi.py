@@ -17,7 +17,8 @@ HEADERS = sorted((ROOT / "include" / "gdb").glob("*.h"))
 
 PLANS = [("Q1", ""), ("Q6", ""), ("Q3", ""), ("Q3", "probe_bitmap=0"), ("Q3_SIZE", ""), ("GROUPBY", ""),
          ("GROUPBY", "variant=global,filter=1"), ("GB_COUNT", ""), ("MAXKEY", "table=G,column=g_key"),
-         ("Q13L", ""), ("Q13L", "negate=1"), ("Q22A", "")]
+         ("Q13L", ""), ("Q13L", "negate=1"), ("Q22A", ""), ("PIPE_AGG", "lds_cap=4096"), ("PIPE_SEMI", ""),
+         ("PIPE_ANTI", "")]
 
 
 def declared_functions():
@@ -118,3 +119,19 @@ def test_plan_hash_depends_on_source_and_options(tmp_path):
                                          C.byref(h), C.byref(wc)) == 0
         keys.add(h.value)
     assert len(keys) == 2
+
+
+def test_batched_kernel_compiles_with_shared_lds(tmp_path):
+    """Every ITEMS variant of a 128 KiB LDS group-by body in one persistent
+    kernel: the bodies share one LDS buffer, so the kernel fits the CU's
+    160 KiB instead of needing the sum."""
+    agg = E.build_hip_plan("PIPE_AGG", "lds_cap=4096")
+    semi, anti = E.build_hip_plan("PIPE_SEMI"), E.build_hip_plan("PIPE_ANTI")
+    plans = [agg] * 5 + [semi, anti]
+    items = [1, 2, 4, 8, 16, 4, 8]
+    src = E.HipQueryPlan.batch_kernel_source(plans, items).encode()
+    assert src.count(b"unsigned char _lds[") == 1 and b"__shared__ u64 _gl_" not in src
+    hip, _ = N.load()
+    h, wc = C.c_uint64(), C.c_int()
+    st = hip.gdb_module_precompile(str(tmp_path).encode(), b"gfx950", src, len(src), None, 0, C.byref(h), C.byref(wc))
+    assert st == 0, hip.gdb_last_error().decode()[-2000:]

@@ -110,7 +110,11 @@ def test_committed_estimator_beats_sample_scaling_and_upper_bound():
     no_resize = (slots >= g).mean()          # the table holds every group without overflow
     assert no_resize >= 0.95
     ub_slots = np.array([EST.next_pow2(2 * u) for u in ub])
-    assert np.median(slots / ub_slots) < 0.5  # memory saved against upper-bound sizing
+    exact_slots = np.array([EST.next_pow2(2 * x) for x in g])
+    loose = ub >= 4 * g                      # runs where the catalog bound is far from the truth
+    assert loose.sum() >= 20
+    assert np.median(slots[loose] / ub_slots[loose]) <= 0.25    # memory saved against upper-bound sizing
+    assert slots.sum() <= 1.3 * exact_slots.sum()              # close to exact pre-pass sizing
 
 
 # ------------------------------------------------------------------ GPU ----
