@@ -212,13 +212,14 @@ GDB_DEV i64 gdb_ht_insert_unique(u64* __restrict__ tab, u32 slot_words, u32 bits
     const u64 mask = (1ull << bits) - 1ull;
     u64 s = gdb_hash_slot(key, bits);
     const u64 mine = (u64)key | ((u64)payload << 32);
+    // CAS first: at the load factors used (<= 1/2) the home slot is usually
+    // empty, and a failed CAS returns the occupant, so no separate load.
     for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
         u64* w = tab + s * slot_words;
-        u64 cur = gdb_load_relaxed_u64(w);
-        if ((u32)cur == GDB_EMPTY_KEY) {
-            u64 exp = cur;
-            if (gdb_cas_u64(w, exp, mine)) return (i64)s;
-            cur = exp;
+        u64 cur = GDB_EMPTY_WORD;
+        if (gdb_cas_u64(w, cur, mine)) return (i64)s;
+        if ((u32)cur == GDB_EMPTY_KEY) {  // empty key, other companion bits: claim it as it is
+            if (gdb_cas_u64(w, cur, mine)) return (i64)s;
         }
         if ((u32)cur == key) return -2;
     }
