@@ -366,6 +366,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--skip-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--only", default="", help="comma-separated extra pipelines to run (w0_6M,w0_120M,w2_groupby,"
+                    "w3_batched,w4_served); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a design "
@@ -385,6 +387,8 @@ def main() -> int:
                          ("w2_groupby", lambda: bench_groupby(ctx, max(3, args.steps // 4), 1, not args.no_cpu)),
                          ("w3_batched", lambda: bench_batch(ctx, args.steps)),
                          ("w4_served", lambda: bench_serve(ctx))]:
+            if args.only and name not in args.only.split(","):
+                continue
             try:
                 pipelines[name] = fn()
             except Exception as e:  # report, keep the headline line

@@ -365,12 +365,6 @@ GDB_DEV void gdb_topk_push(GdbTopK<K>& t, u64 h, u64 l) {
         }
     }
 }
-GDB_DEV u64 gdb_shfl_xor_u64(u64 v, int mask) {
-    const int a = (gdb_lane() ^ mask) << 2;
-    const u32 lo = (u32)__builtin_amdgcn_ds_bpermute(a, (int)(u32)v);
-    const u32 hi = (u32)__builtin_amdgcn_ds_bpermute(a, (int)(u32)(v >> 32));
-    return ((u64)hi << 32) | (u64)lo;
-}
 template <int K>
 GDB_DEV void gdb_topk_pop(GdbTopK<K>& t) {
 #pragma unroll
@@ -381,47 +375,71 @@ GDB_DEV void gdb_topk_pop(GdbTopK<K>& t) {
     t.h[K - 1] = 0;
     t.l[K - 1] = 0;
 }
+// Wave-wide maximum of (h, l) pairs under gdb_rank_gt, in every lane: DPP
+// steps within each row of 16 lanes, then the four row maxima by readlane.
+// All 64 lanes must be active.
+GDB_DEV void gdb_wave_max_pair(u64& h, u64& l) {
+#define GDB_MAXPAIR_STEP(C)                                         \
+    {                                                               \
+        const u64 oh = gdb_dpp64<C>(h), ol = gdb_dpp64<C>(l);      \
+        if (gdb_rank_gt(oh, ol, h, l)) {                            \
+            h = oh;                                                 \
+            l = ol;                                                 \
+        }                                                           \
+    }
+    GDB_MAXPAIR_STEP(0xB1)
+    GDB_MAXPAIR_STEP(0x4E)
+    GDB_MAXPAIR_STEP(0x124)
+    GDB_MAXPAIR_STEP(0x128)
+#undef GDB_MAXPAIR_STEP
+    u64 bh = gdb_readlane64(h, 0), bl = gdb_readlane64(l, 0);
+#pragma unroll
+    for (int r = 16; r < 64; r += 16) {
+        const u64 xh = gdb_readlane64(h, r), xl = gdb_readlane64(l, r);
+        if (gdb_rank_gt(xh, xl, bh, bl)) {
+            bh = xh;
+            bl = xl;
+        }
+    }
+    h = bh;
+    l = bl;
+}
+// The wave's top K over every lane's sorted list: K rounds of a wave
+// argmax over the list heads; the lane owning a round's maximum pops it
+// (entries are unique: the low word carries the group key). Lane 0 writes
+// round r's entry to out[2r], out[2r + 1].
+template <int K>
+GDB_DEV void gdb_topk_wave_select(GdbTopK<K>& t, u64* out) {
+    for (int r = 0; r < K; ++r) {
+        u64 h = t.h[0], l = t.l[0];
+        gdb_wave_max_pair(h, l);
+        if ((h | l) != 0 && t.h[0] == h && t.l[0] == l) gdb_topk_pop<K>(t);
+        if (gdb_lane() == 0) {
+            out[2 * r] = h;
+            out[2 * r + 1] = l;
+        }
+    }
+}
 // Words of LDS gdb_topk_block_merge needs (blocks of up to 1024 threads).
-#define GDB_TOPK_LDS_WORDS(K) (64 + 2 * (K))
-// Block-wide top K over every thread's sorted list: K rounds of a block
-// argmax over the list heads (wave butterfly, then the per-wave maxima in
-// LDS); the thread that owns a round's maximum pops it. Entries are unique
-// (the low word carries the group key), so exactly one thread pops. On
-// return every thread's list holds the block's top K, and res (LDS, 2K
-// words) holds it as (h, l) pairs. All threads of the block must call it.
+#define GDB_TOPK_LDS_WORDS(K) (34 * (K))
+// Block-wide top K: every wave selects its top K (no barrier), wave 0
+// selects the block's top K from the waves' lists. On return every
+// thread's list holds the block's top K, and res = lds + 32K holds it as
+// (h, l) pairs. All threads of the block must call it.
 template <int K>
 GDB_DEV void gdb_topk_block_merge(GdbTopK<K>& t, u64* lds) {
     const int nw = (int)(blockDim.x >> 6);
-    u64* res = lds + 64;
-    for (int r = 0; r < K; ++r) {
-        u64 h = t.h[0], l = t.l[0];
-#pragma unroll
-        for (int m = 32; m >= 1; m >>= 1) {
-            const u64 oh = gdb_shfl_xor_u64(h, m), ol = gdb_shfl_xor_u64(l, m);
-            if (gdb_rank_gt(oh, ol, h, l)) {
-                h = oh;
-                l = ol;
-            }
+    u64* res = lds + 32 * K;
+    gdb_topk_wave_select<K>(t, lds + 2 * K * gdb_wave());
+    __syncthreads();
+    if (gdb_wave() == 0) {
+        GdbTopK<K> m;
+        gdb_topk_init<K>(m);
+        for (int e = gdb_lane(); e < nw * K; e += 64) {
+            const u64 h = lds[2 * e], l = lds[2 * e + 1];
+            if ((h | l) != 0) gdb_topk_push<K>(m, h, l);
         }
-        u64* buf = lds + (r & 1) * 32;  // double-buffered: one barrier per round
-        if (gdb_lane() == 0) {
-            buf[2 * gdb_wave()] = h;
-            buf[2 * gdb_wave() + 1] = l;
-        }
-        __syncthreads();
-        u64 bh = buf[0], bl = buf[1];
-        for (int w = 1; w < nw; ++w) {
-            const u64 xh = buf[2 * w], xl = buf[2 * w + 1];
-            if (gdb_rank_gt(xh, xl, bh, bl)) {
-                bh = xh;
-                bl = xl;
-            }
-        }
-        if ((bh | bl) != 0 && t.h[0] == bh && t.l[0] == bl) gdb_topk_pop<K>(t);
-        if (threadIdx.x == 0) {
-            res[2 * r] = bh;
-            res[2 * r + 1] = bl;
-        }
+        gdb_topk_wave_select<K>(m, res);
     }
     __syncthreads();
 #pragma unroll

@@ -197,12 +197,13 @@ void HipHashTableScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                 cg.addLine("const u64 _slot = _t0 + (u64)threadIdx.x * ITEMS + (u64)_u;");
                 cg.addBlock("if (_slot < " + n + ")", [&]() {
                     cg.addLine("const u64* _sw = " + table_ + " + _slot * " + s(slotWords_) + ";");
-                    cg.addLine("const u64 _w0 = _sw[0];");
+                    // whole slot in one go (16-byte loads), read once: nontemporal
+                    cg.addLine("const u64 _w0 = __builtin_nontemporal_load(_sw);");
+                    for (int w = 1; w < slotWords_; ++w)
+                        cg.addLine("const u64 " + prefix_ + "_w" + s(w) + " = __builtin_nontemporal_load(_sw + " + s(w) + ");");
                     cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
                         cg.addLine("const u32 " + prefix_ + "_key = (u32)_w0;");
                         cg.addLine("const u32 " + prefix_ + "_payload = (u32)(_w0 >> 32);");
-                        for (int w = 1; w < slotWords_; ++w)
-                            cg.addLine("const u64 " + prefix_ + "_w" + s(w) + " = _sw[" + s(w) + "];");
                         consume();
                     });
                 });
@@ -937,7 +938,7 @@ void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.declareLds("u64", "_tk_lds", "GDB_TOPK_LDS_WORDS(" + K + ")");
         cg.declareLds("u64", "_tk_sh", "2");
         cg.addLine("gdb_topk_block_merge<" + K + ">(_tk, _tk_lds);");
-        cg.addLine("const u64* _tk_res = _tk_lds + 64;");
+        cg.addLine("const u64* _tk_res = _tk_lds + 32 * " + K + ";");
         cg.addBlock("if (threadIdx.x == 0)", [&]() {
             cg.addLine("const u64 _thr = gdb_load_relaxed_u64(" + ctl + " + 2);");
             cg.addLine("u64 _cnt = 0;");
