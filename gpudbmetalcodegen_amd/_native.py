@@ -46,6 +46,11 @@ class DeviceInfo(C.Structure):
     ]
 
 
+class ChunkStats(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("stream_ms", C.c_double), ("chunks", C.c_uint64),
+                ("bytes_copied", C.c_uint64), ("first_stream_phase", C.c_int), ("last_stream_phase", C.c_int)]
+
+
 class ExecTiming(C.Structure):
     _fields_ = [
         ("total_ms", C.c_float), ("alloc_ms", C.c_float), ("nphases", C.c_int),
@@ -158,6 +163,8 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_executor_execute", [vp, vp, i32, i32, P(ExecTiming)])
     _sig(eng, "gdb_executor_execute_range", [vp, vp, i32, i32, i32, i32, P(ExecTiming)])
     _sig(eng, "gdb_executor_prepare", [vp, vp])
+    _sig(eng, "gdb_executor_execute_streamed", [vp, vp, cstr, u64, i32, P(cstr), P(vp), P(i32), u64, P(ChunkStats)])
+    _sig(eng, "gdb_plan_chunkable", [vp, P(i32)])
     _sig(eng, "gdb_executor_enqueue", [vp, vp])
     _sig(eng, "gdb_executor_check_status", [vp, vp])
     _sig(eng, "gdb_executor_read_buffer", [vp, cstr, sz, vp, sz])

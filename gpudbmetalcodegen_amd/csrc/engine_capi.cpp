@@ -9,6 +9,7 @@
 #include "planning/hip_plan_builder.h"
 #include "runtime/autotuner.h"
 #include "runtime/batcher.h"
+#include "runtime/streaming.h"
 #include "runtime/estimator.h"
 
 #include <algorithm>
@@ -385,6 +386,40 @@ gdb_status gdb_executor_execute_range(gdb_executor ex, gdb_plan p, int first, in
 
 gdb_status gdb_executor_execute(gdb_executor ex, gdb_plan p, int warmup, int measured, gdb_exec_timing* out) {
     return gdb_executor_execute_range(ex, p, 0, -1, warmup, measured, out);
+}
+
+gdb_status gdb_executor_execute_streamed(gdb_executor ex, gdb_plan p, const char* table, uint64_t rows, int ncols,
+                                         const char* const* names, const void* const* host_cols,
+                                         const int* elem_bytes, uint64_t chunk_rows, gdb_chunk_stats* out) {
+    return guard([&]() -> gdb_status {
+        if (!p->compiled || p->ctx != ex->ctx)
+            return setErr(GDB_ERR_INVALID, "gdb_executor_execute_streamed: plan not compiled for this context");
+        if (!table || ncols < 1 || !names || !host_cols || !elem_bytes)
+            return setErr(GDB_ERR_INVALID, "gdb_executor_execute_streamed: no stream columns");
+        std::vector<runtime::StreamColumn> cols;
+        for (int i = 0; i < ncols; ++i) cols.push_back({names[i], host_cols[i], elem_bytes[i]});
+        ex->ex->registerTable(table, rows);  // row count known to the plan before the pre-stream phases
+        registerPlanTables(ex, p);
+        installSizingRules(ex, p);
+        runtime::ChunkStreamer cs(ex->ctx, *ex->ex, p->plan, p->cg, compilerFor(ex->ctx), p->cq);
+        const auto st = cs.run(table, rows, cols, chunk_rows);
+        if (out) {
+            out->total_ms = st.totalMs;
+            out->stream_ms = st.streamMs;
+            out->chunks = st.chunks;
+            out->bytes_copied = st.bytesCopied;
+            out->first_stream_phase = st.firstStreamPhase;
+            out->last_stream_phase = st.lastStreamPhase;
+        }
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_plan_chunkable(gdb_plan p, int* chunkable) {
+    return guard([&]() -> gdb_status {
+        *chunkable = p->plan.chunkable ? 1 : 0;
+        return GDB_OK;
+    });
 }
 
 gdb_status gdb_executor_prepare(gdb_executor ex, gdb_plan p) {

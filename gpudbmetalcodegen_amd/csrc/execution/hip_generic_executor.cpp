@@ -164,8 +164,14 @@ void HipGenericExecutor::launchPhase(RuntimeCompiler& rc, const RuntimeCompiler:
             case HipParamKind::DeviceBuffer: {
                 Buffer& buf = ensureBuffer(b);
                 if (!skipInit_ && !initDone[b.name]) {
+                    // A buffer is initialised right before the phase that
+                    // defines it (the first phase binding it with an init
+                    // kind), so a run over a phase range never re-fills a
+                    // buffer an earlier phase built.
                     auto sp = initSpec.find(b.name);
-                    if (sp != initSpec.end()) {
+                    const bool owner = sp != initSpec.end() && sp->second >= p.bindings.data() &&
+                                       sp->second < p.bindings.data() + p.bindings.size();
+                    if (owner) {
                         const HipParamBinding* ib = sp->second;
                         const size_t bytes = static_cast<size_t>(sizes_.resolve(ib->sizeExpr)) * ib->elemBytes;
                         if (ib->init == InitKind::Zero) {
@@ -176,8 +182,8 @@ void HipGenericExecutor::launchPhase(RuntimeCompiler& rc, const RuntimeCompiler:
                                                  stream_),
                                   "fill " + b.name);
                         }
+                        initDone[b.name] = true;
                     }
-                    initDone[b.name] = true;
                 }
                 args.push_back(reinterpret_cast<uint64_t>(buf.ptr));
                 break;

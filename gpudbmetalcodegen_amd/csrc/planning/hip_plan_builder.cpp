@@ -102,6 +102,7 @@ HipQueryPlan buildQ1(const PlanOptions& o) {
     ph.launchBound = 256;
     ph.scalarParams = {"q1_cutoff"};
     plan.requiredScalars = {"q1_cutoff"};
+    plan.chunkable = true;
     return plan;
 }
 
@@ -121,6 +122,7 @@ HipQueryPlan buildQ6(const PlanOptions& o) {
     ph.launchBound = 256;
     ph.scalarParams = {"q6_date_lo", "q6_date_hi", "q6_disc_lo", "q6_disc_hi", "q6_qty_hi"};
     plan.requiredScalars = ph.scalarParams;
+    plan.chunkable = true;
     return plan;
 }
 
@@ -209,6 +211,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
     plan.requiredScalars = {"q3_seg", "q3_date"};
     plan.requiredSymbols = {"a_bits", "b_bits"};
     if (!prepass) plan.requiredSymbols.push_back("b_slots");
+    plan.chunkable = true;
     return plan;
 }
 
@@ -247,6 +250,7 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     } else {
         plan.requiredSymbols = {"gb_slots"};
     }
+    plan.chunkable = !prepass;  // the pre-pass needs every row before the table exists
     return plan;
 }
 
@@ -330,6 +334,7 @@ HipQueryPlan buildQ13L(const PlanOptions& o) {
     }
     plan.requiredScalars = {"like_l1", "like_l2"};
     plan.requiredSymbols = {"a_bits"};
+    plan.chunkable = true;
     return plan;
 }
 
@@ -359,6 +364,7 @@ HipQueryPlan buildQ22A(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q22_date_lo", "q22_date_hi"};
     plan.requiredSymbols = {"a_bits"};
+    plan.chunkable = true;
     return plan;
 }
 
@@ -382,6 +388,7 @@ HipQueryPlan buildPipeAgg(const PlanOptions& o) {
     ph.scalarParams = {"pipe_date_lo", "pipe_date_hi", "pipe_flag"};
     plan.requiredScalars = {"pipe_date_lo", "pipe_date_hi", "pipe_flag", "pipe_lds_bits"};
     plan.requiredSymbols = {"pipe_slots"};
+    plan.chunkable = true;
     return plan;
 }
 
@@ -400,6 +407,7 @@ HipQueryPlan buildPipeJoin(const PlanOptions& o, bool anti) {
     ph.launchBound = 1024;
     ph.staged = optInt(o, "staged", 1) != 0;
     plan.requiredSymbols = {"pipe_bits"};
+    plan.chunkable = true;
     return plan;
 }
 

@@ -43,6 +43,12 @@ struct HipQueryPlan {
     };
     std::vector<Phase> phases;
     std::vector<std::string> helpers;
+    // Out-of-core eligible (metal_plan_builder.cpp:2078-2107's allowlist):
+    // every phase that scans the stream table updates its outputs only by
+    // associative atomics, so running those phases once per row chunk (with
+    // buffers initialised before the first chunk only) gives the result of
+    // one run over the whole table.
+    bool chunkable = false;
 
     // Runner-owned buffers that a later phase scans as a table column:
     // (table, column, buffer, rows expression).

@@ -198,6 +198,12 @@ class HipQueryPlan:
         its = (C.c_int * n)(*items)
         return N.two_call_string(_eng().gdb_batch_kernel_source, hs, its, n)
 
+    @property
+    def chunkable(self) -> bool:
+        v = C.c_int()
+        N.eng_check(_eng().gdb_plan_chunkable(self._h, C.byref(v)))
+        return bool(v.value)
+
     def precompile(self, cache_dir: str, arch: str = "gfx950") -> bool:
         wc, h = C.c_int(), C.c_uint64()
         N.eng_check(_eng().gdb_plan_precompile(self._h, cache_dir.encode(), arch.encode(), C.byref(wc), C.byref(h)))
@@ -232,6 +238,47 @@ class GenericResult:
         return 0 if not self.columns else len(next(iter(self.columns.values())))
 
 
+class HostTable:
+    """A table kept in host memory for streamed (out-of-core) execution.
+    pinned=True copies the columns into page-locked memory so the chunk
+    copies run at link speed and overlap the kernels."""
+
+    def __init__(self, name: str, cols: dict, pinned: bool = True):
+        self.name = name
+        self.rows = int(next(iter(cols.values())).shape[0]) if cols else 0
+        self._pinned = []
+        self.columns = {}
+        hip, _ = N.load()
+        for k, a in cols.items():
+            a = np.ascontiguousarray(a)
+            if pinned and a.nbytes:
+                p = C.c_void_p()
+                N.hip_check(hip.gdb_host_alloc_pinned(a.nbytes, C.byref(p)))
+                self._pinned.append(p)
+                buf = (C.c_uint8 * a.nbytes).from_address(p.value)
+                view = np.frombuffer(buf, dtype=a.dtype).reshape(a.shape)
+                view[...] = a
+                self.columns[k] = view
+            else:
+                self.columns[k] = a
+
+    def nbytes(self) -> int:
+        return sum(a.nbytes for a in self.columns.values())
+
+    def close(self) -> None:
+        hip, _ = N.load()
+        self.columns = {}
+        for p in self._pinned:
+            hip.gdb_host_free_pinned(p)
+        self._pinned = []
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class HipGenericExecutor:
     def __init__(self, ctx: Context, stream=None):
         self.ctx = ctx
@@ -260,6 +307,20 @@ class HipGenericExecutor:
 
     def register_scalar(self, name: str, value: int) -> None:
         N.eng_check(_eng().gdb_executor_register_scalar(self._h, name.encode(), int(value)))
+
+    def execute_streamed(self, plan: "HipQueryPlan", table: "HostTable", chunk_rows: int) -> N.ChunkStats:
+        """Out-of-core run: `table` stays in host memory and is streamed into
+        HBM in chunks of chunk_rows rows (phases scanning it run per chunk)."""
+        names = list(table.columns)
+        n = len(names)
+        cn = (C.c_char_p * n)(*[x.encode() for x in names])
+        ptrs = (C.c_void_p * n)(*[table.columns[x].ctypes.data for x in names])
+        eb = (C.c_int * n)(*[table.columns[x].itemsize * (table.columns[x].shape[1] if table.columns[x].ndim == 2 else 1)
+                             for x in names])
+        st = N.ChunkStats()
+        N.eng_check(_eng().gdb_executor_execute_streamed(self._h, plan.handle, table.name.encode(), table.rows, n, cn,
+                                                         ptrs, eb, int(chunk_rows), C.byref(st)))
+        return st
 
     def set_skip_init(self, skip: bool) -> None:
         N.eng_check(_eng().gdb_executor_set_skip_init(self._h, 1 if skip else 0))

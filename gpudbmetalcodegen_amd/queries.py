@@ -33,6 +33,28 @@ class QueryRun:
     timing: E.Timing
     plan: E.HipQueryPlan
     executor: E.HipGenericExecutor
+    stream_stats: object = None
+
+
+@dataclass
+class Stream:
+    """Out-of-core form: this host table is streamed into HBM in chunks of
+    chunk_rows rows instead of being read from a resident copy."""
+    table: E.HostTable
+    chunk_rows: int
+
+
+def _register(ex: E.HipGenericExecutor, dev: dict, names, stream: Stream | None) -> None:
+    for t in names:
+        if stream is not None and t == stream.table.name:
+            continue
+        ex.register_table(dev[t])
+
+
+def _run(ex: E.HipGenericExecutor, plan: E.HipQueryPlan, warmup: int, measured: int, stream: Stream | None):
+    if stream is None:
+        return ex.execute(plan, warmup, measured), None
+    return None, ex.execute_streamed(plan, stream.table, stream.chunk_rows)
 
 
 _plan_cache: dict = {}
@@ -59,13 +81,13 @@ def upload_tables(ctx: E.Context, tables: dict) -> dict:
 # Workload 0: Q1
 # ---------------------------------------------------------------------------
 def run_q1(ctx: E.Context, dev: dict, cutoff: int, warmup: int = 0, measured: int = 1,
-           options: str = "", executor: E.HipGenericExecutor | None = None) -> QueryRun:
+           options: str = "", executor: E.HipGenericExecutor | None = None, stream: Stream | None = None) -> QueryRun:
     plan = get_plan(ctx, "Q1", options)
     ex = executor or E.HipGenericExecutor(ctx)
-    ex.register_table(dev["F"])
+    _register(ex, dev, ["F"], stream)
     ex.register_scalar("q1_cutoff", cutoff)
-    t = ex.execute(plan, warmup, measured)
-    return QueryRun(ex.collect(plan), t, plan, ex)
+    t, st = _run(ex, plan, warmup, measured, stream)
+    return QueryRun(ex.collect(plan), t, plan, ex, st)
 
 
 def _executor(ctx, executor):
@@ -77,15 +99,14 @@ def _executor(ctx, executor):
 # ---------------------------------------------------------------------------
 def run_q3(ctx: E.Context, dev: dict, seg: int = W.W1_SEG, d: int = W.W1_DATE, options: str = "",
            max_a: int | None = None, max_b: int | None = None, b_slots: int | None = None, warmup: int = 0,
-           measured: int = 1, executor: E.HipGenericExecutor | None = None) -> QueryRun:
+           measured: int = 1, executor: E.HipGenericExecutor | None = None, stream: Stream | None = None) -> QueryRun:
     """Plain form: the plan's own exact-count pre-pass sizes the B hash table
     (options ``sizing=given`` takes ``b_slots`` instead). Key-range symbols
     come from table statistics (max key), as the reference's cached max-key
     mode does (codegen/execution/max_key_symbols.cpp:28-41)."""
     plan = get_plan(ctx, "Q3", options)
     ex = _executor(ctx, executor)
-    for t in ("A", "B", "F"):
-        ex.register_table(dev[t])
+    _register(ex, dev, ("A", "B", "F"), stream)
     ex.register_scalar("q3_seg", seg)
     ex.register_scalar("q3_date", d)
     ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
@@ -94,8 +115,8 @@ def run_q3(ctx: E.Context, dev: dict, seg: int = W.W1_SEG, d: int = W.W1_DATE, o
         if b_slots is None:
             raise ValueError("sizing=given needs b_slots")
         ex.register_symbol("b_slots", b_slots)
-    t = ex.execute(plan, warmup, measured)
-    return QueryRun(ex.collect(plan), t, plan, ex)
+    t, st = _run(ex, plan, warmup, measured, stream)
+    return QueryRun(ex.collect(plan), t, plan, ex, st)
 
 
 def q3_hash_groups(run: QueryRun) -> dict:
@@ -139,7 +160,7 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
                 variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
                 key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
                 warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None,
-                collect: bool = True) -> GroupByRun:
+                collect: bool = True, stream: Stream | None = None) -> GroupByRun:
     """prepass=True: the plan counts distinct keys exactly first (key-range
     bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
     `slots` (e.g. from the learned estimator) sizes it, and a reported
@@ -147,17 +168,18 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
     opts = groupby_options(variant, lds_slots, val_cut is not None, prepass, block)
     plan = get_plan(ctx, "GROUPBY", opts)
     ex = _executor(ctx, executor)
-    ex.register_table(dev["G"])
+    _register(ex, dev, ["G"], stream)
     if val_cut is not None:
         ex.register_scalar("gb_val_cut", val_cut)
     times = []
     resizes = 0
+    st = None
     if prepass:
         if key_max is None:
             raise ValueError("prepass sizing needs the key range (key_min, key_max)")
         ex.register_scalar("gb_key_min", key_min)
         ex.register_symbol("gb_key_bits", key_max - key_min + 1)
-        t = ex.execute(plan, warmup, measured)
+        t, st = _run(ex, plan, warmup, measured, stream)   # a streamed pre-pass plan is refused by the engine
         times.append(t.total_ms)
         slots = ex.get_symbol("gb_slots")
     else:
@@ -167,8 +189,8 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
         while True:
             ex.register_symbol("gb_slots", slots)
             try:
-                t = ex.execute(plan, warmup, measured)
-                times.append(t.total_ms)
+                t, st = _run(ex, plan, warmup, measured, stream)
+                times.append(t.total_ms if t else st.total_ms)
                 break
             except E.GdbOverflow:
                 st = ex.read_buffer("d_gb_ht_status", 32, np.uint64)
@@ -178,7 +200,7 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
                     raise
                 slots = next_pow2(max(2 * slots, 2 * int(st[2])))
     res = ex.collect(plan) if collect else None
-    return GroupByRun(res, t, plan, ex, slots=slots, resizes=resizes, distinct=res.rows() if res else 0,
+    return GroupByRun(res, t, plan, ex, st, slots=slots, resizes=resizes, distinct=res.rows() if res else 0,
                       times_ms=times)
 
 
@@ -196,11 +218,10 @@ def like_words_buffer(ctx: E.Context, w1: bytes, w2: bytes) -> int:
 
 def run_q13(ctx: E.Context, dev: dict, w1: bytes, w2: bytes, negate: bool = False, max_a: int | None = None,
             words_ptr: int | None = None, warmup: int = 0, measured: int = 1,
-            executor: E.HipGenericExecutor | None = None) -> QueryRun:
+            executor: E.HipGenericExecutor | None = None, stream: Stream | None = None) -> QueryRun:
     plan = get_plan(ctx, "Q13L", "negate=1" if negate else "")
     ex = _executor(ctx, executor)
-    ex.register_table(dev["S"])
-    ex.register_table(dev["A"])
+    _register(ex, dev, ("S", "A"), stream)
     own = words_ptr is None
     wp = like_words_buffer(ctx, w1, w2) if own else words_ptr
     try:
@@ -208,25 +229,24 @@ def run_q13(ctx: E.Context, dev: dict, w1: bytes, w2: bytes, negate: bool = Fals
         ex.register_scalar("like_l1", len(w1))
         ex.register_scalar("like_l2", len(w2))
         ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
-        t = ex.execute(plan, warmup, measured)
+        t, st = _run(ex, plan, warmup, measured, stream)
         res = ex.collect(plan)
     finally:
         if own:
             ctx.free(wp)
-    return QueryRun(res, t, plan, ex)
+    return QueryRun(res, t, plan, ex, st)
 
 
 def run_q22(ctx: E.Context, dev: dict, date_lo: int, date_hi: int, max_a: int | None = None, warmup: int = 0,
-            measured: int = 1, executor: E.HipGenericExecutor | None = None) -> QueryRun:
+            measured: int = 1, executor: E.HipGenericExecutor | None = None, stream: Stream | None = None) -> QueryRun:
     plan = get_plan(ctx, "Q22A")
     ex = _executor(ctx, executor)
-    ex.register_table(dev["B"])
-    ex.register_table(dev["A"])
+    _register(ex, dev, ("B", "A"), stream)
     ex.register_scalar("q22_date_lo", date_lo)
     ex.register_scalar("q22_date_hi", date_hi)
     ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
-    t = ex.execute(plan, warmup, measured)
-    return QueryRun(ex.collect(plan), t, plan, ex)
+    t, st = _run(ex, plan, warmup, measured, stream)
+    return QueryRun(ex.collect(plan), t, plan, ex, st)
 
 
 def result_as_lists(res: E.GenericResult) -> dict:

@@ -105,6 +105,27 @@ gdb_status gdb_executor_execute(gdb_executor ex, gdb_plan p, int warmup, int mea
 gdb_status gdb_executor_execute_range(gdb_executor ex, gdb_plan p, int first, int last, int warmup, int measured,
                                       gdb_exec_timing* out);
 /* Allocate, then launch without synchronising (graph capture, overlap). */
+/* Out-of-core run of a chunkable plan: table `table` (`rows` rows) is read
+ * from host memory (ncols columns: names, host pointers, element bytes) in
+ * chunks of chunk_rows rows, double-buffered into HBM on a copy stream while
+ * the previous chunk's kernels run; phases before the stream table's run
+ * once, its phases once per chunk, later phases once. Other tables, scalars
+ * and symbols are registered on the executor as for execute. Host columns
+ * should be pinned (gdb_host_alloc_pinned) for the copies to overlap.
+ * Replaces the chunk loop of codegen/codegen_main.cpp:686-808 over
+ * ChunkedColbinTable (codegen/execution/chunked_colbin_loader.h:12-60). */
+typedef struct {
+    double total_ms;     /* host wall time of the whole run, synchronised */
+    double stream_ms;    /* device time, first chunk copy -> last stream-phase kernel */
+    uint64_t chunks;
+    uint64_t bytes_copied;
+    int first_stream_phase, last_stream_phase;
+} gdb_chunk_stats;
+gdb_status gdb_executor_execute_streamed(gdb_executor ex, gdb_plan p, const char* table, uint64_t rows, int ncols,
+                                         const char* const* names, const void* const* host_cols,
+                                         const int* elem_bytes, uint64_t chunk_rows, gdb_chunk_stats* out);
+/* 1 when the plan may be streamed in chunks (metal_plan_builder.cpp:2078-2107). */
+gdb_status gdb_plan_chunkable(gdb_plan p, int* chunkable);
 gdb_status gdb_executor_prepare(gdb_executor ex, gdb_plan p);
 gdb_status gdb_executor_enqueue(gdb_executor ex, gdb_plan p);
 gdb_status gdb_executor_check_status(gdb_executor ex, gdb_plan p);

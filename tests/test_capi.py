@@ -135,3 +135,14 @@ def test_batched_kernel_compiles_with_shared_lds(tmp_path):
     h, wc = C.c_uint64(), C.c_int()
     st = hip.gdb_module_precompile(str(tmp_path).encode(), b"gfx950", src, len(src), None, 0, C.byref(h), C.byref(wc))
     assert st == 0, hip.gdb_last_error().decode()[-2000:]
+
+
+def test_chunkable_plans():
+    """Out-of-core eligibility follows the associative-output rule: plans
+    whose stream-table phases only accumulate are chunkable; a plan whose
+    sizing pre-pass needs the whole table first is not."""
+    for name, opts in [("Q1", ""), ("Q6", ""), ("Q3", ""), ("Q13L", ""), ("Q22A", ""), ("GROUPBY", ""),
+                       ("PIPE_AGG", ""), ("PIPE_SEMI", "")]:
+        assert E.build_hip_plan(name, opts).chunkable, name
+    for name, opts in [("GROUPBY", "sizing=prepass"), ("GB_COUNT", ""), ("MAXKEY", "table=G,column=g_key")]:
+        assert not E.build_hip_plan(name, opts).chunkable, name
