@@ -240,8 +240,9 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     G = W.workload2()["G"]
     dev = Q.upload_tables(ctx, {"G": G})
     out = {"workload": f"hash group-by sum/count/min/max, {n} rows, Zipf(1.1) keys over 1..{nkeys}", "rows": n}
-    # plain form: exact distinct-count pre-pass sizes the table (2 extra kernels + a host sync)
-    run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=nkeys)
+    # plain form: exact distinct-count pre-pass sizes the table (2 extra kernels + a host sync),
+    # then the two-pass partitioned group-by
+    run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=nkeys, variant="part", lds_slots=4096)
     ex, plan = run.executor, run.plan
     ex.execute(plan, warmup, 1)
     ctx.sync()
@@ -252,8 +253,17 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     phases = {name: ms for name, ms, *_ in t.phases}
     out["plain"] = {"ms_per_step": plain_ms, "rows_per_s": n / (plain_ms / 1e3), "slots": run.slots,
                     "phase_ms": {k: round(v, 4) for k, v in phases.items()}}
-    gb_bytes = 12.0 * n + 32.0 * groups
-    out["roofline"] = roofline("GB_agg", phases["GB_agg"], gb_bytes, "w2")
+    # pass 1 reads every row and writes each cold row once; pass 2 reads the
+    # cold rows and hot partials and writes the table once
+    st = t.status
+    cold = int(st[-2][3]) if len(st) >= 2 else 0
+    p1_bytes = 12.0 * n + 12.0 * cold
+    p2_bytes = 12.0 * cold + 32.0 * run.slots
+    dom = max(("GBP_scatter", "GBP_agg"), key=lambda k: phases[k])
+    out["roofline"] = roofline(dom, phases[dom], p1_bytes if dom == "GBP_scatter" else p2_bytes, "w2")
+    out["phase_roofline_frac"] = {"GBP_scatter": round(p1_bytes / (phases["GBP_scatter"] / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
+                                  "GBP_agg": round(p2_bytes / (phases["GBP_agg"] / 1e3) / 1e9 / PEAK_HBM_GBS, 4)}
+    out["cold_rows"] = cold
     plain_res = run.result
     ex.release()
     # learned form: sample + MLP inference size the table and pick the variant; one pass
@@ -288,7 +298,8 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
         learned = {"ms_per_step": learned_ms, "rows_per_s": n / (learned_ms / 1e3), "slots": r2.slots,
                    "variant": sz.variant, "predicted_groups": round(sz.groups_pred), "true_groups": groups,
                    "resizes": resizes, "identical_to_plain": bool(same),
-                   "gb_kernel_ms": round(r2.timing.phases[-1][1], 4)}
+                   "lds_slots": sz.lds_slots,
+                   "kernel_ms": {name: round(ms, 4) for name, ms, *_ in r2.timing.phases}}
         r2.executor.release()
     out["learned"] = learned
     out["rows_per_s"] = (learned or out["plain"])["rows_per_s"]

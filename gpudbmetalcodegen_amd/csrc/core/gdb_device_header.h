@@ -194,6 +194,22 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
     if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(w, m);
 }
 
+// gdb_bitmap_set behind a block-local cache (1024 entries of LDS u64:
+// word index << 32 | bits this block has set in it). An entry is read and
+// written as one 64-bit LDS access; a lost update only costs a redundant
+// atomic, and a bit is recorded only after its atomic was issued.
+GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64* cache) {
+    if ((u64)key >= nbits) return;
+    const u32 w = (u32)((u64)key >> 5);
+    const u32 m = 1u << ((u32)key & 31u);
+    u64* e = cache + (w & 1023u);
+    const u64 c = *(volatile u64*)e;
+    const bool same = (u32)(c >> 32) == w;
+    if (same && ((u32)c & m)) return;
+    gdb_bitmap_set(bm, nbits, key);
+    *(volatile u64*)e = ((u64)w << 32) | (u64)((same ? (u32)c : 0u) | m);
+}
+
 // ---------------------------------------------------------------------------
 // Open-addressing hash tables over 64-bit words. Word 0 of a slot packs the
 // 32-bit key (low half, empty = 0xFFFFFFFF) with a 32-bit companion (high
@@ -296,6 +312,35 @@ GDB_DEV int gdb_lds_gb_update(u64* tab, u32 bits, u32 key, i64 v, u32 max_probe)
     }
     return -1;
 }
+
+// Merge a partial group (cnt rows, sum/min/max) into an LDS table of
+// (mask + 1) slots starting at slot s. Returns the slot or -1 after
+// max_probe slots.
+GDB_DEV int gdb_lds_gb_merge_at(u64* tab, u32 mask, u32 s, u32 key, u32 cnt, i64 sum, i64 mn, i64 mx, u32 max_probe) {
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        u64* w = tab + (u64)s * 4;
+        u64 cur = gdb_lds_load_u64(w);
+        bool inserted = false;
+        if ((u32)cur == GDB_EMPTY_KEY) {
+            u64 exp = cur;
+            if (gdb_lds_cas_u64(w, exp, (u64)key | ((u64)cnt << 32))) {
+                inserted = true;
+            } else {
+                cur = exp;
+            }
+        }
+        if (inserted || (u32)cur == key) {
+            if (!inserted) gdb_lds_add_u64(w, (u64)cnt << 32);
+            gdb_lds_add_u64(w + 1, (u64)sum);
+            gdb_lds_min_i64((i64*)(w + 2), mn);
+            gdb_lds_max_i64((i64*)(w + 3), mx);
+            return (int)s;
+        }
+    }
+    return -1;
+}
+// Hash partition of a key: the top pbits of key * 2654435769 (0 when pbits = 0).
+GDB_DEV u32 gdb_partition(u32 key, u32 pbits) { return pbits ? (u32)((key * GDB_KNUTH) >> (32u - pbits)) : 0u; }
 
 // ---------------------------------------------------------------------------
 // SQL LIKE '%w1%w2%' on a fixed-width NUL-padded field: the value ends at the

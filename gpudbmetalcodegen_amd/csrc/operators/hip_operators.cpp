@@ -21,6 +21,40 @@ constexpr uint64_t kI64Min = 0x8000000000000000ull;
 
 std::string s(long long v) { return std::to_string(v); }
 
+// One term of a status-word update: per-thread counter `var` added into word
+// `word`, or (flag) word set to 1 when any thread's counter is nonzero.
+struct StatusTerm {
+    std::string var;
+    int word;
+    bool flag;
+};
+
+// Status words updated once per block: wave sums into LDS, then thread 0
+// issues one global atomic per nonzero word. (Per-wave global atomics on the
+// same few words serialise at the memory-side atomic unit.) Every thread of
+// the block must reach this code.
+void emitBlockStatus(HipCodegen& cg, const std::string& st, const std::vector<StatusTerm>& terms) {
+    cg.addBlock("", [&]() {
+        cg.declareLds("u64", "_stl", "4");
+        cg.addLine("if (threadIdx.x < 4) _stl[threadIdx.x] = 0;");
+        cg.addLine("__syncthreads();");
+        for (const auto& t : terms) {
+            cg.addLine("{ const u64 _v = gdb_wave_sum_u64(" + t.var + "); if (gdb_lane() == 0 && _v) gdb_lds_add_u64(&_stl[" +
+                       s(t.word) + "], _v); }");
+        }
+        cg.addLine("__syncthreads();");
+        cg.addBlock("if (threadIdx.x == 0)", [&]() {
+            for (const auto& t : terms) {
+                const std::string w = s(t.word);
+                if (t.flag)
+                    cg.addLine("if (_stl[" + w + "]) gdb_atomic_max_u64(" + st + " + " + w + ", 1ull);");
+                else
+                    cg.addLine("if (_stl[" + w + "]) gdb_atomic_add_u64(" + st + " + " + w + ", _stl[" + w + "]);");
+            }
+        });
+    });
+}
+
 }  // namespace
 
 void HipOperator::produce(HipCodegen& cg, ConsumerFn consume) {
@@ -264,8 +298,16 @@ HipBitmapBuild::HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::st
 
 void HipBitmapBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapWriteParam(bitmap_, bits_);
+    // Per-block direct-mapped cache of recently set bitmap words (word index
+    // in the high half, the bits this block set in the low half, one 64-bit
+    // LDS word so an entry is never torn): repeated keys (skewed or foreign-
+    // key columns) skip the global atomic after their first set in a block.
+    const std::string C = "_bmc_" + bitmap_;
+    cg.declareLds("u64", C, "1024");
+    cg.addLine("for (int _c = (int)threadIdx.x; _c < 1024; _c += (int)blockDim.x) " + C + "[_c] = ~0ull;");
+    cg.addLine("__syncthreads();");
     child_->produce(cg, [&]() {
-        cg.addLine("gdb_bitmap_set(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" + key_ + "));");
+        cg.addLine("gdb_bitmap_set_cached(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" + key_ + "), " + C + ");");
         consume();
     });
 }
@@ -338,16 +380,7 @@ void HipHashBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         consume();
     });
     cg.addComment("build status: overflow / duplicate flags and inserted-key count");
-    cg.addBlock("", [&]() {
-        cg.addLine("const u64 _a = gdb_wave_sum_u64(_ins_" + table_ + ");");
-        cg.addLine("const u64 _b = gdb_wave_sum_u64(_ovf_" + table_ + ");");
-        cg.addLine("const u64 _c = gdb_wave_sum_u64(_dup_" + table_ + ");");
-        cg.addBlock("if (gdb_lane() == 0)", [&]() {
-            cg.addLine("if (_a) gdb_atomic_add_u64(" + st + " + 2, _a);");
-            cg.addLine("if (_b) gdb_atomic_max_u64(" + st + " + 0, 1ull);");
-            cg.addLine("if (_c) gdb_atomic_max_u64(" + st + " + 1, 1ull);");
-        });
-    });
+    emitBlockStatus(cg, st, {{"_ins_" + table_, 2, false}, {"_ovf_" + table_, 0, true}, {"_dup_" + table_, 1, true}});
 }
 
 std::string HipHashBuild::describe() const {
@@ -877,16 +910,7 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         });
     }
     cg.addComment("group-by status: [0] overflow flag, [2] keys inserted, [3] rows spilled from LDS");
-    cg.addBlock("", [&]() {
-        cg.addLine("const u64 _a = gdb_wave_sum_u64(_ins_" + T + ");");
-        cg.addLine("const u64 _b = gdb_wave_sum_u64(_ovf_" + T + ");");
-        cg.addLine("const u64 _c = gdb_wave_sum_u64(_spill_" + T + ");");
-        cg.addBlock("if (gdb_lane() == 0)", [&]() {
-            cg.addLine("if (_a) gdb_atomic_add_u64(" + st + " + 2, _a);");
-            cg.addLine("if (_b) gdb_atomic_max_u64(" + st + " + 0, 1ull);");
-            cg.addLine("if (_c) gdb_atomic_add_u64(" + st + " + 3, _c);");
-        });
-    });
+    emitBlockStatus(cg, st, {{"_ins_" + T, 2, false}, {"_ovf_" + T, 0, true}, {"_spill_" + T, 3, false}});
 
     auto& rs = cg.resultSchema();
     rs.kind = HipResultSchema::HASH_AGG;
@@ -903,6 +927,197 @@ std::string HipHashGroupBy::describe() const {
     return std::string("HashGroupBy(") + table_ + ", key=" + key_ + ", value=" + value_ + ", slots=" + slots_ +
            ", variant=" + (variant_ == LDS ? "lds" + s(ldsSlots_) : "global") + ")";
 }
+
+// ===========================================================================
+// Partitioned hash group-by (two passes)
+// ===========================================================================
+
+namespace {
+
+// Size expressions shared by both passes of a partitioned group-by.
+struct PartSizes {
+    std::string parts, cap, pcap;
+};
+
+PartSizes partSizes(const std::string& slots, int ldsSlots) {
+    PartSizes p;
+    const std::string PS = s(HipPartitionScatter::kPartSlots);
+    p.parts = "(" + slots + ") / " + PS;
+    // cold rows per partition: rows * slack% / partitions, plus headroom
+    p.cap = "gbp_rows * gbp_slack / 100 / ((" + slots + ") / " + PS + ") + 1024";
+    // hot partials per partition: at most blocks x LDS slots in total
+    p.pcap = s(static_cast<long long>(HipPartitionScatter::kMaxBlocks) * ldsSlots) + " / ((" + slots + ") / " + PS +
+             ") * 2 + 256";
+    return p;
+}
+
+void bindPartitionBuffers(HipCodegen& cg, const std::string& px, const PartSizes& ps, bool writer) {
+    cg.addBufferParam(px + "_cnt", "u32", 4, "(" + ps.parts + ") * 2", writer ? InitKind::Zero : InitKind::None, {},
+                      !writer, true);
+    cg.addBufferParam(px + "_keys", "u32", 4, "(" + ps.parts + ") * (" + ps.cap + ")", InitKind::None, {}, !writer,
+                      true);
+    cg.addBufferParam(px + "_vals", "i64", 8, "(" + ps.parts + ") * (" + ps.cap + ")", InitKind::None, {}, !writer,
+                      true);
+    cg.addBufferParam(px + "_part", "u64", 8, "(" + ps.parts + ") * (" + ps.pcap + ") * 4", InitKind::None, {},
+                      !writer, true);
+    cg.addSymbolParam("nparts_" + px, ps.parts);
+    cg.addSymbolParam("cap_" + px, ps.cap);
+    cg.addSymbolParam("pcap_" + px, ps.pcap);
+}
+
+}  // namespace
+
+HipPartitionScatter::HipPartitionScatter(std::unique_ptr<HipOperator> child, const std::string& prefix,
+                                         const std::string& table, const std::string& keyExpr,
+                                         const std::string& valueExpr, const std::string& slotsExpr, int ldsSlots)
+    : HipUnaryOperator(std::move(child)),
+      prefix_(prefix),
+      table_(table),
+      key_(keyExpr),
+      value_(valueExpr),
+      slots_(slotsExpr),
+      ldsSlots_(ldsSlots) {
+    if (ldsSlots_ < 64 || (ldsSlots_ & (ldsSlots_ - 1)) || ldsSlots_ > 4096)
+        throw std::runtime_error("HipPartitionScatter: LDS slots must be a power of two in [64, 4096]");
+}
+
+void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string px = prefix_, T = table_;
+    const PartSizes ps = partSizes(slots_, ldsSlots_);
+    bindPartitionBuffers(cg, px, ps, true);
+    const std::string st = T + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    cg.setPhaseMaxThreadgroups(kMaxBlocks);
+    int lbits = 0;
+    while ((1 << lbits) < ldsSlots_) ++lbits;
+    const std::string CAP = s(ldsSlots_), LB = s(lbits) + "u";
+    const std::string cnt = px + "_cnt", keys = px + "_keys", vals = px + "_vals", part = px + "_part";
+    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(nparts_" + px + ");");
+    cg.addLine("u32 _spill = 0, _ovc = 0;");
+    cg.addComment("pass 1: LDS table of " + CAP + " slots absorbs hot keys; cold rows go to their hash partition");
+    cg.declareLds("u64", "_gl", CAP + " * 4");
+    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
+        cg.addLine("_gl[4 * _s] = GDB_EMPTY_WORD; _gl[4 * _s + 1] = 0;");
+        cg.addLine("_gl[4 * _s + 2] = (u64)GDB_I64_MAX; _gl[4 * _s + 3] = (u64)GDB_I64_MIN;");
+    });
+    cg.addLine("__syncthreads();");
+    child_->produce(cg, [&]() {
+        cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
+        cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
+        cg.addBlock("if (gdb_lds_gb_update(_gl, " + LB + ", _gk, _gv, 16u) < 0)", [&]() {
+            cg.addLine("const u32 _p = gdb_partition(_gk, _pb);");
+            cg.addLine("const u32 _i = gdb_atomic_inc_u32(" + cnt + " + 2 * (u64)_p);");
+            cg.addBlock("if ((u64)_i < cap_" + px + ")", [&]() {
+                cg.addLine("const u64 _o = (u64)_p * cap_" + px + " + _i;");
+                cg.addLine(keys + "[_o] = _gk;");
+                cg.addLine(vals + "[_o] = _gv;");
+            }, " else {");
+            {
+                IndentGuard g(cg);
+                cg.addLine("++_ovc;");
+            }
+            cg.addLine("}");
+            cg.addLine("++_spill;");
+        });
+        consume();
+    });
+    cg.addLine("__syncthreads();");
+    cg.addComment("flush the block's hot partial aggregates to their partitions' partial streams");
+    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
+        cg.addLine("const u64 _w0 = _gl[4 * _s];");
+        cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+            cg.addLine("const u32 _p = gdb_partition((u32)_w0, _pb);");
+            cg.addLine("const u32 _j = gdb_atomic_inc_u32(" + cnt + " + 2 * (u64)_p + 1);");
+            cg.addBlock("if ((u64)_j < pcap_" + px + ")", [&]() {
+                cg.addLine("u64* _d = " + part + " + ((u64)_p * pcap_" + px + " + _j) * 4;");
+                cg.addLine("_d[0] = _w0; _d[1] = _gl[4 * _s + 1]; _d[2] = _gl[4 * _s + 2]; _d[3] = _gl[4 * _s + 3];");
+            }, " else {");
+            {
+                IndentGuard g(cg);
+                cg.addLine("++_ovc;");
+            }
+            cg.addLine("}");
+        });
+    });
+    cg.addComment("status: [0] retry needed, [1] partition capacity exceeded, [3] cold rows");
+    emitBlockStatus(cg, st, {{"_ovc", 0, true}, {"_ovc", 1, true}, {"_spill", 3, false}});
+}
+
+std::string HipPartitionScatter::describe() const {
+    return "PartitionScatter(" + prefix_ + ", key=" + key_ + ", value=" + value_ + ", slots=" + slots_ + ", lds" +
+           s(ldsSlots_) + ")";
+}
+
+HipPartitionAgg::HipPartitionAgg(const std::string& prefix, const std::string& table, const std::string& slotsExpr,
+                                 int scatterLdsSlots)
+    : prefix_(prefix), table_(table), slots_(slotsExpr), scatterLds_(scatterLdsSlots) {}
+
+void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string px = prefix_, T = table_;
+    const std::string PS = s(HipPartitionScatter::kPartSlots), PM = s(HipPartitionScatter::kPartSlots - 1);
+    const PartSizes ps = partSizes(slots_, scatterLds_);
+    bindPartitionBuffers(cg, px, ps, false);
+    // one tile (blockDim x ITEMS rows of the slot range) per partition
+    cg.setPhaseRowsExpr(slots_);
+    cg.addBufferParam(T, "u64", 8, slots_ + " * 4", InitKind::None, {}, false, true);
+    cg.addSymbolParam("nslots_" + T, slots_);
+    const std::string st = T + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    const std::string cnt = px + "_cnt", keys = px + "_keys", vals = px + "_vals", part = px + "_part";
+    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(nparts_" + px + ");");
+    cg.addLine("const u32 _sh = 32u - _pb - " + s(12) + "u;  // home slot: the 12 hash bits after the partition bits");
+    cg.addLine("u32 _ins = 0, _ovf = 0;");
+    cg.addComment("pass 2: each partition's group-table slice is built in LDS, then stored whole");
+    cg.declareLds("u64", "_pl", PS + " * 4");
+    cg.addBlock("for (u64 _p = blockIdx.x; _p < nparts_" + px + "; _p += gridDim.x)", [&]() {
+        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + PS + "; _s += (int)blockDim.x)", [&]() {
+            cg.addLine("_pl[4 * _s] = GDB_EMPTY_WORD; _pl[4 * _s + 1] = 0;");
+            cg.addLine("_pl[4 * _s + 2] = (u64)GDB_I64_MAX; _pl[4 * _s + 3] = (u64)GDB_I64_MIN;");
+        });
+        cg.addLine("__syncthreads();");
+        cg.addLine("const u64 _n = gdb_min64((i64)" + cnt + "[2 * _p], (i64)cap_" + px + ");");
+        cg.addBlock("for (u64 _i = threadIdx.x; _i < _n; _i += blockDim.x)", [&]() {
+            cg.addLine("const u64 _o = _p * cap_" + px + " + _i;");
+            cg.addLine("const u32 _k = " + keys + "[_o];");
+            cg.addLine("const i64 _v = " + vals + "[_o];");
+            cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
+                       "u, _k, 1u, _v, _v, _v, " + PS + "u) < 0) ++_ovf;");
+        });
+        cg.addLine("const u64 _m = gdb_min64((i64)" + cnt + "[2 * _p + 1], (i64)pcap_" + px + ");");
+        cg.addBlock("for (u64 _j = threadIdx.x; _j < _m; _j += blockDim.x)", [&]() {
+            cg.addLine("const u64* _d = " + part + " + (_p * pcap_" + px + " + _j) * 4;");
+            cg.addLine("const u64 _w0 = _d[0];");
+            cg.addLine("const u32 _k = (u32)_w0;");
+            cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
+                       "u, _k, (u32)(_w0 >> 32), (i64)_d[1], (i64)_d[2], (i64)_d[3], " + PS + "u) < 0) ++_ovf;");
+        });
+        cg.addLine("__syncthreads();");
+        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + PS + "; _s += (int)blockDim.x)", [&]() {
+            cg.addLine("u64* _g = " + T + " + (_p * " + PS + " + (u64)_s) * 4;");
+            cg.addLine("const u64 _w0 = _pl[4 * _s];");
+            cg.addLine("_g[0] = _w0; _g[1] = _pl[4 * _s + 1]; _g[2] = _pl[4 * _s + 2]; _g[3] = _pl[4 * _s + 3];");
+            cg.addLine("if ((u32)_w0 != GDB_EMPTY_KEY) ++_ins;");
+        });
+        cg.addLine("__syncthreads();");
+    });
+    cg.addComment("status: [0] overflow (a partition holds more groups than its slice), [2] groups");
+    emitBlockStatus(cg, st, {{"_ovf", 0, true}, {"_ins", 2, false}});
+    consume();
+
+    auto& rs = cg.resultSchema();
+    rs.kind = HipResultSchema::HASH_AGG;
+    rs.hash.tableName = T;
+    rs.hash.slotsSymbol = slots_;
+    rs.hash.slotWords = 4;
+    rs.hash.slots = {{"count", AggKind::Count, 0, 0},
+                     {"sum", AggKind::SumI64, 1, 0},
+                     {"min", AggKind::MinI64, 2, 0},
+                     {"max", AggKind::MaxI64, 3, 0}};
+}
+
+std::string HipPartitionAgg::describe() const { return "PartitionAgg(" + prefix_ + " -> " + table_ + ")"; }
 
 // ===========================================================================
 // HipTopK
@@ -922,8 +1137,11 @@ void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const std::string K = s(k_);
     const std::string ctl = out_ + "_ctl";
     const std::string cand = out_ + "_cand";
-    const int maxBlocks = 4096;
-    cg.setPhaseMaxThreadgroups(2048);
+    // Each block writes its top K to its own candidate slots and takes one
+    // ticket; the last block merges. One same-address atomic per block, so
+    // the grid is kept small (every block still scans many tiles).
+    const int maxBlocks = 512;
+    cg.setPhaseMaxThreadgroups(maxBlocks);
     cg.addBufferParam(out_, "u64", 8, s(2LL * k_), InitKind::Zero, {}, false, true);
     cg.addBufferParam(ctl, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.addBufferParam(cand, "u64", 8, s(2LL * k_ * maxBlocks), InitKind::None, {}, false, true);
@@ -933,46 +1151,40 @@ void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addLine("gdb_topk_push<" + K + ">(_tk, (u64)(" + hi_ + "), (u64)(" + lo_ + "));");
         consume();
     });
-    cg.addComment("TopK " + out_ + ": block merge, threshold-pruned candidates, last block merges");
+    cg.addComment("TopK " + out_ + ": block merge into this block's candidate slots; the last block merges all");
     cg.addBlock("", [&]() {
         cg.declareLds("u64", "_tk_lds", "GDB_TOPK_LDS_WORDS(" + K + ")");
         cg.declareLds("u64", "_tk_sh", "2");
         cg.addLine("gdb_topk_block_merge<" + K + ">(_tk, _tk_lds);");
         cg.addLine("const u64* _tk_res = _tk_lds + 32 * " + K + ";");
+        cg.addBlock("if (threadIdx.x < " + s(2LL * k_) + ")", [&]() {
+            cg.addLine(cand + "[(u64)blockIdx.x * " + s(2LL * k_) + " + threadIdx.x] = _tk_res[threadIdx.x];");
+        });
+        cg.addLine("__threadfence();");
+        cg.addLine("__syncthreads();");
         cg.addBlock("if (threadIdx.x == 0)", [&]() {
-            cg.addLine("const u64 _thr = gdb_load_relaxed_u64(" + ctl + " + 2);");
-            cg.addLine("u64 _cnt = 0;");
-            cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) if ((_tk_res[2 * _j] | _tk_res[2 * _j + 1]) != 0 && "
-                       "_tk_res[2 * _j] >= _thr) _cnt = (u64)_j + 1;");
-            cg.addBlock("if (_cnt > 0)", [&]() {
-                cg.addLine("const u64 _base = __hip_atomic_fetch_add(" + ctl +
-                           " + 1, _cnt, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
-                cg.addLine("for (u64 _j = 0; _j < _cnt; ++_j) { " + cand + "[(_base + _j) * 2] = _tk_res[2 * _j]; " +
-                           cand + "[(_base + _j) * 2 + 1] = _tk_res[2 * _j + 1]; }");
-            });
-            cg.addLine("if ((_tk_res[" + s(2 * k_ - 2) + "] | _tk_res[" + s(2 * k_ - 1) + "]) != 0) gdb_atomic_max_u64(" +
-                       ctl + " + 2, _tk_res[" + s(2 * k_ - 2) + "]);");
-            cg.addLine("__threadfence();");
             cg.addLine("const u64 _t = __hip_atomic_fetch_add(" + ctl +
-                       " + 0, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+                       " + 0, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);");
             cg.addLine("_tk_sh[0] = (_t == (u64)gridDim.x - 1ull) ? 1ull : 0ull;");
         });
         cg.addLine("__syncthreads();");
         cg.addBlock("if (_tk_sh[0])", [&]() {
             cg.addLine("__threadfence();");
-            cg.addLine("const u64 _nc = gdb_load_relaxed_u64(" + ctl + " + 1);");
             cg.addLine("GdbTopK<" + K + "> _f;");
             cg.addLine("gdb_topk_init<" + K + ">(_f);");
-            cg.addLine("for (u64 _c = threadIdx.x; _c < _nc; _c += blockDim.x) gdb_topk_push<" + K +
-                       ">(_f, gdb_load_relaxed_u64(" + cand + " + 2 * _c), gdb_load_relaxed_u64(" + cand +
-                       " + 2 * _c + 1));");
+            cg.addLine("const u64 _nc = (u64)gridDim.x * " + K + ";");
+            cg.addBlock("for (u64 _c = threadIdx.x; _c < _nc; _c += blockDim.x)", [&]() {
+                cg.addLine("const u64 _h = gdb_load_relaxed_u64(" + cand + " + 2 * _c), _l = gdb_load_relaxed_u64(" + cand +
+                           " + 2 * _c + 1);");
+                cg.addLine("if ((_h | _l) != 0) gdb_topk_push<" + K + ">(_f, _h, _l);");
+            });
             cg.addLine("gdb_topk_block_merge<" + K + ">(_f, _tk_lds);");
             cg.addBlock("if (threadIdx.x == 0)", [&]() {
                 cg.addLine("u64 _valid = 0;");
                 cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) { " + out_ + "[2 * _j] = _tk_res[2 * _j]; " + out_ +
                            "[2 * _j + 1] = _tk_res[2 * _j + 1]; if ((_tk_res[2 * _j] | _tk_res[2 * _j + 1]) != 0) ++_valid; }");
                 cg.addLine(ctl + "[3] = _valid;");
-                cg.addLine(ctl + "[0] = 0; " + ctl + "[1] = 0; " + ctl + "[2] = 0;");
+                cg.addLine(ctl + "[0] = 0;");
             });
         });
     });

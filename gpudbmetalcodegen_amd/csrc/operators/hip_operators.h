@@ -299,9 +299,55 @@ private:
     std::string ldsBitsScalar_;
 };
 
+// Partitioned hash group-by, pass 1 (no counterpart in the reference, whose
+// high-cardinality aggregation is one atomic per row into a direct-addressed
+// array, metal_operators.cpp:525-533). Hot keys are aggregated in a per-block
+// LDS table exactly as the LDS variant of HipHashGroupBy; a row whose key
+// misses the LDS table ("cold") is appended to its hash partition
+// (partition = top bits of key * 2654435769, partitions = slots / 4096) with
+// one counter atomic, and at block end the LDS table's partial aggregates are
+// appended to a second per-partition stream. Nothing touches the 32-byte
+// group table, so no per-row 64-bit atomics on a table larger than the
+// caches. Status words: [0] overflow (retry), [1] partition capacity
+// exceeded, [3] cold rows.
+class HipPartitionScatter : public HipUnaryOperator {
+public:
+    HipPartitionScatter(std::unique_ptr<HipOperator> child, const std::string& prefix, const std::string& table,
+                        const std::string& keyExpr, const std::string& valueExpr, const std::string& slotsExpr,
+                        int ldsSlots);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+    static constexpr int kPartSlots = 4096;  // group-table slots per partition (one LDS table in pass 2)
+    static constexpr int kMaxBlocks = 1024;  // bound on pass-1 blocks (sizes the partial stream)
+
+private:
+    std::string prefix_, table_, key_, value_, slots_;
+    int ldsSlots_;
+};
+
+// Partitioned hash group-by, pass 2: one block per partition (grid-stride
+// over partitions) aggregates the partition's cold rows and hot partials in
+// an LDS table of 4096 slots that is exactly the partition's slice of the
+// group table [key|count, sum, min, max] (home slot = the next 12 hash bits,
+// linear probing within the slice), then writes the slice with plain 16-byte
+// stores. No global atomics on the table. Status: [0] overflow, [2] groups.
+class HipPartitionAgg : public HipOperator {
+public:
+    // scatterLdsSlots: the pass-1 LDS capacity (sizes the partial streams).
+    HipPartitionAgg(const std::string& prefix, const std::string& table, const std::string& slotsExpr,
+                    int scatterLdsSlots);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string prefix_, table_, slots_;
+    int scatterLds_;
+};
+
 // Top-k by a 128-bit rank (hi, lo), larger first, rank (0,0) reserved.
-// Per-thread register lists, block merge in LDS, candidates pruned by a
-// running global threshold, and a final merge by the last block to finish.
+// Per-thread register lists, wave then block selection (DPP argmax rounds),
+// one candidate list per block, and a final merge by the last block to
+// finish (one ticket atomic per block).
 // Replaces the reference's CPU full sort (codegen/codegen_main.cpp:1211-1248).
 class HipTopK : public HipUnaryOperator {
 public:

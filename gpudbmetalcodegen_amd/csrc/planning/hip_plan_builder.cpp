@@ -149,7 +149,7 @@ void q3CountPhase(HipQueryPlan& plan, const PlanOptions& o) {
     cnt->addAggregate("b_rows", AggKind::Count, "1", 0);
     auto& ph = appendPhase(plan, "Q3_count_b", std::move(cnt), 256, 8);
     ph.scalarParams = {"q3_date"};
-    ph.staged = optInt(o, "staged", 1) != 0;
+    ph.staged = optInt(o, "staged", 1) != 0;  // measured: staged bitmap loads help this phase
 }
 
 HipQueryPlan buildQ3Size(const PlanOptions& o) {
@@ -182,7 +182,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
             top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
         auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
-        ph.staged = optInt(o, "staged", 1) != 0;
+        ph.staged = optInt(o, "staged", 0) != 0;
     }
     {
         auto scan = makeScan("F", {"f_fkey", "f_price", "f_disc", "f_date"});
@@ -192,10 +192,12 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         cur = std::make_unique<HipHashProbe>(std::move(cur), "d_q3_b_ht", "f_fkey", "b_slots", 2, "_bs");
         cur = std::make_unique<HipSlotAtomicAdd>(std::move(cur), "d_q3_b_ht", 2, "_bs", 1,
                                                  "f_price * (i64)(100 - f_disc)");
-        auto& ph = appendPhase(plan, "Q3_probe_f", std::move(cur), optInt(o, "block", 256), optInt(o, "items", 8),
+        // ITEMS=4, per-row emission: the measured best on MI355X (ITEMS=8
+        // halves occupancy; staging adds no overlap here)
+        auto& ph = appendPhase(plan, "Q3_probe_f", std::move(cur), optInt(o, "block", 256), optInt(o, "items", 4),
                                {4, 8});
         ph.scalarParams = {"q3_date"};
-        ph.staged = optInt(o, "staged", 1) != 0;
+        ph.staged = optInt(o, "staged", 0) != 0;
     }
     {
         auto scan = std::make_unique<HipHashTableScan>("d_q3_b_ht", "b_slots", 2, "ht");
@@ -217,8 +219,11 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
 
 // GROUPBY — per-key sum/count/min/max of g_val (cfg 2; the reference's
 // per-key aggregation is Q18, metal_plan_builder.cpp:1529-1545, direct
-// addressed; here a hash table). Options: variant=lds|global, lds_slots,
-// filter=1 adds g_val < gb_val_cut (selectivity control).
+// addressed; here a hash table). Options: variant=lds|global|part, lds_slots,
+// filter=1 adds g_val < gb_val_cut (selectivity control). variant=part is the
+// two-pass partitioned form (hot keys in LDS, cold rows hash-partitioned, one
+// LDS-built table slice per partition); it needs gb_slots >= 4096 and the
+// symbols gbp_rows (rows of G) and gbp_slack (partition headroom, percent).
 void gbCountPhases(HipQueryPlan& plan, bool filter);
 
 HipQueryPlan buildGroupBy(const PlanOptions& o) {
@@ -226,15 +231,36 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     plan.name = "GROUPBY";
     const bool filter = optInt(o, "filter", 0) != 0;
     const bool prepass = opt(o, "sizing", "given") == "prepass";
+    const std::string variant = opt(o, "variant", "lds");
+    const bool part = variant == "part";
     if (prepass) {
         gbCountPhases(plan, filter);
-        plan.sizingRules.push_back({"GBC_popcount", "gb_slots", "d_gb_distinct", 0, 2, 64, true});
+        plan.sizingRules.push_back(
+            {"GBC_popcount", "gb_slots", "d_gb_distinct", 0, 2, part ? HipPartitionScatter::kPartSlots : 64u, true});
     }
     auto scan = makeScan("G", {"g_key", "g_val"});
     std::unique_ptr<HipOperator> cur = std::move(scan);
     if (filter) cur = std::make_unique<HipSelection>(std::move(cur), "g_val < gb_val_cut");
-    const std::string variant = opt(o, "variant", "lds");
     const int ldsSlots = optInt(o, "lds_slots", 2048);
+    if (part) {
+        auto sc = std::make_unique<HipPartitionScatter>(std::move(cur), "d_gbp", "d_gb_ht", "g_key", "g_val", "gb_slots",
+                                                        ldsSlots);
+        auto& p1 = appendPhase(plan, "GBP_scatter", std::move(sc), optInt(o, "block", 1024), optInt(o, "items", 4), {4});
+        p1.launchBound = 1024;
+        if (filter) p1.scalarParams = {"gb_val_cut"};
+        auto agg = std::make_unique<HipPartitionAgg>("d_gbp", "d_gb_ht", "gb_slots", ldsSlots);
+        auto& p2 = appendPhase(plan, "GBP_agg", std::move(agg), 1024, 4, {4});
+        p2.launchBound = 1024;
+        if (filter) plan.requiredScalars = {"gb_val_cut"};
+        if (prepass) {
+            plan.requiredScalars.push_back("gb_key_min");
+            plan.requiredSymbols = {"gb_key_bits", "gbp_rows", "gbp_slack"};
+        } else {
+            plan.requiredSymbols = {"gb_slots", "gbp_rows", "gbp_slack"};
+        }
+        plan.chunkable = !prepass;
+        return plan;
+    }
     auto gb = std::make_unique<HipHashGroupBy>(std::move(cur), "d_gb_ht", "g_key", "g_val", "gb_slots",
                                                variant == "global" ? HipHashGroupBy::GLOBAL : HipHashGroupBy::LDS,
                                                ldsSlots);

@@ -148,10 +148,21 @@ def next_pow2(x: float) -> int:
     return p
 
 
+# Tables above this many slots (2 MiB of 32-byte slots) use the partitioned
+# group-by: below it the table stays cache-resident and per-row atomics on it
+# are cheap.
+PART_MIN_SLOTS = 65536
+
+
 def size_for(est: CardinalityEstimator, X: np.ndarray, ub: float, margin: float = 1.25,
              lds_slots: int = 2048) -> Sizing:
     g = float(est.predict_groups(X, [ub])[0])
     slots = max(64, next_pow2(2.0 * g * margin))
     top8 = float(X[8])  # sample mass of the 8 most frequent keys
+    if slots > PART_MIN_SLOTS:
+        # table far beyond the caches: two-pass partitioned form (no per-row
+        # atomics on the table); a large LDS pass-1 table when the sample
+        # shows heavy hitters
+        return Sizing(g, slots, "part", 4096 if top8 >= 0.02 else 1024, X, ub)
     variant = "lds" if (g <= lds_slots // 2 or top8 >= 0.02) else "global"
     return Sizing(g, slots, variant, lds_slots, X, ub)

@@ -160,47 +160,61 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
                 variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
                 key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
                 warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None,
-                collect: bool = True, stream: Stream | None = None) -> GroupByRun:
+                collect: bool = True, stream: Stream | None = None, slack: int = 125) -> GroupByRun:
     """prepass=True: the plan counts distinct keys exactly first (key-range
     bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
     `slots` (e.g. from the learned estimator) sizes it, and a reported
-    overflow doubles the table and reruns (resize-on-overflow)."""
+    overflow doubles the table and reruns (resize-on-overflow).
+    variant="part" is the two-pass partitioned form: slots >= 4096, and an
+    overflow of a partition's row stream (status word 1) doubles the
+    partition headroom `slack` (percent) instead of the table."""
     opts = groupby_options(variant, lds_slots, val_cut is not None, prepass, block)
     plan = get_plan(ctx, "GROUPBY", opts)
     ex = _executor(ctx, executor)
     _register(ex, dev, ["G"], stream)
     if val_cut is not None:
         ex.register_scalar("gb_val_cut", val_cut)
+    part = variant == "part"
+    if part:
+        rows = stream.table.rows if stream is not None else dev["G"].rows
+        ex.register_symbol("gbp_rows", rows)
     times = []
     resizes = 0
-    st = None
+    sst = None
     if prepass:
         if key_max is None:
             raise ValueError("prepass sizing needs the key range (key_min, key_max)")
         ex.register_scalar("gb_key_min", key_min)
         ex.register_symbol("gb_key_bits", key_max - key_min + 1)
-        t, st = _run(ex, plan, warmup, measured, stream)   # a streamed pre-pass plan is refused by the engine
-        times.append(t.total_ms)
-        slots = ex.get_symbol("gb_slots")
     else:
         if slots is None:
             raise ValueError("give slots or prepass=True")
-        slots = max(64, next_pow2(int(slots)))
-        while True:
+        slots = max(4096 if part else 64, next_pow2(int(slots)))
+    while True:
+        if not prepass:
             ex.register_symbol("gb_slots", slots)
-            try:
-                t, st = _run(ex, plan, warmup, measured, stream)
-                times.append(t.total_ms if t else st.total_ms)
-                break
-            except E.GdbOverflow:
-                st = ex.read_buffer("d_gb_ht_status", 32, np.uint64)
-                times.append(float("nan"))
-                resizes += 1
-                if resizes > max_resizes:
-                    raise
-                slots = next_pow2(max(2 * slots, 2 * int(st[2])))
+        if part:
+            ex.register_symbol("gbp_slack", slack)
+        try:
+            t, sst = _run(ex, plan, warmup, measured, stream)   # a streamed pre-pass plan is refused by the engine
+            times.append(t.total_ms if t else sst.total_ms)
+            break
+        except E.GdbOverflow:
+            status = ex.read_buffer("d_gb_ht_status", 32, np.uint64)
+            times.append(float("nan"))
+            resizes += 1
+            if resizes > max_resizes:
+                raise
+            if part and status[1]:
+                slack *= 2                      # a partition's cold-row stream was full
+            elif prepass:
+                raise                           # exact sizing cannot overflow the table
+            else:
+                slots = next_pow2(max(2 * slots, 2 * int(status[2])))
+    if prepass:
+        slots = ex.get_symbol("gb_slots")
     res = ex.collect(plan) if collect else None
-    return GroupByRun(res, t, plan, ex, st, slots=slots, resizes=resizes, distinct=res.rows() if res else 0,
+    return GroupByRun(res, t, plan, ex, sst, slots=slots, resizes=resizes, distinct=res.rows() if res else 0,
                       times_ms=times)
 
 

@@ -23,7 +23,7 @@ def _gen(n, s, nkeys, seed):
     return W.workload2(n=n, s=s, nkeys=nkeys, seed=seed)["G"]
 
 
-@pytest.mark.parametrize("variant", ["lds", "global"])
+@pytest.mark.parametrize("variant", ["lds", "global", "part"])
 @pytest.mark.parametrize("n,s,nkeys", [(1, 1.1, 10), (5000, 1.1, 1000), (300_000, 1.1, 200_000),
                                        (300_000, 0.0, 50_000), (200_000, 1.5, 20)])
 def test_groupby_prepass(ctx, variant, n, s, nkeys):
@@ -39,12 +39,13 @@ def test_groupby_prepass(ctx, variant, n, s, nkeys):
         dev["G"].free(ctx)
 
 
-def test_groupby_filter_selectivity(ctx):
+@pytest.mark.parametrize("variant", ["lds", "part"])
+def test_groupby_filter_selectivity(ctx, variant):
     G = _gen(200_000, 1.1, 100_000, 7)
     dev = Q.upload_tables(ctx, {"G": G})
     try:
         for cut in (0, 10_000, 500_000, 1_000_001):
-            run = Q.run_groupby(ctx, dev, prepass=True, val_cut=cut, key_min=1, key_max=100_000)
+            run = Q.run_groupby(ctx, dev, prepass=True, val_cut=cut, key_min=1, key_max=100_000, variant=variant)
             _cmp(run.result, oracle.groupby(G, cut))
     finally:
         dev["G"].free(ctx)
@@ -60,6 +61,44 @@ def test_groupby_resize_on_overflow_stays_exact(ctx):
         _cmp(run.result, exp)
         run2 = Q.run_groupby(ctx, dev, slots=4 * len(exp["key"]), variant="global")
         assert run2.resizes == 0
+        _cmp(run2.result, exp)
+    finally:
+        dev["G"].free(ctx)
+
+
+@pytest.mark.parametrize("s,lds", [(1.1, 4096), (0.0, 1024), (1.5, 64)])
+def test_groupby_partitioned_given_sizes(ctx, s, lds):
+    """Two-pass partitioned form at several partition counts (slots / 4096)
+    and pass-1 LDS sizes, skewed and uniform keys; every row is counted once
+    (hot partials + cold rows)."""
+    G = _gen(1_000_003, s, 300_000, 21)
+    exp = oracle.groupby(G)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        for slots in (1 << 20, 1 << 21):
+            run = Q.run_groupby(ctx, dev, slots=slots, variant="part", lds_slots=lds)
+            assert run.resizes == 0
+            _cmp(run.result, exp)
+            assert int(run.result.columns["count"].sum()) == 1_000_003
+            assert run.timing.status[0][3] > 0 or s > 1.0      # cold rows took the partition path
+    finally:
+        dev["G"].free(ctx)
+
+
+def test_groupby_partitioned_overflows_recover_exactly(ctx):
+    """Both overflow kinds of the partitioned form end in exact results: a
+    partition's cold-row stream too small (tiny headroom, retried with double
+    headroom) and a partition slice holding more than 4096 groups (retried
+    with a doubled table)."""
+    G = _gen(1_000_000, 0.0, 200_000, 23)
+    exp = oracle.groupby(G)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, slots=1 << 20, variant="part", lds_slots=64, slack=4)
+        assert run.resizes >= 1
+        _cmp(run.result, exp)
+        run2 = Q.run_groupby(ctx, dev, slots=4096, variant="part", lds_slots=64)
+        assert run2.resizes >= 1 and run2.slots > 4096
         _cmp(run2.result, exp)
     finally:
         dev["G"].free(ctx)
@@ -83,9 +122,10 @@ def test_groupby_full_size(ctx):
     exp = oracle.groupby(G, threads=16)
     dev = Q.upload_tables(ctx, {"G": G})
     try:
-        run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=20_000_000)
-        _cmp(run.result, exp)
-        assert int(run.result.columns["count"].sum()) == G["g_key"].shape[0]
+        for variant, lds in (("lds", 2048), ("part", 4096)):
+            run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=20_000_000, variant=variant, lds_slots=lds)
+            _cmp(run.result, exp)
+            assert int(run.result.columns["count"].sum()) == G["g_key"].shape[0]
         print("GROUPBY full", run.timing.total_ms, run.timing.phases, run.timing.status)
     finally:
         dev["G"].free(ctx)
