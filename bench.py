@@ -87,6 +87,7 @@ def q3_phase_bytes(T: dict, seg: int, d: int, b_slots: int) -> dict:
         "Q3_build_b_table": 12 * n_b + 4 * int(b_date_ok.sum()) + (16 + 4) * int(b_ok.sum()),
         "Q3_probe_f": 20 * n_f + 4 * int(f_date_ok.sum()) + (16 + 8) * int(f_hit.sum()),
         "Q3_topk": 16 * b_slots,
+        "Q3_topk_merge": 16 * 10 * 512,
     }
 
 
@@ -118,7 +119,8 @@ def bench_join(ctx, steps: int, warmup: int, cpu: bool) -> dict:
         "gpu_ms_per_step": t.total_ms,
         "rows_per_s": rows / (wall_ms / 1e3),
         "phase_ms": {k: round(v, 4) for k, v in phases.items()},
-        "phase_roofline_frac": {k: round(pb[k] / (phases[k] / 1e3) / 1e9 / PEAK_HBM_GBS, 4) for k in phases if phases[k] > 0},
+        "phase_roofline_frac": {k: round(pb[k] / (phases[k] / 1e3) / 1e9 / PEAK_HBM_GBS, 4) for k in phases
+                                if phases[k] > 0 and k in pb},
         "roofline": roofline(dom, phases[dom], pb[dom], "w1"),
         "b_slots": b_slots,
     }
@@ -253,16 +255,16 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     phases = {name: ms for name, ms, *_ in t.phases}
     out["plain"] = {"ms_per_step": plain_ms, "rows_per_s": n / (plain_ms / 1e3), "slots": run.slots,
                     "phase_ms": {k: round(v, 4) for k, v in phases.items()}}
-    # pass 1 reads every row and writes each cold row once; pass 2 reads the
-    # cold rows and hot partials and writes the table once
+    # algorithmic bytes per pass: count reads every row; scatter reads every
+    # row again and writes each cold row once; agg reads the cold rows and
+    # hot partials and writes the table once
     st = t.status
-    cold = int(st[-2][3]) if len(st) >= 2 else 0
-    p1_bytes = 12.0 * n + 12.0 * cold
-    p2_bytes = 12.0 * cold + 32.0 * run.slots
-    dom = max(("GBP_scatter", "GBP_agg"), key=lambda k: phases[k])
-    out["roofline"] = roofline(dom, phases[dom], p1_bytes if dom == "GBP_scatter" else p2_bytes, "w2")
-    out["phase_roofline_frac"] = {"GBP_scatter": round(p1_bytes / (phases["GBP_scatter"] / 1e3) / 1e9 / PEAK_HBM_GBS, 4),
-                                  "GBP_agg": round(p2_bytes / (phases["GBP_agg"] / 1e3) / 1e9 / PEAK_HBM_GBS, 4)}
+    cold = int(st[-1][3])
+    pbytes = {"GBP_count": 12.0 * n, "GBP_scatter": 12.0 * n + 12.0 * cold,
+              "GBP_agg": 12.0 * cold + 32.0 * run.slots}
+    dom = max(pbytes, key=lambda k: phases[k])
+    out["roofline"] = roofline(dom, phases[dom], pbytes[dom], "w2")
+    out["phase_roofline_frac"] = {k: round(v / (phases[k] / 1e3) / 1e9 / PEAK_HBM_GBS, 4) for k, v in pbytes.items()}
     out["cold_rows"] = cold
     plain_res = run.result
     ex.release()

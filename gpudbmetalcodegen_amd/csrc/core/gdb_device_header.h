@@ -194,7 +194,7 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
     if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(w, m);
 }
 
-// gdb_bitmap_set behind a block-local cache (1024 entries of LDS u64:
+// A bitmap set behind a block-local cache (1024 entries of LDS u64:
 // word index << 32 | bits this block has set in it). An entry is read and
 // written as one 64-bit LDS access; a lost update only costs a redundant
 // atomic, and a bit is recorded only after its atomic was issued.
@@ -206,7 +206,7 @@ GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64
     const u64 c = *(volatile u64*)e;
     const bool same = (u32)(c >> 32) == w;
     if (same && ((u32)c & m)) return;
-    gdb_bitmap_set(bm, nbits, key);
+    gdb_atomic_or_u32(bm + w, m);  // no return value: nothing waits on it
     *(volatile u64*)e = ((u64)w << 32) | (u64)((same ? (u32)c : 0u) | m);
 }
 
@@ -338,6 +338,19 @@ GDB_DEV int gdb_lds_gb_merge_at(u64* tab, u32 mask, u32 s, u32 key, u32 cnt, i64
         }
     }
     return -1;
+}
+// Membership of key in the key array of an LDS group table built by
+// gdb_lds_gb_update (same home slot and probe window): linear probing from
+// the home slot until the key, an empty slot or max_probe slots.
+GDB_DEV bool gdb_lds_key_member(const u32* keys, u32 bits, u32 key, u32 max_probe) {
+    const u32 mask = (1u << bits) - 1u;
+    u32 s = (u32)gdb_hash_slot(key, bits);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        const u32 k = keys[s];
+        if (k == key) return true;
+        if (k == GDB_EMPTY_KEY) return false;
+    }
+    return false;
 }
 // Hash partition of a key: the top pbits of key * 2654435769 (0 when pbits = 0).
 GDB_DEV u32 gdb_partition(u32 key, u32 pbits) { return pbits ? (u32)((key * GDB_KNUTH) >> (32u - pbits)) : 0u; }

@@ -140,6 +140,11 @@ public:
     // The phase body then takes (..., const u64 _rb, const u64 _re).
     void setRangeMode(bool on) { rangeMode_ = on; }
     bool rangeMode() const { return rangeMode_; }
+    // Segment scans: like range mode for the scan alone (rows [_rb, _re)
+    // walked by this block), with _rb/_re defined by the consuming operator's
+    // own loop (logical blocks of a multi-kernel operator).
+    void setScanSegment(bool on) { scanSegment_ = on; }
+    bool scanSegment() const { return scanSegment_; }
     // The phase body alone (template <int ITEMS> __device__ function named
     // <bodyName>), without helpers, runtime header or kernel wrappers. In
     // range mode the body also takes (unsigned char* _lds) last, and a
@@ -192,6 +197,7 @@ private:
     HipResultSchema resultSchema_;
     unsigned indentLevel_ = 1;
     bool rangeMode_ = false;
+    bool scanSegment_ = false;
     bool staged_ = false;
     bool nextConsumerStageable_ = false;
     std::vector<std::string> rowBindings_;

@@ -97,7 +97,7 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     std::vector<std::string> ptr;
     for (const auto& c : columns_) ptr.push_back(cg.addColumnParam(table_, c.name, c.type));
     const std::string nParam = cg.addTableSizeParam(table_);
-    const bool range = cg.rangeMode();
+    const bool range = cg.rangeMode() || cg.scanSegment();
     // Grid mode: the grid strides over the whole table. Range mode: this
     // block walks the work item [_rb, _re) (_rb is a multiple of the tile).
     const std::string n = range ? "_re" : nParam;
@@ -373,6 +373,50 @@ void HipHashBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                " : 65536ull);");
     cg.addLine("u32 _ins_" + table_ + " = 0, _ovf_" + table_ + " = 0, _dup_" + table_ + " = 0;");
     child_->produce(cg, [&]() {
+        if (cg.staged()) {
+            // Home-slot CAS of every row of the batch in flight together (the
+            // returned occupants are only examined in the next loop); rows
+            // whose home slot holds another key probe on in lock-step rounds.
+            const std::string T = table_, W = s(slotWords_);
+            const std::string HS = cg.uniqueName("_bs"), CU = cg.uniqueName("_bc"), PD = cg.uniqueName("_bp"),
+                              ANY = cg.uniqueName("_ba");
+            const std::string K = "(u32)(" + key_ + ")";
+            const std::string MINE = "((u64)" + K + " | ((u64)(u32)(" + payload_ + ") << 32))";
+            cg.addLine("u64 " + HS + "[ITEMS];");
+            cg.addLine("u64 " + CU + "[ITEMS];");
+            cg.addLine("bool " + PD + "[ITEMS];");
+            cg.addLine("bool " + ANY + " = false;");
+            cg.addLine("#pragma unroll");
+            cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) " + PD + "[_u] = false;");
+            cg.rowLoop([&]() {
+                cg.addLine(HS + "[_u] = gdb_hash_slot(" + K + ", _bits_" + T + ");");
+                cg.addLine("u64 _c = GDB_EMPTY_WORD;");
+                cg.addLine("(void)__hip_atomic_compare_exchange_strong(" + T + " + " + HS + "[_u] * " + W + ", &_c, " +
+                           MINE + ", __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+                cg.addLine(CU + "[_u] = _c;");
+            });
+            auto resolve = [&]() {
+                cg.addLine("const u64 _c = " + CU + "[_u];");
+                cg.addLine("if (_c == GDB_EMPTY_WORD) { ++_ins_" + T + "; " + PD + "[_u] = false; }");
+                cg.addLine("else if ((u32)_c == " + K + ") { ++_dup_" + T + "; " + PD + "[_u] = false; }");
+                cg.addLine("else { " + PD + "[_u] = true; " + ANY + " = true; }");
+            };
+            cg.rowLoop(resolve);
+            cg.addBlock("for (u32 _p = 1; " + ANY + " && _p < _maxp_" + T + "; ++_p)", [&]() {
+                cg.addLine(ANY + " = false;");
+                cg.rowLoop([&]() {
+                    cg.addLine(HS + "[_u] = (" + HS + "[_u] + 1ull) & ((1ull << _bits_" + T + ") - 1ull);");
+                    cg.addLine("u64 _c = GDB_EMPTY_WORD;");
+                    cg.addLine("(void)__hip_atomic_compare_exchange_strong(" + T + " + " + HS + "[_u] * " + W +
+                               ", &_c, " + MINE + ", __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+                    cg.addLine(CU + "[_u] = _c;");
+                }, PD);
+                cg.rowLoop(resolve, PD);
+            });
+            cg.rowLoop([&]() { cg.addLine("if (" + PD + "[_u]) ++_ovf_" + T + ";"); });
+            consume();
+            return;
+        }
         cg.addLine("const i64 _hs = gdb_ht_insert_unique(" + table_ + ", " + s(slotWords_) + ", _bits_" + table_ +
                    ", (u32)(" + key_ + "), (u32)(" + payload_ + "), _maxp_" + table_ + ");");
         cg.addLine("if (_hs >= 0) ++_ins_" + table_ + "; else if (_hs == -1) ++_ovf_" + table_ + "; else ++_dup_" +
@@ -929,165 +973,275 @@ std::string HipHashGroupBy::describe() const {
 }
 
 // ===========================================================================
-// Partitioned hash group-by (two passes)
+// Partitioned hash group-by (five phases, no global atomics)
 // ===========================================================================
 
-namespace {
-
-// Size expressions shared by both passes of a partitioned group-by.
-struct PartSizes {
-    std::string parts, cap, pcap;
-};
-
-PartSizes partSizes(const std::string& slots, int ldsSlots) {
-    PartSizes p;
-    const std::string PS = s(HipPartitionScatter::kPartSlots);
-    p.parts = "(" + slots + ") / " + PS;
-    // cold rows per partition: rows * slack% / partitions, plus headroom
-    p.cap = "gbp_rows * gbp_slack / 100 / ((" + slots + ") / " + PS + ") + 1024";
-    // hot partials per partition: at most blocks x LDS slots in total
-    p.pcap = s(static_cast<long long>(HipPartitionScatter::kMaxBlocks) * ldsSlots) + " / ((" + slots + ") / " + PS +
-             ") * 2 + 256";
+int PartitionConfig::maxParts() const {
+    // LDS: pass-1 table (32 B per slot) + one u32 counter per partition, within
+    // 150 KiB of the CU's 160 KiB; a power of two, at most 16384.
+    const long long room = (150LL * 1024 - 32LL * ldsSlots - 256) / 4;
+    int p = 1;
+    while (2LL * p <= room && p < 16384) p *= 2;
     return p;
 }
 
-void bindPartitionBuffers(HipCodegen& cg, const std::string& px, const PartSizes& ps, bool writer) {
-    cg.addBufferParam(px + "_cnt", "u32", 4, "(" + ps.parts + ") * 2", writer ? InitKind::Zero : InitKind::None, {},
-                      !writer, true);
-    cg.addBufferParam(px + "_keys", "u32", 4, "(" + ps.parts + ") * (" + ps.cap + ")", InitKind::None, {}, !writer,
-                      true);
-    cg.addBufferParam(px + "_vals", "i64", 8, "(" + ps.parts + ") * (" + ps.cap + ")", InitKind::None, {}, !writer,
-                      true);
-    cg.addBufferParam(px + "_part", "u64", 8, "(" + ps.parts + ") * (" + ps.pcap + ") * 4", InitKind::None, {},
-                      !writer, true);
-    cg.addSymbolParam("nparts_" + px, ps.parts);
-    cg.addSymbolParam("cap_" + px, ps.cap);
-    cg.addSymbolParam("pcap_" + px, ps.pcap);
+namespace {
+
+struct PartNames {
+    std::string P, NP, rows, hot, ccnt, pcnt, meta, keys, vals, part;
+};
+
+// Bind the partition buffers and symbols of a phase. Sizes are expressions
+// of the group-table slots and the scanned row count, the same in every phase.
+PartNames bindPartition(HipCodegen& cg, const PartitionConfig& c, bool hotW, bool cntW, bool metaW, bool streamW) {
+    PartNames n;
+    const std::string px = c.prefix;
+    const std::string LB = s(PartitionConfig::kLogicalBlocks), CAP = s(c.ldsSlots);
+    n.P = "(" + c.slots + ") / " + s(PartitionConfig::kPartSlots);
+    n.NP = "nparts_" + px;
+    n.rows = "nrows_" + px;
+    n.hot = px + "_hot";
+    n.ccnt = px + "_ccnt";
+    n.pcnt = px + "_pcnt";
+    n.meta = px + "_meta";
+    n.keys = px + "_keys";
+    n.vals = px + "_vals";
+    n.part = px + "_part";
+    cg.addSymbolParam(n.NP, n.P);
+    cg.addSymbolParam(n.rows, c.rowsSym);
+    cg.addBufferParam(n.hot, "u64", 8, LB + " * " + CAP + " * 4", InitKind::None, {}, !hotW, true);
+    cg.addBufferParam(n.ccnt, "u32", 4, LB + " * (" + n.P + ")", InitKind::None, {}, !cntW, true);
+    cg.addBufferParam(n.pcnt, "u32", 4, LB + " * (" + n.P + ")", InitKind::None, {}, !cntW, true);
+    cg.addBufferParam(n.meta, "u32", 4, "4 * (" + n.P + ")", InitKind::None, {}, !metaW, true);
+    cg.addBufferParam(n.keys, "u32", 4, c.rowsSym + " + 1", InitKind::None, {}, !streamW, true);
+    cg.addBufferParam(n.vals, "i64", 8, c.rowsSym + " + 1", InitKind::None, {}, !streamW, true);
+    cg.addBufferParam(n.part, "u64", 8, LB + " * " + CAP + " * 4", InitKind::None, {}, !streamW, true);
+    return n;
+}
+
+// The logical-block loop shared by pass 1 and pass 4: logical block _lb
+// covers rows [_rb, _re) of the scanned table, a whole number of tiles.
+void openLogicalBlocks(HipCodegen& cg, const std::string& nRows, const std::string& cond) {
+    cg.addLine("const u64 _ltile = (u64)blockDim.x * ITEMS;");
+    cg.addLine("const u64 _chunk = ((" + nRows + " + " + s(PartitionConfig::kLogicalBlocks - 1) + ") / " +
+               s(PartitionConfig::kLogicalBlocks) + " + _ltile - 1) / _ltile * _ltile;");
+    cg.addLine("for (u64 _lb = blockIdx.x; " + cond + " && _lb < " + s(PartitionConfig::kLogicalBlocks) +
+               "; _lb += gridDim.x) {");
+    cg.increaseIndent();
+    cg.addLine("const u64 _rb = _lb * _chunk;");
+    cg.addLine("const u64 _re = _rb + _chunk < " + nRows + " ? _rb + _chunk : " + nRows + ";");
+}
+
+void closeLogicalBlocks(HipCodegen& cg) {
+    cg.decreaseIndent();
+    cg.addLine("}");
 }
 
 }  // namespace
 
-HipPartitionScatter::HipPartitionScatter(std::unique_ptr<HipOperator> child, const std::string& prefix,
-                                         const std::string& table, const std::string& keyExpr,
-                                         const std::string& valueExpr, const std::string& slotsExpr, int ldsSlots)
-    : HipUnaryOperator(std::move(child)),
-      prefix_(prefix),
-      table_(table),
-      key_(keyExpr),
-      value_(valueExpr),
-      slots_(slotsExpr),
-      ldsSlots_(ldsSlots) {
-    if (ldsSlots_ < 64 || (ldsSlots_ & (ldsSlots_ - 1)) || ldsSlots_ > 4096)
-        throw std::runtime_error("HipPartitionScatter: LDS slots must be a power of two in [64, 4096]");
+HipPartitionCount::HipPartitionCount(std::unique_ptr<HipOperator> child, PartitionConfig cfg,
+                                     const std::string& keyExpr, const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)), c_(std::move(cfg)), key_(keyExpr), value_(valueExpr) {
+    if (c_.ldsSlots < 64 || (c_.ldsSlots & (c_.ldsSlots - 1)) || c_.ldsSlots > 4096)
+        throw std::runtime_error("HipPartitionCount: LDS slots must be a power of two in [64, 4096]");
 }
 
-void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const std::string px = prefix_, T = table_;
-    const PartSizes ps = partSizes(slots_, ldsSlots_);
-    bindPartitionBuffers(cg, px, ps, true);
-    const std::string st = T + "_status";
+void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const PartNames n = bindPartition(cg, c_, true, true, false, false);
+    const std::string st = c_.table + "_status";
     cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(st);
-    cg.setPhaseMaxThreadgroups(kMaxBlocks);
+    const std::string nRows = cg.addTableSizeParam(c_.scanTable);
+    const std::string CAP = s(c_.ldsSlots), PMAX = s(c_.maxParts());
     int lbits = 0;
-    while ((1 << lbits) < ldsSlots_) ++lbits;
-    const std::string CAP = s(ldsSlots_), LB = s(lbits) + "u";
-    const std::string cnt = px + "_cnt", keys = px + "_keys", vals = px + "_vals", part = px + "_part";
-    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(nparts_" + px + ");");
-    cg.addLine("u32 _spill = 0, _ovc = 0;");
-    cg.addComment("pass 1: LDS table of " + CAP + " slots absorbs hot keys; cold rows go to their hash partition");
+    while ((1 << lbits) < c_.ldsSlots) ++lbits;
+    cg.addComment("pass 1: LDS table of " + CAP + " slots per logical block; cold rows counted per partition");
     cg.declareLds("u64", "_gl", CAP + " * 4");
+    cg.declareLds("u32", "_pc", PMAX);
+    cg.addLine("const u64 _P = " + n.NP + ";");
+    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(_P);");
+    cg.addLine("const bool _fits = _P <= " + PMAX + "ull;");
+    cg.addLine("u32 _ovf = (!_fits && blockIdx.x == 0 && threadIdx.x == 0) ? 1u : 0u, _spill = 0;");
+    openLogicalBlocks(cg, nRows, "_fits");
     cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
         cg.addLine("_gl[4 * _s] = GDB_EMPTY_WORD; _gl[4 * _s + 1] = 0;");
         cg.addLine("_gl[4 * _s + 2] = (u64)GDB_I64_MAX; _gl[4 * _s + 3] = (u64)GDB_I64_MIN;");
     });
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = 0;");
     cg.addLine("__syncthreads();");
+    cg.setScanSegment(true);
     child_->produce(cg, [&]() {
         cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
         cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
-        cg.addBlock("if (gdb_lds_gb_update(_gl, " + LB + ", _gk, _gv, 16u) < 0)", [&]() {
-            cg.addLine("const u32 _p = gdb_partition(_gk, _pb);");
-            cg.addLine("const u32 _i = gdb_atomic_inc_u32(" + cnt + " + 2 * (u64)_p);");
-            cg.addBlock("if ((u64)_i < cap_" + px + ")", [&]() {
-                cg.addLine("const u64 _o = (u64)_p * cap_" + px + " + _i;");
-                cg.addLine(keys + "[_o] = _gk;");
-                cg.addLine(vals + "[_o] = _gv;");
-            }, " else {");
-            {
-                IndentGuard g(cg);
-                cg.addLine("++_ovc;");
-            }
-            cg.addLine("}");
+        cg.addBlock("if (gdb_lds_gb_update(_gl, " + s(lbits) + "u, _gk, _gv, 16u) < 0)", [&]() {
+            cg.addLine("gdb_lds_add_u32(&_pc[gdb_partition(_gk, _pb)], 1u);");
             cg.addLine("++_spill;");
         });
         consume();
     });
+    cg.setScanSegment(false);
     cg.addLine("__syncthreads();");
-    cg.addComment("flush the block's hot partial aggregates to their partitions' partial streams");
+    cg.addComment("the block's final LDS table, and its cold-row counts per partition");
+    cg.addLine("u64* _h = " + n.hot + " + _lb * " + CAP + " * 4;");
+    cg.addLine("for (int _s = (int)threadIdx.x; _s < " + CAP + " * 4; _s += (int)blockDim.x) _h[_s] = _gl[_s];");
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) { " + n.ccnt + "[_lb * _P + _q] = _pc[_q]; _pc[_q] = 0; }");
+    cg.addLine("__syncthreads();");
+    cg.addComment("partial-aggregate counts per partition");
     cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
         cg.addLine("const u64 _w0 = _gl[4 * _s];");
-        cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
-            cg.addLine("const u32 _p = gdb_partition((u32)_w0, _pb);");
-            cg.addLine("const u32 _j = gdb_atomic_inc_u32(" + cnt + " + 2 * (u64)_p + 1);");
-            cg.addBlock("if ((u64)_j < pcap_" + px + ")", [&]() {
-                cg.addLine("u64* _d = " + part + " + ((u64)_p * pcap_" + px + " + _j) * 4;");
-                cg.addLine("_d[0] = _w0; _d[1] = _gl[4 * _s + 1]; _d[2] = _gl[4 * _s + 2]; _d[3] = _gl[4 * _s + 3];");
-            }, " else {");
-            {
-                IndentGuard g(cg);
-                cg.addLine("++_ovc;");
-            }
-            cg.addLine("}");
+        cg.addLine("if ((u32)_w0 != GDB_EMPTY_KEY) gdb_lds_add_u32(&_pc[gdb_partition((u32)_w0, _pb)], 1u);");
+    });
+    cg.addLine("__syncthreads();");
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) " + n.pcnt + "[_lb * _P + _q] = _pc[_q];");
+    cg.addLine("__syncthreads();");
+    closeLogicalBlocks(cg);
+    cg.addComment("status: [0]+[1] partition count beyond the LDS counters, [3] cold rows");
+    emitBlockStatus(cg, st, {{"_ovf", 0, true}, {"_ovf", 1, true}, {"_spill", 3, false}});
+}
+
+std::string HipPartitionCount::describe() const {
+    return "PartitionCount(" + c_.prefix + ", key=" + key_ + ", value=" + value_ + ", slots=" + c_.slots + ", lds" +
+           s(c_.ldsSlots) + ")";
+}
+
+void HipPartitionOffsets::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const PartNames n = bindPartition(cg, c_, false, true, true, false);
+    cg.setPhaseRowsExpr(n.P);
+    cg.addComment("counts -> exclusive offsets over the logical blocks, per partition; totals");
+    cg.addLine("const u64 _P = " + n.NP + ";");
+    cg.addBlock("for (u64 _q = (u64)blockIdx.x * blockDim.x + threadIdx.x; _q < _P; _q += (u64)gridDim.x * blockDim.x)",
+                [&]() {
+                    cg.addLine("u32 _c = 0, _r = 0;");
+                    cg.addBlock("for (u64 _lb = 0; _lb < " + s(PartitionConfig::kLogicalBlocks) + "; ++_lb)", [&]() {
+                        cg.addLine("const u32 _a = " + n.ccnt + "[_lb * _P + _q], _b = " + n.pcnt + "[_lb * _P + _q];");
+                        cg.addLine(n.ccnt + "[_lb * _P + _q] = _c; " + n.pcnt + "[_lb * _P + _q] = _r;");
+                        cg.addLine("_c += _a; _r += _b;");
+                    });
+                    cg.addLine(n.meta + "[_q] = _c; " + n.meta + "[_P + _q] = _r;");
+                });
+    consume();
+}
+
+void HipPartitionBases::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const PartNames n = bindPartition(cg, c_, false, false, true, false);
+    cg.setPhaseRowsExpr("1");
+    cg.setPhaseMaxThreadgroups(1);
+    cg.addComment("one block: exclusive scans of the partition totals -> partition bases");
+    cg.declareLds("u32", "_sa", "1024");
+    cg.declareLds("u32", "_sb", "1024");
+    cg.addBlock("if (blockIdx.x == 0)", [&]() {
+        cg.addLine("const u64 _P = " + n.NP + ";");
+        cg.addLine("const u64 _seg = (_P + blockDim.x - 1) / blockDim.x;");
+        cg.addLine("const u64 _q0 = threadIdx.x * _seg, _q1 = _q0 + _seg < _P ? _q0 + _seg : _P;");
+        cg.addLine("u32 _a = 0, _b = 0;");
+        cg.addLine("for (u64 _q = _q0; _q < _q1; ++_q) { _a += " + n.meta + "[_q]; _b += " + n.meta + "[_P + _q]; }");
+        cg.addLine("_sa[threadIdx.x] = _a; _sb[threadIdx.x] = _b;");
+        cg.addLine("__syncthreads();");
+        cg.addComment("inclusive Hillis-Steele scan of the per-thread sums");
+        cg.addBlock("for (u32 _d = 1; _d < blockDim.x; _d <<= 1)", [&]() {
+            cg.addLine("const u32 _xa = threadIdx.x >= _d ? _sa[threadIdx.x - _d] : 0u;");
+            cg.addLine("const u32 _xb = threadIdx.x >= _d ? _sb[threadIdx.x - _d] : 0u;");
+            cg.addLine("__syncthreads();");
+            cg.addLine("_sa[threadIdx.x] += _xa; _sb[threadIdx.x] += _xb;");
+            cg.addLine("__syncthreads();");
+        });
+        cg.addLine("u32 _ra = _sa[threadIdx.x] - _a, _rb2 = _sb[threadIdx.x] - _b;");
+        cg.addBlock("for (u64 _q = _q0; _q < _q1; ++_q)", [&]() {
+            cg.addLine("const u32 _ta = " + n.meta + "[_q], _tb = " + n.meta + "[_P + _q];");
+            cg.addLine(n.meta + "[2 * _P + _q] = _ra; " + n.meta + "[3 * _P + _q] = _rb2;");
+            cg.addLine("_ra += _ta; _rb2 += _tb;");
         });
     });
-    cg.addComment("status: [0] retry needed, [1] partition capacity exceeded, [3] cold rows");
-    emitBlockStatus(cg, st, {{"_ovc", 0, true}, {"_ovc", 1, true}, {"_spill", 3, false}});
+    consume();
+}
+
+HipPartitionScatter::HipPartitionScatter(std::unique_ptr<HipOperator> child, PartitionConfig cfg,
+                                         const std::string& keyExpr, const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)), c_(std::move(cfg)), key_(keyExpr), value_(valueExpr) {}
+
+void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const PartNames n = bindPartition(cg, c_, false, false, false, true);
+    const std::string nRows = cg.addTableSizeParam(c_.scanTable);
+    const std::string CAP = s(c_.ldsSlots), PMAX = s(c_.maxParts());
+    int lbits = 0;
+    while ((1 << lbits) < c_.ldsSlots) ++lbits;
+    cg.addComment("pass 4: rows whose key is not in the block's final LDS table are written, packed by partition");
+    cg.declareLds("u32", "_hk", CAP);
+    cg.declareLds("u32", "_pc", PMAX);
+    cg.addLine("const u64 _P = " + n.NP + ";");
+    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(_P);");
+    openLogicalBlocks(cg, nRows, "_P <= " + PMAX + "ull");
+    cg.addLine("const u64* _h = " + n.hot + " + _lb * " + CAP + " * 4;");
+    cg.addLine("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x) _hk[_s] = (u32)_h[4 * _s];");
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = " + n.meta + "[2 * _P + _q] + " + n.ccnt +
+               "[_lb * _P + _q];");
+    cg.addLine("__syncthreads();");
+    cg.setScanSegment(true);
+    child_->produce(cg, [&]() {
+        cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
+        cg.addBlock("if (!gdb_lds_key_member(_hk, " + s(lbits) + "u, _gk, 16u))", [&]() {
+            cg.addLine("const u32 _i = __hip_atomic_fetch_add(&_pc[gdb_partition(_gk, _pb)], 1u, __ATOMIC_RELAXED, "
+                       "__HIP_MEMORY_SCOPE_WORKGROUP);");
+            cg.addLine(n.keys + "[_i] = _gk;");
+            cg.addLine(n.vals + "[_i] = (i64)(" + value_ + ");");
+        });
+        consume();
+    });
+    cg.setScanSegment(false);
+    cg.addLine("__syncthreads();");
+    cg.addComment("the block's hot partial aggregates, packed by partition");
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = " + n.meta + "[3 * _P + _q] + " + n.pcnt +
+               "[_lb * _P + _q];");
+    cg.addLine("__syncthreads();");
+    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
+        cg.addLine("const u64 _w0 = _h[4 * _s];");
+        cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+            cg.addLine("const u32 _j = __hip_atomic_fetch_add(&_pc[gdb_partition((u32)_w0, _pb)], 1u, __ATOMIC_RELAXED, "
+                       "__HIP_MEMORY_SCOPE_WORKGROUP);");
+            cg.addLine("u64* _d = " + n.part + " + (u64)_j * 4;");
+            cg.addLine("_d[0] = _w0; _d[1] = _h[4 * _s + 1]; _d[2] = _h[4 * _s + 2]; _d[3] = _h[4 * _s + 3];");
+        });
+    });
+    cg.addLine("__syncthreads();");
+    closeLogicalBlocks(cg);
 }
 
 std::string HipPartitionScatter::describe() const {
-    return "PartitionScatter(" + prefix_ + ", key=" + key_ + ", value=" + value_ + ", slots=" + slots_ + ", lds" +
-           s(ldsSlots_) + ")";
+    return "PartitionScatter(" + c_.prefix + ", key=" + key_ + ", lds" + s(c_.ldsSlots) + ")";
 }
 
-HipPartitionAgg::HipPartitionAgg(const std::string& prefix, const std::string& table, const std::string& slotsExpr,
-                                 int scatterLdsSlots)
-    : prefix_(prefix), table_(table), slots_(slotsExpr), scatterLds_(scatterLdsSlots) {}
-
 void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const std::string px = prefix_, T = table_;
-    const std::string PS = s(HipPartitionScatter::kPartSlots), PM = s(HipPartitionScatter::kPartSlots - 1);
-    const PartSizes ps = partSizes(slots_, scatterLds_);
-    bindPartitionBuffers(cg, px, ps, false);
-    // one tile (blockDim x ITEMS rows of the slot range) per partition
-    cg.setPhaseRowsExpr(slots_);
-    cg.addBufferParam(T, "u64", 8, slots_ + " * 4", InitKind::None, {}, false, true);
-    cg.addSymbolParam("nslots_" + T, slots_);
+    const PartNames n = bindPartition(cg, c_, false, false, false, false);
+    const std::string T = c_.table;
+    const std::string PS = s(PartitionConfig::kPartSlots), PM = s(PartitionConfig::kPartSlots - 1);
+    cg.setPhaseRowsExpr(c_.slots);  // one tile (blockDim x ITEMS = 4096) of the slot range per partition
+    cg.addBufferParam(T, "u64", 8, c_.slots + " * 4", InitKind::None, {}, false, true);
+    cg.addSymbolParam("nslots_" + T, c_.slots);
     const std::string st = T + "_status";
     cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(st);
-    const std::string cnt = px + "_cnt", keys = px + "_keys", vals = px + "_vals", part = px + "_part";
-    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(nparts_" + px + ");");
-    cg.addLine("const u32 _sh = 32u - _pb - " + s(12) + "u;  // home slot: the 12 hash bits after the partition bits");
+    cg.addLine("const u64 _P = " + n.NP + ";");
+    cg.addLine("const u32 _pb = (u32)__builtin_ctzll(_P);");
+    cg.addLine("const u32 _sh = 32u - _pb - 12u;  // home slot: the 12 hash bits after the partition bits");
     cg.addLine("u32 _ins = 0, _ovf = 0;");
-    cg.addComment("pass 2: each partition's group-table slice is built in LDS, then stored whole");
+    cg.addComment("pass 5: each partition's group-table slice is built in LDS, then stored whole");
     cg.declareLds("u64", "_pl", PS + " * 4");
-    cg.addBlock("for (u64 _p = blockIdx.x; _p < nparts_" + px + "; _p += gridDim.x)", [&]() {
+    cg.addBlock("for (u64 _p = blockIdx.x; _p < _P; _p += gridDim.x)", [&]() {
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + PS + "; _s += (int)blockDim.x)", [&]() {
             cg.addLine("_pl[4 * _s] = GDB_EMPTY_WORD; _pl[4 * _s + 1] = 0;");
             cg.addLine("_pl[4 * _s + 2] = (u64)GDB_I64_MAX; _pl[4 * _s + 3] = (u64)GDB_I64_MIN;");
         });
         cg.addLine("__syncthreads();");
-        cg.addLine("const u64 _n = gdb_min64((i64)" + cnt + "[2 * _p], (i64)cap_" + px + ");");
-        cg.addBlock("for (u64 _i = threadIdx.x; _i < _n; _i += blockDim.x)", [&]() {
-            cg.addLine("const u64 _o = _p * cap_" + px + " + _i;");
-            cg.addLine("const u32 _k = " + keys + "[_o];");
-            cg.addLine("const i64 _v = " + vals + "[_o];");
+        cg.addLine("const u64 _cb = " + n.meta + "[2 * _P + _p], _cn = " + n.meta + "[_p];");
+        cg.addBlock("for (u64 _i = threadIdx.x; _i < _cn; _i += blockDim.x)", [&]() {
+            cg.addLine("const u32 _k = " + n.keys + "[_cb + _i];");
+            cg.addLine("const i64 _v = " + n.vals + "[_cb + _i];");
             cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
                        "u, _k, 1u, _v, _v, _v, " + PS + "u) < 0) ++_ovf;");
         });
-        cg.addLine("const u64 _m = gdb_min64((i64)" + cnt + "[2 * _p + 1], (i64)pcap_" + px + ");");
-        cg.addBlock("for (u64 _j = threadIdx.x; _j < _m; _j += blockDim.x)", [&]() {
-            cg.addLine("const u64* _d = " + part + " + (_p * pcap_" + px + " + _j) * 4;");
+        cg.addLine("const u64 _pbase = " + n.meta + "[3 * _P + _p], _pn = " + n.meta + "[_P + _p];");
+        cg.addBlock("for (u64 _j = threadIdx.x; _j < _pn; _j += blockDim.x)", [&]() {
+            cg.addLine("const u64* _d = " + n.part + " + (_pbase + _j) * 4;");
             cg.addLine("const u64 _w0 = _d[0];");
             cg.addLine("const u32 _k = (u32)_w0;");
             cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
@@ -1109,15 +1263,13 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     auto& rs = cg.resultSchema();
     rs.kind = HipResultSchema::HASH_AGG;
     rs.hash.tableName = T;
-    rs.hash.slotsSymbol = slots_;
+    rs.hash.slotsSymbol = c_.slots;
     rs.hash.slotWords = 4;
     rs.hash.slots = {{"count", AggKind::Count, 0, 0},
                      {"sum", AggKind::SumI64, 1, 0},
                      {"min", AggKind::MinI64, 2, 0},
                      {"max", AggKind::MaxI64, 3, 0}};
 }
-
-std::string HipPartitionAgg::describe() const { return "PartitionAgg(" + prefix_ + " -> " + table_ + ")"; }
 
 // ===========================================================================
 // HipTopK
@@ -1135,60 +1287,62 @@ void HipTopK::addColumn(const std::string& name, int word, int shift, int bits, 
 
 void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const std::string K = s(k_);
-    const std::string ctl = out_ + "_ctl";
     const std::string cand = out_ + "_cand";
-    // Each block writes its top K to its own candidate slots and takes one
-    // ticket; the last block merges. One same-address atomic per block, so
-    // the grid is kept small (every block still scans many tiles).
-    const int maxBlocks = 512;
-    cg.setPhaseMaxThreadgroups(maxBlocks);
-    cg.addBufferParam(out_, "u64", 8, s(2LL * k_), InitKind::Zero, {}, false, true);
-    cg.addBufferParam(ctl, "u64", 8, "4", InitKind::Zero, {}, false, true);
-    cg.addBufferParam(cand, "u64", 8, s(2LL * k_ * maxBlocks), InitKind::None, {}, false, true);
+    // Each block writes its top K to its own candidate slots; a final
+    // one-block phase (HipTopKMerge) merges them.
+    cg.setPhaseMaxThreadgroups(kMaxBlocks);
+    cg.addBufferParam(cand, "u64", 8, s(2LL * k_ * kMaxBlocks), InitKind::Zero, {}, false, true);
     cg.addLine("GdbTopK<" + K + "> _tk;");
     cg.addLine("gdb_topk_init<" + K + ">(_tk);");
     child_->produce(cg, [&]() {
         cg.addLine("gdb_topk_push<" + K + ">(_tk, (u64)(" + hi_ + "), (u64)(" + lo_ + "));");
         consume();
     });
-    cg.addComment("TopK " + out_ + ": block merge into this block's candidate slots; the last block merges all");
+    cg.addComment("TopK " + out_ + ": block selection into this block's candidate slots");
     cg.addBlock("", [&]() {
         cg.declareLds("u64", "_tk_lds", "GDB_TOPK_LDS_WORDS(" + K + ")");
-        cg.declareLds("u64", "_tk_sh", "2");
         cg.addLine("gdb_topk_block_merge<" + K + ">(_tk, _tk_lds);");
-        cg.addLine("const u64* _tk_res = _tk_lds + 32 * " + K + ";");
         cg.addBlock("if (threadIdx.x < " + s(2LL * k_) + ")", [&]() {
-            cg.addLine(cand + "[(u64)blockIdx.x * " + s(2LL * k_) + " + threadIdx.x] = _tk_res[threadIdx.x];");
-        });
-        cg.addLine("__threadfence();");
-        cg.addLine("__syncthreads();");
-        cg.addBlock("if (threadIdx.x == 0)", [&]() {
-            cg.addLine("const u64 _t = __hip_atomic_fetch_add(" + ctl +
-                       " + 0, 1ull, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);");
-            cg.addLine("_tk_sh[0] = (_t == (u64)gridDim.x - 1ull) ? 1ull : 0ull;");
-        });
-        cg.addLine("__syncthreads();");
-        cg.addBlock("if (_tk_sh[0])", [&]() {
-            cg.addLine("__threadfence();");
-            cg.addLine("GdbTopK<" + K + "> _f;");
-            cg.addLine("gdb_topk_init<" + K + ">(_f);");
-            cg.addLine("const u64 _nc = (u64)gridDim.x * " + K + ";");
-            cg.addBlock("for (u64 _c = threadIdx.x; _c < _nc; _c += blockDim.x)", [&]() {
-                cg.addLine("const u64 _h = gdb_load_relaxed_u64(" + cand + " + 2 * _c), _l = gdb_load_relaxed_u64(" + cand +
-                           " + 2 * _c + 1);");
-                cg.addLine("if ((_h | _l) != 0) gdb_topk_push<" + K + ">(_f, _h, _l);");
-            });
-            cg.addLine("gdb_topk_block_merge<" + K + ">(_f, _tk_lds);");
-            cg.addBlock("if (threadIdx.x == 0)", [&]() {
-                cg.addLine("u64 _valid = 0;");
-                cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) { " + out_ + "[2 * _j] = _tk_res[2 * _j]; " + out_ +
-                           "[2 * _j + 1] = _tk_res[2 * _j + 1]; if ((_tk_res[2 * _j] | _tk_res[2 * _j + 1]) != 0) ++_valid; }");
-                cg.addLine(ctl + "[3] = _valid;");
-                cg.addLine(ctl + "[0] = 0;");
-            });
+            cg.addLine(cand + "[(u64)blockIdx.x * " + s(2LL * k_) + " + threadIdx.x] = _tk_lds[32 * " + K +
+                       " + threadIdx.x];");
         });
     });
+}
 
+std::unique_ptr<HipOperator> HipTopK::makeMerge() const { return std::make_unique<HipTopKMerge>(out_, k_, cols_); }
+
+HipTopKMerge::HipTopKMerge(const std::string& outputName, int k, std::vector<HipTopK::Column> cols)
+    : out_(outputName), k_(k), cols_(std::move(cols)) {}
+
+void HipTopKMerge::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string K = s(k_);
+    const std::string ctl = out_ + "_ctl";
+    const std::string cand = out_ + "_cand";
+    const std::string NC = s(static_cast<long long>(k_) * HipTopK::kMaxBlocks);
+    cg.setPhaseRowsExpr("1");
+    cg.setPhaseMaxThreadgroups(1);
+    cg.addBufferParam(out_, "u64", 8, s(2LL * k_), InitKind::Zero, {}, false, true);
+    cg.addBufferParam(ctl, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.addBufferParam(cand, "u64", 8, s(2LL * k_ * HipTopK::kMaxBlocks), InitKind::None, {}, true, true);
+    cg.addComment("TopK " + out_ + ": one block merges every block's candidate list");
+    cg.addBlock("if (blockIdx.x == 0)", [&]() {
+        cg.declareLds("u64", "_tk_lds", "GDB_TOPK_LDS_WORDS(" + K + ")");
+        cg.addLine("GdbTopK<" + K + "> _f;");
+        cg.addLine("gdb_topk_init<" + K + ">(_f);");
+        cg.addBlock("for (u64 _c = threadIdx.x; _c < " + NC + "; _c += blockDim.x)", [&]() {
+            cg.addLine("const u64 _h = " + cand + "[2 * _c], _l = " + cand + "[2 * _c + 1];");
+            cg.addLine("if ((_h | _l) != 0) gdb_topk_push<" + K + ">(_f, _h, _l);");
+        });
+        cg.addLine("gdb_topk_block_merge<" + K + ">(_f, _tk_lds);");
+        cg.addLine("const u64* _tk_res = _tk_lds + 32 * " + K + ";");
+        cg.addBlock("if (threadIdx.x == 0)", [&]() {
+            cg.addLine("u64 _valid = 0;");
+            cg.addLine("for (int _j = 0; _j < " + K + "; ++_j) { " + out_ + "[2 * _j] = _tk_res[2 * _j]; " + out_ +
+                       "[2 * _j + 1] = _tk_res[2 * _j + 1]; if ((_tk_res[2 * _j] | _tk_res[2 * _j + 1]) != 0) ++_valid; }");
+            cg.addLine(ctl + "[3] = _valid;");
+        });
+    });
+    consume();
     auto& rs = cg.resultSchema();
     rs.kind = HipResultSchema::TOPK;
     rs.topk.bufferName = out_;
@@ -1197,6 +1351,8 @@ void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     rs.topk.columns.clear();
     for (const auto& c : cols_) rs.topk.columns.push_back({c.name, c.word, c.shift, c.bits, c.complementOf});
 }
+
+std::string HipTopKMerge::describe() const { return "TopKMerge(" + out_ + ", k=" + s(k_) + ")"; }
 
 std::string HipTopK::describe() const { return "TopK(" + out_ + ", k=" + s(k_) + ", rank=(" + hi_ + ", " + lo_ + "))"; }
 

@@ -159,6 +159,7 @@ private:
 // and its host-built tables (query_preprocessing.cpp:352-378).
 class HipHashBuild : public HipUnaryOperator {
 public:
+    bool stageable() const override { return true; }
     HipHashBuild(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
                  const std::string& payloadExpr, const std::string& slotsExpr, int slotWords);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
@@ -299,49 +300,93 @@ private:
     std::string ldsBitsScalar_;
 };
 
-// Partitioned hash group-by, pass 1 (no counterpart in the reference, whose
+// Partitioned hash group-by: no counterpart in the reference, whose
 // high-cardinality aggregation is one atomic per row into a direct-addressed
-// array, metal_operators.cpp:525-533). Hot keys are aggregated in a per-block
-// LDS table exactly as the LDS variant of HipHashGroupBy; a row whose key
-// misses the LDS table ("cold") is appended to its hash partition
-// (partition = top bits of key * 2654435769, partitions = slots / 4096) with
-// one counter atomic, and at block end the LDS table's partial aggregates are
-// appended to a second per-partition stream. Nothing touches the 32-byte
-// group table, so no per-row 64-bit atomics on a table larger than the
-// caches. Status words: [0] overflow (retry), [1] partition capacity
-// exceeded, [3] cold rows.
-class HipPartitionScatter : public HipUnaryOperator {
-public:
-    HipPartitionScatter(std::unique_ptr<HipOperator> child, const std::string& prefix, const std::string& table,
-                        const std::string& keyExpr, const std::string& valueExpr, const std::string& slotsExpr,
-                        int ldsSlots);
-    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
-    std::string describe() const override;
-    static constexpr int kPartSlots = 4096;  // group-table slots per partition (one LDS table in pass 2)
-    static constexpr int kMaxBlocks = 1024;  // bound on pass-1 blocks (sizes the partial stream)
-
-private:
-    std::string prefix_, table_, key_, value_, slots_;
-    int ldsSlots_;
+// array (metal_operators.cpp:525-533). Five phases, no global atomics:
+//   1. HipPartitionCount   — per logical block (a fixed row range), hot keys
+//      are aggregated in an LDS table; a row whose key the table cannot hold
+//      is "cold" and only counted, per hash partition, in LDS. The final LDS
+//      table (hot partial aggregates) and both count vectors (cold rows,
+//      partials) are written out.
+//   2. HipPartitionOffsets — per partition, counts -> exclusive offsets over
+//      the logical blocks, and totals.
+//   3. HipPartitionBases   — one block scans the totals into each partition's
+//      base in the packed cold-row and partial streams.
+//   4. HipPartitionScatter — the same logical blocks re-read their rows; a
+//      row is cold iff its key is absent from the block's final LDS table (a
+//      full probe window never admits a key later, so this is exactly the
+//      pass-1 decision), and cold rows and partials are written to exact
+//      positions (LDS counters), packed by partition.
+//   5. HipPartitionAgg     — one block per partition aggregates its cold rows
+//      and partials in an LDS table that is the partition's slice of the
+//      group table (4096 slots of [key|count, sum, min, max]; home slot = the
+//      12 hash bits after the partition bits), and stores the slice whole.
+// Partition of a key = top bits of key * 2654435769; partitions = slots / 4096.
+struct PartitionConfig {
+    std::string prefix;     // buffer/symbol prefix, e.g. "d_gbp"
+    std::string table;      // group table, e.g. "d_gb_ht"
+    std::string slots;      // slots expression of the group table
+    std::string rowsSym;    // symbol: rows of the scanned table (all of them)
+    std::string scanTable;  // scanned table name
+    int ldsSlots = 4096;    // pass-1 LDS table capacity (power of two, 64..4096)
+    static constexpr int kPartSlots = 4096;
+    static constexpr int kLogicalBlocks = 512;
+    // Largest partition count whose LDS counters fit beside the LDS table.
+    int maxParts() const;
 };
 
-// Partitioned hash group-by, pass 2: one block per partition (grid-stride
-// over partitions) aggregates the partition's cold rows and hot partials in
-// an LDS table of 4096 slots that is exactly the partition's slice of the
-// group table [key|count, sum, min, max] (home slot = the next 12 hash bits,
-// linear probing within the slice), then writes the slice with plain 16-byte
-// stores. No global atomics on the table. Status: [0] overflow, [2] groups.
-class HipPartitionAgg : public HipOperator {
+class HipPartitionCount : public HipUnaryOperator {
 public:
-    // scatterLdsSlots: the pass-1 LDS capacity (sizes the partial streams).
-    HipPartitionAgg(const std::string& prefix, const std::string& table, const std::string& slotsExpr,
-                    int scatterLdsSlots);
+    HipPartitionCount(std::unique_ptr<HipOperator> child, PartitionConfig cfg, const std::string& keyExpr,
+                      const std::string& valueExpr);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
-    std::string prefix_, table_, slots_;
-    int scatterLds_;
+    PartitionConfig c_;
+    std::string key_, value_;
+};
+
+class HipPartitionOffsets : public HipOperator {
+public:
+    explicit HipPartitionOffsets(PartitionConfig cfg) : c_(std::move(cfg)) {}
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override { return "PartitionOffsets(" + c_.prefix + ")"; }
+
+private:
+    PartitionConfig c_;
+};
+
+class HipPartitionBases : public HipOperator {
+public:
+    explicit HipPartitionBases(PartitionConfig cfg) : c_(std::move(cfg)) {}
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override { return "PartitionBases(" + c_.prefix + ")"; }
+
+private:
+    PartitionConfig c_;
+};
+
+class HipPartitionScatter : public HipUnaryOperator {
+public:
+    HipPartitionScatter(std::unique_ptr<HipOperator> child, PartitionConfig cfg, const std::string& keyExpr,
+                        const std::string& valueExpr);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    PartitionConfig c_;
+    std::string key_, value_;
+};
+
+class HipPartitionAgg : public HipOperator {
+public:
+    explicit HipPartitionAgg(PartitionConfig cfg) : c_(std::move(cfg)) {}
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override { return "PartitionAgg(" + c_.prefix + " -> " + c_.table + ")"; }
+
+private:
+    PartitionConfig c_;
 };
 
 // Top-k by a 128-bit rank (hi, lo), larger first, rank (0,0) reserved.
@@ -363,11 +408,30 @@ public:
     void addColumn(const std::string& name, int word, int shift, int bits, long long complementOf = 0);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
+    // The operator of the final phase: one block merges every block's
+    // candidates (the phase boundary orders the candidate writes, so no
+    // fences or tickets) and defines the result.
+    std::unique_ptr<HipOperator> makeMerge() const;
+    static constexpr int kMaxBlocks = 512;
 
 private:
     std::string out_, hi_, lo_;
     int k_;
     std::vector<Column> cols_;
+};
+
+// Final top-k phase (leaf): one block of the merge of kMaxBlocks candidate
+// lists written by a HipTopK phase.
+class HipTopKMerge : public HipOperator {
+public:
+    HipTopKMerge(const std::string& outputName, int k, std::vector<HipTopK::Column> cols);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string out_;
+    int k_;
+    std::vector<HipTopK::Column> cols_;
 };
 
 }  // namespace codegen

@@ -97,8 +97,11 @@ HipQueryPlan buildQ1(const PlanOptions& o) {
     agg->addAverage("avg_qty", "sum_qty", "count_order");
     agg->addAverage("avg_price", "sum_base_price", "count_order");
     agg->addAverage("avg_disc", "sum_disc", "count_order");
-    auto& ph = appendPhase(plan, "Q1_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 8),
-                           {4, 8});
+    // ITEMS=4 measured best on MI355X: 16-byte lane-contiguous loads for the
+    // int32 columns; at ITEMS=8 each lane's 64-byte int64 chunk halves the
+    // achieved bandwidth
+    auto& ph = appendPhase(plan, "Q1_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 4),
+                           {2, 4, 8});
     ph.launchBound = 256;
     ph.scalarParams = {"q1_cutoff"};
     plan.requiredScalars = {"q1_cutoff"};
@@ -118,7 +121,7 @@ HipQueryPlan buildQ6(const PlanOptions& o) {
     auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q6_sum", "0", 1);
     agg->addAggregate("revenue", AggKind::SumI64, "f_price * (i64)f_disc", 4);
     agg->addAggregate("count", AggKind::Count, "1", 0);
-    auto& ph = appendPhase(plan, "Q6_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 8), {4, 8});
+    auto& ph = appendPhase(plan, "Q6_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 4), {2, 4, 8});
     ph.launchBound = 256;
     ph.scalarParams = {"q6_date_lo", "q6_date_hi", "q6_disc_lo", "q6_disc_hi", "q6_qty_hi"};
     plan.requiredScalars = ph.scalarParams;
@@ -182,7 +185,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
             top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
         auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
-        ph.staged = optInt(o, "staged", 0) != 0;
+        ph.staged = optInt(o, "staged", 1) != 0;  // home-slot CAS of the batch in flight together
     }
     {
         auto scan = makeScan("F", {"f_fkey", "f_price", "f_disc", "f_date"});
@@ -207,8 +210,11 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         top->addColumn("b_key", 1, 0, 32, 0xFFFFFFFFll);
         top->addColumn("revenue", 0, 0, 64, 0);
         top->addColumn("b_date", 1, 32, 32, 0x7FFFFFFFll);
+        auto merge = top->makeMerge();
         auto& ph = appendPhase(plan, "Q3_topk", std::move(top), 256, 4);
         ph.launchBound = 256;
+        auto& pm = appendPhase(plan, "Q3_topk_merge", std::move(merge), 256, 1, {1});
+        pm.launchBound = 256;
     }
     plan.requiredScalars = {"q3_seg", "q3_date"};
     plan.requiredSymbols = {"a_bits", "b_bits"};
@@ -221,9 +227,10 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
 // per-key aggregation is Q18, metal_plan_builder.cpp:1529-1545, direct
 // addressed; here a hash table). Options: variant=lds|global|part, lds_slots,
 // filter=1 adds g_val < gb_val_cut (selectivity control). variant=part is the
-// two-pass partitioned form (hot keys in LDS, cold rows hash-partitioned, one
-// LDS-built table slice per partition); it needs gb_slots >= 4096 and the
-// symbols gbp_rows (rows of G) and gbp_slack (partition headroom, percent).
+// partitioned form without global atomics (hot keys in LDS, cold rows packed
+// by hash partition, one LDS-built table slice per partition; see
+// HipPartitionCount); it needs gb_slots >= 4096, gb_slots / 4096 partitions
+// within PartitionConfig::maxParts() and the symbol gbp_rows (rows of G).
 void gbCountPhases(HipQueryPlan& plan, bool filter);
 
 HipQueryPlan buildGroupBy(const PlanOptions& o) {
@@ -236,29 +243,47 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     if (prepass) {
         gbCountPhases(plan, filter);
         plan.sizingRules.push_back(
-            {"GBC_popcount", "gb_slots", "d_gb_distinct", 0, 2, part ? HipPartitionScatter::kPartSlots : 64u, true});
+            {"GBC_popcount", "gb_slots", "d_gb_distinct", 0, 2, part ? static_cast<uint64_t>(PartitionConfig::kPartSlots) : 64u, true});
     }
     auto scan = makeScan("G", {"g_key", "g_val"});
     std::unique_ptr<HipOperator> cur = std::move(scan);
     if (filter) cur = std::make_unique<HipSelection>(std::move(cur), "g_val < gb_val_cut");
     const int ldsSlots = optInt(o, "lds_slots", 2048);
     if (part) {
-        auto sc = std::make_unique<HipPartitionScatter>(std::move(cur), "d_gbp", "d_gb_ht", "g_key", "g_val", "gb_slots",
-                                                        ldsSlots);
-        auto& p1 = appendPhase(plan, "GBP_scatter", std::move(sc), optInt(o, "block", 1024), optInt(o, "items", 4), {4});
+        PartitionConfig pc;
+        pc.prefix = "d_gbp";
+        pc.table = "d_gb_ht";
+        pc.slots = "gb_slots";
+        pc.rowsSym = "gbp_rows";
+        pc.scanTable = "G";
+        pc.ldsSlots = ldsSlots;
+        const int block = optInt(o, "block", 1024);
+        auto cnt = std::make_unique<HipPartitionCount>(std::move(cur), pc, "g_key", "g_val");
+        auto& p1 = appendPhase(plan, "GBP_count", std::move(cnt), block, optInt(o, "items", 4), {4});
         p1.launchBound = 1024;
-        if (filter) p1.scalarParams = {"gb_val_cut"};
-        auto agg = std::make_unique<HipPartitionAgg>("d_gbp", "d_gb_ht", "gb_slots", ldsSlots);
-        auto& p2 = appendPhase(plan, "GBP_agg", std::move(agg), 1024, 4, {4});
-        p2.launchBound = 1024;
+        if (filter) p1.scalarParams = {"gb_val_cut"};  // (phase references die at the next append)
+        auto& p2 = appendPhase(plan, "GBP_offsets", std::make_unique<HipPartitionOffsets>(pc), 256, 1, {1});
+        p2.launchBound = 256;
+        auto& p3 = appendPhase(plan, "GBP_bases", std::make_unique<HipPartitionBases>(pc), 1024, 1, {1});
+        p3.launchBound = 1024;
+        // pass 4 re-reads the rows (same scan, same filter)
+        auto scan2 = makeScan("G", {"g_key", "g_val"});
+        std::unique_ptr<HipOperator> cur2 = std::move(scan2);
+        if (filter) cur2 = std::make_unique<HipSelection>(std::move(cur2), "g_val < gb_val_cut");
+        auto sc = std::make_unique<HipPartitionScatter>(std::move(cur2), pc, "g_key", "g_val");
+        auto& p4 = appendPhase(plan, "GBP_scatter", std::move(sc), block, optInt(o, "items", 4), {4});
+        p4.launchBound = 1024;
+        if (filter) p4.scalarParams = {"gb_val_cut"};
+        auto& p5 = appendPhase(plan, "GBP_agg", std::make_unique<HipPartitionAgg>(pc), 1024, 4, {4});
+        p5.launchBound = 1024;
         if (filter) plan.requiredScalars = {"gb_val_cut"};
         if (prepass) {
             plan.requiredScalars.push_back("gb_key_min");
-            plan.requiredSymbols = {"gb_key_bits", "gbp_rows", "gbp_slack"};
+            plan.requiredSymbols = {"gb_key_bits", "gbp_rows"};
         } else {
-            plan.requiredSymbols = {"gb_slots", "gbp_rows", "gbp_slack"};
+            plan.requiredSymbols = {"gb_slots", "gbp_rows"};
         }
-        plan.chunkable = !prepass;
+        plan.chunkable = false;  // passes 1 and 4 both need every row of G
         return plan;
     }
     auto gb = std::make_unique<HipHashGroupBy>(std::move(cur), "d_gb_ht", "g_key", "g_val", "gb_slots",

@@ -156,28 +156,46 @@ def groupby_options(variant: str = "lds", lds_slots: int = 2048, filtered: bool 
     return ",".join(o)
 
 
+def part_max_parts(lds_slots: int) -> int:
+    """Largest partition count of the partitioned group-by whose per-partition
+    LDS counters fit beside a pass-1 LDS table of lds_slots slots (mirrors
+    PartitionConfig::maxParts)."""
+    room = (150 * 1024 - 32 * lds_slots - 256) // 4
+    p = 1
+    while 2 * p <= room and p < 16384:
+        p *= 2
+    return p
+
+
+def part_lds_for(slots: int, lds_slots: int) -> int:
+    """The largest pass-1 LDS table <= lds_slots that fits slots / 4096 partitions."""
+    parts = max(1, slots // 4096)
+    if parts > 16384:
+        raise ValueError(f"partitioned group-by supports at most 16384 * 4096 slots, got {slots}")
+    while lds_slots > 64 and part_max_parts(lds_slots) < parts:
+        lds_slots //= 2
+    return lds_slots
+
+
 def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass: bool = False,
                 variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
                 key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
                 warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None,
-                collect: bool = True, stream: Stream | None = None, slack: int = 125) -> GroupByRun:
+                collect: bool = True, stream: Stream | None = None) -> GroupByRun:
     """prepass=True: the plan counts distinct keys exactly first (key-range
     bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
     `slots` (e.g. from the learned estimator) sizes it, and a reported
     overflow doubles the table and reruns (resize-on-overflow).
-    variant="part" is the two-pass partitioned form: slots >= 4096, and an
-    overflow of a partition's row stream (status word 1) doubles the
-    partition headroom `slack` (percent) instead of the table."""
-    opts = groupby_options(variant, lds_slots, val_cut is not None, prepass, block)
-    plan = get_plan(ctx, "GROUPBY", opts)
+    variant="part" is the partitioned form without global atomics: slots >=
+    4096, and its pass-1 LDS table is shrunk when slots / 4096 partitions
+    would not leave room for the per-partition LDS counters."""
+    part = variant == "part"
     ex = _executor(ctx, executor)
     _register(ex, dev, ["G"], stream)
     if val_cut is not None:
         ex.register_scalar("gb_val_cut", val_cut)
-    part = variant == "part"
     if part:
-        rows = stream.table.rows if stream is not None else dev["G"].rows
-        ex.register_symbol("gbp_rows", rows)
+        ex.register_symbol("gbp_rows", stream.table.rows if stream is not None else dev["G"].rows)
     times = []
     resizes = 0
     sst = None
@@ -190,11 +208,12 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
         if slots is None:
             raise ValueError("give slots or prepass=True")
         slots = max(4096 if part else 64, next_pow2(int(slots)))
+        if part:
+            lds_slots = part_lds_for(slots, lds_slots)
     while True:
+        plan = get_plan(ctx, "GROUPBY", groupby_options(variant, lds_slots, val_cut is not None, prepass, block))
         if not prepass:
             ex.register_symbol("gb_slots", slots)
-        if part:
-            ex.register_symbol("gbp_slack", slack)
         try:
             t, sst = _run(ex, plan, warmup, measured, stream)   # a streamed pre-pass plan is refused by the engine
             times.append(t.total_ms if t else sst.total_ms)
@@ -206,11 +225,15 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
             if resizes > max_resizes:
                 raise
             if part and status[1]:
-                slack *= 2                      # a partition's cold-row stream was full
+                if lds_slots <= 64:
+                    raise
+                lds_slots //= 2                 # more partitions than LDS counters beside the table
             elif prepass:
                 raise                           # exact sizing cannot overflow the table
             else:
                 slots = next_pow2(max(2 * slots, 2 * int(status[2])))
+                if part:
+                    lds_slots = part_lds_for(slots, lds_slots)
     if prepass:
         slots = ex.get_symbol("gb_slots")
     res = ex.collect(plan) if collect else None

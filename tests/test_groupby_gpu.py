@@ -81,25 +81,36 @@ def test_groupby_partitioned_given_sizes(ctx, s, lds):
             _cmp(run.result, exp)
             assert int(run.result.columns["count"].sum()) == 1_000_003
             assert run.timing.status[0][3] > 0 or s > 1.0      # cold rows took the partition path
+            assert run.timing.status[-1][2] == len(exp["key"])  # groups written by the last pass
     finally:
         dev["G"].free(ctx)
 
 
-def test_groupby_partitioned_overflows_recover_exactly(ctx):
-    """Both overflow kinds of the partitioned form end in exact results: a
-    partition's cold-row stream too small (tiny headroom, retried with double
-    headroom) and a partition slice holding more than 4096 groups (retried
-    with a doubled table)."""
+def test_groupby_partitioned_table_overflow_recovers_exactly(ctx):
+    """A partition slice holding more than 4096 groups (table far too small)
+    is reported and retried with a doubled table; results stay exact."""
     G = _gen(1_000_000, 0.0, 200_000, 23)
     exp = oracle.groupby(G)
     dev = Q.upload_tables(ctx, {"G": G})
     try:
-        run = Q.run_groupby(ctx, dev, slots=1 << 20, variant="part", lds_slots=64, slack=4)
-        assert run.resizes >= 1
+        run = Q.run_groupby(ctx, dev, slots=4096, variant="part", lds_slots=64)
+        assert run.resizes >= 1 and run.slots > 4096
         _cmp(run.result, exp)
-        run2 = Q.run_groupby(ctx, dev, slots=4096, variant="part", lds_slots=64)
-        assert run2.resizes >= 1 and run2.slots > 4096
-        _cmp(run2.result, exp)
+    finally:
+        dev["G"].free(ctx)
+
+
+def test_groupby_partitioned_lds_shrinks_for_many_partitions(ctx):
+    """slots / 4096 partitions beyond the LDS counters that fit beside a
+    4096-slot pass-1 table: the driver shrinks the pass-1 table; exact."""
+    G = _gen(600_000, 1.1, 400_000, 29)
+    exp = oracle.groupby(G)
+    assert Q.part_lds_for(1 << 25, 4096) < 4096
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, slots=1 << 25, variant="part", lds_slots=4096)
+        assert run.resizes == 0
+        _cmp(run.result, exp)
     finally:
         dev["G"].free(ctx)
 
