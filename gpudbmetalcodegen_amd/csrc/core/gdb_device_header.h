@@ -194,20 +194,25 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
     if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(w, m);
 }
 
-// A bitmap set behind a block-local cache (1024 entries of LDS u64:
-// word index << 32 | bits this block has set in it). An entry is read and
-// written as one 64-bit LDS access; a lost update only costs a redundant
-// atomic, and a bit is recorded only after its atomic was issued.
+// A bitmap set behind a block-local cache (1024 entries of LDS u64: word
+// index << 32 | the bits this block has set in it). A hit skips the global
+// atomic. An entry for the same word gains bits by an LDS atomic OR, so
+// concurrent lanes setting different bits of one hot word all keep theirs;
+// a different word replaces the entry (a lost entry only costs a later
+// redundant atomic). A bit is recorded only after its global atomic was issued.
 GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64* cache) {
     if ((u64)key >= nbits) return;
     const u32 w = (u32)((u64)key >> 5);
     const u32 m = 1u << ((u32)key & 31u);
     u64* e = cache + (w & 1023u);
-    const u64 c = *(volatile u64*)e;
+    const u64 c = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const bool same = (u32)(c >> 32) == w;
     if (same && ((u32)c & m)) return;
     gdb_atomic_or_u32(bm + w, m);  // no return value: nothing waits on it
-    *(volatile u64*)e = ((u64)w << 32) | (u64)((same ? (u32)c : 0u) | m);
+    if (same)
+        __hip_atomic_fetch_or(e, (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    else
+        __hip_atomic_store(e, ((u64)w << 32) | (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // ---------------------------------------------------------------------------

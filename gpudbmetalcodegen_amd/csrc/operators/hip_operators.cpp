@@ -1073,7 +1073,7 @@ void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     child_->produce(cg, [&]() {
         cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
         cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
-        cg.addBlock("if (gdb_lds_gb_update(_gl, " + s(lbits) + "u, _gk, _gv, 16u) < 0)", [&]() {
+        cg.addBlock("if (gdb_lds_gb_update(_gl, " + s(lbits) + "u, _gk, _gv, " + s(c_.window) + "u) < 0)", [&]() {
             cg.addLine("gdb_lds_add_u32(&_pc[gdb_partition(_gk, _pb)], 1u);");
             cg.addLine("++_spill;");
         });
@@ -1179,7 +1179,7 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.setScanSegment(true);
     child_->produce(cg, [&]() {
         cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
-        cg.addBlock("if (!gdb_lds_key_member(_hk, " + s(lbits) + "u, _gk, 16u))", [&]() {
+        cg.addBlock("if (!gdb_lds_key_member(_hk, " + s(lbits) + "u, _gk, " + s(c_.window) + "u))", [&]() {
             cg.addLine("const u32 _i = __hip_atomic_fetch_add(&_pc[gdb_partition(_gk, _pb)], 1u, __ATOMIC_RELAXED, "
                        "__HIP_MEMORY_SCOPE_WORKGROUP);");
             cg.addLine(n.keys + "[_i] = _gk;");

@@ -329,6 +329,10 @@ struct PartitionConfig {
     std::string rowsSym;    // symbol: rows of the scanned table (all of them)
     std::string scanTable;  // scanned table name
     int ldsSlots = 4096;    // pass-1 LDS table capacity (power of two, 64..4096)
+    // Probe window of the pass-1 LDS table (pass 4 uses the same): a key
+    // whose window is full is cold. Short, because most rows of a
+    // high-cardinality column are cold and pay the whole window in LDS loads.
+    int window = 2;
     static constexpr int kPartSlots = 4096;
     static constexpr int kLogicalBlocks = 512;
     // Largest partition count whose LDS counters fit beside the LDS table.

@@ -257,6 +257,7 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
         pc.rowsSym = "gbp_rows";
         pc.scanTable = "G";
         pc.ldsSlots = ldsSlots;
+        pc.window = optInt(o, "window", 2);
         const int block = optInt(o, "block", 1024);
         auto cnt = std::make_unique<HipPartitionCount>(std::move(cur), pc, "g_key", "g_val");
         auto& p1 = appendPhase(plan, "GBP_count", std::move(cnt), block, optInt(o, "items", 4), {4});

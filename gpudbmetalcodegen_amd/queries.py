@@ -145,8 +145,10 @@ class GroupByRun(QueryRun):
 
 
 def groupby_options(variant: str = "lds", lds_slots: int = 2048, filtered: bool = False, prepass: bool = False,
-                    block: int | None = None) -> str:
+                    block: int | None = None, window: int | None = None) -> str:
     o = [f"variant={variant}", f"lds_slots={lds_slots}"]
+    if window:
+        o.append(f"window={window}")
     if filtered:
         o.append("filter=1")
     if prepass:
@@ -181,7 +183,7 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
                 variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
                 key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
                 warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None,
-                collect: bool = True, stream: Stream | None = None) -> GroupByRun:
+                collect: bool = True, stream: Stream | None = None, window: int | None = None) -> GroupByRun:
     """prepass=True: the plan counts distinct keys exactly first (key-range
     bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
     `slots` (e.g. from the learned estimator) sizes it, and a reported
@@ -211,7 +213,7 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
         if part:
             lds_slots = part_lds_for(slots, lds_slots)
     while True:
-        plan = get_plan(ctx, "GROUPBY", groupby_options(variant, lds_slots, val_cut is not None, prepass, block))
+        plan = get_plan(ctx, "GROUPBY", groupby_options(variant, lds_slots, val_cut is not None, prepass, block, window))
         if not prepass:
             ex.register_symbol("gb_slots", slots)
         try:

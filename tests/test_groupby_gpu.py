@@ -66,8 +66,8 @@ def test_groupby_resize_on_overflow_stays_exact(ctx):
         dev["G"].free(ctx)
 
 
-@pytest.mark.parametrize("s,lds", [(1.1, 4096), (0.0, 1024), (1.5, 64)])
-def test_groupby_partitioned_given_sizes(ctx, s, lds):
+@pytest.mark.parametrize("s,lds,window", [(1.1, 4096, 2), (0.0, 1024, 1), (1.5, 64, 16)])
+def test_groupby_partitioned_given_sizes(ctx, s, lds, window):
     """Two-pass partitioned form at several partition counts (slots / 4096)
     and pass-1 LDS sizes, skewed and uniform keys; every row is counted once
     (hot partials + cold rows)."""
@@ -76,7 +76,7 @@ def test_groupby_partitioned_given_sizes(ctx, s, lds):
     dev = Q.upload_tables(ctx, {"G": G})
     try:
         for slots in (1 << 20, 1 << 21):
-            run = Q.run_groupby(ctx, dev, slots=slots, variant="part", lds_slots=lds)
+            run = Q.run_groupby(ctx, dev, slots=slots, variant="part", lds_slots=lds, window=window)
             assert run.resizes == 0
             _cmp(run.result, exp)
             assert int(run.result.columns["count"].sum()) == 1_000_003
