@@ -22,6 +22,7 @@ typedef long long i64;
 typedef unsigned long long u64;
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned long long u64x2 __attribute__((ext_vector_type(2)));
 
 #define GDB_DEV __device__ __forceinline__
 #define GDB_I64_MAX 0x7FFFFFFFFFFFFFFFll
@@ -196,7 +197,7 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
 
 // A bitmap set behind a block-local cache (1024 entries of LDS u64: word
 // index << 32 | the bits this block has set in it). A hit skips the global
-// atomic. An entry for the same word gains bits by an LDS atomic OR, so
+// read and atomic. An entry for the same word gains bits by an LDS atomic OR, so
 // concurrent lanes setting different bits of one hot word all keep theirs;
 // a different word replaces the entry (a lost entry only costs a later
 // redundant atomic). A bit is recorded only after its global atomic was issued.
@@ -208,7 +209,10 @@ GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64
     const u64 c = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const bool same = (u32)(c >> 32) == w;
     if (same && ((u32)c & m)) return;
-    gdb_atomic_or_u32(bm + w, m);  // no return value: nothing waits on it
+    // Second filter: a plain read that already sees the bit skips the atomic.
+    // Without it, hot words re-missed after a cache eviction are hit by every
+    // block, and atomics on one address serialise.
+    if ((__hip_atomic_load(bm + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(bm + w, m);
     if (same)
         __hip_atomic_fetch_or(e, (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     else

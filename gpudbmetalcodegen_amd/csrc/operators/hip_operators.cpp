@@ -988,7 +988,7 @@ int PartitionConfig::maxParts() const {
 namespace {
 
 struct PartNames {
-    std::string P, NP, rows, hot, ccnt, pcnt, meta, keys, vals, part;
+    std::string P, NP, rows, hot, ccnt, pcnt, meta, rec, part;
 };
 
 // Bind the partition buffers and symbols of a phase. Sizes are expressions
@@ -1004,8 +1004,7 @@ PartNames bindPartition(HipCodegen& cg, const PartitionConfig& c, bool hotW, boo
     n.ccnt = px + "_ccnt";
     n.pcnt = px + "_pcnt";
     n.meta = px + "_meta";
-    n.keys = px + "_keys";
-    n.vals = px + "_vals";
+    n.rec = px + "_rec";
     n.part = px + "_part";
     cg.addSymbolParam(n.NP, n.P);
     cg.addSymbolParam(n.rows, c.rowsSym);
@@ -1013,8 +1012,8 @@ PartNames bindPartition(HipCodegen& cg, const PartitionConfig& c, bool hotW, boo
     cg.addBufferParam(n.ccnt, "u32", 4, LB + " * (" + n.P + ")", InitKind::None, {}, !cntW, true);
     cg.addBufferParam(n.pcnt, "u32", 4, LB + " * (" + n.P + ")", InitKind::None, {}, !cntW, true);
     cg.addBufferParam(n.meta, "u32", 4, "4 * (" + n.P + ")", InitKind::None, {}, !metaW, true);
-    cg.addBufferParam(n.keys, "u32", 4, c.rowsSym + " + 1", InitKind::None, {}, !streamW, true);
-    cg.addBufferParam(n.vals, "i64", 8, c.rowsSym + " + 1", InitKind::None, {}, !streamW, true);
+    // cold rows as 16-byte records (key, value): one store per row
+    cg.addBufferParam(n.rec, "u64", 8, "2 * (" + c.rowsSym + " + 1)", InitKind::None, {}, !streamW, true);
     cg.addBufferParam(n.part, "u64", 8, LB + " * " + CAP + " * 4", InitKind::None, {}, !streamW, true);
     return n;
 }
@@ -1182,8 +1181,8 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addBlock("if (!gdb_lds_key_member(_hk, " + s(lbits) + "u, _gk, " + s(c_.window) + "u))", [&]() {
             cg.addLine("const u32 _i = __hip_atomic_fetch_add(&_pc[gdb_partition(_gk, _pb)], 1u, __ATOMIC_RELAXED, "
                        "__HIP_MEMORY_SCOPE_WORKGROUP);");
-            cg.addLine(n.keys + "[_i] = _gk;");
-            cg.addLine(n.vals + "[_i] = (i64)(" + value_ + ");");
+            cg.addLine("u64x2 _rv; _rv.x = (u64)_gk; _rv.y = (u64)(i64)(" + value_ + ");");
+            cg.addLine("*reinterpret_cast<u64x2*>(" + n.rec + " + 2 * (u64)_i) = _rv;");
         });
         consume();
     });
@@ -1234,8 +1233,9 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addLine("__syncthreads();");
         cg.addLine("const u64 _cb = " + n.meta + "[2 * _P + _p], _cn = " + n.meta + "[_p];");
         cg.addBlock("for (u64 _i = threadIdx.x; _i < _cn; _i += blockDim.x)", [&]() {
-            cg.addLine("const u32 _k = " + n.keys + "[_cb + _i];");
-            cg.addLine("const i64 _v = " + n.vals + "[_cb + _i];");
+            cg.addLine("const u64x2 _rv = *reinterpret_cast<const u64x2*>(" + n.rec + " + 2 * (_cb + _i));");
+            cg.addLine("const u32 _k = (u32)_rv.x;");
+            cg.addLine("const i64 _v = (i64)_rv.y;");
             cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
                        "u, _k, 1u, _v, _v, _v, " + PS + "u) < 0) ++_ovf;");
         });
