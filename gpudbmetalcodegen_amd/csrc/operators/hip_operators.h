@@ -131,6 +131,7 @@ private:
 // Semi-join bitmap build: set bit `key` (metal_operators.cpp:108-116).
 class HipBitmapBuild : public HipUnaryOperator {
 public:
+    bool stageable() const override { return true; }
     HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
                    const std::string& bitsExpr);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;

@@ -327,6 +327,7 @@ void gbCountPhases(HipQueryPlan& plan, bool filter) {
                                                    "gb_key_bits");
         auto& ph = appendPhase(plan, "GBC_bitmap", std::move(bm), 256, 8);
         ph.scalarParams = {"gb_key_min"};
+        ph.staged = true;  // word reads of the batch in flight together
         if (filter) ph.scalarParams.push_back("gb_val_cut");
     }
     {
