@@ -197,10 +197,15 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
 
 // A bitmap set behind a block-local cache (1024 entries of LDS u64: word
 // index << 32 | the bits this block has set in it). A hit skips the global
-// read and atomic. An entry for the same word gains bits by an LDS atomic OR, so
-// concurrent lanes setting different bits of one hot word all keep theirs;
-// a different word replaces the entry (a lost entry only costs a later
-// redundant atomic). A bit is recorded only after its global atomic was issued.
+// atomic. An entry for the same word gains bits by an LDS atomic OR, so
+// concurrent lanes setting different bits of one word all keep theirs; a
+// different word replaces the entry (a lost entry only costs a later
+// redundant atomic). A bit is recorded only after its global atomic was
+// issued. tested: also skip the atomic when a plain read already sees the
+// bit — for skewed keys, where hot words re-missed after an eviction would
+// otherwise be hit by every block (atomics on one address serialise); for
+// near-unique keys the extra read only adds latency.
+template <bool TESTED>
 GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64* cache) {
     if ((u64)key >= nbits) return;
     const u32 w = (u32)((u64)key >> 5);
@@ -209,10 +214,8 @@ GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64
     const u64 c = __hip_atomic_load(e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     const bool same = (u32)(c >> 32) == w;
     if (same && ((u32)c & m)) return;
-    // Second filter: a plain read that already sees the bit skips the atomic.
-    // Without it, hot words re-missed after a cache eviction are hit by every
-    // block, and atomics on one address serialise.
-    if ((__hip_atomic_load(bm + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(bm + w, m);
+    if (!TESTED || (__hip_atomic_load(bm + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u)
+        gdb_atomic_or_u32(bm + w, m);  // no return value: nothing waits on it
     if (same)
         __hip_atomic_fetch_or(e, (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
     else

@@ -293,8 +293,8 @@ std::string HipComputeExpr::describe() const { return "ComputeExpr(" + varName_ 
 // ===========================================================================
 
 HipBitmapBuild::HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::string& bitmapName,
-                               const std::string& keyExpr, const std::string& bitsExpr)
-    : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr) {}
+                               const std::string& keyExpr, const std::string& bitsExpr, bool skewed)
+    : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr), skewed_(skewed) {}
 
 void HipBitmapBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapWriteParam(bitmap_, bits_);
@@ -308,41 +308,30 @@ void HipBitmapBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addLine("__syncthreads();");
     child_->produce(cg, [&]() {
         if (cg.staged()) {
-            // LDS cache check for the whole batch, then the global reads of
-            // the words still needed in flight together, then the atomics
-            const std::string NW = cg.uniqueName("_bn"), GW = cg.uniqueName("_bg");
+            // LDS cache check for the whole batch, then the atomics (keys
+            // near-unique: no global read)
             const std::string k = "((u64)(i64)(" + key_ + "))";
             const std::string nb = "nbits_" + bitmap_;
-            cg.addLine("bool " + NW + "[ITEMS];");
-            cg.addLine("u32 " + GW + "[ITEMS];");
-            cg.addLine("#pragma unroll");
-            cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) " + NW + "[_u] = false;");
             cg.rowLoop([&]() {
                 cg.addBlock("if (" + k + " < " + nb + ")", [&]() {
                     cg.addLine("const u32 _w = (u32)(" + k + " >> 5), _m = 1u << ((u32)" + k + " & 31u);");
-                    cg.addLine("const u64 _c = __hip_atomic_load(" + C +
-                               " + (_w & 1023u), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);");
-                    cg.addLine(NW + "[_u] = !((u32)(_c >> 32) == _w && ((u32)_c & _m));");
+                    cg.addLine("u64* _e = " + C + " + (_w & 1023u);");
+                    cg.addLine("const u64 _c = __hip_atomic_load(_e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);");
+                    cg.addLine("const bool _same = (u32)(_c >> 32) == _w;");
+                    cg.addBlock("if (!(_same && ((u32)_c & _m)))", [&]() {
+                        cg.addLine("gdb_atomic_or_u32(" + bitmap_ + " + _w, _m);");
+                        cg.addLine("if (_same) __hip_atomic_fetch_or(_e, (u64)_m, __ATOMIC_RELAXED, "
+                                   "__HIP_MEMORY_SCOPE_WORKGROUP);");
+                        cg.addLine("else __hip_atomic_store(_e, ((u64)_w << 32) | (u64)_m, __ATOMIC_RELAXED, "
+                                   "__HIP_MEMORY_SCOPE_WORKGROUP);");
+                    });
                 });
             });
-            cg.rowLoop([&]() {
-                cg.addLine(GW + "[_u] = __hip_atomic_load(" + bitmap_ + " + (" + k +
-                           " >> 5), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
-            }, NW);
-            cg.rowLoop([&]() {
-                cg.addLine("const u32 _w = (u32)(" + k + " >> 5), _m = 1u << ((u32)" + k + " & 31u);");
-                cg.addLine("if ((" + GW + "[_u] & _m) == 0u) gdb_atomic_or_u32(" + bitmap_ + " + _w, _m);");
-                cg.addLine("u64* _e = " + C + " + (_w & 1023u);");
-                cg.addLine("const u64 _c = __hip_atomic_load(_e, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);");
-                cg.addLine("if ((u32)(_c >> 32) == _w) __hip_atomic_fetch_or(_e, (u64)_m, __ATOMIC_RELAXED, "
-                           "__HIP_MEMORY_SCOPE_WORKGROUP);");
-                cg.addLine("else __hip_atomic_store(_e, ((u64)_w << 32) | (u64)_m, __ATOMIC_RELAXED, "
-                           "__HIP_MEMORY_SCOPE_WORKGROUP);");
-            }, NW);
             consume();
             return;
         }
-        cg.addLine("gdb_bitmap_set_cached(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" + key_ + "), " + C + ");");
+        cg.addLine(std::string("gdb_bitmap_set_cached<") + (skewed_ ? "true" : "false") + ">(" + bitmap_ + ", nbits_" +
+                   bitmap_ + ", (i64)(" + key_ + "), " + C + ");");
         consume();
     });
 }

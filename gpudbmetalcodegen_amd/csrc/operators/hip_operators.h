@@ -131,14 +131,17 @@ private:
 // Semi-join bitmap build: set bit `key` (metal_operators.cpp:108-116).
 class HipBitmapBuild : public HipUnaryOperator {
 public:
-    bool stageable() const override { return true; }
+    // skewed: keys with heavy repeats across blocks (e.g. a Zipf column); the
+    // set then also skips atomics whose bit a plain read already sees.
+    bool stageable() const override { return !skewed_; }
     HipBitmapBuild(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
-                   const std::string& bitsExpr);
+                   const std::string& bitsExpr, bool skewed = false);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string bitmap_, key_, bits_;
+    bool skewed_ = false;
 };
 
 // Semi-join / anti-join probe (metal_operators.cpp:132-165).

@@ -324,10 +324,9 @@ void gbCountPhases(HipQueryPlan& plan, bool filter) {
         std::unique_ptr<HipOperator> cur = filter ? makeScan("G", {"g_key", "g_val"}) : makeScan("G", {"g_key"});
         if (filter) cur = std::make_unique<HipSelection>(std::move(cur), "g_val < gb_val_cut");
         auto bm = std::make_unique<HipBitmapBuild>(std::move(cur), "d_gb_key_bitmap", "(i64)g_key - gb_key_min",
-                                                   "gb_key_bits");
+                                                   "gb_key_bits", true);
         auto& ph = appendPhase(plan, "GBC_bitmap", std::move(bm), 256, 8);
         ph.scalarParams = {"gb_key_min"};
-        ph.staged = true;  // word reads of the batch in flight together
         if (filter) ph.scalarParams.push_back("gb_val_cut");
     }
     {
