@@ -95,11 +95,11 @@ def next_pow2(x: int) -> int:
 class Workload3:
     """Device-resident pipelines, each with its own runner."""
 
-    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096):
+    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096, window: int = 16):
         self.ctx = ctx
         self.specs = specs
         self.lds_cap = lds_cap
-        self.plans = {k: E.build_hip_plan(plan_name(k), f"lds_cap={lds_cap}" if k == "agg" else "").compile(ctx)
+        self.plans = {k: E.build_hip_plan(plan_name(k), f"lds_cap={lds_cap},window={window}" if k == "agg" else "").compile(ctx)
                       for k in {s.kind for s in specs}}
         self.tables, self.execs, self.bitmaps = [], [], []
         self.launch = {}   # pipe index -> (block, items, lds_bits)

@@ -950,7 +950,7 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         child_->produce(cg, [&]() {
             cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
             cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
-            cg.addBlock("if (gdb_lds_gb_update(_gl_" + T + ", " + LB + ", _gk, _gv, 16u) < 0)", [&]() {
+            cg.addBlock("if (gdb_lds_gb_update(_gl_" + T + ", " + LB + ", _gk, _gv, " + s(window_) + "u) < 0)", [&]() {
                 cg.addLine("const int _r = " + merge1 + ";");
                 cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
                 cg.addLine("++_spill_" + T + ";");

@@ -294,6 +294,10 @@ public:
     // at run time, up to the compiled capacity ldsSlots (autotuned per
     // pipeline without recompiling).
     void setRuntimeLdsBits(const std::string& scalarName) { ldsBitsScalar_ = scalarName; }
+    // LDS probe window: rows whose key finds no slot within it spill to the
+    // global table (a long window absorbs more keys, a short one makes the
+    // misses of high-cardinality keys cheap).
+    void setWindow(int w) { window_ = w; }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
@@ -302,6 +306,7 @@ private:
     Variant variant_;
     int ldsSlots_;
     std::string ldsBitsScalar_;
+    int window_ = 16;
 };
 
 // Partitioned hash group-by: no counterpart in the reference, whose

@@ -290,6 +290,7 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     auto gb = std::make_unique<HipHashGroupBy>(std::move(cur), "d_gb_ht", "g_key", "g_val", "gb_slots",
                                                variant == "global" ? HipHashGroupBy::GLOBAL : HipHashGroupBy::LDS,
                                                ldsSlots);
+    gb->setWindow(optInt(o, "window", 16));
     auto& ph = appendPhase(plan, "GB_agg", std::move(gb), optInt(o, "block", 512), optInt(o, "items", 4), {4});
     ph.launchBound = 1024;
     if (filter) {
@@ -434,6 +435,7 @@ HipQueryPlan buildPipeAgg(const PlanOptions& o) {
     auto gb = std::make_unique<HipHashGroupBy>(std::move(sel), "d_pipe_ht", "p_key", "p_val", "pipe_slots",
                                                HipHashGroupBy::LDS, optInt(o, "lds_cap", 2048));
     gb->setRuntimeLdsBits("pipe_lds_bits");
+    gb->setWindow(optInt(o, "window", 16));
     auto& ph = appendPhase(plan, "PIPE_agg", std::move(gb), optInt(o, "block", 256), optInt(o, "items", 4),
                            {1, 2, 4, 8, 16});
     ph.launchBound = 1024;
