@@ -95,7 +95,7 @@ def next_pow2(x: int) -> int:
 class Workload3:
     """Device-resident pipelines, each with its own runner."""
 
-    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096, window: int = 16):
+    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096, window: int = 4):
         self.ctx = ctx
         self.specs = specs
         self.lds_cap = lds_cap

@@ -343,7 +343,7 @@ struct PartitionConfig {
     // high-cardinality column are cold and pay the whole window in LDS loads.
     int window = 2;
     static constexpr int kPartSlots = 4096;
-    static constexpr int kLogicalBlocks = 512;
+    static constexpr int kLogicalBlocks = 256;
     // Largest partition count whose LDS counters fit beside the LDS table.
     int maxParts() const;
 };

@@ -435,7 +435,7 @@ HipQueryPlan buildPipeAgg(const PlanOptions& o) {
     auto gb = std::make_unique<HipHashGroupBy>(std::move(sel), "d_pipe_ht", "p_key", "p_val", "pipe_slots",
                                                HipHashGroupBy::LDS, optInt(o, "lds_cap", 2048));
     gb->setRuntimeLdsBits("pipe_lds_bits");
-    gb->setWindow(optInt(o, "window", 16));
+    gb->setWindow(optInt(o, "window", 4));  // measured best for the batched pipelines
     auto& ph = appendPhase(plan, "PIPE_agg", std::move(gb), optInt(o, "block", 256), optInt(o, "items", 4),
                            {1, 2, 4, 8, 16});
     ph.launchBound = 1024;
