@@ -96,7 +96,11 @@ def bench_join(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     T = W.workload1()
     dev = Q.upload_tables(ctx, {k: T[k] for k in ("A", "B", "F")})
     rows = sum(dev[k].rows for k in ("A", "B", "F"))
-    run = Q.run_q3(ctx, dev, W.W1_SEG, W.W1_DATE)  # compile + first run
+    # load-time column statistics (histograms) size the B hash table; an
+    # overflow would double it and rerun, so the result stays exact
+    stats = Q.q3_table_stats(T)
+    b_slots_est = Q.q3_slots_from_stats(stats, W.W1_SEG, W.W1_DATE)
+    run = Q.run_q3(ctx, dev, W.W1_SEG, W.W1_DATE, options="sizing=given", b_slots=b_slots_est)  # compile + first run
     ex = run.executor
     plan = run.plan
     log(f"workload 1: {rows} rows; timing {steps} steps after {warmup} warmups")
@@ -108,6 +112,16 @@ def bench_join(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     ctx.sync()
     wall_ms = (time.perf_counter() - t0) * 1e3 / steps
     res = ex.collect(plan)
+    # the exact-count pre-pass form (a count kernel and a host round trip before the build)
+    run_pp = Q.run_q3(ctx, dev, W.W1_SEG, W.W1_DATE)
+    run_pp.executor.execute(run_pp.plan, 1, 1)
+    ctx.sync()
+    t0 = time.perf_counter()
+    run_pp.executor.execute(run_pp.plan, 0, steps)
+    ctx.sync()
+    pp_ms = (time.perf_counter() - t0) * 1e3 / steps
+    same_pp = run_pp.executor.collect(run_pp.plan).canonical == res.canonical
+    run_pp.executor.release()
     b_slots = ex.get_symbol("b_slots")
     pb = q3_phase_bytes(T, W.W1_SEG, W.W1_DATE, b_slots)
     phases = {name: ms for name, ms, *_ in t.phases}
@@ -123,6 +137,8 @@ def bench_join(ctx, steps: int, warmup: int, cpu: bool) -> dict:
                                 if phases[k] > 0 and k in pb},
         "roofline": roofline(dom, phases[dom], pb[dom], "w1"),
         "b_slots": b_slots,
+        "sizing": "load-time column statistics (resize on overflow; %d resizes)" % getattr(run, "resizes", 0),
+        "exact_prepass_form": {"ms_per_step": pp_ms, "rows_per_s": rows / (pp_ms / 1e3), "identical": bool(same_pp)},
     }
     gpu_top = {k: [int(x) for x in v] for k, v in res.columns.items()}
     if cpu:
@@ -458,6 +474,7 @@ def main() -> int:
         "dtype": "int64",
         "data": "synthetic (seeded counter-based generators)",
         "config": {"workload": head["workload"], "rows_per_step": head["rows"], "form": "plain per-query",
+                   "sizing": head["sizing"], "exact_prepass_form": head["exact_prepass_form"],
                    "device": ctx.info.name.decode(), "cus": ctx.cu_count},
         "roofline": head["roofline"],
         "cpu_baseline": head.get("cpu_baseline"),

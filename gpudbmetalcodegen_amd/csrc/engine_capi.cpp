@@ -327,7 +327,9 @@ static void registerPlanTables(gdb_executor ex, gdb_plan p) {
 // Install the plan's sizing pre-pass rules as phase hooks on the runner.
 static void installSizingRules(gdb_executor ex, gdb_plan p) {
     ex->ex->clearPhaseHooks();
+    ex->ex->clearLateSymbols();
     for (const auto& rule : p->plan.sizingRules) {
+        ex->ex->addLateSymbol(rule.symbol);
         ex->ex->setPhaseHook(rule.afterPhase, [rule](HipGenericExecutor& e) {
             uint64_t count = 0;
             e.readBuffer(rule.buffer, static_cast<size_t>(rule.word) * 8, &count, 8);

@@ -1,5 +1,7 @@
 #include "hip_generic_executor.h"
 
+#include <cctype>
+
 #include <algorithm>
 #include <chrono>
 #include <cstring>
@@ -108,6 +110,66 @@ std::unordered_map<std::string, const HipParamBinding*> HipGenericExecutor::init
             if (b.kind == HipParamKind::DeviceBuffer && b.init != InitKind::None && !spec.count(b.name))
                 spec[b.name] = &b;
     return spec;
+}
+
+namespace {
+
+bool mentionsSymbol(const std::string& expr, const std::string& sym) {
+    size_t pos = 0;
+    while ((pos = expr.find(sym, pos)) != std::string::npos) {
+        const bool startOk = pos == 0 || !(std::isalnum(static_cast<unsigned char>(expr[pos - 1])) || expr[pos - 1] == '_');
+        const size_t end = pos + sym.size();
+        const bool endOk = end >= expr.size() || !(std::isalnum(static_cast<unsigned char>(expr[end])) || expr[end] == '_');
+        if (startOk && endOk) return true;
+        pos = end;
+    }
+    return false;
+}
+
+}  // namespace
+
+void HipGenericExecutor::hoistInits(const HipCodegen& cg, int first, int last,
+                                    std::unordered_map<std::string, bool>& initDone,
+                                    const std::unordered_map<std::string, const HipParamBinding*>& initSpec) {
+    if (skipInit_) return;
+    std::vector<void*> dst;
+    std::vector<size_t> bytes;
+    std::vector<const void*> pats;
+    std::vector<int> plen;
+    std::vector<uint8_t> zero(8, 0);
+    const auto& phases = cg.getPhases();
+    for (int pi = first; pi < last; ++pi) {
+        const auto& p = phases[static_cast<size_t>(pi)];
+        for (const auto& b : p.bindings) {
+            if (b.kind != HipParamKind::DeviceBuffer || initDone[b.name]) continue;
+            auto sp = initSpec.find(b.name);
+            if (sp == initSpec.end() || sp->second != &b) continue;  // not defined by this phase
+            const HipParamBinding* ib = sp->second;
+            bool late = false;
+            for (const auto& sym : lateSymbols_) late |= mentionsSymbol(ib->sizeExpr, sym);
+            if (late) continue;
+            uint64_t elems = 0;
+            try {
+                elems = sizes_.resolve(ib->sizeExpr);
+            } catch (const std::exception&) {
+                continue;  // sized by something not known yet: initialised at its phase
+            }
+            const size_t nb = static_cast<size_t>(elems) * ib->elemBytes;
+            const int pl = ib->init == InitKind::Zero ? 8 : static_cast<int>(ib->pattern.size());
+            if (nb == 0 || nb % 8 != 0 || pl > 32 || (32 % pl) != 0 || nb % static_cast<size_t>(pl) != 0) continue;
+            Buffer& buf = ensureBuffer(*ib);
+            dst.push_back(buf.ptr);
+            bytes.push_back(nb);
+            pats.push_back(ib->init == InitKind::Zero ? static_cast<const void*>(zero.data())
+                                                      : static_cast<const void*>(ib->pattern.data()));
+            plen.push_back(pl);
+            initDone[b.name] = true;
+        }
+    }
+    if (!dst.empty())
+        check(gdb_fill_multi_async(static_cast<int>(dst.size()), dst.data(), bytes.data(), pats.data(), plen.data(),
+                                   stream_),
+              "gdb_fill_multi_async");
 }
 
 int HipGenericExecutor::occupancy(gdb_kernel k, int block) {
@@ -276,6 +338,7 @@ void HipGenericExecutor::enqueue(RuntimeCompiler& rc, const RuntimeCompiler::Com
     const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
     auto spec = initSpecs(cg);
     std::unordered_map<std::string, bool> initDone;
+    hoistInits(cg, firstPhase, last, initDone, spec);
     for (int pi = firstPhase; pi < last; ++pi)
         launchPhase(rc, cq, phases[static_cast<size_t>(pi)], initDone, spec, nullptr);
 }
@@ -310,6 +373,7 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
         res.phaseBlock.clear();
         res.phaseItems.clear();
         check(gdb_event_record(events_[0], stream_), "gdb_event_record");
+        hoistInits(cg, firstPhase, last, initDone, spec);
         for (int pi = firstPhase; pi < last; ++pi) {
             const auto& p = phases[static_cast<size_t>(pi)];
             unsigned grid = 0;

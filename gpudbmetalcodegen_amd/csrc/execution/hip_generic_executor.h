@@ -71,6 +71,11 @@ public:
     void clearPhaseHooks() { hooks_.clear(); }
     // When true, buffers are not re-initialised (accumulate across runs).
     void setSkipInit(bool skip) { skipInit_ = skip; }
+    // Symbols a phase hook sets during a run: buffers sized by them are
+    // initialised at their phase, the others all at the start of a run in
+    // one multi-buffer fill.
+    void addLateSymbol(const std::string& name) { lateSymbols_.push_back(name); }
+    void clearLateSymbols() { lateSymbols_.clear(); }
 
     // Allocate every buffer of phases [first, last) that can be sized now.
     void prepare(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
@@ -127,6 +132,7 @@ private:
     std::unordered_map<std::string, std::function<void(HipGenericExecutor&)>> hooks_;
     std::map<std::pair<gdb_kernel, int>, int> occupancy_;
     bool skipInit_ = false;
+    std::vector<std::string> lateSymbols_;
     gdb_event evA_ = nullptr;
     std::vector<gdb_event> events_;
     std::vector<float> measuredPhase_, measuredFill_;
@@ -137,6 +143,10 @@ private:
                      std::unordered_map<std::string, bool>& initDone, const std::unordered_map<std::string, const HipParamBinding*>& initSpec,
                      unsigned* gridOut, gdb_event beforeKernel = nullptr);
     std::unordered_map<std::string, const HipParamBinding*> initSpecs(const HipCodegen& cg) const;
+    // Initialise, in one launch, every buffer defined by phases [first, last)
+    // whose size is known now; marks them done.
+    void hoistInits(const HipCodegen& cg, int first, int last, std::unordered_map<std::string, bool>& initDone,
+                    const std::unordered_map<std::string, const HipParamBinding*>& initSpec);
     int occupancy(gdb_kernel k, int block);
 };
 

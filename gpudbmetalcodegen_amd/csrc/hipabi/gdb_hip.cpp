@@ -183,6 +183,25 @@ __global__ void __launch_bounds__(256) gdb_fill_kernel(unsigned long long* __res
     }
 }
 
+// Multi-buffer fill: one launch initialises up to kFillMax buffers, each with
+// a pattern of up to 32 bytes (4 words, repeated); blockIdx.y selects the
+// buffer. Used to issue all of a run's buffer initialisations at once.
+constexpr int kFillMax = 12;
+struct FillDesc {
+    unsigned long long* dst;
+    unsigned long long nwords;
+    unsigned long long w[4];
+};
+struct FillDescs {
+    FillDesc d[kFillMax];
+};
+
+__global__ void __launch_bounds__(256) gdb_fill_multi_kernel(FillDescs ds) {
+    const FillDesc& f = ds.d[blockIdx.y];
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < f.nwords; i += stride) f.dst[i] = f.w[i & 3];
+}
+
 }  // namespace
 
 struct gdb_ctx_s {
@@ -339,6 +358,39 @@ gdb_status gdb_memcpy_d2d_async(void* dst, const void* src, size_t bytes, gdb_st
     if (!bytes) return GDB_OK;
     GDB_HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, static_cast<hipStream_t>(s)),
                 "hipMemcpyAsync(D2D)");
+    return GDB_OK;
+}
+
+gdb_status gdb_fill_multi_async(int n, void* const* dst, const size_t* bytes, const void* const* patterns,
+                                const int* pattern_len, gdb_stream s) {
+    if (n <= 0) return GDB_OK;
+    if (!dst || !bytes || !patterns || !pattern_len) return fail(GDB_ERR_INVALID, "gdb_fill_multi_async: null argument");
+    for (int i = 0; i < n; ++i) {
+        const int pl = pattern_len[i];
+        if (!dst[i] || !patterns[i] || pl < 1 || pl > 32 || (32 % pl) != 0 || bytes[i] % 8 != 0 ||
+            bytes[i] % static_cast<size_t>(pl) != 0 || reinterpret_cast<uintptr_t>(dst[i]) % 8 != 0)
+            return fail(GDB_ERR_INVALID, "gdb_fill_multi_async: buffer " + std::to_string(i) +
+                                             " needs 8-byte alignment and size and a pattern dividing 32 bytes");
+    }
+    for (int b = 0; b < n; b += kFillMax) {
+        const int m = std::min(kFillMax, n - b);
+        FillDescs ds{};
+        size_t maxWords = 0;
+        for (int j = 0; j < m; ++j) {
+            FillDesc& f = ds.d[j];
+            f.dst = static_cast<unsigned long long*>(dst[b + j]);
+            f.nwords = bytes[b + j] / 8;
+            unsigned char* pb = reinterpret_cast<unsigned char*>(f.w);
+            for (int k = 0; k < 32; ++k) pb[k] = static_cast<const unsigned char*>(patterns[b + j])[k % pattern_len[b + j]];
+            maxWords = std::max<size_t>(maxWords, f.nwords);
+        }
+        size_t blocks = (maxWords + 255) / 256;
+        if (blocks > 4096) blocks = 4096;
+        if (blocks == 0) blocks = 1;
+        hipLaunchKernelGGL(gdb_fill_multi_kernel, dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(m)), dim3(256),
+                           0, static_cast<hipStream_t>(s), ds);
+        GDB_HIP_TRY(hipGetLastError(), "gdb_fill_multi_kernel launch");
+    }
     return GDB_OK;
 }
 

@@ -97,13 +97,79 @@ def _executor(ctx, executor):
 # ---------------------------------------------------------------------------
 # Workload 1: Q3 analog (semi-join bitmap -> hash build -> probe -> top-10)
 # ---------------------------------------------------------------------------
+@dataclass
+class ColumnStats:
+    """Load-time statistics of an integer column (computed once, like the
+    reference's cached max-key scan, codegen/execution/max_key_symbols.cpp:28-41):
+    exact value frequencies when the column has at most 64 distinct values,
+    otherwise an equi-width histogram of `bins` buckets over [lo, hi]."""
+    lo: int
+    hi: int
+    n: int
+    counts: np.ndarray
+    values: np.ndarray | None = None   # categorical: the distinct values
+
+    def frac_eq(self, v: int) -> float:
+        if self.n == 0:
+            return 0.0
+        if self.values is not None:
+            hit = np.nonzero(self.values == v)[0]
+            return float(self.counts[hit[0]]) / self.n if hit.size else 0.0
+        return self.frac_lt(v + 1) - self.frac_lt(v)
+
+    def frac_lt(self, v: int) -> float:
+        """Estimated fraction of rows < v (uniform within a bucket)."""
+        if self.n == 0 or v <= self.lo:
+            return 0.0
+        if v > self.hi:
+            return 1.0
+        if self.values is not None:
+            return float(self.counts[self.values < v].sum()) / self.n
+        nb = self.counts.size
+        width = (self.hi - self.lo + 1) / nb
+        pos = (v - self.lo) / width
+        k = int(pos)
+        part = float(self.counts[:k].sum()) + (pos - k) * float(self.counts[k] if k < nb else 0)
+        return part / self.n
+
+
+def column_stats(col: np.ndarray, bins: int = 256) -> ColumnStats:
+    col = np.asarray(col)
+    n = int(col.shape[0])
+    if n == 0:
+        return ColumnStats(0, 0, 0, np.zeros(1, np.int64))
+    lo, hi = int(col.min()), int(col.max())
+    if hi - lo < 64:
+        vals, cnt = np.unique(col, return_counts=True)
+        return ColumnStats(lo, hi, n, cnt.astype(np.int64), vals.astype(np.int64))
+    cnt, _ = np.histogram(col, bins=bins, range=(lo, hi + 1))
+    return ColumnStats(lo, hi, n, cnt.astype(np.int64))
+
+
+def q3_table_stats(T: dict) -> dict:
+    """The statistics the join's table sizing uses, gathered at load time."""
+    return {"b_date": column_stats(T["B"]["b_date"]), "a_seg": column_stats(T["A"]["a_seg"]),
+            "n_b": int(T["B"]["b_key"].shape[0])}
+
+
+def q3_slots_from_stats(stats: dict, seg: int, d: int, margin: float = 2.0) -> int:
+    """B hash-table slots from statistics: estimated qualifying B rows (date
+    selectivity x segment frequency of the referenced A rows, independence
+    assumed) x margin, as a power of two."""
+    est = stats["n_b"] * stats["b_date"].frac_lt(d) * stats["a_seg"].frac_eq(seg)
+    return next_pow2(max(64, int(margin * est) + 1))
+
+
 def run_q3(ctx: E.Context, dev: dict, seg: int = W.W1_SEG, d: int = W.W1_DATE, options: str = "",
            max_a: int | None = None, max_b: int | None = None, b_slots: int | None = None, warmup: int = 0,
-           measured: int = 1, executor: E.HipGenericExecutor | None = None, stream: Stream | None = None) -> QueryRun:
+           measured: int = 1, executor: E.HipGenericExecutor | None = None, stream: Stream | None = None,
+           max_resizes: int = 8) -> QueryRun:
     """Plain form: the plan's own exact-count pre-pass sizes the B hash table
-    (options ``sizing=given`` takes ``b_slots`` instead). Key-range symbols
-    come from table statistics (max key), as the reference's cached max-key
-    mode does (codegen/execution/max_key_symbols.cpp:28-41)."""
+    (options ``sizing=given`` takes ``b_slots`` instead, e.g. from
+    q3_slots_from_stats; a reported overflow doubles it and reruns, so results
+    stay exact). Key-range symbols come from table statistics (max key), as
+    the reference's cached max-key mode does
+    (codegen/execution/max_key_symbols.cpp:28-41)."""
     plan = get_plan(ctx, "Q3", options)
     ex = _executor(ctx, executor)
     _register(ex, dev, ("A", "B", "F"), stream)
@@ -111,12 +177,25 @@ def run_q3(ctx: E.Context, dev: dict, seg: int = W.W1_SEG, d: int = W.W1_DATE, o
     ex.register_scalar("q3_date", d)
     ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
     ex.register_symbol("b_bits", (max_b if max_b is not None else dev["B"].rows) + 1)
-    if "sizing=given" in options:
-        if b_slots is None:
-            raise ValueError("sizing=given needs b_slots")
+    if "sizing=given" not in options:
+        t, st = _run(ex, plan, warmup, measured, stream)
+        return QueryRun(ex.collect(plan), t, plan, ex, st)
+    if b_slots is None:
+        raise ValueError("sizing=given needs b_slots")
+    resizes = 0
+    while True:
         ex.register_symbol("b_slots", b_slots)
-    t, st = _run(ex, plan, warmup, measured, stream)
-    return QueryRun(ex.collect(plan), t, plan, ex, st)
+        try:
+            t, st = _run(ex, plan, warmup, measured, stream)
+            break
+        except E.GdbOverflow:
+            resizes += 1
+            if resizes > max_resizes:
+                raise
+            b_slots *= 2                      # the estimate was too small: double and rerun
+    run = QueryRun(ex.collect(plan), t, plan, ex, st)
+    run.resizes = resizes
+    return run
 
 
 def q3_hash_groups(run: QueryRun) -> dict:

@@ -95,6 +95,12 @@ gdb_status gdb_memcpy_d2d_async(void* dst, const void* src, size_t bytes, gdb_st
  * (codegen/execution/metal_generic_executor.cpp:164-175). */
 gdb_status gdb_fill_async(void* dst, size_t bytes, const void* pattern, int pattern_len,
                           gdb_stream s);
+/* Fill n buffers in one launch (no counterpart in the reference, whose buffer
+ * initialisation is a host memset per buffer, metal_generic_executor.cpp:164-175):
+ * buffer i gets patterns[i] (1..32 bytes, a divisor of 32) repeated;
+ * sizes and addresses must be multiples of 8. Graph-safe like gdb_fill_async. */
+gdb_status gdb_fill_multi_async(int n, void* const* dst, const size_t* bytes, const void* const* patterns,
+                                const int* pattern_len, gdb_stream s);
 gdb_status gdb_memset_async(void* dst, int value, size_t bytes, gdb_stream s);
 
 /* JIT. Compile HIP C++ for the context's architecture with hipRTC, or take the
