@@ -116,6 +116,7 @@ public:
 
     // --- staged (item-batched) emission; see hip_operators.h ---
     void setPhaseStaged(bool staged);
+    void setPhasePrefetch(bool on) { cur("setPhasePrefetch").prefetch = on; }
     bool phaseStaged() const;
     // True while emitting for a batch of ITEMS rows with active mask _act.
     bool staged() const { return staged_; }
@@ -176,6 +177,7 @@ public:
         std::string statusBuffer;  // optional overflow/status words checked after the phase
         std::string ldsBytesExpr = "0u";  // range mode: LDS bytes of the body (constant expression)
         bool staged = false;              // item-batched emission (hip_operators.h)
+        bool prefetch = false;            // scans load the next tile before processing this one
         std::vector<HipParamBinding> bindings;
     };
     const std::vector<PhaseInfo>& getPhases() const { return phases_; }

@@ -103,10 +103,52 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const std::string n = range ? "_re" : nParam;
     const int lb = cg.getPhases().back().launchBound;
 
+    bool anyCharCol = false;
+    for (const auto& c : columns_) anyCharCol |= c.type.type == ColType::CHAR;
+    // Software pipelining across tiles (grid mode, fixed-width columns): the
+    // next tile's column loads are in flight while this tile is processed.
+    const bool prefetch = cg.getPhases().back().prefetch && !range && !anyCharCol;
+    // Column loads of the tile starting at `tileExpr` (empty: the current
+    // tile, _r0/_full in scope) into arrays <prefix><column>.
+    auto emitLoads = [&](const std::string& tileExpr, const std::string& prefix) {
+        cg.addBlock("", [&]() {
+            std::string r0 = "_r0", full = "_full";
+            if (!tileExpr.empty()) {
+                cg.addLine("const u64 _tn = " + tileExpr + ";");
+                cg.addLine("const u64 _rn = _tn + (u64)threadIdx.x * ITEMS;");
+                cg.addLine("const bool _fn = _tn + _tile <= " + n + ";");
+                r0 = "_rn";
+                full = "_fn";
+            }
+            cg.addBlock(tileExpr.empty() ? "if (" + full + ")" : "if (_tn < " + n + " && " + full + ")", [&]() {
+                for (size_t k = 0; k < columns_.size(); ++k) {
+                    const auto& c = columns_[k];
+                    if (c.type.type == ColType::CHAR) continue;
+                    cg.addLine("gdb_load_items<" + c.type.cxx() + ", ITEMS>(" + ptr[k] + " + " + r0 + ", " + prefix +
+                               c.name + ");");
+                }
+            }, tileExpr.empty() ? " else {" : " else if (_tn < " + n + ") {");
+            {
+                IndentGuard g(cg);
+                for (size_t k = 0; k < columns_.size(); ++k) {
+                    const auto& c = columns_[k];
+                    if (c.type.type == ColType::CHAR) continue;
+                    cg.addLine("gdb_load_tail<" + c.type.cxx() + ", ITEMS>(" + ptr[k] + ", " + r0 + ", " + n + ", " +
+                               prefix + c.name + ");");
+                }
+            }
+            cg.addLine("}");
+        });
+    };
+
     cg.addComment("scan " + table_ + ": tiles of blockDim.x*ITEMS rows, ITEMS consecutive rows per thread");
     cg.addBlock("", [&]() {
         cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
         cg.addLine(range ? "const u64 _step = _tile;" : "const u64 _step = (u64)gridDim.x * _tile;");
+        if (prefetch) {
+            for (const auto& c : columns_) cg.addLine(c.type.cxx() + " _p_" + c.name + "[ITEMS];");
+            emitLoads("(u64)blockIdx.x * _tile", "_p_");
+        }
         for (size_t k = 0; k < columns_.size(); ++k) {
             const auto& c = columns_[k];
             if (c.type.type == ColType::CHAR) {
@@ -146,23 +188,16 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                 cg.addLine(T + " _r_" + c.name + "[ITEMS];");
             }
             if (anyChar) cg.addLine("__syncthreads();");
-            cg.addBlock("if (_full)", [&]() {
-                for (size_t k = 0; k < columns_.size(); ++k) {
-                    const auto& c = columns_[k];
-                    if (c.type.type == ColType::CHAR) continue;
-                    cg.addLine("gdb_load_items<" + c.type.cxx() + ", ITEMS>(" + ptr[k] + " + _r0, _r_" + c.name + ");");
+            if (prefetch) {
+                cg.addComment("this tile's columns were loaded one iteration ahead; issue the next tile's now");
+                for (const auto& c : columns_) {
+                    cg.addLine("#pragma unroll");
+                    cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) _r_" + c.name + "[_u] = _p_" + c.name + "[_u];");
                 }
-            }, " else {");
-            {
-                IndentGuard g(cg);
-                for (size_t k = 0; k < columns_.size(); ++k) {
-                    const auto& c = columns_[k];
-                    if (c.type.type == ColType::CHAR) continue;
-                    cg.addLine("gdb_load_tail<" + c.type.cxx() + ", ITEMS>(" + ptr[k] + ", _r0, " + n + ", _r_" +
-                               c.name + ");");
-                }
+                emitLoads("_t0 + _step", "_p_");
+            } else {
+                emitLoads("", "_r_");
             }
-            cg.addLine("}");
             if (cg.phaseStaged()) {
                 cg.addLine("bool _act[ITEMS];");
                 cg.addLine("#pragma unroll");

@@ -104,6 +104,7 @@ HipQueryPlan buildQ1(const PlanOptions& o) {
                            {2, 4, 8});
     ph.launchBound = 256;
     ph.scalarParams = {"q1_cutoff"};
+    ph.prefetch = optInt(o, "prefetch", 0) != 0;
     plan.requiredScalars = {"q1_cutoff"};
     plan.chunkable = true;
     return plan;
@@ -186,6 +187,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
         ph.staged = optInt(o, "staged", 1) != 0;  // home-slot CAS of the batch in flight together
+        ph.prefetch = optInt(o, "prefetch", 0) != 0;
     }
     {
         auto scan = makeScan("F", {"f_fkey", "f_price", "f_disc", "f_date"});
@@ -201,6 +203,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
                                {4, 8});
         ph.scalarParams = {"q3_date"};
         ph.staged = optInt(o, "staged", 0) != 0;
+        ph.prefetch = optInt(o, "prefetch", 0) != 0;
     }
     {
         auto scan = std::make_unique<HipHashTableScan>("d_q3_b_ht", "b_slots", 2, "ht");
@@ -543,6 +546,7 @@ HipCodegen generateFromPlan(const HipQueryPlan& plan, bool rangeMode) {
         cg.setPhaseItems(phase.items);
         cg.setPhaseLaunchBound(phase.launchBound);
         cg.setPhaseStaged(phase.staged);
+        cg.setPhasePrefetch(phase.prefetch);
         if (phase.maxThreadgroups) cg.setPhaseMaxThreadgroups(phase.maxThreadgroups);
         for (const auto& sp : phase.scalarParams) cg.addScalarParam(sp);
         for (const auto& eb : phase.externalBuffers)

@@ -36,6 +36,9 @@ struct HipQueryPlan {
         // Item-batched emission: the random loads (bitmap words, first hash
         // slots) of a thread's ITEMS rows are issued together.
         bool staged = false;
+        // Software-pipelined scan: the next tile's columns load while this
+        // tile is processed.
+        bool prefetch = false;
         std::vector<std::string> scalarParams;  // registered before the operators run
         // Buffers the caller registers (e.g. LIKE words, prebuilt bitmaps):
         // (name, element type) read-only parameters.

@@ -40,7 +40,7 @@ def test_q1_cutoffs(ctx, cutoff):
     assert got == q1_expected(oracle.q1(F, cutoff))
 
 
-@pytest.mark.parametrize("options", ["items=2", "items=8,block=128", "items=4,block=64"])
+@pytest.mark.parametrize("options", ["items=2", "items=8,block=128", "items=4,block=64", "prefetch=1", "prefetch=1,items=2"])
 def test_q1_launch_variants(ctx, options):
     F = W.fact_f(333_333, W.seed_for(0) + 3)
     got, _ = _run(ctx, F, W.q1_cutoff(), options)
