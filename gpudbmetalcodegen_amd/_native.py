@@ -177,7 +177,17 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_result_i64", [vp, i32, vp])
     _sig(eng, "gdb_result_i128", [vp, i32, vp])
     _sig(eng, "gdb_result_str", [vp, i32, vp, sz])
+    _sig(eng, "gdb_result_column_format", [vp, i32, P(i32), P(i32)])
     _sig(eng, "gdb_result_canonical", [vp, vp, sz, P(sz)])
+    # TPC-H ingest and the golden check
+    _sig(eng, "gdb_tpch_column", [cstr, cstr, P(i32), P(i32), P(i32)])
+    _sig(eng, "gdb_tpch_table_columns", [cstr, vp, sz, P(sz)])
+    _sig(eng, "gdb_tbl_to_colbin", [cstr, cstr, i32, i32, P(i32)])
+    _sig(eng, "gdb_colbin_open", [cstr, cstr, P(vp)])
+    _sig(eng, "gdb_colbin_shape", [vp, P(u64), P(i32)])
+    _sig(eng, "gdb_colbin_column", [vp, i32, P(vp), P(u64), P(i32), P(i32)])
+    _sig(eng, "gdb_colbin_close", [vp], None)
+    _sig(eng, "gdb_compare_canonical", [cstr, cstr, C.c_double, C.c_double, vp, sz, P(sz)])
     _sig(eng, "gdb_result_free", [vp])
     # gdb_kernels.h (in libgdbhip.so)
     _sig(hip, "gdb_kernels_last_error", [], C.c_char_p)

@@ -133,6 +133,9 @@ GDB_DEV int gdb_wave() { return (int)(threadIdx.x >> 6); }
 GDB_DEV void gdb_atomic_add_u64(u64* p, u64 v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+GDB_DEV u64 gdb_atomic_add_u64_ret(u64* p, u64 v) {
+    return __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 GDB_DEV void gdb_atomic_add_u32(u32* p, u32 v) {
     __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -397,6 +400,28 @@ GDB_DEV bool gdb_like2(const char* s, int width, const char* w1, int l1, const c
     for (int i = p + l1; i + l2 <= len; ++i)
         if (gdb_match_at(s, i, w2, l2)) return true;
     return false;
+}
+
+// LIKE 'p%' on a fixed-width NUL-padded field: the field starts with the l
+// bytes of p (a field shorter than l, i.e. NUL inside the prefix, fails).
+GDB_DEV bool gdb_like_prefix(const char* s, int width, const char* p, int l) {
+    if (l > width) return false;
+    for (int i = 0; i < l; ++i)
+        if (s[i] != p[i]) return false;
+    return true;
+}
+
+// Wave-aggregated append: every active lane gets a distinct index from the
+// counter; one atomic per wave (the first active lane's), the base is
+// broadcast and each lane adds the number of active lanes below it.
+GDB_DEV u64 gdb_wave_append(u64* counter) {
+    const u64 mask = __ballot(1);
+    const int leader = __builtin_ctzll(mask);
+    const u32 lane = gdb_lane();
+    u64 base = 0;
+    if (lane == (u32)leader) base = gdb_atomic_add_u64_ret(counter, (u64)__builtin_popcountll(mask));
+    base = __shfl(base, leader);
+    return base + (u64)__builtin_popcountll(mask & ((1ull << lane) - 1ull));
 }
 
 // ---------------------------------------------------------------------------

@@ -84,7 +84,7 @@ private:
 enum class AggKind { SumI64, SumI128, Count, MinI64, MaxI64 };
 
 struct HipResultSchema {
-    enum Kind { NONE, KEYED_AGG, HASH_AGG, TOPK, HISTOGRAM, COUNTS };
+    enum Kind { NONE, KEYED_AGG, HASH_AGG, TOPK, HISTOGRAM, COUNTS, ROWS };
     Kind kind = NONE;
 
     struct AggSlot {
@@ -128,6 +128,8 @@ struct HipResultSchema {
         int shift = 0;
         int bits = 64;
         long long complementOf = 0;  // value = complementOf - field when != 0
+        int scale = 0;               // fixed-point digits (display only)
+        bool date = false;           // YYYYMMDD integer shown as YYYY-MM-DD
     };
     struct TopKInfo {
         std::string bufferName;
@@ -146,6 +148,24 @@ struct HipResultSchema {
         bool sortByCountDesc = false;  // Q13-style (custdist desc, c_count desc)
     };
     CountsInfo counts;
+
+    // ROWS: records of `words` u64 fields in bufferName, their number in word
+    // 0 of ctlBuffer; ordered by (field index, descending) keys, then cut to
+    // limit rows (0 = all).
+    struct RowsField {
+        std::string name;
+        int scale = 0;
+        bool date = false;
+    };
+    struct RowsInfo {
+        std::string bufferName;
+        std::string ctlBuffer;
+        std::string capExpr;
+        std::vector<RowsField> fields;
+        std::vector<std::pair<int, bool>> orderBy;
+        int limit = 0;
+    };
+    RowsInfo rows;
 };
 
 }  // namespace codegen

@@ -5,6 +5,8 @@
 
 #include "execution/hip_generic_executor.h"
 #include "execution/runtime_compiler.h"
+#include "infra/canonical_compare.h"
+#include "infra/colbin.h"
 #include "infra/generators.h"
 #include "planning/hip_plan_builder.h"
 #include "runtime/autotuner.h"
@@ -18,6 +20,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 
 using namespace gdb;
 using namespace gdb::codegen;
@@ -500,6 +503,15 @@ gdb_status gdb_result_column(gdb_result r, int idx, char* name, size_t cap, int*
     return GDB_OK;
 }
 
+gdb_status gdb_result_column_format(gdb_result r, int idx, int* scale, int* is_date) {
+    if (idx < 0 || idx >= static_cast<int>(r->r.columns.size()))
+        return setErr(GDB_ERR_INVALID, "gdb_result_column_format: index out of range");
+    const auto& c = r->r.columns[static_cast<size_t>(idx)];
+    if (scale) *scale = c.scale;
+    if (is_date) *is_date = c.date ? 1 : 0;
+    return GDB_OK;
+}
+
 gdb_status gdb_result_i64(gdb_result r, int idx, int64_t* out) {
     const auto& c = r->r.columns.at(static_cast<size_t>(idx));
     if (c.type != ResultColumn::I64) return setErr(GDB_ERR_INVALID, "gdb_result_i64: column is not int64");
@@ -757,3 +769,81 @@ gdb_status gdb_est_collect_runs(gdb_ctx ctx, int nruns, uint64_t seed, uint64_t 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// TPC-H ingest and the golden check
+// ---------------------------------------------------------------------------
+
+struct gdb_colbin_s {
+    std::unique_ptr<infra::ColbinFile> f;
+};
+
+gdb_status gdb_tpch_column(const char* table, const char* column, int* index, int* dtype, int* width) {
+    return guard([&]() -> gdb_status {
+        if (!table || !column) return setErr(GDB_ERR_INVALID, "gdb_tpch_column: null name");
+        const auto& c = infra::tpchTable(table).column(column);
+        if (index) *index = c.index;
+        if (dtype) *dtype = static_cast<int>(c.type);
+        if (width) *width = c.width;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_tpch_table_columns(const char* table, char* buf, size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status {
+        if (!table) return setErr(GDB_ERR_INVALID, "gdb_tpch_table_columns: null table");
+        std::string s;
+        for (const auto& c : infra::tpchTable(table).columns) s += (s.empty() ? "" : ",") + c.name;
+        return copyOut(s, buf, cap, need);
+    });
+}
+
+gdb_status gdb_tbl_to_colbin(const char* tbl_path, const char* table, int force, int threads, int* built) {
+    return guard([&]() -> gdb_status {
+        if (!tbl_path || !table) return setErr(GDB_ERR_INVALID, "gdb_tbl_to_colbin: null argument");
+        const int t = threads > 0 ? threads : static_cast<int>(std::max(1u, std::thread::hardware_concurrency()));
+        const bool b = infra::tblToColbin(tbl_path, table, force != 0, t);
+        if (built) *built = b ? 1 : 0;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_colbin_open(const char* colbin_path, const char* tbl_path, gdb_colbin* out) {
+    return guard([&]() -> gdb_status {
+        if (!colbin_path || !out) return setErr(GDB_ERR_INVALID, "gdb_colbin_open: null argument");
+        auto h = std::make_unique<gdb_colbin_s>();
+        h->f = std::make_unique<infra::ColbinFile>(colbin_path, tbl_path ? tbl_path : "");
+        *out = h.release();
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_colbin_shape(gdb_colbin f, uint64_t* rows, int* ncols) {
+    if (!f) return setErr(GDB_ERR_INVALID, "gdb_colbin_shape: null handle");
+    if (rows) *rows = f->f->rows();
+    if (ncols) *ncols = static_cast<int>(f->f->columns().size());
+    return GDB_OK;
+}
+
+gdb_status gdb_colbin_column(gdb_colbin f, int column_index, const void** data, uint64_t* bytes, int* dtype,
+                             int* width) {
+    return guard([&]() -> gdb_status {
+        if (!f) return setErr(GDB_ERR_INVALID, "gdb_colbin_column: null handle");
+        const auto& c = f->f->column(column_index);
+        if (data) *data = c.data;
+        if (bytes) *bytes = c.bytes;
+        if (dtype) *dtype = static_cast<int>(c.type);
+        if (width) *width = c.width;
+        return GDB_OK;
+    });
+}
+
+void gdb_colbin_close(gdb_colbin f) { delete f; }
+
+gdb_status gdb_compare_canonical(const char* got, const char* expected, double abs_tol, double rel_tol, char* msg,
+                                 size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status {
+        if (!got || !expected) return setErr(GDB_ERR_INVALID, "gdb_compare_canonical: null text");
+        return copyOut(infra::compareCanonical(got, expected, abs_tol, rel_tol), msg, cap, need);
+    });
+}

@@ -1,6 +1,7 @@
 #include "hip_result_collector.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <numeric>
 #include <sstream>
 #include <stdexcept>
@@ -22,6 +23,38 @@ std::string int128ToString(__int128 v) {
     return s;
 }
 
+std::string fixedToString(__int128 v, int scale) {
+    if (scale <= 0) return int128ToString(v);
+    const bool neg = v < 0;
+    std::string d = int128ToString(neg ? -v : v);  // |v| of any sum here is far below 2^127
+    if (d.size() <= static_cast<size_t>(scale)) d.insert(0, static_cast<size_t>(scale) + 1 - d.size(), '0');
+    d.insert(d.size() - static_cast<size_t>(scale), 1, '.');
+    return neg ? "-" + d : d;
+}
+
+namespace {
+
+std::string dateString(int64_t ymd) {
+    char buf[32];
+    std::snprintf(buf, sizeof(buf), "%04lld-%02lld-%02lld", static_cast<long long>(ymd / 10000),
+                  static_cast<long long>(ymd / 100 % 100), static_cast<long long>(ymd % 100));
+    return buf;
+}
+
+// Quoted when it holds a comma, a quote or a line break; quotes doubled
+// (metal_result_collector.cpp:102-115).
+std::string csvField(const std::string& s) {
+    if (s.find_first_of(",\"\n\r") == std::string::npos) return s;
+    std::string q = "\"";
+    for (char ch : s) {
+        if (ch == '"') q += '"';
+        q += ch;
+    }
+    return q + "\"";
+}
+
+}  // namespace
+
 const ResultColumn& GenericResult::column(const std::string& name) const {
     for (const auto& c : columns)
         if (c.name == name) return c;
@@ -37,12 +70,12 @@ std::string GenericResult::toCanonical() const {
         for (size_t c = 0; c < columns.size(); ++c) {
             if (c) os << ",";
             const auto& col = columns[c];
-            if (col.type == ResultColumn::I64)
-                os << col.i64[r];
-            else if (col.type == ResultColumn::I128)
-                os << int128ToString(col.i128[r]);
+            if (col.type == ResultColumn::STR)
+                os << csvField(col.str[r]);
+            else if (col.date)
+                os << dateString(col.type == ResultColumn::I64 ? col.i64[r] : static_cast<int64_t>(col.i128[r]));
             else
-                os << col.str[r];
+                os << fixedToString(col.type == ResultColumn::I64 ? col.i64[r] : col.i128[r], col.scale);
         }
         os << "\n";
     }
@@ -88,6 +121,7 @@ GenericResult HipResultCollector::collect(const HipResultSchema& schema, const B
         case HipResultSchema::TOPK: return collectTopK(schema, read);
         case HipResultSchema::HISTOGRAM:
         case HipResultSchema::COUNTS: return collectCounts(schema, read, sym);
+        case HipResultSchema::ROWS: return collectRows(schema, read, sym);
         default: return GenericResult{};
     }
 }
@@ -108,9 +142,10 @@ GenericResult HipResultCollector::collectKeyedAgg(const HipResultSchema& schema,
     } else {
         res.columns.push_back(mk(k.keyColumn, ResultColumn::I64));
     }
-    for (const auto& s : k.slots)
+    for (const auto& s : k.slots) {
         res.columns.push_back(mk(s.name, s.agg == AggKind::SumI128 ? ResultColumn::I128 : ResultColumn::I64));
-    for (const auto& a : k.averages) res.columns.push_back(mk(a.first, ResultColumn::I128));
+        res.columns.back().scale = s.scaleDigits;
+    }
 
     int countSlot = -1;
     for (size_t i = 0; i < k.slots.size(); ++i)
@@ -121,6 +156,12 @@ GenericResult HipResultCollector::collectKeyedAgg(const HipResultSchema& schema,
             if (k.slots[i].name == name) return static_cast<int>(i);
         throw std::runtime_error("collector: unknown slot " + name);
     };
+    // An average carries 6 more digits than numerator / denominator.
+    for (const auto& a : k.averages) {
+        res.columns.push_back(mk(a.first, ResultColumn::I128));
+        res.columns.back().scale = std::max(0, 6 + k.slots[static_cast<size_t>(slotIndex(a.second.first))].scaleDigits -
+                                                   k.slots[static_cast<size_t>(slotIndex(a.second.second))].scaleDigits);
+    }
 
     for (int b = 0; b < k.numBuckets; ++b) {
         const uint64_t* w = words.data() + static_cast<size_t>(b) * k.wordsPerBucket;
@@ -190,7 +231,11 @@ GenericResult HipResultCollector::collectTopK(const HipResultSchema& schema, con
     read(t.ctlBuffer, 0, ctl.data(), 32);
     read(t.bufferName, 0, out.data(), out.size() * 8);
     GenericResult res;
-    for (const auto& c : t.columns) res.columns.push_back(mk(c.name, ResultColumn::I64));
+    for (const auto& c : t.columns) {
+        res.columns.push_back(mk(c.name, ResultColumn::I64));
+        res.columns.back().scale = c.scale;
+        res.columns.back().date = c.date;
+    }
     const uint64_t valid = std::min<uint64_t>(ctl[3], static_cast<uint64_t>(t.k));
     for (uint64_t j = 0; j < valid; ++j) {
         for (size_t ci = 0; ci < t.columns.size(); ++ci) {
@@ -228,6 +273,42 @@ GenericResult HipResultCollector::collectCounts(const HipResultSchema& schema, c
         res.columns[0].i64.push_back(rw.first);
         res.columns[1].i64.push_back(rw.second);
     }
+    return res;
+}
+
+// Appended records, ordered by the schema's keys with every field ascending
+// as the final tie-break (so the order is total), then cut to the limit.
+GenericResult HipResultCollector::collectRows(const HipResultSchema& schema, const BufferReader& read,
+                                              const SymbolReader& sym) {
+    const auto& r = schema.rows;
+    const size_t W = r.fields.size();
+    std::vector<uint64_t> ctl(4);
+    read(r.ctlBuffer, 0, ctl.data(), 32);
+    const uint64_t n = std::min<uint64_t>(ctl[2], sym(r.capExpr));
+    std::vector<int64_t> rec(static_cast<size_t>(n) * W);
+    if (n) read(r.bufferName, 0, rec.data(), rec.size() * 8);
+    std::vector<size_t> idx(static_cast<size_t>(n));
+    std::iota(idx.begin(), idx.end(), size_t{0});
+    std::sort(idx.begin(), idx.end(), [&](size_t a, size_t b) {
+        const int64_t* x = rec.data() + a * W;
+        const int64_t* y = rec.data() + b * W;
+        for (const auto& o : r.orderBy) {
+            const int64_t u = x[o.first], v = y[o.first];
+            if (u != v) return o.second ? u > v : u < v;
+        }
+        for (size_t f = 0; f < W; ++f)
+            if (x[f] != y[f]) return x[f] < y[f];
+        return false;
+    });
+    if (r.limit > 0 && idx.size() > static_cast<size_t>(r.limit)) idx.resize(static_cast<size_t>(r.limit));
+    GenericResult res;
+    for (const auto& f : r.fields) {
+        res.columns.push_back(mk(f.name, ResultColumn::I64));
+        res.columns.back().scale = f.scale;
+        res.columns.back().date = f.date;
+    }
+    for (size_t i : idx)
+        for (size_t f = 0; f < W; ++f) res.columns[f].i64.push_back(rec[i * W + f]);
     return res;
 }
 

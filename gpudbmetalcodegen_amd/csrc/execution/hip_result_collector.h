@@ -24,6 +24,8 @@ struct ResultColumn {
     std::vector<int64_t> i64;
     std::vector<__int128> i128;
     std::vector<std::string> str;
+    int scale = 0;      // integer columns: fixed-point digits, shown as an exact decimal
+    bool date = false;  // integer columns: YYYYMMDD shown as YYYY-MM-DD
     size_t size() const { return type == I64 ? i64.size() : (type == I128 ? i128.size() : str.size()); }
 };
 
@@ -35,6 +37,8 @@ struct GenericResult {
 };
 
 std::string int128ToString(__int128 v);
+// Exact decimal of v / 10^scale ("-0.05" for v = -5, scale = 2).
+std::string fixedToString(__int128 v, int scale);
 
 // Reads a runner buffer: (name, offset bytes, dst, bytes). Synchronous.
 using BufferReader = std::function<void(const std::string&, size_t, void*, size_t)>;
@@ -51,6 +55,8 @@ private:
     static GenericResult collectTopK(const HipResultSchema& schema, const BufferReader& read);
     static GenericResult collectCounts(const HipResultSchema& schema, const BufferReader& read,
                                        const SymbolReader& sym);
+    static GenericResult collectRows(const HipResultSchema& schema, const BufferReader& read,
+                                     const SymbolReader& sym);
 };
 
 }  // namespace codegen

@@ -592,16 +592,19 @@ std::string HipSlotAtomicAdd::describe() const {
 
 HipArrayLookup::HipArrayLookup(std::unique_ptr<HipOperator> child, const std::string& arrayName,
                                const std::string& keyExpr, const std::string& sizeExpr, const std::string& resultVar,
-                               long long sentinel)
+                               long long sentinel, const std::string& elemType)
     : HipUnaryOperator(std::move(child)),
       array_(arrayName),
       key_(keyExpr),
       size_(sizeExpr),
       var_(resultVar),
-      sentinel_(sentinel) {}
+      elem_(elemType),
+      sentinel_(sentinel) {
+    if (elem_ != "u32" && elem_ != "i64") throw std::runtime_error("HipArrayLookup: element type must be u32 or i64");
+}
 
 void HipArrayLookup::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    cg.addBufferParam(array_, "u32", 4, size_, InitKind::None, {}, true, true);
+    cg.addBufferParam(array_, elem_, elem_ == "u32" ? 4 : 8, size_, InitKind::None, {}, true, true);
     cg.addSymbolParam("nelem_" + array_, size_);
     child_->produce(cg, [&]() {
         cg.addLine("const u64 _ak_" + var_ + " = (u64)(i64)(" + key_ + ");");
@@ -612,6 +615,121 @@ void HipArrayLookup::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 }
 
 std::string HipArrayLookup::describe() const { return "ArrayLookup(" + var_ + " = " + array_ + "[" + key_ + "])"; }
+
+// ===========================================================================
+// HipBitmapFlag, HipBufferScalars, HipDenseAtomicAdd, HipAppendRows
+// ===========================================================================
+
+HipBitmapFlag::HipBitmapFlag(std::unique_ptr<HipOperator> child, const std::string& bitmapName,
+                             const std::string& keyExpr, const std::string& bitsExpr, const std::string& var)
+    : HipUnaryOperator(std::move(child)), bitmap_(bitmapName), key_(keyExpr), bits_(bitsExpr), var_(var) {}
+
+void HipBitmapFlag::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBitmapReadParam(bitmap_, bits_);
+    child_->produce(cg, [&]() {
+        cg.addLine("const u32 " + var_ + " = gdb_bitmap_test(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" + key_ +
+                   ")) ? 1u : 0u;");
+        consume();
+    });
+}
+
+std::string HipBitmapFlag::describe() const {
+    return "BitmapFlag(" + var_ + " = " + key_ + " in " + bitmap_ + ")";
+}
+
+HipBufferScalars::HipBufferScalars(std::unique_ptr<HipOperator> child, const std::string& buffer,
+                                   const std::string& sizeExpr, std::vector<std::pair<std::string, int>> vars)
+    : HipUnaryOperator(std::move(child)), buffer_(buffer), size_(sizeExpr), vars_(std::move(vars)) {}
+
+void HipBufferScalars::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(buffer_, "u64", 8, size_, InitKind::None, {}, true, true);
+    for (const auto& v : vars_) cg.addLine("const i64 " + v.first + " = (i64)" + buffer_ + "[" + s(v.second) + "];");
+    child_->produce(cg, consume);
+}
+
+std::string HipBufferScalars::describe() const {
+    std::string d = "BufferScalars(" + buffer_ + ":";
+    for (const auto& v : vars_) d += " " + v.first + "=[" + s(v.second) + "]";
+    return d + ")";
+}
+
+HipDenseAtomicAdd::HipDenseAtomicAdd(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                                     const std::string& indexExpr, const std::string& sizeExpr,
+                                     const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)), array_(arrayName), index_(indexExpr), size_(sizeExpr), value_(valueExpr) {}
+
+void HipDenseAtomicAdd::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(array_, "u64", 8, size_, InitKind::Zero);
+    cg.addSymbolParam("nelem_" + array_, size_);
+    child_->produce(cg, [&]() {
+        cg.addBlock("", [&]() {
+            cg.addLine("const u64 _di = (u64)(i64)(" + index_ + ");");
+            cg.addLine("if (_di < nelem_" + array_ + ") gdb_atomic_add_u64(" + array_ + " + _di, (u64)(i64)(" + value_ +
+                       "));");
+        });
+        consume();
+    });
+}
+
+std::string HipDenseAtomicAdd::describe() const {
+    return "DenseAtomicAdd(" + array_ + "[" + index_ + "] += " + value_ + ")";
+}
+
+HipAppendRows::HipAppendRows(std::unique_ptr<HipOperator> child, const std::string& out, const std::string& capExpr)
+    : HipUnaryOperator(std::move(child)), out_(out), cap_(capExpr) {}
+
+void HipAppendRows::addField(const std::string& name, const std::string& expr, int scale, bool date) {
+    fields_.push_back({name, expr, scale, date});
+}
+
+void HipAppendRows::orderBy(const std::string& field, bool desc) { order_.push_back({field, desc}); }
+
+void HipAppendRows::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    if (fields_.empty()) throw std::runtime_error("HipAppendRows(" + out_ + "): no fields");
+    const std::string W = s(fields_.size());
+    const std::string st = out_ + "_status";  // word 0 overflow flag, word 2 records appended
+    cg.addBufferParam(out_, "u64", 8, "(" + cap_ + ") * " + W, InitKind::None);
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    cg.addSymbolParam("cap_" + out_, cap_);
+    child_->produce(cg, [&]() {
+        cg.addBlock("", [&]() {
+            cg.addLine("const u64 _ap = gdb_wave_append(" + st + " + 2);");
+            cg.addBlock("if (_ap < cap_" + out_ + ")", [&]() {
+                for (size_t f = 0; f < fields_.size(); ++f)
+                    cg.addLine(out_ + "[_ap * " + W + " + " + s(f) + "] = (u64)(i64)(" + fields_[f].expr + ");");
+            }, " else {");
+            {
+                IndentGuard g(cg);
+                cg.addLine("gdb_atomic_max_u64(" + st + ", 1ull);");
+            }
+            cg.addLine("}");
+        });
+        consume();
+    });
+    auto& rs = cg.resultSchema();
+    rs.kind = HipResultSchema::ROWS;
+    rs.rows.bufferName = out_;
+    rs.rows.ctlBuffer = st;
+    rs.rows.capExpr = cap_;
+    rs.rows.fields.clear();
+    for (const auto& f : fields_) rs.rows.fields.push_back({f.name, f.scale, f.date});
+    rs.rows.orderBy.clear();
+    for (const auto& o : order_) {
+        int idx = -1;
+        for (size_t f = 0; f < fields_.size(); ++f)
+            if (fields_[f].name == o.first) idx = static_cast<int>(f);
+        if (idx < 0) throw std::runtime_error("HipAppendRows(" + out_ + "): order by unknown field " + o.first);
+        rs.rows.orderBy.push_back({idx, o.second});
+    }
+    rs.rows.limit = limit_;
+}
+
+std::string HipAppendRows::describe() const {
+    std::string d = "AppendRows(" + out_ + ":";
+    for (const auto& f : fields_) d += " " + f.name;
+    return d + ")";
+}
 
 // ===========================================================================
 // HipKeyedAgg
@@ -1340,8 +1458,9 @@ HipTopK::HipTopK(std::unique_ptr<HipOperator> child, const std::string& outputNa
     if (k_ < 1 || k_ > 32) throw std::runtime_error("HipTopK: k must be in [1, 32]");
 }
 
-void HipTopK::addColumn(const std::string& name, int word, int shift, int bits, long long complementOf) {
-    cols_.push_back({name, word, shift, bits, complementOf});
+void HipTopK::addColumn(const std::string& name, int word, int shift, int bits, long long complementOf, int scale,
+                        bool date) {
+    cols_.push_back({name, word, shift, bits, complementOf, scale, date});
 }
 
 void HipTopK::produceImpl(HipCodegen& cg, ConsumerFn consume) {
@@ -1408,7 +1527,8 @@ void HipTopKMerge::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     rs.topk.ctlBuffer = ctl;
     rs.topk.k = k_;
     rs.topk.columns.clear();
-    for (const auto& c : cols_) rs.topk.columns.push_back({c.name, c.word, c.shift, c.bits, c.complementOf});
+    for (const auto& c : cols_)
+        rs.topk.columns.push_back({c.name, c.word, c.shift, c.bits, c.complementOf, c.scale, c.date});
 }
 
 std::string HipTopKMerge::describe() const { return "TopKMerge(" + out_ + ", k=" + s(k_) + ")"; }

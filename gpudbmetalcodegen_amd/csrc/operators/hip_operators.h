@@ -207,16 +207,89 @@ private:
 };
 
 // Direct-address lookup with a sentinel guard (metal_operators.cpp:203-225).
+// elemType: "u32" (counters) or "i64" (sums).
 class HipArrayLookup : public HipUnaryOperator {
 public:
     HipArrayLookup(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& keyExpr,
-                   const std::string& sizeExpr, const std::string& resultVar, long long sentinel);
+                   const std::string& sizeExpr, const std::string& resultVar, long long sentinel,
+                   const std::string& elemType = "u32");
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
-    std::string array_, key_, size_, var_;
+    std::string array_, key_, size_, var_, elem_;
     long long sentinel_;
+};
+
+// Semi-join membership as a value: const u32 <var> = bit `key` is set;
+// every row is consumed (CASE WHEN <key in build side> THEN ... expressions,
+// e.g. TPC-H Q12's priority class and Q14's promotional parts).
+class HipBitmapFlag : public HipUnaryOperator {
+public:
+    HipBitmapFlag(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
+                  const std::string& bitsExpr, const std::string& var);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string bitmap_, key_, bits_, var_;
+};
+
+// Values an earlier phase left in a buffer, read once at kernel start:
+// const i64 <name> = (i64)buffer[word]. Carries a scalar subquery's result
+// (TPC-H Q22's average balance as sum and count) to a later phase without a
+// host round trip.
+class HipBufferScalars : public HipUnaryOperator {
+public:
+    HipBufferScalars(std::unique_ptr<HipOperator> child, const std::string& buffer, const std::string& sizeExpr,
+                     std::vector<std::pair<std::string, int>> vars);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string buffer_, size_;
+    std::vector<std::pair<std::string, int>> vars_;
+};
+
+// Direct-address i64 accumulation array[index] += value with global atomics
+// (the reference's per-key atomic aggregation into a key-indexed array,
+// MetalAtomicAgg, metal_operators.cpp:525-533); rows whose index is outside
+// [0, size) are skipped. The array is zeroed before its phase.
+class HipDenseAtomicAdd : public HipUnaryOperator {
+public:
+    HipDenseAtomicAdd(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& indexExpr,
+                      const std::string& sizeExpr, const std::string& valueExpr);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string array_, index_, size_, value_;
+};
+
+// Append one record of u64 fields per consumed row to `out` (capacity
+// capExpr records; a wave-aggregated cursor in <out>_ctl word 0). The
+// records are the plan's result (ROWS): ordered on the host by the given
+// fields, then cut to `limit` rows (0 = all). Rows beyond the capacity set the
+// phase's overflow status.
+class HipAppendRows : public HipUnaryOperator {
+public:
+    HipAppendRows(std::unique_ptr<HipOperator> child, const std::string& out, const std::string& capExpr);
+    void addField(const std::string& name, const std::string& expr, int scale = 0, bool date = false);
+    void orderBy(const std::string& field, bool desc);
+    void setLimit(int limit) { limit_ = limit; }
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    struct Field {
+        std::string name, expr;
+        int scale;
+        bool date;
+    };
+    std::string out_, cap_;
+    std::vector<Field> fields_;
+    std::vector<std::pair<std::string, bool>> order_;
+    int limit_ = 0;
 };
 
 // ---------------------------------------------------------------------------
@@ -415,10 +488,13 @@ public:
         int shift;
         int bits;
         long long complementOf;  // value = complementOf - field when != 0
+        int scale;               // fixed-point digits of the shown value
+        bool date;               // YYYYMMDD shown as a date
     };
     HipTopK(std::unique_ptr<HipOperator> child, const std::string& outputName, int k, const std::string& rankHiExpr,
             const std::string& rankLoExpr);
-    void addColumn(const std::string& name, int word, int shift, int bits, long long complementOf = 0);
+    void addColumn(const std::string& name, int word, int shift, int bits, long long complementOf = 0, int scale = 0,
+                   bool date = false);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
     // The operator of the final phase: one block merges every block's

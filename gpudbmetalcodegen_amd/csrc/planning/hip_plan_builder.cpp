@@ -1,4 +1,6 @@
 #include "hip_plan_builder.h"
+#include "plan_util.h"
+#include "tpch_plans.h"
 
 #include <sstream>
 #include <stdexcept>
@@ -18,6 +20,7 @@ ColumnType I32{ColType::I32, 0};
 ColumnType I64{ColType::I64, 0};
 ColumnType U8{ColType::U8, 0};
 ColumnType CHAR25{ColType::CHAR, 25};
+ColumnType chr(int width) { return ColumnType{ColType::CHAR, width}; }
 
 const std::vector<CatalogTable>& catalog() {
     static const std::vector<CatalogTable> tables = {
@@ -36,40 +39,35 @@ const std::vector<CatalogTable>& catalog() {
         {"G", {{"g_key", I32}, {"g_val", I64}}},
         {"GBBM", {{"gbbm_word", I32}}},
         {"P", {{"p_key", I32}, {"p_date", I32}, {"p_val", I64}, {"p_flag", U8}}},
+        // TPC-H tables as loaded from .colbin (tpch_schema.h:66-162): INT and
+        // DATE (YYYYMMDD) int32, FLOAT as exact int64 hundredths, CHAR1 one
+        // byte, CHAR_FIXED char[width].
+        {"lineitem",
+         {{"l_orderkey", I32}, {"l_partkey", I32}, {"l_suppkey", I32}, {"l_linenumber", I32}, {"l_quantity", I64},
+          {"l_extendedprice", I64}, {"l_discount", I64}, {"l_tax", I64}, {"l_returnflag", U8}, {"l_linestatus", U8},
+          {"l_shipdate", I32}, {"l_commitdate", I32}, {"l_receiptdate", I32}, {"l_shipinstruct", chr(25)},
+          {"l_shipmode", chr(2)}, {"l_comment", chr(44)}}},
+        {"orders",
+         {{"o_orderkey", I32}, {"o_custkey", I32}, {"o_orderstatus", U8}, {"o_totalprice", I64}, {"o_orderdate", I32},
+          {"o_orderpriority", U8}, {"o_clerk", chr(15)}, {"o_shippriority", I32}, {"o_comment", chr(79)}}},
+        {"customer",
+         {{"c_custkey", I32}, {"c_name", chr(25)}, {"c_address", chr(40)}, {"c_nationkey", I32}, {"c_phone", chr(15)},
+          {"c_acctbal", I64}, {"c_mktsegment", U8}, {"c_comment", chr(117)}}},
+        {"part",
+         {{"p_partkey", I32}, {"p_name", chr(55)}, {"p_mfgr", chr(25)}, {"p_brand", chr(10)}, {"p_type", chr(25)},
+          {"p_size", I32}, {"p_container", chr(10)}, {"p_retailprice", I64}, {"p_comment", chr(23)}}},
+        {"supplier",
+         {{"s_suppkey", I32}, {"s_name", chr(25)}, {"s_address", chr(40)}, {"s_nationkey", I32}, {"s_phone", chr(15)},
+          {"s_acctbal", I64}, {"s_comment", chr(101)}}},
+        {"partsupp",
+         {{"ps_partkey", I32}, {"ps_suppkey", I32}, {"ps_availqty", I32}, {"ps_supplycost", I64},
+          {"ps_comment", chr(199)}}},
+        {"nation", {{"n_nationkey", I32}, {"n_name", chr(25)}, {"n_regionkey", I32}, {"n_comment", chr(152)}}},
+        {"region", {{"r_regionkey", I32}, {"r_name", chr(25)}, {"r_comment", chr(152)}}},
     };
     return tables;
 }
 
-std::string opt(const PlanOptions& o, const std::string& k, const std::string& def) {
-    auto it = o.find(k);
-    return it == o.end() ? def : it->second;
-}
-int optInt(const PlanOptions& o, const std::string& k, int def) {
-    auto it = o.find(k);
-    return it == o.end() ? def : std::stoi(it->second);
-}
-
-std::unique_ptr<HipGridStrideScan> makeScan(const std::string& table, std::initializer_list<const char*> cols) {
-    auto scan = std::make_unique<HipGridStrideScan>(table, "i");
-    const auto& t = catalogTable(table);
-    for (const char* c : cols) scan->addColumn(c, t.type(c));
-    return scan;
-}
-
-HipQueryPlan::Phase& appendPhase(HipQueryPlan& plan, const std::string& name, std::unique_ptr<HipOperator> root,
-                                 int block, int items, std::vector<int> variants = {}) {
-    HipQueryPlan::Phase p;
-    p.name = name;
-    p.root = std::move(root);
-    p.threadgroupSize = block;
-    p.items = items;
-    p.itemsVariants = variants.empty() ? std::vector<int>{items} : variants;
-    bool has = false;
-    for (int v : p.itemsVariants) has |= (v == items);
-    if (!has) p.itemsVariants.push_back(items);
-    plan.phases.push_back(std::move(p));
-    return plan.phases.back();
-}
 
 // ---------------------------------------------------------------------------
 // Q1 — filter + group-by (flag, status) with exact fixed-point aggregates.
@@ -502,11 +500,14 @@ PlanOptions parsePlanOptions(const std::string& text) {
 }
 
 std::vector<std::string> knownPlanNames() {
-    return {"Q1", "Q6", "Q3", "Q3_SIZE", "GROUPBY", "GB_COUNT", "MAXKEY", "Q13L", "Q22A", "PIPE_AGG", "PIPE_SEMI",
-            "PIPE_ANTI"};
+    std::vector<std::string> v = {"Q1",      "Q6",       "Q3",   "Q3_SIZE",  "GROUPBY",   "GB_COUNT", "MAXKEY",
+                                  "Q13L",    "Q22A",     "PIPE_AGG", "PIPE_SEMI", "PIPE_ANTI"};
+    for (const auto& n : tpchPlanNames()) v.push_back(n);
+    return v;
 }
 
 std::optional<HipQueryPlan> buildHipPlan(const std::string& name, const PlanOptions& opts) {
+    if (auto tp = buildTpchPlan(name, opts)) return tp;
     if (name == "Q1") return buildQ1(opts);
     if (name == "Q6") return buildQ6(opts);
     if (name == "Q3") return buildQ3(opts);

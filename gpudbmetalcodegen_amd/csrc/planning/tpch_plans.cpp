@@ -1,0 +1,377 @@
+#include "tpch_plans.h"
+
+#include "plan_util.h"
+
+#include <stdexcept>
+
+namespace gdb {
+namespace codegen {
+
+namespace {
+
+// Money and rates are exact int64 hundredths (cents; 0.06 -> 6), so
+// price * (100 - discount) carries 4 decimal digits and the charge 6.
+constexpr const char* kRevenue = "l_extendedprice * (100 - l_discount)";
+
+// ---------------------------------------------------------------------------
+// Q1 — pricing summary (sql/q1.sql; the bucket encoding and slot list of
+// metal_plan_builder.cpp:335-349, in exact integers).
+// ---------------------------------------------------------------------------
+HipQueryPlan tpchQ1(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q1";
+    auto scan = makeScan("lineitem", {"l_quantity", "l_extendedprice", "l_discount", "l_tax", "l_returnflag",
+                                      "l_linestatus", "l_shipdate"});
+    auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)l_shipdate <= q1_cutoff");
+    const std::string bucket =
+        "((l_returnflag == 'A' ? 0 : (l_returnflag == 'N' ? 2 : 4)) + (l_linestatus == 'F' ? 0 : 1))";
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q1_aggs", bucket, 6);
+    agg->addAggregate("sum_qty", AggKind::SumI64, "l_quantity", 2);
+    agg->addAggregate("sum_base_price", AggKind::SumI64, "l_extendedprice", 2);
+    agg->addAggregate("sum_disc_price", AggKind::SumI64, kRevenue, 4);
+    agg->addAggregate("sum_charge", AggKind::SumI128, std::string(kRevenue) + " * (100 + l_tax)", 6);
+    agg->addAggregate("sum_disc", AggKind::SumI64, "l_discount", 2);
+    agg->addAggregate("count_order", AggKind::Count, "1", 0);
+    agg->setBucketLabels({"A|F", "A|O", "N|F", "N|O", "R|F", "R|O"}, "l_returnflag,l_linestatus");
+    agg->addAverage("avg_qty", "sum_qty", "count_order");
+    agg->addAverage("avg_price", "sum_base_price", "count_order");
+    agg->addAverage("avg_disc", "sum_disc", "count_order");
+    auto& ph = appendPhase(plan, "Q1_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 2),
+                           {2, 4});
+    ph.launchBound = 256;
+    ph.scalarParams = {"q1_cutoff"};
+    plan.requiredScalars = {"q1_cutoff"};
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q6 — forecasting revenue change (sql/q6.sql; metal_plan_builder.cpp:253-303).
+HipQueryPlan tpchQ6(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q6";
+    auto scan = makeScan("lineitem", {"l_quantity", "l_extendedprice", "l_discount", "l_shipdate"});
+    auto sel = std::make_unique<HipSelection>(
+        std::move(scan),
+        "(i64)l_shipdate >= q6_date_lo && (i64)l_shipdate < q6_date_hi && l_discount >= q6_disc_lo && "
+        "l_discount <= q6_disc_hi && l_quantity < q6_qty_hi");
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q6_sum", "0", 1);
+    agg->addAggregate("revenue", AggKind::SumI64, "l_extendedprice * l_discount", 4);
+    agg->addAggregate("count", AggKind::Count, "1", 0);
+    auto& ph = appendPhase(plan, "Q6_reduce", std::move(agg), optInt(o, "block", 256), optInt(o, "items", 2), {2, 4});
+    ph.launchBound = 256;
+    ph.scalarParams = {"q6_date_lo", "q6_date_hi", "q6_disc_lo", "q6_disc_hi", "q6_qty_hi"};
+    plan.requiredScalars = ph.scalarParams;
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q3 — shipping priority (sql/q3.sql; buildQ3Plan_byName,
+// metal_plan_builder.cpp:1170-1239, with a device hash join): customer
+// segment bitmap, hash table of qualifying orders keyed by o_orderkey with
+// o_orderdate as payload, lineitem probe summing revenue in the slot, top-10
+// by (revenue desc, o_orderdate asc, l_orderkey asc). o_shippriority is
+// attached by the caller for the 10 result keys.
+HipQueryPlan tpchQ3(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q3";
+    {
+        auto scan = makeScan("customer", {"c_custkey", "c_mktsegment"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)c_mktsegment == q3_seg");
+        auto bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q3_c_bitmap", "c_custkey", "c_bits");
+        auto& ph = appendPhase(plan, "Q3_build_c_bitmap", std::move(bm), 256, 4);
+        ph.scalarParams = {"q3_seg"};
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_custkey", "o_orderdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)o_orderdate < q3_date");
+        auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_c_bitmap", "o_custkey", "c_bits");
+        auto ht = std::make_unique<HipHashBuild>(std::move(probe), "d_q3_o_ht", "o_orderkey", "o_orderdate", "o_slots", 2);
+        auto bm = std::make_unique<HipBitmapBuild>(std::move(ht), "d_q3_o_bitmap", "o_orderkey", "o_bits");
+        auto& ph = appendPhase(plan, "Q3_build_o_table", std::move(bm), 256, 4);
+        ph.scalarParams = {"q3_date"};
+        ph.staged = optInt(o, "staged", 1) != 0;
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_extendedprice", "l_discount", "l_shipdate"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipSelection>(std::move(scan), "(i64)l_shipdate > q3_date");
+        cur = std::make_unique<HipBitmapProbe>(std::move(cur), "d_q3_o_bitmap", "l_orderkey", "o_bits");
+        cur = std::make_unique<HipHashProbe>(std::move(cur), "d_q3_o_ht", "l_orderkey", "o_slots", 2, "_os");
+        cur = std::make_unique<HipSlotAtomicAdd>(std::move(cur), "d_q3_o_ht", 2, "_os", 1, kRevenue);
+        auto& ph = appendPhase(plan, "Q3_probe_l", std::move(cur), optInt(o, "block", 256), 2, {2});
+        ph.scalarParams = {"q3_date"};
+    }
+    {
+        auto scan = std::make_unique<HipHashTableScan>("d_q3_o_ht", "o_slots", 2, "ht");
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)ht_w1 > 0");
+        auto top = std::make_unique<HipTopK>(std::move(sel), "d_q3_top", 10, "ht_w1",
+                                             "((u64)(0x7FFFFFFFu - ht_payload) << 32) | (u64)(0xFFFFFFFFu - ht_key)");
+        top->addColumn("l_orderkey", 1, 0, 32, 0xFFFFFFFFll);
+        top->addColumn("revenue", 0, 0, 64, 0, 4);
+        top->addColumn("o_orderdate", 1, 32, 32, 0x7FFFFFFFll, 0, true);
+        auto merge = top->makeMerge();
+        auto& ph = appendPhase(plan, "Q3_topk", std::move(top), 256, 4);
+        ph.launchBound = 256;
+        auto& pm = appendPhase(plan, "Q3_topk_merge", std::move(merge), 256, 1, {1});
+        pm.launchBound = 256;
+    }
+    plan.requiredScalars = {"q3_seg", "q3_date"};
+    plan.requiredSymbols = {"c_bits", "o_bits", "o_slots"};
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q4 — order priority checking (sql/q4.sql): bitmap of order keys with a
+// late lineitem (commit < receipt), then orders of the quarter probing it,
+// counted per priority ('1'..'5').
+HipQueryPlan tpchQ4(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q4";
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_commitdate", "l_receiptdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "l_commitdate < l_receiptdate");
+        auto bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q4_late", "l_orderkey", "o_bits");
+        appendPhase(plan, "Q4_late_orders", std::move(bm), 256, optInt(o, "items", 4));
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_orderdate", "o_orderpriority"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)o_orderdate >= q4_date_lo && (i64)o_orderdate < q4_date_hi");
+        auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q4_late", "o_orderkey", "o_bits");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(probe), "d_q4_counts", "(int)o_orderpriority - '1'", 5);
+        agg->addAggregate("order_count", AggKind::Count, "1", 0);
+        agg->setBucketLabels({"1-URGENT", "2-HIGH", "3-MEDIUM", "4-NOT SPECIFIED", "5-LOW"}, "o_orderpriority");
+        auto& ph = appendPhase(plan, "Q4_count", std::move(agg), 256, optInt(o, "items", 4));
+        ph.scalarParams = {"q4_date_lo", "q4_date_hi"};
+    }
+    plan.requiredScalars = {"q4_date_lo", "q4_date_hi"};
+    plan.requiredSymbols = {"o_bits"};
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q12 — shipping modes and order priority (sql/q12.sql): bitmaps of all
+// order keys and of high-priority ('1-URGENT', '2-HIGH') order keys, then the
+// qualifying lineitems (ship mode code in {q12_mode1, q12_mode2}, the three
+// date conditions) joined to orders and counted per mode and priority class.
+// A mode code is the first two bytes of l_shipmode (char[2] in the catalog).
+HipQueryPlan tpchQ12(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q12";
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_orderpriority"});
+        auto any = std::make_unique<HipBitmapBuild>(std::move(scan), "d_q12_orders", "o_orderkey", "o_bits");
+        auto sel = std::make_unique<HipSelection>(std::move(any), "o_orderpriority == '1' || o_orderpriority == '2'");
+        auto high = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q12_high", "o_orderkey", "o_bits");
+        appendPhase(plan, "Q12_orders", std::move(high), 256, 4);
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_shipmode", "l_shipdate", "l_commitdate", "l_receiptdate"});
+        auto mc = std::make_unique<HipComputeExpr>(std::move(scan), "_mc", "i64",
+                                                   "(i64)(u8)l_shipmode[0] | ((i64)(u8)l_shipmode[1] << 8)");
+        auto sel = std::make_unique<HipSelection>(
+            std::move(mc),
+            "(_mc == q12_mode1 || _mc == q12_mode2) && l_commitdate < l_receiptdate && l_shipdate < l_commitdate && "
+            "(i64)l_receiptdate >= q12_date_lo && (i64)l_receiptdate < q12_date_hi");
+        auto join = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q12_orders", "l_orderkey", "o_bits");
+        auto flag = std::make_unique<HipBitmapFlag>(std::move(join), "d_q12_high", "l_orderkey", "o_bits", "_high");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(flag), "d_q12_counts", "(_mc == q12_mode1 ? 0 : 1)", 2);
+        agg->addAggregate("high_line_count", AggKind::SumI64, "(i64)_high", 0);
+        agg->addAggregate("low_line_count", AggKind::SumI64, "1 - (i64)_high", 0);
+        agg->setBucketLabels({opt(o, "mode1", "MAIL"), opt(o, "mode2", "SHIP")}, "l_shipmode");
+        auto& ph = appendPhase(plan, "Q12_count", std::move(agg), 256, 4);
+        ph.launchBound = 256;
+        ph.scalarParams = {"q12_mode1", "q12_mode2", "q12_date_lo", "q12_date_hi"};
+    }
+    plan.requiredScalars = {"q12_mode1", "q12_mode2", "q12_date_lo", "q12_date_hi"};
+    plan.requiredSymbols = {"o_bits"};
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q13 — customer distribution (sql/q13.sql; buildQ13Plan_byName,
+// metal_plan_builder.cpp:1244-1299): orders whose o_comment is NOT LIKE
+// '%w1%w2%' counted per customer, then the histogram of those counts over all
+// customers (customers without such orders count 0), ordered custdist desc,
+// c_count desc. The words are the registered buffer d_like_words (w1 at 0,
+// w2 at 32) with lengths like_l1, like_l2.
+HipQueryPlan tpchQ13(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q13";
+    const int histBins = optInt(o, "hist_bins", 1024);
+    {
+        auto scan = makeScan("orders", {"o_custkey", "o_comment"});
+        auto sel = std::make_unique<HipSelection>(
+            std::move(scan),
+            "!gdb_like2(o_comment, 79, (const char*)d_like_words, (int)like_l1, (const char*)d_like_words + 32, "
+            "(int)like_l2)");
+        auto cnt = std::make_unique<HipAtomicCount>(std::move(sel), "d_q13_counts", "o_custkey", "c_bits", 0);
+        auto& ph = appendPhase(plan, "Q13_count", std::move(cnt), 256, 2, {2});
+        ph.launchBound = 256;
+        ph.scalarParams = {"like_l1", "like_l2"};
+        ph.externalBuffers = {{"d_like_words", "u8"}};
+    }
+    {
+        auto scan = makeScan("customer", {"c_custkey"});
+        auto look = std::make_unique<HipArrayLookup>(std::move(scan), "d_q13_counts", "c_custkey", "c_bits", "_cnt",
+                                                     -1ll);
+        auto hist = std::make_unique<HipAtomicCount>(std::move(look), "d_q13_hist", "_cnt", std::to_string(histBins),
+                                                     histBins);
+        hist->setResult("c_count", "custdist", true);
+        appendPhase(plan, "Q13_histogram", std::move(hist), 256, 4);
+    }
+    plan.requiredScalars = {"like_l1", "like_l2"};
+    plan.requiredSymbols = {"c_bits"};
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q14 — promotion effect (sql/q14.sql): bitmaps of all part keys and of
+// parts whose p_type starts with the registered prefix (d_like_words,
+// like_l1 bytes), then the month's lineitems joined to part:
+// promo_revenue = 100 * sum(promo revenue) / sum(revenue), exact to 6 digits.
+HipQueryPlan tpchQ14(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q14";
+    {
+        auto scan = makeScan("part", {"p_partkey", "p_type"});
+        auto any = std::make_unique<HipBitmapBuild>(std::move(scan), "d_q14_parts", "p_partkey", "p_bits");
+        auto sel = std::make_unique<HipSelection>(std::move(any),
+                                                  "gdb_like_prefix(p_type, 25, (const char*)d_like_words, (int)like_l1)");
+        auto promo = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q14_promo", "p_partkey", "p_bits");
+        auto& ph = appendPhase(plan, "Q14_parts", std::move(promo), 256, 4);
+        ph.launchBound = 256;
+        ph.scalarParams = {"like_l1"};
+        ph.externalBuffers = {{"d_like_words", "u8"}};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_extendedprice", "l_discount", "l_shipdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)l_shipdate >= q14_date_lo && (i64)l_shipdate < q14_date_hi");
+        auto join = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q14_parts", "l_partkey", "p_bits");
+        auto flag = std::make_unique<HipBitmapFlag>(std::move(join), "d_q14_promo", "l_partkey", "p_bits", "_promo");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(flag), "d_q14_sums", "0", 1);
+        agg->addAggregate("promo_revenue_x100", AggKind::SumI64, std::string(kRevenue) + " * 100 * (i64)_promo", 4);
+        agg->addAggregate("revenue", AggKind::SumI64, kRevenue, 4);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        agg->addAverage("promo_revenue", "promo_revenue_x100", "revenue");
+        auto& ph = appendPhase(plan, "Q14_sum", std::move(agg), 256, optInt(o, "items", 4));
+        ph.scalarParams = {"q14_date_lo", "q14_date_hi"};
+    }
+    plan.requiredScalars = {"like_l1", "q14_date_lo", "q14_date_hi"};
+    plan.requiredSymbols = {"p_bits"};
+    plan.chunkable = true;
+    return plan;
+}
+
+// Q18 — large volume customer (sql/q18.sql; buildQ18Plan_byName,
+// metal_plan_builder.cpp:1529-1545): sum(l_quantity) per order key into a
+// key-indexed array, then the orders whose sum exceeds q18_qty appended as
+// (o_custkey, o_orderkey, o_orderdate, o_totalprice, sum) records, ordered
+// o_totalprice desc, o_orderdate asc, cut to 100. c_name is attached by the
+// caller for the result's customer keys.
+HipQueryPlan tpchQ18(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q18";
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_quantity"});
+        auto add = std::make_unique<HipDenseAtomicAdd>(std::move(scan), "d_q18_qty", "l_orderkey", "o_bits",
+                                                       "l_quantity");
+        appendPhase(plan, "Q18_order_qty", std::move(add), 256, optInt(o, "items", 4));
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_custkey", "o_orderdate", "o_totalprice"});
+        auto look = std::make_unique<HipArrayLookup>(std::move(scan), "d_q18_qty", "o_orderkey", "o_bits", "_qty", -1ll,
+                                                     "i64");
+        auto sel = std::make_unique<HipSelection>(std::move(look), "_qty > q18_qty");
+        auto rows = std::make_unique<HipAppendRows>(std::move(sel), "d_q18_rows", "n_orders");
+        rows->addField("c_custkey", "o_custkey");
+        rows->addField("o_orderkey", "o_orderkey");
+        rows->addField("o_orderdate", "o_orderdate", 0, true);
+        rows->addField("o_totalprice", "o_totalprice", 2);
+        rows->addField("sum(l_quantity)", "_qty", 2);
+        rows->orderBy("o_totalprice", true);
+        rows->orderBy("o_orderdate", false);
+        rows->setLimit(optInt(o, "limit", 100));
+        auto& ph = appendPhase(plan, "Q18_orders", std::move(rows), 256, 4);
+        ph.scalarParams = {"q18_qty"};
+    }
+    plan.requiredScalars = {"q18_qty"};
+    plan.requiredSymbols = {"o_bits"};
+    return plan;
+}
+
+// Q22 — global sales opportunity (sql/q22.sql; buildQ22Plan_byName,
+// metal_plan_builder.cpp:1304-1357): the average positive balance of
+// customers whose country code (first two phone digits) is in the set
+// (bit c of q22_mask_lo / q22_mask_hi for codes 0-63 / 64-127) is left in a
+// buffer as (sum, count); the customers with a code in the set, a balance
+// above that average (balance * count > sum, exact) and no order are counted
+// and their balances summed per code.
+HipQueryPlan tpchQ22(const PlanOptions& o) {
+    (void)o;
+    HipQueryPlan plan;
+    plan.name = "Q22";
+    const std::string code = "(i64)(((i64)(u8)c_phone[0] - 48) * 10 + ((i64)(u8)c_phone[1] - 48))";
+    const std::string inSet =
+        "(_cc >= 0 && _cc < 128 && (((_cc < 64 ? (u64)q22_mask_lo >> _cc : (u64)q22_mask_hi >> (_cc - 64)) & 1ull) != 0))";
+    {
+        auto scan = makeScan("customer", {"c_phone", "c_acctbal"});
+        auto cc = std::make_unique<HipComputeExpr>(std::move(scan), "_cc", "i64", code);
+        auto sel = std::make_unique<HipSelection>(std::move(cc), "c_acctbal > 0 && " + inSet);
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q22_avg", "0", 1);
+        agg->addAggregate("sum_acctbal", AggKind::SumI64, "c_acctbal", 2);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        auto& ph = appendPhase(plan, "Q22_avg_balance", std::move(agg), 256, 4);
+        ph.launchBound = 256;
+        ph.scalarParams = {"q22_mask_lo", "q22_mask_hi"};
+    }
+    {
+        auto scan = makeScan("orders", {"o_custkey"});
+        auto bm = std::make_unique<HipBitmapBuild>(std::move(scan), "d_q22_has_orders", "o_custkey", "c_bits");
+        appendPhase(plan, "Q22_orders_bitmap", std::move(bm), 256, 4);
+    }
+    {
+        auto scan = makeScan("customer", {"c_custkey", "c_phone", "c_acctbal"});
+        std::vector<std::pair<std::string, int>> vars = {{"_avg_sum", 0}, {"_avg_cnt", 1}};
+        auto avg = std::make_unique<HipBufferScalars>(std::move(scan), "d_q22_avg", "16", vars);
+        auto cc = std::make_unique<HipComputeExpr>(std::move(avg), "_cc", "i64", code);
+        auto sel = std::make_unique<HipSelection>(std::move(cc), inSet + " && c_acctbal * _avg_cnt > _avg_sum");
+        auto anti = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q22_has_orders", "c_custkey", "c_bits", true);
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(anti), "d_q22_out", "(int)_cc", 100);
+        agg->addAggregate("numcust", AggKind::Count, "1", 0);
+        agg->addAggregate("totacctbal", AggKind::SumI64, "c_acctbal", 2);
+        std::vector<std::string> labels;
+        for (int c = 0; c < 100; ++c) labels.push_back((c < 10 ? "0" : "") + std::to_string(c));
+        agg->setBucketLabels(labels, "cntrycode");
+        auto& ph = appendPhase(plan, "Q22_anti", std::move(agg), 256, 4);
+        ph.launchBound = 256;
+        ph.scalarParams = {"q22_mask_lo", "q22_mask_hi"};
+    }
+    plan.requiredScalars = {"q22_mask_lo", "q22_mask_hi"};
+    plan.requiredSymbols = {"c_bits"};
+    return plan;
+}
+
+bool tpchSchema(const PlanOptions& o) { return opt(o, "schema", "") == "tpch"; }
+
+}  // namespace
+
+std::vector<std::string> tpchPlanNames() { return {"Q4", "Q12", "Q13", "Q14", "Q18", "Q22"}; }
+
+std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOptions& opts) {
+    const std::string schema = opt(opts, "schema", "");
+    if (!schema.empty() && schema != "tpch") throw std::runtime_error("unknown schema " + schema);
+    if (name == "Q1" && tpchSchema(opts)) return tpchQ1(opts);
+    if (name == "Q3" && tpchSchema(opts)) return tpchQ3(opts);
+    if (name == "Q6" && tpchSchema(opts)) return tpchQ6(opts);
+    if (name == "Q4") return tpchQ4(opts);
+    if (name == "Q12") return tpchQ12(opts);
+    if (name == "Q13") return tpchQ13(opts);
+    if (name == "Q14") return tpchQ14(opts);
+    if (name == "Q18") return tpchQ18(opts);
+    if (name == "Q22") return tpchQ22(opts);
+    return std::nullopt;
+}
+
+}  // namespace codegen
+}  // namespace gdb

@@ -233,6 +233,7 @@ class Timing:
 class GenericResult:
     columns: dict  # name -> numpy array (int64; int128 as Python-int object arrays; strings)
     canonical: str
+    formats: dict = field(default_factory=dict)  # name -> (scale digits, is_date) of integer columns
 
     def rows(self) -> int:
         return 0 if not self.columns else len(next(iter(self.columns.values())))
@@ -370,7 +371,7 @@ class HipGenericExecutor:
         try:
             rows, cols = C.c_uint64(), C.c_int()
             N.eng_check(_eng().gdb_result_shape(r, C.byref(rows), C.byref(cols)))
-            out = {}
+            out, fmts = {}, {}
             for i in range(cols.value):
                 name = C.create_string_buffer(128)
                 typ = C.c_int()
@@ -391,8 +392,11 @@ class HipGenericExecutor:
                     a = np.array([raw.raw[k * width:(k + 1) * width].rstrip(b"\0").decode() for k in range(rows.value)],
                                  dtype=object)
                 out[nm] = a
+                sc, dt = C.c_int(), C.c_int()
+                N.eng_check(_eng().gdb_result_column_format(r, i, C.byref(sc), C.byref(dt)))
+                fmts[nm] = (sc.value, bool(dt.value))
             canon = N.two_call_string(_eng().gdb_result_canonical, r) if rows.value <= 100000 else ""
-            return GenericResult(out, canon)
+            return GenericResult(out, canon, fmts)
         finally:
             _eng().gdb_result_free(r)
 

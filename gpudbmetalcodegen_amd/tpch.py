@@ -1,0 +1,388 @@
+"""TPC-H front end of the engine: ``.tbl`` -> ``.colbin`` ingest, loading
+``.colbin`` tables into the engine's column formats, the named TPC-H plans
+(Q1, Q3, Q4, Q6, Q12, Q13, Q14, Q18, Q22) with the constants of their SQL text,
+canonical CSV and the golden check.
+
+Counterparts in the reference: ``tools/tbl_to_colbin.cpp:106-172`` (ingest),
+``loadColumnsFromBinary`` (``src/infra.h:579-664``), the per-query driver
+``runCodegenQuery`` (``codegen/codegen_main.cpp:271-1776``) with its CPU
+post-processing (top-k ties, names), and ``--check`` / ``--save-golden``
+(``codegen/codegen_main.cpp:1722-1760``, golden files named
+``<query>_<scale>.csv``).
+
+Column formats after loading: INT int32, DATE int32 YYYYMMDD, FLOAT int64
+hundredths (round(float32 * 100): exact for |v| < 131072, i.e. every TPC-H
+money column except o_totalprice, whose float32 in the .colbin format already
+lost cents above that), CHAR1 uint8, CHAR_FIXED uint8 (rows, width).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import datetime as _dt
+import os
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+from . import _native as N
+from . import engine as E
+
+INT, FLOAT, DATE, CHAR1, CHAR_FIXED = 0, 1, 2, 3, 4
+TABLES = ["lineitem", "orders", "customer", "supplier", "part", "partsupp", "nation", "region"]
+QUERIES = ["Q1", "Q3", "Q4", "Q6", "Q12", "Q13", "Q14", "Q18", "Q22"]
+CHECK_ABS_TOL = 1e-2  # the reference's --check defaults (codegen_main.cpp:47-48)
+CHECK_REL_TOL = 1e-4
+
+
+def _eng():
+    return N.load()[1]
+
+
+# ---------------------------------------------------------------------------
+# Catalog, ingest, .colbin
+# ---------------------------------------------------------------------------
+
+def table_columns(table: str) -> list:
+    """[(name, index, dtype, width)] of a TPC-H table in file order."""
+    names = N.two_call_string(_eng().gdb_tpch_table_columns, table.encode()).split(",")
+    out = []
+    for n in names:
+        idx, dt, w = C.c_int(), C.c_int(), C.c_int()
+        N.eng_check(_eng().gdb_tpch_column(table.encode(), n.encode(), C.byref(idx), C.byref(dt), C.byref(w)))
+        out.append((n, idx.value, dt.value, w.value))
+    return out
+
+
+def tbl_to_colbin(tbl_path, table: str, force: bool = False, threads: int = 0) -> bool:
+    """Build <base>.colbin next to the .tbl unless it is up to date; True when written."""
+    built = C.c_int()
+    N.eng_check(_eng().gdb_tbl_to_colbin(str(tbl_path).encode(), table.encode(), int(force), threads, C.byref(built)))
+    return bool(built.value)
+
+
+def ingest(data_dir, tables=None, force: bool = False, threads: int = 0) -> dict:
+    """The tbl_to_colbin tool over a directory: {table: built?} for every
+    <table>.tbl present."""
+    d = Path(data_dir)
+    out = {}
+    for t in tables or TABLES:
+        p = d / f"{t}.tbl"
+        if p.exists():
+            out[t] = tbl_to_colbin(p, t, force, threads)
+    return out
+
+
+class Colbin:
+    """Memory-mapped .colbin file; tbl_path (optional) enables the staleness check."""
+
+    def __init__(self, path, tbl_path=None):
+        self._h = C.c_void_p()
+        N.eng_check(_eng().gdb_colbin_open(str(path).encode(), str(tbl_path).encode() if tbl_path else None,
+                                           C.byref(self._h)))
+        rows, ncols = C.c_uint64(), C.c_int()
+        N.eng_check(_eng().gdb_colbin_shape(self._h, C.byref(rows), C.byref(ncols)))
+        self.rows, self.ncols = rows.value, ncols.value
+
+    def raw(self, index: int):
+        """(bytes copy as uint8 array, dtype, width) of the column with this index."""
+        p, nb, dt, w = C.c_void_p(), C.c_uint64(), C.c_int(), C.c_int()
+        N.eng_check(_eng().gdb_colbin_column(self._h, index, C.byref(p), C.byref(nb), C.byref(dt), C.byref(w)))
+        buf = (C.c_uint8 * nb.value).from_address(p.value) if nb.value else b""
+        return np.frombuffer(buf, dtype=np.uint8).copy(), dt.value, w.value
+
+    def close(self):
+        if self._h:
+            _eng().gdb_colbin_close(self._h)
+            self._h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def to_engine_format(raw: np.ndarray, dtype: int, width: int, rows: int) -> np.ndarray:
+    if dtype in (INT, DATE):
+        return raw.view(np.int32)
+    if dtype == FLOAT:
+        return np.rint(raw.view(np.float32).astype(np.float64) * 100.0).astype(np.int64)
+    if dtype == CHAR1:
+        return raw
+    return raw.reshape(rows, width)
+
+
+def load_colbin(path, table: str, columns=None, tbl_path=None) -> dict:
+    """Columns of a .colbin file in the engine's formats, {name: array}."""
+    want = set(columns) if columns else None
+    out = {}
+    with Colbin(path, tbl_path) as f:
+        for name, idx, dt, w in table_columns(table):
+            if want is not None and name not in want:
+                continue
+            raw, fdt, fw = f.raw(idx)
+            if fdt != dt or (dt == CHAR_FIXED and fw != w):
+                raise ValueError(f"{path}: column {name} has type {fdt}/{fw}, expected {dt}/{w}")
+            out[name] = to_engine_format(raw, dt, w, f.rows)
+    return out
+
+
+def load_dir(data_dir, tables=None) -> dict:
+    d = Path(data_dir)
+    return {t: load_colbin(d / f"{t}.colbin", t, tbl_path=d / f"{t}.tbl") for t in (tables or TABLES)
+            if (d / f"{t}.colbin").exists()}
+
+
+# ---------------------------------------------------------------------------
+# The queries' constants (from the SQL text, sql/q*.sql)
+# ---------------------------------------------------------------------------
+
+def ymd(y: int, m: int, d: int) -> int:
+    return y * 10000 + m * 100 + d
+
+
+def shift(date: int, days: int = 0, months: int = 0, years: int = 0) -> int:
+    """YYYYMMDD +/- an interval (month arithmetic keeps the day, as SQL's
+    DATE + INTERVAL on the first of a month)."""
+    y, m, d = date // 10000, date // 100 % 100, date % 100
+    m0 = (y + years) * 12 + (m - 1) + months
+    t = _dt.date(m0 // 12, m0 % 12 + 1, d) + _dt.timedelta(days=days)
+    return ymd(t.year, t.month, t.day)
+
+
+def mode_code(mode: str) -> int:
+    b = mode.encode()[:2].ljust(2, b"\0")
+    return b[0] | (b[1] << 8)
+
+
+Q22_CODES = (13, 31, 23, 29, 30, 18, 17)
+
+PARAMS = {
+    "Q1": {"q1_cutoff": shift(ymd(1998, 12, 1), days=-90)},
+    "Q3": {"q3_seg": ord("B"), "q3_date": ymd(1995, 3, 15)},  # BUILDING (CHAR1 keeps the first byte)
+    "Q4": {"q4_date_lo": ymd(1993, 7, 1), "q4_date_hi": shift(ymd(1993, 7, 1), months=3)},
+    "Q6": {"q6_date_lo": ymd(1994, 1, 1), "q6_date_hi": shift(ymd(1994, 1, 1), years=1), "q6_disc_lo": 5,
+           "q6_disc_hi": 7, "q6_qty_hi": 2400},
+    "Q12": {"q12_mode1": mode_code("MAIL"), "q12_mode2": mode_code("SHIP"), "q12_date_lo": ymd(1994, 1, 1),
+            "q12_date_hi": shift(ymd(1994, 1, 1), years=1)},
+    "Q13": {"words": (b"special", b"requests")},
+    "Q14": {"prefix": b"PROMO", "q14_date_lo": ymd(1995, 9, 1), "q14_date_hi": shift(ymd(1995, 9, 1), months=1)},
+    "Q18": {"q18_qty": 30000},
+    "Q22": {"q22_mask_lo": sum(1 << c for c in Q22_CODES if c < 64),
+            "q22_mask_hi": sum(1 << (c - 64) for c in Q22_CODES if c >= 64)},
+}
+
+PLAN_OPTIONS = {"Q1": "schema=tpch", "Q3": "schema=tpch", "Q6": "schema=tpch"}
+SCANNED = {"Q1": ["lineitem"], "Q3": ["customer", "orders", "lineitem"], "Q4": ["lineitem", "orders"],
+           "Q6": ["lineitem"], "Q12": ["orders", "lineitem"], "Q13": ["orders", "customer"],
+           "Q14": ["part", "lineitem"], "Q18": ["lineitem", "orders"], "Q22": ["customer", "orders"]}
+# Output columns in the SQL select list's order.
+SELECT = {
+    "Q1": ["l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price", "sum_charge", "avg_qty",
+           "avg_price", "avg_disc", "count_order"],
+    "Q3": ["l_orderkey", "revenue", "o_orderdate", "o_shippriority"],
+    "Q4": ["o_orderpriority", "order_count"],
+    "Q6": ["revenue"],
+    "Q12": ["l_shipmode", "high_line_count", "low_line_count"],
+    "Q13": ["c_count", "custdist"],
+    "Q14": ["promo_revenue"],
+    "Q18": ["c_name", "c_custkey", "o_orderkey", "o_orderdate", "o_totalprice", "sum(l_quantity)"],
+    "Q22": ["cntrycode", "numcust", "totacctbal"],
+}
+
+
+def _pow2(n: int) -> int:
+    return 1 << max(6, int(n - 1).bit_length())
+
+
+# ---------------------------------------------------------------------------
+# Database and runs
+# ---------------------------------------------------------------------------
+
+class TpchDB:
+    """TPC-H tables in host memory (engine formats) and, on first use, in HBM;
+    key-range symbols computed once (registerMaxKeySymbols,
+    codegen/execution/max_key_symbols.cpp:188-235)."""
+
+    def __init__(self, ctx: E.Context, host: dict):
+        self.ctx = ctx
+        self.host = host
+        self.dev: dict = {}
+        h = host
+        self.symbols = {}
+        if "customer" in h:
+            self.symbols["c_bits"] = int(h["customer"]["c_custkey"].max(initial=0)) + 1
+        if "orders" in h:
+            self.symbols["o_bits"] = int(h["orders"]["o_orderkey"].max(initial=0)) + 1
+        if "lineitem" in h:
+            self.symbols["o_bits"] = max(self.symbols.get("o_bits", 1), int(h["lineitem"]["l_orderkey"].max(initial=0)) + 1)
+        if "part" in h:
+            self.symbols["p_bits"] = int(h["part"]["p_partkey"].max(initial=0)) + 1
+        if "lineitem" in h:
+            self.symbols["p_bits"] = max(self.symbols.get("p_bits", 1), int(h["lineitem"]["l_partkey"].max(initial=0)) + 1)
+        if "orders" in h:
+            self.symbols["c_bits"] = max(self.symbols.get("c_bits", 1), int(h["orders"]["o_custkey"].max(initial=0)) + 1)
+
+    def device(self, table: str) -> E.DeviceTable:
+        t = self.dev.get(table)
+        if t is None:
+            t = E.DeviceTable.from_host(self.ctx, table, self.host[table])
+            self.dev[table] = t
+        return t
+
+    def free(self) -> None:
+        for t in self.dev.values():
+            t.free(self.ctx)
+        self.dev.clear()
+
+
+@dataclass
+class TpchRun:
+    query: str
+    columns: dict            # name -> list of Python values (int, or str)
+    formats: dict            # name -> (scale digits, is_date)
+    timing: E.Timing
+    plan: E.HipQueryPlan = None
+    executor: E.HipGenericExecutor = None
+    raw: E.GenericResult = None
+    canonical: str = field(default="")
+
+
+def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int = 1, options: str = "",
+        executor: E.HipGenericExecutor | None = None, params: dict | None = None) -> TpchRun:
+    """Run a named TPC-H query; the result columns follow the SQL select
+    list. params overrides constants of the query (e.g. q18_qty)."""
+    if query not in QUERIES:
+        raise ValueError(f"unknown TPC-H query {query}; known: {QUERIES}")
+    opts = ",".join(o for o in (PLAN_OPTIONS.get(query, ""), options) if o)
+    from .queries import get_plan, like_words_buffer
+
+    plan = get_plan(ctx, query, opts)
+    ex = executor or E.HipGenericExecutor(ctx)
+    for t in SCANNED[query]:
+        ex.register_table(db.device(t))
+    prm = dict(PARAMS[query], **(params or {}))
+    words_ptr = None
+    for k, v in prm.items():
+        if k.startswith("q"):
+            ex.register_scalar(k, int(v))
+    for k, v in db.symbols.items():
+        ex.register_symbol(k, v)
+    if query == "Q3":
+        od = db.host["orders"]["o_orderdate"]
+        ex.register_symbol("o_slots", _pow2(2 * int((od < prm["q3_date"]).sum())))
+    try:
+        if query == "Q13":
+            w1, w2 = prm["words"]
+            words_ptr = like_words_buffer(ctx, w1, w2)
+            ex.register_buffer("d_like_words", words_ptr, 64)
+            ex.register_scalar("like_l1", len(w1))
+            ex.register_scalar("like_l2", len(w2))
+        elif query == "Q14":
+            words_ptr = like_words_buffer(ctx, prm["prefix"], b"")
+            ex.register_buffer("d_like_words", words_ptr, 64)
+            ex.register_scalar("like_l1", len(prm["prefix"]))
+        t = ex.execute(plan, warmup, measured)
+        res = ex.collect(plan)
+    finally:
+        if words_ptr is not None:
+            ctx.free(words_ptr)
+    cols = {k: [int(x) if not isinstance(x, str) else x for x in v] for k, v in res.columns.items()}
+    fmts = dict(res.formats)
+    _finish(query, db, cols, fmts)
+    out = {name: cols[name] for name in SELECT[query]}
+    run_ = TpchRun(query, out, {k: fmts.get(k, (0, False)) for k in out}, t, plan, ex, res)
+    run_.canonical = canonical(run_.columns, run_.formats)
+    return run_
+
+
+def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
+    """Host post-processing on the few result rows (the reference's CPU
+    post-processing in runCodegenQuery): attach o_shippriority (Q3) and c_name
+    (Q18) by key."""
+    if query == "Q3":
+        o = db.host["orders"]
+        sub = np.flatnonzero(np.isin(o["o_orderkey"], cols["l_orderkey"]))
+        sp = {int(o["o_orderkey"][i]): int(o["o_shippriority"][i]) for i in sub}
+        cols["o_shippriority"] = [sp[k] for k in cols["l_orderkey"]]
+        fmts["o_shippriority"] = (0, False)
+    elif query == "Q18":
+        c = db.host["customer"]
+        sub = np.flatnonzero(np.isin(c["c_custkey"], cols["c_custkey"]))
+        names = {int(c["c_custkey"][i]): bytes(c["c_name"][i]).rstrip(b"\0").decode() for i in sub}
+        cols["c_name"] = [names[k] for k in cols["c_custkey"]]
+
+
+# ---------------------------------------------------------------------------
+# Canonical CSV and the golden check
+# ---------------------------------------------------------------------------
+
+def fixed_str(v: int, scale: int) -> str:
+    if scale <= 0:
+        return str(v)
+    s = str(abs(v)).rjust(scale + 1, "0")
+    return ("-" if v < 0 else "") + s[:-scale] + "." + s[-scale:]
+
+
+def _field(v, scale: int, is_date: bool) -> str:
+    if isinstance(v, str):
+        if any(ch in v for ch in ',"\n\r'):
+            return '"' + v.replace('"', '""') + '"'
+        return v
+    if is_date:
+        return f"{v // 10000:04d}-{v // 100 % 100:02d}-{v % 100:02d}"
+    return fixed_str(int(v), scale)
+
+
+def canonical(columns: dict, formats: dict) -> str:
+    """Header line, then one line per row: exact decimals for scaled integers,
+    dates as YYYY-MM-DD, strings quoted when they hold ',', '"' or a line
+    break (the rules of GenericResult::toCanonical in the engine)."""
+    names = list(columns)
+    lines = [",".join(names)]
+    n = len(columns[names[0]]) if names else 0
+    for r in range(n):
+        lines.append(",".join(_field(columns[c][r], *formats.get(c, (0, False))) for c in names))
+    return "\n".join(lines) + "\n"
+
+
+def compare_canonical(got: str, expected: str, abs_tol: float = CHECK_ABS_TOL, rel_tol: float = CHECK_REL_TOL) -> str:
+    """'' on a match, else the first difference (compareCanonical,
+    codegen/codegen_main.cpp:63-186)."""
+    return N.two_call_string(_eng().gdb_compare_canonical, got.encode(), expected.encode(), C.c_double(abs_tol),
+                             C.c_double(rel_tol))
+
+
+def golden_name(query: str, scale_label: str) -> str:
+    return f"{query}_{scale_label}.csv"
+
+
+def save_golden(canonical_csv: str, golden_dir, query: str, scale_label: str) -> Path:
+    d = Path(golden_dir)
+    d.mkdir(parents=True, exist_ok=True)
+    p = d / golden_name(query, scale_label)
+    p.write_text(canonical_csv)
+    return p
+
+
+def check(canonical_csv: str, golden_dir, query: str, scale_label: str, abs_tol: float = CHECK_ABS_TOL,
+          rel_tol: float = CHECK_REL_TOL) -> tuple:
+    """(status, message): status 0 OK, 1 mismatch, 2 golden file missing (the
+    reference's --check exit codes)."""
+    p = Path(golden_dir) / golden_name(query, scale_label)
+    if not p.exists():
+        return 2, f"golden file missing: {p}"
+    diff = compare_canonical(canonical_csv, p.read_text(), abs_tol, rel_tol)
+    return (0, "OK") if not diff else (1, diff)
+
+
+def data_dir_from_env() -> str | None:
+    return os.environ.get("GDB_TPCH_DIR")

@@ -142,8 +142,43 @@ gdb_status gdb_result_column(gdb_result r, int idx, char* name, size_t cap, int*
 gdb_status gdb_result_i64(gdb_result r, int idx, int64_t* out);
 gdb_status gdb_result_i128(gdb_result r, int idx, int64_t* out);
 gdb_status gdb_result_str(gdb_result r, int idx, char* out, size_t width);
+/* Display format of an integer column: scale = fixed-point digits (the
+ * value is v / 10^scale), is_date = YYYYMMDD shown as YYYY-MM-DD. */
+gdb_status gdb_result_column_format(gdb_result r, int idx, int* scale, int* is_date);
+/* Canonical CSV: header, then one line per row; integers exact (scaled
+ * columns as exact decimals), dates YYYY-MM-DD, strings quoted when they hold
+ * ',' '"' or a line break (GenericResult::toCanonical,
+ * metal_result_collector.cpp:85-122, which prints doubles with %.4f). */
 gdb_status gdb_result_canonical(gdb_result r, char* buf, size_t cap, size_t* need);
 gdb_status gdb_result_free(gdb_result r);
+
+/* ---- TPC-H ingest: .tbl -> .colbin v2, and the golden check ----
+ * Catalog of the TPC-H tables (codegen/planning/tpch_schema.h:47-166):
+ * dtype 0 INT (int32), 1 FLOAT (float32), 2 DATE (int32 YYYYMMDD), 3 CHAR1
+ * (one byte), 4 CHAR_FIXED (width bytes, NUL-padded). */
+gdb_status gdb_tpch_column(const char* table, const char* column, int* index, int* dtype, int* width);
+/* Column names of a table, comma-separated, in file order. */
+gdb_status gdb_tpch_table_columns(const char* table, char* buf, size_t cap, size_t* need);
+/* Parse <dir>/<table>.tbl and write <dir>/<table>.colbin unless it is up to
+ * date (force = 1 rebuilds); *built = 1 when a file was written. threads <= 0
+ * uses the host's hardware threads. Replaces tools/tbl_to_colbin.cpp:106-172
+ * for one table (the .colbin layout of src/infra.h:511-771). */
+gdb_status gdb_tbl_to_colbin(const char* tbl_path, const char* table, int force, int threads, int* built);
+/* Memory-mapped .colbin reader (loadColumnsFromBinary, src/infra.h:579-664).
+ * tbl_path may be NULL; when it names an existing file the recorded source
+ * size and mtime must match it. */
+typedef struct gdb_colbin_s* gdb_colbin;
+gdb_status gdb_colbin_open(const char* colbin_path, const char* tbl_path, gdb_colbin* out);
+gdb_status gdb_colbin_shape(gdb_colbin f, uint64_t* rows, int* ncols);
+/* Column by its TPC-H column index; data stays valid until gdb_colbin_close. */
+gdb_status gdb_colbin_column(gdb_colbin f, int column_index, const void** data, uint64_t* bytes, int* dtype,
+                             int* width);
+void gdb_colbin_close(gdb_colbin f);
+/* Compare a canonical CSV with a golden one (compareCanonical,
+ * codegen/codegen_main.cpp:63-186): GDB_OK with an empty message on a match,
+ * GDB_OK with the first difference in msg otherwise. */
+gdb_status gdb_compare_canonical(const char* got, const char* expected, double abs_tol, double rel_tol, char* msg,
+                                 size_t cap, size_t* need);
 
 /* ---- Autotuner: sweep block x ITEMS x (optional) runtime scalar for one
  * phase; 1 untimed + `trials` timed runs per candidate, slowest dropped, p50

@@ -1,0 +1,253 @@
+"""CPU oracle of the named TPC-H queries (numpy, exact integers).
+
+Test infrastructure only (tests/ and bench.py's cpu_baseline leg): a plain
+restatement of each query's SQL (the reference's sql/q1.sql ... sql/q22.sql)
+over the TPC-H tables in the engine's column formats (INT/DATE int32 with
+dates as YYYYMMDD, money and rates int64 hundredths, CHAR1 uint8, CHAR_FIXED
+uint8 (rows, width)). Results are {column: list} in the select list's order
+with the same integer scaling as the engine (revenue = price * (100 -
+discount), 4 decimal digits; averages trunc(sum * 10^6 / count)), plus the
+display formats. Pinned in tests against SQLite running the SQL text on the
+same rows (an independent engine) and against hand-computed tiny tables.
+"""
+
+from __future__ import annotations
+
+import re
+
+import numpy as np
+
+PRIORITY_LABELS = {ord("1"): "1-URGENT", ord("2"): "2-HIGH", ord("3"): "3-MEDIUM", ord("4"): "4-NOT SPECIFIED",
+                   ord("5"): "5-LOW"}
+
+
+def _s(arr_row) -> bytes:
+    return bytes(arr_row).split(b"\0", 1)[0]
+
+
+def _avg(num: int, den: int) -> int:
+    if den == 0:
+        return 0
+    q = abs(num) * 1_000_000 // abs(den)
+    return q if (num >= 0) == (den > 0) else -q
+
+
+def q1(T: dict, cutoff: int) -> tuple:
+    """sql/q1.sql; groups ordered by (returnflag, linestatus)."""
+    L = T["lineitem"]
+    m = L["l_shipdate"] <= cutoff
+    f, st = L["l_returnflag"][m], L["l_linestatus"][m]
+    qty, price = L["l_quantity"][m].astype(object), L["l_extendedprice"][m].astype(object)
+    disc, tax = L["l_discount"][m].astype(object), L["l_tax"][m].astype(object)
+    cols = {k: [] for k in ("l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price",
+                            "sum_charge", "avg_qty", "avg_price", "avg_disc", "count_order")}
+    for fv in sorted(set(f.tolist())):
+        for sv in sorted(set(st[f == fv].tolist())):
+            g = (f == fv) & (st == sv)
+            n = int(g.sum())
+            sq, sp, sd = int(qty[g].sum()), int(price[g].sum()), int(disc[g].sum())
+            cols["l_returnflag"].append(chr(fv))
+            cols["l_linestatus"].append(chr(sv))
+            cols["sum_qty"].append(sq)
+            cols["sum_base_price"].append(sp)
+            cols["sum_disc_price"].append(int((price[g] * (100 - disc[g])).sum()))
+            cols["sum_charge"].append(int((price[g] * (100 - disc[g]) * (100 + tax[g])).sum()))
+            cols["avg_qty"].append(_avg(sq, n))
+            cols["avg_price"].append(_avg(sp, n))
+            cols["avg_disc"].append(_avg(sd, n))
+            cols["count_order"].append(n)
+    fmt = {"sum_qty": 2, "sum_base_price": 2, "sum_disc_price": 4, "sum_charge": 6, "avg_qty": 8, "avg_price": 8,
+           "avg_disc": 8}
+    return cols, {k: (fmt.get(k, 0), False) for k in cols}
+
+
+def q3(T: dict, seg: int, date: int, k: int = 10) -> tuple:
+    """sql/q3.sql; ties of (revenue, o_orderdate) broken by l_orderkey asc."""
+    C, O, L = T["customer"], T["orders"], T["lineitem"]
+    cust = set(C["c_custkey"][C["c_mktsegment"] == seg].tolist())
+    om = O["o_orderdate"] < date
+    odate, oship = {}, {}
+    for key, ck, d, sp in zip(O["o_orderkey"][om].tolist(), O["o_custkey"][om].tolist(), O["o_orderdate"][om].tolist(),
+                              O["o_shippriority"][om].tolist()):
+        if ck in cust:
+            odate[key] = d
+            oship[key] = sp
+    lm = L["l_shipdate"] > date
+    rev: dict = {}
+    for key, p, d in zip(L["l_orderkey"][lm].tolist(), L["l_extendedprice"][lm].tolist(), L["l_discount"][lm].tolist()):
+        if key in odate:
+            rev[key] = rev.get(key, 0) + p * (100 - d)
+    top = sorted(rev.items(), key=lambda kv: (-kv[1], odate[kv[0]], kv[0]))[:k]
+    cols = {"l_orderkey": [t[0] for t in top], "revenue": [t[1] for t in top],
+            "o_orderdate": [odate[t[0]] for t in top], "o_shippriority": [oship[t[0]] for t in top]}
+    return cols, {"l_orderkey": (0, False), "revenue": (4, False), "o_orderdate": (0, True),
+                  "o_shippriority": (0, False)}
+
+
+def q4(T: dict, lo: int, hi: int) -> tuple:
+    """sql/q4.sql."""
+    O, L = T["orders"], T["lineitem"]
+    late = set(L["l_orderkey"][L["l_commitdate"] < L["l_receiptdate"]].tolist())
+    m = (O["o_orderdate"] >= lo) & (O["o_orderdate"] < hi)
+    counts: dict = {}
+    for key, pr in zip(O["o_orderkey"][m].tolist(), O["o_orderpriority"][m].tolist()):
+        if key in late:
+            counts[pr] = counts.get(pr, 0) + 1
+    prs = sorted(p for p in counts if p in PRIORITY_LABELS)
+    cols = {"o_orderpriority": [PRIORITY_LABELS[p] for p in prs], "order_count": [counts[p] for p in prs]}
+    return cols, {k: (0, False) for k in cols}
+
+
+def q6(T: dict, lo: int, hi: int, disc_lo: int, disc_hi: int, qty_hi: int) -> tuple:
+    """sql/q6.sql."""
+    L = T["lineitem"]
+    m = ((L["l_shipdate"] >= lo) & (L["l_shipdate"] < hi) & (L["l_discount"] >= disc_lo) & (L["l_discount"] <= disc_hi)
+         & (L["l_quantity"] < qty_hi))
+    rev = int((L["l_extendedprice"][m].astype(object) * L["l_discount"][m].astype(object)).sum())
+    return {"revenue": [rev]}, {"revenue": (4, False)}
+
+
+def q12(T: dict, modes: tuple, lo: int, hi: int) -> tuple:
+    """sql/q12.sql; modes are the full names (the catalog keeps 2 bytes)."""
+    O, L = T["orders"], T["lineitem"]
+    pri = dict(zip(O["o_orderkey"].tolist(), O["o_orderpriority"].tolist()))
+    code = {m.encode()[:2]: m for m in modes}
+    out: dict = {}
+    m = ((L["l_commitdate"] < L["l_receiptdate"]) & (L["l_shipdate"] < L["l_commitdate"]) &
+         (L["l_receiptdate"] >= lo) & (L["l_receiptdate"] < hi))
+    for i in np.flatnonzero(m):
+        mode = code.get(_s(L["l_shipmode"][i]))
+        key = int(L["l_orderkey"][i])
+        if mode is None or key not in pri:
+            continue
+        hi_lo = out.setdefault(mode, [0, 0])
+        hi_lo[0 if pri[key] in (ord("1"), ord("2")) else 1] += 1
+    ms = sorted(out)
+    cols = {"l_shipmode": ms, "high_line_count": [out[x][0] for x in ms], "low_line_count": [out[x][1] for x in ms]}
+    return cols, {k: (0, False) for k in cols}
+
+
+def q13(T: dict, w1: bytes, w2: bytes) -> tuple:
+    """sql/q13.sql: per customer the orders whose comment is NOT LIKE
+    '%w1%w2%' (the comment up to its first NUL), then the histogram."""
+    O, C = T["orders"], T["customer"]
+    pat = re.compile(re.escape(w1) + b".*" + re.escape(w2), re.S)
+    per: dict = {}
+    for ck, com in zip(O["o_custkey"].tolist(), O["o_comment"]):
+        if not pat.search(_s(com)):
+            per[ck] = per.get(ck, 0) + 1
+    hist: dict = {}
+    for ck in C["c_custkey"].tolist():
+        c = per.get(ck, 0)
+        hist[c] = hist.get(c, 0) + 1
+    rows = sorted(hist.items(), key=lambda kv: (-kv[1], -kv[0]))
+    cols = {"c_count": [r[0] for r in rows], "custdist": [r[1] for r in rows]}
+    return cols, {k: (0, False) for k in cols}
+
+
+def q14(T: dict, prefix: bytes, lo: int, hi: int) -> tuple:
+    """sql/q14.sql: 100 * promo revenue / revenue, truncated to 6 digits."""
+    P, L = T["part"], T["lineitem"]
+    parts = set(P["p_partkey"].tolist())
+    promo = {int(k) for k, t in zip(P["p_partkey"], P["p_type"]) if bytes(t[:len(prefix)]) == prefix}
+    m = (L["l_shipdate"] >= lo) & (L["l_shipdate"] < hi)
+    tot = pro = 0
+    for pk, p, d in zip(L["l_partkey"][m].tolist(), L["l_extendedprice"][m].tolist(), L["l_discount"][m].tolist()):
+        if pk in parts:
+            r = p * (100 - d)
+            tot += r
+            if pk in promo:
+                pro += r
+    return {"promo_revenue": [_avg(100 * pro, tot)]}, {"promo_revenue": (6, False)}
+
+
+def q18(T: dict, qty: int, k: int = 100) -> tuple:
+    """sql/q18.sql; ordered o_totalprice desc, o_orderdate asc, then by
+    (c_custkey, o_orderkey)."""
+    C, O, L = T["customer"], T["orders"], T["lineitem"]
+    keys, inv = np.unique(L["l_orderkey"], return_inverse=True)
+    sums = np.bincount(inv, weights=L["l_quantity"].astype(np.float64)).astype(np.int64)
+    big = dict(zip(keys[sums > qty].tolist(), sums[sums > qty].tolist()))
+    names = {int(ck): _s(n).decode() for ck, n in zip(C["c_custkey"], C["c_name"])}
+    rows = []
+    for ok, ck, d, tp in zip(O["o_orderkey"].tolist(), O["o_custkey"].tolist(), O["o_orderdate"].tolist(),
+                             O["o_totalprice"].tolist()):
+        if ok in big:
+            rows.append((ck, ok, d, tp, big[ok]))
+    rows.sort(key=lambda r: (-r[3], r[2], r[0], r[1]))
+    rows = rows[:k]
+    cols = {"c_name": [names[r[0]] for r in rows], "c_custkey": [r[0] for r in rows], "o_orderkey": [r[1] for r in rows],
+            "o_orderdate": [r[2] for r in rows], "o_totalprice": [r[3] for r in rows],
+            "sum(l_quantity)": [r[4] for r in rows]}
+    return cols, {"c_name": (0, False), "c_custkey": (0, False), "o_orderkey": (0, False), "o_orderdate": (0, True),
+                  "o_totalprice": (2, False), "sum(l_quantity)": (2, False)}
+
+
+def q22(T: dict, codes) -> tuple:
+    """sql/q22.sql."""
+    C, O = T["customer"], T["orders"]
+    ph = C["c_phone"]
+    cc = (ph[:, 0].astype(np.int64) - 48) * 10 + (ph[:, 1].astype(np.int64) - 48)
+    ins = np.isin(cc, list(codes))
+    bal = C["c_acctbal"]
+    pos = ins & (bal > 0)
+    s, n = int(bal[pos].sum()), int(pos.sum())
+    has = set(O["o_custkey"].tolist())
+    out: dict = {}
+    for ck, c, b, i in zip(C["c_custkey"].tolist(), cc.tolist(), bal.tolist(), ins.tolist()):
+        if i and b * n > s and ck not in has:
+            e = out.setdefault(c, [0, 0])
+            e[0] += 1
+            e[1] += b
+    ks = sorted(out)
+    cols = {"cntrycode": [f"{c:02d}" for c in ks], "numcust": [out[c][0] for c in ks],
+            "totacctbal": [out[c][1] for c in ks]}
+    return cols, {"cntrycode": (0, False), "numcust": (0, False), "totacctbal": (2, False)}
+
+
+def run(T: dict, q: str, P: dict) -> tuple:
+    """Query q with the constants P (the names of the engine's scalars:
+    q1_cutoff, q3_seg, ... plus words / prefix for Q13 / Q14)."""
+    if q == "Q1":
+        return q1(T, P["q1_cutoff"])
+    if q == "Q3":
+        return q3(T, P["q3_seg"], P["q3_date"])
+    if q == "Q4":
+        return q4(T, P["q4_date_lo"], P["q4_date_hi"])
+    if q == "Q6":
+        return q6(T, P["q6_date_lo"], P["q6_date_hi"], P["q6_disc_lo"], P["q6_disc_hi"], P["q6_qty_hi"])
+    if q == "Q12":
+        modes = tuple(bytes([c & 255, c >> 8]).rstrip(b"\0").decode() for c in (P["q12_mode1"], P["q12_mode2"]))
+        full = {"MA": "MAIL", "SH": "SHIP", "RE": "REG AIR", "AI": "AIR", "RA": "RAIL", "TR": "TRUCK", "FO": "FOB"}
+        return q12(T, tuple(full[m] for m in modes), P["q12_date_lo"], P["q12_date_hi"])
+    if q == "Q13":
+        return q13(T, *P["words"])
+    if q == "Q14":
+        return q14(T, P["prefix"], P["q14_date_lo"], P["q14_date_hi"])
+    if q == "Q18":
+        return q18(T, P["q18_qty"])
+    if q == "Q22":
+        codes = [c for c in range(64) if (P["q22_mask_lo"] >> c) & 1] + \
+                [c + 64 for c in range(64) if (P["q22_mask_hi"] >> c) & 1]
+        return q22(T, codes)
+    raise ValueError(q)
+
+
+def canonical(cols: dict, fmts: dict) -> str:
+    """CSV with exact decimals for scaled integers and YYYY-MM-DD dates."""
+
+    def field(v, scale, is_date):
+        if isinstance(v, str):
+            return '"' + v.replace('"', '""') + '"' if any(ch in v for ch in ',"\n\r') else v
+        if is_date:
+            return f"{v // 10000:04d}-{v // 100 % 100:02d}-{v % 100:02d}"
+        if scale <= 0:
+            return str(v)
+        sgn = "-" if v < 0 else ""
+        digits = str(abs(v)).rjust(scale + 1, "0")
+        return f"{sgn}{digits[:-scale]}.{digits[-scale:]}"
+
+    names = list(cols)
+    n = len(cols[names[0]]) if names else 0
+    lines = [",".join(names)] + [",".join(field(cols[c][r], *fmts[c]) for c in names) for r in range(n)]
+    return "\n".join(lines) + "\n"
