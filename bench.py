@@ -424,6 +424,60 @@ def bench_serve(ctx) -> dict:
     return out
 
 
+# The reference's published Apple-M1 GPU times of its TPC-H plans at SF1
+# (median of 5 after 3 warmups; build/timing_20260501_153350/REPORT.md:14-35).
+# Its data is dbgen's; the rows here are generated with the same domains and
+# sizes, so the ratio is an analog comparison, not the same input.
+M1_SF1_MS = {"Q1": 7.80, "Q6": 2.03, "Q4": 3.15, "Q22": 0.97, "Q13": 13.05}
+
+
+def bench_tpch(ctx, sf: float, steps: int, warmup: int, check_sf: float = 0.02) -> dict:
+    """Named TPC-H plans at scale factor sf (generated TPC-H-shaped tables,
+    resident in HBM): per-query device time (mean of `steps` runs after
+    `warmup`), rows of the scanned tables per second; every query's result is
+    first checked bit-exactly against the oracle at check_sf (generated rows
+    through .tbl -> .colbin ingest)."""
+    import tempfile
+
+    from gpudbmetalcodegen_amd import tpch as TP
+    from gpudbmetalcodegen_amd import tpch_gen as TG
+    from oracle import tpch as OT
+
+    exact = {}
+    with tempfile.TemporaryDirectory() as d:
+        TG.write_tbl(TG.generate(check_sf, seed=7), d, TP.table_columns)
+        TP.ingest(d)
+        small = TP.TpchDB(ctx, TP.load_dir(d))
+        for q in TP.QUERIES:
+            prm = {"q18_qty": 25000} if q == "Q18" else None
+            r = TP.run(ctx, small, q, params=prm)
+            exp, fm = OT.run(small.host, q, dict(TP.PARAMS[q], **(prm or {})))
+            exact[q] = r.columns == exp and r.formats == fm
+        small.free()
+    log(f"TPC-H: oracle parity at SF {check_sf}: {exact}")
+    log(f"TPC-H: generating SF {sf}")
+    db = TP.TpchDB(ctx, TG.generate(sf, seed=1))
+    out = {"sf": sf, "data": "generated TPC-H-shaped tables (seeded; TPC-H domains, not dbgen rows)",
+           "bit_exact_at_sf": {"sf": check_sf, "queries": exact}, "queries": {}}
+    total = 0.0
+    for q in TP.QUERIES:
+        r = TP.run(ctx, db, q, warmup=warmup, measured=steps)
+        ms = r.timing.total_ms
+        rows = sum(int(db.device(t).rows) for t in TP.SCANNED[q])
+        e = {"ms": round(ms, 4), "rows_scanned": rows, "rows_per_s": rows / ms * 1e3,
+             "phase_ms": {n: round(m, 4) for n, m, *_ in r.timing.phases}, "result_rows": len(next(iter(r.columns.values())))}
+        if sf == 1.0 and q in M1_SF1_MS:
+            e["m1_sf1_ms"] = M1_SF1_MS[q]
+            e["speedup_vs_m1"] = round(M1_SF1_MS[q] / ms, 1)
+        out["queries"][q] = e
+        total += ms
+        r.executor.release()
+        log(f"TPC-H {q} SF {sf}: {ms:.3f} ms")
+    out["total_ms"] = round(total, 3)
+    db.free()
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -432,7 +486,7 @@ def main() -> int:
     ap.add_argument("--skip-extra", action="store_true", help="headline workload only")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--only", default="", help="comma-separated extra pipelines to run (w0_6M,w0_120M,w0_120M_host,w2_groupby,"
-                    "w3_batched,w4_served); default all")
+                    "w3_batched,w4_served,tpch_sf1); default all")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU; --gpus {args.gpus} is not supported (multi-GPU sharding is a design "
@@ -452,7 +506,8 @@ def main() -> int:
                          ("w0_120M_host", lambda: bench_q1_host(ctx, 120_000_000, args.steps)),
                          ("w2_groupby", lambda: bench_groupby(ctx, max(3, args.steps // 4), 1, not args.no_cpu)),
                          ("w3_batched", lambda: bench_batch(ctx, args.steps)),
-                         ("w4_served", lambda: bench_serve(ctx))]:
+                         ("w4_served", lambda: bench_serve(ctx)),
+                         ("tpch_sf1", lambda: bench_tpch(ctx, 1.0, max(5, args.steps // 4), 3))]:
             if args.only and name not in args.only.split(","):
                 continue
             try:

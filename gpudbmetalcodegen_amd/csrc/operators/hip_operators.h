@@ -235,6 +235,36 @@ private:
     std::string bitmap_, key_, bits_, var_;
 };
 
+// Key-indexed byte flags: flags[key] = 1 with plain byte stores (no
+// read-modify-write; every writer stores the same value, so the races are
+// benign). For a small key domain hit by many rows from every block (e.g. the
+// customers that have orders, ~10 orders per customer), where a bitmap's
+// atomic ORs would serialise on a few thousand words. Keys outside
+// [0, size) are skipped; the flags are zeroed before the phase.
+class HipByteFlagBuild : public HipUnaryOperator {
+public:
+    HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,
+                     const std::string& sizeExpr);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string flags_, key_, size_;
+};
+
+// Semi-/anti-join probe of byte flags built by HipByteFlagBuild.
+class HipByteFlagProbe : public HipUnaryOperator {
+public:
+    HipByteFlagProbe(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,
+                     const std::string& sizeExpr, bool anti);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string flags_, key_, size_;
+    bool anti_;
+};
+
 // Values an earlier phase left in a buffer, read once at kernel start:
 // const i64 <name> = (i64)buffer[word]. Carries a scalar subquery's result
 // (TPC-H Q22's average balance as sum and count) to a later phase without a

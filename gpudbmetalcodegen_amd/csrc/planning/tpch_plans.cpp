@@ -130,7 +130,8 @@ HipQueryPlan tpchQ4(const PlanOptions& o) {
         auto scan = makeScan("lineitem", {"l_orderkey", "l_commitdate", "l_receiptdate"});
         auto sel = std::make_unique<HipSelection>(std::move(scan), "l_commitdate < l_receiptdate");
         auto bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q4_late", "l_orderkey", "o_bits");
-        appendPhase(plan, "Q4_late_orders", std::move(bm), 256, optInt(o, "items", 4));
+        auto& ph = appendPhase(plan, "Q4_late_orders", std::move(bm), 256, optInt(o, "items", 8), {4, 8});
+        ph.staged = optInt(o, "staged", 0) != 0;
     }
     {
         auto scan = makeScan("orders", {"o_orderkey", "o_orderdate", "o_orderpriority"});
@@ -140,7 +141,7 @@ HipQueryPlan tpchQ4(const PlanOptions& o) {
         auto agg = std::make_unique<HipKeyedAgg>(std::move(probe), "d_q4_counts", "(int)o_orderpriority - '1'", 5);
         agg->addAggregate("order_count", AggKind::Count, "1", 0);
         agg->setBucketLabels({"1-URGENT", "2-HIGH", "3-MEDIUM", "4-NOT SPECIFIED", "5-LOW"}, "o_orderpriority");
-        auto& ph = appendPhase(plan, "Q4_count", std::move(agg), 256, optInt(o, "items", 4));
+        auto& ph = appendPhase(plan, "Q4_count", std::move(agg), 256, optInt(o, "items", 8), {4, 8});
         ph.scalarParams = {"q4_date_lo", "q4_date_hi"};
     }
     plan.requiredScalars = {"q4_date_lo", "q4_date_hi"};
@@ -276,7 +277,7 @@ HipQueryPlan tpchQ18(const PlanOptions& o) {
         auto scan = makeScan("lineitem", {"l_orderkey", "l_quantity"});
         auto add = std::make_unique<HipDenseAtomicAdd>(std::move(scan), "d_q18_qty", "l_orderkey", "o_bits",
                                                        "l_quantity");
-        appendPhase(plan, "Q18_order_qty", std::move(add), 256, optInt(o, "items", 4));
+        appendPhase(plan, "Q18_order_qty", std::move(add), 256, optInt(o, "items", 4), {4, 8});
     }
     {
         auto scan = makeScan("orders", {"o_orderkey", "o_custkey", "o_orderdate", "o_totalprice"});
@@ -327,7 +328,9 @@ HipQueryPlan tpchQ22(const PlanOptions& o) {
     }
     {
         auto scan = makeScan("orders", {"o_custkey"});
-        auto bm = std::make_unique<HipBitmapBuild>(std::move(scan), "d_q22_has_orders", "o_custkey", "c_bits");
+        // o_custkey repeats (about 10 orders per customer) from every block
+        // over a small key domain: byte flags written with plain stores
+        auto bm = std::make_unique<HipByteFlagBuild>(std::move(scan), "d_q22_has_orders", "o_custkey", "c_bits");
         appendPhase(plan, "Q22_orders_bitmap", std::move(bm), 256, 4);
     }
     {
@@ -336,7 +339,7 @@ HipQueryPlan tpchQ22(const PlanOptions& o) {
         auto avg = std::make_unique<HipBufferScalars>(std::move(scan), "d_q22_avg", "16", vars);
         auto cc = std::make_unique<HipComputeExpr>(std::move(avg), "_cc", "i64", code);
         auto sel = std::make_unique<HipSelection>(std::move(cc), inSet + " && c_acctbal * _avg_cnt > _avg_sum");
-        auto anti = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q22_has_orders", "c_custkey", "c_bits", true);
+        auto anti = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q22_has_orders", "c_custkey", "c_bits", true);
         auto agg = std::make_unique<HipKeyedAgg>(std::move(anti), "d_q22_out", "(int)_cc", 100);
         agg->addAggregate("numcust", AggKind::Count, "1", 0);
         agg->addAggregate("totacctbal", AggKind::SumI64, "c_acctbal", 2);

@@ -72,6 +72,15 @@ def _pick(rng, choices, n):
     return [choices[i] for i in rng.integers(0, len(choices), n)]
 
 
+def _pick_fixed(rng, choices, n, width):
+    """uint8 (n, width): uniformly chosen strings of `choices`."""
+    return fixed(choices, width)[rng.integers(0, len(choices), n)]
+
+
+def _pick_char1(rng, choices, n):
+    return char1(choices)[rng.integers(0, len(choices), n)]
+
+
 def _comments(rng, n, width, special_frac=0.0) -> np.ndarray:
     """uint8 (n, width) comment text: 3-8 random words; a special_frac share of
     rows gets "special <word> requests" inserted. Above 100,000 rows the
@@ -160,7 +169,7 @@ def generate(sf: float, seed: int = 1) -> dict:
         "c_nationkey": c_nat,
         "c_phone": fixed(_phones(rng, c_nat), 15),
         "c_acctbal": rng.integers(-99999, 1000000, n_cust).astype(np.int64),
-        "c_mktsegment": char1(_pick(rng, SEGMENTS, n_cust)),
+        "c_mktsegment": _pick_char1(rng, SEGMENTS, n_cust),
         "c_comment": _comments(rng, n_cust, 117),
     }
 
@@ -195,8 +204,8 @@ def generate(sf: float, seed: int = 1) -> dict:
         "l_quantity": qty * 100, "l_extendedprice": price, "l_discount": disc, "l_tax": tax,
         "l_returnflag": rflag.astype(np.uint8), "l_linestatus": lstatus.astype(np.uint8),
         "l_shipdate": ship, "l_commitdate": commit, "l_receiptdate": receipt,
-        "l_shipinstruct": fixed(_pick(rng, INSTRUCTIONS, n_line), 25),
-        "l_shipmode": fixed(_pick(rng, SHIPMODES, n_line), 2),
+        "l_shipinstruct": _pick_fixed(rng, INSTRUCTIONS, n_line, 25),
+        "l_shipmode": _pick_fixed(rng, SHIPMODES, n_line, 2),
         "l_comment": _comments(rng, n_line, 44),
     }
 
@@ -207,8 +216,8 @@ def generate(sf: float, seed: int = 1) -> dict:
     status = np.where(n_f == nl, ord("F"), np.where(n_f == 0, ord("O"), ord("P"))).astype(np.uint8)
     orders = {
         "o_orderkey": okey, "o_custkey": ocust, "o_orderstatus": status, "o_totalprice": total,
-        "o_orderdate": odate, "o_orderpriority": char1(_pick(rng, PRIORITIES, n_ord)),
-        "o_clerk": fixed([f"Clerk#{c:09d}" for c in rng.integers(1, max(2, int(1000 * sf)) + 1, n_ord)], 15),
+        "o_orderdate": odate, "o_orderpriority": _pick_char1(rng, PRIORITIES, n_ord),
+        "o_clerk": _pick_fixed(rng, [f"Clerk#{c:09d}" for c in range(1, max(2, int(1000 * sf)) + 1)], n_ord, 15),
         "o_shippriority": np.zeros(n_ord, dtype=np.int32),
         "o_comment": _comments(rng, n_ord, 79, special_frac=0.02),
     }
