@@ -638,32 +638,39 @@ std::string HipBitmapFlag::describe() const {
 }
 
 HipByteFlagBuild::HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std::string& flagsName,
-                                   const std::string& keyExpr, const std::string& sizeExpr)
-    : HipUnaryOperator(std::move(child)), flags_(flagsName), key_(keyExpr), size_(sizeExpr) {}
+                                   const std::string& keyExpr, const std::string& sizeExpr,
+                                   const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)), flags_(flagsName), key_(keyExpr), size_(sizeExpr), value_(valueExpr) {}
 
 void HipByteFlagBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(flags_, "u8", 1, size_, InitKind::Zero);
     cg.addSymbolParam("nflags_" + flags_, size_);
     child_->produce(cg, [&]() {
         cg.addLine("{ const u64 _fk = (u64)(i64)(" + key_ + "); if (_fk < nflags_" + flags_ + ") " + flags_ +
-                   "[_fk] = (u8)1; }");
+                   "[_fk] = (u8)(" + value_ + "); }");
         consume();
     });
 }
 
-std::string HipByteFlagBuild::describe() const { return "ByteFlagBuild(" + flags_ + "[" + key_ + "] = 1)"; }
+std::string HipByteFlagBuild::describe() const {
+    return "ByteFlagBuild(" + flags_ + "[" + key_ + "] = " + value_ + ")";
+}
 
 HipByteFlagProbe::HipByteFlagProbe(std::unique_ptr<HipOperator> child, const std::string& flagsName,
-                                   const std::string& keyExpr, const std::string& sizeExpr, bool anti)
-    : HipUnaryOperator(std::move(child)), flags_(flagsName), key_(keyExpr), size_(sizeExpr), anti_(anti) {}
+                                   const std::string& keyExpr, const std::string& sizeExpr, bool anti,
+                                   const std::string& var)
+    : HipUnaryOperator(std::move(child)), flags_(flagsName), key_(keyExpr), size_(sizeExpr), var_(var), anti_(anti) {
+    if (anti_ && !var_.empty()) throw std::runtime_error("HipByteFlagProbe: an anti probe defines no value");
+}
 
 void HipByteFlagProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(flags_, "u8", 1, size_, InitKind::None, {}, true, true);
     cg.addSymbolParam("nflags_" + flags_, size_);
     child_->produce(cg, [&]() {
         const std::string k = "(u64)(i64)(" + key_ + ")";
-        const std::string hit = "(" + k + " < nflags_" + flags_ + " && " + flags_ + "[" + k + "] != 0)";
-        cg.addIf(anti_ ? "!" + hit : hit, [&]() { consume(); });
+        const std::string v = var_.empty() ? cg.uniqueName("_fv") : var_;
+        cg.addLine("const u32 " + v + " = " + k + " < nflags_" + flags_ + " ? (u32)" + flags_ + "[" + k + "] : 0u;");
+        cg.addIf(anti_ ? v + " == 0u" : v + " != 0u", [&]() { consume(); });
     });
 }
 

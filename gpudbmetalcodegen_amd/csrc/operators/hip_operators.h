@@ -241,27 +241,33 @@ private:
 // customers that have orders, ~10 orders per customer), where a bitmap's
 // atomic ORs would serialise on a few thousand words. Keys outside
 // [0, size) are skipped; the flags are zeroed before the phase.
+// valueExpr (default 1, must be 1..255 for a present key and the same for
+// every row of a key) makes the flags a direct-address byte payload (e.g. an
+// order's priority class), as the reference's ArrayStore
+// (metal_operators.cpp:185-200) with byte values.
 class HipByteFlagBuild : public HipUnaryOperator {
 public:
     HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,
-                     const std::string& sizeExpr);
+                     const std::string& sizeExpr, const std::string& valueExpr = "1");
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
-    std::string flags_, key_, size_;
+    std::string flags_, key_, size_, value_;
 };
 
-// Semi-/anti-join probe of byte flags built by HipByteFlagBuild.
+// Semi-/anti-join probe of byte flags built by HipByteFlagBuild: rows whose
+// key has a non-zero flag (anti: a zero flag) are consumed; with `var` set,
+// const u32 <var> holds the flag's value (semi-join form only).
 class HipByteFlagProbe : public HipUnaryOperator {
 public:
     HipByteFlagProbe(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,
-                     const std::string& sizeExpr, bool anti);
+                     const std::string& sizeExpr, bool anti, const std::string& var = "");
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
-    std::string flags_, key_, size_;
+    std::string flags_, key_, size_, var_;
     bool anti_;
 };
 

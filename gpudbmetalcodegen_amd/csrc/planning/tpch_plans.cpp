@@ -120,16 +120,20 @@ HipQueryPlan tpchQ3(const PlanOptions& o) {
     return plan;
 }
 
-// Q4 — order priority checking (sql/q4.sql): bitmap of order keys with a
-// late lineitem (commit < receipt), then orders of the quarter probing it,
-// counted per priority ('1'..'5').
+// Q4 — order priority checking (sql/q4.sql): flags of the order keys with a
+// late lineitem (commit < receipt; byte flags, or a bitmap with bytes=0),
+// then orders of the quarter probing them, counted per priority ('1'..'5').
 HipQueryPlan tpchQ4(const PlanOptions& o) {
     HipQueryPlan plan;
     plan.name = "Q4";
     {
         auto scan = makeScan("lineitem", {"l_orderkey", "l_commitdate", "l_receiptdate"});
         auto sel = std::make_unique<HipSelection>(std::move(scan), "l_commitdate < l_receiptdate");
-        auto bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q4_late", "l_orderkey", "o_bits");
+        std::unique_ptr<HipOperator> bm;
+        if (optInt(o, "bytes", 1))
+            bm = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q4_late", "l_orderkey", "o_bits");
+        else
+            bm = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q4_late", "l_orderkey", "o_bits");
         auto& ph = appendPhase(plan, "Q4_late_orders", std::move(bm), 256, optInt(o, "items", 8), {4, 8});
         ph.staged = optInt(o, "staged", 0) != 0;
     }
@@ -137,7 +141,11 @@ HipQueryPlan tpchQ4(const PlanOptions& o) {
         auto scan = makeScan("orders", {"o_orderkey", "o_orderdate", "o_orderpriority"});
         auto sel = std::make_unique<HipSelection>(std::move(scan),
                                                   "(i64)o_orderdate >= q4_date_lo && (i64)o_orderdate < q4_date_hi");
-        auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q4_late", "o_orderkey", "o_bits");
+        std::unique_ptr<HipOperator> probe;
+        if (optInt(o, "bytes", 1))
+            probe = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q4_late", "o_orderkey", "o_bits", false);
+        else
+            probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q4_late", "o_orderkey", "o_bits");
         auto agg = std::make_unique<HipKeyedAgg>(std::move(probe), "d_q4_counts", "(int)o_orderpriority - '1'", 5);
         agg->addAggregate("order_count", AggKind::Count, "1", 0);
         agg->setBucketLabels({"1-URGENT", "2-HIGH", "3-MEDIUM", "4-NOT SPECIFIED", "5-LOW"}, "o_orderpriority");
@@ -150,8 +158,8 @@ HipQueryPlan tpchQ4(const PlanOptions& o) {
     return plan;
 }
 
-// Q12 — shipping modes and order priority (sql/q12.sql): bitmaps of all
-// order keys and of high-priority ('1-URGENT', '2-HIGH') order keys, then the
+// Q12 — shipping modes and order priority (sql/q12.sql): a byte per order key
+// with the order's priority class (high = '1-URGENT' or '2-HIGH'), then the
 // qualifying lineitems (ship mode code in {q12_mode1, q12_mode2}, the three
 // date conditions) joined to orders and counted per mode and priority class.
 // A mode code is the first two bytes of l_shipmode (char[2] in the catalog).
@@ -159,11 +167,12 @@ HipQueryPlan tpchQ12(const PlanOptions& o) {
     HipQueryPlan plan;
     plan.name = "Q12";
     {
+        // one byte per order key: 0 no order, 1 other priority, 2 high priority
         auto scan = makeScan("orders", {"o_orderkey", "o_orderpriority"});
-        auto any = std::make_unique<HipBitmapBuild>(std::move(scan), "d_q12_orders", "o_orderkey", "o_bits");
-        auto sel = std::make_unique<HipSelection>(std::move(any), "o_orderpriority == '1' || o_orderpriority == '2'");
-        auto high = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q12_high", "o_orderkey", "o_bits");
-        appendPhase(plan, "Q12_orders", std::move(high), 256, 4);
+        auto cls = std::make_unique<HipByteFlagBuild>(
+            std::move(scan), "d_q12_orders", "o_orderkey", "o_bits",
+            "(o_orderpriority == '1' || o_orderpriority == '2') ? 2 : 1");
+        appendPhase(plan, "Q12_orders", std::move(cls), 256, optInt(o, "items", 8), {4, 8});
     }
     {
         auto scan = makeScan("lineitem", {"l_orderkey", "l_shipmode", "l_shipdate", "l_commitdate", "l_receiptdate"});
@@ -173,11 +182,11 @@ HipQueryPlan tpchQ12(const PlanOptions& o) {
             std::move(mc),
             "(_mc == q12_mode1 || _mc == q12_mode2) && l_commitdate < l_receiptdate && l_shipdate < l_commitdate && "
             "(i64)l_receiptdate >= q12_date_lo && (i64)l_receiptdate < q12_date_hi");
-        auto join = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q12_orders", "l_orderkey", "o_bits");
-        auto flag = std::make_unique<HipBitmapFlag>(std::move(join), "d_q12_high", "l_orderkey", "o_bits", "_high");
-        auto agg = std::make_unique<HipKeyedAgg>(std::move(flag), "d_q12_counts", "(_mc == q12_mode1 ? 0 : 1)", 2);
-        agg->addAggregate("high_line_count", AggKind::SumI64, "(i64)_high", 0);
-        agg->addAggregate("low_line_count", AggKind::SumI64, "1 - (i64)_high", 0);
+        auto join = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q12_orders", "l_orderkey", "o_bits", false,
+                                                       "_pcls");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(join), "d_q12_counts", "(_mc == q12_mode1 ? 0 : 1)", 2);
+        agg->addAggregate("high_line_count", AggKind::SumI64, "(i64)(_pcls == 2u)", 0);
+        agg->addAggregate("low_line_count", AggKind::SumI64, "(i64)(_pcls != 2u)", 0);
         agg->setBucketLabels({opt(o, "mode1", "MAIL"), opt(o, "mode2", "SHIP")}, "l_shipmode");
         auto& ph = appendPhase(plan, "Q12_count", std::move(agg), 256, 4);
         ph.launchBound = 256;
@@ -226,20 +235,20 @@ HipQueryPlan tpchQ13(const PlanOptions& o) {
     return plan;
 }
 
-// Q14 — promotion effect (sql/q14.sql): bitmaps of all part keys and of
-// parts whose p_type starts with the registered prefix (d_like_words,
-// like_l1 bytes), then the month's lineitems joined to part:
+// Q14 — promotion effect (sql/q14.sql): a byte per part key telling whether
+// p_type starts with the registered prefix (d_like_words, like_l1 bytes),
+// then the month's lineitems joined to part:
 // promo_revenue = 100 * sum(promo revenue) / sum(revenue), exact to 6 digits.
 HipQueryPlan tpchQ14(const PlanOptions& o) {
     HipQueryPlan plan;
     plan.name = "Q14";
     {
+        // one byte per part key: 0 no part, 1 other type, 2 promotional type
         auto scan = makeScan("part", {"p_partkey", "p_type"});
-        auto any = std::make_unique<HipBitmapBuild>(std::move(scan), "d_q14_parts", "p_partkey", "p_bits");
-        auto sel = std::make_unique<HipSelection>(std::move(any),
-                                                  "gdb_like_prefix(p_type, 25, (const char*)d_like_words, (int)like_l1)");
-        auto promo = std::make_unique<HipBitmapBuild>(std::move(sel), "d_q14_promo", "p_partkey", "p_bits");
-        auto& ph = appendPhase(plan, "Q14_parts", std::move(promo), 256, 4);
+        auto cls = std::make_unique<HipByteFlagBuild>(
+            std::move(scan), "d_q14_parts", "p_partkey", "p_bits",
+            "gdb_like_prefix(p_type, 25, (const char*)d_like_words, (int)like_l1) ? 2 : 1");
+        auto& ph = appendPhase(plan, "Q14_parts", std::move(cls), 256, 4);
         ph.launchBound = 256;
         ph.scalarParams = {"like_l1"};
         ph.externalBuffers = {{"d_like_words", "u8"}};
@@ -248,10 +257,11 @@ HipQueryPlan tpchQ14(const PlanOptions& o) {
         auto scan = makeScan("lineitem", {"l_partkey", "l_extendedprice", "l_discount", "l_shipdate"});
         auto sel = std::make_unique<HipSelection>(std::move(scan),
                                                   "(i64)l_shipdate >= q14_date_lo && (i64)l_shipdate < q14_date_hi");
-        auto join = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q14_parts", "l_partkey", "p_bits");
-        auto flag = std::make_unique<HipBitmapFlag>(std::move(join), "d_q14_promo", "l_partkey", "p_bits", "_promo");
-        auto agg = std::make_unique<HipKeyedAgg>(std::move(flag), "d_q14_sums", "0", 1);
-        agg->addAggregate("promo_revenue_x100", AggKind::SumI64, std::string(kRevenue) + " * 100 * (i64)_promo", 4);
+        auto join = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q14_parts", "l_partkey", "p_bits", false,
+                                                       "_pcls");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(join), "d_q14_sums", "0", 1);
+        agg->addAggregate("promo_revenue_x100", AggKind::SumI64, std::string(kRevenue) + " * 100 * (i64)(_pcls == 2u)",
+                          4);
         agg->addAggregate("revenue", AggKind::SumI64, kRevenue, 4);
         agg->addAggregate("count", AggKind::Count, "1", 0);
         agg->addAverage("promo_revenue", "promo_revenue_x100", "revenue");
