@@ -257,19 +257,42 @@ void HipHashTableScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(table_, "u64", 8, slotsExpr_ + " * " + s(slotWords_), InitKind::None, {}, true, true);
     const std::string n = "nslots_" + table_;
     cg.addSymbolParam(n, slotsExpr_);
+    const std::string W = s(slotWords_);
+    const int vecs = slotWords_ / 2;  // 16-byte loads per slot (even slot widths)
     cg.addComment("scan the occupied slots of hash table " + table_);
     cg.addBlock("", [&]() {
         cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
         cg.addBlock("for (u64 _t0 = (u64)blockIdx.x * _tile; _t0 < " + n + "; _t0 += (u64)gridDim.x * _tile)", [&]() {
+            // Lane-contiguous slots (consecutive lanes read consecutive
+            // slots); all ITEMS slots of a thread are loaded before any is
+            // examined, so ITEMS 16-byte loads per lane are in flight at once.
+            if (slotWords_ % 2 == 0) {
+                cg.addLine("u64x2 _sv[ITEMS][" + s(vecs) + "];");
+                cg.addLine("#pragma unroll");
+                cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
+                    cg.addLine("const u64 _slot = _t0 + (u64)_u * blockDim.x + (u64)threadIdx.x;");
+                    cg.addLine("#pragma unroll");
+                    cg.addLine("for (int _v = 0; _v < " + s(vecs) + "; ++_v) _sv[_u][_v] = _slot < " + n +
+                               " ? __builtin_nontemporal_load(reinterpret_cast<const u64x2*>(" + table_ + " + _slot * " + W +
+                               ") + _v) : u64x2{(u64)GDB_EMPTY_KEY, 0ull};");
+                });
+            }
             cg.addLine("#pragma unroll");
             cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
-                cg.addLine("const u64 _slot = _t0 + (u64)threadIdx.x * ITEMS + (u64)_u;");
+                cg.addLine("const u64 _slot = _t0 + (u64)_u * blockDim.x + (u64)threadIdx.x;");
                 cg.addBlock("if (_slot < " + n + ")", [&]() {
-                    cg.addLine("const u64* _sw = " + table_ + " + _slot * " + s(slotWords_) + ";");
-                    // whole slot in one go (16-byte loads), read once: nontemporal
-                    cg.addLine("const u64 _w0 = __builtin_nontemporal_load(_sw);");
-                    for (int w = 1; w < slotWords_; ++w)
-                        cg.addLine("const u64 " + prefix_ + "_w" + s(w) + " = __builtin_nontemporal_load(_sw + " + s(w) + ");");
+                    if (slotWords_ % 2 == 0) {
+                        cg.addLine("const u64 _w0 = _sv[_u][0].x;");
+                        for (int w = 1; w < slotWords_; ++w)
+                            cg.addLine("const u64 " + prefix_ + "_w" + s(w) + " = _sv[_u][" + s(w / 2) + "]." +
+                                       (w % 2 ? "y" : "x") + ";");
+                    } else {
+                        cg.addLine("const u64* _sw = " + table_ + " + _slot * " + W + ";");
+                        cg.addLine("const u64 _w0 = __builtin_nontemporal_load(_sw);");
+                        for (int w = 1; w < slotWords_; ++w)
+                            cg.addLine("const u64 " + prefix_ + "_w" + s(w) + " = __builtin_nontemporal_load(_sw + " + s(w) +
+                                       ");");
+                    }
                     cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
                         cg.addLine("const u32 " + prefix_ + "_key = (u32)_w0;");
                         cg.addLine("const u32 " + prefix_ + "_payload = (u32)(_w0 >> 32);");

@@ -212,10 +212,14 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         top->addColumn("revenue", 0, 0, 64, 0);
         top->addColumn("b_date", 1, 32, 32, 0x7FFFFFFFll);
         auto merge = top->makeMerge();
-        auto& ph = appendPhase(plan, "Q3_topk", std::move(top), 256, 4);
+        // ITEMS slots per thread in flight (the grid is capped by the
+        // candidate lists, so per-thread depth carries the bandwidth)
+        auto& ph = appendPhase(plan, "Q3_topk", std::move(top), 256, optInt(o, "items_topk", 16), {4, 8, 16});
         ph.launchBound = 256;
-        auto& pm = appendPhase(plan, "Q3_topk_merge", std::move(merge), 256, 1, {1});
-        pm.launchBound = 256;
+        // one block merges every candidate list: 1024 threads keep each
+        // thread's serial insertion chain short
+        auto& pm = appendPhase(plan, "Q3_topk_merge", std::move(merge), optInt(o, "merge_block", 1024), 1, {1});
+        pm.launchBound = 1024;
     }
     plan.requiredScalars = {"q3_seg", "q3_date"};
     plan.requiredSymbols = {"a_bits", "b_bits"};

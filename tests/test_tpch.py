@@ -334,3 +334,10 @@ def test_oracle_hand_cases():
     C = {"c_custkey": np.array([1, 2, 3], np.int32)}
     cols, _ = OT.q13({"orders": O, "customer": C}, b"special", b"requests")
     assert cols == {"c_count": [1, 0], "custdist": [2, 1]}
+
+
+def test_cli_usage_errors():
+    from gpudbmetalcodegen_amd import tpch_cli
+
+    assert tpch_cli.main(["q2"]) == 3  # not planned by this engine
+    assert tpch_cli.main(["bogus"]) == 3

@@ -72,3 +72,19 @@ def test_tpch_repeat_runs_reuse_executor(env):
     first = T.run(ctx, db, "Q12", executor=ex)
     again = T.run(ctx, db, "Q12", executor=ex, warmup=1, measured=3)
     assert first.columns == again.columns
+
+
+def test_tpch_cli_check_exit_codes(env, tmp_path, capsys):
+    from gpudbmetalcodegen_amd import tpch_cli
+
+    ctx, db, host, _ = env
+    exp, fm = OT.run(host, "Q6", T.PARAMS["Q6"])
+    T.save_golden(OT.canonical(exp, fm), tmp_path, "Q6", "GEN")
+    d = tmp_path / "data"
+    G.write_tbl(G.generate(SF, seed=5), d, T.table_columns)
+    base = ["--data-dir", str(d), "--sf-label", "GEN", "--warmup", "1", "--repeat", "2"]
+    assert tpch_cli.main(base + ["--check", str(tmp_path), "q6"]) == 0
+    assert "[CHECK] Q6 @ GEN: OK" in capsys.readouterr().out
+    (tmp_path / "Q6_GEN.csv").write_text("revenue\n1.0\n")
+    assert tpch_cli.main(base + ["--check", str(tmp_path), "q6"]) == 1
+    assert tpch_cli.main(base + ["--check", str(tmp_path), "q4"]) == 2
