@@ -402,6 +402,52 @@ GDB_DEV bool gdb_like2(const char* s, int width, const char* w1, int l1, const c
     return false;
 }
 
+// Bit-parallel LIKE '%w1%w2%' (Shift-And automata, one per word; w2's runs
+// only from the byte after w1's first occurrence ends, which is the earliest
+// and so the only start that matters). t1/t2 are 256-entry tables in LDS: bit
+// j of t[c] is set when byte j of the word is c (gdb_like2_tables builds
+// them). Words of 1..31 bytes (0 = empty word, always found). The row is read
+// from LDS as aligned 32-bit words realigned with v_alignbyte, so every byte
+// index below is a compile-time constant; a NUL byte ends the string as in
+// gdb_like2. The row's LDS buffer must have 4 readable bytes past its end.
+GDB_DEV void gdb_like2_tables(u32* t1, u32* t2, const char* __restrict__ w1, int l1, const char* __restrict__ w2, int l2) {
+    for (int c = (int)threadIdx.x; c < 256; c += (int)blockDim.x) {
+        u32 m1 = 0, m2 = 0;
+        for (int j = 0; j < l1; ++j) m1 |= ((u32)(u8)w1[j] == (u32)c ? 1u : 0u) << j;
+        for (int j = 0; j < l2; ++j) m2 |= ((u32)(u8)w2[j] == (u32)c ? 1u : 0u) << j;
+        t1[c] = c ? m1 : 0u;
+        t2[c] = c ? m2 : 0u;
+    }
+}
+template <int W>
+GDB_DEV bool gdb_like2_bp(const char* s, const u32* t1, const u32* t2, int l1, int l2) {
+    constexpr int NW = (W + 3) / 4;
+    const u32 off = (u32)(reinterpret_cast<unsigned long long>(s) & 3ull);
+    const u32* wp = reinterpret_cast<const u32*>(s - off);
+    u32 raw[NW + 1];
+#pragma unroll
+    for (int k = 0; k <= NW; ++k) raw[k] = wp[k];
+    u32 a[NW];
+#pragma unroll
+    for (int k = 0; k < NW; ++k) a[k] = __builtin_amdgcn_alignbyte(raw[k + 1], raw[k], off);
+    const u32 top1 = l1 > 0 ? 1u << (l1 - 1) : 0u, top2 = l2 > 0 ? 1u << (l2 - 1) : 0u;
+    u32 d1 = 0, d2 = 0, alive = 1u;
+    u32 found = l1 == 0 ? 1u : 0u;
+    u32 hit = (l1 == 0 && l2 == 0) ? 1u : 0u;
+#pragma unroll
+    for (int j = 0; j < W; ++j) {
+        const u32 c = (a[j >> 2] >> (8 * (j & 3))) & 0xFFu;
+        alive &= c != 0u ? 1u : 0u;
+        const u32 before = found;
+        d1 = ((d1 << 1) | 1u) & t1[c];
+        found |= (d1 & top1) ? alive : 0u;
+        d2 = ((d2 << 1) | before) & t2[c];
+        hit |= (d2 & top2) ? alive : 0u;
+        if (l2 == 0) hit |= found;
+    }
+    return hit != 0u;
+}
+
 // LIKE 'p%' on a fixed-width NUL-padded field: the field starts with the l
 // bytes of p (a field shorter than l, i.e. NUL inside the prefix, fails).
 GDB_DEV bool gdb_like_prefix(const char* s, int width, const char* p, int l) {

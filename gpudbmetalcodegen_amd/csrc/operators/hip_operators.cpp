@@ -152,7 +152,8 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         for (size_t k = 0; k < columns_.size(); ++k) {
             const auto& c = columns_[k];
             if (c.type.type == ColType::CHAR) {
-                cg.declareLds("char", "_s_" + c.name, s(lb) + " * ITEMS * " + s(c.type.width));
+                // + 16: row readers may load the 4-byte word past a row's end
+                cg.declareLds("char", "_s_" + c.name, s(lb) + " * ITEMS * " + s(c.type.width) + " + 16");
             }
         }
         const std::string start = range ? "_rb" : "(u64)blockIdx.x * _tile";
@@ -638,6 +639,39 @@ void HipArrayLookup::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 }
 
 std::string HipArrayLookup::describe() const { return "ArrayLookup(" + var_ + " = " + array_ + "[" + key_ + "])"; }
+
+// ===========================================================================
+// HipLikeFilter
+// ===========================================================================
+
+HipLikeFilter::HipLikeFilter(std::unique_ptr<HipOperator> child, const std::string& column, int width,
+                             const std::string& wordsBuffer, const std::string& l1Scalar, const std::string& l2Scalar,
+                             bool negate)
+    : HipUnaryOperator(std::move(child)),
+      column_(column),
+      width_(width),
+      words_(wordsBuffer),
+      l1_(l1Scalar),
+      l2_(l2Scalar),
+      negate_(negate) {}
+
+void HipLikeFilter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string t1 = cg.uniqueName("_lk1"), t2 = cg.uniqueName("_lk2");
+    cg.declareLds("u32", t1, "256");
+    cg.declareLds("u32", t2, "256");
+    cg.addLine("gdb_like2_tables(" + t1 + ", " + t2 + ", (const char*)" + words_ + ", (int)" + l1_ + ", (const char*)" +
+               words_ + " + 32, (int)" + l2_ + ");");
+    cg.addLine("__syncthreads();");
+    child_->produce(cg, [&]() {
+        cg.addIf(std::string(negate_ ? "!" : "") + "gdb_like2_bp<" + s(width_) + ">(" + column_ + ", " + t1 + ", " + t2 +
+                     ", (int)" + l1_ + ", (int)" + l2_ + ")",
+                 [&]() { consume(); });
+    });
+}
+
+std::string HipLikeFilter::describe() const {
+    return std::string(negate_ ? "NotLike(" : "Like(") + column_ + ", '%w1%w2%')";
+}
 
 // ===========================================================================
 // HipBitmapFlag, HipBufferScalars, HipDenseAtomicAdd, HipAppendRows

@@ -221,6 +221,26 @@ private:
     long long sentinel_;
 };
 
+// LIKE '%w1%w2%' (NOT LIKE with negate) on a char[width] column staged in
+// LDS, byte-exact (q13_comment_match, metal_plan_builder.cpp:1249-1266, for
+// any two words): bit-parallel Shift-And automata over the row with per-block
+// LDS byte tables built from the words buffer (w1 at byte 0, w2 at byte 32;
+// lengths in two scalars). The phase must register the words buffer and the
+// length scalars.
+class HipLikeFilter : public HipUnaryOperator {
+public:
+    HipLikeFilter(std::unique_ptr<HipOperator> child, const std::string& column, int width,
+                  const std::string& wordsBuffer, const std::string& l1Scalar, const std::string& l2Scalar, bool negate);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string column_;
+    int width_;
+    std::string words_, l1_, l2_;
+    bool negate_;
+};
+
 // Semi-join membership as a value: const u32 <var> = bit `key` is set;
 // every row is consumed (CASE WHEN <key in build side> THEN ... expressions,
 // e.g. TPC-H Q12's priority class and Q14's promotional parts).

@@ -371,10 +371,8 @@ HipQueryPlan buildQ13L(const PlanOptions& o) {
     const int histBins = optInt(o, "hist_bins", 1024);
     {
         auto scan = makeScan("S", {"s_fkey", "s_text"});
-        std::string like =
-            "gdb_like2(s_text, 25, (const char*)d_like_words, (int)like_l1, (const char*)d_like_words + 32, "
-            "(int)like_l2)";
-        auto sel = std::make_unique<HipSelection>(std::move(scan), negate ? "!" + like : like);
+        auto sel = std::make_unique<HipLikeFilter>(std::move(scan), "s_text", 25, "d_like_words", "like_l1", "like_l2",
+                                                   negate);
         auto cnt = std::make_unique<HipAtomicCount>(std::move(sel), "d_q13_counts", "s_fkey", "a_bits", 0);
         auto& ph = appendPhase(plan, "Q13_count", std::move(cnt), 256, optInt(o, "items", 4), {4});
         ph.launchBound = 256;

@@ -214,10 +214,8 @@ HipQueryPlan tpchQ13(const PlanOptions& o) {
     const int histBins = optInt(o, "hist_bins", 1024);
     {
         auto scan = makeScan("orders", {"o_custkey", "o_comment"});
-        auto sel = std::make_unique<HipSelection>(
-            std::move(scan),
-            "!gdb_like2(o_comment, 79, (const char*)d_like_words, (int)like_l1, (const char*)d_like_words + 32, "
-            "(int)like_l2)");
+        auto sel = std::make_unique<HipLikeFilter>(std::move(scan), "o_comment", 79, "d_like_words", "like_l1", "like_l2",
+                                                   true);
         auto cnt = std::make_unique<HipAtomicCount>(std::move(sel), "d_q13_counts", "o_custkey", "c_bits", 0);
         auto& ph = appendPhase(plan, "Q13_count", std::move(cnt), 256, 2, {2});
         ph.launchBound = 256;

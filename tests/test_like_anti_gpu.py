@@ -38,7 +38,8 @@ def test_q13_like(ctx, n_s, n_a, negate):
 
 def test_q13_like_edge_strings(ctx):
     rows = [b"special requests", b"requests special", b"specialrequests", b"specialrequest", b"aaa",
-            b"xx\0ab cd", b"ab" + b" " * 21 + b"cd", b"", b"abab", b"ab cd ab cd ab cd ab cd a"]
+            b"xx\0ab cd", b"ab" + b" " * 21 + b"cd", b"", b"abab", b"ab cd ab cd ab cd ab cd a",
+            b"abcdefghijklmnopqrstuvwxy", b"cd ab", b"a" * 25]
     texts = np.zeros((len(rows), 25), np.uint8)
     for i, r in enumerate(rows):
         texts[i, :min(25, len(r))] = np.frombuffer(r[:25], np.uint8)
@@ -46,7 +47,9 @@ def test_q13_like_edge_strings(ctx):
     A = {"a_key": np.arange(1, len(rows) + 1, dtype=np.int32), "a_seg": np.zeros(len(rows), np.uint8)}
     dev = Q.upload_tables(ctx, {"S": S, "A": A})
     try:
-        for w1, w2 in [(b"special", b"requests"), (b"aa", b"a"), (b"ab", b"cd"), (b"ab", b"ab"), (b"", b"")]:
+        for w1, w2 in [(b"special", b"requests"), (b"aa", b"a"), (b"ab", b"cd"), (b"ab", b"ab"), (b"", b""),
+                       (b"", b"cd"), (b"ab", b""), (b"abcdefghijklmnopqrstuvwxy", b""), (b"a" * 24, b"a"),
+                       (b"a" * 25, b"a"), (b"y", b"z")]:
             run = Q.run_q13(ctx, dev, w1, w2)
             exp = oracle.q13(S, A, len(rows), w1, w2)
             got = {k: [int(x) for x in v] for k, v in run.result.columns.items()}
