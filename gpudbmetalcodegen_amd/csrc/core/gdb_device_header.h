@@ -457,6 +457,16 @@ GDB_DEV bool gdb_like_prefix(const char* s, int width, const char* p, int l) {
     return true;
 }
 
+// String equality of a fixed-width NUL-padded field with a constant held in
+// a 32-byte slot: slot[0] is the length (0..31), the bytes follow.
+GDB_DEV bool gdb_str_eq(const char* s, int width, const unsigned char* __restrict__ slot) {
+    const int l = (int)slot[0];
+    if (l > width) return false;
+    for (int i = 0; i < l; ++i)
+        if ((unsigned char)s[i] != slot[1 + i]) return false;
+    return l == width || s[l] == 0;
+}
+
 // Wave-aggregated append: every active lane gets a distinct index from the
 // counter; one atomic per wave (the first active lane's), the base is
 // broadcast and each lane adds the number of active lanes below it.

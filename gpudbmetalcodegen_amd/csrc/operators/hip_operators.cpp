@@ -713,6 +713,22 @@ std::string HipByteFlagBuild::describe() const {
     return "ByteFlagBuild(" + flags_ + "[" + key_ + "] = " + value_ + ")";
 }
 
+HipDenseStore::HipDenseStore(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                             const std::string& keyExpr, const std::string& sizeExpr, const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)), array_(arrayName), key_(keyExpr), size_(sizeExpr), value_(valueExpr) {}
+
+void HipDenseStore::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(array_, "u32", 4, size_, InitKind::Zero);
+    cg.addSymbolParam("nelem_" + array_, size_);
+    child_->produce(cg, [&]() {
+        cg.addLine("{ const u64 _sk = (u64)(i64)(" + key_ + "); if (_sk < nelem_" + array_ + ") " + array_ +
+                   "[_sk] = (u32)(" + value_ + "); }");
+        consume();
+    });
+}
+
+std::string HipDenseStore::describe() const { return "DenseStore(" + array_ + "[" + key_ + "] = " + value_ + ")"; }
+
 HipByteFlagProbe::HipByteFlagProbe(std::unique_ptr<HipOperator> child, const std::string& flagsName,
                                    const std::string& keyExpr, const std::string& sizeExpr, bool anti,
                                    const std::string& var)

@@ -276,6 +276,22 @@ private:
     std::string flags_, key_, size_, value_;
 };
 
+// Key-indexed u32 store array[key] = value (zeroed before its phase, so 0
+// reads as absent; store value + 1 for values that may be 0): the
+// reference's direct-address ArrayStore join map (metal_operators.cpp:185-200)
+// for keys that are unique on the build side (e.g. o_orderkey), read back
+// with HipArrayLookup. Keys outside [0, size) are skipped.
+class HipDenseStore : public HipUnaryOperator {
+public:
+    HipDenseStore(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& keyExpr,
+                  const std::string& sizeExpr, const std::string& valueExpr);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string array_, key_, size_, value_;
+};
+
 // Semi-/anti-join probe of byte flags built by HipByteFlagBuild: rows whose
 // key has a non-zero flag (anti: a zero flag) are consumed; with `var` set,
 // const u32 <var> holds the flag's value (semi-join form only).

@@ -367,11 +367,181 @@ HipQueryPlan tpchQ22(const PlanOptions& o) {
     return plan;
 }
 
+// Q5 — local supplier volume (sql/q5.sql): byte maps carry nation keys
+// (+1, 0 = absent) down the joins: region named like string constant 0 ->
+// its nations -> suppliers and customers of those nations -> orders of the
+// year (the customer's nation) -> lineitems whose order's and supplier's
+// nations agree, revenue per nation. The caller attaches n_name and orders by
+// revenue desc. String constants: registered buffer d_str_consts of 32-byte
+// slots (length byte, then the bytes).
+HipQueryPlan tpchQ5(const PlanOptions& o) {
+    (void)o;
+    HipQueryPlan plan;
+    plan.name = "Q5";
+    {
+        auto scan = makeScan("region", {"r_regionkey", "r_name"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "gdb_str_eq(r_name, 25, (const unsigned char*)d_str_consts)");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q5_region", "r_regionkey", "r_bits");
+        auto& ph = appendPhase(plan, "Q5_region", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("nation", {"n_nationkey", "n_regionkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q5_region", "n_regionkey", "r_bits", false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q5_nation", "n_nationkey", "n_bits",
+                                                     "n_nationkey + 1");
+        auto& ph = appendPhase(plan, "Q5_nations", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q5_nation", "s_nationkey", "n_bits", false,
+                                                     "_sn");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q5_supp", "s_suppkey", "s_bits", "_sn");
+        appendPhase(plan, "Q5_suppliers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScan("customer", {"c_custkey", "c_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q5_nation", "c_nationkey", "n_bits", false,
+                                                     "_cn");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q5_cust", "c_custkey", "c_bits", "_cn");
+        appendPhase(plan, "Q5_customers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_custkey", "o_orderdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)o_orderdate >= q5_date_lo && (i64)o_orderdate < q5_date_hi");
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q5_cust", "o_custkey", "c_bits", false, "_on");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q5_order", "o_orderkey", "o_bits", "_on");
+        auto& ph = appendPhase(plan, "Q5_orders", std::move(fl), 256, 4);
+        ph.scalarParams = {"q5_date_lo", "q5_date_hi"};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_suppkey", "l_extendedprice", "l_discount"});
+        auto ord = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q5_order", "l_orderkey", "o_bits", false,
+                                                      "_ln");
+        auto sup = std::make_unique<HipByteFlagProbe>(std::move(ord), "d_q5_supp", "l_suppkey", "s_bits", false, "_ls");
+        auto same = std::make_unique<HipSelection>(std::move(sup), "_ln == _ls");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(same), "d_q5_rev", "(int)_ln - 1", 256);
+        agg->addAggregate("revenue", AggKind::SumI64, kRevenue, 4);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        auto& ph = appendPhase(plan, "Q5_lineitem", std::move(agg), 256, optInt(o, "items", 4), {4, 8});
+        ph.launchBound = 256;
+    }
+    plan.requiredSymbols = {"r_bits", "n_bits", "s_bits", "c_bits", "o_bits"};
+    plan.requiredScalars = {"q5_date_lo", "q5_date_hi"};
+    return plan;
+}
+
+// Q10 — returned item reporting (sql/q10.sql): orders of the quarter store
+// their customer key (+1) in an order-key-indexed array; lineitems with
+// return flag R add their revenue to a customer-key-indexed array (runs of
+// one order's lines combine before the atomic); customers with revenue go to
+// a top-20 by (revenue desc, c_custkey asc). The caller attaches the
+// customer's name, balance, nation name, address, phone and comment.
+HipQueryPlan tpchQ10(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q10";
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_custkey", "o_orderdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)o_orderdate >= q10_date_lo && (i64)o_orderdate < q10_date_hi");
+        auto st = std::make_unique<HipDenseStore>(std::move(sel), "d_q10_ocust", "o_orderkey", "o_bits", "o_custkey + 1");
+        auto& ph = appendPhase(plan, "Q10_orders", std::move(st), 256, 4);
+        ph.scalarParams = {"q10_date_lo", "q10_date_hi"};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_returnflag", "l_extendedprice", "l_discount"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)l_returnflag == q10_flag");
+        auto look = std::make_unique<HipArrayLookup>(std::move(sel), "d_q10_ocust", "l_orderkey", "o_bits", "_ck1", 0ll);
+        auto add = std::make_unique<HipDenseAtomicAdd>(std::move(look), "d_q10_rev", "_ck1 - 1", "c_bits", kRevenue);
+        auto& ph = appendPhase(plan, "Q10_revenue", std::move(add), 256, optInt(o, "items", 4), {4, 8});
+        ph.scalarParams = {"q10_flag"};
+    }
+    {
+        auto scan = makeScan("customer", {"c_custkey"});
+        auto look = std::make_unique<HipArrayLookup>(std::move(scan), "d_q10_rev", "c_custkey", "c_bits", "_rev", 0ll,
+                                                     "i64");
+        auto top = std::make_unique<HipTopK>(std::move(look), "d_q10_top", 20, "(u64)_rev",
+                                             "(u64)(0xFFFFFFFFu - (u32)c_custkey)");
+        top->addColumn("c_custkey", 1, 0, 32, 0xFFFFFFFFll);
+        top->addColumn("revenue", 0, 0, 64, 0, 4);
+        auto merge = top->makeMerge();
+        auto& ph = appendPhase(plan, "Q10_topk", std::move(top), 256, 4);
+        ph.launchBound = 256;
+        auto& pm = appendPhase(plan, "Q10_topk_merge", std::move(merge), 1024, 1, {1});
+        pm.launchBound = 1024;
+    }
+    plan.requiredScalars = {"q10_date_lo", "q10_date_hi", "q10_flag"};
+    plan.requiredSymbols = {"o_bits", "c_bits"};
+    return plan;
+}
+
+// Q19 — discounted revenue (sql/q19.sql): a byte per part key with the
+// brand/container/size class the part satisfies (1, 2, 3 or 0), then the
+// lineitems shipped by AIR / REG AIR with DELIVER IN PERSON whose quantity
+// lies in their part's class range. String constants in d_str_consts slots:
+// 0-2 brands, 3-6 / 7-10 / 11-14 the classes' containers, 15 the ship
+// instruction; ship modes are 2-byte codes in scalars.
+HipQueryPlan tpchQ19(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q19";
+    auto slot = [](int k) { return "(const unsigned char*)d_str_consts + " + std::to_string(32 * k); };
+    auto anyOf = [&](const std::string& col, int width, int first) {
+        std::string e;
+        for (int k = first; k < first + 4; ++k)
+            e += std::string(e.empty() ? "" : " || ") + "gdb_str_eq(" + col + ", " + std::to_string(width) + ", " + slot(k) + ")";
+        return "(" + e + ")";
+    };
+    {
+        std::string cls;
+        const char* sizes[3] = {"5", "10", "15"};
+        for (int c = 0; c < 3; ++c) {
+            cls += "(gdb_str_eq(p_brand, 10, " + slot(c) + ") && " + anyOf("p_container", 10, 3 + 4 * c) +
+                   " && p_size >= 1 && p_size <= " + sizes[c] + ") ? " + std::to_string(c + 1) + " : ";
+        }
+        cls += "0";
+        auto scan = makeScan("part", {"p_partkey", "p_brand", "p_container", "p_size"});
+        auto cc = std::make_unique<HipComputeExpr>(std::move(scan), "_pcls", "u32", cls);
+        auto sel = std::make_unique<HipSelection>(std::move(cc), "_pcls != 0u");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q19_parts", "p_partkey", "p_bits", "_pcls");
+        auto& ph = appendPhase(plan, "Q19_parts", std::move(fl), 256, 4);
+        ph.launchBound = 256;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_quantity", "l_extendedprice", "l_discount", "l_shipmode",
+                                          "l_shipinstruct"});
+        auto mc = std::make_unique<HipComputeExpr>(std::move(scan), "_mc", "i64",
+                                                   "(i64)(u8)l_shipmode[0] | ((i64)(u8)l_shipmode[1] << 8)");
+        auto sel = std::make_unique<HipSelection>(
+            std::move(mc), "(_mc == q19_mode1 || _mc == q19_mode2) && gdb_str_eq(l_shipinstruct, 25, " + slot(15) + ")");
+        auto cls = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q19_parts", "l_partkey", "p_bits", false,
+                                                      "_pc");
+        auto qty = std::make_unique<HipSelection>(
+            std::move(cls),
+            "(_pc == 1u && l_quantity >= 100 && l_quantity <= 1100) || (_pc == 2u && l_quantity >= 1000 && "
+            "l_quantity <= 2000) || (_pc == 3u && l_quantity >= 2000 && l_quantity <= 3000)");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(qty), "d_q19_sum", "0", 1);
+        agg->addAggregate("revenue", AggKind::SumI64, kRevenue, 4);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        auto& ph = appendPhase(plan, "Q19_lineitem", std::move(agg), 256, optInt(o, "items", 4), {4});
+        ph.launchBound = 256;
+        ph.scalarParams = {"q19_mode1", "q19_mode2"};
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    plan.requiredScalars = {"q19_mode1", "q19_mode2"};
+    plan.requiredSymbols = {"p_bits"};
+    return plan;
+}
+
 bool tpchSchema(const PlanOptions& o) { return opt(o, "schema", "") == "tpch"; }
 
 }  // namespace
 
-std::vector<std::string> tpchPlanNames() { return {"Q4", "Q12", "Q13", "Q14", "Q18", "Q22"}; }
+std::vector<std::string> tpchPlanNames() { return {"Q4", "Q5", "Q10", "Q12", "Q13", "Q14", "Q18", "Q19", "Q22"}; }
 
 std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOptions& opts) {
     const std::string schema = opt(opts, "schema", "");
@@ -380,6 +550,9 @@ std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOpt
     if (name == "Q3" && tpchSchema(opts)) return tpchQ3(opts);
     if (name == "Q6" && tpchSchema(opts)) return tpchQ6(opts);
     if (name == "Q4") return tpchQ4(opts);
+    if (name == "Q5") return tpchQ5(opts);
+    if (name == "Q10") return tpchQ10(opts);
+    if (name == "Q19") return tpchQ19(opts);
     if (name == "Q12") return tpchQ12(opts);
     if (name == "Q13") return tpchQ13(opts);
     if (name == "Q14") return tpchQ14(opts);

@@ -31,7 +31,7 @@ from . import engine as E
 
 INT, FLOAT, DATE, CHAR1, CHAR_FIXED = 0, 1, 2, 3, 4
 TABLES = ["lineitem", "orders", "customer", "supplier", "part", "partsupp", "nation", "region"]
-QUERIES = ["Q1", "Q3", "Q4", "Q6", "Q12", "Q13", "Q14", "Q18", "Q22"]
+QUERIES = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q10", "Q12", "Q13", "Q14", "Q18", "Q19", "Q22"]
 CHECK_ABS_TOL = 1e-2  # the reference's --check defaults (codegen_main.cpp:47-48)
 CHECK_REL_TOL = 1e-4
 
@@ -176,6 +176,12 @@ PARAMS = {
     "Q13": {"words": (b"special", b"requests")},
     "Q14": {"prefix": b"PROMO", "q14_date_lo": ymd(1995, 9, 1), "q14_date_hi": shift(ymd(1995, 9, 1), months=1)},
     "Q18": {"q18_qty": 30000},
+    "Q5": {"strings": (b"ASIA",), "q5_date_lo": ymd(1994, 1, 1), "q5_date_hi": shift(ymd(1994, 1, 1), years=1)},
+    "Q10": {"q10_date_lo": ymd(1993, 10, 1), "q10_date_hi": shift(ymd(1993, 10, 1), months=3), "q10_flag": ord("R")},
+    "Q19": {"q19_mode1": mode_code("AIR"), "q19_mode2": mode_code("REG AIR"),
+            "strings": (b"Brand#12", b"Brand#23", b"Brand#34", b"SM CASE", b"SM BOX", b"SM PACK", b"SM PKG",
+                        b"MED BAG", b"MED BOX", b"MED PKG", b"MED PACK", b"LG CASE", b"LG BOX", b"LG PACK", b"LG PKG",
+                        b"DELIVER IN PERSON")},
     "Q22": {"q22_mask_lo": sum(1 << c for c in Q22_CODES if c < 64),
             "q22_mask_hi": sum(1 << (c - 64) for c in Q22_CODES if c >= 64)},
 }
@@ -183,7 +189,9 @@ PARAMS = {
 PLAN_OPTIONS = {"Q1": "schema=tpch", "Q3": "schema=tpch", "Q6": "schema=tpch"}
 SCANNED = {"Q1": ["lineitem"], "Q3": ["customer", "orders", "lineitem"], "Q4": ["lineitem", "orders"],
            "Q6": ["lineitem"], "Q12": ["orders", "lineitem"], "Q13": ["orders", "customer"],
-           "Q14": ["part", "lineitem"], "Q18": ["lineitem", "orders"], "Q22": ["customer", "orders"]}
+           "Q14": ["part", "lineitem"], "Q18": ["lineitem", "orders"], "Q22": ["customer", "orders"],
+           "Q5": ["region", "nation", "supplier", "customer", "orders", "lineitem"],
+           "Q10": ["orders", "lineitem", "customer"], "Q19": ["part", "lineitem"]}
 # Output columns in the SQL select list's order.
 SELECT = {
     "Q1": ["l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price", "sum_charge", "avg_qty",
@@ -196,6 +204,9 @@ SELECT = {
     "Q14": ["promo_revenue"],
     "Q18": ["c_name", "c_custkey", "o_orderkey", "o_orderdate", "o_totalprice", "sum(l_quantity)"],
     "Q22": ["cntrycode", "numcust", "totacctbal"],
+    "Q5": ["n_name", "revenue"],
+    "Q10": ["c_custkey", "c_name", "revenue", "c_acctbal", "n_name", "c_address", "c_phone", "c_comment"],
+    "Q19": ["revenue"],
 }
 
 
@@ -230,6 +241,14 @@ class TpchDB:
             self.symbols["p_bits"] = max(self.symbols.get("p_bits", 1), int(h["lineitem"]["l_partkey"].max(initial=0)) + 1)
         if "orders" in h:
             self.symbols["c_bits"] = max(self.symbols.get("c_bits", 1), int(h["orders"]["o_custkey"].max(initial=0)) + 1)
+        for t, key, sym in (("region", "r_regionkey", "r_bits"), ("nation", "n_nationkey", "n_bits"),
+                            ("supplier", "s_suppkey", "s_bits")):
+            if t in h:
+                self.symbols[sym] = int(h[t][key].max(initial=0)) + 1
+        if "nation" in h and "region" in h:
+            self.symbols["r_bits"] = max(self.symbols["r_bits"], int(h["nation"]["n_regionkey"].max(initial=0)) + 1)
+        if "lineitem" in h and "supplier" in h:
+            self.symbols["s_bits"] = max(self.symbols["s_bits"], int(h["lineitem"]["l_suppkey"].max(initial=0)) + 1)
 
     def device(self, table: str) -> E.DeviceTable:
         t = self.dev.get(table)
@@ -271,6 +290,7 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
         ex.register_table(db.device(t))
     prm = dict(PARAMS[query], **(params or {}))
     words_ptr = None
+    strings_ptr = None
     for k, v in prm.items():
         if k.startswith("q"):
             ex.register_scalar(k, int(v))
@@ -290,11 +310,17 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
             words_ptr = like_words_buffer(ctx, prm["prefix"], b"")
             ex.register_buffer("d_like_words", words_ptr, 64)
             ex.register_scalar("like_l1", len(prm["prefix"]))
+        if "strings" in prm:
+            buf = string_slots(prm["strings"])
+            strings_ptr = ctx.upload(buf)
+            ex.register_buffer("d_str_consts", strings_ptr, buf.nbytes)
         t = ex.execute(plan, warmup, measured)
         res = ex.collect(plan)
     finally:
         if words_ptr is not None:
             ctx.free(words_ptr)
+        if strings_ptr is not None:
+            ctx.free(strings_ptr)
     cols = {k: [int(x) if not isinstance(x, str) else x for x in v] for k, v in res.columns.items()}
     fmts = dict(res.formats)
     _finish(query, db, cols, fmts)
@@ -302,6 +328,21 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
     run_ = TpchRun(query, out, {k: fmts.get(k, (0, False)) for k in out}, t, plan, ex, res)
     run_.canonical = canonical(run_.columns, run_.formats)
     return run_
+
+
+def string_slots(strings) -> np.ndarray:
+    """32-byte slots: a length byte, then the string's bytes (at most 31)."""
+    buf = np.zeros(32 * max(1, len(strings)), np.uint8)
+    for k, w in enumerate(strings):
+        if len(w) > 31:
+            raise ValueError(f"string constant {w!r} is longer than 31 bytes")
+        buf[32 * k] = len(w)
+        buf[32 * k + 1:32 * k + 1 + len(w)] = np.frombuffer(w, np.uint8)
+    return buf
+
+
+def _text(arr_row) -> str:
+    return bytes(arr_row).split(b"\0", 1)[0].decode()
 
 
 def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
@@ -314,6 +355,25 @@ def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
         sp = {int(o["o_orderkey"][i]): int(o["o_shippriority"][i]) for i in sub}
         cols["o_shippriority"] = [sp[k] for k in cols["l_orderkey"]]
         fmts["o_shippriority"] = (0, False)
+    elif query == "Q5":
+        n = db.host["nation"]
+        names = {int(k): _text(nm) for k, nm in zip(n["n_nationkey"], n["n_name"])}
+        rows = sorted(zip(cols["bucket"], cols["revenue"]), key=lambda r: (-r[1], names[r[0]]))
+        cols["n_name"] = [names[r[0]] for r in rows]
+        cols["revenue"] = [r[1] for r in rows]
+    elif query == "Q10":
+        c, n = db.host["customer"], db.host["nation"]
+        nat = {int(k): _text(nm) for k, nm in zip(n["n_nationkey"], n["n_name"])}
+        sub = np.flatnonzero(np.isin(c["c_custkey"], cols["c_custkey"]))
+        pos = {int(c["c_custkey"][i]): i for i in sub}
+        idx = [pos[k] for k in cols["c_custkey"]]
+        cols["c_name"] = [_text(c["c_name"][i]) for i in idx]
+        cols["c_acctbal"] = [int(c["c_acctbal"][i]) for i in idx]
+        fmts["c_acctbal"] = (2, False)
+        cols["n_name"] = [nat[int(c["c_nationkey"][i])] for i in idx]
+        cols["c_address"] = [_text(c["c_address"][i]) for i in idx]
+        cols["c_phone"] = [_text(c["c_phone"][i]) for i in idx]
+        cols["c_comment"] = [_text(c["c_comment"][i]) for i in idx]
     elif query == "Q18":
         c = db.host["customer"]
         sub = np.flatnonzero(np.isin(c["c_custkey"], cols["c_custkey"]))

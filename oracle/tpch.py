@@ -104,7 +104,7 @@ def q6(T: dict, lo: int, hi: int, disc_lo: int, disc_hi: int, qty_hi: int) -> tu
     m = ((L["l_shipdate"] >= lo) & (L["l_shipdate"] < hi) & (L["l_discount"] >= disc_lo) & (L["l_discount"] <= disc_hi)
          & (L["l_quantity"] < qty_hi))
     rev = int((L["l_extendedprice"][m].astype(object) * L["l_discount"][m].astype(object)).sum())
-    return {"revenue": [rev]}, {"revenue": (4, False)}
+    return {"revenue": [rev] if m.any() else []}, {"revenue": (4, False)}
 
 
 def q12(T: dict, modes: tuple, lo: int, hi: int) -> tuple:
@@ -151,14 +151,15 @@ def q14(T: dict, prefix: bytes, lo: int, hi: int) -> tuple:
     parts = set(P["p_partkey"].tolist())
     promo = {int(k) for k, t in zip(P["p_partkey"], P["p_type"]) if bytes(t[:len(prefix)]) == prefix}
     m = (L["l_shipdate"] >= lo) & (L["l_shipdate"] < hi)
-    tot = pro = 0
+    tot = pro = n = 0
     for pk, p, d in zip(L["l_partkey"][m].tolist(), L["l_extendedprice"][m].tolist(), L["l_discount"][m].tolist()):
         if pk in parts:
             r = p * (100 - d)
             tot += r
+            n += 1
             if pk in promo:
                 pro += r
-    return {"promo_revenue": [_avg(100 * pro, tot)]}, {"promo_revenue": (6, False)}
+    return {"promo_revenue": [_avg(100 * pro, tot)] if n else []}, {"promo_revenue": (6, False)}
 
 
 def q18(T: dict, qty: int, k: int = 100) -> tuple:
@@ -205,6 +206,82 @@ def q22(T: dict, codes) -> tuple:
     return cols, {"cntrycode": (0, False), "numcust": (0, False), "totacctbal": (2, False)}
 
 
+def q5(T: dict, region: bytes, lo: int, hi: int) -> tuple:
+    """sql/q5.sql; ordered revenue desc, then n_name."""
+    R, N, S, C, O, L = (T[t] for t in ("region", "nation", "supplier", "customer", "orders", "lineitem"))
+    rk = {int(k) for k, nm in zip(R["r_regionkey"], R["r_name"]) if _s(nm) == region}
+    nations = {int(k): _s(nm).decode() for k, nm, r in zip(N["n_nationkey"], N["n_name"], N["n_regionkey"]) if int(r) in rk}
+    snat = {int(k): int(n) for k, n in zip(S["s_suppkey"], S["s_nationkey"]) if int(n) in nations}
+    cnat = {int(k): int(n) for k, n in zip(C["c_custkey"], C["c_nationkey"]) if int(n) in nations}
+    m = (O["o_orderdate"] >= lo) & (O["o_orderdate"] < hi)
+    onat = {int(k): cnat[int(c)] for k, c in zip(O["o_orderkey"][m], O["o_custkey"][m]) if int(c) in cnat}
+    rev: dict = {}
+    for ok, sk, p, d in zip(L["l_orderkey"].tolist(), L["l_suppkey"].tolist(), L["l_extendedprice"].tolist(),
+                            L["l_discount"].tolist()):
+        n = onat.get(ok)
+        if n is not None and snat.get(sk) == n:
+            rev[n] = rev.get(n, 0) + p * (100 - d)
+    rows = sorted(((nations[n], r) for n, r in rev.items()), key=lambda x: (-x[1], x[0]))
+    return {"n_name": [r[0] for r in rows], "revenue": [r[1] for r in rows]}, {"n_name": (0, False),
+                                                                                "revenue": (4, False)}
+
+
+def q10(T: dict, lo: int, hi: int, flag: int, k: int = 20) -> tuple:
+    """sql/q10.sql; ties of revenue broken by c_custkey asc."""
+    C, O, L, N = T["customer"], T["orders"], T["lineitem"], T["nation"]
+    m = (O["o_orderdate"] >= lo) & (O["o_orderdate"] < hi)
+    ocust = dict(zip(O["o_orderkey"][m].tolist(), O["o_custkey"][m].tolist()))
+    rev: dict = {}
+    lm = L["l_returnflag"] == flag
+    for ok, p, d in zip(L["l_orderkey"][lm].tolist(), L["l_extendedprice"][lm].tolist(), L["l_discount"][lm].tolist()):
+        c = ocust.get(ok)
+        if c is not None:
+            rev[c] = rev.get(c, 0) + p * (100 - d)
+    top = sorted(rev.items(), key=lambda kv: (-kv[1], kv[0]))[:k]
+    pos = {int(ck): i for i, ck in enumerate(C["c_custkey"].tolist())}
+    nat = {int(n): _s(nm).decode() for n, nm in zip(N["n_nationkey"], N["n_name"])}
+    cols = {"c_custkey": [], "c_name": [], "revenue": [], "c_acctbal": [], "n_name": [], "c_address": [], "c_phone": [],
+            "c_comment": []}
+    for ck, r in top:
+        i = pos[ck]
+        cols["c_custkey"].append(ck)
+        cols["c_name"].append(_s(C["c_name"][i]).decode())
+        cols["revenue"].append(r)
+        cols["c_acctbal"].append(int(C["c_acctbal"][i]))
+        cols["n_name"].append(nat[int(C["c_nationkey"][i])])
+        cols["c_address"].append(_s(C["c_address"][i]).decode())
+        cols["c_phone"].append(_s(C["c_phone"][i]).decode())
+        cols["c_comment"].append(_s(C["c_comment"][i]).decode())
+    fm = {c: (0, False) for c in cols}
+    fm["revenue"] = (4, False)
+    fm["c_acctbal"] = (2, False)
+    return cols, fm
+
+
+def q19(T: dict, strings, modes) -> tuple:
+    """sql/q19.sql; strings = 3 brands, 3 x 4 containers, ship instruction."""
+    P, L = T["part"], T["lineitem"]
+    brands, conts, instr = strings[:3], [set(strings[3 + 4 * c: 7 + 4 * c]) for c in range(3)], strings[15]
+    sizes, qty = (5, 10, 15), ((100, 1100), (1000, 2000), (2000, 3000))
+    cls = {}
+    for k, b, c, sz in zip(P["p_partkey"].tolist(), P["p_brand"], P["p_container"], P["p_size"].tolist()):
+        for j in range(3):
+            if _s(b) == brands[j] and _s(c) in conts[j] and 1 <= sz <= sizes[j]:
+                cls[k] = j
+    mode_codes = {m.encode()[:2] for m in modes}
+    rev = n = 0
+    for i in range(L["l_partkey"].shape[0]):
+        j = cls.get(int(L["l_partkey"][i]))
+        if j is None or _s(L["l_shipmode"][i]) not in mode_codes or _s(L["l_shipinstruct"][i]) != instr:
+            continue
+        q = int(L["l_quantity"][i])
+        if qty[j][0] <= q <= qty[j][1]:
+            rev += int(L["l_extendedprice"][i]) * (100 - int(L["l_discount"][i]))
+            n += 1
+    # no qualifying row: no result row (the engine skips an empty group)
+    return {"revenue": [rev] if n else []}, {"revenue": (4, False)}
+
+
 def run(T: dict, q: str, P: dict) -> tuple:
     """Query q with the constants P (the names of the engine's scalars:
     q1_cutoff, q3_seg, ... plus words / prefix for Q13 / Q14)."""
@@ -226,6 +303,14 @@ def run(T: dict, q: str, P: dict) -> tuple:
         return q14(T, P["prefix"], P["q14_date_lo"], P["q14_date_hi"])
     if q == "Q18":
         return q18(T, P["q18_qty"])
+    if q == "Q5":
+        return q5(T, P["strings"][0], P["q5_date_lo"], P["q5_date_hi"])
+    if q == "Q10":
+        return q10(T, P["q10_date_lo"], P["q10_date_hi"], P["q10_flag"])
+    if q == "Q19":
+        full = {"AI": "AIR", "RE": "REG AIR", "MA": "MAIL", "SH": "SHIP", "RA": "RAIL", "TR": "TRUCK", "FO": "FOB"}
+        modes = tuple(full[bytes([c & 255, c >> 8]).decode()] for c in (P["q19_mode1"], P["q19_mode2"]))
+        return q19(T, P["strings"], modes)
     if q == "Q22":
         codes = [c for c in range(64) if (P["q22_mask_lo"] >> c) & 1] + \
                 [c + 64 for c in range(64) if (P["q22_mask_hi"] >> c) & 1]
