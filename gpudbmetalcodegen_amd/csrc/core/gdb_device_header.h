@@ -457,6 +457,10 @@ GDB_DEV bool gdb_like_prefix(const char* s, int width, const char* p, int l) {
     return true;
 }
 
+// The first two bytes of a char field as a little-endian code (byte 0 low),
+// one 2-byte load (fields of even width are 2-byte aligned).
+GDB_DEV i64 gdb_code2(const char* p) { return (i64)*reinterpret_cast<const unsigned short*>(p); }
+
 // String equality of a fixed-width NUL-padded field with a constant held in
 // a 32-byte slot: slot[0] is the length (0..31), the bytes follow.
 GDB_DEV bool gdb_str_eq(const char* s, int width, const unsigned char* __restrict__ slot) {

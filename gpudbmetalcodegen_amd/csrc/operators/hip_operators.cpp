@@ -87,7 +87,18 @@ HipGridStrideScan::HipGridStrideScan(const std::string& table, const std::string
 void HipGridStrideScan::addColumn(const std::string& column, ColumnType type) {
     for (const auto& c : columns_)
         if (c.name == column) return;
-    columns_.push_back({column, type});
+    columns_.push_back({column, type, false});
+}
+
+void HipGridStrideScan::setLazy(const std::string& column) {
+    for (auto& c : columns_)
+        if (c.name == column) {
+            if (c.type.type != ColType::CHAR)
+                throw std::runtime_error("HipGridStrideScan: only char columns are read lazily (" + column + ")");
+            c.lazy = true;
+            return;
+        }
+    throw std::runtime_error("HipGridStrideScan(" + table_ + "): no column " + column);
 }
 
 void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
@@ -103,8 +114,9 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const std::string n = range ? "_re" : nParam;
     const int lb = cg.getPhases().back().launchBound;
 
+    auto staged = [](const ColumnDesc& c) { return c.type.type == ColType::CHAR && !c.lazy; };
     bool anyCharCol = false;
-    for (const auto& c : columns_) anyCharCol |= c.type.type == ColType::CHAR;
+    for (const auto& c : columns_) anyCharCol |= staged(c);
     // Software pipelining across tiles (grid mode, fixed-width columns): the
     // next tile's column loads are in flight while this tile is processed.
     const bool prefetch = cg.getPhases().back().prefetch && !range && !anyCharCol;
@@ -146,12 +158,13 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
         cg.addLine(range ? "const u64 _step = _tile;" : "const u64 _step = (u64)gridDim.x * _tile;");
         if (prefetch) {
-            for (const auto& c : columns_) cg.addLine(c.type.cxx() + " _p_" + c.name + "[ITEMS];");
+            for (const auto& c : columns_)
+                if (c.type.type != ColType::CHAR) cg.addLine(c.type.cxx() + " _p_" + c.name + "[ITEMS];");
             emitLoads("(u64)blockIdx.x * _tile", "_p_");
         }
         for (size_t k = 0; k < columns_.size(); ++k) {
             const auto& c = columns_[k];
-            if (c.type.type == ColType::CHAR) {
+            if (staged(c)) {
                 // + 16: row readers may load the 4-byte word past a row's end
                 cg.declareLds("char", "_s_" + c.name, s(lb) + " * ITEMS * " + s(c.type.width) + " + 16");
             }
@@ -163,6 +176,7 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             bool anyChar = false;
             for (size_t k = 0; k < columns_.size(); ++k) {
                 const auto& c = columns_[k];
+                if (c.type.type == ColType::CHAR && c.lazy) continue;
                 if (c.type.type == ColType::CHAR) {
                     anyChar = true;
                     const std::string W = s(c.type.width);
@@ -192,6 +206,7 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             if (prefetch) {
                 cg.addComment("this tile's columns were loaded one iteration ahead; issue the next tile's now");
                 for (const auto& c : columns_) {
+                    if (c.type.type == ColType::CHAR) continue;
                     cg.addLine("#pragma unroll");
                     cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) _r_" + c.name + "[_u] = _p_" + c.name + "[_u];");
                 }
@@ -205,8 +220,12 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                 cg.addLine("for (int _u = 0; _u < ITEMS; ++_u) _act[_u] = _full || _r0 + (u64)_u < " + n + ";");
                 const size_t mark = cg.rowBindings().size();
                 cg.addRowBinding("const u64 " + idxVar_ + " = _r0 + (u64)_u;");
-                for (const auto& c : columns_) {
-                    if (c.type.type == ColType::CHAR)
+                for (size_t k = 0; k < columns_.size(); ++k) {
+                    const auto& c = columns_[k];
+                    if (c.type.type == ColType::CHAR && c.lazy)
+                        cg.addRowBinding("const char* " + c.name + " = (const char*)" + ptr[k] + " + (_r0 + (u64)_u) * " +
+                                         s(c.type.width) + ";");
+                    else if (c.type.type == ColType::CHAR)
                         cg.addRowBinding("const char* " + c.name + " = _s_" + c.name +
                                          " + ((u64)threadIdx.x * ITEMS + _u) * " + s(c.type.width) + ";");
                     else
@@ -223,8 +242,12 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
                 cg.addLine("const u64 " + idxVar_ + " = _r0 + (u64)_u;");
                 cg.addBlock("if (_full || " + idxVar_ + " < " + n + ")", [&]() {
-                    for (const auto& c : columns_) {
-                        if (c.type.type == ColType::CHAR) {
+                    for (size_t k = 0; k < columns_.size(); ++k) {
+                        const auto& c = columns_[k];
+                        if (c.type.type == ColType::CHAR && c.lazy) {
+                            cg.addLine("const char* " + c.name + " = (const char*)" + ptr[k] + " + " + idxVar_ + " * " +
+                                       s(c.type.width) + ";");
+                        } else if (c.type.type == ColType::CHAR) {
                             cg.addLine("const char* " + c.name + " = _s_" + c.name + " + ((u64)threadIdx.x * ITEMS + _u) * " +
                                        s(c.type.width) + ";");
                         } else {

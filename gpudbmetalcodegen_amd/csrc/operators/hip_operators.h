@@ -70,9 +70,15 @@ public:
     struct ColumnDesc {
         std::string name;
         ColumnType type;
+        bool lazy = false;  // char columns: read in place from HBM, not staged in LDS
     };
     explicit HipGridStrideScan(const std::string& table, const std::string& idxVar = "i");
     void addColumn(const std::string& column, ColumnType type);
+    // A char column whose rows are read only in part or only for some rows
+    // (a 2-byte code, a prefix, a compare after a selective filter): bound as
+    // a pointer into the column in HBM instead of being staged in LDS with
+    // the whole tile.
+    void setLazy(const std::string& column);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
     const std::string& tableName() const { return table_; }

@@ -28,6 +28,15 @@ inline std::unique_ptr<HipGridStrideScan> makeScan(const std::string& table, std
     return scan;
 }
 
+// Same, with the named char columns read in place (HipGridStrideScan::setLazy).
+inline std::unique_ptr<HipGridStrideScan> makeScanLazy(const std::string& table,
+                                                       std::initializer_list<const char*> cols,
+                                                       std::initializer_list<const char*> lazy) {
+    auto scan = makeScan(table, cols);
+    for (const char* c : lazy) scan->setLazy(c);
+    return scan;
+}
+
 inline HipQueryPlan::Phase& appendPhase(HipQueryPlan& plan, const std::string& name,
                                         std::unique_ptr<HipOperator> root, int block, int items,
                                         std::vector<int> variants = {}) {

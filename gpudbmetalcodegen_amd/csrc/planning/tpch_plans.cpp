@@ -179,9 +179,10 @@ HipQueryPlan tpchQ12(const PlanOptions& o) {
         appendPhase(plan, "Q12_orders", std::move(cls), 256, optInt(o, "items", 8), {4, 8});
     }
     {
-        auto scan = makeScan("lineitem", {"l_orderkey", "l_shipmode", "l_shipdate", "l_commitdate", "l_receiptdate"});
+        auto scan = makeScanLazy("lineitem", {"l_orderkey", "l_shipmode", "l_shipdate", "l_commitdate", "l_receiptdate"},
+                                 {"l_shipmode"});
         auto mc = std::make_unique<HipComputeExpr>(std::move(scan), "_mc", "i64",
-                                                   "(i64)(u8)l_shipmode[0] | ((i64)(u8)l_shipmode[1] << 8)");
+                                                   "gdb_code2(l_shipmode)");
         auto sel = std::make_unique<HipSelection>(
             std::move(mc),
             "(_mc == q12_mode1 || _mc == q12_mode2) && l_commitdate < l_receiptdate && l_shipdate < l_commitdate && "
@@ -246,7 +247,7 @@ HipQueryPlan tpchQ14(const PlanOptions& o) {
     plan.name = "Q14";
     {
         // one byte per part key: 0 no part, 1 other type, 2 promotional type
-        auto scan = makeScan("part", {"p_partkey", "p_type"});
+        auto scan = makeScanLazy("part", {"p_partkey", "p_type"}, {"p_type"});
         auto cls = std::make_unique<HipByteFlagBuild>(
             std::move(scan), "d_q14_parts", "p_partkey", "p_bits",
             "gdb_like_prefix(p_type, 25, (const char*)d_like_words, (int)like_l1) ? 2 : 1");
@@ -328,7 +329,7 @@ HipQueryPlan tpchQ22(const PlanOptions& o) {
     const std::string inSet =
         "(_cc >= 0 && _cc < 128 && (((_cc < 64 ? (u64)q22_mask_lo >> _cc : (u64)q22_mask_hi >> (_cc - 64)) & 1ull) != 0))";
     {
-        auto scan = makeScan("customer", {"c_phone", "c_acctbal"});
+        auto scan = makeScanLazy("customer", {"c_phone", "c_acctbal"}, {"c_phone"});
         auto cc = std::make_unique<HipComputeExpr>(std::move(scan), "_cc", "i64", code);
         auto sel = std::make_unique<HipSelection>(std::move(cc), "c_acctbal > 0 && " + inSet);
         auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q22_avg", "0", 1);
@@ -346,7 +347,7 @@ HipQueryPlan tpchQ22(const PlanOptions& o) {
         appendPhase(plan, "Q22_orders_bitmap", std::move(bm), 256, 4);
     }
     {
-        auto scan = makeScan("customer", {"c_custkey", "c_phone", "c_acctbal"});
+        auto scan = makeScanLazy("customer", {"c_custkey", "c_phone", "c_acctbal"}, {"c_phone"});
         std::vector<std::pair<std::string, int>> vars = {{"_avg_sum", 0}, {"_avg_cnt", 1}};
         auto avg = std::make_unique<HipBufferScalars>(std::move(scan), "d_q22_avg", "16", vars);
         auto cc = std::make_unique<HipComputeExpr>(std::move(avg), "_cc", "i64", code);
@@ -471,8 +472,9 @@ HipQueryPlan tpchQ10(const PlanOptions& o) {
         auto merge = top->makeMerge();
         auto& ph = appendPhase(plan, "Q10_topk", std::move(top), 256, 4);
         ph.launchBound = 256;
-        auto& pm = appendPhase(plan, "Q10_topk_merge", std::move(merge), 1024, 1, {1});
-        pm.launchBound = 1024;
+        // K = 20 lists need more than the 128 VGPRs a 1024-thread block allows
+        auto& pm = appendPhase(plan, "Q10_topk_merge", std::move(merge), 256, 1, {1});
+        pm.launchBound = 256;
     }
     plan.requiredScalars = {"q10_date_lo", "q10_date_hi", "q10_flag"};
     plan.requiredSymbols = {"o_bits", "c_bits"};
@@ -512,10 +514,10 @@ HipQueryPlan tpchQ19(const PlanOptions& o) {
         ph.externalBuffers = {{"d_str_consts", "u8"}};
     }
     {
-        auto scan = makeScan("lineitem", {"l_partkey", "l_quantity", "l_extendedprice", "l_discount", "l_shipmode",
-                                          "l_shipinstruct"});
+        auto scan = makeScanLazy("lineitem", {"l_partkey", "l_quantity", "l_extendedprice", "l_discount", "l_shipmode",
+                                              "l_shipinstruct"}, {"l_shipmode", "l_shipinstruct"});
         auto mc = std::make_unique<HipComputeExpr>(std::move(scan), "_mc", "i64",
-                                                   "(i64)(u8)l_shipmode[0] | ((i64)(u8)l_shipmode[1] << 8)");
+                                                   "gdb_code2(l_shipmode)");
         auto sel = std::make_unique<HipSelection>(
             std::move(mc), "(_mc == q19_mode1 || _mc == q19_mode2) && gdb_str_eq(l_shipinstruct, 25, " + slot(15) + ")");
         auto cls = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q19_parts", "l_partkey", "p_bits", false,
