@@ -154,7 +154,10 @@ void HipGenericExecutor::hoistInits(const HipCodegen& cg, int first, int last,
             } catch (const std::exception&) {
                 continue;  // sized by something not known yet: initialised at its phase
             }
-            const size_t nb = static_cast<size_t>(elems) * ib->elemBytes;
+            size_t nb = static_cast<size_t>(elems) * ib->elemBytes;
+            // zero fills may cover the allocation's padding (allocations are
+            // whole 256-byte multiples), so byte arrays of any length join
+            if (ib->init == InitKind::Zero) nb = (nb + 7) & ~static_cast<size_t>(7);
             const int pl = ib->init == InitKind::Zero ? 8 : static_cast<int>(ib->pattern.size());
             if (nb == 0 || nb % 8 != 0 || pl > 32 || (32 % pl) != 0 || nb % static_cast<size_t>(pl) != 0) continue;
             Buffer& buf = ensureBuffer(*ib);

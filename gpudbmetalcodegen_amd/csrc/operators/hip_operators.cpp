@@ -1659,9 +1659,18 @@ void HipTopKMerge::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.declareLds("u64", "_tk_lds", "GDB_TOPK_LDS_WORDS(" + K + ")");
         cg.addLine("GdbTopK<" + K + "> _f;");
         cg.addLine("gdb_topk_init<" + K + ">(_f);");
-        cg.addBlock("for (u64 _c = threadIdx.x; _c < " + NC + "; _c += blockDim.x)", [&]() {
-            cg.addLine("const u64 _h = " + cand + "[2 * _c], _l = " + cand + "[2 * _c + 1];");
-            cg.addLine("if ((_h | _l) != 0) gdb_topk_push<" + K + ">(_f, _h, _l);");
+        // Candidates in batches of 8 per thread: the batch's 16-byte loads
+        // are all issued before the first insertion, so the (cross-XCD,
+        // uncached) candidate reads cost one round trip per batch instead of
+        // one per candidate.
+        cg.addBlock("for (u64 _c0 = 0; _c0 < " + NC + "; _c0 += 8ull * blockDim.x)", [&]() {
+            cg.addLine("u64x2 _cv[8];");
+            cg.addLine("#pragma unroll");
+            cg.addLine("for (int _j = 0; _j < 8; ++_j) { const u64 _c = _c0 + (u64)_j * blockDim.x + threadIdx.x; _cv[_j] = _c < " +
+                       NC + " ? reinterpret_cast<const u64x2*>(" + cand + ")[_c] : u64x2{0ull, 0ull}; }");
+            cg.addLine("#pragma unroll");
+            cg.addLine("for (int _j = 0; _j < 8; ++_j) if ((_cv[_j].x | _cv[_j].y) != 0) gdb_topk_push<" + K +
+                       ">(_f, _cv[_j].x, _cv[_j].y);");
         });
         cg.addLine("gdb_topk_block_merge<" + K + ">(_f, _tk_lds);");
         cg.addLine("const u64* _tk_res = _tk_lds + 32 * " + K + ";");
