@@ -539,11 +539,152 @@ HipQueryPlan tpchQ19(const PlanOptions& o) {
     return plan;
 }
 
+// Byte flags of the suppliers of the nation named like string constant 0
+// (two phases: nation name -> nation key flag -> supplier flag).
+void nationSupplierFlags(HipQueryPlan& plan, const std::string& prefix, const std::string& flags) {
+    {
+        auto scan = makeScan("nation", {"n_nationkey", "n_name"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "gdb_str_eq(n_name, 25, (const unsigned char*)d_str_consts)");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_" + prefix + "_nation", "n_nationkey", "n_bits");
+        auto& ph = appendPhase(plan, prefix + "_nation", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_" + prefix + "_nation", "s_nationkey", "n_bits",
+                                                     false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), flags, "s_suppkey", "s_bits");
+        appendPhase(plan, prefix + "_suppliers", std::move(fl), 256, 4);
+    }
+}
+
+// Q11 — important stock identification (sql/q11.sql): supply value per part
+// over the suppliers of one nation (key-indexed sums) and its total in the
+// same pass, then the parts whose value exceeds total * fraction
+// (value * q11_frac_den > total, exact), ordered by value desc.
+HipQueryPlan tpchQ11(const PlanOptions& o) {
+    (void)o;
+    HipQueryPlan plan;
+    plan.name = "Q11";
+    nationSupplierFlags(plan, "Q11", "d_q11_supp");
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey", "ps_supplycost", "ps_availqty"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q11_supp", "ps_suppkey", "s_bits", false);
+        auto add = std::make_unique<HipDenseAtomicAdd>(std::move(in), "d_q11_value", "ps_partkey", "p_bits",
+                                                       "ps_supplycost * (i64)ps_availqty");
+        auto tot = std::make_unique<HipKeyedAgg>(std::move(add), "d_q11_total", "0", 1);
+        tot->addAggregate("total", AggKind::SumI64, "ps_supplycost * (i64)ps_availqty", 2);
+        appendPhase(plan, "Q11_value", std::move(tot), 256, 4);
+    }
+    {
+        auto scan = makeScan("part", {"p_partkey"});
+        auto tot = std::make_unique<HipBufferScalars>(std::move(scan), "d_q11_total", "16",
+                                                      std::vector<std::pair<std::string, int>>{{"_tot", 0}});
+        auto look = std::make_unique<HipArrayLookup>(std::move(tot), "d_q11_value", "p_partkey", "p_bits", "_val", 0ll,
+                                                     "i64");
+        auto sel = std::make_unique<HipSelection>(std::move(look), "_val * q11_frac_den > _tot");
+        auto rows = std::make_unique<HipAppendRows>(std::move(sel), "d_q11_rows", "n_part");
+        rows->addField("ps_partkey", "p_partkey");
+        rows->addField("value", "_val", 2);
+        rows->orderBy("value", true);
+        auto& ph = appendPhase(plan, "Q11_parts", std::move(rows), 256, 4);
+        ph.scalarParams = {"q11_frac_den"};
+    }
+    plan.requiredScalars = {"q11_frac_den"};
+    plan.requiredSymbols = {"n_bits", "s_bits", "p_bits"};
+    return plan;
+}
+
+// Q15 — top supplier (sql/q15.sql): revenue per supplier over the quarter
+// (key-indexed sums), its maximum, then the suppliers whose revenue equals
+// it, ordered by s_suppkey. The caller attaches name, address and phone.
+HipQueryPlan tpchQ15(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q15";
+    {
+        auto scan = makeScan("lineitem", {"l_suppkey", "l_extendedprice", "l_discount", "l_shipdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)l_shipdate >= q15_date_lo && (i64)l_shipdate < q15_date_hi");
+        auto add = std::make_unique<HipDenseAtomicAdd>(std::move(sel), "d_q15_rev", "l_suppkey", "s_bits", kRevenue);
+        auto& ph = appendPhase(plan, "Q15_revenue", std::move(add), 256, optInt(o, "items", 4));
+        ph.scalarParams = {"q15_date_lo", "q15_date_hi"};
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey"});
+        auto look = std::make_unique<HipArrayLookup>(std::move(scan), "d_q15_rev", "s_suppkey", "s_bits", "_rev", 0ll,
+                                                     "i64");
+        auto mx = std::make_unique<HipKeyedAgg>(std::move(look), "d_q15_max", "0", 1);
+        mx->addAggregate("max_revenue", AggKind::MaxI64, "_rev", 4);
+        appendPhase(plan, "Q15_max", std::move(mx), 256, 4);
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey"});
+        auto mx = std::make_unique<HipBufferScalars>(std::move(scan), "d_q15_max", "16",
+                                                     std::vector<std::pair<std::string, int>>{{"_max", 0}});
+        auto look = std::make_unique<HipArrayLookup>(std::move(mx), "d_q15_rev", "s_suppkey", "s_bits", "_rev", 0ll,
+                                                     "i64");
+        auto sel = std::make_unique<HipSelection>(std::move(look), "_rev == _max");
+        auto rows = std::make_unique<HipAppendRows>(std::move(sel), "d_q15_rows", "n_supplier");
+        rows->addField("s_suppkey", "s_suppkey");
+        rows->addField("total_revenue", "_rev", 4);
+        rows->orderBy("s_suppkey", false);
+        appendPhase(plan, "Q15_top", std::move(rows), 256, 4);
+    }
+    plan.requiredScalars = {"q15_date_lo", "q15_date_hi"};
+    plan.requiredSymbols = {"s_bits"};
+    return plan;
+}
+
+// Q17 — small-quantity-order revenue (sql/q17.sql): parts of one brand and
+// container (string constants 0 and 1) flagged; per flagged part the sum and
+// count of l_quantity (key-indexed); then the lineitems whose quantity is
+// below 0.2 * the part's average (5 * qty * count < sum, exact) summed. The
+// caller divides by 7 (avg_yearly).
+HipQueryPlan tpchQ17(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q17";
+    {
+        auto scan = makeScan("part", {"p_partkey", "p_brand", "p_container"});
+        auto sel = std::make_unique<HipSelection>(
+            std::move(scan),
+            "gdb_str_eq(p_brand, 10, (const unsigned char*)d_str_consts) && "
+            "gdb_str_eq(p_container, 10, (const unsigned char*)d_str_consts + 32)");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q17_parts", "p_partkey", "p_bits");
+        auto& ph = appendPhase(plan, "Q17_parts", std::move(fl), 256, 4);
+        ph.launchBound = 256;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_quantity"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q17_parts", "l_partkey", "p_bits", false);
+        auto sum = std::make_unique<HipDenseAtomicAdd>(std::move(in), "d_q17_qsum", "l_partkey", "p_bits", "l_quantity");
+        auto cnt = std::make_unique<HipDenseAtomicAdd>(std::move(sum), "d_q17_qcnt", "l_partkey", "p_bits", "1");
+        appendPhase(plan, "Q17_part_qty", std::move(cnt), 256, optInt(o, "items", 4));
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_quantity", "l_extendedprice"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q17_parts", "l_partkey", "p_bits", false);
+        auto qs = std::make_unique<HipArrayLookup>(std::move(in), "d_q17_qsum", "l_partkey", "p_bits", "_qs", -1ll, "i64");
+        auto qc = std::make_unique<HipArrayLookup>(std::move(qs), "d_q17_qcnt", "l_partkey", "p_bits", "_qc", -1ll, "i64");
+        auto sel = std::make_unique<HipSelection>(std::move(qc), "5 * l_quantity * _qc < _qs");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q17_sum", "0", 1);
+        agg->addAggregate("sum_price", AggKind::SumI64, "l_extendedprice", 2);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        appendPhase(plan, "Q17_small_qty", std::move(agg), 256, optInt(o, "items", 4));
+    }
+    plan.requiredSymbols = {"p_bits"};
+    return plan;
+}
+
 bool tpchSchema(const PlanOptions& o) { return opt(o, "schema", "") == "tpch"; }
 
 }  // namespace
 
-std::vector<std::string> tpchPlanNames() { return {"Q4", "Q5", "Q10", "Q12", "Q13", "Q14", "Q18", "Q19", "Q22"}; }
+std::vector<std::string> tpchPlanNames() {
+    return {"Q4", "Q5", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q22"};
+}
 
 std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOptions& opts) {
     const std::string schema = opt(opts, "schema", "");
@@ -555,6 +696,9 @@ std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOpt
     if (name == "Q5") return tpchQ5(opts);
     if (name == "Q10") return tpchQ10(opts);
     if (name == "Q19") return tpchQ19(opts);
+    if (name == "Q11") return tpchQ11(opts);
+    if (name == "Q15") return tpchQ15(opts);
+    if (name == "Q17") return tpchQ17(opts);
     if (name == "Q12") return tpchQ12(opts);
     if (name == "Q13") return tpchQ13(opts);
     if (name == "Q14") return tpchQ14(opts);

@@ -31,7 +31,7 @@ from . import engine as E
 
 INT, FLOAT, DATE, CHAR1, CHAR_FIXED = 0, 1, 2, 3, 4
 TABLES = ["lineitem", "orders", "customer", "supplier", "part", "partsupp", "nation", "region"]
-QUERIES = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q10", "Q12", "Q13", "Q14", "Q18", "Q19", "Q22"]
+QUERIES = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q22"]
 CHECK_ABS_TOL = 1e-2  # the reference's --check defaults (codegen_main.cpp:47-48)
 CHECK_REL_TOL = 1e-4
 
@@ -177,6 +177,9 @@ PARAMS = {
     "Q14": {"prefix": b"PROMO", "q14_date_lo": ymd(1995, 9, 1), "q14_date_hi": shift(ymd(1995, 9, 1), months=1)},
     "Q18": {"q18_qty": 30000},
     "Q5": {"strings": (b"ASIA",), "q5_date_lo": ymd(1994, 1, 1), "q5_date_hi": shift(ymd(1994, 1, 1), years=1)},
+    "Q11": {"strings": (b"GERMANY",), "q11_frac_den": 10000},  # value > total * 0.0001
+    "Q15": {"q15_date_lo": ymd(1996, 1, 1), "q15_date_hi": shift(ymd(1996, 1, 1), months=3)},
+    "Q17": {"strings": (b"Brand#23", b"MED BOX")},
     "Q10": {"q10_date_lo": ymd(1993, 10, 1), "q10_date_hi": shift(ymd(1993, 10, 1), months=3), "q10_flag": ord("R")},
     "Q19": {"q19_mode1": mode_code("AIR"), "q19_mode2": mode_code("REG AIR"),
             "strings": (b"Brand#12", b"Brand#23", b"Brand#34", b"SM CASE", b"SM BOX", b"SM PACK", b"SM PKG",
@@ -191,7 +194,9 @@ SCANNED = {"Q1": ["lineitem"], "Q3": ["customer", "orders", "lineitem"], "Q4": [
            "Q6": ["lineitem"], "Q12": ["orders", "lineitem"], "Q13": ["orders", "customer"],
            "Q14": ["part", "lineitem"], "Q18": ["lineitem", "orders"], "Q22": ["customer", "orders"],
            "Q5": ["region", "nation", "supplier", "customer", "orders", "lineitem"],
-           "Q10": ["orders", "lineitem", "customer"], "Q19": ["part", "lineitem"]}
+           "Q10": ["orders", "lineitem", "customer"], "Q19": ["part", "lineitem"],
+           "Q11": ["nation", "supplier", "partsupp", "part"], "Q15": ["lineitem", "supplier"],
+           "Q17": ["part", "lineitem"]}
 # Output columns in the SQL select list's order.
 SELECT = {
     "Q1": ["l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price", "sum_charge", "avg_qty",
@@ -207,6 +212,9 @@ SELECT = {
     "Q5": ["n_name", "revenue"],
     "Q10": ["c_custkey", "c_name", "revenue", "c_acctbal", "n_name", "c_address", "c_phone", "c_comment"],
     "Q19": ["revenue"],
+    "Q11": ["ps_partkey", "value"],
+    "Q15": ["s_suppkey", "s_name", "s_address", "s_phone", "total_revenue"],
+    "Q17": ["avg_yearly"],
 }
 
 
@@ -249,6 +257,10 @@ class TpchDB:
             self.symbols["r_bits"] = max(self.symbols["r_bits"], int(h["nation"]["n_regionkey"].max(initial=0)) + 1)
         if "lineitem" in h and "supplier" in h:
             self.symbols["s_bits"] = max(self.symbols["s_bits"], int(h["lineitem"]["l_suppkey"].max(initial=0)) + 1)
+        if "partsupp" in h:
+            ps = h["partsupp"]
+            self.symbols["p_bits"] = max(self.symbols.get("p_bits", 1), int(ps["ps_partkey"].max(initial=0)) + 1)
+            self.symbols["s_bits"] = max(self.symbols.get("s_bits", 1), int(ps["ps_suppkey"].max(initial=0)) + 1)
 
     def device(self, table: str) -> E.DeviceTable:
         t = self.dev.get(table)
@@ -345,6 +357,11 @@ def _text(arr_row) -> str:
     return bytes(arr_row).split(b"\0", 1)[0].decode()
 
 
+# Queries whose result the host finishes (attached columns, reordering, a
+# final division); the others are the engine's result as collected.
+HOST_FINISH = {"Q3", "Q5", "Q10", "Q15", "Q17", "Q18"}
+
+
 def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
     """Host post-processing on the few result rows (the reference's CPU
     post-processing in runCodegenQuery): attach o_shippriority (Q3) and c_name
@@ -374,6 +391,17 @@ def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
         cols["c_address"] = [_text(c["c_address"][i]) for i in idx]
         cols["c_phone"] = [_text(c["c_phone"][i]) for i in idx]
         cols["c_comment"] = [_text(c["c_comment"][i]) for i in idx]
+    elif query == "Q15":
+        sp = db.host["supplier"]
+        sub = np.flatnonzero(np.isin(sp["s_suppkey"], cols["s_suppkey"]))
+        pos = {int(sp["s_suppkey"][i]): i for i in sub}
+        idx = [pos[k] for k in cols["s_suppkey"]]
+        for c in ("s_name", "s_address", "s_phone"):
+            cols[c] = [_text(sp[c][i]) for i in idx]
+    elif query == "Q17":
+        # SUM(l_extendedprice) / 7.0, exact to 6 more digits: trunc(sum * 10^6 / 7)
+        cols["avg_yearly"] = [v * 1_000_000 // 7 for v in cols["sum_price"]]
+        fmts["avg_yearly"] = (fmts["sum_price"][0] + 6, False)
     elif query == "Q18":
         c = db.host["customer"]
         sub = np.flatnonzero(np.isin(c["c_custkey"], cols["c_custkey"]))

@@ -282,6 +282,66 @@ def q19(T: dict, strings, modes) -> tuple:
     return {"revenue": [rev] if n else []}, {"revenue": (4, False)}
 
 
+def q11(T: dict, nation: bytes, frac_den: int) -> tuple:
+    """sql/q11.sql with HAVING value > total / frac_den; value desc, then key."""
+    N, S, PS = T["nation"], T["supplier"], T["partsupp"]
+    nk = {int(k) for k, nm in zip(N["n_nationkey"], N["n_name"]) if _s(nm) == nation}
+    sup = {int(k) for k, n in zip(S["s_suppkey"], S["s_nationkey"]) if int(n) in nk}
+    val: dict = {}
+    for pk, sk, c, a in zip(PS["ps_partkey"].tolist(), PS["ps_suppkey"].tolist(), PS["ps_supplycost"].tolist(),
+                            PS["ps_availqty"].tolist()):
+        if sk in sup:
+            val[pk] = val.get(pk, 0) + c * a
+    total = sum(val.values())
+    rows = sorted(((k, v) for k, v in val.items() if v * frac_den > total), key=lambda r: (-r[1], r[0]))
+    return {"ps_partkey": [r[0] for r in rows], "value": [r[1] for r in rows]}, {"ps_partkey": (0, False),
+                                                                                "value": (2, False)}
+
+
+def q15(T: dict, lo: int, hi: int) -> tuple:
+    """sql/q15.sql; suppliers with the maximum quarter revenue, by key."""
+    S, L = T["supplier"], T["lineitem"]
+    m = (L["l_shipdate"] >= lo) & (L["l_shipdate"] < hi)
+    rev: dict = {}
+    for sk, p, d in zip(L["l_suppkey"][m].tolist(), L["l_extendedprice"][m].tolist(), L["l_discount"][m].tolist()):
+        rev[sk] = rev.get(sk, 0) + p * (100 - d)
+    suppliers = set(S["s_suppkey"].tolist())
+    rev = {k: v for k, v in rev.items() if k in suppliers}
+    cols = {"s_suppkey": [], "s_name": [], "s_address": [], "s_phone": [], "total_revenue": []}
+    if rev:
+        mx = max(rev.values())
+        pos = {int(k): i for i, k in enumerate(S["s_suppkey"].tolist())}
+        for k in sorted(k for k, v in rev.items() if v == mx):
+            i = pos[k]
+            cols["s_suppkey"].append(k)
+            cols["s_name"].append(_s(S["s_name"][i]).decode())
+            cols["s_address"].append(_s(S["s_address"][i]).decode())
+            cols["s_phone"].append(_s(S["s_phone"][i]).decode())
+            cols["total_revenue"].append(mx)
+    fm = {c: (0, False) for c in cols}
+    fm["total_revenue"] = (4, False)
+    return cols, fm
+
+
+def q17(T: dict, brand: bytes, container: bytes) -> tuple:
+    """sql/q17.sql: sum(l_extendedprice) / 7 as trunc(sum * 10^6 / 7), 8 digits."""
+    P, L = T["part"], T["lineitem"]
+    parts = {int(k) for k, b, c in zip(P["p_partkey"], P["p_brand"], P["p_container"])
+             if _s(b) == brand and _s(c) == container}
+    qs: dict = {}
+    qc: dict = {}
+    for pk, q in zip(L["l_partkey"].tolist(), L["l_quantity"].tolist()):
+        if pk in parts:
+            qs[pk] = qs.get(pk, 0) + q
+            qc[pk] = qc.get(pk, 0) + 1
+    tot = n = 0
+    for pk, q, p in zip(L["l_partkey"].tolist(), L["l_quantity"].tolist(), L["l_extendedprice"].tolist()):
+        if pk in parts and 5 * q * qc[pk] < qs[pk]:
+            tot += p
+            n += 1
+    return {"avg_yearly": [tot * 1_000_000 // 7] if n else []}, {"avg_yearly": (8, False)}
+
+
 def run(T: dict, q: str, P: dict) -> tuple:
     """Query q with the constants P (the names of the engine's scalars:
     q1_cutoff, q3_seg, ... plus words / prefix for Q13 / Q14)."""
@@ -307,6 +367,12 @@ def run(T: dict, q: str, P: dict) -> tuple:
         return q5(T, P["strings"][0], P["q5_date_lo"], P["q5_date_hi"])
     if q == "Q10":
         return q10(T, P["q10_date_lo"], P["q10_date_hi"], P["q10_flag"])
+    if q == "Q11":
+        return q11(T, P["strings"][0], P["q11_frac_den"])
+    if q == "Q15":
+        return q15(T, P["q15_date_lo"], P["q15_date_hi"])
+    if q == "Q17":
+        return q17(T, P["strings"][0], P["strings"][1])
     if q == "Q19":
         full = {"AI": "AIR", "RE": "REG AIR", "MA": "MAIL", "SH": "SHIP", "RA": "RAIL", "TR": "TRUCK", "FO": "FOB"}
         modes = tuple(full[bytes([c & 255, c >> 8]).decode()] for c in (P["q19_mode1"], P["q19_mode2"]))

@@ -26,15 +26,17 @@ GOLDEN = Path(__file__).resolve().parent / "golden" / "tpch"
 
 def q19_parts(data: dict) -> dict:
     """Give a fifth of the parts each of Q19's three brand/container/size
-    classes, so the small generated data selects Q19 rows."""
+    classes and another fifth Q17's brand and container, so the small
+    generated data selects Q17 and Q19 rows."""
     P = data["part"]
     k = P["p_partkey"] % 5
     for c, (brand, cont, size) in enumerate([("Brand#12", "SM BOX", 3), ("Brand#23", "MED BAG", 7),
-                                             ("Brand#34", "LG PKG", 12)]):
+                                             ("Brand#34", "LG PKG", 12), ("Brand#23", "MED BOX", None)]):
         sel = k == c
         P["p_brand"][sel] = G.fixed([brand], 10)[0]
         P["p_container"][sel] = G.fixed([cont], 10)[0]
-        P["p_size"][sel] = size
+        if size is not None:
+            P["p_size"][sel] = size
     return data
 
 
@@ -305,6 +307,19 @@ SQL = {
         OR (p_partkey = l_partkey AND p_brand = 'Brand#34' AND p_container IN ('LG CASE', 'LG BOX', 'LG PACK', 'LG PKG')
          AND l_quantity >= 20 AND l_quantity <= 30 AND p_size BETWEEN 1 AND 15 AND l_shipmode IN ('AI', 'RE')
          AND l_shipinstruct = 'DELIVER IN PERSON')""",
+    "Q11": """SELECT ps_partkey, SUM(ps_supplycost * ps_availqty) AS value FROM partsupp, supplier, nation
+        WHERE ps_suppkey = s_suppkey AND s_nationkey = n_nationkey AND n_name = 'GERMANY' GROUP BY ps_partkey
+        HAVING SUM(ps_supplycost * ps_availqty) > (SELECT SUM(ps_supplycost * ps_availqty) * 0.0001
+        FROM partsupp, supplier, nation WHERE ps_suppkey = s_suppkey AND s_nationkey = n_nationkey
+        AND n_name = 'GERMANY') ORDER BY value DESC, ps_partkey""",
+    "Q15": """WITH revenue0 (supplier_no, total_revenue) AS (SELECT l_suppkey, SUM(l_extendedprice * (1 - l_discount))
+        FROM lineitem WHERE l_shipdate >= '1996-01-01' AND l_shipdate < '1996-04-01' GROUP BY l_suppkey)
+        SELECT s_suppkey, s_name, s_address, s_phone, total_revenue FROM supplier, revenue0
+        WHERE s_suppkey = supplier_no AND total_revenue = (SELECT MAX(total_revenue) FROM revenue0)
+        ORDER BY s_suppkey""",
+    "Q17": """SELECT SUM(l_extendedprice) / 7.0 FROM lineitem, part WHERE p_partkey = l_partkey
+        AND p_brand = 'Brand#23' AND p_container = 'MED BOX' AND l_quantity < (SELECT 0.2 * AVG(l_quantity)
+        FROM lineitem WHERE l_partkey = p_partkey)""",
     "Q22": """SELECT cntrycode, COUNT(*), SUM(c_acctbal) FROM (SELECT SUBSTR(c_phone, 1, 2) AS cntrycode, c_acctbal
         FROM customer WHERE SUBSTR(c_phone, 1, 2) IN ('13', '31', '23', '29', '30', '18', '17')
         AND c_acctbal > (SELECT AVG(c_acctbal) FROM customer WHERE c_acctbal > 0.00
