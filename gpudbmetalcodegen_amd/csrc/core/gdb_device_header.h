@@ -471,6 +471,21 @@ GDB_DEV bool gdb_str_eq(const char* s, int width, const unsigned char* __restric
     return l == width || s[l] == 0;
 }
 
+// LIKE '%p' on a fixed-width NUL-padded field: the string (up to its first
+// NUL) ends with the l bytes of p.
+GDB_DEV bool gdb_like_suffix(const char* s, int width, const char* p, int l) {
+    int len = width;
+    for (int i = 0; i < width; ++i)
+        if (s[i] == 0) {
+            len = i;
+            break;
+        }
+    if (l > len) return false;
+    for (int i = 0; i < l; ++i)
+        if (s[len - l + i] != p[i]) return false;
+    return true;
+}
+
 // Wave-aggregated append: every active lane gets a distinct index from the
 // counter; one atomic per wave (the first active lane's), the base is
 // broadcast and each lane adds the number of active lanes below it.

@@ -754,9 +754,17 @@ std::string HipDenseStore::describe() const { return "DenseStore(" + array_ + "[
 
 HipByteFlagProbe::HipByteFlagProbe(std::unique_ptr<HipOperator> child, const std::string& flagsName,
                                    const std::string& keyExpr, const std::string& sizeExpr, bool anti,
-                                   const std::string& var)
-    : HipUnaryOperator(std::move(child)), flags_(flagsName), key_(keyExpr), size_(sizeExpr), var_(var), anti_(anti) {
+                                   const std::string& var, bool keepAll)
+    : HipUnaryOperator(std::move(child)),
+      flags_(flagsName),
+      key_(keyExpr),
+      size_(sizeExpr),
+      var_(var),
+      anti_(anti),
+      keepAll_(keepAll) {
     if (anti_ && !var_.empty()) throw std::runtime_error("HipByteFlagProbe: an anti probe defines no value");
+    if (keepAll_ && (anti_ || var_.empty()))
+        throw std::runtime_error("HipByteFlagProbe: keepAll needs a value variable and no anti");
 }
 
 void HipByteFlagProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
@@ -766,6 +774,10 @@ void HipByteFlagProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         const std::string k = "(u64)(i64)(" + key_ + ")";
         const std::string v = var_.empty() ? cg.uniqueName("_fv") : var_;
         cg.addLine("const u32 " + v + " = " + k + " < nflags_" + flags_ + " ? (u32)" + flags_ + "[" + k + "] : 0u;");
+        if (keepAll_) {
+            consume();
+            return;
+        }
         cg.addIf(anti_ ? v + " == 0u" : v + " != 0u", [&]() { consume(); });
     });
 }
@@ -825,6 +837,25 @@ void HipDenseAtomicAdd::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 
 std::string HipDenseAtomicAdd::describe() const {
     return "DenseAtomicAdd(" + array_ + "[" + index_ + "] += " + value_ + ")";
+}
+
+HipDenseAtomicMin::HipDenseAtomicMin(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                                     const std::string& indexExpr, const std::string& sizeExpr,
+                                     const std::string& valueExpr)
+    : HipUnaryOperator(std::move(child)), array_(arrayName), index_(indexExpr), size_(sizeExpr), value_(valueExpr) {}
+
+void HipDenseAtomicMin::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(array_, "i64", 8, size_, InitKind::Pattern, wordsPattern({kI64Max}));
+    cg.addSymbolParam("nelem_" + array_, size_);
+    child_->produce(cg, [&]() {
+        cg.addLine("{ const u64 _mi = (u64)(i64)(" + index_ + "); if (_mi < nelem_" + array_ + ") gdb_atomic_min_i64(" +
+                   array_ + " + _mi, (i64)(" + value_ + ")); }");
+        consume();
+    });
+}
+
+std::string HipDenseAtomicMin::describe() const {
+    return "DenseAtomicMin(" + array_ + "[" + index_ + "] min= " + value_ + ")";
 }
 
 HipAppendRows::HipAppendRows(std::unique_ptr<HipOperator> child, const std::string& out, const std::string& capExpr)

@@ -301,16 +301,33 @@ private:
 // Semi-/anti-join probe of byte flags built by HipByteFlagBuild: rows whose
 // key has a non-zero flag (anti: a zero flag) are consumed; with `var` set,
 // const u32 <var> holds the flag's value (semi-join form only).
+// keepAll: every row is consumed and <var> is the flag (0 when absent), a
+// CASE WHEN <key in build side> value rather than a join.
 class HipByteFlagProbe : public HipUnaryOperator {
 public:
     HipByteFlagProbe(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,
-                     const std::string& sizeExpr, bool anti, const std::string& var = "");
+                     const std::string& sizeExpr, bool anti, const std::string& var = "", bool keepAll = false);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string flags_, key_, size_, var_;
-    bool anti_;
+    bool anti_, keepAll_;
+};
+
+// Direct-address i64 minimum array[index] = min(array[index], value) with
+// global atomics (the array starts at INT64_MAX); rows whose index is outside
+// [0, size) are skipped. A per-key MIN subquery (TPC-H Q2's minimum supply
+// cost per part).
+class HipDenseAtomicMin : public HipUnaryOperator {
+public:
+    HipDenseAtomicMin(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& indexExpr,
+                      const std::string& sizeExpr, const std::string& valueExpr);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string array_, index_, size_, value_;
 };
 
 // Values an earlier phase left in a buffer, read once at kernel start:

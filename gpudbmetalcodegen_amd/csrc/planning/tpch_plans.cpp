@@ -678,12 +678,208 @@ HipQueryPlan tpchQ17(const PlanOptions& o) {
     return plan;
 }
 
+std::string strSlot(int k) { return "(const unsigned char*)d_str_consts + " + std::to_string(32 * k); }
+
+// Byte flags of the nations of the region named like string constant k.
+void regionNationFlags(HipQueryPlan& plan, const std::string& prefix, int k, const std::string& flags) {
+    {
+        auto scan = makeScan("region", {"r_regionkey", "r_name"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "gdb_str_eq(r_name, 25, " + strSlot(k) + ")");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_" + prefix + "_region", "r_regionkey", "r_bits");
+        auto& ph = appendPhase(plan, prefix + "_region", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("nation", {"n_nationkey", "n_regionkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_" + prefix + "_region", "n_regionkey", "r_bits",
+                                                     false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), flags, "n_nationkey", "n_bits");
+        auto& ph = appendPhase(plan, prefix + "_nations", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+    }
+}
+
+// Q2 — minimum cost supplier (sql/q2.sql): suppliers of the region (string
+// constant 0), parts of size q2_size whose type ends with string constant 1;
+// the minimum supply cost per part over those suppliers (key-indexed atomic
+// minimum), then the partsupp rows at that minimum appended as (supplier,
+// part). The caller attaches supplier, nation and part columns, orders by
+// s_acctbal desc, n_name, s_name, p_partkey and keeps 100.
+HipQueryPlan tpchQ2(const PlanOptions& o) {
+    (void)o;
+    HipQueryPlan plan;
+    plan.name = "Q2";
+    regionNationFlags(plan, "Q2", 0, "d_q2_nation");
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q2_nation", "s_nationkey", "n_bits", false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q2_supp", "s_suppkey", "s_bits");
+        appendPhase(plan, "Q2_suppliers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScanLazy("part", {"p_partkey", "p_size", "p_type"}, {"p_type"});
+        auto sel = std::make_unique<HipSelection>(
+            std::move(scan), "(i64)p_size == q2_size && gdb_like_suffix(p_type, 25, (const char*)d_str_consts + 33, "
+                             "(int)((const unsigned char*)d_str_consts)[32])");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q2_parts", "p_partkey", "p_bits");
+        auto& ph = appendPhase(plan, "Q2_parts", std::move(fl), 256, 4);
+        ph.scalarParams = {"q2_size"};
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    auto probes = [](std::unique_ptr<HipOperator> scan) {
+        auto p = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q2_parts", "ps_partkey", "p_bits", false);
+        return std::make_unique<HipByteFlagProbe>(std::move(p), "d_q2_supp", "ps_suppkey", "s_bits", false);
+    };
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey", "ps_supplycost"});
+        auto mn = std::make_unique<HipDenseAtomicMin>(probes(std::move(scan)), "d_q2_min", "ps_partkey", "p_bits",
+                                                      "ps_supplycost");
+        appendPhase(plan, "Q2_min_cost", std::move(mn), 256, 4);
+    }
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey", "ps_supplycost"});
+        auto look = std::make_unique<HipArrayLookup>(probes(std::move(scan)), "d_q2_min", "ps_partkey", "p_bits", "_mn",
+                                                     -1ll, "i64");
+        auto sel = std::make_unique<HipSelection>(std::move(look), "ps_supplycost == _mn");
+        auto rows = std::make_unique<HipAppendRows>(std::move(sel), "d_q2_rows", "n_partsupp");
+        rows->addField("s_suppkey", "ps_suppkey");
+        rows->addField("p_partkey", "ps_partkey");
+        appendPhase(plan, "Q2_rows", std::move(rows), 256, 4);
+    }
+    plan.requiredScalars = {"q2_size"};
+    plan.requiredSymbols = {"r_bits", "n_bits", "s_bits", "p_bits"};
+    return plan;
+}
+
+// Q7 — volume shipping (sql/q7.sql): nations named like string constants 0
+// and 1 get codes 1 and 2; suppliers, customers and (through the customer)
+// orders carry their nation's code; lineitems shipped in the two years
+// (q7_date_lo .. q7_date_hi inclusive) whose supplier and order codes differ
+// are summed per (supplier code, customer code, year - q7_year0). The caller
+// names the groups and orders them.
+HipQueryPlan tpchQ7(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q7";
+    {
+        auto scan = makeScan("nation", {"n_nationkey", "n_name"});
+        auto cc = std::make_unique<HipComputeExpr>(std::move(scan), "_nc", "u32",
+                                                   "gdb_str_eq(n_name, 25, " + strSlot(0) + ") ? 1u : (gdb_str_eq(n_name, 25, " +
+                                                       strSlot(1) + ") ? 2u : 0u)");
+        auto sel = std::make_unique<HipSelection>(std::move(cc), "_nc != 0u");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q7_nation", "n_nationkey", "n_bits", "_nc");
+        auto& ph = appendPhase(plan, "Q7_nations", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q7_nation", "s_nationkey", "n_bits", false, "_sc");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q7_supp", "s_suppkey", "s_bits", "_sc");
+        appendPhase(plan, "Q7_suppliers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScan("customer", {"c_custkey", "c_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q7_nation", "c_nationkey", "n_bits", false, "_cc");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q7_cust", "c_custkey", "c_bits", "_cc");
+        appendPhase(plan, "Q7_customers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_custkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q7_cust", "o_custkey", "c_bits", false, "_oc");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q7_order", "o_orderkey", "o_bits", "_oc");
+        appendPhase(plan, "Q7_orders", std::move(fl), 256, optInt(o, "items", 8), {4, 8});
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_suppkey", "l_shipdate", "l_extendedprice", "l_discount"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)l_shipdate >= q7_date_lo && (i64)l_shipdate <= q7_date_hi");
+        auto sup = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q7_supp", "l_suppkey", "s_bits", false, "_ls");
+        auto ord = std::make_unique<HipByteFlagProbe>(std::move(sup), "d_q7_order", "l_orderkey", "o_bits", false, "_lo");
+        auto diff = std::make_unique<HipSelection>(std::move(ord), "_ls != _lo");
+        auto agg = std::make_unique<HipKeyedAgg>(
+            std::move(diff), "d_q7_rev", "((int)_ls - 1) * 4 + ((int)_lo - 1) * 2 + (int)((i64)l_shipdate / 10000 - q7_year0)",
+            8);
+        agg->addAggregate("revenue", AggKind::SumI64, kRevenue, 4);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        auto& ph = appendPhase(plan, "Q7_lineitem", std::move(agg), 256, optInt(o, "items", 4), {4, 8});
+        ph.scalarParams = {"q7_date_lo", "q7_date_hi", "q7_year0"};
+    }
+    plan.requiredScalars = {"q7_date_lo", "q7_date_hi", "q7_year0"};
+    plan.requiredSymbols = {"n_bits", "s_bits", "c_bits", "o_bits"};
+    return plan;
+}
+
+// Q8 — national market share (sql/q8.sql): customers of the region (string
+// constant 0) through their orders of the two years (year code 1/2),
+// suppliers marked when their nation is string constant 1, parts of type
+// string constant 2; per year the volume and the marked nation's volume,
+// mkt_share = marked / volume (exact to 6 digits).
+HipQueryPlan tpchQ8(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q8";
+    regionNationFlags(plan, "Q8", 0, "d_q8_amer");
+    {
+        auto scan = makeScan("nation", {"n_nationkey", "n_name"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "gdb_str_eq(n_name, 25, " + strSlot(1) + ")");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q8_mark", "n_nationkey", "n_bits");
+        auto& ph = appendPhase(plan, "Q8_marked_nation", std::move(fl), 64, 1, {1});
+        ph.launchBound = 64;
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_nationkey"});
+        auto mk = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q8_mark", "s_nationkey", "n_bits", false, "_mk",
+                                                     true);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(mk), "d_q8_supp", "s_suppkey", "s_bits", "1u + _mk");
+        appendPhase(plan, "Q8_suppliers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScanLazy("part", {"p_partkey", "p_type"}, {"p_type"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "gdb_str_eq(p_type, 25, " + strSlot(2) + ")");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q8_parts", "p_partkey", "p_bits");
+        auto& ph = appendPhase(plan, "Q8_parts", std::move(fl), 256, 4);
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("customer", {"c_custkey", "c_nationkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q8_amer", "c_nationkey", "n_bits", false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q8_cust", "c_custkey", "c_bits");
+        appendPhase(plan, "Q8_customers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_custkey", "o_orderdate"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "(i64)o_orderdate >= q8_date_lo && (i64)o_orderdate <= q8_date_hi");
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(sel), "d_q8_cust", "o_custkey", "c_bits", false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(in), "d_q8_order", "o_orderkey", "o_bits",
+                                                     "(i64)o_orderdate / 10000 - q8_year0 + 1");
+        auto& ph = appendPhase(plan, "Q8_orders", std::move(fl), 256, 4);
+        ph.scalarParams = {"q8_date_lo", "q8_date_hi", "q8_year0"};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_suppkey", "l_orderkey", "l_extendedprice", "l_discount"});
+        auto part = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q8_parts", "l_partkey", "p_bits", false);
+        auto ord = std::make_unique<HipByteFlagProbe>(std::move(part), "d_q8_order", "l_orderkey", "o_bits", false, "_oy");
+        auto sup = std::make_unique<HipByteFlagProbe>(std::move(ord), "d_q8_supp", "l_suppkey", "s_bits", false, "_sv");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(sup), "d_q8_vol", "(int)_oy - 1", 2);
+        agg->addAggregate("volume", AggKind::SumI64, kRevenue, 4);
+        agg->addAggregate("marked_volume", AggKind::SumI64, std::string(kRevenue) + " * (i64)(_sv == 2u)", 4);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        agg->addAverage("mkt_share", "marked_volume", "volume");
+        appendPhase(plan, "Q8_lineitem", std::move(agg), 256, optInt(o, "items", 4), {4, 8});
+    }
+    plan.requiredScalars = {"q8_date_lo", "q8_date_hi", "q8_year0"};
+    plan.requiredSymbols = {"r_bits", "n_bits", "s_bits", "c_bits", "o_bits", "p_bits"};
+    return plan;
+}
+
 bool tpchSchema(const PlanOptions& o) { return opt(o, "schema", "") == "tpch"; }
 
 }  // namespace
 
 std::vector<std::string> tpchPlanNames() {
-    return {"Q4", "Q5", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q22"};
+    return {"Q2", "Q4", "Q5", "Q7", "Q8", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q22"};
 }
 
 std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOptions& opts) {
@@ -697,6 +893,9 @@ std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOpt
     if (name == "Q10") return tpchQ10(opts);
     if (name == "Q19") return tpchQ19(opts);
     if (name == "Q11") return tpchQ11(opts);
+    if (name == "Q2") return tpchQ2(opts);
+    if (name == "Q7") return tpchQ7(opts);
+    if (name == "Q8") return tpchQ8(opts);
     if (name == "Q15") return tpchQ15(opts);
     if (name == "Q17") return tpchQ17(opts);
     if (name == "Q12") return tpchQ12(opts);

@@ -31,7 +31,8 @@ from . import engine as E
 
 INT, FLOAT, DATE, CHAR1, CHAR_FIXED = 0, 1, 2, 3, 4
 TABLES = ["lineitem", "orders", "customer", "supplier", "part", "partsupp", "nation", "region"]
-QUERIES = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q22"]
+QUERIES = ["Q1", "Q2", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18",
+           "Q19", "Q22"]
 CHECK_ABS_TOL = 1e-2  # the reference's --check defaults (codegen_main.cpp:47-48)
 CHECK_REL_TOL = 1e-4
 
@@ -180,6 +181,11 @@ PARAMS = {
     "Q11": {"strings": (b"GERMANY",), "q11_frac_den": 10000},  # value > total * 0.0001
     "Q15": {"q15_date_lo": ymd(1996, 1, 1), "q15_date_hi": shift(ymd(1996, 1, 1), months=3)},
     "Q17": {"strings": (b"Brand#23", b"MED BOX")},
+    "Q2": {"strings": (b"EUROPE", b"BRASS"), "q2_size": 15},
+    "Q7": {"strings": (b"FRANCE", b"GERMANY"), "q7_date_lo": ymd(1995, 1, 1), "q7_date_hi": ymd(1996, 12, 31),
+           "q7_year0": 1995},
+    "Q8": {"strings": (b"AMERICA", b"BRAZIL", b"ECONOMY ANODIZED STEEL"), "q8_date_lo": ymd(1995, 1, 1),
+           "q8_date_hi": ymd(1996, 12, 31), "q8_year0": 1995},
     "Q10": {"q10_date_lo": ymd(1993, 10, 1), "q10_date_hi": shift(ymd(1993, 10, 1), months=3), "q10_flag": ord("R")},
     "Q19": {"q19_mode1": mode_code("AIR"), "q19_mode2": mode_code("REG AIR"),
             "strings": (b"Brand#12", b"Brand#23", b"Brand#34", b"SM CASE", b"SM BOX", b"SM PACK", b"SM PKG",
@@ -196,7 +202,9 @@ SCANNED = {"Q1": ["lineitem"], "Q3": ["customer", "orders", "lineitem"], "Q4": [
            "Q5": ["region", "nation", "supplier", "customer", "orders", "lineitem"],
            "Q10": ["orders", "lineitem", "customer"], "Q19": ["part", "lineitem"],
            "Q11": ["nation", "supplier", "partsupp", "part"], "Q15": ["lineitem", "supplier"],
-           "Q17": ["part", "lineitem"]}
+           "Q17": ["part", "lineitem"], "Q2": ["region", "nation", "supplier", "part", "partsupp"],
+           "Q7": ["nation", "supplier", "customer", "orders", "lineitem"],
+           "Q8": ["region", "nation", "supplier", "part", "customer", "orders", "lineitem"]}
 # Output columns in the SQL select list's order.
 SELECT = {
     "Q1": ["l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price", "sum_charge", "avg_qty",
@@ -215,6 +223,9 @@ SELECT = {
     "Q11": ["ps_partkey", "value"],
     "Q15": ["s_suppkey", "s_name", "s_address", "s_phone", "total_revenue"],
     "Q17": ["avg_yearly"],
+    "Q2": ["s_acctbal", "s_name", "n_name", "p_partkey", "p_mfgr", "s_address", "s_phone", "s_comment"],
+    "Q7": ["supp_nation", "cust_nation", "l_year", "revenue"],
+    "Q8": ["o_year", "mkt_share"],
 }
 
 
@@ -335,7 +346,7 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
             ctx.free(strings_ptr)
     cols = {k: [int(x) if not isinstance(x, str) else x for x in v] for k, v in res.columns.items()}
     fmts = dict(res.formats)
-    _finish(query, db, cols, fmts)
+    _finish(query, db, cols, fmts, prm)
     out = {name: cols[name] for name in SELECT[query]}
     run_ = TpchRun(query, out, {k: fmts.get(k, (0, False)) for k in out}, t, plan, ex, res)
     run_.canonical = canonical(run_.columns, run_.formats)
@@ -359,10 +370,10 @@ def _text(arr_row) -> str:
 
 # Queries whose result the host finishes (attached columns, reordering, a
 # final division); the others are the engine's result as collected.
-HOST_FINISH = {"Q3", "Q5", "Q10", "Q15", "Q17", "Q18"}
+HOST_FINISH = {"Q2", "Q3", "Q5", "Q7", "Q8", "Q10", "Q15", "Q17", "Q18"}
 
 
-def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
+def _finish(query: str, db: TpchDB, cols: dict, fmts: dict, prm: dict) -> None:
     """Host post-processing on the few result rows (the reference's CPU
     post-processing in runCodegenQuery): attach o_shippriority (Q3) and c_name
     (Q18) by key."""
@@ -391,6 +402,32 @@ def _finish(query: str, db: TpchDB, cols: dict, fmts: dict) -> None:
         cols["c_address"] = [_text(c["c_address"][i]) for i in idx]
         cols["c_phone"] = [_text(c["c_phone"][i]) for i in idx]
         cols["c_comment"] = [_text(c["c_comment"][i]) for i in idx]
+    elif query == "Q2":
+        sp, n, pt = db.host["supplier"], db.host["nation"], db.host["part"]
+        nat = {int(k): _text(nm) for k, nm in zip(n["n_nationkey"], n["n_name"])}
+        spos = {int(k): i for i, k in enumerate(sp["s_suppkey"].tolist())}
+        ppos = {int(k): i for i, k in enumerate(pt["p_partkey"].tolist())}
+        rows = []
+        for sk, pk in zip(cols["s_suppkey"], cols["p_partkey"]):
+            i, j = spos[sk], ppos[pk]
+            rows.append((int(sp["s_acctbal"][i]), _text(sp["s_name"][i]), nat[int(sp["s_nationkey"][i])], pk,
+                         _text(pt["p_mfgr"][j]), _text(sp["s_address"][i]), _text(sp["s_phone"][i]),
+                         _text(sp["s_comment"][i])))
+        rows.sort(key=lambda r: (-r[0], r[2], r[1], r[3]))
+        rows = rows[:100]
+        for c, name in enumerate(SELECT["Q2"]):
+            cols[name] = [r[c] for r in rows]
+        fmts["s_acctbal"] = (2, False)
+    elif query == "Q7":
+        names = [w.decode() for w in prm["strings"]]
+        y0 = prm["q7_year0"]
+        rows = sorted((names[b // 4], names[(b // 2) % 2], y0 + b % 2, r) for b, r in zip(cols["bucket"], cols["revenue"]))
+        for c, name in enumerate(SELECT["Q7"]):
+            cols[name] = [r[c] for r in rows]
+        fmts["l_year"] = (0, False)
+    elif query == "Q8":
+        cols["o_year"] = [prm["q8_year0"] + b for b in cols["bucket"]]
+        fmts["o_year"] = (0, False)
     elif query == "Q15":
         sp = db.host["supplier"]
         sub = np.flatnonzero(np.isin(sp["s_suppkey"], cols["s_suppkey"]))

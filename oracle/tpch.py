@@ -342,6 +342,88 @@ def q17(T: dict, brand: bytes, container: bytes) -> tuple:
     return {"avg_yearly": [tot * 1_000_000 // 7] if n else []}, {"avg_yearly": (8, False)}
 
 
+def q2(T: dict, region: bytes, suffix: bytes, size: int, k: int = 100) -> tuple:
+    """sql/q2.sql; ordered s_acctbal desc, n_name, s_name, p_partkey."""
+    R, N, S, P, PS = (T[t] for t in ("region", "nation", "supplier", "part", "partsupp"))
+    rk = {int(x) for x, nm in zip(R["r_regionkey"], R["r_name"]) if _s(nm) == region}
+    nat = {int(x): _s(nm).decode() for x, nm, r in zip(N["n_nationkey"], N["n_name"], N["n_regionkey"]) if int(r) in rk}
+    spos = {int(x): i for i, (x, n) in enumerate(zip(S["s_suppkey"], S["s_nationkey"])) if int(n) in nat}
+    parts = {int(x): i for i, (x, sz, t) in enumerate(zip(P["p_partkey"], P["p_size"], P["p_type"]))
+             if int(sz) == size and _s(t).endswith(suffix)}
+    mn: dict = {}
+    cand = []
+    for pk, sk, c in zip(PS["ps_partkey"].tolist(), PS["ps_suppkey"].tolist(), PS["ps_supplycost"].tolist()):
+        if pk in parts and sk in spos:
+            mn[pk] = min(mn.get(pk, c), c)
+            cand.append((pk, sk, c))
+    rows = []
+    for pk, sk, c in cand:
+        if c == mn[pk]:
+            i, j = spos[sk], parts[pk]
+            rows.append((int(S["s_acctbal"][i]), _s(S["s_name"][i]).decode(), nat[int(S["s_nationkey"][i])], pk,
+                         _s(P["p_mfgr"][j]).decode(), _s(S["s_address"][i]).decode(), _s(S["s_phone"][i]).decode(),
+                         _s(S["s_comment"][i]).decode()))
+    rows.sort(key=lambda r: (-r[0], r[2], r[1], r[3]))
+    rows = rows[:k]
+    names = ["s_acctbal", "s_name", "n_name", "p_partkey", "p_mfgr", "s_address", "s_phone", "s_comment"]
+    fm = {c: (0, False) for c in names}
+    fm["s_acctbal"] = (2, False)
+    return {c: [r[i] for r in rows] for i, c in enumerate(names)}, fm
+
+
+def q7(T: dict, n1: bytes, n2: bytes, lo: int, hi: int) -> tuple:
+    """sql/q7.sql; ordered supp_nation, cust_nation, l_year."""
+    N, S, C, O, L = (T[t] for t in ("nation", "supplier", "customer", "orders", "lineitem"))
+    code = {}
+    for x, nm in zip(N["n_nationkey"], N["n_name"]):
+        if _s(nm) in (n1, n2):
+            code[int(x)] = _s(nm).decode()
+    snat = {int(x): code[int(n)] for x, n in zip(S["s_suppkey"], S["s_nationkey"]) if int(n) in code}
+    cnat = {int(x): code[int(n)] for x, n in zip(C["c_custkey"], C["c_nationkey"]) if int(n) in code}
+    onat = {int(x): cnat[int(c)] for x, c in zip(O["o_orderkey"], O["o_custkey"]) if int(c) in cnat}
+    m = (L["l_shipdate"] >= lo) & (L["l_shipdate"] <= hi)
+    rev: dict = {}
+    for ok, sk, d8, p, d in zip(L["l_orderkey"][m].tolist(), L["l_suppkey"][m].tolist(), L["l_shipdate"][m].tolist(),
+                                L["l_extendedprice"][m].tolist(), L["l_discount"][m].tolist()):
+        a, b = snat.get(sk), onat.get(ok)
+        if a is not None and b is not None and a != b:
+            key = (a, b, d8 // 10000)
+            rev[key] = rev.get(key, 0) + p * (100 - d)
+    rows = sorted(rev.items())
+    cols = {"supp_nation": [r[0][0] for r in rows], "cust_nation": [r[0][1] for r in rows],
+            "l_year": [r[0][2] for r in rows], "revenue": [r[1] for r in rows]}
+    fm = {c: (0, False) for c in cols}
+    fm["revenue"] = (4, False)
+    return cols, fm
+
+
+def q8(T: dict, region: bytes, nation: bytes, ptype: bytes, lo: int, hi: int) -> tuple:
+    """sql/q8.sql; mkt_share = trunc(marked * 10^6 / volume), 6 digits."""
+    R, N, S, P, C, O, L = (T[t] for t in ("region", "nation", "supplier", "part", "customer", "orders", "lineitem"))
+    rk = {int(x) for x, nm in zip(R["r_regionkey"], R["r_name"]) if _s(nm) == region}
+    amer = {int(x) for x, r in zip(N["n_nationkey"], N["n_regionkey"]) if int(r) in rk}
+    mark = {int(x) for x, nm in zip(N["n_nationkey"], N["n_name"]) if _s(nm) == nation}
+    smark = {int(x): int(n) in mark for x, n in zip(S["s_suppkey"], S["s_nationkey"])}
+    parts = {int(x) for x, t in zip(P["p_partkey"], P["p_type"]) if _s(t) == ptype}
+    cust = {int(x) for x, n in zip(C["c_custkey"], C["c_nationkey"]) if int(n) in amer}
+    m = (O["o_orderdate"] >= lo) & (O["o_orderdate"] <= hi)
+    oyear = {int(x): int(d) // 10000 for x, c, d in zip(O["o_orderkey"][m], O["o_custkey"][m], O["o_orderdate"][m])
+             if int(c) in cust}
+    vol: dict = {}
+    for pk, sk, ok, p, d in zip(L["l_partkey"].tolist(), L["l_suppkey"].tolist(), L["l_orderkey"].tolist(),
+                                L["l_extendedprice"].tolist(), L["l_discount"].tolist()):
+        y = oyear.get(ok)
+        if y is None or pk not in parts or sk not in smark:
+            continue
+        e = vol.setdefault(y, [0, 0])
+        e[0] += p * (100 - d)
+        if smark[sk]:
+            e[1] += p * (100 - d)
+    ys = sorted(vol)
+    return {"o_year": ys, "mkt_share": [_avg(vol[y][1], vol[y][0]) for y in ys]}, {"o_year": (0, False),
+                                                                                  "mkt_share": (6, False)}
+
+
 def run(T: dict, q: str, P: dict) -> tuple:
     """Query q with the constants P (the names of the engine's scalars:
     q1_cutoff, q3_seg, ... plus words / prefix for Q13 / Q14)."""
@@ -367,6 +449,12 @@ def run(T: dict, q: str, P: dict) -> tuple:
         return q5(T, P["strings"][0], P["q5_date_lo"], P["q5_date_hi"])
     if q == "Q10":
         return q10(T, P["q10_date_lo"], P["q10_date_hi"], P["q10_flag"])
+    if q == "Q2":
+        return q2(T, P["strings"][0], P["strings"][1], P["q2_size"])
+    if q == "Q7":
+        return q7(T, P["strings"][0], P["strings"][1], P["q7_date_lo"], P["q7_date_hi"])
+    if q == "Q8":
+        return q8(T, P["strings"][0], P["strings"][1], P["strings"][2], P["q8_date_lo"], P["q8_date_hi"])
     if q == "Q11":
         return q11(T, P["strings"][0], P["q11_frac_den"])
     if q == "Q15":
