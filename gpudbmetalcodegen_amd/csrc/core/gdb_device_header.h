@@ -270,6 +270,40 @@ GDB_DEV i64 gdb_ht_find(const u64* __restrict__ tab, u32 slot_words, u32 bits, u
     return -1;
 }
 
+// 64-bit-key table: slots of [key, payload] u64 words, empty key all ones
+// (keys must not be ~0). Home slot: the high bits of key * 2^64/phi (64-bit
+// Fibonacci hashing). The reference folds 64-bit keys to 32 bits with
+// k ^ (k >> 32) (metal_param_binding.h:20-26); for a composite key
+// (partkey << 32 | suppkey) that fold is partkey ^ suppkey, which clusters
+// the pairs of structured keys into long probe chains, so it is not used.
+// Insert claims the key word with a CAS and then writes the payload (a later
+// phase reads it); a key already present keeps its first payload. Returns 1
+// inserted, 0 present, -1 when max_probe slots are full.
+GDB_DEV u64 gdb_hash_slot64(u64 key, u32 bits) { return (key * 0x9E3779B97F4A7C15ull) >> (64u - bits); }
+GDB_DEV int gdb_ht64_insert(u64* __restrict__ tab, u32 bits, u64 key, u64 payload, u32 max_probe) {
+    const u64 mask = (1ull << bits) - 1ull;
+    u64 s = gdb_hash_slot64(key, bits);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        u64 expected = ~0ull;
+        if (gdb_cas_u64(tab + 2 * s, expected, key)) {
+            tab[2 * s + 1] = payload;
+            return 1;
+        }
+        if (expected == key) return 0;
+    }
+    return -1;
+}
+GDB_DEV i64 gdb_ht64_find(const u64* __restrict__ tab, u32 bits, u64 key, u32 max_probe) {
+    const u64 mask = (1ull << bits) - 1ull;
+    u64 s = gdb_hash_slot64(key, bits);
+    for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        const u64 k = tab[2 * s];
+        if (k == key) return (i64)s;
+        if (k == ~0ull) return -1;
+    }
+    return -1;
+}
+
 // Group-by slot: [key | count<<32, sum, min, max]. Merge one partial group
 // (cnt rows with the given sum/min/max) into a global table. Returns 1 when a
 // new key was inserted, 0 when merged, -1 on overflow.

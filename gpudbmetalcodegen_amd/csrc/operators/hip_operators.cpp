@@ -839,23 +839,97 @@ std::string HipDenseAtomicAdd::describe() const {
     return "DenseAtomicAdd(" + array_ + "[" + index_ + "] += " + value_ + ")";
 }
 
-HipDenseAtomicMin::HipDenseAtomicMin(std::unique_ptr<HipOperator> child, const std::string& arrayName,
-                                     const std::string& indexExpr, const std::string& sizeExpr,
-                                     const std::string& valueExpr)
-    : HipUnaryOperator(std::move(child)), array_(arrayName), index_(indexExpr), size_(sizeExpr), value_(valueExpr) {}
+HipHash64Build::HipHash64Build(std::unique_ptr<HipOperator> child, const std::string& table,
+                               const std::string& keyExpr, const std::string& payloadExpr, const std::string& slotsExpr)
+    : HipUnaryOperator(std::move(child)), table_(table), key_(keyExpr), payload_(payloadExpr), slots_(slotsExpr) {}
 
-void HipDenseAtomicMin::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    cg.addBufferParam(array_, "i64", 8, size_, InitKind::Pattern, wordsPattern({kI64Max}));
-    cg.addSymbolParam("nelem_" + array_, size_);
+void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(table_, "u64", 8, slots_ + " * 2", InitKind::Pattern, wordsPattern({~0ull, 0ull}), false, true);
+    cg.addSymbolParam("nslots_" + table_, slots_);
+    const std::string st = table_ + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    cg.addLine("const u32 _bits_" + table_ + " = (u32)__builtin_ctzll(nslots_" + table_ + ");");
+    cg.addLine("const u32 _maxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
+               " : 65536ull);");
+    cg.addLine("u32 _ins_" + table_ + " = 0, _ovf_" + table_ + " = 0;");
     child_->produce(cg, [&]() {
-        cg.addLine("{ const u64 _mi = (u64)(i64)(" + index_ + "); if (_mi < nelem_" + array_ + ") gdb_atomic_min_i64(" +
-                   array_ + " + _mi, (i64)(" + value_ + ")); }");
+        cg.addBlock("", [&]() {
+            cg.addLine("const int _r = gdb_ht64_insert(" + table_ + ", _bits_" + table_ + ", (u64)(" + key_ + "), (u64)(" +
+                       payload_ + "), _maxp_" + table_ + ");");
+            cg.addLine("_ins_" + table_ + " += _r == 1 ? 1u : 0u;");
+            cg.addLine("_ovf_" + table_ + " |= _r < 0 ? 1u : 0u;");
+        });
         consume();
+    });
+    emitBlockStatus(cg, st, {{"_ins_" + table_, 2, false}, {"_ovf_" + table_, 0, true}});
+}
+
+std::string HipHash64Build::describe() const {
+    return "Hash64Build(" + table_ + ", key=" + key_ + ", payload=" + payload_ + ", slots=" + slots_ + ")";
+}
+
+HipHash64Probe::HipHash64Probe(std::unique_ptr<HipOperator> child, const std::string& table,
+                               const std::string& keyExpr, const std::string& slotsExpr, const std::string& slotVar)
+    : HipUnaryOperator(std::move(child)), table_(table), key_(keyExpr), slots_(slotsExpr), slotVar_(slotVar) {}
+
+void HipHash64Probe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(table_, "u64", 8, slots_ + " * 2", InitKind::None, {}, false, true);
+    cg.addSymbolParam("nslots_" + table_, slots_);
+    cg.addLine("const u32 _pbits_" + table_ + " = (u32)__builtin_ctzll(nslots_" + table_ + ");");
+    cg.addLine("const u32 _pmaxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
+               " : 65536ull);");
+    child_->produce(cg, [&]() {
+        cg.addLine("const i64 " + slotVar_ + " = gdb_ht64_find(" + table_ + ", _pbits_" + table_ + ", (u64)(" + key_ +
+                   "), _pmaxp_" + table_ + ");");
+        cg.addIf(slotVar_ + " >= 0", [&]() {
+            cg.addLine("const u64 " + slotVar_ + "_payload = " + table_ + "[(u64)" + slotVar_ + " * 2 + 1];");
+            consume();
+        });
     });
 }
 
-std::string HipDenseAtomicMin::describe() const {
-    return "DenseAtomicMin(" + array_ + "[" + index_ + "] min= " + value_ + ")";
+std::string HipHash64Probe::describe() const { return "Hash64Probe(" + table_ + ", key=" + key_ + ")"; }
+
+HipDenseAtomicMinMax::HipDenseAtomicMinMax(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                                           const std::string& indexExpr, const std::string& sizeExpr,
+                                           const std::string& valueExpr, bool isMax)
+    : HipUnaryOperator(std::move(child)),
+      array_(arrayName),
+      index_(indexExpr),
+      size_(sizeExpr),
+      value_(valueExpr),
+      max_(isMax) {}
+
+void HipDenseAtomicMinMax::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const uint64_t neutral = max_ ? 0x8000000000000000ull : kI64Max;
+    cg.addBufferParam(array_, "i64", 8, size_, InitKind::Pattern, wordsPattern({neutral}));
+    cg.addSymbolParam("nelem_" + array_, size_);
+    const std::string PK = "_pk_" + array_, PV = "_pv_" + array_, N = "nelem_" + array_;
+    const std::string op = max_ ? "gdb_max64" : "gdb_min64";
+    const std::string atomic = max_ ? "gdb_atomic_max_i64" : "gdb_atomic_min_i64";
+    cg.addLine("u64 " + PK + " = ~0ull;");
+    cg.addLine("i64 " + PV + " = " + std::string(max_ ? "GDB_I64_MIN" : "GDB_I64_MAX") + ";");
+    child_->produce(cg, [&]() {
+        cg.addBlock("", [&]() {
+            cg.addLine("const u64 _mi = (u64)(i64)(" + index_ + ");");
+            cg.addLine("const i64 _mv = (i64)(" + value_ + ");");
+            cg.addBlock("if (_mi == " + PK + ")", [&]() { cg.addLine(PV + " = " + op + "(" + PV + ", _mv);"); }, " else {");
+            {
+                IndentGuard g(cg);
+                cg.addLine("if (" + PK + " < " + N + ") " + atomic + "(" + array_ + " + " + PK + ", " + PV + ");");
+                cg.addLine(PK + " = _mi;");
+                cg.addLine(PV + " = _mv;");
+            }
+            cg.addLine("}");
+        });
+        consume();
+    });
+    cg.addLine("if (" + PK + " < " + N + ") " + atomic + "(" + array_ + " + " + PK + ", " + PV + ");");
+}
+
+std::string HipDenseAtomicMinMax::describe() const {
+    return "DenseAtomic" + std::string(max_ ? "Max(" : "Min(") + array_ + "[" + index_ + "], " + value_ + ")";
 }
 
 HipAppendRows::HipAppendRows(std::unique_ptr<HipOperator> child, const std::string& out, const std::string& capExpr)

@@ -197,6 +197,35 @@ private:
     int slotWords_;
 };
 
+// Open-addressing table with 64-bit keys (composite keys such as
+// (ps_partkey, ps_suppkey) packed as partkey << 32 | suppkey) and a u64
+// payload: slots [key, payload], empty key all ones. Build inserts each key
+// once (first payload wins; overflow sets the phase status); probe consumes
+// rows whose key is present with <slotVar> (i64 slot) and <slotVar>_payload.
+// The reference hashes 64-bit keys by folding them the same way
+// (metal_param_binding.h:20-26); its tables are host-built.
+class HipHash64Build : public HipUnaryOperator {
+public:
+    HipHash64Build(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                   const std::string& payloadExpr, const std::string& slotsExpr);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, key_, payload_, slots_;
+};
+
+class HipHash64Probe : public HipUnaryOperator {
+public:
+    HipHash64Probe(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
+                   const std::string& slotsExpr, const std::string& slotVar);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, key_, slots_, slotVar_;
+};
+
 // Atomic add of an i64 value into word `word` of a hash-table slot
 // (the group-by-join-key aggregate; counterpart of MetalAtomicAgg,
 // metal_operators.cpp:525-533, with the slot as the bucket).
@@ -315,19 +344,23 @@ private:
     bool anti_, keepAll_;
 };
 
-// Direct-address i64 minimum array[index] = min(array[index], value) with
-// global atomics (the array starts at INT64_MAX); rows whose index is outside
-// [0, size) are skipped. A per-key MIN subquery (TPC-H Q2's minimum supply
-// cost per part).
-class HipDenseAtomicMin : public HipUnaryOperator {
+// Direct-address i64 minimum (or maximum) array[index] = min(array[index],
+// value) with global atomics (the array starts at INT64_MAX, or INT64_MIN);
+// rows whose index is outside [0, size) are skipped; runs of equal indices in
+// a thread's rows combine before the atomic, as in HipDenseAtomicAdd. A
+// per-key MIN/MAX subquery (TPC-H Q2's minimum supply cost per part, Q21's
+// smallest and largest supplier per order).
+class HipDenseAtomicMinMax : public HipUnaryOperator {
 public:
-    HipDenseAtomicMin(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& indexExpr,
-                      const std::string& sizeExpr, const std::string& valueExpr);
+    HipDenseAtomicMinMax(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                         const std::string& indexExpr, const std::string& sizeExpr, const std::string& valueExpr,
+                         bool isMax);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string array_, index_, size_, value_;
+    bool max_;
 };
 
 // Values an earlier phase left in a buffer, read once at kernel start:

@@ -733,8 +733,8 @@ HipQueryPlan tpchQ2(const PlanOptions& o) {
     };
     {
         auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey", "ps_supplycost"});
-        auto mn = std::make_unique<HipDenseAtomicMin>(probes(std::move(scan)), "d_q2_min", "ps_partkey", "p_bits",
-                                                      "ps_supplycost");
+        auto mn = std::make_unique<HipDenseAtomicMinMax>(probes(std::move(scan)), "d_q2_min", "ps_partkey", "p_bits",
+                                                         "ps_supplycost", false);
         appendPhase(plan, "Q2_min_cost", std::move(mn), 256, 4);
     }
     {
@@ -874,12 +874,197 @@ HipQueryPlan tpchQ8(const PlanOptions& o) {
     return plan;
 }
 
+// Q21 — suppliers who kept orders waiting (sql/q21.sql). Per order the
+// smallest and largest l_suppkey of all its lines and of its late lines
+// (receipt > commit) reduce the EXISTS / NOT EXISTS subqueries to
+// comparisons: another supplier exists when min != s or max != s; no other
+// late supplier when late min == late max == s. A late line of a supplier of
+// the nation (string constant 0) on an order with status q21_status passing
+// both counts for its supplier; the caller attaches s_name, orders by
+// numwait desc, s_name and keeps 100.
+HipQueryPlan tpchQ21(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q21";
+    nationSupplierFlags(plan, "Q21", "d_q21_supp");
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_orderstatus"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)o_orderstatus == q21_status");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q21_orders", "o_orderkey", "o_bits");
+        auto& ph = appendPhase(plan, "Q21_orders", std::move(fl), 256, 8, {8});
+        ph.scalarParams = {"q21_status"};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_suppkey", "l_receiptdate", "l_commitdate"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipDenseAtomicMinMax>(
+            std::move(scan), "d_q21_min", "l_orderkey", "o_bits", "l_suppkey", false);
+        cur = std::make_unique<HipDenseAtomicMinMax>(std::move(cur), "d_q21_max", "l_orderkey", "o_bits", "l_suppkey", true);
+        cur = std::make_unique<HipSelection>(std::move(cur), "l_receiptdate > l_commitdate");
+        cur = std::make_unique<HipDenseAtomicMinMax>(std::move(cur), "d_q21_lmin", "l_orderkey", "o_bits", "l_suppkey",
+                                                     false);
+        cur = std::make_unique<HipDenseAtomicMinMax>(std::move(cur), "d_q21_lmax", "l_orderkey", "o_bits", "l_suppkey",
+                                                     true);
+        appendPhase(plan, "Q21_order_suppliers", std::move(cur), 256, optInt(o, "items", 8), {4, 8});
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_orderkey", "l_suppkey", "l_receiptdate", "l_commitdate"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipSelection>(std::move(scan), "l_receiptdate > l_commitdate");
+        cur = std::make_unique<HipByteFlagProbe>(std::move(cur), "d_q21_supp", "l_suppkey", "s_bits", false);
+        cur = std::make_unique<HipByteFlagProbe>(std::move(cur), "d_q21_orders", "l_orderkey", "o_bits", false);
+        for (const char* a : {"min", "max", "lmin", "lmax"})
+            cur = std::make_unique<HipArrayLookup>(std::move(cur), std::string("d_q21_") + a, "l_orderkey", "o_bits",
+                                                   std::string("_") + a, -1ll, "i64");
+        cur = std::make_unique<HipSelection>(
+            std::move(cur), "(_min != (i64)l_suppkey || _max != (i64)l_suppkey) && _lmin == (i64)l_suppkey && "
+                            "_lmax == (i64)l_suppkey");
+        cur = std::make_unique<HipDenseAtomicAdd>(std::move(cur), "d_q21_wait", "l_suppkey", "s_bits", "1");
+        appendPhase(plan, "Q21_waiting", std::move(cur), 256, optInt(o, "items", 8), {4, 8});
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey"});
+        auto look = std::make_unique<HipArrayLookup>(std::move(scan), "d_q21_wait", "s_suppkey", "s_bits", "_nw", 0ll,
+                                                     "i64");
+        auto rows = std::make_unique<HipAppendRows>(std::move(look), "d_q21_rows", "n_supplier");
+        rows->addField("s_suppkey", "s_suppkey");
+        rows->addField("numwait", "_nw");
+        appendPhase(plan, "Q21_suppliers_out", std::move(rows), 256, 4);
+    }
+    plan.requiredScalars = {"q21_status"};
+    plan.requiredSymbols = {"n_bits", "s_bits", "o_bits"};
+    return plan;
+}
+
+// The (partkey, suppkey) pair of a partsupp or lineitem row as one 64-bit key.
+std::string pairKey(const std::string& part, const std::string& supp) {
+    return "(((u64)(u32)" + part + " << 32) | (u64)(u32)" + supp + ")";
+}
+
+// Q9 — product type profit (sql/q9.sql): parts whose name contains word 1 of
+// the LIKE words (LIKE '%w%': w2 empty); their partsupp rows in a 64-bit-key
+// hash table (partkey, suppkey) -> supply cost; suppliers carry nation key +
+// 1, orders their year - q9_year0 + 1; lineitems of those parts sum
+// price * (1 - discount) - cost * quantity per (nation, year). The caller
+// names the groups and orders them nation asc, year desc.
+HipQueryPlan tpchQ9(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q9";
+    {
+        auto scan = makeScan("part", {"p_partkey", "p_name"});
+        auto like = std::make_unique<HipLikeFilter>(std::move(scan), "p_name", 55, "d_like_words", "like_l1", "like_l2",
+                                                    false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(like), "d_q9_parts", "p_partkey", "p_bits");
+        auto& ph = appendPhase(plan, "Q9_parts", std::move(fl), 256, 2, {2});
+        ph.launchBound = 256;
+        ph.scalarParams = {"like_l1", "like_l2"};
+        ph.externalBuffers = {{"d_like_words", "u8"}};
+    }
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey", "ps_supplycost"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q9_parts", "ps_partkey", "p_bits", false);
+        auto ht = std::make_unique<HipHash64Build>(std::move(in), "d_q9_ps", pairKey("ps_partkey", "ps_suppkey"),
+                                                   "ps_supplycost", "q9_slots");
+        appendPhase(plan, "Q9_partsupp", std::move(ht), 256, 4);
+    }
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_nationkey"});
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(scan), "d_q9_supp", "s_suppkey", "s_bits", "s_nationkey + 1");
+        appendPhase(plan, "Q9_suppliers", std::move(fl), 256, 4);
+    }
+    {
+        auto scan = makeScan("orders", {"o_orderkey", "o_orderdate"});
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(scan), "d_q9_year", "o_orderkey", "o_bits",
+                                                     "(i64)o_orderdate / 10000 - q9_year0 + 1");
+        auto& ph = appendPhase(plan, "Q9_orders", std::move(fl), 256, optInt(o, "items", 8), {4, 8});
+        ph.scalarParams = {"q9_year0"};
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_suppkey", "l_orderkey", "l_quantity", "l_extendedprice",
+                                          "l_discount"});
+        std::unique_ptr<HipOperator> cur =
+            std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q9_parts", "l_partkey", "p_bits", false);
+        cur = std::make_unique<HipHash64Probe>(std::move(cur), "d_q9_ps", pairKey("l_partkey", "l_suppkey"), "q9_slots",
+                                               "_ps");
+        cur = std::make_unique<HipByteFlagProbe>(std::move(cur), "d_q9_supp", "l_suppkey", "s_bits", false, "_sn");
+        cur = std::make_unique<HipByteFlagProbe>(std::move(cur), "d_q9_year", "l_orderkey", "o_bits", false, "_oy");
+        auto agg = std::make_unique<HipKeyedAgg>(std::move(cur), "d_q9_profit", "((int)_sn - 1) * 8 + ((int)_oy - 1)",
+                                                 256 * 8);
+        agg->addAggregate("sum_profit", AggKind::SumI64,
+                          std::string(kRevenue) + " - (i64)_ps_payload * l_quantity", 4);
+        agg->addAggregate("count", AggKind::Count, "1", 0);
+        appendPhase(plan, "Q9_lineitem", std::move(agg), 256, optInt(o, "items", 4), {4, 8});
+    }
+    plan.requiredScalars = {"like_l1", "like_l2", "q9_year0"};
+    plan.requiredSymbols = {"p_bits", "s_bits", "o_bits", "q9_slots"};
+    return plan;
+}
+
+// Q20 — potential part promotion (sql/q20.sql): parts whose name starts with
+// the LIKE word; their partsupp pairs in a 64-bit-key table whose payload
+// sums the year's l_quantity of the pair; pairs with lines and
+// ps_availqty > 0.5 * sum (200 * availqty > sum of hundredths, exact) flag
+// their supplier; the flagged suppliers of the nation (string constant 0)
+// are returned. The caller attaches name and address and orders by name.
+HipQueryPlan tpchQ20(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q20";
+    {
+        auto scan = makeScanLazy("part", {"p_partkey", "p_name"}, {"p_name"});
+        auto sel = std::make_unique<HipSelection>(std::move(scan),
+                                                  "gdb_like_prefix(p_name, 55, (const char*)d_like_words, (int)like_l1)");
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(sel), "d_q20_parts", "p_partkey", "p_bits");
+        auto& ph = appendPhase(plan, "Q20_parts", std::move(fl), 256, 4);
+        ph.scalarParams = {"like_l1"};
+        ph.externalBuffers = {{"d_like_words", "u8"}};
+    }
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey"});
+        auto in = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q20_parts", "ps_partkey", "p_bits", false);
+        auto ht = std::make_unique<HipHash64Build>(std::move(in), "d_q20_ps", pairKey("ps_partkey", "ps_suppkey"), "0",
+                                                   "q20_slots");
+        appendPhase(plan, "Q20_partsupp", std::move(ht), 256, 4);
+    }
+    {
+        auto scan = makeScan("lineitem", {"l_partkey", "l_suppkey", "l_quantity", "l_shipdate"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipSelection>(
+            std::move(scan), "(i64)l_shipdate >= q20_date_lo && (i64)l_shipdate < q20_date_hi");
+        cur = std::make_unique<HipByteFlagProbe>(std::move(cur), "d_q20_parts", "l_partkey", "p_bits", false);
+        cur = std::make_unique<HipHash64Probe>(std::move(cur), "d_q20_ps", pairKey("l_partkey", "l_suppkey"), "q20_slots",
+                                               "_ps");
+        cur = std::make_unique<HipSlotAtomicAdd>(std::move(cur), "d_q20_ps", 2, "_ps", 1, "l_quantity");
+        auto& ph = appendPhase(plan, "Q20_lineitem", std::move(cur), 256, optInt(o, "items", 4), {4, 8});
+        ph.scalarParams = {"q20_date_lo", "q20_date_hi"};
+    }
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey", "ps_availqty"});
+        std::unique_ptr<HipOperator> cur =
+            std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q20_parts", "ps_partkey", "p_bits", false);
+        cur = std::make_unique<HipHash64Probe>(std::move(cur), "d_q20_ps", pairKey("ps_partkey", "ps_suppkey"),
+                                               "q20_slots", "_ps");
+        cur = std::make_unique<HipSelection>(std::move(cur),
+                                             "(i64)_ps_payload > 0 && 200 * (i64)ps_availqty > (i64)_ps_payload");
+        cur = std::make_unique<HipByteFlagBuild>(std::move(cur), "d_q20_supp_ok", "ps_suppkey", "s_bits");
+        appendPhase(plan, "Q20_excess", std::move(cur), 256, 4);
+    }
+    nationSupplierFlags(plan, "Q20", "d_q20_nation_supp");
+    {
+        auto scan = makeScan("supplier", {"s_suppkey"});
+        auto a = std::make_unique<HipByteFlagProbe>(std::move(scan), "d_q20_nation_supp", "s_suppkey", "s_bits", false);
+        auto b = std::make_unique<HipByteFlagProbe>(std::move(a), "d_q20_supp_ok", "s_suppkey", "s_bits", false);
+        auto rows = std::make_unique<HipAppendRows>(std::move(b), "d_q20_rows", "n_supplier");
+        rows->addField("s_suppkey", "s_suppkey");
+        appendPhase(plan, "Q20_out", std::move(rows), 256, 4);
+    }
+    plan.requiredScalars = {"like_l1", "q20_date_lo", "q20_date_hi"};
+    plan.requiredSymbols = {"p_bits", "s_bits", "n_bits", "q20_slots"};
+    return plan;
+}
+
 bool tpchSchema(const PlanOptions& o) { return opt(o, "schema", "") == "tpch"; }
 
 }  // namespace
 
 std::vector<std::string> tpchPlanNames() {
-    return {"Q2", "Q4", "Q5", "Q7", "Q8", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q22"};
+    return {"Q2",  "Q4",  "Q5",  "Q7",  "Q8",  "Q9",  "Q10", "Q11", "Q12",
+            "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q20", "Q21", "Q22"};
 }
 
 std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOptions& opts) {
@@ -894,6 +1079,9 @@ std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOpt
     if (name == "Q19") return tpchQ19(opts);
     if (name == "Q11") return tpchQ11(opts);
     if (name == "Q2") return tpchQ2(opts);
+    if (name == "Q21") return tpchQ21(opts);
+    if (name == "Q9") return tpchQ9(opts);
+    if (name == "Q20") return tpchQ20(opts);
     if (name == "Q7") return tpchQ7(opts);
     if (name == "Q8") return tpchQ8(opts);
     if (name == "Q15") return tpchQ15(opts);

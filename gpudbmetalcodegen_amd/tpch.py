@@ -31,8 +31,8 @@ from . import engine as E
 
 INT, FLOAT, DATE, CHAR1, CHAR_FIXED = 0, 1, 2, 3, 4
 TABLES = ["lineitem", "orders", "customer", "supplier", "part", "partsupp", "nation", "region"]
-QUERIES = ["Q1", "Q2", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18",
-           "Q19", "Q22"]
+QUERIES = ["Q1", "Q2", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q9", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17",
+           "Q18", "Q19", "Q20", "Q21", "Q22"]
 CHECK_ABS_TOL = 1e-2  # the reference's --check defaults (codegen_main.cpp:47-48)
 CHECK_REL_TOL = 1e-4
 
@@ -182,6 +182,10 @@ PARAMS = {
     "Q15": {"q15_date_lo": ymd(1996, 1, 1), "q15_date_hi": shift(ymd(1996, 1, 1), months=3)},
     "Q17": {"strings": (b"Brand#23", b"MED BOX")},
     "Q2": {"strings": (b"EUROPE", b"BRASS"), "q2_size": 15},
+    "Q21": {"strings": (b"SAUDI ARABIA",), "q21_status": ord("F")},
+    "Q9": {"words": (b"green", b""), "q9_year0": 1992},
+    "Q20": {"prefix": b"forest", "strings": (b"CANADA",), "q20_date_lo": ymd(1994, 1, 1),
+            "q20_date_hi": shift(ymd(1994, 1, 1), years=1)},
     "Q7": {"strings": (b"FRANCE", b"GERMANY"), "q7_date_lo": ymd(1995, 1, 1), "q7_date_hi": ymd(1996, 12, 31),
            "q7_year0": 1995},
     "Q8": {"strings": (b"AMERICA", b"BRAZIL", b"ECONOMY ANODIZED STEEL"), "q8_date_lo": ymd(1995, 1, 1),
@@ -204,7 +208,10 @@ SCANNED = {"Q1": ["lineitem"], "Q3": ["customer", "orders", "lineitem"], "Q4": [
            "Q11": ["nation", "supplier", "partsupp", "part"], "Q15": ["lineitem", "supplier"],
            "Q17": ["part", "lineitem"], "Q2": ["region", "nation", "supplier", "part", "partsupp"],
            "Q7": ["nation", "supplier", "customer", "orders", "lineitem"],
-           "Q8": ["region", "nation", "supplier", "part", "customer", "orders", "lineitem"]}
+           "Q8": ["region", "nation", "supplier", "part", "customer", "orders", "lineitem"],
+           "Q21": ["nation", "supplier", "orders", "lineitem"],
+           "Q9": ["part", "partsupp", "supplier", "orders", "lineitem"],
+           "Q20": ["part", "partsupp", "lineitem", "nation", "supplier"]}
 # Output columns in the SQL select list's order.
 SELECT = {
     "Q1": ["l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price", "sum_charge", "avg_qty",
@@ -226,6 +233,9 @@ SELECT = {
     "Q2": ["s_acctbal", "s_name", "n_name", "p_partkey", "p_mfgr", "s_address", "s_phone", "s_comment"],
     "Q7": ["supp_nation", "cust_nation", "l_year", "revenue"],
     "Q8": ["o_year", "mkt_share"],
+    "Q21": ["s_name", "numwait"],
+    "Q9": ["nation", "o_year", "sum_profit"],
+    "Q20": ["s_name", "s_address"],
 }
 
 
@@ -322,14 +332,16 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
     if query == "Q3":
         od = db.host["orders"]["o_orderdate"]
         ex.register_symbol("o_slots", _pow2(2 * int((od < prm["q3_date"]).sum())))
+    if query in ("Q9", "Q20"):  # partsupp pairs of the selected parts: at most every row
+        ex.register_symbol(f"{query.lower()}_slots", _pow2(2 * int(db.host["partsupp"]["ps_partkey"].shape[0])))
     try:
-        if query == "Q13":
+        if "words" in prm:  # LIKE '%w1%w2%'
             w1, w2 = prm["words"]
             words_ptr = like_words_buffer(ctx, w1, w2)
             ex.register_buffer("d_like_words", words_ptr, 64)
             ex.register_scalar("like_l1", len(w1))
             ex.register_scalar("like_l2", len(w2))
-        elif query == "Q14":
+        elif "prefix" in prm:  # LIKE 'p%'
             words_ptr = like_words_buffer(ctx, prm["prefix"], b"")
             ex.register_buffer("d_like_words", words_ptr, 64)
             ex.register_scalar("like_l1", len(prm["prefix"]))
@@ -370,7 +382,7 @@ def _text(arr_row) -> str:
 
 # Queries whose result the host finishes (attached columns, reordering, a
 # final division); the others are the engine's result as collected.
-HOST_FINISH = {"Q2", "Q3", "Q5", "Q7", "Q8", "Q10", "Q15", "Q17", "Q18"}
+HOST_FINISH = {"Q2", "Q3", "Q5", "Q7", "Q8", "Q9", "Q10", "Q15", "Q17", "Q18", "Q20", "Q21"}
 
 
 def _finish(query: str, db: TpchDB, cols: dict, fmts: dict, prm: dict) -> None:
@@ -418,6 +430,27 @@ def _finish(query: str, db: TpchDB, cols: dict, fmts: dict, prm: dict) -> None:
         for c, name in enumerate(SELECT["Q2"]):
             cols[name] = [r[c] for r in rows]
         fmts["s_acctbal"] = (2, False)
+    elif query == "Q9":
+        n = db.host["nation"]
+        names = {int(k): _text(nm) for k, nm in zip(n["n_nationkey"], n["n_name"])}
+        rows = sorted(((names[b // 8], prm["q9_year0"] + b % 8, v) for b, v in zip(cols["bucket"], cols["sum_profit"])),
+                      key=lambda r: (r[0], -r[1]))
+        for c, name in enumerate(SELECT["Q9"]):
+            cols[name] = [r[c] for r in rows]
+        fmts["o_year"] = (0, False)
+    elif query == "Q20":
+        sp = db.host["supplier"]
+        pos = {int(k): i for i, k in enumerate(sp["s_suppkey"].tolist())}
+        rows = sorted((_text(sp["s_name"][pos[k]]), _text(sp["s_address"][pos[k]])) for k in cols["s_suppkey"])
+        cols["s_name"] = [r[0] for r in rows]
+        cols["s_address"] = [r[1] for r in rows]
+    elif query == "Q21":
+        sp = db.host["supplier"]
+        names = {int(k): _text(nm) for k, nm in zip(sp["s_suppkey"], sp["s_name"])}
+        rows = sorted(((names[k], n) for k, n in zip(cols["s_suppkey"], cols["numwait"])), key=lambda r: (-r[1], r[0]))
+        rows = rows[:100]
+        cols["s_name"] = [r[0] for r in rows]
+        cols["numwait"] = [r[1] for r in rows]
     elif query == "Q7":
         names = [w.decode() for w in prm["strings"]]
         y0 = prm["q7_year0"]

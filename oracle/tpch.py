@@ -424,6 +424,75 @@ def q8(T: dict, region: bytes, nation: bytes, ptype: bytes, lo: int, hi: int) ->
                                                                                   "mkt_share": (6, False)}
 
 
+def q21(T: dict, nation: bytes, status: int, k: int = 100) -> tuple:
+    """sql/q21.sql; ordered numwait desc, s_name."""
+    N, S, O, L = T["nation"], T["supplier"], T["orders"], T["lineitem"]
+    nk = {int(x) for x, nm in zip(N["n_nationkey"], N["n_name"]) if _s(nm) == nation}
+    sname = {int(x): _s(nm).decode() for x, nm, n in zip(S["s_suppkey"], S["s_name"], S["s_nationkey"]) if int(n) in nk}
+    fo = set(O["o_orderkey"][O["o_orderstatus"] == status].tolist())
+    supp: dict = {}
+    late: dict = {}
+    lines = list(zip(L["l_orderkey"].tolist(), L["l_suppkey"].tolist(),
+                     (L["l_receiptdate"] > L["l_commitdate"]).tolist()))
+    for ok, sk, lt in lines:
+        supp.setdefault(ok, set()).add(sk)
+        if lt:
+            late.setdefault(ok, set()).add(sk)
+    wait: dict = {}
+    for ok, sk, lt in lines:
+        if not lt or sk not in sname or ok not in fo:
+            continue
+        if supp[ok] - {sk} and not (late[ok] - {sk}):
+            wait[sk] = wait.get(sk, 0) + 1
+    rows = sorted(((sname[x], n) for x, n in wait.items()), key=lambda r: (-r[1], r[0]))[:k]
+    return {"s_name": [r[0] for r in rows], "numwait": [r[1] for r in rows]}, {"s_name": (0, False),
+                                                                              "numwait": (0, False)}
+
+
+def q9(T: dict, word: bytes) -> tuple:
+    """sql/q9.sql; ordered nation asc, o_year desc."""
+    P, S, PS, O, L, N = (T[t] for t in ("part", "supplier", "partsupp", "orders", "lineitem", "nation"))
+    parts = {int(k) for k, nm in zip(P["p_partkey"], P["p_name"]) if word in _s(nm)}
+    cost = {(int(p), int(sk)): int(c) for p, sk, c in zip(PS["ps_partkey"], PS["ps_suppkey"], PS["ps_supplycost"])
+            if int(p) in parts}
+    nat = {int(k): _s(nm).decode() for k, nm in zip(N["n_nationkey"], N["n_name"])}
+    snat = {int(k): nat[int(n)] for k, n in zip(S["s_suppkey"], S["s_nationkey"])}
+    oyear = dict(zip(O["o_orderkey"].tolist(), (O["o_orderdate"] // 10000).tolist()))
+    prof: dict = {}
+    for pk, sk, ok, q, p, d in zip(L["l_partkey"].tolist(), L["l_suppkey"].tolist(), L["l_orderkey"].tolist(),
+                                   L["l_quantity"].tolist(), L["l_extendedprice"].tolist(), L["l_discount"].tolist()):
+        c = cost.get((pk, sk))
+        if c is None or sk not in snat or ok not in oyear:
+            continue
+        key = (snat[sk], oyear[ok])
+        prof[key] = prof.get(key, 0) + p * (100 - d) - c * q
+    rows = sorted(prof.items(), key=lambda kv: (kv[0][0], -kv[0][1]))
+    return ({"nation": [r[0][0] for r in rows], "o_year": [r[0][1] for r in rows], "sum_profit": [r[1] for r in rows]},
+            {"nation": (0, False), "o_year": (0, False), "sum_profit": (4, False)})
+
+
+def q20(T: dict, prefix: bytes, nation: bytes, lo: int, hi: int) -> tuple:
+    """sql/q20.sql; ordered by s_name."""
+    P, PS, L, N, S = (T[t] for t in ("part", "partsupp", "lineitem", "nation", "supplier"))
+    parts = {int(k) for k, nm in zip(P["p_partkey"], P["p_name"]) if _s(nm).startswith(prefix)}
+    m = (L["l_shipdate"] >= lo) & (L["l_shipdate"] < hi)
+    qty: dict = {}
+    for pk, sk, q in zip(L["l_partkey"][m].tolist(), L["l_suppkey"][m].tolist(), L["l_quantity"][m].tolist()):
+        if pk in parts:
+            qty[(pk, sk)] = qty.get((pk, sk), 0) + q
+    ok = set()
+    for pk, sk, a in zip(PS["ps_partkey"].tolist(), PS["ps_suppkey"].tolist(), PS["ps_availqty"].tolist()):
+        s_ = qty.get((pk, sk))
+        if pk in parts and s_ is not None and 200 * a > s_:
+            ok.add(sk)
+    nk = {int(k) for k, nm in zip(N["n_nationkey"], N["n_name"]) if _s(nm) == nation}
+    rows = sorted((_s(nm).decode(), _s(ad).decode()) for k, nm, ad, n in zip(S["s_suppkey"], S["s_name"], S["s_address"],
+                                                                           S["s_nationkey"])
+                  if int(k) in ok and int(n) in nk)
+    return {"s_name": [r[0] for r in rows], "s_address": [r[1] for r in rows]}, {"s_name": (0, False),
+                                                                                "s_address": (0, False)}
+
+
 def run(T: dict, q: str, P: dict) -> tuple:
     """Query q with the constants P (the names of the engine's scalars:
     q1_cutoff, q3_seg, ... plus words / prefix for Q13 / Q14)."""
@@ -449,6 +518,12 @@ def run(T: dict, q: str, P: dict) -> tuple:
         return q5(T, P["strings"][0], P["q5_date_lo"], P["q5_date_hi"])
     if q == "Q10":
         return q10(T, P["q10_date_lo"], P["q10_date_hi"], P["q10_flag"])
+    if q == "Q9":
+        return q9(T, P["words"][0])
+    if q == "Q20":
+        return q20(T, P["prefix"], P["strings"][0], P["q20_date_lo"], P["q20_date_hi"])
+    if q == "Q21":
+        return q21(T, P["strings"][0], P["q21_status"])
     if q == "Q2":
         return q2(T, P["strings"][0], P["strings"][1], P["q2_size"])
     if q == "Q7":
