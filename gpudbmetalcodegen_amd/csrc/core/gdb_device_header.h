@@ -280,18 +280,30 @@ GDB_DEV i64 gdb_ht_find(const u64* __restrict__ tab, u32 slot_words, u32 bits, u
 // phase reads it); a key already present keeps its first payload. Returns 1
 // inserted, 0 present, -1 when max_probe slots are full.
 GDB_DEV u64 gdb_hash_slot64(u64 key, u32 bits) { return (key * 0x9E3779B97F4A7C15ull) >> (64u - bits); }
-GDB_DEV int gdb_ht64_insert(u64* __restrict__ tab, u32 bits, u64 key, u64 payload, u32 max_probe) {
+// Insert key (payload stored by the inserting row only); *slot receives the
+// key's slot, new or existing (-1 on overflow). Returns 1 when this call
+// inserted the key, 0 when it was present, -1 on overflow.
+GDB_DEV int gdb_ht64_insert_at(u64* __restrict__ tab, u32 bits, u64 key, u64 payload, u32 max_probe, i64* slot) {
     const u64 mask = (1ull << bits) - 1ull;
     u64 s = gdb_hash_slot64(key, bits);
     for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
         u64 expected = ~0ull;
         if (gdb_cas_u64(tab + 2 * s, expected, key)) {
             tab[2 * s + 1] = payload;
+            *slot = (i64)s;
             return 1;
         }
-        if (expected == key) return 0;
+        if (expected == key) {
+            *slot = (i64)s;
+            return 0;
+        }
     }
+    *slot = -1;
     return -1;
+}
+GDB_DEV int gdb_ht64_insert(u64* __restrict__ tab, u32 bits, u64 key, u64 payload, u32 max_probe) {
+    i64 slot;
+    return gdb_ht64_insert_at(tab, bits, key, payload, max_probe, &slot);
 }
 GDB_DEV i64 gdb_ht64_find(const u64* __restrict__ tab, u32 bits, u64 key, u32 max_probe) {
     const u64 mask = (1ull << bits) - 1ull;
@@ -484,11 +496,15 @@ GDB_DEV bool gdb_like2_bp(const char* s, const u32* t1, const u32* t2, int l1, i
 
 // LIKE 'p%' on a fixed-width NUL-padded field: the field starts with the l
 // bytes of p (a field shorter than l, i.e. NUL inside the prefix, fails).
+// The string compares here accumulate differences instead of returning at
+// the first one, so a field read in place from HBM has all its byte loads in
+// flight at once rather than one load latency per byte.
 GDB_DEV bool gdb_like_prefix(const char* s, int width, const char* p, int l) {
     if (l > width) return false;
-    for (int i = 0; i < l; ++i)
-        if (s[i] != p[i]) return false;
-    return true;
+    u32 d = 0;
+#pragma unroll 8
+    for (int i = 0; i < l; ++i) d |= (u32)(u8)(s[i] ^ p[i]);
+    return d == 0;
 }
 
 // The first two bytes of a char field as a little-endian code (byte 0 low),
@@ -500,24 +516,41 @@ GDB_DEV i64 gdb_code2(const char* p) { return (i64)*reinterpret_cast<const unsig
 GDB_DEV bool gdb_str_eq(const char* s, int width, const unsigned char* __restrict__ slot) {
     const int l = (int)slot[0];
     if (l > width) return false;
-    for (int i = 0; i < l; ++i)
-        if ((unsigned char)s[i] != slot[1 + i]) return false;
-    return l == width || s[l] == 0;
+    u32 d = l == width ? 0u : (u32)(u8)s[l];
+#pragma unroll 8
+    for (int i = 0; i < l; ++i) d |= (u32)((unsigned char)s[i] ^ slot[1 + i]);
+    return d == 0;
+}
+
+// 64-bit FNV-1a over n bytes, continuing from h (the offset basis
+// 0xcbf29ce484222325 starts a fresh hash); gdb_fnv64_u32 folds the four
+// little-endian bytes of v. A fingerprint of a multi-column group key.
+GDB_DEV u64 gdb_fnv64(u64 h, const char* p, int n) {
+    for (int i = 0; i < n; ++i) h = (h ^ (u64)(unsigned char)p[i]) * 0x100000001b3ull;
+    return h;
+}
+GDB_DEV u64 gdb_fnv64_u32(u64 h, u32 v) {
+    for (int i = 0; i < 4; ++i) h = (h ^ (u64)((v >> (8 * i)) & 255u)) * 0x100000001b3ull;
+    return h;
+}
+
+// Byte equality of two n-byte fields.
+GDB_DEV bool gdb_bytes_eq(const char* a, const char* b, int n) {
+    u32 d = 0;
+    for (int i = 0; i < n; ++i) d |= (u32)(u8)(a[i] ^ b[i]);
+    return d == 0;
 }
 
 // LIKE '%p' on a fixed-width NUL-padded field: the string (up to its first
 // NUL) ends with the l bytes of p.
 GDB_DEV bool gdb_like_suffix(const char* s, int width, const char* p, int l) {
     int len = width;
-    for (int i = 0; i < width; ++i)
-        if (s[i] == 0) {
-            len = i;
-            break;
-        }
+    for (int i = width - 1; i >= 0; --i) len = s[i] == 0 ? i : len;
     if (l > len) return false;
-    for (int i = 0; i < l; ++i)
-        if (s[len - l + i] != p[i]) return false;
-    return true;
+    u32 d = 0;
+#pragma unroll 8
+    for (int i = 0; i < l; ++i) d |= (u32)(u8)(s[len - l + i] ^ p[i]);
+    return d == 0;
 }
 
 // Wave-aggregated append: every active lane gets a distinct index from the

@@ -79,6 +79,12 @@ void HipCodegen::setPhaseItemsVariants(const std::vector<int>& items) {
 }
 void HipCodegen::setPhaseLaunchBound(int bound) { cur("setPhaseLaunchBound").launchBound = bound; }
 void HipCodegen::setPhaseStatusBuffer(const std::string& name) { cur("setPhaseStatusBuffer").statusBuffer = name; }
+void HipCodegen::setPhaseCheck(const std::string& checkBuffer, const std::string& message) {
+    auto& p = cur("setPhaseCheck");
+    if (!p.checkBuffer.empty()) throw std::runtime_error("phase " + p.name + ": one row check per phase");
+    p.checkBuffer = checkBuffer;
+    p.checkMessage = message;
+}
 
 // --- parameters -------------------------------------------------------------
 

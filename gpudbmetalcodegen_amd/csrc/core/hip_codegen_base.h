@@ -92,6 +92,9 @@ public:
     void setPhaseItemsVariants(const std::vector<int>& items);
     void setPhaseLaunchBound(int bound);
     void setPhaseStatusBuffer(const std::string& name);
+    // A row check's status words (HipRowCheck): the run fails with `message`
+    // when any row of the phase failed it.
+    void setPhaseCheck(const std::string& checkBuffer, const std::string& message);
 
     // --- parameters ---
     // Column of a registered table; returns the pointer parameter name.
@@ -175,6 +178,8 @@ public:
         std::vector<int> itemsVariants{4};
         int launchBound = 1024;
         std::string statusBuffer;  // optional overflow/status words checked after the phase
+        std::string checkBuffer;   // optional row check's words (failed flag, -, failed rows) and its message
+        std::string checkMessage;
         std::string ldsBytesExpr = "0u";  // range mode: LDS bytes of the body (constant expression)
         bool staged = false;              // item-batched emission (hip_operators.h)
         bool prefetch = false;            // scans load the next tile before processing this one

@@ -427,6 +427,13 @@ std::vector<std::array<uint64_t, 4>> HipGenericExecutor::checkStatus(const HipCo
             readBuffer(p.statusBuffer, 0, w.data(), 32);
             if (w[0] != 0) throw GdbOverflowError(p.name, w[2]);
         }
+        if (!p.checkBuffer.empty() && buffers_.count(p.checkBuffer)) {
+            std::array<uint64_t, 4> c{0, 0, 0, 0};
+            readBuffer(p.checkBuffer, 0, c.data(), 32);
+            if (c[0] != 0)
+                throw std::runtime_error("phase " + p.name + ": " + p.checkMessage + " (" + std::to_string(c[2]) +
+                                         " rows)");
+        }
         out.push_back(w);
     }
     return out;

@@ -186,9 +186,20 @@ void HipGridStrideScan::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                         cg.addLine("const u64 _nb = _nrows * " + W + ";");
                         cg.addLine("const char* _src = (const char*)" + ptr[k] + " + _t0 * " + W + ";");
                         cg.addBlock("if (_full && (_nb % 16) == 0)", [&]() {
-                            cg.addLine("for (u64 _k = threadIdx.x; _k < _nb / 16; _k += blockDim.x)");
-                            cg.addLine("    reinterpret_cast<u32x4*>(_s_" + c.name +
-                                       ")[_k] = gdb_ld_stream(reinterpret_cast<const u32x4*>(_src) + _k);");
+                            // four 16-byte loads in flight per thread before their LDS stores
+                            const std::string dst = "reinterpret_cast<u32x4*>(_s_" + c.name + ")";
+                            const std::string src = "reinterpret_cast<const u32x4*>(_src)";
+                            cg.addLine("const u64 _nv = _nb / 16, _bd = blockDim.x;");
+                            cg.addLine("u64 _k = threadIdx.x;");
+                            cg.addBlock("for (; _k + 3 * _bd < _nv; _k += 4 * _bd)", [&]() {
+                                cg.addLine("const u32x4 _v0 = gdb_ld_stream(" + src + " + _k), _v1 = gdb_ld_stream(" +
+                                           src + " + _k + _bd);");
+                                cg.addLine("const u32x4 _v2 = gdb_ld_stream(" + src + " + _k + 2 * _bd), _v3 = gdb_ld_stream(" +
+                                           src + " + _k + 3 * _bd);");
+                                cg.addLine(dst + "[_k] = _v0; " + dst + "[_k + _bd] = _v1; " + dst + "[_k + 2 * _bd] = _v2; " +
+                                           dst + "[_k + 3 * _bd] = _v3;");
+                            });
+                            cg.addLine("for (; _k < _nv; _k += _bd) " + dst + "[_k] = gdb_ld_stream(" + src + " + _k);");
                         }, " else {");
                         {
                             IndentGuard g(cg);
@@ -840,8 +851,15 @@ std::string HipDenseAtomicAdd::describe() const {
 }
 
 HipHash64Build::HipHash64Build(std::unique_ptr<HipOperator> child, const std::string& table,
-                               const std::string& keyExpr, const std::string& payloadExpr, const std::string& slotsExpr)
-    : HipUnaryOperator(std::move(child)), table_(table), key_(keyExpr), payload_(payloadExpr), slots_(slotsExpr) {}
+                               const std::string& keyExpr, const std::string& payloadExpr, const std::string& slotsExpr,
+                               bool distinct, const std::string& slotVar)
+    : HipUnaryOperator(std::move(child)),
+      table_(table),
+      key_(keyExpr),
+      payload_(payloadExpr),
+      slots_(slotsExpr),
+      distinct_(distinct),
+      slotVar_(slotVar) {}
 
 void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(table_, "u64", 8, slots_ + " * 2", InitKind::Pattern, wordsPattern({~0ull, 0ull}), false, true);
@@ -854,20 +872,46 @@ void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                " : 65536ull);");
     cg.addLine("u32 _ins_" + table_ + " = 0, _ovf_" + table_ + " = 0;");
     child_->produce(cg, [&]() {
-        cg.addBlock("", [&]() {
-            cg.addLine("const int _r = gdb_ht64_insert(" + table_ + ", _bits_" + table_ + ", (u64)(" + key_ + "), (u64)(" +
-                       payload_ + "), _maxp_" + table_ + ");");
-            cg.addLine("_ins_" + table_ + " += _r == 1 ? 1u : 0u;");
-            cg.addLine("_ovf_" + table_ + " |= _r < 0 ? 1u : 0u;");
-        });
-        consume();
+        const std::string r = cg.uniqueName("_hr");
+        const std::string slot = slotVar_.empty() ? cg.uniqueName("_hs") : slotVar_;
+        cg.addLine("i64 " + slot + ";");
+        cg.addLine("const int " + r + " = gdb_ht64_insert_at(" + table_ + ", _bits_" + table_ + ", (u64)(" + key_ +
+                   "), (u64)(" + payload_ + "), _maxp_" + table_ + ", &" + slot + ");");
+        cg.addLine("_ins_" + table_ + " += " + r + " == 1 ? 1u : 0u;");
+        cg.addLine("_ovf_" + table_ + " |= " + r + " < 0 ? 1u : 0u;");
+        if (distinct_)
+            cg.addIf(r + " == 1", [&]() { consume(); });
+        else if (!slotVar_.empty())
+            cg.addIf(r + " >= 0", [&]() { consume(); });
+        else
+            consume();
     });
     emitBlockStatus(cg, st, {{"_ins_" + table_, 2, false}, {"_ovf_" + table_, 0, true}});
 }
 
 std::string HipHash64Build::describe() const {
-    return "Hash64Build(" + table_ + ", key=" + key_ + ", payload=" + payload_ + ", slots=" + slots_ + ")";
+    return std::string(distinct_ ? "Hash64BuildDistinct(" : "Hash64Build(") + table_ + ", key=" + key_ +
+           ", payload=" + payload_ + ", slots=" + slots_ + ")";
 }
+
+HipRowCheck::HipRowCheck(std::unique_ptr<HipOperator> child, const std::string& name, const std::string& condition,
+                         const std::string& message)
+    : HipUnaryOperator(std::move(child)), name_(name), cond_(condition), message_(message) {}
+
+void HipRowCheck::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string st = name_ + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseCheck(st, message_);
+    const std::string bad = "_bad_" + name_;
+    cg.addLine("u32 " + bad + " = 0;");
+    child_->produce(cg, [&]() {
+        cg.addLine(bad + " += (" + cond_ + ") ? 0u : 1u;");
+        consume();
+    });
+    emitBlockStatus(cg, st, {{bad, 2, false}, {bad, 0, true}});
+}
+
+std::string HipRowCheck::describe() const { return "RowCheck(" + cond_ + ")"; }
 
 HipHash64Probe::HipHash64Probe(std::unique_ptr<HipOperator> child, const std::string& table,
                                const std::string& keyExpr, const std::string& slotsExpr, const std::string& slotVar)

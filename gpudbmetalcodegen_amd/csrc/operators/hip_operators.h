@@ -204,15 +204,36 @@ private:
 // rows whose key is present with <slotVar> (i64 slot) and <slotVar>_payload.
 // The reference hashes 64-bit keys by folding them the same way
 // (metal_param_binding.h:20-26); its tables are host-built.
+// distinct: only the row whose insert created the key is consumed (a key's
+// first arrival), so a count downstream is COUNT(DISTINCT key). slotVar set:
+// const i64 <slotVar> holds the key's slot for the consumer (a dense group
+// id), and rows whose insert overflowed are not consumed.
 class HipHash64Build : public HipUnaryOperator {
 public:
     HipHash64Build(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
-                   const std::string& payloadExpr, const std::string& slotsExpr);
+                   const std::string& payloadExpr, const std::string& slotsExpr, bool distinct = false,
+                   const std::string& slotVar = "");
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string table_, key_, payload_, slots_;
+    bool distinct_ = false;
+    std::string slotVar_;
+};
+
+// Row check: counts the rows for which `condition` is false and fails the
+// run after the phase with `message` when any did (e.g. a fingerprint whose
+// group representative holds other key bytes). Every row is consumed.
+class HipRowCheck : public HipUnaryOperator {
+public:
+    HipRowCheck(std::unique_ptr<HipOperator> child, const std::string& name, const std::string& condition,
+                const std::string& message);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string name_, cond_, message_;
 };
 
 class HipHash64Probe : public HipUnaryOperator {

@@ -1058,13 +1058,96 @@ HipQueryPlan tpchQ20(const PlanOptions& o) {
     return plan;
 }
 
+// Q16 — parts/supplier relationship (sql/q16.sql). Suppliers whose comment
+// is LIKE '%w1%w2%' (the LIKE words) are flagged. Parts whose brand is not
+// string constant 0, whose type does not start with string constant 1 and
+// whose size is a set bit of q16_sizes are grouped by a 63-bit FNV-1a
+// fingerprint of (p_brand, p_type, p_size) in a 64-bit-key hash table whose
+// payload is the first part row to arrive (the group's representative); each
+// part key stores its group's slot + 1. Partsupp rows of grouped parts whose
+// supplier is not flagged insert (group, supplier) into a second table; first
+// arrivals count per group (COUNT(DISTINCT ps_suppkey)). The last pass checks
+// every grouped part's key bytes against its representative's (a fingerprint
+// collision fails the run) and returns the representatives with their
+// counts; the caller attaches brand, type and size and orders the groups.
+HipQueryPlan tpchQ16(const PlanOptions& o) {
+    HipQueryPlan plan;
+    plan.name = "Q16";
+    {
+        auto scan = makeScan("supplier", {"s_suppkey", "s_comment"});
+        auto like = std::make_unique<HipLikeFilter>(std::move(scan), "s_comment", 101, "d_like_words", "like_l1",
+                                                    "like_l2", false);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(like), "d_q16_complaints", "s_suppkey", "s_bits");
+        const int block = optInt(o, "supp_block", 64);
+        auto& ph = appendPhase(plan, "Q16_complaints", std::move(fl), block, 1, {1});
+        ph.launchBound = block;
+        ph.scalarParams = {"like_l1", "like_l2"};
+        ph.externalBuffers = {{"d_like_words", "u8"}};
+    }
+    {
+        auto scan = optInt(o, "lazy", 0)
+                        ? makeScanLazy("part", {"p_partkey", "p_brand", "p_type", "p_size"}, {"p_brand", "p_type"})
+                        : makeScan("part", {"p_partkey", "p_brand", "p_type", "p_size"});
+        auto sel = std::make_unique<HipSelection>(
+            std::move(scan),
+            "(u64)p_size < 64ull && ((q16_sizes >> p_size) & 1) && !gdb_str_eq(p_brand, 10, " + strSlot(0) +
+                ") && !gdb_like_prefix(p_type, 25, (const char*)d_str_consts + 33, "
+                "(int)((const unsigned char*)d_str_consts)[32])");
+        auto ht = std::make_unique<HipHash64Build>(
+            std::move(sel), "d_q16_groups",
+            "(gdb_fnv64_u32(gdb_fnv64(gdb_fnv64(0xcbf29ce484222325ull, p_brand, 10), p_type, 25), (u32)p_size) & "
+            "0x7fffffffffffffffull)",
+            "i", "q16_slots", false, "_g");
+        auto st = std::make_unique<HipDenseStore>(std::move(ht), "d_q16_part_group", "p_partkey", "p_bits", "_g + 1");
+        auto& ph = appendPhase(plan, "Q16_groups", std::move(st), 256, optInt(o, "part_items", 2), {1, 2, 4});
+        ph.launchBound = 256;
+        ph.scalarParams = {"q16_sizes"};
+        ph.externalBuffers = {{"d_str_consts", "u8"}};
+    }
+    {
+        auto scan = makeScan("partsupp", {"ps_partkey", "ps_suppkey"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipArrayLookup>(
+            std::move(scan), "d_q16_part_group", "ps_partkey", "p_bits", "_pg", 0ll, "u32");
+        cur = std::make_unique<HipByteFlagProbe>(std::move(cur), "d_q16_complaints", "ps_suppkey", "s_bits", true);
+        cur = std::make_unique<HipHash64Build>(std::move(cur), "d_q16_pairs",
+                                               "(((u64)_pg << 32) | (u64)(u32)ps_suppkey)", "0", "q16_pair_slots", true);
+        cur = std::make_unique<HipDenseAtomicAdd>(std::move(cur), "d_q16_count", "_pg - 1", "q16_slots", "1");
+        appendPhase(plan, "Q16_supplier_pairs", std::move(cur), 256, optInt(o, "items", 4), {4, 8});
+    }
+    {
+        auto scan = makeScanLazy("part", {"p_partkey", "p_brand", "p_type", "p_size"}, {"p_brand", "p_type"});
+        std::unique_ptr<HipOperator> cur = std::make_unique<HipArrayLookup>(
+            std::move(scan), "d_q16_part_group", "p_partkey", "p_bits", "_pg", 0ll, "u32");
+        cur = std::make_unique<HipArrayLookup>(std::move(cur), "d_q16_groups", "(_pg - 1) * 2 + 1", "q16_slots * 2",
+                                               "_rep", -1ll, "i64");
+        const std::string R = "(u64)_rep";
+        cur = std::make_unique<HipRowCheck>(
+            std::move(cur), "q16_fp",
+            "gdb_bytes_eq(p_brand, (const char*)" + tableDataName("part", "p_brand") + " + " + R +
+                " * 10, 10) && gdb_bytes_eq(p_type, (const char*)" + tableDataName("part", "p_type") + " + " + R +
+                " * 25, 25) && " + tableDataName("part", "p_size") + "[" + R + "] == p_size",
+            "two (p_brand, p_type, p_size) groups share a fingerprint");
+        cur = std::make_unique<HipSelection>(std::move(cur), "(u64)_rep == i");
+        cur = std::make_unique<HipArrayLookup>(std::move(cur), "d_q16_count", "_pg - 1", "q16_slots", "_cnt", 0ll,
+                                               "i64");
+        auto rows = std::make_unique<HipAppendRows>(std::move(cur), "d_q16_rows", "n_part");
+        rows->addField("p_partkey", "p_partkey");
+        rows->addField("supplier_cnt", "_cnt");
+        auto& ph = appendPhase(plan, "Q16_out", std::move(rows), 256, optInt(o, "part_items", 2), {1, 2, 4});
+        ph.launchBound = 256;
+    }
+    plan.requiredScalars = {"like_l1", "like_l2", "q16_sizes"};
+    plan.requiredSymbols = {"s_bits", "p_bits", "q16_slots", "q16_pair_slots"};
+    return plan;
+}
+
 bool tpchSchema(const PlanOptions& o) { return opt(o, "schema", "") == "tpch"; }
 
 }  // namespace
 
 std::vector<std::string> tpchPlanNames() {
     return {"Q2",  "Q4",  "Q5",  "Q7",  "Q8",  "Q9",  "Q10", "Q11", "Q12",
-            "Q13", "Q14", "Q15", "Q17", "Q18", "Q19", "Q20", "Q21", "Q22"};
+            "Q13", "Q14", "Q15", "Q16", "Q17", "Q18", "Q19", "Q20", "Q21", "Q22"};
 }
 
 std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOptions& opts) {
@@ -1085,6 +1168,7 @@ std::optional<HipQueryPlan> buildTpchPlan(const std::string& name, const PlanOpt
     if (name == "Q7") return tpchQ7(opts);
     if (name == "Q8") return tpchQ8(opts);
     if (name == "Q15") return tpchQ15(opts);
+    if (name == "Q16") return tpchQ16(opts);
     if (name == "Q17") return tpchQ17(opts);
     if (name == "Q12") return tpchQ12(opts);
     if (name == "Q13") return tpchQ13(opts);

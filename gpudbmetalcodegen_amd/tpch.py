@@ -31,8 +31,8 @@ from . import engine as E
 
 INT, FLOAT, DATE, CHAR1, CHAR_FIXED = 0, 1, 2, 3, 4
 TABLES = ["lineitem", "orders", "customer", "supplier", "part", "partsupp", "nation", "region"]
-QUERIES = ["Q1", "Q2", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q9", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17",
-           "Q18", "Q19", "Q20", "Q21", "Q22"]
+QUERIES = ["Q1", "Q2", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q9", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q16",
+           "Q17", "Q18", "Q19", "Q20", "Q21", "Q22"]
 CHECK_ABS_TOL = 1e-2  # the reference's --check defaults (codegen_main.cpp:47-48)
 CHECK_REL_TOL = 1e-4
 
@@ -165,6 +165,7 @@ def mode_code(mode: str) -> int:
 
 
 Q22_CODES = (13, 31, 23, 29, 30, 18, 17)
+Q16_SIZES = (49, 14, 23, 45, 19, 3, 36, 9)
 
 PARAMS = {
     "Q1": {"q1_cutoff": shift(ymd(1998, 12, 1), days=-90)},
@@ -180,6 +181,8 @@ PARAMS = {
     "Q5": {"strings": (b"ASIA",), "q5_date_lo": ymd(1994, 1, 1), "q5_date_hi": shift(ymd(1994, 1, 1), years=1)},
     "Q11": {"strings": (b"GERMANY",), "q11_frac_den": 10000},  # value > total * 0.0001
     "Q15": {"q15_date_lo": ymd(1996, 1, 1), "q15_date_hi": shift(ymd(1996, 1, 1), months=3)},
+    "Q16": {"words": (b"Customer", b"Complaints"), "strings": (b"Brand#45", b"MEDIUM POLISHED"),
+            "q16_sizes": sum(1 << v for v in Q16_SIZES)},
     "Q17": {"strings": (b"Brand#23", b"MED BOX")},
     "Q2": {"strings": (b"EUROPE", b"BRASS"), "q2_size": 15},
     "Q21": {"strings": (b"SAUDI ARABIA",), "q21_status": ord("F")},
@@ -206,7 +209,7 @@ SCANNED = {"Q1": ["lineitem"], "Q3": ["customer", "orders", "lineitem"], "Q4": [
            "Q5": ["region", "nation", "supplier", "customer", "orders", "lineitem"],
            "Q10": ["orders", "lineitem", "customer"], "Q19": ["part", "lineitem"],
            "Q11": ["nation", "supplier", "partsupp", "part"], "Q15": ["lineitem", "supplier"],
-           "Q17": ["part", "lineitem"], "Q2": ["region", "nation", "supplier", "part", "partsupp"],
+           "Q16": ["supplier", "part", "partsupp"], "Q17": ["part", "lineitem"], "Q2": ["region", "nation", "supplier", "part", "partsupp"],
            "Q7": ["nation", "supplier", "customer", "orders", "lineitem"],
            "Q8": ["region", "nation", "supplier", "part", "customer", "orders", "lineitem"],
            "Q21": ["nation", "supplier", "orders", "lineitem"],
@@ -229,6 +232,7 @@ SELECT = {
     "Q19": ["revenue"],
     "Q11": ["ps_partkey", "value"],
     "Q15": ["s_suppkey", "s_name", "s_address", "s_phone", "total_revenue"],
+    "Q16": ["p_brand", "p_type", "p_size", "supplier_cnt"],
     "Q17": ["avg_yearly"],
     "Q2": ["s_acctbal", "s_name", "n_name", "p_partkey", "p_mfgr", "s_address", "s_phone", "s_comment"],
     "Q7": ["supp_nation", "cust_nation", "l_year", "revenue"],
@@ -334,6 +338,9 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
         ex.register_symbol("o_slots", _pow2(2 * int((od < prm["q3_date"]).sum())))
     if query in ("Q9", "Q20"):  # partsupp pairs of the selected parts: at most every row
         ex.register_symbol(f"{query.lower()}_slots", _pow2(2 * int(db.host["partsupp"]["ps_partkey"].shape[0])))
+    if query == "Q16":  # groups: at most one per part; (group, supplier) pairs: at most one per partsupp row
+        ex.register_symbol("q16_slots", _pow2(2 * int(db.host["part"]["p_partkey"].shape[0])))
+        ex.register_symbol("q16_pair_slots", _pow2(2 * int(db.host["partsupp"]["ps_partkey"].shape[0])))
     try:
         if "words" in prm:  # LIKE '%w1%w2%'
             w1, w2 = prm["words"]
@@ -382,7 +389,7 @@ def _text(arr_row) -> str:
 
 # Queries whose result the host finishes (attached columns, reordering, a
 # final division); the others are the engine's result as collected.
-HOST_FINISH = {"Q2", "Q3", "Q5", "Q7", "Q8", "Q9", "Q10", "Q15", "Q17", "Q18", "Q20", "Q21"}
+HOST_FINISH = {"Q2", "Q3", "Q5", "Q7", "Q8", "Q9", "Q10", "Q15", "Q16", "Q17", "Q18", "Q20", "Q21"}
 
 
 def _finish(query: str, db: TpchDB, cols: dict, fmts: dict, prm: dict) -> None:
@@ -468,6 +475,16 @@ def _finish(query: str, db: TpchDB, cols: dict, fmts: dict, prm: dict) -> None:
         idx = [pos[k] for k in cols["s_suppkey"]]
         for c in ("s_name", "s_address", "s_phone"):
             cols[c] = [_text(sp[c][i]) for i in idx]
+    elif query == "Q16":
+        pt = db.host["part"]
+        sub = np.flatnonzero(np.isin(pt["p_partkey"], cols["p_partkey"]))
+        pos = {int(pt["p_partkey"][i]): i for i in sub}
+        rows = sorted(((_text(pt["p_brand"][pos[k]]), _text(pt["p_type"][pos[k]]), int(pt["p_size"][pos[k]]), n)
+                       for k, n in zip(cols["p_partkey"], cols["supplier_cnt"])),
+                      key=lambda r: (-r[3], r[0], r[1], r[2]))
+        for c, name in enumerate(SELECT["Q16"]):
+            cols[name] = [r[c] for r in rows]
+        fmts["p_size"] = (0, False)
     elif query == "Q17":
         # SUM(l_extendedprice) / 7.0, exact to 6 more digits: trunc(sum * 10^6 / 7)
         cols["avg_yearly"] = [v * 1_000_000 // 7 for v in cols["sum_price"]]

@@ -1,7 +1,6 @@
 """Command-line driver of the named TPC-H plans, the counterpart of the
 reference's ``GPUDBCodegen [flags] [sf1|sf10|...] q<N>|all``
-(``codegen/codegen_main.cpp:1782-1960``) for the queries this engine plans
-(Q1, Q3, Q4, Q6, Q12, Q13, Q14, Q18, Q22).
+(``codegen/codegen_main.cpp:1782-1960``) for the 22 TPC-H queries.
 
     python -m gpudbmetalcodegen_amd.tpch_cli [flags] [sf1|sf10|sf20|sf50|sf100] q<N>|all
 

@@ -104,6 +104,26 @@ def _comments(rng, n, width, special_frac=0.0) -> np.ndarray:
     return fixed(out, width)
 
 
+def _customer_remarks(seed: int, comments: np.ndarray) -> None:
+    """TPC-H 4.2.3: of every 10,000 suppliers, 5 have "Customer ... Complaints"
+    and 5 "Customer ... Recommends" in their comment (here at least one of
+    each), written in place at a random offset. A generator of its own, so
+    the other columns do not depend on it."""
+    n, width = comments.shape
+    rng = np.random.default_rng(seed + 0x5151)
+    k = max(1, n // 2000)
+    rows = rng.permutation(n)[:2 * k]
+    for j, r in enumerate(rows.tolist()):
+        text = b"Customer " + WORDS[int(rng.integers(0, len(WORDS)))].encode() + \
+            (b" Complaints" if j < k else b" Recommends")
+        if len(text) > width:
+            continue
+        off = int(rng.integers(0, width - len(text) + 1))
+        cur = bytes(comments[r]).split(b"\0", 1)[0].ljust(off, b" ")
+        line = (cur[:off] + text + cur[off + len(text):])[:width]
+        comments[r] = np.frombuffer(line.ljust(width, b"\0"), np.uint8)
+
+
 def retail_price(partkey: np.ndarray) -> np.ndarray:
     """Hundredths: 90000 + (key / 10) mod 20001 + 100 * (key mod 1000) (TPC-H 4.2.3)."""
     k = partkey.astype(np.int64)
@@ -134,6 +154,7 @@ def generate(sf: float, seed: int = 1) -> dict:
         "s_acctbal": rng.integers(-99999, 1000000, n_supp).astype(np.int64),
         "s_comment": _comments(rng, n_supp, 101),
     }
+    _customer_remarks(seed, supplier["s_comment"])
 
     pk = np.arange(1, n_part + 1, dtype=np.int32)
     types = [f"{a} {b} {c}" for a, b, c in zip(_pick(rng, TYPE1, n_part), _pick(rng, TYPE2, n_part),

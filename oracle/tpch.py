@@ -323,6 +323,30 @@ def q15(T: dict, lo: int, hi: int) -> tuple:
     return cols, fm
 
 
+def q16(T: dict, brand: bytes, type_prefix: bytes, sizes, w1: bytes, w2: bytes) -> tuple:
+    """sql/q16.sql: per (p_brand, p_type, p_size) of the parts passing the
+    brand / type-prefix / size-set filters, the number of distinct suppliers
+    of their partsupp rows whose comment is not LIKE '%w1%w2%'; ordered by
+    supplier_cnt desc, brand, type, size."""
+    P, PS, S = T["part"], T["partsupp"], T["supplier"]
+    pat = re.compile(re.escape(w1) + b".*" + re.escape(w2), re.S)
+    complained = {int(k) for k, c in zip(S["s_suppkey"], S["s_comment"]) if pat.search(_s(c))}
+    sizes = set(sizes)
+    group = {}
+    for k, b, t, z in zip(P["p_partkey"].tolist(), P["p_brand"], P["p_type"], P["p_size"].tolist()):
+        b, t = _s(b), _s(t)
+        if b != brand and not t.startswith(type_prefix) and z in sizes:
+            group[k] = (b.decode(), t.decode(), z)
+    supp: dict = {}
+    for pk, sk in zip(PS["ps_partkey"].tolist(), PS["ps_suppkey"].tolist()):
+        g = group.get(pk)
+        if g is not None and sk not in complained:
+            supp.setdefault(g, set()).add(sk)
+    rows = sorted(((g[0], g[1], g[2], len(v)) for g, v in supp.items()), key=lambda r: (-r[3], r[0], r[1], r[2]))
+    names = ["p_brand", "p_type", "p_size", "supplier_cnt"]
+    return {n: [r[c] for r in rows] for c, n in enumerate(names)}, {n: (0, False) for n in names}
+
+
 def q17(T: dict, brand: bytes, container: bytes) -> tuple:
     """sql/q17.sql: sum(l_extendedprice) / 7 as trunc(sum * 10^6 / 7), 8 digits."""
     P, L = T["part"], T["lineitem"]
@@ -534,6 +558,9 @@ def run(T: dict, q: str, P: dict) -> tuple:
         return q11(T, P["strings"][0], P["q11_frac_den"])
     if q == "Q15":
         return q15(T, P["q15_date_lo"], P["q15_date_hi"])
+    if q == "Q16":
+        sizes = [v for v in range(64) if (P["q16_sizes"] >> v) & 1]
+        return q16(T, P["strings"][0], P["strings"][1], sizes, *P["words"])
     if q == "Q17":
         return q17(T, P["strings"][0], P["strings"][1])
     if q == "Q19":

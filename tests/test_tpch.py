@@ -4,7 +4,7 @@ and the TPC-H oracle (CPU).
 The .colbin layout is checked byte by byte with struct against the format of
 the reference (src/infra.h:511-771); the comparator against the reference's
 own golden files (tests/golden/tpch, DuckDB results at SF1); the oracle of the
-nine named queries against SQLite running the SQL text on the same rows.
+22 named queries against SQLite running the SQL text on the same rows.
 """
 
 from __future__ import annotations
@@ -40,10 +40,29 @@ def q19_parts(data: dict) -> dict:
     return data
 
 
+def q16_rows(data: dict) -> dict:
+    """Q16 cases the small generated data rarely holds: one group (brand,
+    type, size) shared by every eleventh part, so many of its parts have the
+    same suppliers (COUNT DISTINCT); parts of the excluded brand and type
+    prefix with a size of the set; every ninth supplier complained about."""
+    P, S = data["part"], data["supplier"]
+    k = P["p_partkey"]
+    for sel, brand, ptype, size in ((k % 11 == 0, "Brand#11", "SMALL PLATED TIN", 49),
+                                    (k % 13 == 0, "Brand#45", None, 14),
+                                    (k % 17 == 0, None, "MEDIUM POLISHED TIN", 23)):
+        if brand:
+            P["p_brand"][sel] = G.fixed([brand], 10)[0]
+        if ptype:
+            P["p_type"][sel] = G.fixed([ptype], 25)[0]
+        P["p_size"][sel] = size
+    S["s_comment"][S["s_suppkey"] % 9 == 0] = G.fixed(["slyly Customer final Complaints"], 101)[0]
+    return data
+
+
 @pytest.fixture(scope="module")
 def gen_dir(tmp_path_factory):
     d = tmp_path_factory.mktemp("tpch")
-    data = q19_parts(G.generate(0.005, seed=11))
+    data = q16_rows(q19_parts(G.generate(0.005, seed=11)))
     G.write_tbl(data, d, T.table_columns)
     T.ingest(d)
     return d, data
@@ -235,6 +254,7 @@ def test_canonical_rendering():
 
 def _sqlite(host: dict) -> sqlite3.Connection:
     con = sqlite3.connect(":memory:")
+    con.execute("PRAGMA case_sensitive_like = ON")  # LIKE as in SQL (SQLite folds ASCII case by default)
     for t, cols in host.items():
         spec = T.table_columns(t)
         con.execute(f"CREATE TABLE {t} ({', '.join(n for n, *_ in spec)})")
@@ -357,6 +377,11 @@ SQL = {
         SELECT s_suppkey, s_name, s_address, s_phone, total_revenue FROM supplier, revenue0
         WHERE s_suppkey = supplier_no AND total_revenue = (SELECT MAX(total_revenue) FROM revenue0)
         ORDER BY s_suppkey""",
+    "Q16": """SELECT p_brand, p_type, p_size, COUNT(DISTINCT ps_suppkey) AS supplier_cnt FROM partsupp, part
+        WHERE p_partkey = ps_partkey AND p_brand <> 'Brand#45' AND p_type NOT LIKE 'MEDIUM POLISHED%'
+        AND p_size IN (49, 14, 23, 45, 19, 3, 36, 9) AND ps_suppkey NOT IN (SELECT s_suppkey FROM supplier
+        WHERE s_comment LIKE '%Customer%Complaints%') GROUP BY p_brand, p_type, p_size
+        ORDER BY supplier_cnt DESC, p_brand, p_type, p_size""",
     "Q17": """SELECT SUM(l_extendedprice) / 7.0 FROM lineitem, part WHERE p_partkey = l_partkey
         AND p_brand = 'Brand#23' AND p_container = 'MED BOX' AND l_quantity < (SELECT 0.2 * AVG(l_quantity)
         FROM lineitem WHERE l_partkey = p_partkey)""",
@@ -429,5 +454,5 @@ def test_oracle_hand_cases():
 def test_cli_usage_errors():
     from gpudbmetalcodegen_amd import tpch_cli
 
-    assert tpch_cli.main(["q2"]) == 3  # not planned by this engine
+    assert tpch_cli.main(["q23"]) == 3  # no such query
     assert tpch_cli.main(["bogus"]) == 3

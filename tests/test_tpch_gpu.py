@@ -31,9 +31,9 @@ PARAMS["Q18"]["q18_qty"] = 25000
 @pytest.fixture(scope="module")
 def env(tmp_path_factory):
     d = tmp_path_factory.mktemp("tpch_gpu")
-    from test_tpch import q19_parts
+    from test_tpch import q16_rows, q19_parts
 
-    G.write_tbl(q19_parts(G.generate(SF, seed=5)), d, T.table_columns)
+    G.write_tbl(q16_rows(q19_parts(G.generate(SF, seed=5))), d, T.table_columns)
     T.ingest(d)
     host = T.load_dir(d)
     ctx = E.Context(0)
