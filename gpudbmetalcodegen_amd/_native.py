@@ -142,6 +142,8 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_gen_vocabulary", [u64, i32, i32, i32, vp])
     _sig(eng, "gdb_gen_text", [u64, u64, u64, i32, vp, i32, vp])
     _sig(eng, "gdb_plan_build", [cstr, cstr, P(vp)])
+    _sig(eng, "gdb_plan_build_sql", [cstr, cstr, P(vp)])
+    _sig(eng, "gdb_plan_sql_layout", [vp, vp, sz, P(sz)])
     _sig(eng, "gdb_plan_describe", [vp, vp, sz, P(sz)])
     _sig(eng, "gdb_plan_source", [vp, vp, sz, P(sz)])
     _sig(eng, "gdb_plan_phase_count", [vp, P(i32)])

@@ -282,11 +282,21 @@ GDB_DEV i64 gdb_ht_find(const u64* __restrict__ tab, u32 slot_words, u32 bits, u
 GDB_DEV u64 gdb_hash_slot64(u64 key, u32 bits) { return (key * 0x9E3779B97F4A7C15ull) >> (64u - bits); }
 // Insert key (payload stored by the inserting row only); *slot receives the
 // key's slot, new or existing (-1 on overflow). Returns 1 when this call
-// inserted the key, 0 when it was present, -1 on overflow.
+// inserted the key, 0 when it was present, -1 on overflow. A plain read
+// comes first: a key, once written, never changes, so a slot already
+// holding it (a hot group key) is recognised without an atomic; the
+// compare-and-swap decides only for slots read as empty (a stale empty read
+// just falls through to it).
 GDB_DEV int gdb_ht64_insert_at(u64* __restrict__ tab, u32 bits, u64 key, u64 payload, u32 max_probe, i64* slot) {
     const u64 mask = (1ull << bits) - 1ull;
     u64 s = gdb_hash_slot64(key, bits);
     for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
+        const u64 seen = tab[2 * s];
+        if (seen == key) {
+            *slot = (i64)s;
+            return 0;
+        }
+        if (seen != ~0ull) continue;
         u64 expected = ~0ull;
         if (gdb_cas_u64(tab + 2 * s, expected, key)) {
             tab[2 * s + 1] = payload;
@@ -300,6 +310,27 @@ GDB_DEV int gdb_ht64_insert_at(u64* __restrict__ tab, u32 bits, u64 key, u64 pay
     }
     *slot = -1;
     return -1;
+}
+// Block-level first-arrival filter for a distinct insert: an LDS set (512
+// slots, ~0 = empty) of the keys this block has already sent to the global
+// table. True when this lane is the block's first with the key, or when 8
+// probes find no room (many distinct keys): only those lanes touch the
+// global table, so a few hot keys cost a few global operations per block
+// instead of one per row.
+GDB_DEV bool gdb_block_first(u64* seen, u64 key) {
+    u32 h = (u32)((key * 0x9E3779B97F4A7C15ull) >> 55);
+    for (int p = 0; p < 8; ++p, h = (h + 1) & 511u) {
+        const u64 cur = __hip_atomic_load(seen + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        if (cur == key) return false;
+        if (cur == ~0ull) {
+            u64 exp = ~0ull;
+            if (__hip_atomic_compare_exchange_strong(seen + h, &exp, key, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                     __HIP_MEMORY_SCOPE_WORKGROUP))
+                return true;
+            if (exp == key) return false;
+        }
+    }
+    return true;
 }
 GDB_DEV int gdb_ht64_insert(u64* __restrict__ tab, u32 bits, u64 key, u64 payload, u32 max_probe) {
     i64 slot;
@@ -532,6 +563,58 @@ GDB_DEV u64 gdb_fnv64(u64 h, const char* p, int n) {
 GDB_DEV u64 gdb_fnv64_u32(u64 h, u32 v) {
     for (int i = 0; i < 4; ++i) h = (h ^ (u64)((v >> (8 * i)) & 255u)) * 0x100000001b3ull;
     return h;
+}
+
+GDB_DEV u64 gdb_fnv64_u64(u64 h, u64 v) {
+    for (int i = 0; i < 8; ++i) h = (h ^ ((v >> (8 * i)) & 255ull)) * 0x100000001b3ull;
+    return h;
+}
+
+// A NUL-padded field against a string literal of l bytes: equality, and
+// the SQL ordering (bytes compared up to the shorter length, then length).
+GDB_DEV bool gdb_str_eq_lit(const char* s, int width, const char* p, int l) {
+    if (l > width) return false;
+    u32 d = l == width ? 0u : (u32)(u8)s[l];
+    for (int i = 0; i < l; ++i) d |= (u32)(u8)(s[i] ^ p[i]);
+    return d == 0;
+}
+GDB_DEV int gdb_str_len(const char* s, int width) {
+    int n = width;
+    for (int i = width - 1; i >= 0; --i) n = s[i] == 0 ? i : n;
+    return n;
+}
+GDB_DEV int gdb_str_cmp_lit(const char* s, int width, const char* p, int l) {
+    const int n = gdb_str_len(s, width);
+    const int m = n < l ? n : l;
+    for (int i = 0; i < m; ++i) {
+        const int d = (int)(u8)s[i] - (int)(u8)p[i];
+        if (d != 0) return d;
+    }
+    return n - l;
+}
+
+// SQL LIKE of a NUL-padded field against a pattern of pl bytes with '%'
+// (any run) and '_' (any one byte): greedy matching that backtracks to the
+// last '%' (O(len * pl) at worst).
+GDB_DEV bool gdb_like_match(const char* s, int width, const char* p, int pl) {
+    const int n = gdb_str_len(s, width);
+    int i = 0, j = 0, star = -1, mark = 0;
+    while (i < n) {
+        if (j < pl && p[j] != '%' && (p[j] == '_' || p[j] == s[i])) {
+            ++i;
+            ++j;
+        } else if (j < pl && p[j] == '%') {
+            star = j++;
+            mark = i;
+        } else if (star >= 0) {
+            j = star + 1;
+            i = ++mark;
+        } else {
+            return false;
+        }
+    }
+    while (j < pl && p[j] == '%') ++j;
+    return j == pl;
 }
 
 // Byte equality of two n-byte fields.

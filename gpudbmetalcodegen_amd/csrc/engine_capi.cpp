@@ -9,6 +9,7 @@
 #include "infra/colbin.h"
 #include "infra/generators.h"
 #include "planning/hip_plan_builder.h"
+#include "planning/sql_frontend.h"
 #include "runtime/autotuner.h"
 #include "runtime/batcher.h"
 #include "runtime/streaming.h"
@@ -174,6 +175,26 @@ gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out) 
         *out = p;
         return GDB_OK;
     });
+}
+
+gdb_status gdb_plan_build_sql(const char* sql, const char* options, gdb_plan* out) {
+    return guard([&]() -> gdb_status {
+        if (!sql || !out) return setErr(GDB_ERR_INVALID, "gdb_plan_build_sql: null argument");
+        auto* p = new gdb_plan_s();
+        try {
+            p->plan = buildSqlPlan(sql, parsePlanOptions(options ? options : ""));
+            p->regenerate();
+        } catch (...) {
+            delete p;
+            throw;
+        }
+        *out = p;
+        return GDB_OK;
+    });
+}
+
+gdb_status gdb_plan_sql_layout(gdb_plan p, char* buf, size_t cap, size_t* need) {
+    return guard([&]() -> gdb_status { return copyOut(p->plan.sqlLayout, buf, cap, need); });
 }
 
 gdb_status gdb_plan_describe(gdb_plan p, char* buf, size_t cap, size_t* need) {

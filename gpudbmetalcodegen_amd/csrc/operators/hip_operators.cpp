@@ -871,12 +871,24 @@ void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addLine("const u32 _maxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
                " : 65536ull);");
     cg.addLine("u32 _ins_" + table_ + " = 0, _ovf_" + table_ + " = 0;");
+    if (blockFirst_) {
+        if (!distinct_) throw std::runtime_error("HipHash64Build(" + table_ + "): the block filter needs distinct");
+        cg.declareLds("u64", "_seen_" + table_, "512");
+        cg.addLine("for (int _k = (int)threadIdx.x; _k < 512; _k += (int)blockDim.x) _seen_" + table_ + "[_k] = ~0ull;");
+        cg.addLine("__syncthreads();");
+    }
     child_->produce(cg, [&]() {
         const std::string r = cg.uniqueName("_hr");
         const std::string slot = slotVar_.empty() ? cg.uniqueName("_hs") : slotVar_;
-        cg.addLine("i64 " + slot + ";");
-        cg.addLine("const int " + r + " = gdb_ht64_insert_at(" + table_ + ", _bits_" + table_ + ", (u64)(" + key_ +
-                   "), (u64)(" + payload_ + "), _maxp_" + table_ + ", &" + slot + ");");
+        const std::string k = cg.uniqueName("_hk");
+        cg.addLine("i64 " + slot + " = -1;");
+        cg.addLine("const u64 " + k + " = (u64)(" + key_ + ");");
+        const std::string call = "gdb_ht64_insert_at(" + table_ + ", _bits_" + table_ + ", " + k + ", (u64)(" +
+                                 payload_ + "), _maxp_" + table_ + ", &" + slot + ")";
+        if (blockFirst_)
+            cg.addLine("const int " + r + " = gdb_block_first(_seen_" + table_ + ", " + k + ") ? " + call + " : 0;");
+        else
+            cg.addLine("const int " + r + " = " + call + ";");
         cg.addLine("_ins_" + table_ + " += " + r + " == 1 ? 1u : 0u;");
         cg.addLine("_ovf_" + table_ + " |= " + r + " < 0 ? 1u : 0u;");
         if (distinct_)
@@ -892,6 +904,68 @@ void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 std::string HipHash64Build::describe() const {
     return std::string(distinct_ ? "Hash64BuildDistinct(" : "Hash64Build(") + table_ + ", key=" + key_ +
            ", payload=" + payload_ + ", slots=" + slots_ + ")";
+}
+
+HipDenseGroupId::HipDenseGroupId(std::unique_ptr<HipOperator> child, const std::string& table,
+                                 const std::string& slotVar, const std::string& slotsExpr, const std::string& idArray,
+                                 long long cap)
+    : HipUnaryOperator(std::move(child)), table_(table), slotVar_(slotVar), slots_(slotsExpr), ids_(idArray), cap_(cap) {}
+
+void HipDenseGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string st = table_ + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.addBufferParam(ids_, "u32", 4, slots_, InitKind::Zero, {}, false, true);
+    child_->produce(cg, [&]() {
+        cg.addBlock("", [&]() {
+            cg.addLine("const u64 _id = gdb_atomic_add_u64_ret(" + st + " + 3, 1ull);");
+            cg.addLine("if (_id < " + s(cap_) + "ull) " + ids_ + "[(u64)" + slotVar_ + "] = (u32)(_id + 1);");
+            cg.addLine("else gdb_atomic_max_u64(" + st + ", 1ull);");
+        });
+        consume();
+    });
+}
+
+std::string HipDenseGroupId::describe() const {
+    return "DenseGroupId(" + ids_ + "[" + slotVar_ + "] = next id + 1, cap " + s(cap_) + ")";
+}
+
+HipDirectGroupId::HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr, long long size,
+                                   const std::string& idArray, const std::string& status, long long cap)
+    : HipUnaryOperator(std::move(child)), key_(keyExpr), ids_(idArray), status_(status), size_(size), cap_(cap) {}
+
+void HipDirectGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(ids_, "u32", 4, s(size_), InitKind::Zero, {}, false, true);
+    cg.addBufferParam(status_, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(status_);
+    cg.declareLds("u64", "_seen_" + ids_, "512");
+    cg.addLine("for (int _k = (int)threadIdx.x; _k < 512; _k += (int)blockDim.x) _seen_" + ids_ + "[_k] = ~0ull;");
+    cg.addLine("__syncthreads();");
+    child_->produce(cg, [&]() {
+        cg.addBlock("", [&]() {
+            cg.addLine("const u64 _dk = (u64)(" + key_ + ");");
+            cg.addBlock("if (_dk < " + s(size_) + "ull && gdb_block_first(_seen_" + ids_ + ", _dk) && " + ids_ +
+                            "[_dk] == 0u)",
+                        [&]() {
+                            cg.addLine("u32 _exp = 0u;");
+                            cg.addBlock("if (__hip_atomic_compare_exchange_strong(" + ids_ +
+                                            " + _dk, &_exp, 0xFFFFFFFFu, __ATOMIC_RELAXED, __ATOMIC_RELAXED, "
+                                            "__HIP_MEMORY_SCOPE_AGENT))",
+                                        [&]() {
+                                            cg.addLine("const u64 _id = gdb_atomic_add_u64_ret(" + status_ +
+                                                       " + 3, 1ull);");
+                                            cg.addLine("if (_id < " + s(cap_) + "ull) __hip_atomic_store(" + ids_ +
+                                                       " + _dk, (u32)(_id + 1), __ATOMIC_RELAXED, "
+                                                       "__HIP_MEMORY_SCOPE_AGENT);");
+                                            cg.addLine("else gdb_atomic_max_u64(" + status_ + ", 1ull);");
+                                        });
+                        });
+        });
+        consume();
+    });
+}
+
+std::string HipDirectGroupId::describe() const {
+    return "DirectGroupId(" + ids_ + "[" + key_ + "] = next id + 1, cap " + s(cap_) + ")";
 }
 
 HipRowCheck::HipRowCheck(std::unique_ptr<HipOperator> child, const std::string& name, const std::string& condition,
@@ -1058,14 +1132,15 @@ void HipKeyedAgg::addAverage(const std::string& name, const std::string& numerat
 
 void HipKeyedAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     if (aggs_.empty()) throw std::runtime_error("HipKeyedAgg(" + out_ + "): no aggregates");
-    // Word layout per bucket; buckets are padded to 128 bytes so the per-block
-    // atomics of different buckets land on different cache lines.
+    // Word layout per bucket; up to 4096 buckets are padded to 128 bytes so
+    // the per-block atomics of different buckets land on different cache
+    // lines (more buckets spread their atomics by themselves: packed).
     int words = 0;
     for (auto& a : aggs_) {
         a.wordOffset = words;
         words += (a.kind == AggKind::SumI128) ? 2 : 1;
     }
-    const int wpb = ((words + 15) / 16) * 16;
+    const int wpb = numBuckets_ > 4096 ? words : ((words + 15) / 16) * 16;
     std::vector<uint64_t> pat(static_cast<size_t>(wpb), 0);
     for (const auto& a : aggs_) {
         if (a.kind == AggKind::MinI64) pat[a.wordOffset] = kI64Max;

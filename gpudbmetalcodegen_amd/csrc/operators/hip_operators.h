@@ -213,6 +213,10 @@ public:
     HipHash64Build(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
                    const std::string& payloadExpr, const std::string& slotsExpr, bool distinct = false,
                    const std::string& slotVar = "");
+    // Distinct builds with a few hot keys (GROUP BY flags): an LDS set of the
+    // keys each block already sent (gdb_block_first) keeps all but a block's
+    // first arrival per key away from the global table.
+    void setBlockFirst() { blockFirst_ = true; }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
@@ -220,6 +224,41 @@ private:
     std::string table_, key_, payload_, slots_;
     bool distinct_ = false;
     std::string slotVar_;
+    bool blockFirst_ = false;
+};
+
+// Dense group numbering over a HipHash64Build(distinct, slotVar) in the same
+// phase: each first arrival takes the next id from word 3 of the table's
+// status words and stores id + 1 at its key's slot in `idArray` (u32, one
+// per slot); an id at or beyond `cap` flags the table's overflow word, so
+// the caller can rebuild with a larger cap.
+class HipDenseGroupId : public HipUnaryOperator {
+public:
+    HipDenseGroupId(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& slotVar,
+                    const std::string& slotsExpr, const std::string& idArray, long long cap);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_, slotVar_, slots_, ids_;
+    long long cap_;
+};
+
+// Dense group numbering over a small direct-addressed key domain (a packed
+// GROUP BY key below `size`): the block's first arrival per key (an LDS set)
+// claims ids[key] with a compare-and-swap from 0, takes the next id from word
+// 3 of `status` and stores id + 1; ids at or beyond `cap` leave the claim
+// mark (~0) and flag word 0, so the caller rebuilds with the exact count.
+class HipDirectGroupId : public HipUnaryOperator {
+public:
+    HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr, long long size,
+                     const std::string& idArray, const std::string& status, long long cap);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string key_, ids_, status_;
+    long long size_, cap_;
 };
 
 // Row check: counts the rows for which `condition` is false and fails the

@@ -79,6 +79,10 @@ struct HipQueryPlan {
     // Size symbols the caller (or a sizing pre-pass) must register.
     std::vector<std::string> requiredSymbols;
 
+    // Plans built from SQL text: how the caller finishes the result rows
+    // (JSON; sql_frontend.h). Empty for named plans.
+    std::string sqlLayout;
+
     std::string describe() const;  // the --print-plan view
 };
 

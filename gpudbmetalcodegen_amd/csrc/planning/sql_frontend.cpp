@@ -1,0 +1,1100 @@
+#include "sql_frontend.h"
+
+#include "../infra/colbin.h"
+#include "../operators/hip_operators.h"
+#include "plan_util.h"
+
+#include <algorithm>
+#include <cctype>
+#include <cstdio>
+#include <map>
+#include <memory>
+#include <set>
+#include <stdexcept>
+#include <vector>
+
+namespace gdb {
+namespace codegen {
+
+namespace {
+
+[[noreturn]] void fail(const std::string& m) { throw std::runtime_error("SQL: " + m); }
+
+// ===========================================================================
+// Lexer
+// ===========================================================================
+
+struct Tok {
+    enum Kind { Ident, Number, String, Sym, End } kind;
+    std::string v;  // identifiers lower-cased; strings unquoted
+};
+
+std::vector<Tok> lex(const std::string& q) {
+    std::vector<Tok> out;
+    const size_t n = q.size();
+    size_t i = 0;
+    while (i < n) {
+        const unsigned char c = static_cast<unsigned char>(q[i]);
+        if (std::isspace(c)) {
+            ++i;
+        } else if (c == '-' && i + 1 < n && q[i + 1] == '-') {
+            while (i < n && q[i] != '\n') ++i;
+        } else if (std::isalpha(c) || c == '_') {
+            size_t j = i;
+            std::string w;
+            while (j < n && (std::isalnum(static_cast<unsigned char>(q[j])) || q[j] == '_'))
+                w += static_cast<char>(std::tolower(static_cast<unsigned char>(q[j++])));
+            out.push_back({Tok::Ident, w});
+            i = j;
+        } else if (std::isdigit(c) || (c == '.' && i + 1 < n && std::isdigit(static_cast<unsigned char>(q[i + 1])))) {
+            size_t j = i;
+            bool dot = false;
+            while (j < n && (std::isdigit(static_cast<unsigned char>(q[j])) || (q[j] == '.' && !dot))) dot |= q[j++] == '.';
+            out.push_back({Tok::Number, q.substr(i, j - i)});
+            i = j;
+        } else if (c == '\'') {
+            std::string w;
+            size_t j = i + 1;
+            for (;;) {
+                if (j >= n) fail("unterminated string literal");
+                if (q[j] == '\'') {
+                    if (j + 1 < n && q[j + 1] == '\'') {
+                        w += '\'';
+                        j += 2;
+                        continue;
+                    }
+                    break;
+                }
+                w += q[j++];
+            }
+            out.push_back({Tok::String, w});
+            i = j + 1;
+        } else {
+            static const char* two[] = {"<=", ">=", "<>", "!="};
+            bool done = false;
+            for (const char* t : two)
+                if (q.compare(i, 2, t) == 0) {
+                    out.push_back({Tok::Sym, t});
+                    i += 2;
+                    done = true;
+                    break;
+                }
+            if (done) continue;
+            if (std::string("(),*+-/=<>;.").find(static_cast<char>(c)) == std::string::npos)
+                fail(std::string("unexpected character '") + static_cast<char>(c) + "'");
+            out.push_back({Tok::Sym, std::string(1, static_cast<char>(c))});
+            ++i;
+        }
+    }
+    out.push_back({Tok::End, ""});
+    return out;
+}
+
+// ===========================================================================
+// AST and parser
+// ===========================================================================
+
+enum class NK { Col, Num, Str, Date, Interval, Neg, Arith, Cmp, And, Or, Not, Between, In, Like, Case, Agg, Extract, Star };
+
+struct Node;
+using NodeP = std::shared_ptr<Node>;
+struct Node {
+    NK k;
+    std::string s;           // column, operator, function, string value, interval unit, extract field
+    std::vector<NodeP> a;    // operands
+    bool neg = false;        // NOT BETWEEN / NOT IN / NOT LIKE
+    bool distinct = false;   // COUNT(DISTINCT ...)
+    long long iv = 0;        // Num: value * 10^-scale; Date: YYYYMMDD; Interval: count
+    int scale = 0;
+};
+
+NodeP mk(NK k, const std::string& s = "", std::vector<NodeP> a = {}) {
+    auto n = std::make_shared<Node>();
+    n->k = k;
+    n->s = s;
+    n->a = std::move(a);
+    return n;
+}
+
+// Canonical text of an expression (matching GROUP BY / ORDER BY items with
+// select items).
+std::string text(const NodeP& n) {
+    switch (n->k) {
+        case NK::Col: return n->s;
+        case NK::Num: {
+            std::string d = std::to_string(n->iv < 0 ? -n->iv : n->iv);
+            if (n->scale > 0) {
+                if (static_cast<int>(d.size()) <= n->scale) d.insert(0, static_cast<size_t>(n->scale) + 1 - d.size(), '0');
+                d.insert(d.size() - static_cast<size_t>(n->scale), ".");
+            }
+            return (n->iv < 0 ? "-" : "") + d;
+        }
+        case NK::Str: return "'" + n->s + "'";
+        case NK::Date: return "date " + std::to_string(n->iv);
+        case NK::Interval: return "interval " + std::to_string(n->iv) + " " + n->s;
+        case NK::Neg: return "(-" + text(n->a[0]) + ")";
+        case NK::Arith:
+        case NK::Cmp: return "(" + text(n->a[0]) + " " + n->s + " " + text(n->a[1]) + ")";
+        case NK::And: return "(" + text(n->a[0]) + " and " + text(n->a[1]) + ")";
+        case NK::Or: return "(" + text(n->a[0]) + " or " + text(n->a[1]) + ")";
+        case NK::Not: return "(not " + text(n->a[0]) + ")";
+        case NK::Star: return "*";
+        case NK::Agg: return n->s + "(" + (n->distinct ? "distinct " : "") + text(n->a[0]) + ")";
+        case NK::Extract: return "extract(" + n->s + " from " + text(n->a[0]) + ")";
+        default: {
+            std::string t = "#" + std::to_string(static_cast<int>(n->k)) + (n->neg ? "!" : "") + n->s + "(";
+            for (const auto& c : n->a) t += text(c) + ",";
+            return t + ")";
+        }
+    }
+}
+
+struct SelectItem {
+    NodeP e;
+    std::string alias;
+};
+struct OrderItem {
+    NodeP e;
+    bool desc = false;
+};
+struct Select {
+    std::vector<SelectItem> items;
+    std::vector<std::string> tables;
+    NodeP where, having;
+    std::vector<NodeP> groupBy;
+    std::vector<OrderItem> orderBy;
+    long long limit = -1;
+};
+
+bool reserved(const std::string& w) {
+    static const std::set<std::string> r = {"select", "from", "where", "group", "order", "by", "limit", "having",
+                                            "as", "and", "or", "not", "between", "in", "like", "case", "when",
+                                            "then", "else", "end", "asc", "desc", "distinct", "is", "null",
+                                            "join", "on", "inner", "left", "right", "outer", "union", "exists"};
+    return r.count(w) > 0;
+}
+
+// YYYY-MM-DD -> YYYYMMDD
+long long parseDate(const std::string& s) {
+    int y = 0, m = 0, d = 0;
+    if (s.size() != 10 || s[4] != '-' || s[7] != '-' || std::sscanf(s.c_str(), "%4d-%2d-%2d", &y, &m, &d) != 3 ||
+        m < 1 || m > 12 || d < 1 || d > 31)
+        fail("bad date literal '" + s + "' (YYYY-MM-DD)");
+    return y * 10000LL + m * 100 + d;
+}
+
+class Parser {
+public:
+    explicit Parser(std::vector<Tok> t) : t_(std::move(t)) {}
+
+    Select select() {
+        Select q;
+        expectKw("select");
+        if (isKw("distinct")) fail("SELECT DISTINCT is not supported");
+        if (isSym("*")) fail("SELECT * (row output) is not supported; select aggregates");
+        do {
+            SelectItem it;
+            it.e = expr();
+            if (acceptKw("as")) {
+                it.alias = ident("an alias");
+            } else if (peek().kind == Tok::Ident && !reserved(peek().v)) {
+                it.alias = ident("an alias");
+            }
+            q.items.push_back(it);
+        } while (acceptSym(","));
+        expectKw("from");
+        do {
+            if (acceptSym("(")) fail("subqueries in FROM are not supported");
+            q.tables.push_back(ident("a table name"));
+            if (acceptKw("as")) ident("a table alias");
+            else if (peek().kind == Tok::Ident && !reserved(peek().v)) ident("a table alias");
+        } while (acceptSym(","));
+        if (isKw("join") || isKw("inner") || isKw("left") || isKw("right")) fail("JOIN is not supported");
+        if (acceptKw("where")) q.where = expr();
+        if (acceptKw("group")) {
+            expectKw("by");
+            do q.groupBy.push_back(expr());
+            while (acceptSym(","));
+        }
+        if (acceptKw("having")) q.having = expr();
+        if (acceptKw("order")) {
+            expectKw("by");
+            do {
+                OrderItem o;
+                o.e = expr();
+                if (acceptKw("desc")) o.desc = true;
+                else acceptKw("asc");
+                q.orderBy.push_back(o);
+            } while (acceptSym(","));
+        }
+        if (acceptKw("limit")) {
+            if (peek().kind != Tok::Number || peek().v.find('.') != std::string::npos) fail("LIMIT takes an integer");
+            q.limit = std::stoll(next().v);
+        }
+        acceptSym(";");
+        if (peek().kind != Tok::End) fail("unexpected '" + peek().v + "' after the query");
+        return q;
+    }
+
+private:
+    std::vector<Tok> t_;
+    size_t p_ = 0;
+
+    const Tok& peek(size_t k = 0) const { return t_[std::min(p_ + k, t_.size() - 1)]; }
+    const Tok& next() { return t_[std::min(p_++, t_.size() - 1)]; }
+    bool isKw(const char* w, size_t k = 0) const { return peek(k).kind == Tok::Ident && peek(k).v == w; }
+    bool isSym(const char* s, size_t k = 0) const { return peek(k).kind == Tok::Sym && peek(k).v == s; }
+    bool acceptKw(const char* w) {
+        if (!isKw(w)) return false;
+        ++p_;
+        return true;
+    }
+    bool acceptSym(const char* s) {
+        if (!isSym(s)) return false;
+        ++p_;
+        return true;
+    }
+    void expectKw(const char* w) {
+        if (!acceptKw(w)) fail(std::string("expected ") + w + " near '" + peek().v + "'");
+    }
+    void expectSym(const char* s) {
+        if (!acceptSym(s)) fail(std::string("expected '") + s + "' near '" + peek().v + "'");
+    }
+    std::string ident(const char* what) {
+        if (peek().kind != Tok::Ident) fail(std::string("expected ") + what + " near '" + peek().v + "'");
+        return next().v;
+    }
+
+    NodeP expr() { return orE(); }
+    NodeP orE() {
+        NodeP l = andE();
+        while (acceptKw("or")) l = mk(NK::Or, "", {l, andE()});
+        return l;
+    }
+    NodeP andE() {
+        NodeP l = notE();
+        while (acceptKw("and")) l = mk(NK::And, "", {l, notE()});
+        return l;
+    }
+    NodeP notE() {
+        if (acceptKw("not")) return mk(NK::Not, "", {notE()});
+        if (isKw("exists")) fail("EXISTS subqueries are not supported");
+        return pred();
+    }
+    NodeP pred() {
+        NodeP l = add();
+        for (const char* op : {"=", "<>", "!=", "<=", ">=", "<", ">"})
+            if (acceptSym(op)) {
+                if (isSym("(") && isKw("select", 1)) fail("scalar subqueries are not supported");
+                return mk(NK::Cmp, std::string(op) == "!=" ? "<>" : op, {l, add()});
+            }
+        if (isKw("is")) fail("IS [NOT] NULL is not supported (the TPC-H columns hold no NULLs)");
+        const bool neg = isKw("not") && (isKw("between", 1) || isKw("in", 1) || isKw("like", 1));
+        if (neg) ++p_;
+        if (acceptKw("between")) {
+            NodeP lo = add();
+            expectKw("and");
+            NodeP n = mk(NK::Between, "", {l, lo, add()});
+            n->neg = neg;
+            return n;
+        }
+        if (acceptKw("in")) {
+            expectSym("(");
+            if (isKw("select")) fail("IN (subquery) is not supported");
+            NodeP n = mk(NK::In, "", {l});
+            do n->a.push_back(add());
+            while (acceptSym(","));
+            expectSym(")");
+            n->neg = neg;
+            return n;
+        }
+        if (acceptKw("like")) {
+            if (peek().kind != Tok::String) fail("LIKE takes a string pattern");
+            NodeP n = mk(NK::Like, next().v, {l});
+            n->neg = neg;
+            return n;
+        }
+        return l;
+    }
+    NodeP add() {
+        NodeP l = mul();
+        for (;;) {
+            if (acceptSym("+")) l = mk(NK::Arith, "+", {l, mul()});
+            else if (acceptSym("-")) l = mk(NK::Arith, "-", {l, mul()});
+            else return l;
+        }
+    }
+    NodeP mul() {
+        NodeP l = unary();
+        for (;;) {
+            if (acceptSym("*")) l = mk(NK::Arith, "*", {l, unary()});
+            else if (acceptSym("/")) l = mk(NK::Arith, "/", {l, unary()});
+            else return l;
+        }
+    }
+    NodeP unary() {
+        if (acceptSym("-")) {
+            NodeP v = unary();
+            if (v->k == NK::Num) {
+                v->iv = -v->iv;
+                return v;
+            }
+            return mk(NK::Neg, "", {v});
+        }
+        acceptSym("+");
+        return primary();
+    }
+    NodeP primary() {
+        const Tok& t = peek();
+        if (t.kind == Tok::Number) {
+            std::string d = next().v;
+            NodeP n = mk(NK::Num);
+            const size_t dot = d.find('.');
+            if (dot != std::string::npos) {
+                n->scale = static_cast<int>(d.size() - dot - 1);
+                d.erase(dot, 1);
+            }
+            if (d.size() > 18) fail("numeric literal " + t.v + " has too many digits");
+            n->iv = d.empty() ? 0 : std::stoll(d);
+            return n;
+        }
+        if (t.kind == Tok::String) return mk(NK::Str, next().v);
+        if (acceptSym("(")) {
+            if (isKw("select")) fail("subqueries are not supported");
+            NodeP e = expr();
+            expectSym(")");
+            return e;
+        }
+        if (t.kind != Tok::Ident) fail("unexpected '" + t.v + "'");
+        const std::string w = next().v;
+        if (w == "date" && peek().kind == Tok::String) {
+            NodeP n = mk(NK::Date);
+            n->iv = parseDate(next().v);
+            return n;
+        }
+        if (w == "interval") {
+            if (peek().kind != Tok::String) fail("INTERVAL takes a quoted count");
+            NodeP n = mk(NK::Interval);
+            n->iv = std::stoll(next().v);
+            std::string unit = ident("an interval unit");
+            if (!unit.empty() && unit.back() == 's') unit.pop_back();
+            if (unit != "day" && unit != "month" && unit != "year") fail("interval unit " + unit);
+            n->s = unit;
+            return n;
+        }
+        if (w == "case") {
+            if (!isKw("when")) fail("only searched CASE WHEN ... is supported");
+            NodeP n = mk(NK::Case);
+            while (acceptKw("when")) {
+                n->a.push_back(expr());
+                expectKw("then");
+                n->a.push_back(expr());
+            }
+            if (!acceptKw("else")) fail("CASE without ELSE (NULL results) is not supported");
+            n->a.push_back(expr());
+            expectKw("end");
+            return n;
+        }
+        if (w == "extract") {
+            expectSym("(");
+            const std::string f = ident("a date field");
+            if (f != "year" && f != "month" && f != "day") fail("EXTRACT(" + f + ")");
+            expectKw("from");
+            NodeP n = mk(NK::Extract, f, {expr()});
+            expectSym(")");
+            return n;
+        }
+        if ((w == "sum" || w == "avg" || w == "min" || w == "max" || w == "count") && isSym("(")) {
+            ++p_;
+            NodeP n = mk(NK::Agg, w);
+            n->distinct = acceptKw("distinct");
+            if (acceptSym("*")) {
+                if (w != "count") fail(w + "(*)");
+                n->a.push_back(mk(NK::Star));
+            } else {
+                n->a.push_back(expr());
+            }
+            expectSym(")");
+            return n;
+        }
+        if (isSym("(")) fail("function " + w + "() is not supported");
+        if (reserved(w)) fail("unexpected keyword " + w);
+        if (acceptSym(".")) return mk(NK::Col, ident("a column name"));  // qualifier dropped
+        return mk(NK::Col, w);
+    }
+};
+
+// ===========================================================================
+// Dates
+// ===========================================================================
+
+long long daysFromCivil(long long y, long long m, long long d) {  // days since 1970-01-01 (proleptic Gregorian)
+    y -= m <= 2;
+    const long long era = (y >= 0 ? y : y - 399) / 400;
+    const long long yoe = y - era * 400;
+    const long long doy = (153 * (m + (m > 2 ? -3 : 9)) + 2) / 5 + d - 1;
+    const long long doe = yoe * 365 + yoe / 4 - yoe / 100 + doy;
+    return era * 146097 + doe - 719468;
+}
+long long civilFromDays(long long z) {  // -> YYYYMMDD
+    z += 719468;
+    const long long era = (z >= 0 ? z : z - 146096) / 146097;
+    const long long doe = z - era * 146097;
+    const long long yoe = (doe - doe / 1460 + doe / 36524 - doe / 146096) / 365;
+    const long long doy = doe - (365 * yoe + yoe / 4 - yoe / 100);
+    const long long mp = (5 * doy + 2) / 153;
+    const long long d = doy - (153 * mp + 2) / 5 + 1;
+    const long long m = mp + (mp < 10 ? 3 : -9);
+    const long long y = yoe + era * 400 + (m <= 2);
+    return y * 10000 + m * 100 + d;
+}
+long long shiftDate(long long ymd, long long n, const std::string& unit) {
+    long long y = ymd / 10000, m = ymd / 100 % 100, d = ymd % 100;
+    if (unit == "day") return civilFromDays(daysFromCivil(y, m, d) + n);
+    long long months = y * 12 + (m - 1) + (unit == "year" ? 12 * n : n);
+    y = months / 12;
+    m = months % 12 + 1;
+    static const int mdays[] = {31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31};
+    long long last = mdays[m - 1] + (m == 2 && ((y % 4 == 0 && y % 100 != 0) || y % 400 == 0));
+    return y * 10000 + m * 100 + std::min(d, last);  // clamped to the month's end
+}
+
+// ===========================================================================
+// Compiler: expressions -> HIP code in exact fixed point
+// ===========================================================================
+
+enum class VT { Num, Date, Char1, Str, Bool, Interval };
+const char* vtName(VT t) {
+    switch (t) {
+        case VT::Num: return "number";
+        case VT::Date: return "date";
+        case VT::Char1: return "char(1)";
+        case VT::Str: return "string";
+        case VT::Bool: return "boolean";
+        case VT::Interval: return "interval";
+    }
+    return "?";
+}
+
+struct Val {
+    VT t = VT::Num;
+    std::string code;
+    int scale = 0;
+    bool cst = false;
+    long long cv = 0;  // Num (scaled), Date (YYYYMMDD), Char1 (byte), Interval (count)
+    std::string sv;    // Str constant; Interval unit; Str column name
+    int width = 0;     // Str column
+};
+
+long long pow10(int k) {
+    long long p = 1;
+    for (int i = 0; i < k; ++i) {
+        if (p > 922337203685477580LL) fail("fixed-point scale overflow");
+        p *= 10;
+    }
+    return p;
+}
+
+long long mulChecked(long long a, long long b) {
+    long long r;
+    if (__builtin_mul_overflow(a, b, &r)) fail("constant overflows 64 bits");
+    return r;
+}
+
+std::string lit(long long v) { return "(" + std::to_string(v) + "ll)"; }
+
+std::string cstr(const std::string& s) {
+    std::string o = "\"";
+    for (unsigned char c : s) {
+        if (std::isalnum(c) || c == ' ' || std::string("#%_-:.,!?@&*+=<>/|~^[]{}();$").find(static_cast<char>(c)) !=
+                                                 std::string::npos) {
+            o += static_cast<char>(c);
+        } else {
+            char b[8];
+            std::snprintf(b, sizeof b, "\\%03o", c);
+            o += b;
+        }
+    }
+    return o + "\"";
+}
+
+Val numConst(long long v, int scale) {
+    Val r;
+    r.t = VT::Num;
+    r.cst = true;
+    r.cv = v;
+    r.scale = scale;
+    r.code = lit(v);
+    return r;
+}
+
+Val upscale(const Val& v, int s) {
+    if (v.scale == s) return v;
+    if (v.cst) return numConst(mulChecked(v.cv, pow10(s - v.scale)), s);
+    Val r = v;
+    r.code = "(" + v.code + " * " + lit(pow10(s - v.scale)) + ")";
+    r.scale = s;
+    return r;
+}
+
+struct Compiler {
+    std::string table;
+    const infra::TpchTable* cat = nullptr;
+    std::set<std::string> used;  // scanned columns
+
+    // rep empty: the scanned row's bindings; else the row index `rep`.
+    Val column(const std::string& name, const std::string& rep) {
+        const infra::TpchColumn* c = nullptr;
+        for (const auto& col : cat->columns)
+            if (col.name == name) c = &col;
+        if (!c) fail("no column " + name + " in " + table);
+        used.insert(name);
+        const std::string ptr = tableDataName(table, name);
+        Val v;
+        const std::string at = rep.empty() ? name : ptr + "[(u64)(" + rep + ")]";
+        switch (c->type) {
+            case infra::TpchType::INT: v.t = VT::Num; v.code = "(i64)" + at; break;
+            case infra::TpchType::FLOAT: v.t = VT::Num; v.scale = 2; v.code = "(i64)" + at; break;
+            case infra::TpchType::DATE: v.t = VT::Date; v.code = "(i64)" + at; break;
+            case infra::TpchType::CHAR1: v.t = VT::Char1; v.code = "(i64)(u8)" + at; break;
+            case infra::TpchType::CHAR_FIXED:
+                v.t = VT::Str;
+                v.width = c->width;
+                v.sv = name;
+                v.code = rep.empty() ? name
+                                     : "((const char*)" + ptr + " + (u64)(" + rep + ") * " + std::to_string(c->width) + "ull)";
+                break;
+        }
+        return v;
+    }
+
+    Val arith(const std::string& op, Val a, Val b) {
+        if (a.t == VT::Date && b.t == VT::Interval && (op == "+" || op == "-")) {
+            if (!a.cst) fail("date arithmetic on a column is not supported");
+            Val r = a;
+            r.cv = shiftDate(a.cv, op == "+" ? b.cv : -b.cv, b.sv);
+            r.code = lit(r.cv);
+            return r;
+        }
+        if (a.t != VT::Num || b.t != VT::Num)
+            fail(std::string("arithmetic on ") + vtName(a.t) + " and " + vtName(b.t));
+        if (op == "/") fail("division of row values is not exact (divide aggregates instead)");
+        if (op == "*") {
+            if (a.cst && b.cst) return numConst(mulChecked(a.cv, b.cv), a.scale + b.scale);
+            Val r;
+            r.t = VT::Num;
+            r.scale = a.scale + b.scale;
+            r.code = "(" + a.code + " * " + b.code + ")";
+            return r;
+        }
+        const int s = std::max(a.scale, b.scale);
+        a = upscale(a, s);
+        b = upscale(b, s);
+        if (a.cst && b.cst) {
+            long long r;
+            if (op == "+" ? __builtin_add_overflow(a.cv, b.cv, &r) : __builtin_sub_overflow(a.cv, b.cv, &r))
+                fail("constant overflows 64 bits");
+            return numConst(r, s);
+        }
+        Val r;
+        r.t = VT::Num;
+        r.scale = s;
+        r.code = "(" + a.code + " " + op + " " + b.code + ")";
+        return r;
+    }
+
+    static Val boolOf(const std::string& code) {
+        Val r;
+        r.t = VT::Bool;
+        r.code = code;
+        return r;
+    }
+
+    // A string constant met in a date or char(1) context.
+    static Val coerce(const Val& v, VT to) {
+        if (v.t != VT::Str || !v.cst) return v;
+        Val r;
+        r.cst = true;
+        if (to == VT::Date) {
+            r.t = VT::Date;
+            r.cv = parseDate(v.sv);
+        } else if (to == VT::Char1) {
+            if (v.sv.size() > 1) fail("char(1) column compared with '" + v.sv + "'");
+            r.t = VT::Char1;
+            r.cv = v.sv.empty() ? 0 : static_cast<unsigned char>(v.sv[0]);
+        } else {
+            return v;
+        }
+        r.code = lit(r.cv);
+        return r;
+    }
+
+    Val cmp(std::string op, Val a, Val b) {
+        a = coerce(a, b.t);
+        b = coerce(b, a.t);
+        const std::string cop = op == "=" ? "==" : op == "<>" ? "!=" : op;
+        if (a.t == VT::Str && b.t == VT::Str) {
+            if (a.cst && !b.cst) {
+                std::swap(a, b);
+                static const std::map<std::string, std::string> flip = {{"=", "="}, {"<>", "<>"}, {"<", ">"},
+                                                                        {">", "<"}, {"<=", ">="}, {">=", "<="}};
+                op = flip.at(op);
+            }
+            if (a.cst || !b.cst) fail("string comparison needs a column and a constant");
+            const std::string args =
+                a.code + ", " + std::to_string(a.width) + ", " + cstr(b.sv) + ", " + std::to_string(b.sv.size());
+            if (op == "=") return boolOf("gdb_str_eq_lit(" + args + ")");
+            if (op == "<>") return boolOf("!gdb_str_eq_lit(" + args + ")");
+            return boolOf("(gdb_str_cmp_lit(" + args + ") " + op + " 0)");
+        }
+        if (a.t != b.t) fail(std::string("comparison of ") + vtName(a.t) + " with " + vtName(b.t));
+        if (a.t == VT::Num) {
+            const int s = std::max(a.scale, b.scale);
+            a = upscale(a, s);
+            b = upscale(b, s);
+        } else if (a.t != VT::Date && a.t != VT::Char1) {
+            fail(std::string("comparison of ") + vtName(a.t) + " values");
+        }
+        return boolOf("(" + a.code + " " + cop + " " + b.code + ")");
+    }
+
+    Val like(const Val& x, const std::string& pat, bool neg) {
+        if (x.t != VT::Str || x.cst) fail("LIKE needs a string column");
+        const size_t pct = pat.find('%');
+        const bool plainPrefix = pat.find('_') == std::string::npos && pct != std::string::npos &&
+                                 pct == pat.size() - 1;
+        std::string c;
+        const std::string W = std::to_string(x.width);
+        if (pat.find('%') == std::string::npos && pat.find('_') == std::string::npos)
+            c = "gdb_str_eq_lit(" + x.code + ", " + W + ", " + cstr(pat) + ", " + std::to_string(pat.size()) + ")";
+        else if (plainPrefix)
+            c = "gdb_like_prefix(" + x.code + ", " + W + ", " + cstr(pat.substr(0, pct)) + ", " + std::to_string(pct) + ")";
+        else
+            c = "gdb_like_match(" + x.code + ", " + W + ", " + cstr(pat) + ", " + std::to_string(pat.size()) + ")";
+        return boolOf(neg ? "!" + c : c);
+    }
+
+    Val expr(const NodeP& n, const std::string& rep = "") {
+        switch (n->k) {
+            case NK::Col: return column(n->s, rep);
+            case NK::Num: return numConst(n->iv, n->scale);
+            case NK::Str: {
+                Val v;
+                v.t = VT::Str;
+                v.cst = true;
+                v.sv = n->s;
+                return v;
+            }
+            case NK::Date: {
+                Val v;
+                v.t = VT::Date;
+                v.cst = true;
+                v.cv = n->iv;
+                v.code = lit(n->iv);
+                return v;
+            }
+            case NK::Interval: {
+                Val v;
+                v.t = VT::Interval;
+                v.cst = true;
+                v.cv = n->iv;
+                v.sv = n->s;
+                return v;
+            }
+            case NK::Neg: {
+                Val v = expr(n->a[0], rep);
+                if (v.t != VT::Num) fail("negation of a non-number");
+                if (v.cst) return numConst(-v.cv, v.scale);
+                v.code = "(-" + v.code + ")";
+                return v;
+            }
+            case NK::Arith: return arith(n->s, expr(n->a[0], rep), expr(n->a[1], rep));
+            case NK::Cmp: return cmp(n->s, expr(n->a[0], rep), expr(n->a[1], rep));
+            case NK::And:
+            case NK::Or: {
+                Val a = expr(n->a[0], rep), b = expr(n->a[1], rep);
+                if (a.t != VT::Bool || b.t != VT::Bool) fail("AND / OR of non-boolean operands");
+                return boolOf("(" + a.code + (n->k == NK::And ? " && " : " || ") + b.code + ")");
+            }
+            case NK::Not: {
+                Val a = expr(n->a[0], rep);
+                if (a.t != VT::Bool) fail("NOT of a non-boolean");
+                return boolOf("(!" + a.code + ")");
+            }
+            case NK::Between: {
+                Val x = expr(n->a[0], rep);
+                Val c = boolOf("(" + cmp(">=", x, expr(n->a[1], rep)).code + " && " + cmp("<=", x, expr(n->a[2], rep)).code +
+                               ")");
+                if (n->neg) c.code = "(!" + c.code + ")";
+                return c;
+            }
+            case NK::In: {
+                Val x = expr(n->a[0], rep);
+                std::string c;
+                for (size_t i = 1; i < n->a.size(); ++i) c += (i > 1 ? " || " : "") + cmp("=", x, expr(n->a[i], rep)).code;
+                return boolOf(n->neg ? "(!(" + c + "))" : "(" + c + ")");
+            }
+            case NK::Like: return like(expr(n->a[0], rep), n->s, n->neg);
+            case NK::Case: {
+                std::vector<Val> conds, vals;
+                for (size_t i = 0; i + 1 < n->a.size(); i += 2) {
+                    conds.push_back(expr(n->a[i], rep));
+                    if (conds.back().t != VT::Bool) fail("CASE WHEN needs a boolean condition");
+                    vals.push_back(expr(n->a[i + 1], rep));
+                }
+                vals.push_back(expr(n->a.back(), rep));
+                int s = 0;
+                for (const auto& v : vals) {
+                    if (v.t != VT::Num) fail("CASE results must be numbers");
+                    s = std::max(s, v.scale);
+                }
+                std::string c = upscale(vals.back(), s).code;
+                for (size_t i = conds.size(); i-- > 0;)
+                    c = "(" + conds[i].code + " ? " + upscale(vals[i], s).code + " : " + c + ")";
+                Val r;
+                r.t = VT::Num;
+                r.scale = s;
+                r.code = c;
+                return r;
+            }
+            case NK::Extract: {
+                Val d = expr(n->a[0], rep);
+                if (d.t != VT::Date) fail("EXTRACT from a non-date");
+                const long long div = n->s == "year" ? 10000 : n->s == "month" ? 100 : 1;
+                const long long mod = n->s == "year" ? 0 : 100;
+                if (d.cst) return numConst(mod ? d.cv / div % mod : d.cv / div, 0);
+                Val r;
+                r.t = VT::Num;
+                r.code = "(" + d.code + " / " + lit(div) + (mod ? " % " + lit(mod) : "") + ")";
+                return r;
+            }
+            case NK::Agg: fail("aggregate " + text(n) + " inside a row expression");
+            case NK::Star: fail("* outside COUNT(*)");
+        }
+        fail("unsupported expression");
+    }
+};
+
+// ===========================================================================
+// Layout (JSON) and planning
+// ===========================================================================
+
+std::string jstr(const std::string& s) {
+    std::string o = "\"";
+    for (unsigned char c : s) {
+        if (c == '"' || c == '\\') o += '\\', o += static_cast<char>(c);
+        else if (c < 0x20) {
+            char b[8];
+            std::snprintf(b, sizeof b, "\\u%04x", c);
+            o += b;
+        } else o += static_cast<char>(c);
+    }
+    return o + "\"";
+}
+
+struct AggDef {
+    std::string name;
+    AggKind kind;
+    std::string code;
+    int scale;
+};
+
+struct Planner {
+    Compiler cc;
+    std::vector<AggDef> aggs;
+    std::map<std::string, std::string> aggByKey;
+    std::vector<NodeP> keys;
+    std::vector<Val> keyVals;
+    bool grouped = false;
+
+    std::string addAgg(AggKind kind, const std::string& code, int scale, const std::string& prefix) {
+        const std::string key = std::to_string(static_cast<int>(kind)) + "|" + code;
+        auto it = aggByKey.find(key);
+        if (it != aggByKey.end()) return it->second;
+        const std::string name = prefix + std::to_string(aggs.size());
+        aggs.push_back({name, kind, code, scale});
+        aggByKey[key] = name;
+        return name;
+    }
+
+    static const char* typeName(VT t) {
+        switch (t) {
+            case VT::Date: return "date";
+            case VT::Char1: return "char1";
+            case VT::Str: return "str";
+            default: return "num";
+        }
+    }
+
+    // Post-aggregation expression as JSON; scale = its display digits.
+    std::string post(const NodeP& n, int& scale, std::string& type) {
+        type = "num";
+        for (size_t j = 0; j < keys.size(); ++j)
+            if (text(keys[j]) == text(n)) {
+                scale = keyVals[j].t == VT::Num ? keyVals[j].scale : 0;
+                type = typeName(keyVals[j].t);
+                return "{\"key\":" + std::to_string(j) + "}";
+            }
+        switch (n->k) {
+            case NK::Num:
+                scale = n->scale;
+                return "{\"const\":\"" + std::to_string(n->iv) + "\",\"scale\":" + std::to_string(n->scale) + "}";
+            case NK::Neg: {
+                const std::string a = post(n->a[0], scale, type);
+                if (type != "num") fail("negation of a non-number");
+                return "{\"op\":\"neg\",\"args\":[" + a + "]}";
+            }
+            case NK::Arith: {
+                int sa = 0, sb = 0;
+                std::string ta, tb;
+                const std::string a = post(n->a[0], sa, ta), b = post(n->a[1], sb, tb);
+                if (ta != "num" || tb != "num") fail("arithmetic on non-numbers in the select list");
+                if (n->s == "*") scale = sa + sb;
+                else if (n->s == "/") scale = std::max(0, sa - sb + 6);
+                else scale = std::max(sa, sb);
+                return "{\"op\":\"" + n->s + "\",\"args\":[" + a + "," + b + "],\"scale\":" + std::to_string(scale) + "}";
+            }
+            case NK::Agg: {
+                if (n->distinct) fail("COUNT(DISTINCT) is not supported from SQL");
+                if (n->s == "count") {
+                    scale = 0;
+                    return "{\"agg\":\"_n\"}";
+                }
+                Val v = cc.expr(n->a[0]);
+                if (n->s == "sum" || n->s == "avg") {
+                    if (v.t != VT::Num) fail(n->s + " of a non-number");
+                    const std::string a = addAgg(AggKind::SumI128, v.code, v.scale, "a");
+                    if (n->s == "sum") {
+                        scale = v.scale;
+                        return "{\"agg\":\"" + a + "\"}";
+                    }
+                    scale = v.scale + 6;
+                    return "{\"avg\":[\"" + a + "\",\"_n\"],\"scale\":" + std::to_string(scale) + "}";
+                }
+                if (v.t == VT::Str || v.t == VT::Bool) fail(n->s + " of a " + vtName(v.t));
+                const std::string a = addAgg(n->s == "min" ? AggKind::MinI64 : AggKind::MaxI64, v.code, v.scale, "a");
+                scale = v.t == VT::Num ? v.scale : 0;
+                type = typeName(v.t);
+                return "{\"agg\":\"" + a + "\"}";
+            }
+            default:
+                fail("'" + text(n) + "' must be an aggregate or appear in GROUP BY");
+        }
+    }
+};
+
+}  // namespace
+
+HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
+    Select q = Parser(lex(sql)).select();
+    if (q.tables.size() != 1)
+        fail("only single-table queries are planned from SQL; the multi-table TPC-H queries are planned by name");
+    if (q.having) fail("HAVING is not supported");
+    Planner pl;
+    pl.cc.table = q.tables[0];
+    try {
+        pl.cc.cat = &infra::tpchTable(pl.cc.table);
+    } catch (const std::exception&) {
+        fail("unknown table " + pl.cc.table);
+    }
+    long long cap = std::stoll(opt(opts, "groups", "0"));
+    if (cap < 0 || cap > (1LL << 26)) fail("groups option out of range");
+
+    std::string where;
+    if (q.where) {
+        Val w = pl.cc.expr(q.where);
+        if (w.t != VT::Bool) fail("WHERE needs a boolean condition");
+        where = w.code;
+    }
+    // GROUP BY items may name a select item's alias.
+    for (auto g : q.groupBy) {
+        if (g->k == NK::Col)
+            for (const auto& it : q.items)
+                if (!it.alias.empty() && it.alias == g->s) {
+                    bool isColumn = false;
+                    for (const auto& c : pl.cc.cat->columns) isColumn |= c.name == g->s;
+                    if (!isColumn) g = it.e;
+                }
+        pl.keys.push_back(g);
+        pl.keyVals.push_back(pl.cc.expr(g));
+        if (pl.keyVals.back().t == VT::Bool || pl.keyVals.back().cst) fail("GROUP BY " + text(g));
+    }
+    pl.grouped = !pl.keys.empty();
+    const std::set<std::string> keyCols = pl.cc.used;  // WHERE and key columns: all the grouping pass reads
+    pl.aggs.push_back({"_n", AggKind::Count, "1", 0});  // row count: empty groups are dropped, AVG divides by it
+    pl.aggByKey[std::to_string(static_cast<int>(AggKind::Count)) + "|1"] = "_n";
+
+    std::string targets, keysJson, order;
+    bool anyAgg = false;
+    std::vector<std::string> names;
+    for (size_t i = 0; i < q.items.size(); ++i) {
+        const auto& it = q.items[i];
+        int sc = 0;
+        std::string ty;
+        const std::string e = pl.post(it.e, sc, ty);
+        anyAgg |= e.find("\"agg\"") != std::string::npos || e.find("\"avg\"") != std::string::npos;
+        const std::string name = !it.alias.empty() ? it.alias : it.e->k == NK::Col ? it.e->s : text(it.e);
+        names.push_back(name);
+        targets += std::string(i ? "," : "") + "{\"name\":" + jstr(name) + ",\"expr\":" + e + ",\"scale\":" +
+                   std::to_string(sc) + ",\"type\":\"" + ty + "\"}";
+    }
+    if (!anyAgg && !pl.grouped) fail("a select list without aggregates or GROUP BY (row output) is not supported");
+    for (size_t i = 0; i < q.orderBy.size(); ++i) {
+        const auto& o = q.orderBy[i];
+        std::string ref;
+        if (o.e->k == NK::Num && o.e->scale == 0 && o.e->iv >= 1 && o.e->iv <= static_cast<long long>(names.size())) {
+            ref = "{\"target\":" + std::to_string(o.e->iv - 1) + "}";
+        } else {
+            for (size_t t = 0; t < q.items.size() && ref.empty(); ++t)
+                if ((o.e->k == NK::Col && names[t] == o.e->s) || text(q.items[t].e) == text(o.e))
+                    ref = "{\"target\":" + std::to_string(t) + "}";
+            if (ref.empty()) {
+                int sc = 0;
+                std::string ty;
+                ref = "{\"expr\":" + pl.post(o.e, sc, ty) + ",\"type\":\"" + ty + "\"}";
+            }
+        }
+        order += std::string(i ? "," : "") + "{\"by\":" + ref + ",\"desc\":" + (o.desc ? "true" : "false") + "}";
+    }
+
+    // Key values: number / date / char(1) keys as MIN aggregates (every row of
+    // a group holds the same value); string keys through the group's
+    // representative row, attached by the caller.
+    bool strKey = false;
+    for (size_t j = 0; j < pl.keys.size(); ++j) {
+        const Val& v = pl.keyVals[j];
+        std::string src;
+        if (v.t == VT::Str) {
+            strKey = true;
+            if (pl.keys[j]->k != NK::Col) fail("GROUP BY over a string expression");
+            src = "\"column\":" + jstr(pl.keys[j]->s);
+        } else {
+            src = "\"agg\":\"" + pl.addAgg(AggKind::MinI64, v.code, v.scale, "k") + "\"";
+        }
+        keysJson += std::string(j ? "," : "") + "{" + src + ",\"type\":\"" + Planner::typeName(v.t) +
+                    "\",\"scale\":" + std::to_string(v.t == VT::Num ? v.scale : 0) + "}";
+    }
+
+    // Default dense capacity: as many groups as fit the aggregation's
+    // register accumulators (buckets x words <= 64 i64 per thread, at most
+    // 8), where a few hot groups aggregate without shared atomics; an
+    // overflow rebuilds with more (LDS, then global tables).
+    if (cap == 0) {
+        int words = 0;
+        for (const auto& a : pl.aggs) words += a.kind == AggKind::SumI128 ? 2 : 1;
+        if (strKey) ++words;
+        cap = pl.grouped ? std::max(1, std::min(8, 64 / std::max(words, 1))) : 1;
+    }
+
+    auto scan = [&](const std::set<std::string>& cols) {
+        auto sc = std::make_unique<HipGridStrideScan>(pl.cc.table, "i");
+        const auto& ct = catalogTable(pl.cc.table);
+        for (const auto& c : pl.cc.cat->columns)
+            if (cols.count(c.name)) {
+                sc->addColumn(c.name, ct.type(c.name));
+                if (c.type == infra::TpchType::CHAR_FIXED) sc->setLazy(c.name);
+            }
+        if (cols.empty()) {  // COUNT(*) alone still needs rows
+            const auto& c0 = pl.cc.cat->columns.front();
+            sc->addColumn(c0.name, ct.type(c0.name));
+        }
+        std::unique_ptr<HipOperator> r = std::move(sc);
+        if (!where.empty()) r = std::make_unique<HipSelection>(std::move(r), where);
+        return r;
+    };
+    auto addAggs = [&](HipKeyedAgg& agg) {
+        for (const auto& a : pl.aggs) agg.addAggregate(a.name, a.kind, a.code, a.scale);
+    };
+
+    HipQueryPlan plan;
+    plan.name = "SQL";
+    const int items = optInt(opts, "items", 4);
+    if (!pl.grouped) {
+        auto agg = std::make_unique<HipKeyedAgg>(scan(pl.cc.used), "d_sql_agg", "0", 1);
+        addAggs(*agg);
+        // (launch bound 256: up to 256 VGPRs for the register accumulators)
+        appendPhase(plan, "SQL_reduce", std::move(agg), 256, items, {2, 4}).launchBound = 256;
+    } else {
+        // Keys of at most 63 bits together (char(1) 8, dates and INT
+        // columns 32) are packed into the hash key itself: exact, no check.
+        // Wider keys hash to a 63-bit FNV-1a fingerprint, and a row check
+        // compares every row's key values with its group representative's.
+        int bits = 0;
+        for (size_t j = 0; j < pl.keys.size(); ++j) {
+            const Val& v = pl.keyVals[j];
+            const bool int32 = v.t == VT::Date || (v.t == VT::Num && v.scale == 0 &&
+                                                    (pl.keys[j]->k == NK::Extract ||
+                                                     (pl.keys[j]->k == NK::Col && v.code.rfind("(i64)", 0) == 0 &&
+                                                      pl.cc.cat->column(pl.keys[j]->s).type == infra::TpchType::INT)));
+            bits += v.t == VT::Char1 ? 8 : int32 ? 32 : 64;
+        }
+        const bool packed = bits <= 63;
+        std::string fp, check;
+        if (packed) {
+            int shift = 0;
+            for (size_t j = 0; j < pl.keys.size(); ++j) {
+                const Val& v = pl.keyVals[j];
+                const int w = v.t == VT::Char1 ? 8 : 32;
+                fp += std::string(j ? " | " : "") + "((u64)(" + (w == 8 ? "u8" : "u32") + ")(" + v.code + ") << " +
+                      std::to_string(shift) + ")";
+                shift += w;
+            }
+            fp = "(" + fp + ")";
+        } else {
+            fp = "0xcbf29ce484222325ull";
+            for (size_t j = 0; j < pl.keys.size(); ++j) {
+                const Val& v = pl.keyVals[j];
+                const Val r = pl.cc.expr(pl.keys[j], "_g_payload");
+                if (v.t == VT::Str) {
+                    fp = "gdb_fnv64(" + fp + ", " + v.code + ", " + std::to_string(v.width) + ")";
+                    check += std::string(j ? " && " : "") + "gdb_bytes_eq(" + v.code + ", " + r.code + ", " +
+                             std::to_string(v.width) + ")";
+                } else {
+                    fp = "gdb_fnv64_u64(" + fp + ", (u64)(" + v.code + "))";
+                    check += std::string(j ? " && " : "") + "(" + v.code + " == " + r.code + ")";
+                }
+            }
+            fp = "(" + fp + " & 0x7fffffffffffffffull)";
+        }
+        if (packed && bits <= 16) {
+            // Direct-addressed key domain (e.g. two char(1) flags): one u32
+            // id per possible key, no hashing.
+            const long long size = 1LL << bits;
+            auto id = std::make_unique<HipDirectGroupId>(scan(keyCols), fp, size, "d_sql_gid", "d_sql_groups_status",
+                                                         cap);
+            appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4});
+            std::unique_ptr<HipOperator> cur = std::make_unique<HipArrayLookup>(
+                scan(pl.cc.used), "d_sql_gid", fp, std::to_string(size), "_gid", 0ll, "u32");
+            cur = std::make_unique<HipSelection>(std::move(cur), "_gid <= " + std::to_string(cap) + "ll");
+            auto agg = std::make_unique<HipKeyedAgg>(std::move(cur), "d_sql_agg", "_gid - 1", static_cast<int>(cap));
+            addAggs(*agg);
+            appendPhase(plan, "SQL_aggregate", std::move(agg), 256, items, {2, 4}).launchBound = 256;
+        } else {
+            auto ht = std::make_unique<HipHash64Build>(scan(keyCols), "d_sql_groups", fp, "i", "sql_slots", true, "_g");
+            ht->setBlockFirst();
+            auto id = std::make_unique<HipDenseGroupId>(std::move(ht), "d_sql_groups", "_g", "sql_slots", "d_sql_gid", cap);
+            appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4});
+            std::unique_ptr<HipOperator> cur =
+                std::make_unique<HipHash64Probe>(scan(pl.cc.used), "d_sql_groups", fp, "sql_slots", "_g");
+            if (!packed)
+                cur = std::make_unique<HipRowCheck>(std::move(cur), "sql_keys", check,
+                                                    "two GROUP BY keys share a fingerprint");
+            cur = std::make_unique<HipArrayLookup>(std::move(cur), "d_sql_gid", "_g", "sql_slots", "_gid", 0ll, "u32");
+            if (strKey) pl.addAgg(AggKind::MinI64, "(i64)_g_payload", 0, "_rep");
+            auto agg = std::make_unique<HipKeyedAgg>(std::move(cur), "d_sql_agg", "_gid - 1", static_cast<int>(cap));
+            addAggs(*agg);
+            appendPhase(plan, "SQL_aggregate", std::move(agg), 256, items, {2, 4}).launchBound = 256;
+            plan.requiredSymbols = {"sql_slots"};
+        }
+    }
+    std::string rep;
+    for (const auto& a : pl.aggs)
+        if (a.code == "(i64)_g_payload") rep = a.name;
+    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
+                     keysJson + "],\"rep\":" + jstr(rep) + ",\"targets\":[" + targets + "],\"order\":[" + order +
+                     "],\"limit\":" + std::to_string(q.limit) + "}";
+    return plan;
+}
+
+}  // namespace codegen
+}  // namespace gdb

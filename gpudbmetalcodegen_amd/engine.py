@@ -144,11 +144,14 @@ class DeviceTable:
 class HipQueryPlan:
     """A built plan plus its generated HIP source (buildMetalPlan + generateFromPlan)."""
 
-    def __init__(self, name: str, options: str = ""):
+    def __init__(self, name: str, options: str = "", sql: str | None = None):
         self.name = name
         self.options = options
         self._h = C.c_void_p()
-        N.eng_check(_eng().gdb_plan_build(name.encode(), options.encode(), C.byref(self._h)))
+        if sql is None:
+            N.eng_check(_eng().gdb_plan_build(name.encode(), options.encode(), C.byref(self._h)))
+        else:
+            N.eng_check(_eng().gdb_plan_build_sql(sql.encode(), options.encode(), C.byref(self._h)))
         self.compiled_for = None
         self.compile_ms = 0.0
         self.cache_hit = False
@@ -163,6 +166,12 @@ class HipQueryPlan:
 
     def source(self) -> str:
         return N.two_call_string(_eng().gdb_plan_source, self._h)
+
+    def sql_layout(self) -> dict:
+        """How to finish a SQL plan's rows (gdb_plan_sql_layout)."""
+        import json
+
+        return json.loads(N.two_call_string(_eng().gdb_plan_sql_layout, self._h))
 
     def phases(self) -> list[str]:
         n = C.c_int()
@@ -219,6 +228,11 @@ class HipQueryPlan:
 
 def build_hip_plan(name: str, options: str = "") -> HipQueryPlan:
     return HipQueryPlan(name, options)
+
+
+def build_sql_plan(sql: str, options: str = "") -> HipQueryPlan:
+    """Plan a single-table SELECT from its text (gdb_plan_build_sql)."""
+    return HipQueryPlan("SQL", options, sql=sql)
 
 
 @dataclass

@@ -3,6 +3,7 @@ reference's ``GPUDBCodegen [flags] [sf1|sf10|...] q<N>|all``
 (``codegen/codegen_main.cpp:1782-1960``) for the 22 TPC-H queries.
 
     python -m gpudbmetalcodegen_amd.tpch_cli [flags] [sf1|sf10|sf20|sf50|sf100] q<N>|all
+    python -m gpudbmetalcodegen_amd.tpch_cli [flags] [sf1|...] --sql FILE [--sql FILE ...]
 
 Data comes from ``data/SF-<n>/<table>.colbin`` (built from ``<table>.tbl`` on
 first use, as tools/tbl_to_colbin does), from ``--data-dir DIR``, or from
@@ -12,7 +13,10 @@ reference: ``--warmup N`` (3), ``--repeat N`` (1, the median is reported),
 counterpart of ``--dump-msl``), ``--check DIR``, ``--save-golden DIR``,
 ``--check-abs-tol``, ``--check-rel-tol``, ``--threadgroup-size N``. Exit
 status: 0 on success, 1 when a check fails, 2 when a golden file is missing
-(the reference's g_checkExitCode), 3 on a usage error.
+(the reference's g_checkExitCode), 3 on a usage error. ``--sql FILE`` runs
+a single-table SELECT through the engine's SQL front end (the file's stem
+names it for --check / --save-golden), the counterpart of the reference's
+analysis-based path from sql/q<N>.sql text.
 """
 
 from __future__ import annotations
@@ -34,7 +38,8 @@ BLOCK_OPTION = {"Q1", "Q6"}
 
 def parse(argv):
     ap = argparse.ArgumentParser(prog="tpch_cli", description=__doc__.split("\n\n")[0])
-    ap.add_argument("args", nargs="+", help="[sf1|sf10|sf20|sf50|sf100] q<N>|all")
+    ap.add_argument("args", nargs="*", help="[sf1|sf10|sf20|sf50|sf100] q<N>|all")
+    ap.add_argument("--sql", action="append", default=[], help="a file holding one single-table SELECT")
     ap.add_argument("--data-dir", default=None)
     ap.add_argument("--generate", type=float, default=None, help="generate TPC-H-shaped tables at this scale factor")
     ap.add_argument("--sf-label", default=None, help="scale label of golden file names (default from the data set)")
@@ -49,6 +54,21 @@ def parse(argv):
     ap.add_argument("--check-rel-tol", type=float, default=T.CHECK_REL_TOL)
     ap.add_argument("--threadgroup-size", type=int, default=None)
     return ap.parse_args(argv)
+
+
+def _golden(a, canonical: str, q: str, label: str) -> int:
+    """--save-golden / --check for one result; the check's exit status."""
+    if a.save_golden:
+        p = T.save_golden(canonical, a.save_golden, q, label)
+        print(f"[GOLDEN] saved {p}")
+    if not a.check:
+        return 0
+    st, msg = T.check(canonical, a.check, q, label, a.check_abs_tol, a.check_rel_tol)
+    if st == 0:
+        print(f"[CHECK] {q} @ {label}: OK")
+        return 0
+    print(f"[CHECK] {q} @ {label}: {'FAIL — ' if st == 1 else ''}{msg}", file=sys.stderr)
+    return st
 
 
 def main(argv=None) -> int:
@@ -71,8 +91,14 @@ def main(argv=None) -> int:
         else:
             print(f"unexpected argument {x}", file=sys.stderr)
             return 3
-    if not queries:
-        print("usage: tpch_cli [flags] [sf1|sf10|sf20|sf50|sf100] q<N>|all", file=sys.stderr)
+    sql_files = []
+    for f in a.sql:
+        if not Path(f).is_file():
+            print(f"tpch_cli: no SQL file {f}", file=sys.stderr)
+            return 3
+        sql_files.append(Path(f))
+    if not queries and not sql_files:
+        print("usage: tpch_cli [flags] [sf1|sf10|sf20|sf50|sf100] q<N>|all | --sql FILE", file=sys.stderr)
         return 3
     data_dir = a.data_dir or data_dir
     if a.generate is None and data_dir is None:
@@ -107,6 +133,27 @@ def main(argv=None) -> int:
     db = T.TpchDB(ctx, host)
     status = 0
     try:
+        for f in sql_files:
+            from . import sql as S
+
+            q = f.stem
+            run = S.run_sql(ctx, db, f.read_text(), warmup=a.warmup, measured=1)
+            times = [run.timing.total_ms]
+            for _ in range(max(0, a.repeat - 1)):
+                times.append(S.run_sql(ctx, db, f.read_text()).timing.total_ms)
+            med = statistics.median(times)
+            if a.print_plan:
+                print(run.plan.describe())
+            if a.dump_src:
+                Path(a.dump_src).mkdir(parents=True, exist_ok=True)
+                (Path(a.dump_src) / f"{q}.hip").write_text(run.plan.source())
+            if a.csv:
+                print(f"TIMING_CSV,{q},{label},{med:.4f},{run.plan.compile_ms:.2f},{len(times)}")
+            else:
+                print(f"{q} (SQL) @ {label}: GPU {med:.3f} ms (median of {len(times)}), "
+                      f"{len(next(iter(run.columns.values())))} rows")
+                print(run.canonical.rstrip("\n"))
+            status = max(status, _golden(a, run.canonical, q, label))
         for q in queries:
             opts = f"block={a.threadgroup_size}" if a.threadgroup_size and q in BLOCK_OPTION else ""
             from .queries import get_plan
@@ -129,16 +176,7 @@ def main(argv=None) -> int:
                 print(f"{q} @ {label}: GPU {med:.3f} ms (median of {len(times)}), compile {plan.compile_ms:.1f} ms"
                       f"{' (cache hit)' if plan.cache_hit else ''}, {len(next(iter(run.columns.values())))} rows")
                 print(run.canonical.rstrip("\n"))
-            if a.save_golden:
-                p = T.save_golden(run.canonical, a.save_golden, q, label)
-                print(f"[GOLDEN] saved {p}")
-            if a.check:
-                st, msg = T.check(run.canonical, a.check, q, label, a.check_abs_tol, a.check_rel_tol)
-                if st == 0:
-                    print(f"[CHECK] {q} @ {label}: OK")
-                else:
-                    print(f"[CHECK] {q} @ {label}: {'FAIL — ' if st == 1 else ''}{msg}", file=sys.stderr)
-                    status = max(status, 1) if st == 1 else 2
+            status = max(status, _golden(a, run.canonical, q, label))
             ex.release()
     finally:
         db.free()

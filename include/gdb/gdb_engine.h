@@ -51,6 +51,21 @@ gdb_status gdb_gen_text(uint64_t seed, uint64_t stream, uint64_t n, int width, c
  * "key=value,..." and run the emitters. Replaces buildMetalPlan +
  * generateFromPlan (codegen/planning/metal_plan_builder.h:73-78). */
 gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out);
+/* Build a plan from SQL text: a single-table SELECT (any TPC-H table) with
+ * WHERE, GROUP BY, SUM / AVG / MIN / MAX / COUNT, ORDER BY, LIMIT, compiled
+ * to exact fixed-point HIP expressions. Replaces analyzeSQL + the
+ * analysis-based buildMetalPlan dispatch (codegen/planning/query_analyzer.h:80-85,
+ * metal_plan_builder.cpp:2035-2076) for single-table queries; multi-table
+ * queries go through gdb_plan_build by name. Options: groups=N (dense group
+ * capacity, default 256; execute reports GDB_ERR_OVERFLOW beyond it).
+ * GDB_ERR_INVALID with gdb_last_error() naming the construct otherwise. */
+gdb_status gdb_plan_build_sql(const char* sql, const char* options, gdb_plan* out);
+/* JSON layout of a SQL plan's result: table, group keys (aggregate column or
+ * string column attached from the representative row), select list as
+ * expressions over the aggregate columns, ORDER BY, LIMIT. Two-call
+ * pattern as gdb_plan_describe. No counterpart in the reference (its
+ * post-processing is per query in C++). */
+gdb_status gdb_plan_sql_layout(gdb_plan p, char* buf, size_t cap, size_t* need);
 /* Plan tree (the reference's --print-plan) and generated HIP source
  * (--dump-msl). Two-call pattern: need receives the size incl. NUL. */
 gdb_status gdb_plan_describe(gdb_plan p, char* buf, size_t cap, size_t* need);

@@ -1,0 +1,149 @@
+"""SQL front end on the CPU: parsing, exact fixed-point compilation and the
+result finishing (no device needed: plans are built and their HIP source and
+layout inspected; finishing runs on hand-made aggregate columns).
+
+The reference parses SQL with libpg_query (codegen/planning/query_analyzer.cpp)
+and plans the analysis-based shapes from it (metal_plan_builder.cpp:253-360);
+here the engine's own parser covers single-table SELECTs.
+"""
+
+from __future__ import annotations
+
+from fractions import Fraction
+from pathlib import Path
+
+import pytest
+
+from gpudbmetalcodegen_amd import _native as N
+from gpudbmetalcodegen_amd import engine as E
+from gpudbmetalcodegen_amd import sql as S
+
+SQL_DIR = Path(__file__).resolve().parent / "sql"
+Q1 = (SQL_DIR / "tpch_q1.sql").read_text()
+Q6 = (SQL_DIR / "tpch_q6.sql").read_text()
+
+
+def test_q6_constants_fold_exactly():
+    p = E.build_sql_plan(Q6)
+    d = p.describe()
+    # dates folded with calendar arithmetic; 0.06 +- 0.01 in hundredths; 24 -> 2400 hundredths
+    assert "(19940101ll)" in d and "(19950101ll)" in d
+    assert "(5ll)" in d and "(7ll)" in d and "(2400ll)" in d
+    lay = p.sql_layout()
+    assert lay["table"] == "lineitem" and lay["keys"] == []
+    assert lay["targets"] == [{"name": "revenue", "expr": {"agg": "a1"}, "scale": 4, "type": "num"}]
+    assert p.phases() == ["SQL_reduce"]
+
+
+def test_q1_layout_and_packed_keys():
+    p = E.build_sql_plan(Q1)
+    assert p.phases() == ["SQL_groups", "SQL_aggregate"]
+    d = p.describe()
+    assert "(19980902ll)" in d                  # 1998-12-01 - 90 days
+    assert "RowCheck" not in d                  # two char(1) keys pack into 16 bits
+    lay = p.sql_layout()
+    names = [t["name"] for t in lay["targets"]]
+    assert names == ["l_returnflag", "l_linestatus", "sum_qty", "sum_base_price", "sum_disc_price", "sum_charge",
+                     "avg_qty", "avg_price", "avg_disc", "count_order"]
+    scales = {t["name"]: t["scale"] for t in lay["targets"]}
+    assert scales["sum_disc_price"] == 4 and scales["sum_charge"] == 6 and scales["avg_qty"] == 8
+    assert [o["desc"] for o in lay["order"]] == [False, False]
+
+
+@pytest.mark.parametrize("expr,date", [
+    ("date '1998-12-01' - interval '90' day", 19980902),
+    ("date '1995-01-31' + interval '1' month", 19950228),   # clamped to the month's end
+    ("date '1996-01-31' + interval '1' month", 19960229),   # leap year
+    ("date '1994-03-01' - interval '1' day", 19940228),
+    ("date '1993-10-01' + interval '3' month", 19940101),
+    ("date '2000-02-29' + interval '1' year", 20010228),
+])
+def test_date_interval_folding(expr, date):
+    p = E.build_sql_plan(f"select count(*) from orders where o_orderdate < {expr}")
+    assert f"({date}ll)" in p.describe()
+
+
+def test_string_predicates_compile():
+    p = E.build_sql_plan("select count(*) from part where p_type like 'PROMO%' and p_brand <> 'Brand#45' "
+                         "and p_container in ('SM BOX', 'LG CASE') and p_name not like '%gre_n%' "
+                         "and p_mfgr > 'Manufacturer#2'")
+    d = p.describe()
+    assert 'gdb_like_prefix(p_type, 25, "PROMO", 5)' in d
+    assert '!gdb_str_eq_lit(p_brand, 10, "Brand#45", 8)' in d
+    assert 'gdb_str_eq_lit(p_container, 10, "SM BOX", 6)' in d
+    assert '!gdb_like_match(p_name, 55, "%gre_n%", 7)' in d
+    assert '(gdb_str_cmp_lit(p_mfgr, 25, "Manufacturer#2", 14) > 0)' in d
+
+
+def test_quotes_and_escapes_in_literals():
+    p = E.build_sql_plan("select count(*) from customer where c_name = 'O''Brien\"x'")
+    assert r'"O\047Brien\042x", 9' in p.describe()
+
+
+def test_case_when_aligns_scales():
+    p = E.build_sql_plan("select sum(case when l_shipmode = 'MAIL' then l_extendedprice * (1 - l_discount) "
+                         "else 0 end) as r from lineitem")
+    lay = p.sql_layout()
+    assert lay["targets"][0]["scale"] == 4
+
+
+@pytest.mark.parametrize("sql,msg", [
+    ("select * from lineitem", "SELECT *"),
+    ("select sum(l_quantity) from lineitem, orders where l_orderkey = o_orderkey", "single-table"),
+    ("select l_quantity from lineitem", "must be an aggregate or appear in GROUP BY"),
+    ("select sum(l_quantity / 2) from lineitem", "division of row values"),
+    ("select sum(nope) from lineitem", "no column nope"),
+    ("select count(distinct l_suppkey) from lineitem", "COUNT(DISTINCT)"),
+    ("select sum(l_quantity) from lineitem where l_orderkey in (select o_orderkey from orders)", "IN (subquery)"),
+    ("select sum(l_quantity) from lineitem where l_shipdate < '1994-13-01'", "bad date literal"),
+    ("select sum(l_quantity) from nosuch", "unknown table"),
+    ("select sum(l_quantity) from lineitem where l_comment", "boolean"),
+    ("select sum(l_quantity) from lineitem group by l_returnflag having count(*) > 1", "HAVING"),
+    ("select sum(l_quantity) from lineitem where l_returnflag = 'AB'", "char(1) column compared"),
+    ("select sum(l_quantity) from lineitem where l_shipdate = 3", "comparison of date with number"),
+    ("select sum(l_quantity) from lineitem limit 2.5", "LIMIT takes an integer"),
+])
+def test_unsupported_sql_is_named(sql, msg):
+    with pytest.raises(N.GdbError) as ei:
+        E.build_sql_plan(sql)
+    assert msg in str(ei.value)
+
+
+def test_finish_orders_limits_and_averages():
+    layout = {"keys": [{"agg": "k1", "type": "char1", "scale": 0}], "rep": "",
+              "targets": [{"name": "f", "expr": {"key": 0}, "scale": 0, "type": "char1"},
+                          {"name": "s", "expr": {"agg": "a0"}, "scale": 2, "type": "num"},
+                          {"name": "avg", "expr": {"avg": ["a0", "_n"], "scale": 8}, "scale": 8, "type": "num"},
+                          {"name": "ratio", "expr": {"op": "/", "args": [{"agg": "a0"}, {"agg": "_n"}], "scale": 8},
+                           "scale": 8, "type": "num"}],
+              "order": [{"by": {"target": 1}, "desc": True}], "limit": 2}
+    cols = {"_n": [3, 1, 2], "a0": [100, -7, 100], "k1": [ord("A"), ord("N"), ord("R")],
+            "__scales": {"a0": 2, "_n": 0}}
+    out, fmts = S.finish(layout, cols, None)
+    # ties on s broken by the remaining columns ascending
+    assert out["f"] == ["A", "R"] and out["s"] == [100, 100]
+    assert out["avg"] == [33333333, 50000000]      # 1.00 / 3 truncated to 8 digits
+    assert out["ratio"] == out["avg"]
+    assert fmts["s"] == (2, False) and fmts["avg"] == (8, False)
+    neg = S._trunc(Fraction(-7, 300), 8)
+    assert neg == -2333333                          # truncation toward zero
+
+
+def test_finish_string_keys_from_representative_rows():
+    import numpy as np
+
+    host = {"p_brand": np.frombuffer(b"Brand#11\0\0Brand#22\0\0", np.uint8).reshape(2, 10)}
+    layout = {"keys": [{"column": "p_brand", "type": "str", "scale": 0}], "rep": "_rep2",
+              "targets": [{"name": "p_brand", "expr": {"key": 0}, "scale": 0, "type": "str"},
+                          {"name": "c", "expr": {"agg": "_n"}, "scale": 0, "type": "num"}],
+              "order": [], "limit": -1}
+    cols = {"_n": [5, 7], "_rep2": [1, 0], "__scales": {}}
+    out, _ = S.finish(layout, cols, host)
+    assert out == {"p_brand": ["Brand#11", "Brand#22"], "c": [7, 5]}
+
+
+def test_cli_sql_usage_errors(tmp_path):
+    from gpudbmetalcodegen_amd import tpch_cli
+
+    assert tpch_cli.main(["--sql", str(tmp_path / "missing.sql")]) == 3
+    assert tpch_cli.main([]) == 3
