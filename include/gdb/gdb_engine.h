@@ -58,7 +58,7 @@ gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out);
  * metal_plan_builder.cpp:2035-2076) for single-table queries; multi-table
  * queries go through gdb_plan_build by name. Options: groups=N (dense group
  * capacity, default 256; execute reports GDB_ERR_OVERFLOW beyond it).
- * GDB_ERR_INVALID with gdb_last_error() naming the construct otherwise. */
+ * GDB_ERR_INVALID with gdb_engine_last_error() naming the construct otherwise. */
 gdb_status gdb_plan_build_sql(const char* sql, const char* options, gdb_plan* out);
 /* JSON layout of a SQL plan's result: table, group keys (aggregate column or
  * string column attached from the representative row), select list as
