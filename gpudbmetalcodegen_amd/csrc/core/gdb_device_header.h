@@ -617,6 +617,14 @@ GDB_DEV bool gdb_like_match(const char* s, int width, const char* p, int pl) {
     return j == pl;
 }
 
+// The n (<= 8) bytes of a short fixed-width field as one integer (byte 0
+// lowest): an exact group key.
+GDB_DEV u64 gdb_pack_bytes(const char* p, int n) {
+    u64 v = 0;
+    for (int i = 0; i < n; ++i) v |= (u64)(u8)p[i] << (8 * i);
+    return v;
+}
+
 // Byte equality of two n-byte fields.
 GDB_DEV bool gdb_bytes_eq(const char* a, const char* b, int n) {
     u32 d = 0;

@@ -871,6 +871,12 @@ void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addLine("const u32 _maxp_" + table_ + " = (u32)(nslots_" + table_ + " < 65536ull ? nslots_" + table_ +
                " : 65536ull);");
     cg.addLine("u32 _ins_" + table_ + " = 0, _ovf_" + table_ + " = 0;");
+    if (!uniqueMsg_.empty()) {
+        const std::string ck = table_ + "_unique";
+        cg.addBufferParam(ck, "u64", 8, "4", InitKind::Zero, {}, false, true);
+        cg.setPhaseCheck(ck, uniqueMsg_);
+        cg.addLine("u32 _dup_" + table_ + " = 0;");
+    }
     if (blockFirst_) {
         if (!distinct_) throw std::runtime_error("HipHash64Build(" + table_ + "): the block filter needs distinct");
         cg.declareLds("u64", "_seen_" + table_, "512");
@@ -889,6 +895,7 @@ void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             cg.addLine("const int " + r + " = gdb_block_first(_seen_" + table_ + ", " + k + ") ? " + call + " : 0;");
         else
             cg.addLine("const int " + r + " = " + call + ";");
+        if (!uniqueMsg_.empty()) cg.addLine("_dup_" + table_ + " += " + r + " == 0 ? 1u : 0u;");
         cg.addLine("_ins_" + table_ + " += " + r + " == 1 ? 1u : 0u;");
         cg.addLine("_ovf_" + table_ + " |= " + r + " < 0 ? 1u : 0u;");
         if (distinct_)
@@ -899,6 +906,23 @@ void HipHash64Build::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             consume();
     });
     emitBlockStatus(cg, st, {{"_ins_" + table_, 2, false}, {"_ovf_" + table_, 0, true}});
+    if (!uniqueMsg_.empty())
+        emitBlockStatus(cg, table_ + "_unique", {{"_dup_" + table_, 2, false}, {"_dup_" + table_, 0, true}});
+}
+
+HipBindColumns::HipBindColumns(std::unique_ptr<HipOperator> child, const std::string& table,
+                               std::vector<std::pair<std::string, ColumnType>> columns)
+    : HipUnaryOperator(std::move(child)), table_(table), columns_(std::move(columns)) {}
+
+void HipBindColumns::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    for (const auto& c : columns_) cg.addColumnParam(table_, c.first, c.second);
+    child_->produce(cg, consume);
+}
+
+std::string HipBindColumns::describe() const {
+    std::string d = "BindColumns(" + table_ + ":";
+    for (const auto& c : columns_) d += " " + c.first;
+    return d + ")";
 }
 
 std::string HipHash64Build::describe() const {

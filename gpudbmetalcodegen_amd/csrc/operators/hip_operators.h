@@ -217,6 +217,10 @@ public:
     // keys each block already sent (gdb_block_first) keeps all but a block's
     // first arrival per key away from the global table.
     void setBlockFirst() { blockFirst_ = true; }
+    // Build side of a key join: a second row with a key already present
+    // fails the run with `message` after the phase (the join would
+    // otherwise drop rows).
+    void setUniqueKeys(const std::string& message) { uniqueMsg_ = message; }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
@@ -225,6 +229,21 @@ private:
     bool distinct_ = false;
     std::string slotVar_;
     bool blockFirst_ = false;
+    std::string uniqueMsg_;
+};
+
+// Binds columns of another table as kernel parameters (p_<column>), for
+// expressions that read a joined row by its index (p_<column>[row]).
+class HipBindColumns : public HipUnaryOperator {
+public:
+    HipBindColumns(std::unique_ptr<HipOperator> child, const std::string& table,
+                   std::vector<std::pair<std::string, ColumnType>> columns);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string table_;
+    std::vector<std::pair<std::string, ColumnType>> columns_;
 };
 
 // Dense group numbering over a HipHash64Build(distinct, slotVar) in the same

@@ -537,21 +537,41 @@ Val upscale(const Val& v, int s) {
     return r;
 }
 
+const infra::TpchColumn* findColumn(const infra::TpchTable* t, const std::string& name) {
+    if (t)
+        for (const auto& col : t->columns)
+            if (col.name == name) return &col;
+    return nullptr;
+}
+
 struct Compiler {
-    std::string table;
+    std::string table;  // the scanned table
     const infra::TpchTable* cat = nullptr;
-    std::set<std::string> used;  // scanned columns
+    std::set<std::string> used;  // its columns read
+    // A joined table, read at row index `otherRow` (the join's match).
+    std::string other;
+    const infra::TpchTable* otherCat = nullptr;
+    std::string otherRow;
+    std::set<std::string> usedOther;
 
     // rep empty: the scanned row's bindings; else the row index `rep`.
     Val column(const std::string& name, const std::string& rep) {
-        const infra::TpchColumn* c = nullptr;
-        for (const auto& col : cat->columns)
-            if (col.name == name) c = &col;
-        if (!c) fail("no column " + name + " in " + table);
-        used.insert(name);
-        const std::string ptr = tableDataName(table, name);
+        const infra::TpchColumn* c = findColumn(cat, name);
+        std::string t = table;
+        std::string row = rep;
+        if (c) {
+            used.insert(name);
+        } else if ((c = findColumn(otherCat, name)) != nullptr) {
+            if (otherRow.empty()) fail("column " + name + " of " + other + " outside the join");
+            t = other;
+            row = otherRow;
+            usedOther.insert(name);
+        } else {
+            fail("no column " + name + " in " + table + (other.empty() ? "" : " or " + other));
+        }
+        const std::string ptr = tableDataName(t, name);
         Val v;
-        const std::string at = rep.empty() ? name : ptr + "[(u64)(" + rep + ")]";
+        const std::string at = row.empty() ? name : ptr + "[(u64)(" + row + ")]";
         switch (c->type) {
             case infra::TpchType::INT: v.t = VT::Num; v.code = "(i64)" + at; break;
             case infra::TpchType::FLOAT: v.t = VT::Num; v.scale = 2; v.code = "(i64)" + at; break;
@@ -561,8 +581,8 @@ struct Compiler {
                 v.t = VT::Str;
                 v.width = c->width;
                 v.sv = name;
-                v.code = rep.empty() ? name
-                                     : "((const char*)" + ptr + " + (u64)(" + rep + ") * " + std::to_string(c->width) + "ull)";
+                v.code = row.empty() ? name
+                                     : "((const char*)" + ptr + " + (u64)(" + row + ") * " + std::to_string(c->width) + "ull)";
                 break;
         }
         return v;
@@ -619,7 +639,8 @@ struct Compiler {
             r.t = VT::Date;
             r.cv = parseDate(v.sv);
         } else if (to == VT::Char1) {
-            if (v.sv.size() > 1) fail("char(1) column compared with '" + v.sv + "'");
+            // A char(1) column holds the first byte of the source value
+            // (the catalog's CHAR1): a longer literal compares by its first.
             r.t = VT::Char1;
             r.cv = v.sv.empty() ? 0 : static_cast<unsigned char>(v.sv[0]);
         } else {
@@ -641,6 +662,8 @@ struct Compiler {
                 op = flip.at(op);
             }
             if (a.cst || !b.cst) fail("string comparison needs a column and a constant");
+            // Fixed-width columns hold the source value cut to the width.
+            if (static_cast<int>(b.sv.size()) > a.width) b.sv.resize(static_cast<size_t>(a.width));
             const std::string args =
                 a.code + ", " + std::to_string(a.width) + ", " + cstr(b.sv) + ", " + std::to_string(b.sv.size());
             if (op == "=") return boolOf("gdb_str_eq_lit(" + args + ")");
@@ -885,35 +908,130 @@ struct Planner {
 
 }  // namespace
 
+namespace {
+
+void conjuncts(const NodeP& n, std::vector<NodeP>& out) {
+    if (n->k == NK::And) {
+        conjuncts(n->a[0], out);
+        conjuncts(n->a[1], out);
+    } else {
+        out.push_back(n);
+    }
+}
+
+void columnsOf(const NodeP& n, std::set<std::string>& out) {
+    if (n->k == NK::Col) out.insert(n->s);
+    for (const auto& c : n->a) columnsOf(c, out);
+}
+
+const infra::TpchTable* catalogOf(const std::string& t) {
+    try {
+        return &infra::tpchTable(t);
+    } catch (const std::exception&) {
+        fail("unknown table " + t);
+    }
+}
+
+std::string andAll(Compiler& cc, const std::vector<NodeP>& conj) {
+    std::string code;
+    for (const auto& n : conj) {
+        Val v = cc.expr(n);
+        if (v.t != VT::Bool) fail("WHERE needs boolean conditions (" + text(n) + ")");
+        code += (code.empty() ? "" : " && ") + v.code;
+    }
+    return code;
+}
+
+}  // namespace
+
 HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     Select q = Parser(lex(sql)).select();
-    if (q.tables.size() != 1)
-        fail("only single-table queries are planned from SQL; the multi-table TPC-H queries are planned by name");
+    if (q.tables.size() > 2)
+        fail("one table, or two joined on a key column, are planned from SQL; other multi-table queries are "
+             "planned by name");
     if (q.having) fail("HAVING is not supported");
     Planner pl;
-    pl.cc.table = q.tables[0];
-    try {
-        pl.cc.cat = &infra::tpchTable(pl.cc.table);
-    } catch (const std::exception&) {
-        fail("unknown table " + pl.cc.table);
-    }
     long long cap = std::stoll(opt(opts, "groups", "0"));
     if (cap < 0 || cap > (1LL << 26)) fail("groups option out of range");
 
-    std::string where;
-    if (q.where) {
-        Val w = pl.cc.expr(q.where);
-        if (w.t != VT::Bool) fail("WHERE needs a boolean condition");
-        where = w.code;
+    // Two tables: one equality between a column of each, where one side is
+    // its table's key (the build side, read through a hash join by row
+    // index); the other table is scanned. Conjuncts reading only the build
+    // table filter the build; the rest run on the scanned rows, before the
+    // join when they read only the scanned table.
+    std::vector<NodeP> conj, buildConj, probeConj, crossConj;
+    if (q.where) conjuncts(q.where, conj);
+    pl.cc.table = q.tables[0];
+    pl.cc.cat = catalogOf(q.tables[0]);
+    Compiler bc;  // the build side's scan
+    std::string probeKey, buildKey;
+    const bool joined = q.tables.size() == 2;
+    if (joined) {
+        const infra::TpchTable* c1 = catalogOf(q.tables[1]);
+        int ji = -1;
+        std::string k0, k1;
+        for (size_t i = 0; i < conj.size(); ++i) {
+            const NodeP& n = conj[i];
+            if (n->k != NK::Cmp || n->s != "=" || n->a[0]->k != NK::Col || n->a[1]->k != NK::Col) continue;
+            const std::string x = n->a[0]->s, y = n->a[1]->s;
+            std::string a, b;
+            if (findColumn(pl.cc.cat, x) && findColumn(c1, y)) a = x, b = y;
+            else if (findColumn(pl.cc.cat, y) && findColumn(c1, x)) a = y, b = x;
+            else continue;
+            if (ji >= 0) fail("more than one join condition between the two tables");
+            ji = static_cast<int>(i);
+            k0 = a;
+            k1 = b;
+        }
+        if (ji < 0) fail("two tables without an equality between their columns (no cross products)");
+        conj.erase(conj.begin() + ji);
+        static const std::map<std::string, std::string> keyOf = {
+            {"customer", "c_custkey"}, {"orders", "o_orderkey"}, {"part", "p_partkey"},
+            {"supplier", "s_suppkey"}, {"nation", "n_nationkey"}, {"region", "r_regionkey"}};
+        auto isKey = [&](const std::string& t, const std::string& c) { return keyOf.count(t) && keyOf.at(t) == c; };
+        std::string probeT = q.tables[0], buildT = q.tables[1];
+        probeKey = k0;
+        buildKey = k1;
+        if (!isKey(buildT, buildKey)) {
+            if (!isKey(probeT, probeKey)) fail("join on " + k0 + " = " + k1 + ": neither is its table's key");
+            std::swap(probeT, buildT);
+            std::swap(probeKey, buildKey);
+        }
+        pl.cc.table = probeT;
+        pl.cc.cat = catalogOf(probeT);
+        pl.cc.other = buildT;
+        pl.cc.otherCat = catalogOf(buildT);
+        pl.cc.otherRow = "_jb_payload";
+        bc.table = buildT;
+        bc.cat = pl.cc.otherCat;
+        for (const auto& n : conj) {
+            std::set<std::string> cols;
+            columnsOf(n, cols);
+            bool probe = false, build = false;
+            for (const auto& c : cols) {
+                probe |= findColumn(pl.cc.cat, c) != nullptr;
+                build |= findColumn(bc.cat, c) != nullptr;
+            }
+            (build && !probe ? buildConj : probe && build ? crossConj : probeConj).push_back(n);
+        }
+    } else {
+        probeConj = conj;
+    }
+
+    const std::string where = andAll(pl.cc, probeConj);
+    std::string probeKeyCode, buildKeyCode, buildWhere, cross;
+    if (joined) {
+        probeKeyCode = pl.cc.column(probeKey, "").code;
+        buildKeyCode = bc.column(buildKey, "").code;
+        buildWhere = andAll(bc, buildConj);
+        cross = andAll(pl.cc, crossConj);
     }
     // GROUP BY items may name a select item's alias.
     for (auto g : q.groupBy) {
         if (g->k == NK::Col)
             for (const auto& it : q.items)
                 if (!it.alias.empty() && it.alias == g->s) {
-                    bool isColumn = false;
-                    for (const auto& c : pl.cc.cat->columns) isColumn |= c.name == g->s;
-                    if (!isColumn) g = it.e;
+                    if (!findColumn(pl.cc.cat, g->s) && !findColumn(pl.cc.otherCat, g->s)) g = it.e;
                 }
         pl.keys.push_back(g);
         pl.keyVals.push_back(pl.cc.expr(g));
@@ -964,7 +1082,12 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     for (size_t j = 0; j < pl.keys.size(); ++j) {
         const Val& v = pl.keyVals[j];
         std::string src;
-        if (v.t == VT::Str) {
+        if (v.t == VT::Str && v.width <= 7) {  // short strings: the bytes themselves, packed
+            src = "\"agg\":\"" +
+                  pl.addAgg(AggKind::MinI64, "(i64)gdb_pack_bytes(" + v.code + ", " + std::to_string(v.width) + ")", 0,
+                            "k") +
+                  "\",\"width\":" + std::to_string(v.width);
+        } else if (v.t == VT::Str) {
             strKey = true;
             if (pl.keys[j]->k != NK::Col) fail("GROUP BY over a string expression");
             src = "\"column\":" + jstr(pl.keys[j]->s);
@@ -1000,6 +1123,14 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         }
         std::unique_ptr<HipOperator> r = std::move(sc);
         if (!where.empty()) r = std::make_unique<HipSelection>(std::move(r), where);
+        if (joined) {
+            std::vector<std::pair<std::string, ColumnType>> bind;
+            for (const auto& c : pl.cc.usedOther) bind.push_back({c, catalogTable(pl.cc.other).type(c)});
+            if (!bind.empty()) r = std::make_unique<HipBindColumns>(std::move(r), pl.cc.other, bind);
+            r = std::make_unique<HipHash64Probe>(std::move(r), "d_sql_join", "(u64)(u32)(" + probeKeyCode + ")",
+                                                 "sql_join_slots", "_jb");
+            if (!cross.empty()) r = std::make_unique<HipSelection>(std::move(r), cross);
+        }
         return r;
     };
     auto addAggs = [&](HipKeyedAgg& agg) {
@@ -1009,6 +1140,23 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     HipQueryPlan plan;
     plan.name = "SQL";
     const int items = optInt(opts, "items", 4);
+    if (joined) {
+        // Build side: key -> row index; a repeated key fails the run.
+        auto sc = std::make_unique<HipGridStrideScan>(bc.table, "i");
+        const auto& ct = catalogTable(bc.table);
+        for (const auto& c : bc.cat->columns)
+            if (bc.used.count(c.name)) {
+                sc->addColumn(c.name, ct.type(c.name));
+                if (c.type == infra::TpchType::CHAR_FIXED) sc->setLazy(c.name);
+            }
+        std::unique_ptr<HipOperator> r = std::move(sc);
+        if (!buildWhere.empty()) r = std::make_unique<HipSelection>(std::move(r), buildWhere);
+        auto ht = std::make_unique<HipHash64Build>(std::move(r), "d_sql_join", "(u64)(u32)(" + buildKeyCode + ")", "i",
+                                                   "sql_join_slots");
+        ht->setUniqueKeys("the join column " + buildKey + " repeats a value");
+        appendPhase(plan, "SQL_join_build", std::move(ht), 256, items, {2, 4});
+        plan.requiredSymbols.push_back("sql_join_slots");
+    }
     if (!pl.grouped) {
         auto agg = std::make_unique<HipKeyedAgg>(scan(pl.cc.used), "d_sql_agg", "0", 1);
         addAggs(*agg);
@@ -1019,21 +1167,39 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         // columns 32) are packed into the hash key itself: exact, no check.
         // Wider keys hash to a 63-bit FNV-1a fingerprint, and a row check
         // compares every row's key values with its group representative's.
+        auto isIntColumn = [&](const std::string& name) {
+            const infra::TpchColumn* c = findColumn(pl.cc.cat, name);
+            if (!c) c = findColumn(pl.cc.otherCat, name);
+            return c && c->type == infra::TpchType::INT;
+        };
         int bits = 0;
         for (size_t j = 0; j < pl.keys.size(); ++j) {
             const Val& v = pl.keyVals[j];
             const bool int32 = v.t == VT::Date || (v.t == VT::Num && v.scale == 0 &&
                                                     (pl.keys[j]->k == NK::Extract ||
-                                                     (pl.keys[j]->k == NK::Col && v.code.rfind("(i64)", 0) == 0 &&
-                                                      pl.cc.cat->column(pl.keys[j]->s).type == infra::TpchType::INT)));
-            bits += v.t == VT::Char1 ? 8 : int32 ? 32 : 64;
+                                                     (pl.keys[j]->k == NK::Col && isIntColumn(pl.keys[j]->s))));
+            bits += v.t == VT::Char1 ? 8 : (v.t == VT::Str && v.width <= 7) ? 8 * v.width : int32 ? 32 : 64;
         }
         const bool packed = bits <= 63;
+        if (!packed)
+            for (const auto& k : pl.keys) {
+                std::set<std::string> cols;
+                columnsOf(k, cols);
+                for (const auto& c : cols)
+                    if (findColumn(pl.cc.otherCat, c) && !findColumn(pl.cc.cat, c))
+                        fail("GROUP BY " + text(k) + ": wide keys of the joined table are not supported");
+            }
         std::string fp, check;
         if (packed) {
             int shift = 0;
             for (size_t j = 0; j < pl.keys.size(); ++j) {
                 const Val& v = pl.keyVals[j];
+                if (v.t == VT::Str) {
+                    fp += std::string(j ? " | " : "") + "(gdb_pack_bytes(" + v.code + ", " + std::to_string(v.width) +
+                          ") << " + std::to_string(shift) + ")";
+                    shift += 8 * v.width;
+                    continue;
+                }
                 const int w = v.t == VT::Char1 ? 8 : 32;
                 fp += std::string(j ? " | " : "") + "((u64)(" + (w == 8 ? "u8" : "u32") + ")(" + v.code + ") << " +
                       std::to_string(shift) + ")";
@@ -1084,13 +1250,13 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             auto agg = std::make_unique<HipKeyedAgg>(std::move(cur), "d_sql_agg", "_gid - 1", static_cast<int>(cap));
             addAggs(*agg);
             appendPhase(plan, "SQL_aggregate", std::move(agg), 256, items, {2, 4}).launchBound = 256;
-            plan.requiredSymbols = {"sql_slots"};
+            plan.requiredSymbols.push_back("sql_slots");
         }
     }
     std::string rep;
     for (const auto& a : pl.aggs)
         if (a.code == "(i64)_g_payload") rep = a.name;
-    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
+    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":" + jstr(pl.cc.other) + ",\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
                      keysJson + "],\"rep\":" + jstr(rep) + ",\"targets\":[" + targets + "],\"order\":[" + order +
                      "],\"limit\":" + std::to_string(q.limit) + "}";
     return plan;

@@ -1,6 +1,7 @@
 """SQL text -> GPU plan -> exact result rows.
 
-``run_sql`` plans a single-table SELECT from its text (``gdb_plan_build_sql``:
+``run_sql`` plans a SELECT over one table, or two joined on a key column,
+from its text (``gdb_plan_build_sql``:
 the engine's own parser and fixed-point compiler, the counterpart of the
 reference's analyzeSQL + analysis-based plan builders,
 ``codegen/planning/query_analyzer.cpp:640-727``), runs it on the TPC-H tables
@@ -56,11 +57,13 @@ def finish(layout: dict, cols: dict, host_table: dict | None) -> tuple[dict, dic
 
     def key_value(j: int, r: int):
         k = keys[j]
-        if k["type"] == "str":
+        if k["type"] == "str" and "column" in k:
             if host_table is None:
                 raise ValueError("string GROUP BY keys need the host table to attach")
             return _text(host_table[k["column"]][int(cols[rep][r])])
         v = int(cols[k["agg"]][r])
+        if k["type"] == "str":  # a short string, its bytes packed (byte 0 lowest)
+            return v.to_bytes(8, "little")[:k["width"]].split(b"\0", 1)[0].decode()
         if k["type"] == "char1":
             return chr(v) if v else ""
         if k["type"] == "date":
@@ -147,6 +150,13 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
         ex = executor or E.HipGenericExecutor(ctx)
         ex.register_table(db.device(table))
         ex.register_symbol("sql_slots", T._pow2(2 * max(rows, 1)))
+        join = layout.get("join") or ""
+        if join:  # the key side of a two-table join
+            if join not in db.host:
+                raise ValueError(f"table {join} is not loaded")
+            ex.register_table(db.device(join))
+            jrows = int(next(iter(db.host[join].values())).shape[0])
+            ex.register_symbol("sql_join_slots", T._pow2(2 * max(jrows, 1)))
         try:
             t = ex.execute(plan, warmup, measured)
         except N.GdbOverflow:

@@ -51,13 +51,16 @@ gdb_status gdb_gen_text(uint64_t seed, uint64_t stream, uint64_t n, int width, c
  * "key=value,..." and run the emitters. Replaces buildMetalPlan +
  * generateFromPlan (codegen/planning/metal_plan_builder.h:73-78). */
 gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out);
-/* Build a plan from SQL text: a single-table SELECT (any TPC-H table) with
+/* Build a plan from SQL text: a SELECT over one TPC-H table, or two joined
+ * by an equality on one table's key, with
  * WHERE, GROUP BY, SUM / AVG / MIN / MAX / COUNT, ORDER BY, LIMIT, compiled
  * to exact fixed-point HIP expressions. Replaces analyzeSQL + the
  * analysis-based buildMetalPlan dispatch (codegen/planning/query_analyzer.h:80-85,
- * metal_plan_builder.cpp:2035-2076) for single-table queries; multi-table
- * queries go through gdb_plan_build by name. Options: groups=N (dense group
- * capacity, default 256; execute reports GDB_ERR_OVERFLOW beyond it).
+ * metal_plan_builder.cpp:2035-2076) for those queries; the others go through
+ * gdb_plan_build by name. Options: groups=N (dense group capacity;
+ * default what fits register accumulators, at most 8; execute reports
+ * GDB_ERR_OVERFLOW beyond it, word 3 of buffer d_sql_groups_status holding
+ * the exact group count).
  * GDB_ERR_INVALID with gdb_engine_last_error() naming the construct otherwise. */
 gdb_status gdb_plan_build_sql(const char* sql, const char* options, gdb_plan* out);
 /* JSON layout of a SQL plan's result: table, group keys (aggregate column or
