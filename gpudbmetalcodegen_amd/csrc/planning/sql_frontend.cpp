@@ -858,10 +858,32 @@ struct Planner {
                 type = typeName(keyVals[j].t);
                 return "{\"key\":" + std::to_string(j) + "}";
             }
+        auto boolean = [&](const std::string& op, const std::vector<NodeP>& args) {
+            std::string a;
+            for (const auto& x : args) {
+                int sc = 0;
+                std::string ty;
+                a += (a.empty() ? "" : ",") + post(x, sc, ty);
+            }
+            type = "bool";
+            scale = 0;
+            return "{\"op\":\"" + op + "\",\"args\":[" + a + "]}";
+        };
         switch (n->k) {
             case NK::Num:
                 scale = n->scale;
                 return "{\"const\":\"" + std::to_string(n->iv) + "\",\"scale\":" + std::to_string(n->scale) + "}";
+            case NK::Str: type = "str"; return "{\"str\":" + jstr(n->s) + "}";
+            case NK::Date: type = "date"; return "{\"date\":" + std::to_string(n->iv) + "}";
+            case NK::Cmp: return boolean(n->s == "<>" ? "!=" : n->s == "=" ? "==" : n->s, n->a);
+            case NK::And: return boolean("and", n->a);
+            case NK::Or: return boolean("or", n->a);
+            case NK::Not: return boolean("not", n->a);
+            case NK::Between: {
+                const std::string c = boolean("and", {mk(NK::Cmp, ">=", {n->a[0], n->a[1]}),
+                                                      mk(NK::Cmp, "<=", {n->a[0], n->a[2]})});
+                return n->neg ? "{\"op\":\"not\",\"args\":[" + c + "]}" : c;
+            }
             case NK::Neg: {
                 const std::string a = post(n->a[0], scale, type);
                 if (type != "num") fail("negation of a non-number");
@@ -949,7 +971,6 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     if (q.tables.size() > 2)
         fail("one table, or two joined on a key column, are planned from SQL; other multi-table queries are "
              "planned by name");
-    if (q.having) fail("HAVING is not supported");
     Planner pl;
     long long cap = std::stoll(opt(opts, "groups", "0"));
     if (cap < 0 || cap > (1LL << 26)) fail("groups option out of range");
@@ -1042,7 +1063,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     pl.aggs.push_back({"_n", AggKind::Count, "1", 0});  // row count: empty groups are dropped, AVG divides by it
     pl.aggByKey[std::to_string(static_cast<int>(AggKind::Count)) + "|1"] = "_n";
 
-    std::string targets, keysJson, order;
+    std::string targets, keysJson, order, having = "null";
     bool anyAgg = false;
     std::vector<std::string> names;
     for (size_t i = 0; i < q.items.size(); ++i) {
@@ -1057,6 +1078,12 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
                    std::to_string(sc) + ",\"type\":\"" + ty + "\"}";
     }
     if (!anyAgg && !pl.grouped) fail("a select list without aggregates or GROUP BY (row output) is not supported");
+    if (q.having) {  // a filter on the finished groups, evaluated with the select list
+        int sc = 0;
+        std::string ty;
+        having = pl.post(q.having, sc, ty);
+        if (ty != "bool") fail("HAVING needs a boolean condition");
+    }
     for (size_t i = 0; i < q.orderBy.size(); ++i) {
         const auto& o = q.orderBy[i];
         std::string ref;
@@ -1258,7 +1285,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         if (a.code == "(i64)_g_payload") rep = a.name;
     plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":" + jstr(pl.cc.other) + ",\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
                      keysJson + "],\"rep\":" + jstr(rep) + ",\"targets\":[" + targets + "],\"order\":[" + order +
-                     "],\"limit\":" + std::to_string(q.limit) + "}";
+                     "],\"having\":" + having + ",\"limit\":" + std::to_string(q.limit) + "}";
     return plan;
 }
 

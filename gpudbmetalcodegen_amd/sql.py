@@ -9,7 +9,7 @@ of a ``TpchDB`` and finishes the rows the way the plan's layout says: group
 keys (aggregate columns, or string columns attached from each group's
 representative row), the select list evaluated exactly over the aggregate
 columns (rationals; a column is shown with its fixed-point digits, an
-average or quotient with six more), ORDER BY, LIMIT. A GROUP BY with more
+average or quotient with six more), HAVING, ORDER BY, LIMIT. A GROUP BY with more
 groups than the plan's dense capacity is rebuilt once, sized by the exact
 group count the first run took.
 """
@@ -36,6 +36,10 @@ class SqlRun:
     plan: E.HipQueryPlan = None
     layout: dict = field(default_factory=dict)
     canonical: str = ""
+
+
+_CMP = {"==": lambda a, b: a == b, "!=": lambda a, b: a != b, "<": lambda a, b: a < b,
+        "<=": lambda a, b: a <= b, ">": lambda a, b: a > b, ">=": lambda a, b: a >= b}
 
 
 def _trunc(v: Fraction, scale: int) -> int:
@@ -83,8 +87,19 @@ def finish(layout: dict, cols: dict, host_table: dict | None) -> tuple[dict, dic
             return Fraction(int(cols[num][r]), 10 ** agg_scale(num)) / int(cols[den][r])
         if "const" in e:
             return Fraction(int(e["const"]), 10 ** e["scale"])
-        args = [ev(a, r) for a in e["args"]]
+        if "str" in e:
+            return e["str"]
+        if "date" in e:
+            return e["date"]
         op = e["op"]
+        if op in ("and", "or"):  # short-circuit, as SQL may
+            vals = (ev(a, r) for a in e["args"])
+            return all(vals) if op == "and" else any(vals)
+        args = [ev(a, r) for a in e["args"]]
+        if op == "not":
+            return not args[0]
+        if op in _CMP:
+            return _CMP[op](args[0], args[1])
         if op == "neg":
             return -args[0]
         if op == "+":
@@ -98,8 +113,11 @@ def finish(layout: dict, cols: dict, host_table: dict | None) -> tuple[dict, dic
         return args[0] / args[1]
 
     targets = layout["targets"]
+    having = layout.get("having")
     rows = []
     for r in range(n):
+        if having is not None and not ev(having, r):
+            continue
         vals = [ev(t["expr"], r) for t in targets]
         order = []
         for o in layout["order"]:

@@ -53,7 +53,7 @@ gdb_status gdb_gen_text(uint64_t seed, uint64_t stream, uint64_t n, int width, c
 gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out);
 /* Build a plan from SQL text: a SELECT over one TPC-H table, or two joined
  * by an equality on one table's key, with
- * WHERE, GROUP BY, SUM / AVG / MIN / MAX / COUNT, ORDER BY, LIMIT, compiled
+ * WHERE, GROUP BY, SUM / AVG / MIN / MAX / COUNT, HAVING, ORDER BY, LIMIT, compiled
  * to exact fixed-point HIP expressions. Replaces analyzeSQL + the
  * analysis-based buildMetalPlan dispatch (codegen/planning/query_analyzer.h:80-85,
  * metal_plan_builder.cpp:2035-2076) for those queries; the others go through

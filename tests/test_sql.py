@@ -121,7 +121,9 @@ def test_case_when_aligns_scales():
     ("select sum(l_quantity) from lineitem where l_shipdate < '1994-13-01'", "bad date literal"),
     ("select sum(l_quantity) from nosuch", "unknown table"),
     ("select sum(l_quantity) from lineitem where l_comment", "boolean"),
-    ("select sum(l_quantity) from lineitem group by l_returnflag having count(*) > 1", "HAVING"),
+    ("select sum(l_quantity) from lineitem group by l_returnflag having l_quantity > 1",
+     "must be an aggregate or appear in GROUP BY"),
+    ("select sum(l_quantity) from lineitem group by l_returnflag having sum(l_quantity)", "HAVING needs a boolean"),
     ("select sum(l_quantity) from lineitem where l_shipdate = 3", "comparison of date with number"),
     ("select sum(l_quantity) from lineitem limit 2.5", "LIMIT takes an integer"),
 ])
@@ -169,3 +171,18 @@ def test_cli_sql_usage_errors(tmp_path):
 
     assert tpch_cli.main(["--sql", str(tmp_path / "missing.sql")]) == 3
     assert tpch_cli.main([]) == 3
+
+
+def test_having_filters_finished_groups():
+    p = E.build_sql_plan("select l_returnflag, count(*) from lineitem group by l_returnflag "
+                         "having sum(l_quantity) > 2.5 and not l_returnflag = 'N' order by l_returnflag")
+    lay = p.sql_layout()
+    assert lay["having"]["op"] == "and"
+    cols = {"_n": [4, 2, 9], "k2": [ord("A"), ord("N"), ord("R")], "a1": [300, 900, 250],
+            "__scales": {"a1": 2, "_n": 0}}
+    key = lay["keys"][0]["agg"]
+    cols[key] = cols.pop("k2")
+    sum_name = lay["having"]["args"][0]["args"][0]["agg"]
+    cols[sum_name] = cols.pop("a1")
+    out, _ = S.finish(lay, cols, None)
+    assert out == {"l_returnflag": ["A"], "count(*)": [4]}   # N excluded by name, R by its sum (2.50)

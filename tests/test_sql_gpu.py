@@ -121,6 +121,12 @@ QUERIES = [
     # a condition over both tables (after the join)
     "select count(*), sum(l_extendedprice) from lineitem, part where p_partkey = l_partkey "
     "and (p_size > 25 or l_quantity < 10)",
+    # HAVING over an aggregate (the inner query of TPC-H Q18, at this scale's bound)
+    "select l_orderkey, sum(l_quantity) as q from lineitem group by l_orderkey having sum(l_quantity) > 250 "
+    "order by q desc, l_orderkey limit 20",
+    # HAVING mixing an aggregate and a key
+    "select l_returnflag, count(*) from lineitem group by l_returnflag having count(*) > 1000 "
+    "and l_returnflag <> 'N' order by l_returnflag",
     # INT key of the joined table, ORDER BY an aggregate
     "select c_nationkey, count(*), avg(o_totalprice) from orders, customer where o_custkey = c_custkey "
     "and o_totalprice > 100000 group by c_nationkey order by count(*) desc, c_nationkey limit 5",
