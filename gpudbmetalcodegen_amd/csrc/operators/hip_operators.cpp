@@ -1012,8 +1012,10 @@ void HipRowCheck::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 std::string HipRowCheck::describe() const { return "RowCheck(" + cond_ + ")"; }
 
 HipHash64Probe::HipHash64Probe(std::unique_ptr<HipOperator> child, const std::string& table,
-                               const std::string& keyExpr, const std::string& slotsExpr, const std::string& slotVar)
-    : HipUnaryOperator(std::move(child)), table_(table), key_(keyExpr), slots_(slotsExpr), slotVar_(slotVar) {}
+                               const std::string& keyExpr, const std::string& slotsExpr, const std::string& slotVar,
+                               bool anti)
+    : HipUnaryOperator(std::move(child)), table_(table), key_(keyExpr), slots_(slotsExpr), slotVar_(slotVar),
+      anti_(anti) {}
 
 void HipHash64Probe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(table_, "u64", 8, slots_ + " * 2", InitKind::None, {}, false, true);
@@ -1024,6 +1026,10 @@ void HipHash64Probe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     child_->produce(cg, [&]() {
         cg.addLine("const i64 " + slotVar_ + " = gdb_ht64_find(" + table_ + ", _pbits_" + table_ + ", (u64)(" + key_ +
                    "), _pmaxp_" + table_ + ");");
+        if (anti_) {
+            cg.addIf(slotVar_ + " < 0", [&]() { consume(); });
+            return;
+        }
         cg.addIf(slotVar_ + " >= 0", [&]() {
             cg.addLine("const u64 " + slotVar_ + "_payload = " + table_ + "[(u64)" + slotVar_ + " * 2 + 1];");
             consume();
@@ -1031,7 +1037,9 @@ void HipHash64Probe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     });
 }
 
-std::string HipHash64Probe::describe() const { return "Hash64Probe(" + table_ + ", key=" + key_ + ")"; }
+std::string HipHash64Probe::describe() const {
+    return std::string(anti_ ? "Hash64AntiProbe(" : "Hash64Probe(") + table_ + ", key=" + key_ + ")";
+}
 
 HipDenseAtomicMinMax::HipDenseAtomicMinMax(std::unique_ptr<HipOperator> child, const std::string& arrayName,
                                            const std::string& indexExpr, const std::string& sizeExpr,

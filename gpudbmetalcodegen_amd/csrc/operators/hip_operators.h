@@ -294,15 +294,18 @@ private:
     std::string name_, cond_, message_;
 };
 
+// anti: rows whose key is absent are consumed (NOT EXISTS / NOT IN); the
+// slot variable is then not defined.
 class HipHash64Probe : public HipUnaryOperator {
 public:
     HipHash64Probe(std::unique_ptr<HipOperator> child, const std::string& table, const std::string& keyExpr,
-                   const std::string& slotsExpr, const std::string& slotVar);
+                   const std::string& slotsExpr, const std::string& slotVar, bool anti = false);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string table_, key_, slots_, slotVar_;
+    bool anti_ = false;
 };
 
 // Atomic add of an i64 value into word `word` of a hash-table slot

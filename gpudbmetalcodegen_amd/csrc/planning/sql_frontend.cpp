@@ -94,8 +94,12 @@ std::vector<Tok> lex(const std::string& q) {
 // AST and parser
 // ===========================================================================
 
-enum class NK { Col, Num, Str, Date, Interval, Neg, Arith, Cmp, And, Or, Not, Between, In, Like, Case, Agg, Extract, Star };
+enum class NK {
+    Col, Num, Str, Date, Interval, Neg, Arith, Cmp, And, Or, Not, Between, In, Like, Case, Agg, Extract, Star,
+    Exists, InSub
+};
 
+struct Select;
 struct Node;
 using NodeP = std::shared_ptr<Node>;
 struct Node {
@@ -106,6 +110,7 @@ struct Node {
     bool distinct = false;   // COUNT(DISTINCT ...)
     long long iv = 0;        // Num: value * 10^-scale; Date: YYYYMMDD; Interval: count
     int scale = 0;
+    std::shared_ptr<Select> sub;  // EXISTS (subquery), x IN (subquery)
 };
 
 NodeP mk(NK k, const std::string& s = "", std::vector<NodeP> a = {}) {
@@ -188,20 +193,31 @@ public:
     explicit Parser(std::vector<Tok> t) : t_(std::move(t)) {}
 
     Select select() {
+        Select q = selectBody();
+        acceptSym(";");
+        if (peek().kind != Tok::End) fail("unexpected '" + peek().v + "' after the query");
+        return q;
+    }
+
+private:
+    Select selectBody(bool subquery = false) {
         Select q;
         expectKw("select");
         if (isKw("distinct")) fail("SELECT DISTINCT is not supported");
-        if (isSym("*")) fail("SELECT * (row output) is not supported; select aggregates");
-        do {
-            SelectItem it;
-            it.e = expr();
-            if (acceptKw("as")) {
-                it.alias = ident("an alias");
-            } else if (peek().kind == Tok::Ident && !reserved(peek().v)) {
-                it.alias = ident("an alias");
-            }
-            q.items.push_back(it);
-        } while (acceptSym(","));
+        const bool star = subquery && acceptSym("*");  // EXISTS (SELECT * ...): no select list
+        if (!star) {
+            if (isSym("*")) fail("SELECT * (row output) is not supported; select aggregates");
+            do {
+                SelectItem it;
+                it.e = expr();
+                if (acceptKw("as")) {
+                    it.alias = ident("an alias");
+                } else if (peek().kind == Tok::Ident && !reserved(peek().v)) {
+                    it.alias = ident("an alias");
+                }
+                q.items.push_back(it);
+            } while (acceptSym(","));
+        }
         expectKw("from");
         do {
             if (acceptSym("(")) fail("subqueries in FROM are not supported");
@@ -231,12 +247,16 @@ public:
             if (peek().kind != Tok::Number || peek().v.find('.') != std::string::npos) fail("LIMIT takes an integer");
             q.limit = std::stoll(next().v);
         }
-        acceptSym(";");
-        if (peek().kind != Tok::End) fail("unexpected '" + peek().v + "' after the query");
         return q;
     }
 
-private:
+    std::shared_ptr<Select> subquery() {
+        expectSym("(");
+        auto q = std::make_shared<Select>(selectBody(true));
+        expectSym(")");
+        return q;
+    }
+
     std::vector<Tok> t_;
     size_t p_ = 0;
 
@@ -278,7 +298,11 @@ private:
     }
     NodeP notE() {
         if (acceptKw("not")) return mk(NK::Not, "", {notE()});
-        if (isKw("exists")) fail("EXISTS subqueries are not supported");
+        if (acceptKw("exists")) {
+            NodeP n = mk(NK::Exists);
+            n->sub = subquery();
+            return n;
+        }
         return pred();
     }
     NodeP pred() {
@@ -299,8 +323,13 @@ private:
             return n;
         }
         if (acceptKw("in")) {
+            if (isSym("(") && isKw("select", 1)) {
+                NodeP n = mk(NK::InSub, "", {l});
+                n->sub = subquery();
+                n->neg = neg;
+                return n;
+            }
             expectSym("(");
-            if (isKw("select")) fail("IN (subquery) is not supported");
             NodeP n = mk(NK::In, "", {l});
             do n->a.push_back(add());
             while (acceptSym(","));
@@ -792,6 +821,8 @@ struct Compiler {
                 return r;
             }
             case NK::Agg: fail("aggregate " + text(n) + " inside a row expression");
+            case NK::Exists:
+            case NK::InSub: fail("EXISTS / IN (subquery) are planned only as AND conditions of WHERE");
             case NK::Star: fail("* outside COUNT(*)");
         }
         fail("unsupported expression");
@@ -982,6 +1013,74 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     // join when they read only the scanned table.
     std::vector<NodeP> conj, buildConj, probeConj, crossConj;
     if (q.where) conjuncts(q.where, conj);
+
+    // [NOT] EXISTS (correlated by one equality) and x [NOT] IN (subquery)
+    // conditions become semi- / anti-joins: the subquery's table, filtered
+    // by its own conditions, builds a set of its key column; the outer rows
+    // probe it.
+    struct Semi {
+        std::string table, innerKey;
+        NodeP outerKey;
+        std::vector<NodeP> filter;
+        bool anti = false;
+    };
+    std::vector<Semi> semis;
+    for (auto it = conj.begin(); it != conj.end();) {
+        NodeP n = *it;
+        bool anti = false;
+        if (n->k == NK::Not && n->a[0]->k == NK::Exists) {
+            anti = true;
+            n = n->a[0];
+        }
+        if (n->k != NK::Exists && n->k != NK::InSub) {
+            ++it;
+            continue;
+        }
+        if (n->k == NK::InSub) anti = n->neg;
+        const Select& sq = *n->sub;
+        if (sq.tables.size() != 1 || !sq.groupBy.empty() || sq.having || !sq.orderBy.empty() || sq.limit >= 0)
+            fail("subqueries are planned over one table, without GROUP BY, HAVING, ORDER BY or LIMIT");
+        for (const auto& t : q.tables)
+            if (t == sq.tables[0]) fail("a subquery over the outer query's own table " + t + " is not supported");
+        Semi sm;
+        sm.table = sq.tables[0];
+        sm.anti = anti;
+        const infra::TpchTable* sc = catalogOf(sm.table);
+        std::vector<NodeP> sub;
+        if (sq.where) conjuncts(sq.where, sub);
+        if (n->k == NK::InSub) {
+            if (sq.items.size() != 1 || sq.items[0].e->k != NK::Col || !findColumn(sc, sq.items[0].e->s))
+                fail("IN (subquery) selects one column of its table");
+            sm.innerKey = sq.items[0].e->s;
+            sm.outerKey = n->a[0];
+            sm.filter = sub;
+        } else {
+            for (const auto& c : sub) {
+                std::string inner, outer;
+                if (c->k == NK::Cmp && c->s == "=" && c->a[0]->k == NK::Col && c->a[1]->k == NK::Col) {
+                    const std::string x = c->a[0]->s, y = c->a[1]->s;
+                    if (findColumn(sc, x) && !findColumn(sc, y)) inner = x, outer = y;
+                    else if (findColumn(sc, y) && !findColumn(sc, x)) inner = y, outer = x;
+                }
+                if (inner.empty()) {
+                    sm.filter.push_back(c);
+                    continue;
+                }
+                if (!sm.innerKey.empty()) fail("EXISTS with more than one correlation");
+                sm.innerKey = inner;
+                sm.outerKey = mk(NK::Col, outer);
+            }
+            if (sm.innerKey.empty()) fail("EXISTS without an equality to a column of the outer query");
+        }
+        for (const auto& f : sm.filter) {
+            std::set<std::string> cols;
+            columnsOf(f, cols);
+            for (const auto& c : cols)
+                if (!findColumn(sc, c)) fail("subquery condition " + text(f) + " reads another table");
+        }
+        semis.push_back(sm);
+        it = conj.erase(it);
+    }
     pl.cc.table = q.tables[0];
     pl.cc.cat = catalogOf(q.tables[0]);
     Compiler bc;  // the build side's scan
@@ -1040,6 +1139,24 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     }
 
     const std::string where = andAll(pl.cc, probeConj);
+    // Semi-join keys: INT columns on both sides (32-bit set keys).
+    std::vector<std::string> semiOuter, semiInner, semiFilter;
+    std::vector<Compiler> semiCc(semis.size());
+    for (size_t k = 0; k < semis.size(); ++k) {
+        const Semi& sm = semis[k];
+        Compiler& sc = semiCc[k];
+        sc.table = sm.table;
+        sc.cat = catalogOf(sm.table);
+        auto intCol = [&](const infra::TpchTable* t, const NodeP& e) {
+            const infra::TpchColumn* c = e->k == NK::Col ? findColumn(t, e->s) : nullptr;
+            return c && c->type == infra::TpchType::INT;
+        };
+        if (!intCol(pl.cc.cat, sm.outerKey) || !intCol(sc.cat, mk(NK::Col, sm.innerKey)))
+            fail("semi-join keys must be integer columns (" + text(sm.outerKey) + ", " + sm.innerKey + ")");
+        semiOuter.push_back(pl.cc.expr(sm.outerKey).code);
+        semiInner.push_back(sc.column(sm.innerKey, "").code);
+        semiFilter.push_back(andAll(sc, sm.filter));
+    }
     std::string probeKeyCode, buildKeyCode, buildWhere, cross;
     if (joined) {
         probeKeyCode = pl.cc.column(probeKey, "").code;
@@ -1150,6 +1267,11 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         }
         std::unique_ptr<HipOperator> r = std::move(sc);
         if (!where.empty()) r = std::make_unique<HipSelection>(std::move(r), where);
+        for (size_t k = 0; k < semis.size(); ++k)
+            r = std::make_unique<HipHash64Probe>(std::move(r), "d_sql_semi" + std::to_string(k),
+                                                 "(u64)(u32)(" + semiOuter[k] + ")",
+                                                 "sql_semi" + std::to_string(k) + "_slots", "_s" + std::to_string(k),
+                                                 semis[k].anti);
         if (joined) {
             std::vector<std::pair<std::string, ColumnType>> bind;
             for (const auto& c : pl.cc.usedOther) bind.push_back({c, catalogTable(pl.cc.other).type(c)});
@@ -1167,6 +1289,27 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     HipQueryPlan plan;
     plan.name = "SQL";
     const int items = optInt(opts, "items", 4);
+    std::string semiJson;
+    for (size_t k = 0; k < semis.size(); ++k) {
+        // The subquery's key set (first arrivals only: a key repeats freely).
+        const std::string K = std::to_string(k);
+        auto sc = std::make_unique<HipGridStrideScan>(semis[k].table, "i");
+        const auto& ct = catalogTable(semis[k].table);
+        for (const auto& c : semiCc[k].cat->columns)
+            if (semiCc[k].used.count(c.name)) {
+                sc->addColumn(c.name, ct.type(c.name));
+                if (c.type == infra::TpchType::CHAR_FIXED) sc->setLazy(c.name);
+            }
+        std::unique_ptr<HipOperator> r = std::move(sc);
+        if (!semiFilter[k].empty()) r = std::make_unique<HipSelection>(std::move(r), semiFilter[k]);
+        auto ht = std::make_unique<HipHash64Build>(std::move(r), "d_sql_semi" + K, "(u64)(u32)(" + semiInner[k] + ")",
+                                                   "0", "sql_semi" + K + "_slots", true);
+        ht->setBlockFirst();
+        appendPhase(plan, "SQL_semi" + K + "_build", std::move(ht), 256, items, {2, 4});
+        plan.requiredSymbols.push_back("sql_semi" + K + "_slots");
+        semiJson += std::string(k ? "," : "") + "{\"table\":" + jstr(semis[k].table) + ",\"slots\":\"sql_semi" + K +
+                    "_slots\",\"anti\":" + (semis[k].anti ? "true" : "false") + "}";
+    }
     if (joined) {
         // Build side: key -> row index; a repeated key fails the run.
         auto sc = std::make_unique<HipGridStrideScan>(bc.table, "i");
@@ -1283,7 +1426,8 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     std::string rep;
     for (const auto& a : pl.aggs)
         if (a.code == "(i64)_g_payload") rep = a.name;
-    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":" + jstr(pl.cc.other) + ",\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
+    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":" + jstr(pl.cc.other) + ",\"semi\":[" +
+                     semiJson + "],\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
                      keysJson + "],\"rep\":" + jstr(rep) + ",\"targets\":[" + targets + "],\"order\":[" + order +
                      "],\"having\":" + having + ",\"limit\":" + std::to_string(q.limit) + "}";
     return plan;

@@ -168,6 +168,12 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
         ex = executor or E.HipGenericExecutor(ctx)
         ex.register_table(db.device(table))
         ex.register_symbol("sql_slots", T._pow2(2 * max(rows, 1)))
+        for sm in layout.get("semi", []):  # the key sets of [NOT] EXISTS / IN subqueries
+            if sm["table"] not in db.host:
+                raise ValueError(f"table {sm['table']} is not loaded")
+            ex.register_table(db.device(sm["table"]))
+            srows = int(next(iter(db.host[sm["table"]].values())).shape[0])
+            ex.register_symbol(sm["slots"], T._pow2(2 * max(srows, 1)))
         join = layout.get("join") or ""
         if join:  # the key side of a two-table join
             if join not in db.host:

@@ -117,7 +117,18 @@ def test_case_when_aligns_scales():
     ("select sum(l_quantity / 2) from lineitem", "division of row values"),
     ("select sum(nope) from lineitem", "no column nope"),
     ("select count(distinct l_suppkey) from lineitem", "COUNT(DISTINCT)"),
-    ("select sum(l_quantity) from lineitem where l_orderkey in (select o_orderkey from orders)", "IN (subquery)"),
+    ("select count(*) from orders where exists (select * from orders where o_custkey = o_orderkey)",
+     "outer query's own table"),
+    ("select count(*) from orders where exists (select * from lineitem where l_quantity > 3)",
+     "EXISTS without an equality"),
+    ("select count(*) from orders where o_orderkey in (select l_orderkey, l_suppkey from lineitem)",
+     "selects one column"),
+    ("select count(*) from orders where exists (select * from lineitem where l_orderkey = o_orderkey) "
+     "or o_orderkey < 3", "only as AND conditions"),
+    ("select count(*) from orders where o_totalprice in (select l_extendedprice from lineitem)",
+     "integer columns"),
+    ("select count(*) from orders where o_orderkey in (select l_orderkey from lineitem group by l_orderkey)",
+     "without GROUP BY"),
     ("select sum(l_quantity) from lineitem where l_shipdate < '1994-13-01'", "bad date literal"),
     ("select sum(l_quantity) from nosuch", "unknown table"),
     ("select sum(l_quantity) from lineitem where l_comment", "boolean"),
@@ -186,3 +197,17 @@ def test_having_filters_finished_groups():
     cols[sum_name] = cols.pop("a1")
     out, _ = S.finish(lay, cols, None)
     assert out == {"l_returnflag": ["A"], "count(*)": [4]}   # N excluded by name, R by its sum (2.50)
+
+
+def test_exists_and_not_in_become_semi_and_anti_joins():
+    p = E.build_sql_plan((SQL_DIR / "tpch_q4.sql").read_text())
+    assert p.phases() == ["SQL_semi0_build", "SQL_groups", "SQL_aggregate"]
+    d = p.describe()
+    assert "Hash64BuildDistinct(d_sql_semi0, key=(u64)(u32)((i64)l_orderkey)" in d
+    assert "Selection(((i64)l_commitdate < (i64)l_receiptdate))" in d   # the subquery's own condition
+    assert p.sql_layout()["semi"] == [{"table": "lineitem", "slots": "sql_semi0_slots", "anti": False}]
+    q = E.build_sql_plan("select count(*) from customer where c_custkey not in (select o_custkey from orders)")
+    assert "Hash64AntiProbe" in q.describe() and q.sql_layout()["semi"][0]["anti"] is True
+    r = E.build_sql_plan("select count(*) from customer where not exists "
+                         "(select * from orders where o_custkey = c_custkey)")
+    assert "Hash64AntiProbe" in r.describe()

@@ -1,8 +1,8 @@
 """SQL front end on the GPU: SELECT text -> engine parser -> HIP plan ->
 exact rows, on generated TPC-H tables (SF 0.01).
 
-TPC-H Q1, Q6, Q12 and Q14 in their SQL text are compared bit-exactly with
-the oracle (oracle/tpch.py, itself pinned against SQLite). The other queries, written in
+TPC-H Q1, Q4, Q6, Q12 and Q14 in their SQL text are compared bit-exactly
+with the oracle (oracle/tpch.py, itself pinned against SQLite). The other queries, written in
 the dialect both engines read, are compared with SQLite running the same text
 on the same rows: strings, dates and counts exactly, decimals within 1e-9
 relative (SQLite sums REALs; the engine's values are exact fixed point).
@@ -75,6 +75,15 @@ def test_tpch_q12_sql_text_matches_oracle(env):
     assert r.columns["low_line_count"] == exp["low_line_count"]
 
 
+def test_tpch_q4_sql_text_matches_oracle(env):
+    ctx, db, host = env
+    r = S.run_sql(ctx, db, (SQL_DIR / "tpch_q4.sql").read_text())
+    exp, _ = OT.run(host, "Q4", T.PARAMS["Q4"])
+    # the catalog keeps the priority's first byte: '1' for '1-URGENT'
+    assert r.columns["o_orderpriority"] == [p[0] for p in exp["o_orderpriority"]]
+    assert r.columns["order_count"] == exp["order_count"]
+
+
 QUERIES = [
     # two char(1) keys (packed), decimals, dates
     "select l_returnflag, l_linestatus, sum(l_quantity), sum(l_extendedprice * (1 - l_discount)), avg(l_discount), "
@@ -127,6 +136,15 @@ QUERIES = [
     # HAVING mixing an aggregate and a key
     "select l_returnflag, count(*) from lineitem group by l_returnflag having count(*) > 1000 "
     "and l_returnflag <> 'N' order by l_returnflag",
+    # NOT EXISTS: customers without orders (an anti-join)
+    "select c_mktsegment, count(*), sum(c_acctbal) from customer where not exists "
+    "(select * from orders where o_custkey = c_custkey) group by c_mktsegment order by c_mktsegment",
+    # IN (subquery) with its own condition, beside a join
+    "select o_orderstatus, count(*) from lineitem, orders where l_orderkey = o_orderkey and l_partkey in "
+    "(select p_partkey from part where p_size < 10) group by o_orderstatus order by o_orderstatus",
+    # NOT IN with a filtered subquery
+    "select count(*), sum(ps_availqty) from partsupp where ps_suppkey not in "
+    "(select s_suppkey from supplier where s_acctbal > 5000)",
     # INT key of the joined table, ORDER BY an aggregate
     "select c_nationkey, count(*), avg(o_totalprice) from orders, customer where o_custkey = c_custkey "
     "and o_totalprice > 100000 group by c_nationkey order by count(*) desc, c_nationkey limit 5",
