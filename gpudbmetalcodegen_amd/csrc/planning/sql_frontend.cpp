@@ -573,15 +573,26 @@ const infra::TpchColumn* findColumn(const infra::TpchTable* t, const std::string
     return nullptr;
 }
 
+// A table joined to the scanned one by its key, read at row index `row`
+// (the join's match).
+struct Joined {
+    std::string table;
+    const infra::TpchTable* cat = nullptr;
+    std::string row;
+    std::set<std::string> used;
+};
+
 struct Compiler {
     std::string table;  // the scanned table
     const infra::TpchTable* cat = nullptr;
     std::set<std::string> used;  // its columns read
-    // A joined table, read at row index `otherRow` (the join's match).
-    std::string other;
-    const infra::TpchTable* otherCat = nullptr;
-    std::string otherRow;
-    std::set<std::string> usedOther;
+    std::vector<Joined> joins;
+
+    const Joined* joinedOf(const std::string& name) const {
+        for (const auto& j : joins)
+            if (findColumn(j.cat, name)) return &j;
+        return nullptr;
+    }
 
     // rep empty: the scanned row's bindings; else the row index `rep`.
     Val column(const std::string& name, const std::string& rep) {
@@ -590,13 +601,15 @@ struct Compiler {
         std::string row = rep;
         if (c) {
             used.insert(name);
-        } else if ((c = findColumn(otherCat, name)) != nullptr) {
-            if (otherRow.empty()) fail("column " + name + " of " + other + " outside the join");
-            t = other;
-            row = otherRow;
-            usedOther.insert(name);
         } else {
-            fail("no column " + name + " in " + table + (other.empty() ? "" : " or " + other));
+            Joined* j = nullptr;
+            for (auto& x : joins)
+                if (findColumn(x.cat, name)) j = &x;
+            if (!j) fail("no column " + name + " in the query's tables");
+            c = findColumn(j->cat, name);
+            t = j->table;
+            row = j->row;
+            j->used.insert(name);
         }
         const std::string ptr = tableDataName(t, name);
         Val v;
@@ -999,19 +1012,17 @@ std::string andAll(Compiler& cc, const std::vector<NodeP>& conj) {
 
 HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     Select q = Parser(lex(sql)).select();
-    if (q.tables.size() > 2)
-        fail("one table, or two joined on a key column, are planned from SQL; other multi-table queries are "
-             "planned by name");
     Planner pl;
     long long cap = std::stoll(opt(opts, "groups", "0"));
     if (cap < 0 || cap > (1LL << 26)) fail("groups option out of range");
 
-    // Two tables: one equality between a column of each, where one side is
-    // its table's key (the build side, read through a hash join by row
-    // index); the other table is scanned. Conjuncts reading only the build
-    // table filter the build; the rest run on the scanned rows, before the
-    // join when they read only the scanned table.
-    std::vector<NodeP> conj, buildConj, probeConj, crossConj;
+    // Several tables: a tree of key joins. Every table but one (the scanned
+    // table, the root) is joined by an equality between its key and a column
+    // of its parent; it is built into a hash table key -> row index and read
+    // at the matched row. Conjuncts over one joined table filter its build;
+    // conjuncts over the scanned table alone run before the joins; the rest
+    // after them.
+    std::vector<NodeP> conj, probeConj, crossConj;
     if (q.where) conjuncts(q.where, conj);
 
     // [NOT] EXISTS (correlated by one equality) and x [NOT] IN (subquery)
@@ -1081,61 +1092,101 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         semis.push_back(sm);
         it = conj.erase(it);
     }
-    pl.cc.table = q.tables[0];
-    pl.cc.cat = catalogOf(q.tables[0]);
-    Compiler bc;  // the build side's scan
-    std::string probeKey, buildKey;
-    const bool joined = q.tables.size() == 2;
-    if (joined) {
-        const infra::TpchTable* c1 = catalogOf(q.tables[1]);
-        int ji = -1;
-        std::string k0, k1;
-        for (size_t i = 0; i < conj.size(); ++i) {
-            const NodeP& n = conj[i];
-            if (n->k != NK::Cmp || n->s != "=" || n->a[0]->k != NK::Col || n->a[1]->k != NK::Col) continue;
-            const std::string x = n->a[0]->s, y = n->a[1]->s;
-            std::string a, b;
-            if (findColumn(pl.cc.cat, x) && findColumn(c1, y)) a = x, b = y;
-            else if (findColumn(pl.cc.cat, y) && findColumn(c1, x)) a = y, b = x;
-            else continue;
-            if (ji >= 0) fail("more than one join condition between the two tables");
-            ji = static_cast<int>(i);
-            k0 = a;
-            k1 = b;
-        }
-        if (ji < 0) fail("two tables without an equality between their columns (no cross products)");
-        conj.erase(conj.begin() + ji);
-        static const std::map<std::string, std::string> keyOf = {
-            {"customer", "c_custkey"}, {"orders", "o_orderkey"}, {"part", "p_partkey"},
-            {"supplier", "s_suppkey"}, {"nation", "n_nationkey"}, {"region", "r_regionkey"}};
-        auto isKey = [&](const std::string& t, const std::string& c) { return keyOf.count(t) && keyOf.at(t) == c; };
-        std::string probeT = q.tables[0], buildT = q.tables[1];
-        probeKey = k0;
-        buildKey = k1;
-        if (!isKey(buildT, buildKey)) {
-            if (!isKey(probeT, probeKey)) fail("join on " + k0 + " = " + k1 + ": neither is its table's key");
-            std::swap(probeT, buildT);
-            std::swap(probeKey, buildKey);
-        }
-        pl.cc.table = probeT;
-        pl.cc.cat = catalogOf(probeT);
-        pl.cc.other = buildT;
-        pl.cc.otherCat = catalogOf(buildT);
-        pl.cc.otherRow = "_jb_payload";
-        bc.table = buildT;
-        bc.cat = pl.cc.otherCat;
-        for (const auto& n : conj) {
-            std::set<std::string> cols;
-            columnsOf(n, cols);
-            bool probe = false, build = false;
-            for (const auto& c : cols) {
-                probe |= findColumn(pl.cc.cat, c) != nullptr;
-                build |= findColumn(bc.cat, c) != nullptr;
+    static const std::map<std::string, std::string> keyOf = {
+        {"customer", "c_custkey"}, {"orders", "o_orderkey"}, {"part", "p_partkey"},
+        {"supplier", "s_suppkey"}, {"nation", "n_nationkey"}, {"region", "r_regionkey"}};
+    struct Dim {
+        std::string table, key, parentCol;  // key: this table's key column; parentCol: the parent's join column
+        int parent = -1;                    // index into dims, -1 = the scanned table
+        Compiler build;                     // the build scan
+        std::vector<NodeP> conj;
+        std::string keyCode, probeCode, filter;
+    };
+    std::vector<Dim> dims;
+    std::map<std::string, const infra::TpchTable*> cats;
+    for (const auto& t : q.tables) {
+        if (cats.count(t)) fail("table " + t + " named twice (self-joins are planned by name)");
+        cats[t] = catalogOf(t);
+    }
+    auto ownerOf = [&](const std::string& col) -> std::string {
+        for (const auto& kv : cats)
+            if (findColumn(kv.second, col)) return kv.first;
+        return "";
+    };
+    std::string root = q.tables[0];
+    if (q.tables.size() > 1) {
+        // (child table, its key, parent table, parent column) per join equality
+        struct Edge {
+            std::string child, key, parent, pcol;
+        };
+        std::vector<Edge> edges;
+        for (auto it = conj.begin(); it != conj.end();) {
+            const NodeP& n = *it;
+            std::string ta, tb;
+            if (n->k == NK::Cmp && n->s == "=" && n->a[0]->k == NK::Col && n->a[1]->k == NK::Col) {
+                ta = ownerOf(n->a[0]->s);
+                tb = ownerOf(n->a[1]->s);
             }
-            (build && !probe ? buildConj : probe && build ? crossConj : probeConj).push_back(n);
+            if (ta.empty() || tb.empty() || ta == tb) {
+                ++it;
+                continue;
+            }
+            const std::string x = n->a[0]->s, y = n->a[1]->s;
+            const bool xk = keyOf.count(ta) && keyOf.at(ta) == x, yk = keyOf.count(tb) && keyOf.at(tb) == y;
+            if (yk) edges.push_back({tb, y, ta, x});
+            else if (xk) edges.push_back({ta, x, tb, y});
+            else fail("join on " + x + " = " + y + ": neither is its table's key");
+            it = conj.erase(it);
         }
-    } else {
-        probeConj = conj;
+        std::map<std::string, int> parents;
+        for (const auto& e : edges) ++parents[e.child];
+        std::vector<std::string> roots;
+        for (const auto& t : q.tables) {
+            if (parents[t] > 1) fail("table " + t + " is joined by its key twice (a cycle; planned by name)");
+            if (!parents[t]) roots.push_back(t);
+        }
+        if (roots.size() != 1)
+            fail(roots.empty() ? "the joins form a cycle" : "tables without a join condition (no cross products)");
+        root = roots[0];
+        // breadth-first from the root: parents before children
+        std::vector<std::string> order = {root};
+        for (size_t i = 0; i < order.size(); ++i)
+            for (const auto& e : edges)
+                if (e.parent == order[i]) {
+                    Dim d;
+                    d.table = e.child;
+                    d.key = e.key;
+                    d.parentCol = e.pcol;
+                    for (size_t k = 0; k < dims.size(); ++k)
+                        if (dims[k].table == e.parent) d.parent = static_cast<int>(k);
+                    dims.push_back(d);
+                    order.push_back(e.child);
+                }
+        if (order.size() != q.tables.size()) fail("the join conditions do not connect every table");
+    }
+    pl.cc.table = root;
+    pl.cc.cat = cats[root];
+    for (size_t k = 0; k < dims.size(); ++k) {
+        Joined j;
+        j.table = dims[k].table;
+        j.cat = cats[dims[k].table];
+        j.row = "_j" + std::to_string(k) + "_payload";
+        pl.cc.joins.push_back(j);
+        dims[k].build.table = j.table;
+        dims[k].build.cat = j.cat;
+    }
+    for (const auto& n : conj) {
+        std::set<std::string> cols, owners;
+        columnsOf(n, cols);
+        for (const auto& c : cols) owners.insert(ownerOf(c));
+        if (owners.size() == 1 && *owners.begin() != root && !owners.begin()->empty()) {
+            for (auto& d : dims)
+                if (d.table == *owners.begin()) d.conj.push_back(n);
+        } else if (owners.size() <= 1) {
+            probeConj.push_back(n);
+        } else {
+            crossConj.push_back(n);
+        }
     }
 
     const std::string where = andAll(pl.cc, probeConj);
@@ -1157,19 +1208,21 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         semiInner.push_back(sc.column(sm.innerKey, "").code);
         semiFilter.push_back(andAll(sc, sm.filter));
     }
-    std::string probeKeyCode, buildKeyCode, buildWhere, cross;
-    if (joined) {
-        probeKeyCode = pl.cc.column(probeKey, "").code;
-        buildKeyCode = bc.column(buildKey, "").code;
-        buildWhere = andAll(bc, buildConj);
-        cross = andAll(pl.cc, crossConj);
+    for (auto& d : dims) {
+        const std::string pt = ownerOf(d.parentCol);
+        if (findColumn(cats[pt], d.parentCol)->type != infra::TpchType::INT)
+            fail("join column " + d.parentCol + " is not an integer column");
+        d.keyCode = d.build.column(d.key, "").code;
+        d.filter = andAll(d.build, d.conj);
+        d.probeCode = pl.cc.column(d.parentCol, "").code;  // the parent's column: scanned, or read at its row
     }
+    const std::string cross = andAll(pl.cc, crossConj);
     // GROUP BY items may name a select item's alias.
     for (auto g : q.groupBy) {
         if (g->k == NK::Col)
             for (const auto& it : q.items)
                 if (!it.alias.empty() && it.alias == g->s) {
-                    if (!findColumn(pl.cc.cat, g->s) && !findColumn(pl.cc.otherCat, g->s)) g = it.e;
+                    if (!findColumn(pl.cc.cat, g->s) && !pl.cc.joinedOf(g->s)) g = it.e;
                 }
         pl.keys.push_back(g);
         pl.keyVals.push_back(pl.cc.expr(g));
@@ -1177,6 +1230,53 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     }
     pl.grouped = !pl.keys.empty();
     const std::set<std::string> keyCols = pl.cc.used;  // WHERE and key columns: all the grouping pass reads
+
+    // Functional dependence: a key column whose row is fixed by another key
+    // (its table's key column, the join column equal to it, or an
+    // ancestor's in the join tree) is no part of the group key; the caller
+    // attaches it by walking that row's foreign keys (`via`, `path`).
+    std::vector<int> via(pl.keys.size(), -1);
+    std::vector<std::string> pathJson(pl.keys.size());
+    {
+        auto tableOf = [&](int t) { return t < 0 ? root : dims[static_cast<size_t>(t)].table; };
+        auto keyColOf = [&](int t) { return t < 0 ? (keyOf.count(root) ? keyOf.at(root) : "") : dims[static_cast<size_t>(t)].key; };
+        auto identifies = [&](const NodeP& k, int t) {
+            if (k->k != NK::Col) return false;
+            if (t < 0) return !keyColOf(-1).empty() && k->s == keyColOf(-1);
+            return k->s == dims[static_cast<size_t>(t)].key || k->s == dims[static_cast<size_t>(t)].parentCol;
+        };
+        for (size_t j = 0; j < pl.keys.size(); ++j) {
+            if (pl.keys[j]->k != NK::Col) continue;
+            const std::string own = ownerOf(pl.keys[j]->s);
+            int d = -1;
+            for (size_t k = 0; k < dims.size(); ++k)
+                if (dims[k].table == own) d = static_cast<int>(k);
+            std::vector<int> chain;  // the owner, then its ancestors up to the scanned table
+            for (int t = d;; t = dims[static_cast<size_t>(t)].parent) {
+                chain.push_back(t);
+                if (t < 0) break;
+            }
+            for (size_t up = 0; up < chain.size() && via[j] < 0; ++up)
+                for (size_t i = 0; i < pl.keys.size(); ++i) {
+                    if (i == j || via[i] >= 0 || pl.keys[i]->k != NK::Col || pl.keys[i]->s == pl.keys[j]->s ||
+                        !identifies(pl.keys[i], chain[up]))
+                        continue;
+                    if (up == 0 && identifies(pl.keys[j], chain[0])) continue;  // a key column itself stays a key
+                    via[j] = static_cast<int>(i);
+                    std::string path = "{\"table\":" + jstr(tableOf(chain[up])) + ",\"key\":" + jstr(keyColOf(chain[up])) + "}";
+                    for (size_t m = up; m-- > 0;) {
+                        const Dim& dm = dims[static_cast<size_t>(chain[m])];
+                        path += ",{\"table\":" + jstr(dm.table) + ",\"key\":" + jstr(dm.key) + ",\"fk\":" +
+                                jstr(dm.parentCol) + "}";
+                    }
+                    pathJson[j] = path;
+                    break;
+                }
+        }
+    }
+    std::vector<size_t> ind;  // the independent keys: the group key proper
+    for (size_t j = 0; j < pl.keys.size(); ++j)
+        if (via[j] < 0) ind.push_back(j);
     pl.aggs.push_back({"_n", AggKind::Count, "1", 0});  // row count: empty groups are dropped, AVG divides by it
     pl.aggByKey[std::to_string(static_cast<int>(AggKind::Count)) + "|1"] = "_n";
 
@@ -1226,7 +1326,10 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     for (size_t j = 0; j < pl.keys.size(); ++j) {
         const Val& v = pl.keyVals[j];
         std::string src;
-        if (v.t == VT::Str && v.width <= 7) {  // short strings: the bytes themselves, packed
+        if (via[j] >= 0) {
+            src = "\"via\":" + std::to_string(via[j]) + ",\"path\":[" + pathJson[j] + "],\"column\":" +
+                  jstr(pl.keys[j]->s);
+        } else if (v.t == VT::Str && v.width <= 7) {  // short strings: the bytes themselves, packed
             src = "\"agg\":\"" +
                   pl.addAgg(AggKind::MinI64, "(i64)gdb_pack_bytes(" + v.code + ", " + std::to_string(v.width) + ")", 0,
                             "k") +
@@ -1234,6 +1337,9 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         } else if (v.t == VT::Str) {
             strKey = true;
             if (pl.keys[j]->k != NK::Col) fail("GROUP BY over a string expression");
+            if (ownerOf(pl.keys[j]->s) != root)
+                fail("GROUP BY " + pl.keys[j]->s + ": a string of a joined table, without that table's key among "
+                     "the keys");
             src = "\"column\":" + jstr(pl.keys[j]->s);
         } else {
             src = "\"agg\":\"" + pl.addAgg(AggKind::MinI64, v.code, v.scale, "k") + "\"";
@@ -1272,14 +1378,16 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
                                                  "(u64)(u32)(" + semiOuter[k] + ")",
                                                  "sql_semi" + std::to_string(k) + "_slots", "_s" + std::to_string(k),
                                                  semis[k].anti);
-        if (joined) {
+        for (size_t k = 0; k < dims.size(); ++k) {  // parents before children
+            const Joined& j = pl.cc.joins[k];
             std::vector<std::pair<std::string, ColumnType>> bind;
-            for (const auto& c : pl.cc.usedOther) bind.push_back({c, catalogTable(pl.cc.other).type(c)});
-            if (!bind.empty()) r = std::make_unique<HipBindColumns>(std::move(r), pl.cc.other, bind);
-            r = std::make_unique<HipHash64Probe>(std::move(r), "d_sql_join", "(u64)(u32)(" + probeKeyCode + ")",
-                                                 "sql_join_slots", "_jb");
-            if (!cross.empty()) r = std::make_unique<HipSelection>(std::move(r), cross);
+            for (const auto& c : j.used) bind.push_back({c, catalogTable(j.table).type(c)});
+            if (!bind.empty()) r = std::make_unique<HipBindColumns>(std::move(r), j.table, bind);
+            const std::string K = std::to_string(k);
+            r = std::make_unique<HipHash64Probe>(std::move(r), "d_sql_join" + K, "(u64)(u32)(" + dims[k].probeCode + ")",
+                                                 "sql_join" + K + "_slots", "_j" + K);
         }
+        if (!cross.empty()) r = std::make_unique<HipSelection>(std::move(r), cross);
         return r;
     };
     auto addAggs = [&](HipKeyedAgg& agg) {
@@ -1310,22 +1418,26 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         semiJson += std::string(k ? "," : "") + "{\"table\":" + jstr(semis[k].table) + ",\"slots\":\"sql_semi" + K +
                     "_slots\",\"anti\":" + (semis[k].anti ? "true" : "false") + "}";
     }
-    if (joined) {
-        // Build side: key -> row index; a repeated key fails the run.
-        auto sc = std::make_unique<HipGridStrideScan>(bc.table, "i");
-        const auto& ct = catalogTable(bc.table);
-        for (const auto& c : bc.cat->columns)
-            if (bc.used.count(c.name)) {
+    std::string joinJson;
+    for (size_t k = 0; k < dims.size(); ++k) {
+        // Each joined table: key -> row index; a repeated key fails the run.
+        const Dim& d = dims[k];
+        const std::string K = std::to_string(k);
+        auto sc = std::make_unique<HipGridStrideScan>(d.table, "i");
+        const auto& ct = catalogTable(d.table);
+        for (const auto& c : d.build.cat->columns)
+            if (d.build.used.count(c.name)) {
                 sc->addColumn(c.name, ct.type(c.name));
                 if (c.type == infra::TpchType::CHAR_FIXED) sc->setLazy(c.name);
             }
         std::unique_ptr<HipOperator> r = std::move(sc);
-        if (!buildWhere.empty()) r = std::make_unique<HipSelection>(std::move(r), buildWhere);
-        auto ht = std::make_unique<HipHash64Build>(std::move(r), "d_sql_join", "(u64)(u32)(" + buildKeyCode + ")", "i",
-                                                   "sql_join_slots");
-        ht->setUniqueKeys("the join column " + buildKey + " repeats a value");
-        appendPhase(plan, "SQL_join_build", std::move(ht), 256, items, {2, 4});
-        plan.requiredSymbols.push_back("sql_join_slots");
+        if (!d.filter.empty()) r = std::make_unique<HipSelection>(std::move(r), d.filter);
+        auto ht = std::make_unique<HipHash64Build>(std::move(r), "d_sql_join" + K, "(u64)(u32)(" + d.keyCode + ")", "i",
+                                                   "sql_join" + K + "_slots");
+        ht->setUniqueKeys("the join column " + d.key + " repeats a value");
+        appendPhase(plan, "SQL_join" + K + "_build", std::move(ht), 256, items, {2, 4});
+        plan.requiredSymbols.push_back("sql_join" + K + "_slots");
+        joinJson += std::string(k ? "," : "") + "{\"table\":" + jstr(d.table) + ",\"slots\":\"sql_join" + K + "_slots\"}";
     }
     if (!pl.grouped) {
         auto agg = std::make_unique<HipKeyedAgg>(scan(pl.cc.used), "d_sql_agg", "0", 1);
@@ -1338,12 +1450,12 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         // Wider keys hash to a 63-bit FNV-1a fingerprint, and a row check
         // compares every row's key values with its group representative's.
         auto isIntColumn = [&](const std::string& name) {
-            const infra::TpchColumn* c = findColumn(pl.cc.cat, name);
-            if (!c) c = findColumn(pl.cc.otherCat, name);
+            const std::string t = ownerOf(name);
+            const infra::TpchColumn* c = t.empty() ? nullptr : findColumn(cats[t], name);
             return c && c->type == infra::TpchType::INT;
         };
         int bits = 0;
-        for (size_t j = 0; j < pl.keys.size(); ++j) {
+        for (size_t j : ind) {
             const Val& v = pl.keyVals[j];
             const bool int32 = v.t == VT::Date || (v.t == VT::Num && v.scale == 0 &&
                                                     (pl.keys[j]->k == NK::Extract ||
@@ -1352,42 +1464,42 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         }
         const bool packed = bits <= 63;
         if (!packed)
-            for (const auto& k : pl.keys) {
+            for (size_t j : ind) {
                 std::set<std::string> cols;
-                columnsOf(k, cols);
+                columnsOf(pl.keys[j], cols);
                 for (const auto& c : cols)
-                    if (findColumn(pl.cc.otherCat, c) && !findColumn(pl.cc.cat, c))
-                        fail("GROUP BY " + text(k) + ": wide keys of the joined table are not supported");
+                    if (ownerOf(c) != root)
+                        fail("GROUP BY " + text(pl.keys[j]) + ": wide keys of a joined table are not supported");
             }
         std::string fp, check;
         if (packed) {
             int shift = 0;
-            for (size_t j = 0; j < pl.keys.size(); ++j) {
+            for (size_t j : ind) {
                 const Val& v = pl.keyVals[j];
+                const std::string sep = fp.empty() ? "" : " | ";
                 if (v.t == VT::Str) {
-                    fp += std::string(j ? " | " : "") + "(gdb_pack_bytes(" + v.code + ", " + std::to_string(v.width) +
-                          ") << " + std::to_string(shift) + ")";
+                    fp += sep + "(gdb_pack_bytes(" + v.code + ", " + std::to_string(v.width) + ") << " +
+                          std::to_string(shift) + ")";
                     shift += 8 * v.width;
                     continue;
                 }
                 const int w = v.t == VT::Char1 ? 8 : 32;
-                fp += std::string(j ? " | " : "") + "((u64)(" + (w == 8 ? "u8" : "u32") + ")(" + v.code + ") << " +
-                      std::to_string(shift) + ")";
+                fp += sep + "((u64)(" + (w == 8 ? "u8" : "u32") + ")(" + v.code + ") << " + std::to_string(shift) + ")";
                 shift += w;
             }
             fp = "(" + fp + ")";
         } else {
             fp = "0xcbf29ce484222325ull";
-            for (size_t j = 0; j < pl.keys.size(); ++j) {
+            for (size_t j : ind) {
                 const Val& v = pl.keyVals[j];
                 const Val r = pl.cc.expr(pl.keys[j], "_g_payload");
+                const std::string sep = check.empty() ? "" : " && ";
                 if (v.t == VT::Str) {
                     fp = "gdb_fnv64(" + fp + ", " + v.code + ", " + std::to_string(v.width) + ")";
-                    check += std::string(j ? " && " : "") + "gdb_bytes_eq(" + v.code + ", " + r.code + ", " +
-                             std::to_string(v.width) + ")";
+                    check += sep + "gdb_bytes_eq(" + v.code + ", " + r.code + ", " + std::to_string(v.width) + ")";
                 } else {
                     fp = "gdb_fnv64_u64(" + fp + ", (u64)(" + v.code + "))";
-                    check += std::string(j ? " && " : "") + "(" + v.code + " == " + r.code + ")";
+                    check += sep + "(" + v.code + " == " + r.code + ")";
                 }
             }
             fp = "(" + fp + " & 0x7fffffffffffffffull)";
@@ -1426,7 +1538,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     std::string rep;
     for (const auto& a : pl.aggs)
         if (a.code == "(i64)_g_payload") rep = a.name;
-    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":" + jstr(pl.cc.other) + ",\"semi\":[" +
+    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":[" + joinJson + "],\"semi\":[" +
                      semiJson + "],\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
                      keysJson + "],\"rep\":" + jstr(rep) + ",\"targets\":[" + targets + "],\"order\":[" + order +
                      "],\"having\":" + having + ",\"limit\":" + std::to_string(q.limit) + "}";

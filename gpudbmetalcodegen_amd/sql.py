@@ -52,19 +52,46 @@ def _text(row) -> str:
     return bytes(row).split(b"\0", 1)[0].decode()
 
 
-def finish(layout: dict, cols: dict, host_table: dict | None) -> tuple[dict, dict]:
+def _value(table: dict, column: str, row: int, k: dict):
+    v = table[column][row]
+    if k["type"] == "str":
+        return _text(v)
+    if k["type"] == "char1":
+        return chr(int(v)) if int(v) else ""
+    if k["type"] == "date":
+        return int(v)
+    return Fraction(int(v), 10 ** k["scale"])
+
+
+def finish(layout: dict, cols: dict, host: dict | None) -> tuple[dict, dict]:
     """Result columns of a SQL plan (the collector's aggregate columns) ->
-    the select list's columns and formats, ordered and limited."""
+    the select list's columns and formats, ordered and limited. host holds
+    the tables ({table: {column: array}}) that string keys and keys fixed by
+    another key are attached from."""
     n = len(cols["_n"]) if "_n" in cols else 0
     keys = layout["keys"]
     rep = layout.get("rep") or ""
+    index: dict = {}  # (table, key column) -> {key value: row}
+
+    def row_of(table: str, key: str, value: int) -> int:
+        ix = index.get((table, key))
+        if ix is None:
+            ix = index[(table, key)] = {int(v): i for i, v in enumerate(host[table][key].tolist())}
+        return ix[int(value)]
 
     def key_value(j: int, r: int):
         k = keys[j]
+        if host is None and ("via" in k or "column" in k):
+            raise ValueError("these GROUP BY keys need the host tables to attach")
+        if "via" in k:  # fixed by key k["via"]: walk its row's foreign keys
+            v = key_value(k["via"], r)
+            path = k["path"]
+            row = row_of(path[0]["table"], path[0]["key"], int(v))
+            for prev, step in zip(path, path[1:]):
+                row = row_of(step["table"], step["key"], int(host[prev["table"]][step["fk"]][row]))
+            return _value(host[path[-1]["table"]], k["column"], row, k)
         if k["type"] == "str" and "column" in k:
-            if host_table is None:
-                raise ValueError("string GROUP BY keys need the host table to attach")
-            return _text(host_table[k["column"]][int(cols[rep][r])])
+            return _text(host[layout["table"]][k["column"]][int(cols[rep][r])])
         v = int(cols[k["agg"]][r])
         if k["type"] == "str":  # a short string, its bytes packed (byte 0 lowest)
             return v.to_bytes(8, "little")[:k["width"]].split(b"\0", 1)[0].decode()
@@ -174,13 +201,12 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
             ex.register_table(db.device(sm["table"]))
             srows = int(next(iter(db.host[sm["table"]].values())).shape[0])
             ex.register_symbol(sm["slots"], T._pow2(2 * max(srows, 1)))
-        join = layout.get("join") or ""
-        if join:  # the key side of a two-table join
-            if join not in db.host:
-                raise ValueError(f"table {join} is not loaded")
-            ex.register_table(db.device(join))
-            jrows = int(next(iter(db.host[join].values())).shape[0])
-            ex.register_symbol("sql_join_slots", T._pow2(2 * max(jrows, 1)))
+        for jn in layout.get("join", []):  # the tables joined by their key
+            if jn["table"] not in db.host:
+                raise ValueError(f"table {jn['table']} is not loaded")
+            ex.register_table(db.device(jn["table"]))
+            jrows = int(next(iter(db.host[jn["table"]].values())).shape[0])
+            ex.register_symbol(jn["slots"], T._pow2(2 * max(jrows, 1)))
         try:
             t = ex.execute(plan, warmup, measured)
         except N.GdbOverflow:
@@ -197,7 +223,7 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
         raise RuntimeError("SQL: group capacity did not converge")
     cols = {k: list(v) for k, v in res.columns.items()}
     cols["__scales"] = {k: f[0] for k, f in res.formats.items()}
-    out, fmts = finish(layout, cols, db.host.get(table))
+    out, fmts = finish(layout, cols, db.host)
     run = SqlRun(sql, out, fmts, t, plan, layout)
     run.canonical = T.canonical(out, fmts)
     return run

@@ -83,13 +83,13 @@ def test_quotes_and_escapes_in_literals():
 def test_join_plans_split_conjuncts():
     p = E.build_sql_plan((SQL_DIR / "tpch_q12.sql").read_text())
     d = p.describe()
-    assert p.phases() == ["SQL_join_build", "SQL_groups", "SQL_aggregate"]
+    assert p.phases() == ["SQL_join0_build", "SQL_groups", "SQL_aggregate"]
     assert "GridStrideScan(orders: o_orderkey/i32)" in d          # build side: the key (and its filters)
     assert "BindColumns(orders: o_orderpriority)" in d              # read at the matched row
     assert '"MA", 2' in d and '"SH", 2' in d                          # literals cut to the 2-byte catalog width
     assert "(49ll)" in p.source()                                     # '1-URGENT' by its first byte ('1' = 49)
     lay = p.sql_layout()
-    assert lay["table"] == "lineitem" and lay["join"] == "orders"
+    assert lay["table"] == "lineitem" and lay["join"] == [{"table": "orders", "slots": "sql_join0_slots"}]
     q = E.build_sql_plan("select count(*) from lineitem, part where p_partkey = l_partkey and p_size > 10 "
                          "and l_quantity < 5 and (p_size < 20 or l_tax > 0.02)")
     d = q.describe()
@@ -106,13 +106,18 @@ def test_case_when_aligns_scales():
 
 @pytest.mark.parametrize("sql,msg", [
     ("select * from lineitem", "SELECT *"),
-    ("select sum(l_quantity) from lineitem, orders, part where l_orderkey = o_orderkey", "two joined"),
+    ("select sum(l_quantity) from lineitem, orders, part where l_orderkey = o_orderkey",
+     "without a join condition"),
+    ("select count(*) from customer, orders, nation, supplier where c_custkey = o_custkey "
+     "and c_nationkey = n_nationkey and s_nationkey = n_nationkey", "joined by its key twice"),
     ("select count(*) from lineitem, orders where l_quantity < 3", "no cross products"),
     ("select count(*) from lineitem, partsupp where l_partkey = ps_partkey", "neither is its table's key"),
     ("select count(*) from lineitem, orders where l_orderkey = o_orderkey and l_suppkey = o_custkey",
-     "more than one join condition"),
+     "neither is its table's key"),
+    ("select count(*) from lineitem, orders where l_orderkey = o_orderkey and l_partkey = o_orderkey",
+     "joined by its key twice"),
     ("select p_type, count(*) from lineitem, part where l_partkey = p_partkey group by p_type",
-     "wide keys of the joined table"),
+     "a string of a joined table"),
     ("select l_quantity from lineitem", "must be an aggregate or appear in GROUP BY"),
     ("select sum(l_quantity / 2) from lineitem", "division of row values"),
     ("select sum(nope) from lineitem", "no column nope"),
@@ -173,7 +178,7 @@ def test_finish_string_keys_from_representative_rows():
                           {"name": "c", "expr": {"agg": "_n"}, "scale": 0, "type": "num"}],
               "order": [], "limit": -1}
     cols = {"_n": [5, 7], "_rep2": [1, 0], "__scales": {}}
-    out, _ = S.finish(layout, cols, host)
+    out, _ = S.finish(dict(layout, table="part"), cols, {"part": host})
     assert out == {"p_brand": ["Brand#11", "Brand#22"], "c": [7, 5]}
 
 

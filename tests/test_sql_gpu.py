@@ -1,8 +1,8 @@
 """SQL front end on the GPU: SELECT text -> engine parser -> HIP plan ->
 exact rows, on generated TPC-H tables (SF 0.01).
 
-TPC-H Q1, Q4, Q6, Q12 and Q14 in their SQL text are compared bit-exactly
-with the oracle (oracle/tpch.py, itself pinned against SQLite). The other queries, written in
+TPC-H Q1, Q3, Q4, Q6, Q10, Q12 and Q14 in their SQL text are compared
+bit-exactly with the oracle (oracle/tpch.py, itself pinned against SQLite). The other queries, written in
 the dialect both engines read, are compared with SQLite running the same text
 on the same rows: strings, dates and counts exactly, decimals within 1e-9
 relative (SQLite sums REALs; the engine's values are exact fixed point).
@@ -73,6 +73,17 @@ def test_tpch_q12_sql_text_matches_oracle(env):
     assert [full[m] for m in r.columns["l_shipmode"]] == exp["l_shipmode"]
     assert r.columns["high_line_count"] == exp["high_line_count"]
     assert r.columns["low_line_count"] == exp["low_line_count"]
+
+
+@pytest.mark.parametrize("q", ["Q3", "Q10"])
+def test_tpch_key_join_tree_sql_text_matches_oracle(env, q):
+    ctx, db, host = env
+    print(f"SQL text of TPC-H {q}", flush=True)
+    r = S.run_sql(ctx, db, (SQL_DIR / f"tpch_{q.lower()}.sql").read_text())
+    exp, fm = OT.run(host, q, T.PARAMS[q])
+    assert len(exp[next(iter(exp))]) > 0
+    assert r.columns == exp
+    assert r.formats == fm
 
 
 def test_tpch_q4_sql_text_matches_oracle(env):
