@@ -1,7 +1,7 @@
 """SQL text -> GPU plan -> exact result rows.
 
-``run_sql`` plans a SELECT over one table, or two joined on a key column,
-from its text (``gdb_plan_build_sql``:
+``run_sql`` plans a SELECT over TPC-H tables joined in a tree of key
+equalities from its text (``gdb_plan_build_sql``:
 the engine's own parser and fixed-point compiler, the counterpart of the
 reference's analyzeSQL + analysis-based plan builders,
 ``codegen/planning/query_analyzer.cpp:640-727``), runs it on the TPC-H tables

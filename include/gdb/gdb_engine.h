@@ -51,9 +51,9 @@ gdb_status gdb_gen_text(uint64_t seed, uint64_t stream, uint64_t n, int width, c
  * "key=value,..." and run the emitters. Replaces buildMetalPlan +
  * generateFromPlan (codegen/planning/metal_plan_builder.h:73-78). */
 gdb_status gdb_plan_build(const char* name, const char* options, gdb_plan* out);
-/* Build a plan from SQL text: a SELECT over one TPC-H table, or two joined
- * by an equality on one table's key, [NOT] EXISTS / [NOT] IN subqueries as
- * semi- / anti-joins, with
+/* Build a plan from SQL text: a SELECT over TPC-H tables joined in a tree
+ * of key equalities, [NOT] EXISTS / [NOT] IN subqueries as semi- /
+ * anti-joins, with
  * WHERE, GROUP BY, SUM / AVG / MIN / MAX / COUNT, HAVING, ORDER BY, LIMIT, compiled
  * to exact fixed-point HIP expressions. Replaces analyzeSQL + the
  * analysis-based buildMetalPlan dispatch (codegen/planning/query_analyzer.h:80-85,
