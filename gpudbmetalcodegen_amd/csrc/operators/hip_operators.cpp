@@ -953,12 +953,14 @@ std::string HipDenseGroupId::describe() const {
     return "DenseGroupId(" + ids_ + "[" + slotVar_ + "] = next id + 1, cap " + s(cap_) + ")";
 }
 
-HipDirectGroupId::HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr, long long size,
-                                   const std::string& idArray, const std::string& status, long long cap)
-    : HipUnaryOperator(std::move(child)), key_(keyExpr), ids_(idArray), status_(status), size_(size), cap_(cap) {}
+HipDirectGroupId::HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr,
+                                   const std::string& sizeExpr, const std::string& idArray, const std::string& status,
+                                   long long cap)
+    : HipUnaryOperator(std::move(child)), key_(keyExpr), size_(sizeExpr), ids_(idArray), status_(status), cap_(cap) {}
 
 void HipDirectGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    cg.addBufferParam(ids_, "u32", 4, s(size_), InitKind::Zero, {}, false, true);
+    cg.addBufferParam(ids_, "u32", 4, size_, InitKind::Zero, {}, false, true);
+    cg.addSymbolParam("nids_" + ids_, size_);
     cg.addBufferParam(status_, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(status_);
     cg.declareLds("u64", "_seen_" + ids_, "512");
@@ -967,7 +969,7 @@ void HipDirectGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     child_->produce(cg, [&]() {
         cg.addBlock("", [&]() {
             cg.addLine("const u64 _dk = (u64)(" + key_ + ");");
-            cg.addBlock("if (_dk < " + s(size_) + "ull && gdb_block_first(_seen_" + ids_ + ", _dk) && " + ids_ +
+            cg.addBlock("if (_dk < nids_" + ids_ + " && gdb_block_first(_seen_" + ids_ + ", _dk) && " + ids_ +
                             "[_dk] == 0u)",
                         [&]() {
                             cg.addLine("u32 _exp = 0u;");
@@ -986,6 +988,42 @@ void HipDirectGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         });
         consume();
     });
+}
+
+HipDenseUniqueStore::HipDenseUniqueStore(std::unique_ptr<HipOperator> child, const std::string& arrayName,
+                                         const std::string& keyExpr, const std::string& sizeExpr,
+                                         const std::string& valueExpr, const std::string& message)
+    : HipUnaryOperator(std::move(child)),
+      array_(arrayName),
+      key_(keyExpr),
+      size_(sizeExpr),
+      value_(valueExpr),
+      message_(message) {}
+
+void HipDenseUniqueStore::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(array_, "u32", 4, size_, InitKind::Zero, {}, false, true);
+    cg.addSymbolParam("nelem_" + array_, size_);
+    const std::string ck = array_ + "_unique";
+    cg.addBufferParam(ck, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseCheck(ck, message_);
+    const std::string dup = "_dup_" + array_;
+    cg.addLine("u32 " + dup + " = 0;");
+    child_->produce(cg, [&]() {
+        cg.addBlock("", [&]() {
+            cg.addLine("const u64 _sk = (u64)(i64)(" + key_ + ");");
+            cg.addBlock("if (_sk < nelem_" + array_ + ")", [&]() {
+                cg.addLine("u32 _e = 0u;");
+                cg.addLine(dup + " += __hip_atomic_compare_exchange_strong(" + array_ + " + _sk, &_e, (u32)(" + value_ +
+                           "), __ATOMIC_RELAXED, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ? 0u : 1u;");
+            });
+        });
+        consume();
+    });
+    emitBlockStatus(cg, ck, {{dup, 2, false}, {dup, 0, true}});
+}
+
+std::string HipDenseUniqueStore::describe() const {
+    return "DenseUniqueStore(" + array_ + "[" + key_ + "] = " + value_ + ")";
 }
 
 std::string HipDirectGroupId::describe() const {

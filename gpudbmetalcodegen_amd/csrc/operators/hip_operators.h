@@ -270,14 +270,31 @@ private:
 // mark (~0) and flag word 0, so the caller rebuilds with the exact count.
 class HipDirectGroupId : public HipUnaryOperator {
 public:
-    HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr, long long size,
+    // sizeExpr: the key domain (a constant, or a symbol such as a column's
+    // largest value + 1).
+    HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr, const std::string& sizeExpr,
                      const std::string& idArray, const std::string& status, long long cap);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
-    std::string key_, ids_, status_;
-    long long size_, cap_;
+    std::string key_, size_, ids_, status_;
+    long long cap_;
+};
+
+// Key-indexed u32 store array[key] = value (value != 0) for keys that must
+// be unique (a join's key side): a compare-and-swap from 0 claims the
+// entry, and a key seen twice fails the run with `message` after the phase.
+// Keys outside [0, size) are skipped.
+class HipDenseUniqueStore : public HipUnaryOperator {
+public:
+    HipDenseUniqueStore(std::unique_ptr<HipOperator> child, const std::string& arrayName, const std::string& keyExpr,
+                        const std::string& sizeExpr, const std::string& valueExpr, const std::string& message);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::string array_, key_, size_, value_, message_;
 };
 
 // Row check: counts the rows for which `condition` is false and fails the

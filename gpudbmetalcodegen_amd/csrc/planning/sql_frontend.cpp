@@ -1170,7 +1170,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         Joined j;
         j.table = dims[k].table;
         j.cat = cats[dims[k].table];
-        j.row = "_j" + std::to_string(k) + "_payload";
+        j.row = "(_j" + std::to_string(k) + " - 1)";  // the key-indexed array holds row + 1
         pl.cc.joins.push_back(j);
         dims[k].build.table = j.table;
         dims[k].build.cat = j.cat;
@@ -1374,18 +1374,16 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
         std::unique_ptr<HipOperator> r = std::move(sc);
         if (!where.empty()) r = std::make_unique<HipSelection>(std::move(r), where);
         for (size_t k = 0; k < semis.size(); ++k)
-            r = std::make_unique<HipHash64Probe>(std::move(r), "d_sql_semi" + std::to_string(k),
-                                                 "(u64)(u32)(" + semiOuter[k] + ")",
-                                                 "sql_semi" + std::to_string(k) + "_slots", "_s" + std::to_string(k),
-                                                 semis[k].anti);
+            r = std::make_unique<HipByteFlagProbe>(std::move(r), "d_sql_semi" + std::to_string(k), semiOuter[k],
+                                                   "sql_semi" + std::to_string(k) + "_keys", semis[k].anti);
         for (size_t k = 0; k < dims.size(); ++k) {  // parents before children
             const Joined& j = pl.cc.joins[k];
             std::vector<std::pair<std::string, ColumnType>> bind;
             for (const auto& c : j.used) bind.push_back({c, catalogTable(j.table).type(c)});
             if (!bind.empty()) r = std::make_unique<HipBindColumns>(std::move(r), j.table, bind);
             const std::string K = std::to_string(k);
-            r = std::make_unique<HipHash64Probe>(std::move(r), "d_sql_join" + K, "(u64)(u32)(" + dims[k].probeCode + ")",
-                                                 "sql_join" + K + "_slots", "_j" + K);
+            r = std::make_unique<HipArrayLookup>(std::move(r), "d_sql_join" + K, dims[k].probeCode,
+                                                 "sql_join" + K + "_keys", "_j" + K, 0ll, "u32");
         }
         if (!cross.empty()) r = std::make_unique<HipSelection>(std::move(r), cross);
         return r;
@@ -1399,7 +1397,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     const int items = optInt(opts, "items", 4);
     std::string semiJson;
     for (size_t k = 0; k < semis.size(); ++k) {
-        // The subquery's key set (first arrivals only: a key repeats freely).
+        // The subquery's key set.
         const std::string K = std::to_string(k);
         auto sc = std::make_unique<HipGridStrideScan>(semis[k].table, "i");
         const auto& ct = catalogTable(semis[k].table);
@@ -1410,17 +1408,19 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             }
         std::unique_ptr<HipOperator> r = std::move(sc);
         if (!semiFilter[k].empty()) r = std::make_unique<HipSelection>(std::move(r), semiFilter[k]);
-        auto ht = std::make_unique<HipHash64Build>(std::move(r), "d_sql_semi" + K, "(u64)(u32)(" + semiInner[k] + ")",
-                                                   "0", "sql_semi" + K + "_slots", true);
-        ht->setBlockFirst();
-        appendPhase(plan, "SQL_semi" + K + "_build", std::move(ht), 256, items, {2, 4});
-        plan.requiredSymbols.push_back("sql_semi" + K + "_slots");
-        semiJson += std::string(k ? "," : "") + "{\"table\":" + jstr(semis[k].table) + ",\"slots\":\"sql_semi" + K +
-                    "_slots\",\"anti\":" + (semis[k].anti ? "true" : "false") + "}";
+        // key-indexed byte flags over the key column's domain (its largest value + 1)
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(r), "d_sql_semi" + K, semiInner[k], "sql_semi" + K + "_keys");
+        appendPhase(plan, "SQL_semi" + K + "_build", std::move(fl), 256, items, {2, 4});
+        plan.requiredSymbols.push_back("sql_semi" + K + "_keys");
+        semiJson += std::string(k ? "," : "") + "{\"table\":" + jstr(semis[k].table) + ",\"keys\":\"sql_semi" + K +
+                    "_keys\",\"column\":" + jstr(semis[k].innerKey) + ",\"anti\":" + (semis[k].anti ? "true" : "false") +
+                    "}";
     }
-    std::string joinJson;
+    std::string joinJson, domainJson = "null";
     for (size_t k = 0; k < dims.size(); ++k) {
-        // Each joined table: key -> row index; a repeated key fails the run.
+        // Each joined table: a key-indexed array of row + 1 over the key's
+        // domain (its largest value + 1, the reference's max-key symbols); a
+        // repeated key fails the run.
         const Dim& d = dims[k];
         const std::string K = std::to_string(k);
         auto sc = std::make_unique<HipGridStrideScan>(d.table, "i");
@@ -1432,12 +1432,12 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             }
         std::unique_ptr<HipOperator> r = std::move(sc);
         if (!d.filter.empty()) r = std::make_unique<HipSelection>(std::move(r), d.filter);
-        auto ht = std::make_unique<HipHash64Build>(std::move(r), "d_sql_join" + K, "(u64)(u32)(" + d.keyCode + ")", "i",
-                                                   "sql_join" + K + "_slots");
-        ht->setUniqueKeys("the join column " + d.key + " repeats a value");
-        appendPhase(plan, "SQL_join" + K + "_build", std::move(ht), 256, items, {2, 4});
-        plan.requiredSymbols.push_back("sql_join" + K + "_slots");
-        joinJson += std::string(k ? "," : "") + "{\"table\":" + jstr(d.table) + ",\"slots\":\"sql_join" + K + "_slots\"}";
+        auto st = std::make_unique<HipDenseUniqueStore>(std::move(r), "d_sql_join" + K, d.keyCode, "sql_join" + K + "_keys",
+                                                        "i + 1", "the join column " + d.key + " repeats a value");
+        appendPhase(plan, "SQL_join" + K + "_build", std::move(st), 256, items, {2, 4});
+        plan.requiredSymbols.push_back("sql_join" + K + "_keys");
+        joinJson += std::string(k ? "," : "") + "{\"table\":" + jstr(d.table) + ",\"keys\":\"sql_join" + K +
+                    "_keys\",\"column\":" + jstr(d.key) + "}";
     }
     if (!pl.grouped) {
         auto agg = std::make_unique<HipKeyedAgg>(scan(pl.cc.used), "d_sql_agg", "0", 1);
@@ -1504,15 +1504,24 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             }
             fp = "(" + fp + " & 0x7fffffffffffffffull)";
         }
-        if (packed && bits <= 16) {
-            // Direct-addressed key domain (e.g. two char(1) flags): one u32
-            // id per possible key, no hashing.
-            const long long size = 1LL << bits;
+        // One INT column as the whole key: its domain (largest value + 1,
+        // registered by the caller as sql_key_domain) is direct-addressed.
+        const bool intKey = ind.size() == 1 && pl.keys[ind[0]]->k == NK::Col && isIntColumn(pl.keys[ind[0]]->s);
+        if (intKey) {
+            domainJson = "{\"table\":" + jstr(ownerOf(pl.keys[ind[0]]->s)) + ",\"column\":" +
+                         jstr(pl.keys[ind[0]]->s) + "}";
+            fp = "(" + pl.keyVals[ind[0]].code + ")";
+        }
+        if ((packed && bits <= 16) || intKey) {
+            // Direct-addressed key domain (two char(1) flags, or one INT
+            // column): one u32 id per possible key, no hashing.
+            const std::string size = intKey ? std::string("sql_key_domain") : std::to_string(1LL << bits);
+            if (intKey) plan.requiredSymbols.push_back("sql_key_domain");
             auto id = std::make_unique<HipDirectGroupId>(scan(keyCols), fp, size, "d_sql_gid", "d_sql_groups_status",
                                                          cap);
             appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4});
-            std::unique_ptr<HipOperator> cur = std::make_unique<HipArrayLookup>(
-                scan(pl.cc.used), "d_sql_gid", fp, std::to_string(size), "_gid", 0ll, "u32");
+            std::unique_ptr<HipOperator> cur =
+                std::make_unique<HipArrayLookup>(scan(pl.cc.used), "d_sql_gid", fp, size, "_gid", 0ll, "u32");
             cur = std::make_unique<HipSelection>(std::move(cur), "_gid <= " + std::to_string(cap) + "ll");
             auto agg = std::make_unique<HipKeyedAgg>(std::move(cur), "d_sql_agg", "_gid - 1", static_cast<int>(cap));
             addAggs(*agg);
@@ -1538,7 +1547,8 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     std::string rep;
     for (const auto& a : pl.aggs)
         if (a.code == "(i64)_g_payload") rep = a.name;
-    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"join\":[" + joinJson + "],\"semi\":[" +
+    plan.sqlLayout = "{\"table\":" + jstr(pl.cc.table) + ",\"domain\":" + domainJson + ",\"join\":[" + joinJson +
+                     "],\"semi\":[" +
                      semiJson + "],\"groups\":" + std::to_string(cap) + ",\"keys\":[" +
                      keysJson + "],\"rep\":" + jstr(rep) + ",\"targets\":[" + targets + "],\"order\":[" + order +
                      "],\"having\":" + having + ",\"limit\":" + std::to_string(q.limit) + "}";

@@ -195,18 +195,19 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
         ex = executor or E.HipGenericExecutor(ctx)
         ex.register_table(db.device(table))
         ex.register_symbol("sql_slots", T._pow2(2 * max(rows, 1)))
-        for sm in layout.get("semi", []):  # the key sets of [NOT] EXISTS / IN subqueries
-            if sm["table"] not in db.host:
-                raise ValueError(f"table {sm['table']} is not loaded")
-            ex.register_table(db.device(sm["table"]))
-            srows = int(next(iter(db.host[sm["table"]].values())).shape[0])
-            ex.register_symbol(sm["slots"], T._pow2(2 * max(srows, 1)))
-        for jn in layout.get("join", []):  # the tables joined by their key
-            if jn["table"] not in db.host:
-                raise ValueError(f"table {jn['table']} is not loaded")
-            ex.register_table(db.device(jn["table"]))
-            jrows = int(next(iter(db.host[jn["table"]].values())).shape[0])
-            ex.register_symbol(jn["slots"], T._pow2(2 * max(jrows, 1)))
+        # Tables of the joins and subqueries, and the key domains (largest
+        # value + 1) their key-indexed arrays are sized by.
+        doms = [(e["table"], e["column"], e["keys"]) for e in layout.get("join", []) + layout.get("semi", [])]
+        if layout.get("domain"):
+            doms.append((layout["domain"]["table"], layout["domain"]["column"], "sql_key_domain"))
+        for tab, col, sym in doms:
+            if tab not in db.host:
+                raise ValueError(f"table {tab} is not loaded")
+            ex.register_table(db.device(tab))
+            v = db.host[tab][col]
+            if v.shape[0] and int(v.min()) < 0:
+                raise ValueError(f"{col}: negative key values")
+            ex.register_symbol(sym, int(v.max(initial=0)) + 1)
         try:
             t = ex.execute(plan, warmup, measured)
         except N.GdbOverflow:

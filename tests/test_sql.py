@@ -89,7 +89,9 @@ def test_join_plans_split_conjuncts():
     assert '"MA", 2' in d and '"SH", 2' in d                          # literals cut to the 2-byte catalog width
     assert "(49ll)" in p.source()                                     # '1-URGENT' by its first byte ('1' = 49)
     lay = p.sql_layout()
-    assert lay["table"] == "lineitem" and lay["join"] == [{"table": "orders", "slots": "sql_join0_slots"}]
+    assert lay["table"] == "lineitem"
+    assert lay["join"] == [{"table": "orders", "keys": "sql_join0_keys", "column": "o_orderkey"}]
+    assert "DenseUniqueStore(d_sql_join0[(i64)o_orderkey] = i + 1)" in d   # row + 1 by key, repeats fail
     q = E.build_sql_plan("select count(*) from lineitem, part where p_partkey = l_partkey and p_size > 10 "
                          "and l_quantity < 5 and (p_size < 20 or l_tax > 0.02)")
     d = q.describe()
@@ -208,11 +210,22 @@ def test_exists_and_not_in_become_semi_and_anti_joins():
     p = E.build_sql_plan((SQL_DIR / "tpch_q4.sql").read_text())
     assert p.phases() == ["SQL_semi0_build", "SQL_groups", "SQL_aggregate"]
     d = p.describe()
-    assert "Hash64BuildDistinct(d_sql_semi0, key=(u64)(u32)((i64)l_orderkey)" in d
+    assert "ByteFlagBuild(d_sql_semi0[(i64)l_orderkey] = 1)" in d
     assert "Selection(((i64)l_commitdate < (i64)l_receiptdate))" in d   # the subquery's own condition
-    assert p.sql_layout()["semi"] == [{"table": "lineitem", "slots": "sql_semi0_slots", "anti": False}]
+    assert p.sql_layout()["semi"] == [{"table": "lineitem", "keys": "sql_semi0_keys", "column": "l_orderkey",
+                                       "anti": False}]
     q = E.build_sql_plan("select count(*) from customer where c_custkey not in (select o_custkey from orders)")
-    assert "Hash64AntiProbe" in q.describe() and q.sql_layout()["semi"][0]["anti"] is True
+    assert "AntiByteFlagProbe" in q.describe() and q.sql_layout()["semi"][0]["anti"] is True
     r = E.build_sql_plan("select count(*) from customer where not exists "
                          "(select * from orders where o_custkey = c_custkey)")
-    assert "Hash64AntiProbe" in r.describe()
+    assert "AntiByteFlagProbe" in r.describe()
+
+
+def test_single_int_key_is_direct_addressed():
+    p = E.build_sql_plan((SQL_DIR / "tpch_q10.sql").read_text())
+    lay = p.sql_layout()
+    assert lay["domain"] == {"table": "customer", "column": "c_custkey"}
+    assert "Hash64" not in p.describe()              # joins and groups all key-indexed
+    assert [k.get("via") for k in lay["keys"]] == [None, 0, 0, 0, 0, 0, 0]
+    assert lay["keys"][4]["path"] == [{"table": "customer", "key": "c_custkey"},
+                                      {"table": "nation", "key": "n_nationkey", "fk": "c_nationkey"}]
