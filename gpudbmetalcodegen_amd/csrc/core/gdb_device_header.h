@@ -313,13 +313,13 @@ GDB_DEV int gdb_ht64_insert_at(u64* __restrict__ tab, u32 bits, u64 key, u64 pay
 }
 // Block-level first-arrival filter for a distinct insert: an LDS set (512
 // slots, ~0 = empty) of the keys this block has already sent to the global
-// table. True when this lane is the block's first with the key, or when 8
-// probes find no room (many distinct keys): only those lanes touch the
-// global table, so a few hot keys cost a few global operations per block
-// instead of one per row.
+// table. True when this lane is the block's first with the key, or when 2
+// probes find no room (many distinct keys: the filter then costs two LDS
+// reads a row): only those lanes touch the global table, so a few hot keys
+// cost a few global operations per block instead of one per row.
 GDB_DEV bool gdb_block_first(u64* seen, u64 key) {
     u32 h = (u32)((key * 0x9E3779B97F4A7C15ull) >> 55);
-    for (int p = 0; p < 8; ++p, h = (h + 1) & 511u) {
+    for (int p = 0; p < 2; ++p, h = (h + 1) & 511u) {
         const u64 cur = __hip_atomic_load(seen + h, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         if (cur == key) return false;
         if (cur == ~0ull) {

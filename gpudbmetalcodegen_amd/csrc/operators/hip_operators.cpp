@@ -941,7 +941,7 @@ void HipDenseGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(ids_, "u32", 4, slots_, InitKind::Zero, {}, false, true);
     child_->produce(cg, [&]() {
         cg.addBlock("", [&]() {
-            cg.addLine("const u64 _id = gdb_atomic_add_u64_ret(" + st + " + 3, 1ull);");
+            cg.addLine("const u64 _id = gdb_wave_append(" + st + " + 3);  // one counter atomic per wave");
             cg.addLine("if (_id < " + s(cap_) + "ull) " + ids_ + "[(u64)" + slotVar_ + "] = (u32)(_id + 1);");
             cg.addLine("else gdb_atomic_max_u64(" + st + ", 1ull);");
         });
@@ -955,30 +955,40 @@ std::string HipDenseGroupId::describe() const {
 
 HipDirectGroupId::HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr,
                                    const std::string& sizeExpr, const std::string& idArray, const std::string& status,
-                                   long long cap)
-    : HipUnaryOperator(std::move(child)), key_(keyExpr), size_(sizeExpr), ids_(idArray), status_(status), cap_(cap) {}
+                                   long long cap, bool blockFirst)
+    : HipUnaryOperator(std::move(child)),
+      key_(keyExpr),
+      size_(sizeExpr),
+      ids_(idArray),
+      status_(status),
+      cap_(cap),
+      blockFirst_(blockFirst) {}
 
 void HipDirectGroupId::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(ids_, "u32", 4, size_, InitKind::Zero, {}, false, true);
     cg.addSymbolParam("nids_" + ids_, size_);
     cg.addBufferParam(status_, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(status_);
-    cg.declareLds("u64", "_seen_" + ids_, "512");
-    cg.addLine("for (int _k = (int)threadIdx.x; _k < 512; _k += (int)blockDim.x) _seen_" + ids_ + "[_k] = ~0ull;");
-    cg.addLine("__syncthreads();");
+    if (blockFirst_) {
+        cg.declareLds("u64", "_seen_" + ids_, "512");
+        cg.addLine("for (int _k = (int)threadIdx.x; _k < 512; _k += (int)blockDim.x) _seen_" + ids_ + "[_k] = ~0ull;");
+        cg.addLine("__syncthreads();");
+    }
     child_->produce(cg, [&]() {
         cg.addBlock("", [&]() {
             cg.addLine("const u64 _dk = (u64)(" + key_ + ");");
-            cg.addBlock("if (_dk < nids_" + ids_ + " && gdb_block_first(_seen_" + ids_ + ", _dk) && " + ids_ +
-                            "[_dk] == 0u)",
+            // the id is read past the CU cache: a stale zero there would send
+            // every later row of the group to the compare-and-swap
+            cg.addBlock("if (_dk < nids_" + ids_ + (blockFirst_ ? " && gdb_block_first(_seen_" + ids_ + ", _dk)" : "") +
+                            " && __hip_atomic_load(" + ids_ + " + _dk, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u)",
                         [&]() {
                             cg.addLine("u32 _exp = 0u;");
                             cg.addBlock("if (__hip_atomic_compare_exchange_strong(" + ids_ +
                                             " + _dk, &_exp, 0xFFFFFFFFu, __ATOMIC_RELAXED, __ATOMIC_RELAXED, "
                                             "__HIP_MEMORY_SCOPE_AGENT))",
                                         [&]() {
-                                            cg.addLine("const u64 _id = gdb_atomic_add_u64_ret(" + status_ +
-                                                       " + 3, 1ull);");
+                                            cg.addLine("const u64 _id = gdb_wave_append(" + status_ +
+                                                       " + 3);  // one counter atomic per wave");
                                             cg.addLine("if (_id < " + s(cap_) + "ull) __hip_atomic_store(" + ids_ +
                                                        " + _dk, (u32)(_id + 1), __ATOMIC_RELAXED, "
                                                        "__HIP_MEMORY_SCOPE_AGENT);");

@@ -272,14 +272,17 @@ class HipDirectGroupId : public HipUnaryOperator {
 public:
     // sizeExpr: the key domain (a constant, or a symbol such as a column's
     // largest value + 1).
+    // blockFirst: the LDS first-arrival filter (few hot keys); without it an
+    // existing group costs one plain read of its id.
     HipDirectGroupId(std::unique_ptr<HipOperator> child, const std::string& keyExpr, const std::string& sizeExpr,
-                     const std::string& idArray, const std::string& status, long long cap);
+                     const std::string& idArray, const std::string& status, long long cap, bool blockFirst = true);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string key_, size_, ids_, status_;
     long long cap_;
+    bool blockFirst_;
 };
 
 // Key-indexed u32 store array[key] = value (value != 0) for keys that must

@@ -1518,7 +1518,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             const std::string size = intKey ? std::string("sql_key_domain") : std::to_string(1LL << bits);
             if (intKey) plan.requiredSymbols.push_back("sql_key_domain");
             auto id = std::make_unique<HipDirectGroupId>(scan(keyCols), fp, size, "d_sql_gid", "d_sql_groups_status",
-                                                         cap);
+                                                         cap, /*blockFirst=*/!intKey);
             appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4});
             std::unique_ptr<HipOperator> cur =
                 std::make_unique<HipArrayLookup>(scan(pl.cc.used), "d_sql_gid", fp, size, "_gid", 0ll, "u32");
