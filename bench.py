@@ -70,6 +70,11 @@ def roofline(phase: str, phase_ms: float, alg_bytes: float, workload: str) -> di
 # Workload 1 — three-way join (headline)
 # ---------------------------------------------------------------------------
 def q3_phase_bytes(T: dict, seg: int, d: int, b_slots: int) -> dict:
+    """Algorithmic bytes per phase: every column the query reads, once, plus
+    the bitmap words, hash slots and slot updates its filters reach. F counts
+    20 bytes a row (key, price, discount, date) although the probe reads price
+    and discount only for the rows that pass both filters; the HBM traffic
+    counters show the bytes it actually moves."""
     A, B, F = T["A"], T["B"], T["F"]
     a_sel = A["a_seg"] == seg
     aset = np.zeros(int(A["a_key"].max()) + 2, bool)

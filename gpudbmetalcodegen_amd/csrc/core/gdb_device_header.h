@@ -125,6 +125,13 @@ __host__ __device__ constexpr u32 gdb_lds_align16(u32 x) { return (x + 15u) & ~1
 __host__ __device__ constexpr u32 gdb_cmax(u32 a, u32 b) { return a > b ? a : b; }
 GDB_DEV int gdb_lane() { return (int)(threadIdx.x & 63u); }
 GDB_DEV int gdb_wave() { return (int)(threadIdx.x >> 6); }
+GDB_DEV u64 gdb_lanemask_lt() { return (1ull << gdb_lane()) - 1ull; }
+// Orders one wave's LDS stores before its lanes read each other's entries.
+GDB_DEV void gdb_wave_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // ---------------------------------------------------------------------------
 // Atomics (relaxed). Global ones are device (agent) scope, LDS ones are

@@ -382,6 +382,61 @@ void HipComputeExpr::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 std::string HipComputeExpr::describe() const { return "ComputeExpr(" + varName_ + " = " + expression_ + ")"; }
 
 // ===========================================================================
+// HipWaveCompact
+// ===========================================================================
+
+HipWaveCompact::HipWaveCompact(std::unique_ptr<HipOperator> child, std::vector<Var> vars)
+    : HipUnaryOperator(std::move(child)), vars_(std::move(vars)) {
+    if (vars_.empty()) throw std::runtime_error("HipWaveCompact: no values to queue");
+}
+
+void HipWaveCompact::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    if (!cg.phaseStaged()) throw std::runtime_error("HipWaveCompact: the phase must be staged");
+    // one queue region of 64 x ITEMS entries per wave the launch bound allows
+    const int lb = cg.getPhases().back().launchBound;
+    const std::string cap = s((lb + 63) / 64 * 64) + " * ITEMS";
+    std::vector<std::string> q;
+    for (const auto& v : vars_) {
+        q.push_back(cg.uniqueName("_wq"));
+        cg.declareLds(v.type, q.back(), cap);
+    }
+    child_->produce(cg, [&]() {
+        if (!cg.staged()) throw std::runtime_error("HipWaveCompact: the child must hand over a staged batch");
+        const std::string N = cg.uniqueName("_wn"), B = cg.uniqueName("_wb");
+        cg.addLine("const u32 " + B + " = (u32)gdb_wave() * 64u * ITEMS;");
+        cg.addLine("u32 " + N + " = 0;");
+        cg.addComment("append the batch's active rows to this wave's queue (slot order: row slot, then lane)");
+        cg.addLine("#pragma unroll");
+        cg.addBlock("for (int _u = 0; _u < ITEMS; ++_u)", [&]() {
+            cg.addLine("const u64 _m = __ballot(_act[_u]);");
+            cg.addBlock("if (_act[_u])", [&]() {
+                const std::vector<std::string> binds = cg.rowBindings();
+                for (const auto& b : binds) cg.addLine(b);
+                cg.addLine("const u32 _p = " + B + " + " + N + " + (u32)__builtin_popcountll(_m & gdb_lanemask_lt());");
+                for (size_t k = 0; k < vars_.size(); ++k) cg.addLine(q[k] + "[_p] = " + vars_[k].name + ";");
+            });
+            cg.addLine(N + " += (u32)__builtin_popcountll(_m);");
+        });
+        cg.addLine("gdb_wave_sync();");
+        cg.addComment("the consumer runs once per queued row, lanes dense");
+        cg.addBlock("for (u32 _j = (u32)gdb_lane(); _j < " + N + "; _j += 64u)", [&]() {
+            for (size_t k = 0; k < vars_.size(); ++k)
+                cg.addLine("const " + vars_[k].type + " " + vars_[k].name + " = " + q[k] + "[" + B + " + _j];");
+            cg.setStaged(false);
+            consume();
+            cg.setStaged(true);
+        });
+        cg.addLine("gdb_wave_sync();  // the next tile's appends reuse the queue");
+    });
+}
+
+std::string HipWaveCompact::describe() const {
+    std::string d = "WaveCompact(";
+    for (size_t k = 0; k < vars_.size(); ++k) d += (k ? ", " : "") + vars_[k].type + " " + vars_[k].name;
+    return d + ")";
+}
+
+// ===========================================================================
 // Bitmaps
 // ===========================================================================
 

@@ -134,6 +134,27 @@ private:
     std::string varName_, varType_, expression_;
 };
 
+// Wave-level stream compaction of a staged batch (no reference counterpart:
+// the Metal pipelines run one row per thread). The active rows of the wave's
+// 64 x ITEMS batch append the named values to a per-wave LDS queue (ballot +
+// prefix popcount, no atomics); the consumer then runs once per queued row
+// with the wave's lanes dense, so a selective probe issues one dependent
+// random load per lane instead of one per row slot. Needs a staged phase
+// whose launch bound sets the queue size (launchBound x ITEMS entries).
+class HipWaveCompact : public HipUnaryOperator {
+public:
+    struct Var {
+        std::string type, name;
+    };
+    bool stageable() const override { return true; }
+    HipWaveCompact(std::unique_ptr<HipOperator> child, std::vector<Var> vars);
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    std::vector<Var> vars_;
+};
+
 // Semi-join bitmap build: set bit `key` (metal_operators.cpp:108-116).
 class HipBitmapBuild : public HipUnaryOperator {
 public:

@@ -1519,7 +1519,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             if (intKey) plan.requiredSymbols.push_back("sql_key_domain");
             auto id = std::make_unique<HipDirectGroupId>(scan(keyCols), fp, size, "d_sql_gid", "d_sql_groups_status",
                                                          cap, /*blockFirst=*/!intKey);
-            appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4});
+            appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4}).launchBound = 256;
             std::unique_ptr<HipOperator> cur =
                 std::make_unique<HipArrayLookup>(scan(pl.cc.used), "d_sql_gid", fp, size, "_gid", 0ll, "u32");
             cur = std::make_unique<HipSelection>(std::move(cur), "_gid <= " + std::to_string(cap) + "ll");
@@ -1530,7 +1530,7 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
             auto ht = std::make_unique<HipHash64Build>(scan(keyCols), "d_sql_groups", fp, "i", "sql_slots", true, "_g");
             ht->setBlockFirst();
             auto id = std::make_unique<HipDenseGroupId>(std::move(ht), "d_sql_groups", "_g", "sql_slots", "d_sql_gid", cap);
-            appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4});
+            appendPhase(plan, "SQL_groups", std::move(id), 256, items, {2, 4}).launchBound = 256;
             std::unique_ptr<HipOperator> cur =
                 std::make_unique<HipHash64Probe>(scan(pl.cc.used), "d_sql_groups", fp, "sql_slots", "_g");
             if (!packed)

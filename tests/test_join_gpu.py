@@ -43,7 +43,11 @@ def test_q3_small(ctx, seed, seg, d):
     _check(ctx, A, B, F, seg, d)
 
 
-@pytest.mark.parametrize("options", ["probe_bitmap=0", "sizing=given", "items=8", "staged=0", "staged=1", "staged=1,probe_bitmap=0", "prefetch=1", "prefetch=1,items=8"])
+@pytest.mark.parametrize("options", ["probe_bitmap=0", "sizing=given", "items=8", "staged=0", "staged=1", "prefetch=1",
+                                     "late=0", "late=0,items=8", "late=0,staged=1", "late=0,staged=1,probe_bitmap=0",
+                                     "late=0,prefetch=1", "late=0,prefetch=1,items=8", "late=0,compact=1",
+                                     "late=0,compact=1,items=8", "late=0,compact=1,probe_bitmap=0", "late=1,items=8",
+                                     "late=1,probe_bitmap=0", "late=1,sizing=given,block=128"])
 def test_q3_plan_variants(ctx, options):
     A, B, F = small_join(11, n_a=3000, n_b=40_000)
     _check(ctx, A, B, F, 2, 60, options)
