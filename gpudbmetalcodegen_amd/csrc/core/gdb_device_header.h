@@ -205,16 +205,31 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
     if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(w, m);
 }
 
+// Records in a block-local bitmap cache entry `e` (read as `c`) that bit m of
+// word w has had its global atomic issued. An entry may claim only bits whose
+// global atomic was issued. A different word replaces the entry. The same
+// word gains the bit by a compare-exchange against the entry as read, never
+// by an OR into whatever the entry holds by then: between the read and an OR
+// another lane may have replaced it with a different word, which would then
+// claim a bit nobody set there. A failed exchange only loses the note (a
+// later redundant atomic).
+GDB_DEV void gdb_bitmap_cache_note(u64* e, u64 c, bool same, u32 w, u32 m) {
+    if (same) {
+        u64 expect = c;
+        (void)__hip_atomic_compare_exchange_strong(e, &expect, c | (u64)m, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    } else {
+        __hip_atomic_store(e, ((u64)w << 32) | (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+}
+
 // A bitmap set behind a block-local cache (1024 entries of LDS u64: word
 // index << 32 | the bits this block has set in it). A hit skips the global
-// atomic. An entry for the same word gains bits by an LDS atomic OR, so
-// concurrent lanes setting different bits of one word all keep theirs; a
-// different word replaces the entry (a lost entry only costs a later
-// redundant atomic). A bit is recorded only after its global atomic was
-// issued. tested: also skip the atomic when a plain read already sees the
-// bit — for skewed keys, where hot words re-missed after an eviction would
-// otherwise be hit by every block (atomics on one address serialise); for
-// near-unique keys the extra read only adds latency.
+// atomic; gdb_bitmap_cache_note keeps every claimed bit backed by an issued
+// global atomic. tested: also skip the atomic when a plain read already sees
+// the bit — for skewed keys, where hot words re-missed after an eviction
+// would otherwise be hit by every block (atomics on one address serialise);
+// for near-unique keys the extra read only adds latency.
 template <bool TESTED>
 GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64* cache) {
     if ((u64)key >= nbits) return;
@@ -226,10 +241,7 @@ GDB_DEV void gdb_bitmap_set_cached(u32* __restrict__ bm, u64 nbits, i64 key, u64
     if (same && ((u32)c & m)) return;
     if (!TESTED || (__hip_atomic_load(bm + w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u)
         gdb_atomic_or_u32(bm + w, m);  // no return value: nothing waits on it
-    if (same)
-        __hip_atomic_fetch_or(e, (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-    else
-        __hip_atomic_store(e, ((u64)w << 32) | (u64)m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    gdb_bitmap_cache_note(e, c, same, w, m);
 }
 
 // ---------------------------------------------------------------------------

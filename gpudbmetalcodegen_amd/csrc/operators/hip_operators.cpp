@@ -468,10 +468,7 @@ void HipBitmapBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
                     cg.addLine("const bool _same = (u32)(_c >> 32) == _w;");
                     cg.addBlock("if (!(_same && ((u32)_c & _m)))", [&]() {
                         cg.addLine("gdb_atomic_or_u32(" + bitmap_ + " + _w, _m);");
-                        cg.addLine("if (_same) __hip_atomic_fetch_or(_e, (u64)_m, __ATOMIC_RELAXED, "
-                                   "__HIP_MEMORY_SCOPE_WORKGROUP);");
-                        cg.addLine("else __hip_atomic_store(_e, ((u64)_w << 32) | (u64)_m, __ATOMIC_RELAXED, "
-                                   "__HIP_MEMORY_SCOPE_WORKGROUP);");
+                        cg.addLine("gdb_bitmap_cache_note(_e, _c, _same, _w, _m);");
                     });
                 });
             });

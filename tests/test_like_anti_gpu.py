@@ -79,3 +79,31 @@ def test_q22_anti_join(ctx, n_b, n_a):
     finally:
         for t in dev.values():
             t.free(ctx)
+
+
+def test_q22_bitmap_cache_collisions(ctx):
+    """Referenced keys concentrated on bitmap words that share a few entries of
+    the build kernel's per-block LDS word cache (word index mod 1024 in
+    {0..3}), so lanes of one block replace each other's entries while others
+    add bits to them: every referenced key must still be set in the bitmap."""
+    rng = np.random.default_rng(2024)
+    n_a, n_b = 4_000_000, 2_000_000
+    A = {"a_key": np.arange(1, n_a + 1, dtype=np.int32), "a_seg": (np.arange(n_a) % 5).astype(np.uint8)}
+    words = rng.integers(0, n_a // (32 * 1024), n_b) * 1024 + rng.integers(0, 4, n_b)
+    fkey = words * 32 + rng.integers(0, 32, n_b)
+    fkey = np.clip(fkey, 1, n_a).astype(np.int32)
+    B = {"b_key": np.arange(1, n_b + 1, dtype=np.int32), "b_fkey": fkey,
+         "b_date": rng.integers(0, 2406, n_b).astype(np.int32)}
+    dev = Q.upload_tables(ctx, {"A": A, "B": B})
+    try:
+        for lo, hi in [(0, 2406), (300, 1700)]:
+            run = Q.run_q22(ctx, dev, lo, hi)
+            cnt, sm = oracle.q22(B, A, n_a, lo, hi)
+            cols = run.result.columns
+            exp_segs = [i for i in range(5) if cnt[i] > 0]
+            assert [int(x) for x in cols["a_seg"]] == exp_segs
+            assert [int(x) for x in cols["num_keys"]] == [int(cnt[i]) for i in exp_segs]
+            assert [int(x) for x in cols["sum_keys"]] == [int(sm[i]) for i in exp_segs]
+    finally:
+        for t in dev.values():
+            t.free(ctx)
