@@ -311,6 +311,7 @@ HipQueryPlan tpchQ18(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q18_qty"};
     plan.requiredSymbols = {"o_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -365,6 +366,7 @@ HipQueryPlan tpchQ22(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q22_mask_lo", "q22_mask_hi"};
     plan.requiredSymbols = {"c_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -433,6 +435,7 @@ HipQueryPlan tpchQ5(const PlanOptions& o) {
     }
     plan.requiredSymbols = {"r_bits", "n_bits", "s_bits", "c_bits", "o_bits"};
     plan.requiredScalars = {"q5_date_lo", "q5_date_hi"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -478,6 +481,7 @@ HipQueryPlan tpchQ10(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q10_date_lo", "q10_date_hi", "q10_flag"};
     plan.requiredSymbols = {"o_bits", "c_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -536,6 +540,7 @@ HipQueryPlan tpchQ19(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q19_mode1", "q19_mode2"};
     plan.requiredSymbols = {"p_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -634,6 +639,7 @@ HipQueryPlan tpchQ15(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q15_date_lo", "q15_date_hi"};
     plan.requiredSymbols = {"s_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -807,6 +813,7 @@ HipQueryPlan tpchQ7(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q7_date_lo", "q7_date_hi", "q7_year0"};
     plan.requiredSymbols = {"n_bits", "s_bits", "c_bits", "o_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -871,6 +878,7 @@ HipQueryPlan tpchQ8(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q8_date_lo", "q8_date_hi", "q8_year0"};
     plan.requiredSymbols = {"r_bits", "n_bits", "s_bits", "c_bits", "o_bits", "p_bits"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -994,6 +1002,7 @@ HipQueryPlan tpchQ9(const PlanOptions& o) {
     }
     plan.requiredScalars = {"like_l1", "like_l2", "q9_year0"};
     plan.requiredSymbols = {"p_bits", "s_bits", "o_bits", "q9_slots"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 
@@ -1055,6 +1064,7 @@ HipQueryPlan tpchQ20(const PlanOptions& o) {
     }
     plan.requiredScalars = {"like_l1", "q20_date_lo", "q20_date_hi"};
     plan.requiredSymbols = {"p_bits", "s_bits", "n_bits", "q20_slots"};
+    plan.chunkable = true;  // the streamed table's phase only accumulates (associative atomics, byte flags)
     return plan;
 }
 

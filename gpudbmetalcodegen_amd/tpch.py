@@ -21,6 +21,7 @@ from __future__ import annotations
 import ctypes as C
 import datetime as _dt
 import os
+import re
 from dataclasses import dataclass, field
 from pathlib import Path
 
@@ -312,19 +313,46 @@ class TpchRun:
     canonical: str = field(default="")
 
 
+def stream_table(query: str) -> str:
+    """The table a chunked run streams: lineitem where the query reads it,
+    otherwise orders (the reference's chunk loop streams one table per query,
+    codegen_main.cpp:686-808)."""
+    return "lineitem" if "lineitem" in SCANNED[query] else "orders"
+
+
+def parse_rows(text: str) -> int:
+    """N[K|M|G] row counts of --chunk (K = 1024, as codegen_main.cpp parses them)."""
+    t = text.strip().upper()
+    mult = {"K": 1 << 10, "M": 1 << 20, "G": 1 << 30}.get(t[-1:], 1)
+    n = int(t[:-1] if t[-1:] in "KMG" else t) * mult
+    if n <= 0:
+        raise ValueError(f"chunk rows must be > 0: {text}")
+    return n
+
+
 def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int = 1, options: str = "",
-        executor: E.HipGenericExecutor | None = None, params: dict | None = None) -> TpchRun:
+        executor: E.HipGenericExecutor | None = None, params: dict | None = None, chunk_rows: int = 0) -> TpchRun:
     """Run a named TPC-H query; the result columns follow the SQL select
-    list. params overrides constants of the query (e.g. q18_qty)."""
+    list. params overrides constants of the query (e.g. q18_qty).
+    chunk_rows > 0: out-of-core form; the stream table stays in host memory
+    and its phases run once per chunk of that many rows, copied into HBM
+    with double-buffered copies (plans not marked chunkable are refused, as
+    codegen_main.cpp:320-335 refuses an explicit --chunk)."""
     if query not in QUERIES:
         raise ValueError(f"unknown TPC-H query {query}; known: {QUERIES}")
     opts = ",".join(o for o in (PLAN_OPTIONS.get(query, ""), options) if o)
     from .queries import get_plan, like_words_buffer
 
     plan = get_plan(ctx, query, opts)
+    streamed = None
+    if chunk_rows:
+        if not plan.chunkable:
+            raise ValueError(f"{query} does not support chunked execution (its plan is not marked chunkable)")
+        streamed = stream_table(query)
     ex = executor or E.HipGenericExecutor(ctx)
     for t in SCANNED[query]:
-        ex.register_table(db.device(t))
+        if t != streamed:
+            ex.register_table(db.device(t))
     prm = dict(PARAMS[query], **(params or {}))
     words_ptr = None
     strings_ptr = None
@@ -356,7 +384,19 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
             buf = string_slots(prm["strings"])
             strings_ptr = ctx.upload(buf)
             ex.register_buffer("d_str_consts", strings_ptr, buf.nbytes)
-        t = ex.execute(plan, warmup, measured)
+        if streamed is None:
+            t = ex.execute(plan, warmup, measured)
+        else:
+            # only the columns the plan's kernels take cross the link
+            src = plan.source()
+            cols = {c: a for c, a in db.host[streamed].items() if re.search(r"\bp_" + c + r"\b", src)}
+            host = E.HostTable(streamed, cols)
+            try:
+                st = ex.execute_streamed(plan, host, chunk_rows)
+            finally:
+                host.close()
+            t = E.Timing(st.total_ms, 0.0, [], [])
+            t.chunks, t.stream_ms, t.bytes_copied = int(st.chunks), float(st.stream_ms), int(st.bytes_copied)
         res = ex.collect(plan)
     finally:
         if words_ptr is not None:

@@ -11,7 +11,10 @@ first use, as tools/tbl_to_colbin does), from ``--data-dir DIR``, or from
 reference: ``--warmup N`` (3), ``--repeat N`` (1, the median is reported),
 ``--csv``, ``--print-plan``, ``--dump-src DIR`` (generated HIP source, the
 counterpart of ``--dump-msl``), ``--check DIR``, ``--save-golden DIR``,
-``--check-abs-tol``, ``--check-rel-tol``, ``--threadgroup-size N``. Exit
+``--check-abs-tol``, ``--check-rel-tol``, ``--threadgroup-size N``,
+``--chunk N[K|M|G]`` (stream lineitem, or orders for queries without it, in
+chunks of N rows from host memory; queries whose plan is not chunkable are
+refused with status 3, as codegen_main.cpp:320-335 refuses them). Exit
 status: 0 on success, 1 when a check fails, 2 when a golden file is missing
 (the reference's g_checkExitCode), 3 on a usage error. ``--sql FILE`` runs
 a single-table SELECT through the engine's SQL front end (the file's stem
@@ -53,6 +56,7 @@ def parse(argv):
     ap.add_argument("--check-abs-tol", type=float, default=T.CHECK_ABS_TOL)
     ap.add_argument("--check-rel-tol", type=float, default=T.CHECK_REL_TOL)
     ap.add_argument("--threadgroup-size", type=int, default=None)
+    ap.add_argument("--chunk", default=None, help="N[K|M|G]: stream the query's largest table in chunks of N rows")
     return ap.parse_args(argv)
 
 
@@ -101,6 +105,11 @@ def main(argv=None) -> int:
         print("usage: tpch_cli [flags] [sf1|sf10|sf20|sf50|sf100] q<N>|all | --sql FILE", file=sys.stderr)
         return 3
     data_dir = a.data_dir or data_dir
+    try:
+        chunk = T.parse_rows(a.chunk) if a.chunk else 0
+    except ValueError as e:
+        print(f"tpch_cli: --chunk: {e}", file=sys.stderr)
+        return 3
     if a.generate is None and data_dir is None:
         data_dir, label = SF_DIRS["sf1"][0], label or "SF1"
     if E.device_count() < 1:
@@ -165,10 +174,19 @@ def main(argv=None) -> int:
                 Path(a.dump_src).mkdir(parents=True, exist_ok=True)
                 (Path(a.dump_src) / f"{q}.hip").write_text(plan.source())
             ex = E.HipGenericExecutor(ctx)
-            run = T.run(ctx, db, q, warmup=a.warmup, measured=1, options=opts, executor=ex)
+            if chunk and not plan.chunkable:
+                print(f"tpch_cli: {q} does not support chunked execution yet (--chunk is unsafe for this query)",
+                      file=sys.stderr)
+                status = max(status, 3)
+                continue
+            run = T.run(ctx, db, q, warmup=a.warmup, measured=1, options=opts, executor=ex, chunk_rows=chunk)
             times = [run.timing.total_ms]
             for _ in range(max(0, a.repeat - 1)):
-                times.append(T.run(ctx, db, q, warmup=0, measured=1, options=opts, executor=ex).timing.total_ms)
+                times.append(T.run(ctx, db, q, warmup=0, measured=1, options=opts, executor=ex,
+                                   chunk_rows=chunk).timing.total_ms)
+            if chunk and not a.csv:
+                print(f"{q}: streamed {T.stream_table(q)} in {run.timing.chunks} chunks of {chunk} rows, "
+                      f"{run.timing.bytes_copied / 1e6:.1f} MB copied")
             med = statistics.median(times)
             if a.csv:
                 print(f"TIMING_CSV,{q},{label},{med:.4f},{plan.compile_ms:.2f},{len(times)}")

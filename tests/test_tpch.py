@@ -456,3 +456,25 @@ def test_cli_usage_errors():
 
     assert tpch_cli.main(["q23"]) == 3  # no such query
     assert tpch_cli.main(["bogus"]) == 3
+
+
+def test_chunk_rows_parse():
+    assert T.parse_rows("16K") == 16 * 1024
+    assert T.parse_rows("2m") == 2 * 1024 * 1024
+    assert T.parse_rows("1G") == 1 << 30
+    assert T.parse_rows("12345") == 12345
+    for bad in ("0", "-5K", "K", "x"):
+        with pytest.raises(ValueError):
+            T.parse_rows(bad)
+
+
+def test_chunkable_plans_and_stream_tables():
+    """The plans marked chunkable are those whose stream-table phases only
+    accumulate (the GPU tests run each of them streamed); lineitem is the
+    stream table wherever a query reads it."""
+    from gpudbmetalcodegen_amd import engine as E
+    from test_tpch_gpu import CHUNKED
+
+    got = [q for q in T.QUERIES if E.build_hip_plan(q, T.PLAN_OPTIONS.get(q, "")).chunkable]
+    assert got == CHUNKED
+    assert [q for q in CHUNKED if T.stream_table(q) == "orders"] == ["Q13", "Q22"]

@@ -92,3 +92,33 @@ def test_tpch_cli_check_exit_codes(env, tmp_path, capsys):
     (tmp_path / "Q6_GEN.csv").write_text("revenue\n1.0\n")
     assert tpch_cli.main(base + ["--check", str(tmp_path), "q6"]) == 1
     assert tpch_cli.main(base + ["--check", str(tmp_path), "q4"]) == 2
+
+
+CHUNKED = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q9", "Q10", "Q12", "Q13", "Q14", "Q15", "Q18", "Q19", "Q20",
+           "Q22"]
+
+
+@pytest.mark.parametrize("q", CHUNKED)
+def test_tpch_chunked_matches_oracle(env, q):
+    """Out-of-core form: the stream table crosses the link in several chunks
+    (lineitem ~60K rows in 17K-row chunks, orders ~15K rows in 4K-row
+    chunks); the result is bit-exact with the oracle, as the resident run is."""
+    ctx, db, host, _ = env
+    chunk = 4_000 if T.stream_table(q) == "orders" else 17_000
+    print(f"TPC-H {q} streamed ({T.stream_table(q)}, {chunk}-row chunks)", flush=True)
+    r = T.run(ctx, db, q, params=PARAMS[q], chunk_rows=chunk)
+    exp, fm = OT.run(host, q, PARAMS[q])
+    assert r.columns == exp
+    assert r.timing.chunks == -(-host[T.stream_table(q)][next(iter(host[T.stream_table(q)]))].shape[0] // chunk)
+    assert r.timing.chunks >= 3
+
+
+def test_tpch_chunk_refused_for_unchunkable_plan(env, capsys):
+    from gpudbmetalcodegen_amd import tpch_cli
+
+    ctx, db, host, _ = env
+    with pytest.raises(ValueError, match="chunked"):
+        T.run(ctx, db, "Q17", chunk_rows=10_000)
+    assert tpch_cli.main(["--generate", "0.002", "--warmup", "0", "--chunk", "4K", "q17"]) == 3
+    assert tpch_cli.main(["--generate", "0.002", "--warmup", "0", "--chunk", "4K", "q6"]) == 0
+    assert "Q6: streamed lineitem in" in capsys.readouterr().out
