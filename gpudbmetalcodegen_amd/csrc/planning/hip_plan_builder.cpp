@@ -184,7 +184,11 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
             top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
         auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
-        ph.staged = optInt(o, "staged", 1) != 0;  // home-slot CAS of the batch in flight together
+        // staged=1 puts the home-slot CAS of a thread's ITEMS rows in flight
+        // together; per-row emission measures faster here (0.166 vs 0.177 ms
+        // at 15M rows: ~10% of the rows reach the CAS, so a batch rarely
+        // holds more than one)
+        ph.staged = optInt(o, "staged", 0) != 0;
         ph.prefetch = optInt(o, "prefetch", 0) != 0;
     }
     {
