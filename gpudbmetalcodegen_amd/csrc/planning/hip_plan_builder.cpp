@@ -203,11 +203,12 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
             cur = std::make_unique<HipBitmapProbe>(std::move(cur), "d_q3_b_bitmap", "f_fkey", "b_bits");
         // compact=1: the rows that pass the date filter and the bitmap are
         // queued per wave, so the hash probe and slot add run lanes-dense
-        const bool compact = late || optInt(o, "compact", 0) != 0;
+        const bool compact = optInt(o, "compact", late ? 1 : 0) != 0;
         std::string rev = "f_price * (i64)(100 - f_disc)";
         if (late) {
-            cur = std::make_unique<HipWaveCompact>(std::move(cur),
-                                                   std::vector<HipWaveCompact::Var>{{"i32", "f_fkey"}, {"u64", "i"}});
+            if (compact)
+                cur = std::make_unique<HipWaveCompact>(
+                    std::move(cur), std::vector<HipWaveCompact::Var>{{"i32", "f_fkey"}, {"u64", "i"}});
             const auto& ft = catalogTable("F");
             cur = std::make_unique<HipBindColumns>(
                 std::move(cur), "F",
