@@ -118,6 +118,7 @@ class DeviceTable:
     rows: int
     columns: dict = field(default_factory=dict)  # column -> device pointer
     nbytes: int = 0
+    key_max: dict = field(default_factory=dict)  # column -> largest value (device max-key scan, cached)
 
     @classmethod
     def from_host(cls, ctx: Context, name: str, cols: dict) -> "DeviceTable":

@@ -166,3 +166,27 @@ def size_for(est: CardinalityEstimator, X: np.ndarray, ub: float, margin: float 
         return Sizing(g, slots, "part", 4096 if top8 >= 0.02 else 1024, X, ub)
     variant = "lds" if (g <= lds_slots // 2 or top8 >= 0.02) else "global"
     return Sizing(g, slots, variant, lds_slots, X, ub)
+
+
+def run_groupby_learned(ctx: E.Context, dev: dict, est: CardinalityEstimator, key_min: int, key_max: int,
+                        val_cut: int | None = None, executor: E.HipGenericExecutor | None = None,
+                        collect: bool = True, slots_scale: float = 1.0):
+    """The learned-sizing form of the hash group-by over table G: a device
+    row sample -> 16 features -> MLP -> table slots and variant -> one
+    group-by run (a reported overflow doubles the table and reruns, so the
+    result stays exact whatever the estimate). slots_scale scales the
+    predicted table (below 1: a deliberately low estimate, to exercise the
+    resize path). Returns (GroupByRun, Sizing)."""
+    from . import queries as Q
+
+    t = dev["G"]
+    kptr = t.columns["g_key"]
+    vptr = t.columns["g_val"] if val_cut is not None else None
+    hk, hv = device_sample(ctx, kptr, vptr, t.rows)
+    cut = (1 << 63) - 1 if val_cut is None else val_cut
+    x, ub = features(t.rows, key_min, key_max, hk, hv, cut)
+    sz = size_for(est, x, ub)
+    slots = max(64, int(sz.slots * slots_scale))
+    run = Q.run_groupby(ctx, dev, slots=slots, variant=sz.variant, lds_slots=sz.lds_slots, val_cut=val_cut,
+                        executor=executor, collect=collect)
+    return run, sz

@@ -77,6 +77,38 @@ def upload_tables(ctx: E.Context, tables: dict) -> dict:
     return {name: E.DeviceTable.from_host(ctx, name, cols) for name, cols in tables.items() if isinstance(cols, dict)}
 
 
+def max_key(ctx: E.Context, dev: dict, table: str, column: str) -> int:
+    """Largest value of an int32 key column, from the MAXKEY plan's scan on the
+    device (cached on the table): the key-range symbol of a bitmap or
+    key-indexed array is this + 1, as the reference registers max + 1 from an
+    exact scan (codegen/execution/max_key_symbols.cpp:188-235). 0 for an
+    empty table."""
+    t = dev.get(table)
+    if t is None:
+        raise ValueError(f"max_key: table {table} is not resident; pass the key bound explicitly")
+    if column in t.key_max:
+        return t.key_max[column]
+    mk = 0
+    if t.rows > 0:
+        plan = get_plan(ctx, "MAXKEY", f"table={table},column={column}")
+        ex = E.HipGenericExecutor(ctx)
+        try:
+            ex.register_table(t)
+            ex.execute(plan)
+            res = ex.collect(plan)
+        finally:
+            ex.close()
+        if int(res.columns["rows"][0]) != t.rows:
+            raise RuntimeError(f"max_key: scanned {int(res.columns['rows'][0])} rows of {table}, expected {t.rows}")
+        mk = max(0, int(res.columns["max_key"][0]))
+    t.key_max[column] = mk
+    return mk
+
+
+def _key_bits(ctx: E.Context, dev: dict, table: str, column: str, given: int | None) -> int:
+    return (given if given is not None else max_key(ctx, dev, table, column)) + 1
+
+
 # ---------------------------------------------------------------------------
 # Workload 0: Q1
 # ---------------------------------------------------------------------------
@@ -169,14 +201,15 @@ def run_q3(ctx: E.Context, dev: dict, seg: int = W.W1_SEG, d: int = W.W1_DATE, o
     q3_slots_from_stats; a reported overflow doubles it and reruns, so results
     stay exact). Key-range symbols come from table statistics (max key), as
     the reference's cached max-key mode does
-    (codegen/execution/max_key_symbols.cpp:28-41)."""
+    (codegen/execution/max_key_symbols.cpp:28-41): given, or else the
+    device max-key scan of max_key()."""
     plan = get_plan(ctx, "Q3", options)
     ex = _executor(ctx, executor)
     _register(ex, dev, ("A", "B", "F"), stream)
     ex.register_scalar("q3_seg", seg)
     ex.register_scalar("q3_date", d)
-    ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
-    ex.register_symbol("b_bits", (max_b if max_b is not None else dev["B"].rows) + 1)
+    ex.register_symbol("a_bits", _key_bits(ctx, dev, "A", "a_key", max_a))
+    ex.register_symbol("b_bits", _key_bits(ctx, dev, "B", "b_key", max_b))
     if "sizing=given" not in options:
         t, st = _run(ex, plan, warmup, measured, stream)
         return QueryRun(ex.collect(plan), t, plan, ex, st)
@@ -346,7 +379,7 @@ def run_q13(ctx: E.Context, dev: dict, w1: bytes, w2: bytes, negate: bool = Fals
         ex.register_buffer("d_like_words", wp, 64)
         ex.register_scalar("like_l1", len(w1))
         ex.register_scalar("like_l2", len(w2))
-        ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
+        ex.register_symbol("a_bits", _key_bits(ctx, dev, "A", "a_key", max_a))
         t, st = _run(ex, plan, warmup, measured, stream)
         res = ex.collect(plan)
     finally:
@@ -362,7 +395,7 @@ def run_q22(ctx: E.Context, dev: dict, date_lo: int, date_hi: int, max_a: int | 
     _register(ex, dev, ("B", "A"), stream)
     ex.register_scalar("q22_date_lo", date_lo)
     ex.register_scalar("q22_date_hi", date_hi)
-    ex.register_symbol("a_bits", (max_a if max_a is not None else dev["A"].rows) + 1)
+    ex.register_symbol("a_bits", _key_bits(ctx, dev, "A", "a_key", max_a))
     t, st = _run(ex, plan, warmup, measured, stream)
     return QueryRun(ex.collect(plan), t, plan, ex, st)
 

@@ -3,24 +3,30 @@
 Four plan types arrive as a Poisson stream (25% each): scan-aggregate (Q1 on
 F), three-way join (Q3 over A/B/F), bitmap anti-join (Q22A over B/A) and
 byte-exact LIKE '%w1%w2%' + per-key count + histogram (Q13L over S/A).
+Parameters are drawn from a per-kind menu (prepared statements with a fixed
+set of bindings), so each distinct query can be checked once against an
+independent evaluation and every served answer compared with it.
 
 Plain form: one query at a time in arrival order.
-Served form (continuous batching): a request loop admits each query when a
-worker slot is free and its HBM footprint fits the scratch budget next to
-the queries already running; each worker owns a stream and its own runners
-(disjoint buffers), so admitted queries overlap on the device and a query
-that arrives while others run starts as soon as a slot and memory free up.
+Served form (continuous batching): a dispatcher admits the oldest waiting
+query as soon as a worker is idle and its HBM footprint fits the scratch
+budget next to the queries already running; each worker is a persistent
+thread that owns a stream and its own runners (disjoint buffers), so
+admitted queries overlap on the device and a query that arrives while
+others run starts the moment a slot and memory free up. The dispatcher
+sleeps on a condition variable until the next arrival or the next
+completion; nothing polls.
 
-Latency = completion - arrival; reported as p50/p99 with throughput.
+Latency = completion (result on the host) - arrival; reported as p50/p99
+(nearest rank) with throughput.
 """
 
 from __future__ import annotations
 
 import ctypes as C
-import heapq
-import queue
 import threading
 import time
+from collections import deque
 from dataclasses import dataclass, field
 
 import numpy as np
@@ -41,7 +47,9 @@ class Query:
     arrival: float = 0.0          # seconds from stream start
     start: float = 0.0
     done: float = 0.0
-    result: str = ""
+    result: str = ""              # canonical CSV of the answer
+    columns: dict = field(default_factory=dict)   # the answer's columns (ints / strings)
+    groups: dict | None = None    # join only, with Worker(verify=True): every group of the hash table
     rows: int = 0
     footprint: int = 0
     error: str = ""
@@ -49,6 +57,11 @@ class Query:
     @property
     def latency_ms(self) -> float:
         return (self.done - self.arrival) * 1e3
+
+    @property
+    def key(self) -> tuple:
+        """Identity of the query's answer: kind and parameters."""
+        return (self.kind,) + tuple(sorted(self.params.items()))
 
 
 def percentile(xs, p):
@@ -68,8 +81,10 @@ class ResidentDB:
         self.vocab = tables.get("vocab", [])
         self.dev = Q.upload_tables(ctx, {k: v for k, v in tables.items() if k in ("F", "A", "B", "S")})
         self.n = {k: t.rows for k, t in self.dev.items()}
-        self.max_a = int(tables["A"]["a_key"].max())
-        self.max_b = int(tables["B"]["b_key"].max())
+        # key-range symbols from an exact max-key scan on the device (the
+        # reference's registerMaxKeySymbols, max_key_symbols.cpp:188-235)
+        self.max_a = Q.max_key(ctx, self.dev, "A", "a_key")
+        self.max_b = Q.max_key(ctx, self.dev, "B", "b_key")
         self.resident_bytes = sum(t.nbytes for t in self.dev.values())
 
     def rows_for(self, kind: str) -> int:
@@ -95,8 +110,28 @@ class ResidentDB:
             t.free(self.ctx)
 
 
-def make_stream(db: ResidentDB, rate: float, duration: float, seed: int, backlog: bool = False) -> list[Query]:
-    """Poisson arrivals at `rate` q/s over `duration` s (backlog: all at t=0)."""
+def param_menu(db: ResidentDB, seed: int = 4, per_kind: int = 8) -> dict:
+    """A fixed menu of `per_kind` parameter bindings for each plan type."""
+    r = np.random.default_rng(seed)
+    menu = {k: [] for k in KINDS}
+    for _ in range(per_kind):
+        menu["scan"].append({"cutoff": int(r.integers(2000, W.DATE_DAYS_W0))})
+        menu["join"].append({"seg": int(r.integers(0, 5)), "d": int(r.integers(600, 1800))})
+        lo = int(r.integers(0, 2000))
+        menu["anti"].append({"lo": lo, "hi": lo + int(r.integers(50, 400))})
+        a = db.vocab[int(r.integers(0, len(db.vocab)))]
+        b = db.vocab[int(r.integers(0, len(db.vocab)))]
+        if r.random() < 0.5:
+            a, b = a[:2], b[:2]
+        menu["like"].append({"w1": a, "w2": b})
+    return menu
+
+
+def make_stream(db: ResidentDB, rate: float, duration: float, seed: int, backlog: bool = False,
+                menu: dict | None = None) -> list[Query]:
+    """Poisson arrivals at `rate` q/s over `duration` s (backlog: all at t=0);
+    plan types uniform, parameters uniform over the menu."""
+    menu = menu or param_menu(db)
     r = np.random.default_rng(seed)
     qs, t, i = [], 0.0, 0
     while True:
@@ -104,19 +139,7 @@ def make_stream(db: ResidentDB, rate: float, duration: float, seed: int, backlog
         if t > duration:
             break
         kind = KINDS[int(r.integers(0, 4))]
-        if kind == "scan":
-            p = {"cutoff": int(r.integers(2000, W.DATE_DAYS_W0))}
-        elif kind == "join":
-            p = {"seg": int(r.integers(0, 5)), "d": int(r.integers(600, 1800))}
-        elif kind == "anti":
-            lo = int(r.integers(0, 2000))
-            p = {"lo": lo, "hi": lo + int(r.integers(50, 400))}
-        else:
-            a = db.vocab[int(r.integers(0, len(db.vocab)))]
-            b = db.vocab[int(r.integers(0, len(db.vocab)))]
-            if r.random() < 0.5:
-                a, b = a[:2], b[:2]
-            p = {"w1": a, "w2": b}
+        p = dict(menu[kind][int(r.integers(0, len(menu[kind])))])
         q = Query(i, kind, p, 0.0 if backlog else t, rows=db.rows_for(kind))
         q.footprint = db.footprint(q)
         qs.append(q)
@@ -124,13 +147,20 @@ def make_stream(db: ResidentDB, rate: float, duration: float, seed: int, backlog
     return qs
 
 
-class Worker:
-    """One device stream with its own runners (one per plan type)."""
+def _plain_columns(res: E.GenericResult) -> dict:
+    return {k: [x if isinstance(x, str) else int(x) for x in v] for k, v in res.columns.items()}
 
-    def __init__(self, db: ResidentDB):
+
+class Worker:
+    """One device stream with its own runners (one per plan type).
+    verify=True also reads every group of a join's hash table after the run
+    (a check of the whole join, not only its top-10; costs a table read)."""
+
+    def __init__(self, db: ResidentDB, verify: bool = False):
         hip, _ = N.load()
         self.db = db
         self.ctx = db.ctx
+        self.verify = verify
         s = C.c_void_p()
         N.hip_check(hip.gdb_stream_create(self.ctx.handle, C.byref(s)))
         self.stream = s
@@ -138,13 +168,15 @@ class Worker:
         self.words = self.ctx.alloc(64)
         self.hwords = np.zeros(64, np.uint8)
 
-    def run(self, q: Query) -> str:
+    def run(self, q: Query) -> None:
         db, ex = self.db, self.ex[q.kind]
         p = q.params
         if q.kind == "scan":
             run = Q.run_q1(self.ctx, db.dev, p["cutoff"], executor=ex)
         elif q.kind == "join":
             run = Q.run_q3(self.ctx, db.dev, p["seg"], p["d"], max_a=db.max_a, max_b=db.max_b, executor=ex)
+            if self.verify:
+                q.groups = Q.q3_hash_groups(run)
         elif q.kind == "anti":
             run = Q.run_q22(self.ctx, db.dev, p["lo"], p["hi"], max_a=db.max_a, executor=ex)
         else:
@@ -155,7 +187,8 @@ class Worker:
             N.hip_check(hip.gdb_memcpy_h2d_async(C.c_void_p(self.words), self.hwords.ctypes.data_as(C.c_void_p), 64,
                                                  self.stream))
             run = Q.run_q13(self.ctx, db.dev, p["w1"], p["w2"], max_a=db.max_a, words_ptr=self.words, executor=ex)
-        return run.result.canonical
+        q.result = run.result.canonical
+        q.columns = _plain_columns(run.result)
 
     def close(self):
         hip, _ = N.load()
@@ -165,23 +198,34 @@ class Worker:
         hip.gdb_stream_destroy(self.stream)
 
 
-class Budget:
-    """HBM scratch budget: blocks until `nbytes` fits next to running queries."""
+class _Runner(threading.Thread):
+    """Persistent thread of one worker: runs the queries the dispatcher hands it."""
 
-    def __init__(self, total: int):
-        self.total, self.used = total, 0
+    def __init__(self, worker: Worker, t0: float, on_done):
+        super().__init__(daemon=True)
+        self.worker, self.t0, self.on_done = worker, t0, on_done
+        self.jobs: deque = deque()
         self.cv = threading.Condition()
 
-    def fits(self, nbytes: int) -> bool:
-        return self.used + nbytes <= self.total or self.used == 0
-
-    def take(self, nbytes: int) -> None:
-        self.used += nbytes
-
-    def give(self, nbytes: int) -> None:
+    def submit(self, q: Query | None) -> None:
         with self.cv:
-            self.used -= nbytes
-            self.cv.notify_all()
+            self.jobs.append(q)
+            self.cv.notify()
+
+    def run(self) -> None:
+        while True:
+            with self.cv:
+                while not self.jobs:
+                    self.cv.wait()
+                q = self.jobs.popleft()
+            if q is None:
+                return
+            try:
+                self.worker.run(q)
+            except Exception as e:  # surfaced in the report
+                q.error = str(e)[:300]
+            q.done = time.perf_counter() - self.t0
+            self.on_done(self, q)
 
 
 def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: int = 2 << 30,
@@ -189,76 +233,52 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
     """Run a query stream. workers=1 with an unlimited budget is the plain
     one-at-a-time form; workers>1 is the served (continuous batching) form."""
     own = pool is None
-    pool = pool or [Worker(db) for _ in range(workers)]
-    pool = pool[:workers]
-    budget = Budget(budget_bytes)
-    free_workers = queue.Queue()
-    for w in pool:
-        free_workers.put(w)
-    pending = []                      # admission queue: (arrival, qid)
-    by_id = {q.qid: q for q in stream}
+    pool = (pool or [Worker(db) for _ in range(workers)])[:workers]
     arrivals = sorted(stream, key=lambda q: (q.arrival, q.qid))
-    done_count = [0]
-    lock = threading.Lock()
-    finished = threading.Event()
-    max_inflight = [0]
-    inflight = [0]
+    cv = threading.Condition()
+    state = {"used": 0, "done": 0, "inflight": 0, "max_inflight": 0}
+    idle: deque = deque()
 
-    def finish(w: Worker, q: Query):
-        try:
-            q.result = w.run(q)
-        except Exception as e:  # surfaced in the report
-            q.error = str(e)[:300]
-        q.done = time.perf_counter() - t0
-        budget.give(q.footprint)
-        free_workers.put(w)
-        with lock:
-            inflight[0] -= 1
-            done_count[0] += 1
-            if done_count[0] == len(stream):
-                finished.set()
+    def on_done(runner: _Runner, q: Query) -> None:
+        with cv:
+            state["used"] -= q.footprint
+            state["done"] += 1
+            state["inflight"] -= 1
+            idle.append(runner)
+            cv.notify()
 
-    threads = []
     t0 = time.perf_counter()
+    runners = [_Runner(w, t0, on_done) for w in pool]
+    for r in runners:
+        r.start()
+        idle.append(r)
+    pending: deque = deque()          # arrived, not yet admitted (FIFO)
     ai = 0
-    while done_count[0] < len(stream):
-        now = time.perf_counter() - t0
-        while ai < len(arrivals) and arrivals[ai].arrival <= now:
-            heapq.heappush(pending, (arrivals[ai].arrival, arrivals[ai].qid))
-            ai += 1
-        admitted = False
-        if pending:
-            with budget.cv:
-                q = by_id[pending[0][1]]
-                ok = budget.fits(q.footprint)
-                if ok:
-                    budget.take(q.footprint)
-            if ok:
-                try:
-                    w = free_workers.get(timeout=0.0005)
-                except queue.Empty:
-                    budget.give(q.footprint)
-                    w = None
-                if w is not None:
-                    heapq.heappop(pending)
-                    q.start = time.perf_counter() - t0
-                    with lock:
-                        inflight[0] += 1
-                        max_inflight[0] = max(max_inflight[0], inflight[0])
-                    th = threading.Thread(target=finish, args=(w, q), daemon=True)
-                    th.start()
-                    threads.append(th)
-                    admitted = True
-        if not admitted:
-            if ai < len(arrivals) and not pending:
-                time.sleep(max(0.0, min(0.001, arrivals[ai].arrival - (time.perf_counter() - t0))))
-            else:
-                time.sleep(0.0002)
-        if not pending and ai >= len(arrivals):
-            finished.wait()
-            break
-    for th in threads:
-        th.join()
+    with cv:
+        while state["done"] < len(stream):
+            now = time.perf_counter() - t0
+            while ai < len(arrivals) and arrivals[ai].arrival <= now:
+                pending.append(arrivals[ai])
+                ai += 1
+            # admit the oldest waiting query while a worker is idle and its
+            # footprint fits (an empty device always admits one)
+            while pending and idle and (state["used"] + pending[0].footprint <= budget_bytes or state["inflight"] == 0):
+                q = pending.popleft()
+                state["used"] += q.footprint
+                state["inflight"] += 1
+                state["max_inflight"] = max(state["max_inflight"], state["inflight"])
+                q.start = time.perf_counter() - t0
+                idle.popleft().submit(q)
+            if state["done"] >= len(stream):
+                break
+            timeout = None
+            if ai < len(arrivals) and not (pending and idle):
+                timeout = max(0.0, arrivals[ai].arrival - (time.perf_counter() - t0))
+            cv.wait(timeout)
+    for r in runners:
+        r.submit(None)
+    for r in runners:
+        r.join()
     makespan = max(q.done for q in stream) if stream else 0.0
     lat = [q.latency_ms for q in stream]
     rows = sum(q.rows for q in stream)
@@ -266,7 +286,7 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
         "queries": len(stream), "workers": len(pool), "budget_bytes": budget_bytes,
         "p50_ms": percentile(lat, 0.50), "p99_ms": percentile(lat, 0.99), "mean_ms": float(np.mean(lat)) if lat else 0,
         "makespan_s": makespan, "queries_per_s": len(stream) / makespan if makespan else 0.0,
-        "rows_per_s": rows / makespan if makespan else 0.0, "max_inflight": max_inflight[0],
+        "rows_per_s": rows / makespan if makespan else 0.0, "max_inflight": state["max_inflight"],
         "errors": sum(1 for q in stream if q.error),
         "per_kind_p99_ms": {k: percentile([q.latency_ms for q in stream if q.kind == k], 0.99) for k in KINDS},
     }

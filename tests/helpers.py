@@ -34,3 +34,23 @@ def q1_expected(words: np.ndarray) -> dict:
 
 def as_lists(columns: dict) -> dict:
     return {k: [x if isinstance(x, str) else int(x) for x in v] for k, v in columns.items()}
+
+
+def served_expected(T: dict, kind: str, params: dict, max_a: int, threads: int = 0) -> dict:
+    """The oracle's answer to one served query of workload 4, in the
+    collector's columns: scan = Q1 over F, join = top-10 of the three-way
+    join, anti = the anti-join's per-segment count and key sum, like = the
+    LIKE count histogram."""
+    if kind == "scan":
+        return q1_expected(oracle.q1(T["F"], params["cutoff"], threads=threads))
+    if kind == "join":
+        top = oracle.q3_topk(T["A"], T["B"], T["F"], params["seg"], params["d"], 10, threads=threads)
+        return {k: [int(x) for x in v] for k, v in top.items()}
+    if kind == "anti":
+        cnt, sm = oracle.q22(T["B"], T["A"], max_a, params["lo"], params["hi"])
+        segs = [i for i in range(5) if cnt[i] > 0]
+        return {"a_seg": segs, "num_keys": [int(cnt[i]) for i in segs], "sum_keys": [int(sm[i]) for i in segs]}
+    if kind == "like":
+        h = oracle.q13(T["S"], T["A"], max_a, params["w1"], params["w2"], threads=threads)
+        return {k: [int(x) for x in v] for k, v in h.items()}
+    raise ValueError(kind)
