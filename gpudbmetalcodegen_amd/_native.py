@@ -166,6 +166,7 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_executor_execute_range", [vp, vp, i32, i32, i32, i32, P(ExecTiming)])
     _sig(eng, "gdb_executor_prepare", [vp, vp])
     _sig(eng, "gdb_executor_execute_streamed", [vp, vp, cstr, u64, i32, P(cstr), P(vp), P(i32), u64, P(ChunkStats)])
+    _sig(eng, "gdb_executor_execute_streamed_colbin", [vp, vp, cstr, cstr, u64, P(ChunkStats)])
     _sig(eng, "gdb_plan_chunkable", [vp, P(i32)])
     _sig(eng, "gdb_executor_enqueue", [vp, vp])
     _sig(eng, "gdb_executor_check_status", [vp, vp])

@@ -16,10 +16,12 @@
 #include "runtime/estimator.h"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <set>
 #include <string>
 #include <thread>
 
@@ -429,6 +431,91 @@ gdb_status gdb_executor_execute_streamed(gdb_executor ex, gdb_plan p, const char
         installSizingRules(ex, p);
         runtime::ChunkStreamer cs(ex->ctx, *ex->ex, p->plan, p->cg, compilerFor(ex->ctx), p->cq);
         const auto st = cs.run(table, rows, cols, chunk_rows);
+        if (out) {
+            out->total_ms = st.totalMs;
+            out->stream_ms = st.streamMs;
+            out->chunks = st.chunks;
+            out->bytes_copied = st.bytesCopied;
+            out->first_stream_phase = st.firstStreamPhase;
+            out->last_stream_phase = st.lastStreamPhase;
+        }
+        return GDB_OK;
+    });
+}
+
+namespace {
+
+// Rows [r0, r0 + n) of one .colbin column in the engine's format: INT/DATE
+// int32 and CHAR1/CHAR_FIXED bytes as stored, FLOAT (float32 in the file)
+// as exact int64 hundredths rounded half to even (the loader's rule,
+// to_engine_format in the Python ingest). Large chunks are split over threads.
+void convertColbinRows(const infra::ColbinColumn& fc, void* dst, uint64_t r0, uint64_t n) {
+    const size_t fileEb = infra::elemBytes(fc.type, fc.width);
+    auto part = [&](uint64_t a, uint64_t b) {
+        if (fc.type == infra::TpchType::FLOAT) {
+            const float* src = reinterpret_cast<const float*>(fc.data) + r0;
+            int64_t* out = static_cast<int64_t*>(dst);
+            for (uint64_t i = a; i < b; ++i) out[i] = static_cast<int64_t>(std::nearbyint(static_cast<double>(src[i]) * 100.0));
+        } else {
+            std::memcpy(static_cast<uint8_t*>(dst) + a * fileEb, fc.data + (r0 + a) * fileEb, (b - a) * fileEb);
+        }
+    };
+    const uint64_t kPar = 1u << 20;
+    if (n < 2 * kPar) {
+        part(0, n);
+        return;
+    }
+    const unsigned t = std::min<unsigned>(8, std::max(1u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const uint64_t step = (n + t - 1) / t;
+    for (unsigned k = 0; k < t; ++k) {
+        const uint64_t a = k * step, b = std::min<uint64_t>(n, a + step);
+        if (a < b) th.emplace_back(part, a, b);
+    }
+    for (auto& x : th) x.join();
+}
+
+}  // namespace
+
+gdb_status gdb_executor_execute_streamed_colbin(gdb_executor ex, gdb_plan p, const char* table, const char* colbin_path,
+                                                uint64_t chunk_rows, gdb_chunk_stats* out) {
+    return guard([&]() -> gdb_status {
+        if (!p->compiled || p->ctx != ex->ctx)
+            return setErr(GDB_ERR_INVALID, "gdb_executor_execute_streamed_colbin: plan not compiled for this context");
+        if (!table || !colbin_path)
+            return setErr(GDB_ERR_INVALID, "gdb_executor_execute_streamed_colbin: no table or file");
+        const infra::TpchTable& tdef = infra::tpchTable(table);
+        infra::ColbinFile file(colbin_path, "");
+        // the columns the stream phases read, each converted chunk by chunk
+        std::set<std::string> names;
+        for (const auto& ph : p->cg.getPhases())
+            if (ph.scannedTable == table)
+                for (const auto& b : ph.bindings)
+                    if (b.kind == codegen::HipParamKind::TableColumn && b.tableName == table) names.insert(b.columnName);
+        std::vector<runtime::StreamColumn> cols;
+        for (const auto& name : names) {
+            const infra::TpchColumn& c = tdef.column(name);
+            const infra::ColbinColumn& fc = file.column(c.index);
+            if (fc.type != c.type || (c.type == infra::TpchType::CHAR_FIXED && fc.width != c.width))
+                return setErr(GDB_ERR_INVALID, std::string("gdb_executor_execute_streamed_colbin: column ") + name +
+                                                   " has another type or width in " + colbin_path);
+            if (fc.bytes < file.rows() * infra::elemBytes(fc.type, fc.width))
+                return setErr(GDB_ERR_INVALID, std::string("gdb_executor_execute_streamed_colbin: column ") + name +
+                                                   " is shorter than the file's row count");
+            runtime::StreamColumn sc;
+            sc.name = name;
+            sc.elemBytes = c.type == infra::TpchType::FLOAT ? 8 : static_cast<int>(infra::elemBytes(c.type, c.width));
+            const infra::ColbinColumn* fcp = &fc;
+            sc.produce = [fcp](void* dst, uint64_t r0, uint64_t n) { convertColbinRows(*fcp, dst, r0, n); };
+            cols.push_back(std::move(sc));
+        }
+        if (cols.empty())
+            return setErr(GDB_ERR_INVALID, std::string("gdb_executor_execute_streamed_colbin: no phase reads ") + table);
+        ex->ex->registerTable(table, file.rows());
+        registerPlanTables(ex, p);
+        installSizingRules(ex, p);
+        runtime::ChunkStreamer cs(ex->ctx, *ex->ex, p->plan, p->cg, compilerFor(ex->ctx), p->cq);
+        const auto st = cs.run(table, file.rows(), cols, chunk_rows);
         if (out) {
             out->total_ms = st.totalMs;
             out->stream_ms = st.streamMs;

@@ -1,5 +1,6 @@
 #include "hip_operators.h"
 
+#include <cstdlib>
 #include <sstream>
 #include <stdexcept>
 
@@ -782,21 +783,31 @@ std::string HipBitmapFlag::describe() const {
 
 HipByteFlagBuild::HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std::string& flagsName,
                                    const std::string& keyExpr, const std::string& sizeExpr,
-                                   const std::string& valueExpr)
-    : HipUnaryOperator(std::move(child)), flags_(flagsName), key_(keyExpr), size_(sizeExpr), value_(valueExpr) {}
+                                   const std::string& valueExpr, bool tested)
+    : HipUnaryOperator(std::move(child)),
+      flags_(flagsName),
+      key_(keyExpr),
+      size_(sizeExpr),
+      value_(valueExpr),
+      tested_(tested) {}
 
 void HipByteFlagBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(flags_, "u8", 1, size_, InitKind::Zero);
     cg.addSymbolParam("nflags_" + flags_, size_);
     child_->produce(cg, [&]() {
-        cg.addLine("{ const u64 _fk = (u64)(i64)(" + key_ + "); if (_fk < nflags_" + flags_ + ") " + flags_ +
-                   "[_fk] = (u8)(" + value_ + "); }");
+        if (tested_) {
+            cg.addLine("{ const u64 _fk = (u64)(i64)(" + key_ + "); const u8 _fv = (u8)(" + value_ + "); if (_fk < nflags_" +
+                       flags_ + " && " + flags_ + "[_fk] != _fv) " + flags_ + "[_fk] = _fv; }");
+        } else {
+            cg.addLine("{ const u64 _fk = (u64)(i64)(" + key_ + "); if (_fk < nflags_" + flags_ + ") " + flags_ +
+                       "[_fk] = (u8)(" + value_ + "); }");
+        }
         consume();
     });
 }
 
 std::string HipByteFlagBuild::describe() const {
-    return "ByteFlagBuild(" + flags_ + "[" + key_ + "] = " + value_ + ")";
+    return "ByteFlagBuild(" + flags_ + "[" + key_ + "] = " + value_ + (tested_ ? ", tested" : "") + ")";
 }
 
 HipDenseStore::HipDenseStore(std::unique_ptr<HipOperator> child, const std::string& arrayName,
@@ -1262,6 +1273,15 @@ void HipKeyedAgg::addAverage(const std::string& name, const std::string& numerat
     avgs_.push_back({name, {numerator, denominator}});
 }
 
+// GDB_SCALAR_ATOMIC=1 (read when a plan is generated): the reduction
+// ablation of small-bucket KeyedAgg, every row updating the output with
+// global atomics instead of registers + wave/LDS reduction + one atomic per
+// block (the reference's GPUDB_SCALAR_ATOMIC, metal_operators.cpp:7-13).
+static bool scalarAtomicMode() {
+    const char* e = std::getenv("GDB_SCALAR_ATOMIC");
+    return e && e[0] && e[0] != '0';
+}
+
 void HipKeyedAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     if (aggs_.empty()) throw std::runtime_error("HipKeyedAgg(" + out_ + "): no aggregates");
     // Word layout per bucket; up to 4096 buckets are padded to 128 bytes so
@@ -1291,7 +1311,34 @@ void HipKeyedAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const long long ldsBytes = (long long)numBuckets_ * words * 8;
     const bool lds = !regs && ldsBytes <= 48 * 1024;
 
-    if (regs) {
+    if (regs && scalarAtomicMode()) {
+        // Ablation (GDB_SCALAR_ATOMIC=1): no register accumulation and no
+        // block reduction; every row issues its own global atomics, as the
+        // reference's GPUDB_SCALAR_ATOMIC (metal_operators.cpp:7-13, :259).
+        cg.addComment("KeyedAgg " + out_ + ": one global atomic per row and slot (reduction ablation)");
+        child_->produce(cg, [&]() {
+            cg.addLine("u64* _ka_row = " + out_ + " + (u64)(" + (numBuckets_ > 1 ? "(int)(" + bucket_ + ")" : std::string("0")) +
+                       ") * " + s(wpb) + "ull;");
+            for (const auto& a : aggs_) {
+                const std::string dst = "_ka_row + " + s(a.wordOffset);
+                const std::string v = "(i64)(" + a.valueExpr + ")";
+                switch (a.kind) {
+                    case AggKind::SumI64: cg.addLine("gdb_atomic_add_u64(" + dst + ", (u64)" + v + ");"); break;
+                    case AggKind::SumI128:
+                        cg.addBlock("", [&]() {
+                            cg.addLine("const i64 _sv = " + v + ";");
+                            cg.addLine("gdb_atomic_add_u64(" + dst + ", (u64)(u32)_sv);");
+                            cg.addLine("gdb_atomic_add_u64(_ka_row + " + s(a.wordOffset + 1) + ", (u64)(_sv >> 32));");
+                        });
+                        break;
+                    case AggKind::Count: cg.addLine("gdb_atomic_add_u64(" + dst + ", 1ull);"); break;
+                    case AggKind::MinI64: cg.addLine("gdb_atomic_min_i64((i64*)(" + dst + "), " + v + ");"); break;
+                    case AggKind::MaxI64: cg.addLine("gdb_atomic_max_i64((i64*)(" + dst + "), " + v + ");"); break;
+                }
+            }
+            consume();
+        });
+    } else if (regs) {
         cg.addComment("KeyedAgg " + out_ + ": register accumulators, " + NB + " bucket(s)");
         for (int b = 0; b < numBuckets_; ++b) {
             for (const auto& a : aggs_) {

@@ -423,15 +423,19 @@ private:
 // every row of a key) makes the flags a direct-address byte payload (e.g. an
 // order's priority class), as the reference's ArrayStore
 // (metal_operators.cpp:185-200) with byte values.
+// tested=true reads the flag first and skips the store when it already
+// holds the value: for skewed keys (a Zipf column), whose hot keys' lines
+// then stay clean in cache instead of being rewritten by every block.
 class HipByteFlagBuild : public HipUnaryOperator {
 public:
     HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,
-                     const std::string& sizeExpr, const std::string& valueExpr = "1");
+                     const std::string& sizeExpr, const std::string& valueExpr = "1", bool tested = false);
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string flags_, key_, size_, value_;
+    bool tested_ = false;
 };
 
 // Key-indexed u32 store array[key] = value (zeroed before its phase, so 0

@@ -37,7 +37,7 @@ const std::vector<CatalogTable>& catalog() {
         {"B", {{"b_key", I32}, {"b_fkey", I32}, {"b_date", I32}}},
         {"S", {{"s_fkey", I32}, {"s_text", CHAR25}}},
         {"G", {{"g_key", I32}, {"g_val", I64}}},
-        {"GBBM", {{"gbbm_word", I32}}},
+        {"GBFL", {{"gbfl_flag", U8}}},
         {"P", {{"p_key", I32}, {"p_date", I32}, {"p_val", I64}, {"p_flag", U8}}},
         // TPC-H tables as loaded from .colbin (tpch_schema.h:66-162): INT and
         // DATE (YYYYMMDD) int32, FLOAT as exact int64 hundredths, CHAR1 one
@@ -278,7 +278,7 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     if (prepass) {
         gbCountPhases(plan, filter);
         plan.sizingRules.push_back(
-            {"GBC_popcount", "gb_slots", "d_gb_distinct", 0, 2, part ? static_cast<uint64_t>(PartitionConfig::kPartSlots) : 64u, true});
+            {"GBC_count", "gb_slots", "d_gb_distinct", 0, 2, part ? static_cast<uint64_t>(PartitionConfig::kPartSlots) : 64u, true});
     }
     auto scan = makeScan("G", {"g_key", "g_val"});
     std::unique_ptr<HipOperator> cur = std::move(scan);
@@ -342,9 +342,13 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     return plan;
 }
 
-// GB_COUNT — exact distinct-key count (the plain form's pre-pass): set bit
-// (key - gb_key_min) in a key-range bitmap, then popcount the bitmap words
-// (the bitmap is scanned as table GBBM). Counterpart of the exact max-key
+// GB_COUNT — exact distinct-key count (the plain form's pre-pass): a byte
+// flag per key of the range, set by (key - gb_key_min) with plain stores
+// after a plain read (no atomics: every writer stores the same byte, and hot
+// keys of a skewed column find their flag set in cache), then the set flags
+// counted (the flags are scanned as table GBFL). On 120M Zipf(1.1) keys over
+// 20M this measured 0.49 ms against 1.7-4.2 ms for bitmaps set by device-scope
+// atomic ORs (one bitmap, or one per XCD). Counterpart of the exact max-key
 // pre-pass (codegen/execution/max_key_symbols.cpp:188-235).
 HipQueryPlan buildGbCount(const PlanOptions& o) {
     HipQueryPlan plan;
@@ -359,19 +363,19 @@ void gbCountPhases(HipQueryPlan& plan, bool filter) {
     {
         std::unique_ptr<HipOperator> cur = filter ? makeScan("G", {"g_key", "g_val"}) : makeScan("G", {"g_key"});
         if (filter) cur = std::make_unique<HipSelection>(std::move(cur), "g_val < gb_val_cut");
-        auto bm = std::make_unique<HipBitmapBuild>(std::move(cur), "d_gb_key_bitmap", "(i64)g_key - gb_key_min",
-                                                   "gb_key_bits", true);
-        auto& ph = appendPhase(plan, "GBC_bitmap", std::move(bm), 256, 8);
+        auto fl = std::make_unique<HipByteFlagBuild>(std::move(cur), "d_gb_key_flags", "(i64)g_key - gb_key_min",
+                                                     "gb_key_bits", "1", true);
+        auto& ph = appendPhase(plan, "GBC_flags", std::move(fl), 256, 8);
         ph.scalarParams = {"gb_key_min"};
         if (filter) ph.scalarParams.push_back("gb_val_cut");
     }
     {
-        auto scan = makeScan("GBBM", {"gbbm_word"});
+        auto scan = makeScan("GBFL", {"gbfl_flag"});
         auto agg = std::make_unique<HipKeyedAgg>(std::move(scan), "d_gb_distinct", "0", 1);
-        agg->addAggregate("distinct_keys", AggKind::SumI64, "__popc((u32)gbbm_word)", 0);
-        appendPhase(plan, "GBC_popcount", std::move(agg), 256, 8);
+        agg->addAggregate("distinct_keys", AggKind::SumI64, "(i64)(gbfl_flag != 0)", 0);
+        appendPhase(plan, "GBC_count", std::move(agg), 256, 16, {16});
     }
-    plan.bufferTables.push_back({"GBBM", "gbbm_word", "d_gb_key_bitmap", "(gb_key_bits + 31) / 32 + 1"});
+    plan.bufferTables.push_back({"GBFL", "gbfl_flag", "d_gb_key_flags", "gb_key_bits"});
 }
 
 // MAXKEY — min and max of one int32 column (table=..., column=...).

@@ -39,6 +39,9 @@ struct HipQueryPlan {
         // Software-pipelined scan: the next tile's columns load while this
         // tile is processed.
         bool prefetch = false;
+        // Out-of-core runs: this phase scans the stream table after earlier
+        // stream phases have accumulated over every chunk (a second pass).
+        bool newStreamPass = false;
         std::vector<std::string> scalarParams;  // registered before the operators run
         // Buffers the caller registers (e.g. LIKE words, prebuilt bitmaps):
         // (name, element type) read-only parameters.

@@ -599,6 +599,10 @@ HipQueryPlan tpchQ11(const PlanOptions& o) {
     }
     plan.requiredScalars = {"q11_frac_den"};
     plan.requiredSymbols = {"n_bits", "s_bits", "p_bits"};
+    // partsupp streams (Q11_value only accumulates: key-indexed atomic sums
+    // and the total), as the reference's Tier C streams partsupp for Q11
+    // (metal_plan_builder.cpp:2089-2102)
+    plan.chunkable = true;
     return plan;
 }
 
@@ -678,9 +682,13 @@ HipQueryPlan tpchQ17(const PlanOptions& o) {
         auto agg = std::make_unique<HipKeyedAgg>(std::move(sel), "d_q17_sum", "0", 1);
         agg->addAggregate("sum_price", AggKind::SumI64, "l_extendedprice", 2);
         agg->addAggregate("count", AggKind::Count, "1", 0);
-        appendPhase(plan, "Q17_small_qty", std::move(agg), 256, optInt(o, "items", 4));
+        auto& ph = appendPhase(plan, "Q17_small_qty", std::move(agg), 256, optInt(o, "items", 4));
+        ph.newStreamPass = true;  // reads every part's quantity sum and count over the whole of lineitem
     }
     plan.requiredSymbols = {"p_bits"};
+    // streamed in two passes over lineitem: per-part sums and counts, then
+    // the rows below a fifth of their part's average
+    plan.chunkable = true;
     return plan;
 }
 

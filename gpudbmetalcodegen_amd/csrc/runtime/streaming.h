@@ -6,7 +6,10 @@
 // codegen/execution/chunked_colbin_loader.h:12-60, and the chunk loop of
 // codegen/codegen_main.cpp:686-808): phases before the first phase that scans
 // the stream table run once, the stream phases run once per row chunk, the
-// phases after them run once. Differences that follow from the target:
+// phases after them run once. A stream phase that needs what earlier stream
+// phases accumulated over the whole table (plan phase flag newStreamPass)
+// starts a second pass over the chunks; phases between passes run once.
+// Differences that follow from the target:
 //   - chunks are copied host->HBM with asynchronous copies on a copy stream
 //     into two device slots per column (double buffering), so the copy of
 //     chunk c+1 overlaps the kernels of chunk c; the compute stream waits on
@@ -14,6 +17,10 @@
 //     event, and the host never blocks inside the loop;
 //   - host columns should be pinned (gdb_host_alloc_pinned) for the copies to
 //     run at link speed and overlap; pageable memory works, serialised;
+//   - a column given by a producer (a .colbin file read chunk by chunk,
+//     chunked_colbin_loader.cpp:167-194) is written into one of two pinned
+//     staging slots while the device works on the previous chunk; the host
+//     waits only for the copy that last used the slot;
 //   - buffers of the stream phases are initialised before the first chunk
 //     only (the runner's skip-init for later chunks), and overflow status is
 //     checked once after the last chunk.
@@ -23,6 +30,7 @@
 #include "../planning/hip_plan_builder.h"
 #include "gdb/gdb_hip.h"
 
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -31,8 +39,12 @@ namespace runtime {
 
 struct StreamColumn {
     std::string name;
-    const void* host = nullptr;
+    const void* host = nullptr;  // the engine-format column in host memory, or
     int elemBytes = 0;
+    // ... a producer that writes rows [r0, r0 + n) in the engine's format to
+    // dst (pinned staging the streamer owns): file-backed columns converted
+    // chunk by chunk (the table is never held whole in host memory)
+    std::function<void(void* dst, uint64_t r0, uint64_t n)> produce;
 };
 
 struct StreamStats {
@@ -41,6 +53,7 @@ struct StreamStats {
     uint64_t chunks = 0;
     uint64_t bytesCopied = 0;
     int firstStreamPhase = 0, lastStreamPhase = 0;  // [first, last)
+    int passes = 0;                                   // passes over the stream table
 };
 
 class ChunkStreamer {
@@ -69,6 +82,7 @@ private:
     gdb_event copied_[2] = {nullptr, nullptr}, consumed_[2] = {nullptr, nullptr};
     gdb_event t0_ = nullptr, t1_ = nullptr;
     std::vector<void*> slots_[2];
+    std::vector<void*> staging_[2];  // pinned host slots of produced columns
     void freeSlots();
 };
 

@@ -338,6 +338,16 @@ class HipGenericExecutor:
                                                          ptrs, eb, int(chunk_rows), C.byref(st)))
         return st
 
+    def execute_streamed_colbin(self, plan: "HipQueryPlan", table: str, colbin_path, chunk_rows: int) -> N.ChunkStats:
+        """Out-of-core run with `table` read from its .colbin file chunk by
+        chunk (converted to the engine's formats in pinned staging); the
+        table is never held whole in host memory."""
+        st = N.ChunkStats()
+        N.eng_check(_eng().gdb_executor_execute_streamed_colbin(self._h, plan.handle, table.encode(),
+                                                                str(colbin_path).encode(), int(chunk_rows),
+                                                                C.byref(st)))
+        return st
+
     def set_skip_init(self, skip: bool) -> None:
         N.eng_check(_eng().gdb_executor_set_skip_init(self._h, 1 if skip else 0))
 

@@ -34,6 +34,7 @@ import numpy as np
 from . import _native as N
 from . import engine as E
 from . import queries as Q
+from . import stats as ST
 from . import workloads as W
 
 KINDS = ("scan", "join", "anti", "like")
@@ -65,12 +66,9 @@ class Query:
 
 
 def percentile(xs, p):
-    """Nearest-rank percentile (the reference's trial statistics, codegen_main.cpp:1012-1020)."""
-    v = sorted(xs)
-    if not v:
-        return 0.0
-    i = int(min(max(np.ceil(p * len(v)) - 1, 0), len(v) - 1))
-    return v[i]
+    """Nearest-rank percentile as the reference's trial statistics take it
+    (codegen_main.cpp:1012-1020; stats.percentile)."""
+    return ST.percentile(list(xs), p)
 
 
 class ResidentDB:
@@ -289,6 +287,7 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
         "rows_per_s": rows / makespan if makespan else 0.0, "max_inflight": state["max_inflight"],
         "errors": sum(1 for q in stream if q.error),
         "per_kind_p99_ms": {k: percentile([q.latency_ms for q in stream if q.kind == k], 0.99) for k in KINDS},
+        "latency_ms": ST.trial_stats(lat),
     }
     if own:
         for w in pool:

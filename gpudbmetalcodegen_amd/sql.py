@@ -104,12 +104,20 @@ def finish(layout: dict, cols: dict, host: dict | None) -> tuple[dict, dict]:
     def agg_scale(name: str) -> int:
         return int(cols.get("__scales", {}).get(name, 0))
 
+    # An aggregate without GROUP BY over no rows is still one row (SQL):
+    # COUNT 0, every other aggregate NULL (None), and NULL propagates.
+    empty_global = n == 0 and not keys
+
     def ev(e: dict, r: int):
         if "key" in e:
             return key_value(e["key"], r)
         if "agg" in e:
+            if empty_global:
+                return Fraction(0) if e["agg"] == "_n" else None
             return Fraction(int(cols[e["agg"]][r]), 10 ** agg_scale(e["agg"]))
         if "avg" in e:
+            if empty_global:
+                return None
             num, den = e["avg"]
             return Fraction(int(cols[num][r]), 10 ** agg_scale(num)) / int(cols[den][r])
         if "const" in e:
@@ -123,6 +131,8 @@ def finish(layout: dict, cols: dict, host: dict | None) -> tuple[dict, dict]:
             vals = (ev(a, r) for a in e["args"])
             return all(vals) if op == "and" else any(vals)
         args = [ev(a, r) for a in e["args"]]
+        if any(a is None for a in args):
+            return None
         if op == "not":
             return not args[0]
         if op in _CMP:
@@ -142,7 +152,7 @@ def finish(layout: dict, cols: dict, host: dict | None) -> tuple[dict, dict]:
     targets = layout["targets"]
     having = layout.get("having")
     rows = []
-    for r in range(n):
+    for r in range(1 if empty_global else n):
         if having is not None and not ev(having, r):
             continue
         vals = [ev(t["expr"], r) for t in targets]
@@ -156,10 +166,12 @@ def finish(layout: dict, cols: dict, host: dict | None) -> tuple[dict, dict]:
 
     def cmp(a, b):
         for x, y, d in zip(a[1], b[1], desc):
+            if x is None or y is None:
+                continue
             if x != y:
                 return (1 if x < y else -1) if d else (-1 if x < y else 1)
         for x, y in zip(a[0], b[0]):  # deterministic order for ties and no ORDER BY
-            if x != y:
+            if x is not None and y is not None and x != y:
                 return -1 if x < y else 1
         return 0
 
@@ -176,7 +188,7 @@ def finish(layout: dict, cols: dict, host: dict | None) -> tuple[dict, dict]:
             out[name] = [int(v[c]) for v, _ in rows]
             fmts[name] = (0, True)
         else:
-            out[name] = [_trunc(v[c], t["scale"]) for v, _ in rows]
+            out[name] = [None if v[c] is None else _trunc(v[c], t["scale"]) for v, _ in rows]
             fmts[name] = (t["scale"], False)
     return out, fmts
 
@@ -228,3 +240,30 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
     run = SqlRun(sql, out, fmts, t, plan, layout)
     run.canonical = T.canonical(out, fmts)
     return run
+
+
+# The reference's seven scan microbenchmarks over lineitem (sql/mb1.sql ...
+# sql/mb7.sql; dispatched as chunkable single-table reductions,
+# codegen/planning/metal_plan_builder.cpp:2104): column footprint, filter
+# selectivity and arithmetic against bandwidth. Each is one fused
+# scan-filter-reduce kernel here. MB1 sums l_returnflag's byte codes.
+MICROBENCH = {
+    "MB1": "select sum(l_returnflag) as s from lineitem",
+    "MB2": "select sum(l_extendedprice) as total from lineitem",
+    "MB3": "select sum(l_extendedprice) as revenue from lineitem "
+           "where l_shipdate >= date '1994-01-01' and l_shipdate < date '1996-01-01'",
+    "MB4": "select sum(l_extendedprice) as revenue from lineitem "
+           "where l_shipdate >= date '1998-09-01' and l_shipdate < date '1998-10-01'",
+    "MB5": "select sum(l_extendedprice) as revenue from lineitem where l_shipdate >= date '1992-01-01'",
+    "MB6": "select sum(l_extendedprice * (1 - l_discount) * (1 + l_tax)) as charge from lineitem",
+    "MB7": "select sum(l_quantity + l_extendedprice + l_discount + l_tax) as s from lineitem "
+           "where l_shipdate >= date '1992-01-01'",
+}
+
+# lineitem columns each microbenchmark reads (its algorithmic bytes per row)
+MICROBENCH_COLUMNS = {
+    "MB1": ["l_returnflag"], "MB2": ["l_extendedprice"], "MB3": ["l_shipdate", "l_extendedprice"],
+    "MB4": ["l_shipdate", "l_extendedprice"], "MB5": ["l_shipdate", "l_extendedprice"],
+    "MB6": ["l_extendedprice", "l_discount", "l_tax"],
+    "MB7": ["l_shipdate", "l_quantity", "l_extendedprice", "l_discount", "l_tax"],
+}

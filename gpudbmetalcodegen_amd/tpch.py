@@ -313,10 +313,16 @@ class TpchRun:
     canonical: str = field(default="")
 
 
+STREAM_TABLE = {"Q11": "partsupp"}
+
+
 def stream_table(query: str) -> str:
     """The table a chunked run streams: lineitem where the query reads it,
+    partsupp for Q11 (the reference's Tier C, metal_plan_builder.cpp:2089-2102),
     otherwise orders (the reference's chunk loop streams one table per query,
     codegen_main.cpp:686-808)."""
+    if query in STREAM_TABLE:
+        return STREAM_TABLE[query]
     return "lineitem" if "lineitem" in SCANNED[query] else "orders"
 
 
@@ -330,14 +336,85 @@ def parse_rows(text: str) -> int:
     return n
 
 
+# Automatic chunking (codegen_main.cpp:453-624): a chunkable plan streams its
+# stream table when the projected working set (the columns its kernels read,
+# resident tables plus the whole stream table) exceeds 3/4 of the memory
+# budget; the chunk then takes (budget/2 - resident bytes) over two slots,
+# at least budget/8 and 64 MiB, at least 256K rows and at most the table.
+# An explicit chunk size for a working set that fits is downgraded to a
+# direct run unless forced (the reference's GPUDB_FORCE_CHUNK). The budget
+# here is free HBM (the reference takes min(physical memory, the Metal
+# working-set limit) of its unified memory).
+AUTO_CHUNK_THRESHOLD = 0.75
+AUTO_CHUNK_BUDGET_FRACTION = 0.50
+AUTO_CHUNK_MIN_ROWS = 256 * 1024
+AUTO_CHUNK_MIN_STREAM_BYTES = 64 << 20
+
+
+def chunk_decision(resident_bytes: int, stream_rows: int, stream_bytes_per_row: int, budget_bytes: int,
+                   requested_rows: int = 0, force: bool = False) -> tuple[int, str]:
+    """Chunk rows for one chunkable query (0 = run directly) and the reason."""
+    ws = resident_bytes + stream_rows * stream_bytes_per_row
+    fits = budget_bytes > 0 and ws <= int(budget_bytes * AUTO_CHUNK_THRESHOLD)
+    if requested_rows:
+        if fits and not force:
+            return 0, (f"requested chunk of {requested_rows} rows downgraded to a direct run: working set "
+                       f"{ws / 1e9:.2f} GB fits in {budget_bytes * AUTO_CHUNK_THRESHOLD / 1e9:.2f} GB")
+        return requested_rows, "requested"
+    if fits:
+        return 0, f"working set {ws / 1e9:.2f} GB fits"
+    bpr = max(1, stream_bytes_per_row)
+    stream_budget = int(budget_bytes * AUTO_CHUNK_BUDGET_FRACTION) - resident_bytes
+    stream_budget = max(stream_budget, budget_bytes // 8, AUTO_CHUNK_MIN_STREAM_BYTES)
+    rows = max(stream_budget // (bpr * 2), AUTO_CHUNK_MIN_ROWS)
+    rows = min(rows, max(1, stream_rows))
+    return int(rows), (f"working set {ws / 1e9:.2f} GB exceeds {budget_bytes * AUTO_CHUNK_THRESHOLD / 1e9:.2f} GB: "
+                       f"chunks of {rows} rows ({rows * bpr / 2**20:.0f} MiB a slot)")
+
+
+def projected_bytes_per_row(plan_source: str, columns: dict) -> int:
+    """Bytes a row of one table contributes to the working set: the columns
+    the plan's kernels take (p_<column> parameters)."""
+    n = 0
+    for c, a in columns.items():
+        if re.search(r"\bp_" + c + r"\b", plan_source):
+            n += a.itemsize * (a.shape[1] if a.ndim == 2 else 1)
+    return n
+
+
+def auto_chunk_rows(db: "TpchDB", query: str, plan: E.HipQueryPlan, requested_rows: int = 0,
+                    budget_bytes: int | None = None, force: bool | None = None) -> tuple[int, str]:
+    """chunk_decision for a named query on db's tables; the budget defaults
+    to the device's free HBM, force to GDB_FORCE_CHUNK=1 in the environment."""
+    if not plan.chunkable:
+        return 0, "plan not chunkable"
+    if force is None:
+        force = os.environ.get("GDB_FORCE_CHUNK", "0") not in ("", "0")
+    src = plan.source()
+    stream = stream_table(query)
+    resident = 0
+    for t in SCANNED[query]:
+        if t != stream:
+            cols = db.host[t]
+            resident += int(next(iter(cols.values())).shape[0]) * projected_bytes_per_row(src, cols)
+    scols = db.host[stream]
+    rows = int(next(iter(scols.values())).shape[0])
+    if budget_bytes is None:
+        budget_bytes = db.ctx.mem_info()[0]
+    return chunk_decision(resident, rows, projected_bytes_per_row(src, scols), budget_bytes, requested_rows, force)
+
+
 def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int = 1, options: str = "",
-        executor: E.HipGenericExecutor | None = None, params: dict | None = None, chunk_rows: int = 0) -> TpchRun:
+        executor: E.HipGenericExecutor | None = None, params: dict | None = None, chunk_rows: int = 0,
+        colbin_dir=None) -> TpchRun:
     """Run a named TPC-H query; the result columns follow the SQL select
     list. params overrides constants of the query (e.g. q18_qty).
     chunk_rows > 0: out-of-core form; the stream table stays in host memory
     and its phases run once per chunk of that many rows, copied into HBM
     with double-buffered copies (plans not marked chunkable are refused, as
-    codegen_main.cpp:320-335 refuses an explicit --chunk)."""
+    codegen_main.cpp:320-335 refuses an explicit --chunk). With colbin_dir,
+    the stream table's chunks are read from <colbin_dir>/<table>.colbin
+    instead of host memory."""
     if query not in QUERIES:
         raise ValueError(f"unknown TPC-H query {query}; known: {QUERIES}")
     opts = ",".join(o for o in (PLAN_OPTIONS.get(query, ""), options) if o)
@@ -386,6 +463,10 @@ def run(ctx: E.Context, db: TpchDB, query: str, warmup: int = 0, measured: int =
             ex.register_buffer("d_str_consts", strings_ptr, buf.nbytes)
         if streamed is None:
             t = ex.execute(plan, warmup, measured)
+        elif colbin_dir is not None:
+            st = ex.execute_streamed_colbin(plan, streamed, Path(colbin_dir) / f"{streamed}.colbin", chunk_rows)
+            t = E.Timing(st.total_ms, 0.0, [], [])
+            t.chunks, t.stream_ms, t.bytes_copied = int(st.chunks), float(st.stream_ms), int(st.bytes_copied)
         else:
             # only the columns the plan's kernels take cross the link
             src = plan.source()
@@ -548,6 +629,8 @@ def fixed_str(v: int, scale: int) -> str:
 
 
 def _field(v, scale: int, is_date: bool) -> str:
+    if v is None:  # SQL NULL (an aggregate over no rows): an empty field
+        return ""
     if isinstance(v, str):
         if any(ch in v for ch in ',"\n\r'):
             return '"' + v.replace('"', '""') + '"'

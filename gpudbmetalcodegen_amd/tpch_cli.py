@@ -25,6 +25,7 @@ analysis-based path from sql/q<N>.sql text.
 from __future__ import annotations
 
 import argparse
+import os
 import statistics
 import sys
 import time
@@ -56,6 +57,9 @@ def parse(argv):
     ap.add_argument("--check-abs-tol", type=float, default=T.CHECK_ABS_TOL)
     ap.add_argument("--check-rel-tol", type=float, default=T.CHECK_REL_TOL)
     ap.add_argument("--threadgroup-size", type=int, default=None)
+    ap.add_argument("--scalar-atomic", action="store_true",
+                    help="reduction ablation: one global atomic per row instead of the block reduction "
+                         "(sets GDB_SCALAR_ATOMIC=1, as the reference's --scalar-atomic)")
     ap.add_argument("--chunk", default=None, help="N[K|M|G]: stream the query's largest table in chunks of N rows")
     return ap.parse_args(argv)
 
@@ -104,6 +108,8 @@ def main(argv=None) -> int:
     if not queries and not sql_files:
         print("usage: tpch_cli [flags] [sf1|sf10|sf20|sf50|sf100] q<N>|all | --sql FILE", file=sys.stderr)
         return 3
+    if a.scalar_atomic:
+        os.environ["GDB_SCALAR_ATOMIC"] = "1"
     data_dir = a.data_dir or data_dir
     try:
         chunk = T.parse_rows(a.chunk) if a.chunk else 0
@@ -179,13 +185,22 @@ def main(argv=None) -> int:
                       file=sys.stderr)
                 status = max(status, 3)
                 continue
-            run = T.run(ctx, db, q, warmup=a.warmup, measured=1, options=opts, executor=ex, chunk_rows=chunk)
+            # the reference's chunk decision: automatic when the working set
+            # does not fit, an explicit --chunk downgraded when it does
+            # (GDB_FORCE_CHUNK=1 keeps it)
+            q_chunk, why = T.auto_chunk_rows(db, q, plan, requested_rows=chunk)
+            if plan.chunkable and (q_chunk != chunk or q_chunk) and not a.csv:
+                print(f"[auto-chunk] {q}: {why}")
+            # chunks of tables read from .colbin files come straight from the file
+            cdir = data_dir if (q_chunk and a.generate is None) else None
+            run = T.run(ctx, db, q, warmup=a.warmup, measured=1, options=opts, executor=ex, chunk_rows=q_chunk,
+                        colbin_dir=cdir)
             times = [run.timing.total_ms]
             for _ in range(max(0, a.repeat - 1)):
                 times.append(T.run(ctx, db, q, warmup=0, measured=1, options=opts, executor=ex,
-                                   chunk_rows=chunk).timing.total_ms)
-            if chunk and not a.csv:
-                print(f"{q}: streamed {T.stream_table(q)} in {run.timing.chunks} chunks of {chunk} rows, "
+                                   chunk_rows=q_chunk, colbin_dir=cdir).timing.total_ms)
+            if q_chunk and not a.csv:
+                print(f"{q}: streamed {T.stream_table(q)} in {run.timing.chunks} chunks of {q_chunk} rows, "
                       f"{run.timing.bytes_copied / 1e6:.1f} MB copied")
             med = statistics.median(times)
             if a.csv:

@@ -143,6 +143,16 @@ typedef struct {
 gdb_status gdb_executor_execute_streamed(gdb_executor ex, gdb_plan p, const char* table, uint64_t rows, int ncols,
                                          const char* const* names, const void* const* host_cols,
                                          const int* elem_bytes, uint64_t chunk_rows, gdb_chunk_stats* out);
+/* The same with the stream table read from its .colbin file (a TPC-H table
+ * of the catalog): each chunk of the columns the stream phases read is
+ * converted from the file's format into the engine's (FLOAT -> int64
+ * hundredths) in pinned staging while the previous chunk's kernels run, so
+ * the table is never loaded whole into host memory. A plan whose later
+ * stream phase needs the whole table first streams it once per pass
+ * (chunks counts every pass). Replaces ChunkedColbinTable's per-chunk reads
+ * (codegen/execution/chunked_colbin_loader.cpp:167-194). */
+gdb_status gdb_executor_execute_streamed_colbin(gdb_executor ex, gdb_plan p, const char* table, const char* colbin_path,
+                                                uint64_t chunk_rows, gdb_chunk_stats* out);
 /* 1 when the plan may be streamed in chunks (metal_plan_builder.cpp:2078-2107). */
 gdb_status gdb_plan_chunkable(gdb_plan p, int* chunkable);
 gdb_status gdb_executor_prepare(gdb_executor ex, gdb_plan p);

@@ -49,7 +49,8 @@ def served_expected(T: dict, kind: str, params: dict, max_a: int, threads: int =
     if kind == "anti":
         cnt, sm = oracle.q22(T["B"], T["A"], max_a, params["lo"], params["hi"])
         segs = [i for i in range(5) if cnt[i] > 0]
-        return {"a_seg": segs, "num_keys": [int(cnt[i]) for i in segs], "sum_keys": [int(sm[i]) for i in segs]}
+        # the plan labels its five segment buckets "0".."4" (string labels)
+        return {"a_seg": [str(i) for i in segs], "num_keys": [int(cnt[i]) for i in segs], "sum_keys": [int(sm[i]) for i in segs]}
     if kind == "like":
         h = oracle.q13(T["S"], T["A"], max_a, params["w1"], params["w2"], threads=threads)
         return {k: [int(x) for x in v] for k, v in h.items()}

@@ -146,3 +146,18 @@ def test_chunkable_plans():
         assert E.build_hip_plan(name, opts).chunkable, name
     for name, opts in [("GROUPBY", "sizing=prepass"), ("GB_COUNT", ""), ("MAXKEY", "table=G,column=g_key")]:
         assert not E.build_hip_plan(name, opts).chunkable, name
+
+
+def test_scalar_atomic_ablation_changes_the_reduction(monkeypatch):
+    """GDB_SCALAR_ATOMIC=1 (the reference's GPUDB_SCALAR_ATOMIC ablation):
+    small-bucket aggregates update the output with one global atomic per
+    row instead of the register + wave + LDS block reduction."""
+    from gpudbmetalcodegen_amd import engine as E
+
+    base = E.build_hip_plan("Q1", "").source()
+    assert "block reduction: wave DPP sums" in base and "reduction ablation" not in base
+    monkeypatch.setenv("GDB_SCALAR_ATOMIC", "1")
+    abl = E.build_hip_plan("Q1", "").source()
+    assert "reduction ablation" in abl and "block reduction: wave DPP sums" not in abl
+    monkeypatch.setenv("GDB_SCALAR_ATOMIC", "0")
+    assert E.build_hip_plan("Q1", "").source() == base

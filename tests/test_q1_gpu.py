@@ -74,3 +74,26 @@ def test_q1_full_6m(ctx):
     assert got == q1_expected(oracle.q1(F, W.q1_cutoff()))
     assert sum(got["count_order"]) == int((F["f_date"] <= W.q1_cutoff()).sum())
     print("Q1 6M timing ms", run.timing.total_ms, run.timing.phases)
+
+
+def test_scalar_atomic_ablation_is_bit_exact(ctx, monkeypatch):
+    """The reduction ablation (one global atomic per row and slot,
+    GDB_SCALAR_ATOMIC=1) gives the same exact Q1 as the block reduction."""
+    from gpudbmetalcodegen_amd import engine as E
+
+    F = W.fact_f(200_000, W.seed_for(0) + 7)
+    dev = Q.upload_tables(ctx, {"F": F})
+    try:
+        monkeypatch.setenv("GDB_SCALAR_ATOMIC", "1")
+        plan = E.build_hip_plan("Q1", "").compile(ctx)
+        assert "reduction ablation" in plan.source()
+        ex = E.HipGenericExecutor(ctx)
+        ex.register_table(dev["F"])
+        ex.register_scalar("q1_cutoff", W.q1_cutoff())
+        ex.execute(plan, 0, 1)
+        got = ex.collect(plan).columns
+        ex.close()
+        exp = q1_expected(oracle.q1(F, W.q1_cutoff()))
+        assert as_lists(got) == exp
+    finally:
+        dev["F"].free(ctx)

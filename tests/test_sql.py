@@ -229,3 +229,31 @@ def test_single_int_key_is_direct_addressed():
     assert [k.get("via") for k in lay["keys"]] == [None, 0, 0, 0, 0, 0, 0]
     assert lay["keys"][4]["path"] == [{"table": "customer", "key": "c_custkey"},
                                       {"table": "nation", "key": "n_nationkey", "fk": "c_nationkey"}]
+
+
+@pytest.mark.parametrize("mb", sorted(S.MICROBENCH))
+def test_microbenchmarks_plan_as_one_fused_reduction(mb):
+    """The seven scan microbenchmarks each plan as one scan-filter-reduce
+    phase that reads exactly the listed lineitem columns."""
+    p = E.build_sql_plan(S.MICROBENCH[mb])
+    assert p.phases() == ["SQL_reduce"]
+    src = p.source()
+    for c in S.MICROBENCH_COLUMNS[mb]:
+        assert c in src, (mb, c)
+    others = {"l_shipdate", "l_quantity", "l_extendedprice", "l_discount", "l_tax", "l_returnflag"}
+    for c in others - set(S.MICROBENCH_COLUMNS[mb]):
+        assert f"p_{c}" not in src and f" {c}[" not in src, (mb, c)
+
+
+def test_global_aggregate_over_no_rows_is_one_null_row():
+    """SELECT SUM(x), COUNT(*) ... WHERE <nothing qualifies>: one row, the sum
+    NULL and the count 0 (the collector holds no group for it)."""
+    layout = {"keys": [], "targets": [{"name": "s", "type": "num", "scale": 2, "expr": {"agg": "a1"}},
+                                      {"name": "c", "type": "num", "scale": 0, "expr": {"agg": "_n"}},
+                                      {"name": "avg_s", "type": "num", "scale": 8,
+                                       "expr": {"op": "/", "args": [{"agg": "a1"}, {"agg": "_n"}], "scale": 8}}],
+              "order": [], "limit": -1}
+    out, fmts = S.finish(layout, {"_n": [], "a1": []}, None)
+    assert out == {"s": [None], "c": [0], "avg_s": [None]}
+    from gpudbmetalcodegen_amd import tpch as T
+    assert T.canonical(out, fmts) == "s,c,avg_s\n,0,\n"

@@ -163,6 +163,8 @@ QUERIES = [
 
 
 def _cell(v, scale, is_date):
+    if v is None:
+        return None
     if isinstance(v, str):
         return v
     if is_date:
@@ -211,3 +213,34 @@ def test_cli_runs_sql_files(tmp_path, capsys):
     assert tpch_cli.main(base + ["--check", str(tmp_path)]) == 0
     assert "[CHECK] q6 @ GEN: OK" in capsys.readouterr().out
     assert tpch_cli.main(["--generate", "0.01", "--sql", str(tmp_path / "missing.sql")]) == 3
+
+
+@pytest.mark.parametrize("mb", ["MB2", "MB3", "MB4", "MB5", "MB6", "MB7"])
+def test_microbench_sql_matches_sqlite(env, mb):
+    from test_tpch import _sqlite
+
+    ctx, db, host = env
+    sql = S.MICROBENCH[mb]
+    print(f"microbenchmark {mb}", flush=True)
+    r = S.run_sql(ctx, db, sql)
+    (want,) = _sqlite(host).execute(sql.replace("date '", "'")).fetchall()[0]
+    (name,) = list(r.columns)
+    assert len(r.columns[name]) == 1                 # one row, also when no row qualifies
+    got = _cell(r.columns[name][0], *r.formats[name])
+    if want is None:                                 # SUM over no rows is NULL
+        assert got is None, (mb, got)
+    else:
+        assert abs(got - want) <= 1e-6 + 1e-9 * abs(want), (mb, got, want)
+
+
+def test_microbench_mb1_sums_flag_byte_codes(env):
+    """MB1 sums l_returnflag as byte codes ('A' 65, 'N' 78, 'R' 82): the
+    count of each flag times its code."""
+    import numpy as np
+
+    ctx, db, host = env
+    r = S.run_sql(ctx, db, S.MICROBENCH["MB1"])
+    flags = np.asarray(host["lineitem"]["l_returnflag"]).astype(np.int64).ravel()
+    assert set(np.unique(flags).tolist()) == {ord("A"), ord("N"), ord("R")}
+    assert r.columns["s"] == [int(flags.sum())]
+    assert r.formats["s"] == (0, False)

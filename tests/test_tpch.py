@@ -478,3 +478,33 @@ def test_chunkable_plans_and_stream_tables():
     got = [q for q in T.QUERIES if E.build_hip_plan(q, T.PLAN_OPTIONS.get(q, "")).chunkable]
     assert got == CHUNKED
     assert [q for q in CHUNKED if T.stream_table(q) == "orders"] == ["Q13", "Q22"]
+
+
+def test_chunk_decision_follows_the_reference_policy():
+    """codegen_main.cpp:453-624: chunk when the working set exceeds 3/4 of
+    the budget; rows = (budget/2 - resident) / (2 slots x bytes/row), at least
+    budget/8 and 64 MiB of stream budget and 256K rows, at most the table;
+    an explicit chunk for a working set that fits is downgraded unless forced."""
+    GiB = 1 << 30
+    # fits: 1 GiB resident + 100M rows x 20 B = 3 GiB-ish < 0.75 x 16 GiB
+    assert T.chunk_decision(GiB, 100_000_000, 20, 16 * GiB)[0] == 0
+    r, why = T.chunk_decision(GiB, 100_000_000, 20, 16 * GiB, requested_rows=1 << 20)
+    assert r == 0 and "downgraded" in why
+    assert T.chunk_decision(GiB, 100_000_000, 20, 16 * GiB, requested_rows=1 << 20, force=True)[0] == 1 << 20
+    # does not fit: 1 GiB resident + 1e9 rows x 20 B against 16 GiB
+    r, why = T.chunk_decision(GiB, 1_000_000_000, 20, 16 * GiB)
+    assert r == (8 * GiB - GiB) // 40 and "exceeds" in why          # 187,904,819 rows
+    # resident tables eat the half budget: the 1/8 floor holds
+    r, _ = T.chunk_decision(10 * GiB, 1_000_000_000, 20, 16 * GiB)
+    assert r == (2 * GiB) // 40
+    # tiny budget: 64 MiB floor, then the 256K-row floor, capped at the table
+    assert T.chunk_decision(0, 50_000_000, 64, 100 << 20)[0] == max((64 << 20) // 128, 256 * 1024)
+    assert T.chunk_decision(0, 100_000, 4096, 1 << 20)[0] == 100_000
+
+
+def test_projected_bytes_count_only_the_plans_columns():
+    import numpy as np
+
+    cols = {"l_a": np.zeros(4, np.int32), "l_b": np.zeros(4, np.int64), "l_c": np.zeros((4, 10), np.uint8)}
+    assert T.projected_bytes_per_row("f(p_l_a, p_l_c)", cols) == 4 + 10
+    assert T.projected_bytes_per_row("p_l_ab p_l_b", cols) == 8

@@ -40,6 +40,7 @@ def env(tmp_path_factory):
     if COCACHE.exists():
         ctx.set_cache_dir(str(COCACHE))
     db = T.TpchDB(ctx, host)
+    db.data_dir = d  # the .colbin files the tables were loaded from
     yield ctx, db, host, tmp_path_factory.mktemp("golden")
     db.free()
     ctx.close()
@@ -94,31 +95,82 @@ def test_tpch_cli_check_exit_codes(env, tmp_path, capsys):
     assert tpch_cli.main(base + ["--check", str(tmp_path), "q4"]) == 2
 
 
-CHUNKED = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q9", "Q10", "Q12", "Q13", "Q14", "Q15", "Q18", "Q19", "Q20",
-           "Q22"]
+CHUNKED = ["Q1", "Q3", "Q4", "Q5", "Q6", "Q7", "Q8", "Q9", "Q10", "Q11", "Q12", "Q13", "Q14", "Q15", "Q17", "Q18",
+           "Q19", "Q20", "Q22"]
+CHUNK_ROWS = {"orders": 4_000, "partsupp": 2_500, "lineitem": 17_000}
+PASSES = {"Q17": 2}  # lineitem twice: per-part sums and counts, then the rows below a fifth of the average
 
 
 @pytest.mark.parametrize("q", CHUNKED)
 def test_tpch_chunked_matches_oracle(env, q):
     """Out-of-core form: the stream table crosses the link in several chunks
     (lineitem ~60K rows in 17K-row chunks, orders ~15K rows in 4K-row
-    chunks); the result is bit-exact with the oracle, as the resident run is."""
+    chunks, partsupp 8K rows in 2.5K-row chunks; Q17 streams lineitem in two
+    passes); the result is bit-exact with the oracle, as the resident run is."""
     ctx, db, host, _ = env
-    chunk = 4_000 if T.stream_table(q) == "orders" else 17_000
+    chunk = CHUNK_ROWS[T.stream_table(q)]
     print(f"TPC-H {q} streamed ({T.stream_table(q)}, {chunk}-row chunks)", flush=True)
     r = T.run(ctx, db, q, params=PARAMS[q], chunk_rows=chunk)
     exp, fm = OT.run(host, q, PARAMS[q])
     assert r.columns == exp
-    assert r.timing.chunks == -(-host[T.stream_table(q)][next(iter(host[T.stream_table(q)]))].shape[0] // chunk)
-    assert r.timing.chunks >= 3
+    per_pass = -(-host[T.stream_table(q)][next(iter(host[T.stream_table(q)]))].shape[0] // chunk)
+    assert r.timing.chunks == per_pass * PASSES.get(q, 1)
+    assert per_pass >= 3
 
 
-def test_tpch_chunk_refused_for_unchunkable_plan(env, capsys):
+def test_tpch_chunk_refused_for_unchunkable_plan(env, capsys, monkeypatch):
     from gpudbmetalcodegen_amd import tpch_cli
 
     ctx, db, host, _ = env
+    monkeypatch.setenv("GDB_FORCE_CHUNK", "1")  # the tiny tables fit: keep the requested chunking
     with pytest.raises(ValueError, match="chunked"):
-        T.run(ctx, db, "Q17", chunk_rows=10_000)
-    assert tpch_cli.main(["--generate", "0.002", "--warmup", "0", "--chunk", "4K", "q17"]) == 3
+        T.run(ctx, db, "Q21", chunk_rows=10_000)
+    assert tpch_cli.main(["--generate", "0.002", "--warmup", "0", "--chunk", "4K", "q21"]) == 3
     assert tpch_cli.main(["--generate", "0.002", "--warmup", "0", "--chunk", "4K", "q6"]) == 0
     assert "Q6: streamed lineitem in" in capsys.readouterr().out
+
+
+def test_explicit_chunk_downgraded_when_working_set_fits(env, capsys, monkeypatch):
+    """The reference's downgrade: an explicit --chunk for a working set that
+    fits in 3/4 of the budget runs directly, unless GDB_FORCE_CHUNK=1."""
+    from gpudbmetalcodegen_amd import tpch_cli
+
+    monkeypatch.delenv("GDB_FORCE_CHUNK", raising=False)
+    assert tpch_cli.main(["--generate", "0.002", "--warmup", "0", "--chunk", "4K", "q6"]) == 0
+    out = capsys.readouterr().out
+    assert "[auto-chunk] Q6: requested chunk of 4096 rows downgraded to a direct run" in out
+    assert "Q6: streamed" not in out
+
+
+def test_auto_chunk_engages_under_a_small_budget(env):
+    """With a budget the working set exceeds, a chunkable query streams in
+    the chunk size the policy picks and still matches the oracle."""
+    ctx, db, host, _ = env
+    from gpudbmetalcodegen_amd.queries import get_plan
+
+    plan = get_plan(ctx, "Q6", T.PLAN_OPTIONS.get("Q6", ""))
+    rows, why = T.auto_chunk_rows(db, "Q6", plan, budget_bytes=64 << 10, force=False)
+    n = host["lineitem"]["l_shipdate"].shape[0]
+    assert rows == n  # floor of 256K rows, capped at the table
+    rows, why = T.auto_chunk_rows(db, "Q6", plan, budget_bytes=1 << 20, force=False)
+    assert 0 < rows <= n and "exceeds" in why
+    r = T.run(ctx, db, "Q6", chunk_rows=20_000)
+    exp, _ = OT.run(host, "Q6", T.PARAMS["Q6"])
+    assert r.columns == exp
+
+
+@pytest.mark.parametrize("q", ["Q1", "Q6", "Q11", "Q13", "Q17", "Q19"])
+def test_tpch_streamed_from_colbin_matches_oracle(env, q):
+    """File-backed out-of-core form: each chunk of the stream table is read
+    from its .colbin file and converted (FLOAT -> int64 hundredths) in pinned
+    staging, never the whole table in host memory; bit-exact with the oracle
+    (Q17 streams lineitem twice, Q11 streams partsupp, Q13 orders)."""
+    ctx, db, host, _ = env
+    table = T.stream_table(q)
+    chunk = CHUNK_ROWS[table]
+    print(f"TPC-H {q} streamed from {table}.colbin ({chunk}-row chunks)", flush=True)
+    r = T.run(ctx, db, q, params=PARAMS[q], chunk_rows=chunk, colbin_dir=db.data_dir)
+    exp, fm = OT.run(host, q, PARAMS[q])
+    assert r.columns == exp
+    rows = host[table][next(iter(host[table]))].shape[0]
+    assert r.timing.chunks == -(-rows // chunk) * PASSES.get(q, 1)
