@@ -197,6 +197,8 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(hip, "gdb_mlp_forward", [vp, vp, vp, i32, vp])
     _sig(hip, "gdb_mlp_backward", [vp, vp, vp, vp, vp, i32, C.c_float, vp])
     _sig(hip, "gdb_mlp_reduce_adam", [vp, vp, vp, vp, vp, i32, i32, C.c_float, C.c_float, C.c_float, C.c_float, vp])
+    f32 = C.c_float
+    _sig(hip, "gdb_mlp_train_step_noalloc", [vp, vp, vp, vp, vp, vp, vp, i32, f32, i32, f32, f32, f32, f32, vp])
     _sig(hip, "gdb_gen_device_zipf_i32", [vp, u64, C.c_double, i64, u64, vp])
     _sig(hip, "gdb_gen_device_uniform_i64", [vp, u64, i64, i64, u64, vp])
     _sig(hip, "gdb_copy_async", [vp, vp, sz, vp])

@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <mutex>
 #include <string>
 
 namespace {
@@ -226,6 +227,34 @@ __global__ void __launch_bounds__(256) copyKernel(uint4* __restrict__ dst, const
         dst[i] = src[i];
 }
 
+// Workspace of the allocation-free training step: the per-sample gradient
+// rows G[s][NP] of up to GDB_MLP_NOALLOC_MAX_SAMPLES samples (5.4 MB), held by
+// every GPU the library runs on until the process ends. Host code never
+// passes `scratch` itself to a kernel (in host code it names a host-side
+// stand-in); it passes the device address that hipGetSymbolAddress returned
+// for the current device, looked up once per device.
+constexpr int kMaxDevices = 64;
+__device__ __align__(256) float scratch[(size_t)GDB_MLP_NOALLOC_MAX_SAMPLES * NP];
+std::once_flag g_scratch_once[kMaxDevices];
+float* g_scratch_ptr[kMaxDevices] = {};
+hipError_t g_scratch_err[kMaxDevices] = {};
+
+gdb_status scratchFor(float** out) {
+    int dev = -1;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return kfail(GDB_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return kfail(GDB_ERR_INVALID, "scratch: device index out of range");
+    std::call_once(g_scratch_once[dev], [dev]() {
+        void* p = nullptr;
+        g_scratch_err[dev] = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (g_scratch_err[dev] == hipSuccess) g_scratch_ptr[dev] = static_cast<float*>(p);
+    });
+    if (g_scratch_err[dev] != hipSuccess || !g_scratch_ptr[dev])
+        return kfail(GDB_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(g_scratch_err[dev]));
+    *out = g_scratch_ptr[dev];
+    return GDB_OK;
+}
+
 unsigned gridFor(unsigned long long n, unsigned cap = 8192) {
     unsigned long long g = (n + 255) / 256;
     if (g > cap) g = cap;
@@ -256,6 +285,31 @@ gdb_status gdb_mlp_backward(const float* params, const float* X, const float* T,
 gdb_status gdb_mlp_reduce_adam(float* params, float* m, float* v, const float* G, float* grad, int n, int step,
                                float lr, float beta1, float beta2, float eps, gdb_stream s) {
     if (n <= 0) return kfail(GDB_ERR_INVALID, "gdb_mlp_reduce_adam: empty batch");
+    const float c1 = 1.0f - std::pow(beta1, (float)step);
+    const float c2 = 1.0f - std::pow(beta2, (float)step);
+    hipLaunchKernelGGL(mlpReduceAdamKernel, dim3((NP + 255) / 256), dim3(256), 0, (hipStream_t)s, params, m, v, G,
+                       grad, n, lr, beta1, beta2, eps, c1 > 0 ? c1 : 1.0f, c2 > 0 ? c2 : 1.0f);
+    return launchCheck("mlpReduceAdamKernel");
+}
+
+gdb_status gdb_mlp_train_step_noalloc(float* params, float* m, float* v, const float* X, const float* T, float* L,
+                                     float* grad, int n, float scale, int step, float lr, float beta1, float beta2,
+                                     float eps, gdb_stream s) {
+    if (n <= 0) return kfail(GDB_ERR_INVALID, "gdb_mlp_train_step_noalloc: empty batch");
+    if (n > GDB_MLP_NOALLOC_MAX_SAMPLES)
+        return kfail(GDB_ERR_INVALID, "gdb_mlp_train_step_noalloc: batch of " + std::to_string(n) +
+                                          " samples exceeds the static workspace (" +
+                                          std::to_string(GDB_MLP_NOALLOC_MAX_SAMPLES) + " samples)");
+    if (!params || !X || !T || !L) return kfail(GDB_ERR_INVALID, "gdb_mlp_train_step_noalloc: null argument");
+    if (lr > 0.0f && (!m || !v)) return kfail(GDB_ERR_INVALID, "gdb_mlp_train_step_noalloc: Adam moments missing");
+    float* G = nullptr;
+    const gdb_status st = scratchFor(&G);
+    if (st != GDB_OK) return st;
+    // the backward kernel writes every element of rows 0..n-1: nothing to clear
+    hipLaunchKernelGGL(mlpBackwardKernel, dim3((n + SPB - 1) / SPB), dim3(256), 0, (hipStream_t)s, params, X, T, G, L,
+                       n, scale);
+    const gdb_status b = launchCheck("mlpBackwardKernel");
+    if (b != GDB_OK) return b;
     const float c1 = 1.0f - std::pow(beta1, (float)step);
     const float c2 = 1.0f - std::pow(beta2, (float)step);
     hipLaunchKernelGGL(mlpReduceAdamKernel, dim3((NP + 255) / 256), dim3(256), 0, (hipStream_t)s, params, m, v, G,

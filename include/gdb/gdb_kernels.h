@@ -38,6 +38,20 @@ gdb_status gdb_mlp_backward(const float* params, const float* X, const float* T,
 gdb_status gdb_mlp_reduce_adam(float* params, float* m, float* v, const float* G, float* grad, int n, int step,
                                float lr, float beta1, float beta2, float eps, gdb_stream s);
 
+/* Allocation-free training step: gdb_mlp_backward then gdb_mlp_reduce_adam
+ * with the per-sample gradient rows held in the library's static device
+ * workspace instead of a caller buffer G. No counterpart in the reference;
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops). At most GDB_MLP_NOALLOC_MAX_SAMPLES samples
+ * per call (larger batches are rejected); no hipMalloc/hipFree. Calls that
+ * overlap on one GPU share its workspace and race. L[n] receives the
+ * per-sample losses; grad (NULL allowed) the summed gradient; m, v may be
+ * NULL when lr <= 0. */
+#define GDB_MLP_NOALLOC_MAX_SAMPLES 256
+gdb_status gdb_mlp_train_step_noalloc(float* params, float* m, float* v, const float* X, const float* T, float* L,
+                                     float* grad, int n, float scale, int step, float lr, float beta1, float beta2,
+                                     float eps, gdb_stream s);
+
 /* Device column generators (counter-based; estimator training runs). */
 gdb_status gdb_gen_device_zipf_i32(int32_t* out, uint64_t n, double s, int64_t nkeys, uint64_t seed, gdb_stream st);
 gdb_status gdb_gen_device_uniform_i64(int64_t* out, uint64_t n, int64_t lo, int64_t hi, uint64_t seed,
