@@ -3,6 +3,7 @@
 #include "../infra/colbin.h"
 #include "../operators/hip_operators.h"
 #include "plan_util.h"
+#include "tpch_plans.h"
 
 #include <algorithm>
 #include <cctype>
@@ -110,6 +111,7 @@ struct Node {
     bool distinct = false;   // COUNT(DISTINCT ...)
     long long iv = 0;        // Num: value * 10^-scale; Date: YYYYMMDD; Interval: count
     int scale = 0;
+    std::string qual;             // Col: the table or alias qualifier ("n1" of n1.n_name), if any
     std::shared_ptr<Select> sub;  // EXISTS (subquery), x IN (subquery)
 };
 
@@ -165,6 +167,8 @@ struct OrderItem {
 struct Select {
     std::vector<SelectItem> items;
     std::vector<std::string> tables;
+    std::vector<std::string> aliases;  // per table ("" when none)
+    std::shared_ptr<Select> derived;   // FROM (SELECT ...) [AS] alias
     NodeP where, having;
     std::vector<NodeP> groupBy;
     std::vector<OrderItem> orderBy;
@@ -220,10 +224,20 @@ private:
         }
         expectKw("from");
         do {
-            if (acceptSym("(")) fail("subqueries in FROM are not supported");
+            if (acceptSym("(")) {
+                if (q.derived || !q.tables.empty()) fail("a subquery in FROM must be the only FROM item");
+                q.derived = std::make_shared<Select>(selectBody(false));
+                expectSym(")");
+                if (acceptKw("as")) ident("a derived table alias");
+                else if (peek().kind == Tok::Ident && !reserved(peek().v)) ident("a derived table alias");
+                continue;
+            }
+            if (q.derived) fail("a subquery in FROM must be the only FROM item");
             q.tables.push_back(ident("a table name"));
-            if (acceptKw("as")) ident("a table alias");
-            else if (peek().kind == Tok::Ident && !reserved(peek().v)) ident("a table alias");
+            std::string alias;
+            if (acceptKw("as")) alias = ident("a table alias");
+            else if (peek().kind == Tok::Ident && !reserved(peek().v)) alias = ident("a table alias");
+            q.aliases.push_back(alias);
         } while (acceptSym(","));
         if (isKw("join") || isKw("inner") || isKw("left") || isKw("right")) fail("JOIN is not supported");
         if (acceptKw("where")) q.where = expr();
@@ -448,7 +462,11 @@ private:
         }
         if (isSym("(")) fail("function " + w + "() is not supported");
         if (reserved(w)) fail("unexpected keyword " + w);
-        if (acceptSym(".")) return mk(NK::Col, ident("a column name"));  // qualifier dropped
+        if (acceptSym(".")) {  // qualified column: the qualifier is kept for shape detection only
+            NodeP c = mk(NK::Col, ident("a column name"));
+            c->qual = w;
+            return c;
+        }
         return mk(NK::Col, w);
     }
 };
@@ -1012,8 +1030,80 @@ std::string andAll(Compiler& cc, const std::vector<NodeP>& conj) {
 
 }  // namespace
 
+// Shapes planned by name from their text (the reference's analysis-based
+// detectors that return a named plan, metal_plan_builder.cpp:757-777): TPC-H
+// Q7, a derived table over supplier, lineitem, orders, customer and nation
+// twice, grouped by three keys. Its constants come from the text: the two
+// nation names compared with n_name and the BETWEEN dates on l_shipdate.
+// The layout names the plan and its parameters; the caller runs it as the
+// named query and renames the columns to the select list.
+void collectNodes(const NodeP& n, NK k, std::vector<NodeP>& out) {
+    if (!n) return;
+    if (n->k == k) out.push_back(n);
+    for (const auto& c : n->a) collectNodes(c, k, out);
+}
+
+HipQueryPlan namedShapePlan(const Select& q, const PlanOptions& opts) {
+    const Select& in = *q.derived;
+    std::vector<std::string> tabs = in.tables;
+    std::sort(tabs.begin(), tabs.end());
+    const std::vector<std::string> q7 = {"customer", "lineitem", "nation", "nation", "orders", "supplier"};
+    if (tabs != q7 || q.groupBy.size() != 3 || !in.where || !in.groupBy.empty() || in.having)
+        fail("a subquery in FROM is planned only in TPC-H Q7's shape (supplier, lineitem, orders, customer, nation "
+             "twice; three GROUP BY keys)");
+    // the two nations: string literals compared with n_name, in order of appearance
+    std::vector<NodeP> cmps;
+    collectNodes(in.where, NK::Cmp, cmps);
+    std::vector<std::string> nations;
+    for (const auto& c : cmps) {
+        if (c->s != "=") continue;
+        for (int side = 0; side < 2; ++side) {
+            const NodeP& col = c->a[static_cast<size_t>(side)];
+            const NodeP& lit = c->a[static_cast<size_t>(1 - side)];
+            if (col->k == NK::Col && col->s == "n_name" && lit->k == NK::Str &&
+                std::find(nations.begin(), nations.end(), lit->s) == nations.end())
+                nations.push_back(lit->s);
+        }
+    }
+    if (nations.size() != 2) fail("Q7 shape: expected two nation names compared with n_name");
+    std::vector<NodeP> betw;
+    collectNodes(in.where, NK::Between, betw);
+    long long lo = -1, hi = -1;
+    for (const auto& b : betw)
+        if (b->a[0]->k == NK::Col && b->a[0]->s == "l_shipdate" && b->a[1]->k == NK::Date && b->a[2]->k == NK::Date &&
+            !b->neg) {
+            lo = b->a[1]->iv;
+            hi = b->a[2]->iv;
+        }
+    if (lo < 0) fail("Q7 shape: expected l_shipdate BETWEEN DATE ... AND DATE ...");
+    if (hi < lo || hi / 10000 - lo / 10000 > 1)
+        fail("Q7 shape: the ship dates must lie within two consecutive years (the plan's year buckets)");
+    // the outer select list: three keys and SUM over the inner volume expression
+    int sums = 0;
+    for (const auto& it : q.items)
+        if (it.e->k == NK::Agg) {
+            if (it.e->s != "sum") fail("Q7 shape: the aggregate must be SUM");
+            ++sums;
+        }
+    if (q.items.size() != 4 || sums != 1) fail("Q7 shape: expected three keys and one SUM in the select list");
+    auto named = buildTpchPlan("Q7", opts);
+    if (!named) fail("no named plan Q7");
+    HipQueryPlan plan = std::move(*named);
+    std::string names;
+    for (const auto& it : q.items) {
+        std::string nm = it.alias;
+        if (nm.empty()) nm = it.e->k == NK::Col ? it.e->s : "expr";
+        names += (names.empty() ? "" : ",") + jstr(nm);
+    }
+    plan.sqlLayout = "{\"named\":\"Q7\",\"params\":{\"strings\":[" + jstr(nations[0]) + "," + jstr(nations[1]) +
+                     "],\"q7_date_lo\":" + std::to_string(lo) + ",\"q7_date_hi\":" + std::to_string(hi) +
+                     ",\"q7_year0\":" + std::to_string(lo / 10000) + "},\"names\":[" + names + "]}";
+    return plan;
+}
+
 HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
     Select q = Parser(lex(sql)).select();
+    if (q.derived) return namedShapePlan(q, opts);
     Planner pl;
     long long cap = std::stoll(opt(opts, "groups", "0"));
     if (cap < 0 || cap > (1LL << 26)) fail("groups option out of range");

@@ -197,6 +197,20 @@ def run_sql(ctx: E.Context, db: T.TpchDB, sql: str, warmup: int = 0, measured: i
             executor: E.HipGenericExecutor | None = None) -> SqlRun:
     """Plan, run and finish one SELECT over the TPC-H tables of db."""
     opts = options
+    first = E.build_sql_plan(sql, opts)
+    lay = first.sql_layout()
+    if "named" in lay:
+        # a shape the front end maps to a named plan (TPC-H Q7): run it with
+        # the constants taken from the text, then rename to the select list
+        prm = dict(lay["params"])
+        if "strings" in prm:
+            prm["strings"] = tuple(x.encode() for x in prm["strings"])
+        r = T.run(ctx, db, lay["named"], warmup=warmup, measured=measured, executor=executor, params=prm)
+        out = {new: r.columns[old] for new, old in zip(lay["names"], T.SELECT[lay["named"]])}
+        fmts = {new: r.formats[old] for new, old in zip(lay["names"], T.SELECT[lay["named"]])}
+        run = SqlRun(sql, out, fmts, r.timing, r.plan, lay)
+        run.canonical = T.canonical(out, fmts)
+        return run
     for _ in range(8):
         plan = E.build_sql_plan(sql, opts).compile(ctx)
         layout = plan.sql_layout()

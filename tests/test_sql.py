@@ -257,3 +257,33 @@ def test_global_aggregate_over_no_rows_is_one_null_row():
     assert out == {"s": [None], "c": [0], "avg_s": [None]}
     from gpudbmetalcodegen_amd import tpch as T
     assert T.canonical(out, fmts) == "s,c,avg_s\n,0,\n"
+
+
+def test_q7_text_plans_as_the_named_shape_with_its_constants():
+    """TPC-H Q7's text (a derived table over nation twice) is recognised by
+    shape, as the reference's buildQ7Plan detector does, and planned as the
+    named Q7 plan with the nations and dates read from the text."""
+    sql = (SQL_DIR / "tpch_q7.sql").read_text()
+    lay = E.build_sql_plan(sql).sql_layout()
+    assert lay["named"] == "Q7"
+    assert lay["params"] == {"strings": ["FRANCE", "GERMANY"], "q7_date_lo": 19950101, "q7_date_hi": 19961231,
+                             "q7_year0": 1995}
+    assert lay["names"] == ["supp_nation", "cust_nation", "l_year", "revenue"]
+    other = sql.replace("FRANCE", "JAPAN").replace("GERMANY", "CHINA").replace("1995-01-01", "1993-01-01")
+    other = other.replace("1996-12-31", "1994-06-30")
+    lay = E.build_sql_plan(other).sql_layout()
+    assert lay["params"]["strings"] == ["JAPAN", "CHINA"] and lay["params"]["q7_date_lo"] == 19930101
+    assert lay["params"]["q7_year0"] == 1993 and lay["params"]["q7_date_hi"] == 19940630
+    # the plan buckets two years: a wider range is refused, not aliased
+    with pytest.raises(N.GdbError, match="two consecutive years"):
+        E.build_sql_plan(sql.replace("1995-01-01", "1994-01-01"))
+
+
+@pytest.mark.parametrize("sql,msg", [
+    ("select a, sum(b) from (select l_orderkey as a, l_quantity as b from lineitem) as t group by a",
+     "TPC-H Q7's shape"),
+    ("select count(*) from lineitem, (select * from orders) as o", "only FROM item"),
+])
+def test_other_derived_tables_are_refused_by_name(sql, msg):
+    with pytest.raises(N.GdbError, match=msg):
+        E.build_sql_plan(sql)

@@ -244,3 +244,23 @@ def test_microbench_mb1_sums_flag_byte_codes(env):
     assert set(np.unique(flags).tolist()) == {ord("A"), ord("N"), ord("R")}
     assert r.columns["s"] == [int(flags.sum())]
     assert r.formats["s"] == (0, False)
+
+
+@pytest.mark.parametrize("nations,lo,hi", [(("FRANCE", "GERMANY"), "1995-01-01", "1996-12-31"),
+                                           (("JAPAN", "CHINA"), "1993-06-01", "1994-12-31")])
+def test_q7_sql_text_matches_oracle(env, nations, lo, hi):
+    """TPC-H Q7 from its text (the derived table over nation twice) runs as
+    the named plan with the text's nations and dates, bit-exact with the
+    oracle evaluated on the same constants."""
+    ctx, db, host = env
+    sql = (SQL_DIR / "tpch_q7.sql").read_text().replace("FRANCE", "@1").replace("GERMANY", "@2")
+    sql = sql.replace("@1", nations[0]).replace("@2", nations[1]).replace("1995-01-01", lo).replace("1996-12-31", hi)
+    print(f"SQL text of TPC-H Q7 with {nations}, {lo}..{hi}", flush=True)
+    r = S.run_sql(ctx, db, sql)
+    ymd = [int(x.replace("-", "")) for x in (lo, hi)]
+    prm = dict(T.PARAMS["Q7"], strings=tuple(n.encode() for n in nations), q7_date_lo=ymd[0], q7_date_hi=ymd[1],
+               q7_year0=ymd[0] // 10000)
+    exp, fm = OT.run(host, "Q7", prm)
+    assert len(exp["revenue"]) > 0
+    assert r.columns == exp
+    assert r.formats == fm
