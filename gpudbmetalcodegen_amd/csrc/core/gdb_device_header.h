@@ -416,8 +416,12 @@ GDB_DEV int gdb_lds_gb_update(u64* tab, u32 bits, u32 key, i64 v, u32 max_probe)
         if (inserted || (u32)cur == key) {
             if (!inserted) gdb_lds_add_u64(w, 1ull << 32);
             gdb_lds_add_u64(w + 1, (u64)v);
-            gdb_lds_min_i64((i64*)(w + 2), v);
-            gdb_lds_max_i64((i64*)(w + 3), v);
+            // min only falls and max only rises: a value no better than the
+            // word as read (possibly stale, never better than current) cannot
+            // change it, so the atomic is skipped (hot keys of a skewed column
+            // otherwise serialise four same-address LDS atomics per row)
+            if (v < (i64)gdb_lds_load_u64(w + 2)) gdb_lds_min_i64((i64*)(w + 2), v);
+            if (v > (i64)gdb_lds_load_u64(w + 3)) gdb_lds_max_i64((i64*)(w + 3), v);
             return (int)s;
         }
     }
@@ -443,8 +447,8 @@ GDB_DEV int gdb_lds_gb_merge_at(u64* tab, u32 mask, u32 s, u32 key, u32 cnt, i64
         if (inserted || (u32)cur == key) {
             if (!inserted) gdb_lds_add_u64(w, (u64)cnt << 32);
             gdb_lds_add_u64(w + 1, (u64)sum);
-            gdb_lds_min_i64((i64*)(w + 2), mn);
-            gdb_lds_max_i64((i64*)(w + 3), mx);
+            if (mn < (i64)gdb_lds_load_u64(w + 2)) gdb_lds_min_i64((i64*)(w + 2), mn);
+            if (mx > (i64)gdb_lds_load_u64(w + 3)) gdb_lds_max_i64((i64*)(w + 3), mx);
             return (int)s;
         }
     }

@@ -356,9 +356,11 @@ static void installSizingRules(gdb_executor ex, gdb_plan p) {
     ex->ex->clearLateSymbols();
     for (const auto& rule : p->plan.sizingRules) {
         ex->ex->addLateSymbol(rule.symbol);
-        ex->ex->setPhaseHook(rule.afterPhase, [rule](HipGenericExecutor& e) {
+        ex->ex->setPhaseHook(
+            rule.afterPhase,
+            [rule](HipGenericExecutor& e) {
             uint64_t count = 0;
-            e.readBuffer(rule.buffer, static_cast<size_t>(rule.word) * 8, &count, 8);
+            std::memcpy(&count, e.hookBytes(), 8);  // read behind the phase (one synchronisation)
             uint64_t v = std::max<uint64_t>(rule.minimum, count * rule.multiplier);
             if (rule.pow2) {
                 uint64_t q = 1;
@@ -367,7 +369,8 @@ static void installSizingRules(gdb_executor ex, gdb_plan p) {
             }
             e.registerSymbol(rule.symbol, v);
             e.registerSymbol(rule.symbol + "_count", count);
-        });
+            },
+            rule.buffer, static_cast<size_t>(rule.word) * 8, 8);
     }
 }
 

@@ -1,5 +1,7 @@
 #include "hip_generic_executor.h"
 
+#include "gdb/gdb_kernels.h"
+
 #include <cctype>
 
 #include <algorithm>
@@ -27,6 +29,24 @@ HipGenericExecutor::HipGenericExecutor(gdb_ctx ctx, gdb_stream stream)
 HipGenericExecutor::~HipGenericExecutor() {
     releaseAllocatedBuffers();
     for (auto e : events_) gdb_event_destroy(e);
+    if (dStatusPtrs_) gdb_free(ctx_, dStatusPtrs_);
+    if (dStatusOut_) gdb_free(ctx_, dStatusOut_);
+    if (pinned_) gdb_host_free_pinned(pinned_);
+}
+
+uint64_t* HipGenericExecutor::ensurePinned(size_t bytes) {
+    if (bytes > pinnedBytes_) {
+        check(gdb_stream_sync(stream_), "gdb_stream_sync");  // no copy into the old block is in flight
+        if (pinned_) gdb_host_free_pinned(pinned_);
+        pinned_ = nullptr;
+        pinnedBytes_ = 0;
+        const size_t nb = std::max<size_t>(bytes, 8192);
+        void* h = nullptr;
+        check(gdb_host_alloc_pinned(nb, &h), "gdb_host_alloc_pinned");
+        pinned_ = static_cast<uint64_t*>(h);
+        pinnedBytes_ = nb;
+    }
+    return pinned_;
 }
 
 void HipGenericExecutor::registerTable(const std::string& table, uint64_t rows) {
@@ -67,8 +87,10 @@ bool HipGenericExecutor::tryGetSymbol(const std::string& name, uint64_t& out) co
 
 void HipGenericExecutor::registerScalarInt(const std::string& name, int64_t value) { scalars_[name] = value; }
 
-void HipGenericExecutor::setPhaseHook(const std::string& phase, std::function<void(HipGenericExecutor&)> hook) {
-    hooks_[phase] = std::move(hook);
+void HipGenericExecutor::setPhaseHook(const std::string& phase, std::function<void(HipGenericExecutor&)> hook,
+                                      const std::string& readBuffer, size_t readOffset, size_t readBytes) {
+    if (readBytes > 4096) throw std::runtime_error("setPhaseHook: read-ahead of at most 4096 bytes");
+    hooks_[phase] = Hook{std::move(hook), readBuffer, readOffset, readBytes};
 }
 
 HipGenericExecutor::Buffer& HipGenericExecutor::ensureBuffer(const HipParamBinding& b) {
@@ -320,6 +342,48 @@ std::vector<uint64_t> HipGenericExecutor::packPhaseArgs(const HipCodegen& cg, in
     return args;
 }
 
+void HipGenericExecutor::collectInits(const HipCodegen& cg, FillList& out) {
+    for (const auto& kv : initSpecs(cg)) {
+        const HipParamBinding* ib = kv.second;
+        Buffer& buf = ensureBuffer(*ib);
+        size_t bytes = static_cast<size_t>(sizes_.resolve(ib->sizeExpr)) * ib->elemBytes;
+        if (!bytes) continue;
+        std::vector<uint8_t> pat = ib->init == InitKind::Zero ? std::vector<uint8_t>(8, 0) : ib->pattern;
+        // zero fills may cover the allocation's padding (whole 256-byte multiples)
+        if (ib->init == InitKind::Zero) bytes = (bytes + 7) & ~static_cast<size_t>(7);
+        const int pl = static_cast<int>(pat.size());
+        const bool multi = pl >= 1 && pl <= 32 && 32 % pl == 0 && bytes % 8 == 0 && bytes % static_cast<size_t>(pl) == 0 &&
+                           reinterpret_cast<uintptr_t>(buf.ptr) % 8 == 0 && bytes <= buf.bytes;
+        if (!multi) {
+            check(gdb_fill_async(buf.ptr, bytes, pat.data(), pl, stream_), "fill " + ib->name);
+            continue;
+        }
+        std::array<uint8_t, 32> w{};
+        for (int k = 0; k < 32; ++k) w[static_cast<size_t>(k)] = pat[static_cast<size_t>(k % pl)];
+        out.dst.push_back(buf.ptr);
+        out.bytes.push_back(bytes);
+        out.pattern.push_back(w);
+        out.plen.push_back(pl);
+    }
+}
+
+std::vector<HipGenericExecutor::StatusRef> HipGenericExecutor::statusRefs(const HipCodegen& cg) const {
+    std::vector<StatusRef> out;
+    for (const auto& p : cg.getPhases()) {
+        auto st = buffers_.find(p.statusBuffer);
+        if (!p.statusBuffer.empty() && st != buffers_.end()) out.push_back({st->second.ptr, p.name, false, ""});
+        auto ck = buffers_.find(p.checkBuffer);
+        if (!p.checkBuffer.empty() && ck != buffers_.end()) out.push_back({ck->second.ptr, p.name, true, p.checkMessage});
+    }
+    return out;
+}
+
+void HipGenericExecutor::judgeStatus(const StatusRef& r, const uint64_t* w) {
+    if (!r.check && w[0] != 0) throw GdbOverflowError(r.phase, w[2]);
+    if (r.check && w[0] != 0)
+        throw std::runtime_error("phase " + r.phase + ": " + r.message + " (" + std::to_string(w[2]) + " rows)");
+}
+
 void HipGenericExecutor::enqueueInits(const HipCodegen& cg) {
     for (const auto& kv : initSpecs(cg)) {
         const HipParamBinding* ib = kv.second;
@@ -389,8 +453,18 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
             res.phaseItems.push_back(static_cast<unsigned>(p.isSingleThread ? 1 : p.items));
             auto h = hooks_.find(p.name);
             if (h != hooks_.end()) {
+                if (h->second.bytes) {  // the hook's read rides behind the phase: one synchronisation
+                    auto bit = buffers_.find(h->second.buffer);
+                    if (bit == buffers_.end()) throw std::runtime_error("executor: no buffer " + h->second.buffer);
+                    if (h->second.offset + h->second.bytes > bit->second.bytes)
+                        throw std::runtime_error("executor: hook read past the end of " + h->second.buffer);
+                    uint64_t* dst = ensurePinned(h->second.bytes);
+                    check(gdb_memcpy_d2h_async(dst, static_cast<const char*>(bit->second.ptr) + h->second.offset,
+                                               h->second.bytes, stream_),
+                          "hook read " + h->second.buffer);
+                }
                 check(gdb_stream_sync(stream_), "gdb_stream_sync");
-                h->second(*this);
+                h->second.fn(*this);
             }
         }
         check(gdb_event_sync(events_[static_cast<size_t>(2 * np)]), "gdb_event_sync");
@@ -417,24 +491,54 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
 
 std::vector<std::array<uint64_t, 4>> HipGenericExecutor::checkStatus(const HipCodegen& cg, int firstPhase,
                                                                      int lastPhase) {
+    // Every status and check word of the range is gathered on the device and
+    // read back with one copy and one synchronisation.
     const auto& phases = cg.getPhases();
     const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
-    std::vector<std::array<uint64_t, 4>> out;
+    std::vector<std::array<uint64_t, 4>> out(static_cast<size_t>(std::max(0, last - firstPhase)), {0, 0, 0, 0});
+    std::vector<const void*> ptrs;
+    std::vector<std::pair<int, StatusRef>> refs;  // (index in out, reference)
     for (int pi = firstPhase; pi < last; ++pi) {
         const auto& p = phases[static_cast<size_t>(pi)];
-        std::array<uint64_t, 4> w{0, 0, 0, 0};
-        if (!p.statusBuffer.empty() && buffers_.count(p.statusBuffer)) {
-            readBuffer(p.statusBuffer, 0, w.data(), 32);
-            if (w[0] != 0) throw GdbOverflowError(p.name, w[2]);
+        auto st = buffers_.find(p.statusBuffer);
+        if (!p.statusBuffer.empty() && st != buffers_.end()) {
+            refs.push_back({pi - firstPhase, {st->second.ptr, p.name, false, ""}});
+            ptrs.push_back(st->second.ptr);
         }
-        if (!p.checkBuffer.empty() && buffers_.count(p.checkBuffer)) {
-            std::array<uint64_t, 4> c{0, 0, 0, 0};
-            readBuffer(p.checkBuffer, 0, c.data(), 32);
-            if (c[0] != 0)
-                throw std::runtime_error("phase " + p.name + ": " + p.checkMessage + " (" + std::to_string(c[2]) +
-                                         " rows)");
+        auto ck = buffers_.find(p.checkBuffer);
+        if (!p.checkBuffer.empty() && ck != buffers_.end()) {
+            refs.push_back({pi - firstPhase, {ck->second.ptr, p.name, true, p.checkMessage}});
+            ptrs.push_back(ck->second.ptr);
         }
-        out.push_back(w);
+    }
+    if (refs.empty()) return out;
+    const size_t n = refs.size();
+    uint64_t* host = ensurePinned(n * 32 + n * sizeof(void*));
+    if (n > statusCap_) {
+        check(gdb_stream_sync(stream_), "gdb_stream_sync");
+        if (dStatusPtrs_) gdb_free(ctx_, dStatusPtrs_);
+        if (dStatusOut_) gdb_free(ctx_, dStatusOut_);
+        dStatusPtrs_ = dStatusOut_ = nullptr;
+        statusCap_ = std::max<size_t>(n, 64);
+        check(gdb_alloc(ctx_, statusCap_ * sizeof(void*), &dStatusPtrs_), "gdb_alloc");
+        check(gdb_alloc(ctx_, statusCap_ * 32, &dStatusOut_), "gdb_alloc");
+        statusPtrCache_.clear();
+    }
+    if (ptrs != statusPtrCache_) {
+        std::memcpy(host + 4 * n, ptrs.data(), n * sizeof(void*));
+        check(gdb_memcpy_h2d_async(dStatusPtrs_, host + 4 * n, n * sizeof(void*), stream_), "status pointers");
+        statusPtrCache_ = ptrs;
+    }
+    check(gdb_gather_words(static_cast<const uint64_t* const*>(dStatusPtrs_), static_cast<int>(n), 4,
+                           static_cast<uint64_t*>(dStatusOut_), stream_),
+          "gather status words");
+    check(gdb_memcpy_d2h_async(host, dStatusOut_, n * 32, stream_), "read status words");
+    check(gdb_stream_sync(stream_), "gdb_stream_sync");
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t* w = host + 4 * i;
+        judgeStatus(refs[i].second, w);
+        if (!refs[i].second.check)
+            for (int k = 0; k < 4; ++k) out[static_cast<size_t>(refs[i].first)][static_cast<size_t>(k)] = w[k];
     }
     return out;
 }

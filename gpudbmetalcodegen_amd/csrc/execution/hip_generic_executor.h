@@ -67,8 +67,14 @@ public:
     bool tryGetSymbol(const std::string& name, uint64_t& out) const;
     void registerScalarInt(const std::string& name, int64_t value);
     // Host callback after a phase completes (the runner synchronises first).
-    void setPhaseHook(const std::string& phase, std::function<void(HipGenericExecutor&)> hook);
+    // With readBytes > 0, bytes [readOffset, readOffset + readBytes) of
+    // buffer readBuffer are copied behind the phase into pinned memory in
+    // the same stream, so one synchronisation serves both; the hook finds
+    // them at hookBytes().
+    void setPhaseHook(const std::string& phase, std::function<void(HipGenericExecutor&)> hook,
+                      const std::string& readBuffer = "", size_t readOffset = 0, size_t readBytes = 0);
     void clearPhaseHooks() { hooks_.clear(); }
+    const void* hookBytes() const { return pinned_; }
     // When true, buffers are not re-initialised (accumulate across runs).
     void setSkipInit(bool skip) { skipInit_ = skip; }
     // Symbols a phase hook sets during a run: buffers sized by them are
@@ -92,6 +98,27 @@ public:
     // the initialisation fills of every buffer of the plan.
     std::vector<uint64_t> packPhaseArgs(const HipCodegen& cg, int phase);
     void enqueueInits(const HipCodegen& cg);
+    // The same fills gathered for one multi-buffer fill launch
+    // (gdb_fill_multi_async) by a caller that batches many runners; a fill
+    // whose pattern or size does not fit that form is enqueued here.
+    struct FillList {
+        std::vector<void*> dst;
+        std::vector<size_t> bytes;
+        std::vector<std::array<uint8_t, 32>> pattern;  // the pattern repeated to 32 bytes
+        std::vector<int> plen;                          // its length (divides 32 and the size)
+    };
+    void collectInits(const HipCodegen& cg, FillList& out);
+    // Device addresses of the phases' status and check words (4 x u64 each)
+    // for a caller that reads them in bulk, and how to judge them.
+    struct StatusRef {
+        const void* ptr = nullptr;
+        std::string phase;
+        bool check = false;       // a check word (failure message) rather than an overflow status
+        std::string message;
+    };
+    std::vector<StatusRef> statusRefs(const HipCodegen& cg) const;
+    // Throws as checkStatus does for the words of one StatusRef.
+    static void judgeStatus(const StatusRef& r, const uint64_t* words);
     // Read the status words of the plan's phases; throws GdbOverflowError.
     std::vector<std::array<uint64_t, 4>> checkStatus(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
 
@@ -129,7 +156,21 @@ private:
     std::unordered_map<std::string, Buffer> buffers_;
     std::vector<BufferTable> bufferTables_;
     std::unordered_map<std::string, int64_t> scalars_;
-    std::unordered_map<std::string, std::function<void(HipGenericExecutor&)>> hooks_;
+    struct Hook {
+        std::function<void(HipGenericExecutor&)> fn;
+        std::string buffer;
+        size_t offset = 0, bytes = 0;
+    };
+    std::unordered_map<std::string, Hook> hooks_;
+    // pinned host staging (hook reads, status words) and the device side of
+    // the bulk status read: the status buffers' addresses and the gathered words
+    uint64_t* pinned_ = nullptr;
+    size_t pinnedBytes_ = 0;
+    void* dStatusPtrs_ = nullptr;
+    void* dStatusOut_ = nullptr;
+    size_t statusCap_ = 0;
+    std::vector<const void*> statusPtrCache_;
+    uint64_t* ensurePinned(size_t bytes);
     std::map<std::pair<gdb_kernel, int>, int> occupancy_;
     bool skipInit_ = false;
     std::vector<std::string> lateSymbols_;

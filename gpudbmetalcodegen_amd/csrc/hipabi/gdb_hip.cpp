@@ -185,8 +185,9 @@ __global__ void __launch_bounds__(256) gdb_fill_kernel(unsigned long long* __res
 
 // Multi-buffer fill: one launch initialises up to kFillMax buffers, each with
 // a pattern of up to 32 bytes (4 words, repeated); blockIdx.y selects the
-// buffer. Used to issue all of a run's buffer initialisations at once.
-constexpr int kFillMax = 12;
+// buffer. Used to issue all of a run's buffer initialisations at once (48
+// descriptors of 48 bytes: 2.3 KB of kernel arguments).
+constexpr int kFillMax = 48;
 struct FillDesc {
     unsigned long long* dst;
     unsigned long long nwords;

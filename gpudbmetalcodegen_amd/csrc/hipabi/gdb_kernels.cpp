@@ -219,6 +219,14 @@ __global__ void __launch_bounds__(256) uniformI64Kernel(long long* __restrict__ 
     }
 }
 
+// dst[i * words + w] = src[i][w]: status words of many runners gathered into
+// one buffer, so the host reads them with one copy.
+__global__ void __launch_bounds__(256) gatherWordsKernel(const unsigned long long* const* __restrict__ src, int n,
+                                                         int words, unsigned long long* __restrict__ dst) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n * words) dst[i] = src[i / words][i % words];
+}
+
 // Streaming copy, 16 bytes per lane per iteration.
 __global__ void __launch_bounds__(256) copyKernel(uint4* __restrict__ dst, const uint4* __restrict__ src,
                                                   unsigned long long n16) {
@@ -331,6 +339,16 @@ gdb_status gdb_gen_device_uniform_i64(int64_t* out, uint64_t n, int64_t lo, int6
     hipLaunchKernelGGL(uniformI64Kernel, dim3(gridFor(n)), dim3(256), 0, (hipStream_t)st, (long long*)out,
                        (unsigned long long)n, (long long)lo, (long long)hi, (unsigned long long)seed);
     return launchCheck("uniformI64Kernel");
+}
+
+gdb_status gdb_gather_words(const uint64_t* const* src, int n, int words, uint64_t* dst, gdb_stream st) {
+    if (n < 0 || words < 1 || (n && (!src || !dst))) return kfail(GDB_ERR_INVALID, "gdb_gather_words: bad arguments");
+    if (n == 0) return GDB_OK;
+    const long long total = (long long)n * words;
+    if (total > (1ll << 30)) return kfail(GDB_ERR_INVALID, "gdb_gather_words: too many words");
+    hipLaunchKernelGGL(gatherWordsKernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, (hipStream_t)st,
+                       (const unsigned long long* const*)src, n, words, (unsigned long long*)dst);
+    return launchCheck("gatherWordsKernel");
 }
 
 gdb_status gdb_copy_async(void* dst, const void* src, size_t bytes, gdb_stream st) {

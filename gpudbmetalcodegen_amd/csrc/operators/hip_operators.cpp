@@ -794,10 +794,30 @@ HipByteFlagBuild::HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std
 void HipByteFlagBuild::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(flags_, "u8", 1, size_, InitKind::Zero);
     cg.addSymbolParam("nflags_" + flags_, size_);
+    const std::string C = "_fc_" + flags_;
+    if (tested_) {
+        // Per-block direct-mapped LDS cache of keys whose flag this block has
+        // seen set or set itself: a hit skips the global read (the random
+        // byte reads, not HBM, bound this loop). A key enters the cache only
+        // after its lane has read the flag set or issued the store, so a hit
+        // never hides a missing flag; the cache holds key + 1 (0 = empty).
+        cg.declareLds("u32", C, "4096");
+        cg.addLine("for (int _c = (int)threadIdx.x; _c < 4096; _c += (int)blockDim.x) " + C + "[_c] = 0u;");
+        cg.addLine("__syncthreads();");
+    }
     child_->produce(cg, [&]() {
         if (tested_) {
-            cg.addLine("{ const u64 _fk = (u64)(i64)(" + key_ + "); const u8 _fv = (u8)(" + value_ + "); if (_fk < nflags_" +
-                       flags_ + " && " + flags_ + "[_fk] != _fv) " + flags_ + "[_fk] = _fv; }");
+            cg.addBlock("", [&]() {
+                cg.addLine("const u64 _fk = (u64)(i64)(" + key_ + ");");
+                cg.addLine("const u8 _fv = (u8)(" + value_ + ");");
+                cg.addBlock("if (_fk < nflags_" + flags_ + ")", [&]() {
+                    cg.addLine("u32* _ce = " + C + " + (((u32)_fk ^ ((u32)_fk >> 12)) & 4095u);");
+                    cg.addBlock("if (*_ce != (u32)_fk + 1u)", [&]() {
+                        cg.addLine("if (" + flags_ + "[_fk] != _fv) " + flags_ + "[_fk] = _fv;");
+                        cg.addLine("*_ce = (u32)_fk + 1u;");
+                    });
+                });
+            });
         } else {
             cg.addLine("{ const u64 _fk = (u64)(i64)(" + key_ + "); if (_fk < nflags_" + flags_ + ") " + flags_ +
                        "[_fk] = (u8)(" + value_ + "); }");

@@ -424,8 +424,9 @@ private:
 // order's priority class), as the reference's ArrayStore
 // (metal_operators.cpp:185-200) with byte values.
 // tested=true reads the flag first and skips the store when it already
-// holds the value: for skewed keys (a Zipf column), whose hot keys' lines
-// then stay clean in cache instead of being rewritten by every block.
+// holds the value, behind a per-block LDS cache of recently seen keys: for
+// skewed keys (a Zipf column), whose hot keys then cost an LDS read instead
+// of a random global read, and whose lines are not rewritten by every block.
 class HipByteFlagBuild : public HipUnaryOperator {
 public:
     HipByteFlagBuild(std::unique_ptr<HipOperator> child, const std::string& flagsName, const std::string& keyExpr,

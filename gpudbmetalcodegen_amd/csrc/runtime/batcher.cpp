@@ -1,6 +1,7 @@
 #include "batcher.h"
 
 #include "../core/gdb_device_header.h"
+#include "gdb/gdb_kernels.h"
 
 #include <algorithm>
 #include <memory>
@@ -42,10 +43,13 @@ PipelineBatch::~PipelineBatch() {
 }
 
 void PipelineBatch::freeDevice() {
-    for (void** p : {&dArgs_, &dKindItems_, &dItemPipe_, &dItemBegin_, &dItemEnd_, &dTicket_}) {
+    for (void** p : {&dArgs_, &dKindItems_, &dItemPipe_, &dItemBegin_, &dItemEnd_, &dTicket_, &dStatusSrc_,
+                     &dStatusOut_}) {
         if (*p) gdb_free(ctx_, *p);
         *p = nullptr;
     }
+    if (hStatus_) gdb_host_free_pinned(hStatus_);
+    hStatus_ = nullptr;
 }
 
 int BatchKernelSource::add(const HipQueryPlan* plan, const HipCodegen* cg, int items) {
@@ -198,6 +202,18 @@ void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
     up(dItemEnd_, itemEnd_.data(), itemEnd_.size() * 8);
     const uint64_t zero = 0;
     up(dTicket_, &zero, 8);
+    statusRefs_.clear();
+    for (const auto& p : pipes_)
+        for (const auto& r : p.ex->statusRefs(*p.cg)) statusRefs_.push_back(r);
+    if (!statusRefs_.empty()) {
+        std::vector<const void*> src;
+        for (const auto& r : statusRefs_) src.push_back(r.ptr);
+        up(dStatusSrc_, src.data(), src.size() * sizeof(void*));
+        check(gdb_alloc(ctx_, statusRefs_.size() * 32, &dStatusOut_), "gdb_alloc");
+        void* h = nullptr;
+        check(gdb_host_alloc_pinned(statusRefs_.size() * 32, &h), "gdb_host_alloc_pinned");
+        hStatus_ = static_cast<uint64_t*>(h);
+    }
     check(gdb_stream_sync(stream_), "sync");
     int occ = 1;
     if (gdb_kernel_occupancy(kernel_, block, 0, &occ) != GDB_OK || occ < 1) occ = 1;
@@ -214,9 +230,19 @@ void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
 
 void PipelineBatch::enqueue() {
     if (!kernel_) throw std::runtime_error("PipelineBatch: build() first");
-    for (auto& p : pipes_) p.ex->enqueueInits(*p.cg);
-    const uint8_t z = 0;
-    check(gdb_fill_async(dTicket_, 8, &z, 1, stream_), "fill ticket");
+    // every pipeline's buffer fills and the work ticket in one multi-buffer
+    // fill (a few launches for 64 pipelines instead of one per buffer)
+    HipGenericExecutor::FillList fl;
+    for (auto& p : pipes_) p.ex->collectInits(*p.cg, fl);
+    fl.dst.push_back(dTicket_);
+    fl.bytes.push_back(8);
+    fl.pattern.push_back({});
+    fl.plen.push_back(8);
+    std::vector<const void*> pats;
+    for (const auto& w : fl.pattern) pats.push_back(w.data());
+    check(gdb_fill_multi_async(static_cast<int>(fl.dst.size()), fl.dst.data(), fl.bytes.data(), pats.data(),
+                               fl.plen.data(), stream_),
+          "fill pipeline buffers");
     // Kernel arguments: pointers 8 bytes; the two u32 arguments are packed at
     // their natural 4-byte alignment by the layout below.
     std::vector<uint8_t> buf;
@@ -242,6 +268,13 @@ void PipelineBatch::enqueue() {
     while (buf.size() % 8) buf.push_back(0);
     check(gdb_launch(kernel_, grid_, static_cast<unsigned>(block_), 0, buf.data(), buf.size(), stream_),
           "launch gdb_batch_kernel");
+    if (!statusRefs_.empty()) {
+        const int n = static_cast<int>(statusRefs_.size());
+        check(gdb_gather_words(static_cast<const uint64_t* const*>(dStatusSrc_), n, 4, static_cast<uint64_t*>(dStatusOut_),
+                               stream_),
+              "gather status words");
+        check(gdb_memcpy_d2h_async(hStatus_, dStatusOut_, static_cast<size_t>(n) * 32, stream_), "read status words");
+    }
 }
 
 void PipelineBatch::capture() {
@@ -273,7 +306,7 @@ float PipelineBatch::run(bool useGraph) {
     check(gdb_event_sync(ev1_), "event sync");
     float ms = 0.0f;
     check(gdb_event_elapsed_ms(ev0_, ev1_, &ms), "elapsed");
-    for (auto& p : pipes_) p.ex->checkStatus(*p.cg);
+    for (size_t i = 0; i < statusRefs_.size(); ++i) HipGenericExecutor::judgeStatus(statusRefs_[i], hStatus_ + 4 * i);
     return ms;
 }
 

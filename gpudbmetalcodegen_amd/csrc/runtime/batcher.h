@@ -109,6 +109,12 @@ private:
     void* dItemBegin_ = nullptr;
     void* dItemEnd_ = nullptr;
     void* dTicket_ = nullptr;
+    // every pipeline's status and check words: gathered on the device after
+    // the fused kernel and read back with one copy into pinned memory
+    std::vector<codegen::HipGenericExecutor::StatusRef> statusRefs_;
+    void* dStatusSrc_ = nullptr;
+    void* dStatusOut_ = nullptr;
+    uint64_t* hStatus_ = nullptr;
     gdb_graph graph_ = nullptr;
     gdb_event ev0_ = nullptr, ev1_ = nullptr;
     void freeDevice();

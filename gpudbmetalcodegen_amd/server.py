@@ -266,7 +266,10 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
                 state["inflight"] += 1
                 state["max_inflight"] = max(state["max_inflight"], state["inflight"])
                 q.start = time.perf_counter() - t0
-                idle.popleft().submit(q)
+                # the most recently idle worker takes it (LIFO): at low load
+                # one worker, its thread and its stream serve every query, as
+                # in the plain form, instead of rotating through cold ones
+                idle.pop().submit(q)
             if state["done"] >= len(stream):
                 break
             timeout = None

@@ -57,6 +57,12 @@ gdb_status gdb_gen_device_zipf_i32(int32_t* out, uint64_t n, double s, int64_t n
 gdb_status gdb_gen_device_uniform_i64(int64_t* out, uint64_t n, int64_t lo, int64_t hi, uint64_t seed,
                                       gdb_stream st);
 
+/* dst[i * words + w] = src[i][w] for i < n, w < words: src is a device array
+ * of n device pointers (e.g. the status words of many runners, read back
+ * with one copy instead of one synchronising read each). No counterpart in
+ * the reference (its runner reads status per query from shared memory). */
+gdb_status gdb_gather_words(const uint64_t* const* src, int n, int words, uint64_t* dst, gdb_stream st);
+
 /* Streaming device copy (16-byte aligned pointers and size). */
 gdb_status gdb_copy_async(void* dst, const void* src, size_t bytes, gdb_stream st);
 
