@@ -205,6 +205,20 @@ GDB_DEV void gdb_bitmap_set(u32* __restrict__ bm, u64 nbits, i64 key) {
     if ((__hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) == 0u) gdb_atomic_or_u32(w, m);
 }
 
+// 16 byte flags (nonzero = set) -> 16 bits, flag k at bit k. Per 32-bit word:
+// each byte's bits are ORed into its bit 0 (shifts stay inside the byte for
+// bit 0), then ((v & 0x01010101) * 0x01020408) >> 24 gathers the four bit 0s
+// at bits 24..27 (the ten partial products land on distinct bits: no carries).
+GDB_DEV u32 gdb_pack_flags4(u32 v) {
+    v |= v >> 4;
+    v |= v >> 2;
+    v |= v >> 1;
+    return (((v & 0x01010101u) * 0x01020408u) >> 24) & 0xFu;
+}
+GDB_DEV u32 gdb_pack_flags16(u32x4 x) {
+    return gdb_pack_flags4(x.x) | (gdb_pack_flags4(x.y) << 4) | (gdb_pack_flags4(x.z) << 8) | (gdb_pack_flags4(x.w) << 12);
+}
+
 // Records in a block-local bitmap cache entry `e` (read as `c`) that bit m of
 // word w has had its global atomic issued. An entry may claim only bits whose
 // global atomic was issued. A different word replaces the entry. The same

@@ -830,6 +830,31 @@ std::string HipByteFlagBuild::describe() const {
     return "ByteFlagBuild(" + flags_ + "[" + key_ + "] = " + value_ + (tested_ ? ", tested" : "") + ")";
 }
 
+void HipFlagsToBitmap::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    cg.addBufferParam(flags_, "u8", 1, bits_, InitKind::None, {}, true, true);
+    cg.addBitmapWriteParam(bitmap_, bits_);
+    cg.setPhaseRowsExpr("(" + bits_ + " + 31) / 32");
+    const std::string nb = "nbits_" + bitmap_;
+    cg.addComment("pack 32 byte flags per bitmap word");
+    cg.addLine("const u64 _nw = (" + nb + " + 31) / 32;");
+    cg.addBlock("for (u64 _w = (u64)blockIdx.x * blockDim.x + threadIdx.x; _w < _nw; _w += (u64)gridDim.x * blockDim.x)",
+                [&]() {
+                    cg.addLine("const u64 _b0 = _w * 32;");
+                    cg.addLine("u32 _word = 0;");
+                    cg.addBlock("if (_b0 + 32 <= " + nb + ")", [&]() {
+                        cg.addLine("const u32x4 _x = *reinterpret_cast<const u32x4*>(" + flags_ + " + _b0);");
+                        cg.addLine("const u32x4 _y = *reinterpret_cast<const u32x4*>(" + flags_ + " + _b0 + 16);");
+                        cg.addLine("_word = gdb_pack_flags16(_x) | (gdb_pack_flags16(_y) << 16);");
+                    });
+                    cg.addBlock("else", [&]() {
+                        cg.addLine("for (u32 _k = 0; _k < 32 && _b0 + _k < " + nb + "; ++_k) _word |= (u32)(" + flags_ +
+                                   "[_b0 + _k] != 0) << _k;");
+                    });
+                    cg.addLine(bitmap_ + "[_w] = _word;");
+                });
+    consume();
+}
+
 HipDenseStore::HipDenseStore(std::unique_ptr<HipOperator> child, const std::string& arrayName,
                              const std::string& keyExpr, const std::string& sizeExpr, const std::string& valueExpr)
     : HipUnaryOperator(std::move(child)), array_(arrayName), key_(keyExpr), size_(sizeExpr), value_(valueExpr) {}

@@ -306,6 +306,21 @@ private:
     bool blockFirst_;
 };
 
+// Packs key-indexed byte flags (nonzero = set) into a bitmap (bit k of word
+// k >> 5, the layout HipBitmapProbe reads): one thread per bitmap word, two
+// 16-byte loads of its 32 flags. With HipByteFlagBuild in front it builds a
+// bitmap without atomic ORs (plain byte stores, then this pass).
+class HipFlagsToBitmap : public HipOperator {
+public:
+    HipFlagsToBitmap(const std::string& flagsName, const std::string& bitmapName, const std::string& bitsExpr)
+        : flags_(flagsName), bitmap_(bitmapName), bits_(bitsExpr) {}
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override { return "FlagsToBitmap(" + flags_ + " -> " + bitmap_ + ")"; }
+
+private:
+    std::string flags_, bitmap_, bits_;
+};
+
 // Key-indexed u32 store array[key] = value (value != 0) for keys that must
 // be unique (a join's key side): a compare-and-swap from 0 claims the
 // entry, and a key seen twice fails the run with `message` after the phase.

@@ -168,6 +168,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
     HipQueryPlan plan;
     plan.name = "Q3";
     const bool probeBitmap = optInt(o, "probe_bitmap", 1) != 0;
+    const bool bflags = optInt(o, "bflags", 0) != 0;
     const bool prepass = opt(o, "sizing", "prepass") == "prepass";
     q3BitmapPhase(plan, o);
     if (prepass) {
@@ -180,7 +181,11 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
         std::unique_ptr<HipOperator> top =
             std::make_unique<HipHashBuild>(std::move(probe), "d_q3_b_ht", "b_key", "b_date", "b_slots", 2);
-        if (probeBitmap)
+        // bflags=1: the B-key bitmap is built as byte flags (plain stores, no
+        // atomic ORs) and packed into the bitmap by the next phase
+        if (probeBitmap && bflags)
+            top = std::make_unique<HipByteFlagBuild>(std::move(top), "d_q3_b_flags", "b_key", "b_bits");
+        else if (probeBitmap)
             top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
         auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
@@ -190,6 +195,11 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         // holds more than one)
         ph.staged = optInt(o, "staged", 0) != 0;
         ph.prefetch = optInt(o, "prefetch", 0) != 0;
+    }
+    if (probeBitmap && bflags) {
+        auto& pk = appendPhase(plan, "Q3_pack_b_bitmap",
+                               std::make_unique<HipFlagsToBitmap>("d_q3_b_flags", "d_q3_b_bitmap", "b_bits"), 256, 1, {1});
+        pk.launchBound = 256;
     }
     {
         // late=1 (default): price and discount are read only for the rows

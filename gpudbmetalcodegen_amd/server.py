@@ -10,7 +10,8 @@ independent evaluation and every served answer compared with it.
 Plain form: one query at a time in arrival order.
 Served form (continuous batching): a dispatcher admits the oldest waiting
 query as soon as a worker is idle and its HBM footprint fits the scratch
-budget next to the queries already running; each worker is a persistent
+budget next to the queries already running, with at most one heavy
+(bandwidth-bound) query on the device at a time, light ones overlapping it; each worker is a persistent
 thread that owns a stream and its own runners (disjoint buffers), so
 admitted queries overlap on the device and a query that arrives while
 others run starts the moment a slot and memory free up. The dispatcher
@@ -227,14 +228,22 @@ class _Runner(threading.Thread):
 
 
 def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: int = 2 << 30,
-          pool: list[Worker] | None = None) -> dict:
+          pool: list[Worker] | None = None, max_heavy: int = 1, heavy_rows: int | None = None) -> dict:
     """Run a query stream. workers=1 with an unlimited budget is the plain
-    one-at-a-time form; workers>1 is the served (continuous batching) form."""
+    one-at-a-time form; workers>1 is the served (continuous batching) form.
+    Served admission: the oldest waiting query that fits starts as soon as a
+    worker is idle, except that at most max_heavy heavy queries (at least
+    heavy_rows rows scanned; default half the largest plan type's) run at
+    once: heavy scans are bandwidth-bound, so running two at once only
+    stretches both, while a light query (anti-join, LIKE) overlaps a heavy
+    one almost for free and may pass a heavy query that waits."""
     own = pool is None
     pool = (pool or [Worker(db) for _ in range(workers)])[:workers]
     arrivals = sorted(stream, key=lambda q: (q.arrival, q.qid))
+    if heavy_rows is None:
+        heavy_rows = max(db.rows_for(k) for k in KINDS) // 2
     cv = threading.Condition()
-    state = {"used": 0, "done": 0, "inflight": 0, "max_inflight": 0}
+    state = {"used": 0, "done": 0, "inflight": 0, "max_inflight": 0, "heavy": 0}
     idle: deque = deque()
 
     def on_done(runner: _Runner, q: Query) -> None:
@@ -242,6 +251,7 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             state["used"] -= q.footprint
             state["done"] += 1
             state["inflight"] -= 1
+            state["heavy"] -= q.rows >= heavy_rows
             idle.append(runner)
             cv.notify()
 
@@ -258,12 +268,27 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             while ai < len(arrivals) and arrivals[ai].arrival <= now:
                 pending.append(arrivals[ai])
                 ai += 1
-            # admit the oldest waiting query while a worker is idle and its
-            # footprint fits (an empty device always admits one)
-            while pending and idle and (state["used"] + pending[0].footprint <= budget_bytes or state["inflight"] == 0):
-                q = pending.popleft()
+            # admit, oldest first, every waiting query that fits while a
+            # worker is idle (an empty device always admits one); a heavy
+            # query waits while max_heavy heavy ones run, and light queries
+            # behind it may pass
+            while pending and idle:
+                pick = None
+                for j, c in enumerate(pending):
+                    fits = state["used"] + c.footprint <= budget_bytes or state["inflight"] == 0
+                    heavy_ok = c.rows < heavy_rows or state["heavy"] < max_heavy or state["inflight"] == 0
+                    if fits and heavy_ok:
+                        pick = j
+                        break
+                    if not fits:
+                        break  # memory order is kept: nothing passes a query that does not fit
+                if pick is None:
+                    break
+                q = pending[pick]
+                del pending[pick]
                 state["used"] += q.footprint
                 state["inflight"] += 1
+                state["heavy"] += q.rows >= heavy_rows
                 state["max_inflight"] = max(state["max_inflight"], state["inflight"])
                 q.start = time.perf_counter() - t0
                 # the most recently idle worker takes it (LIFO): at low load
@@ -272,8 +297,11 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
                 idle.pop().submit(q)
             if state["done"] >= len(stream):
                 break
+            # sleep until the next arrival or a completion (a waiting query
+            # that cannot start now needs a completion; a new arrival may be
+            # a light query that can)
             timeout = None
-            if ai < len(arrivals) and not (pending and idle):
+            if ai < len(arrivals):
                 timeout = max(0.0, arrivals[ai].arrival - (time.perf_counter() - t0))
             cv.wait(timeout)
     for r in runners:
