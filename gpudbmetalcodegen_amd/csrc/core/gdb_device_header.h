@@ -410,13 +410,17 @@ GDB_DEV int gdb_gb_merge(u64* __restrict__ tab, u32 bits, u32 key, u32 cnt, i64 
     return -1;
 }
 
-// Block-local (LDS) group-by table with the same slot layout. Returns the
-// slot index, or -1 when the probe window is full (caller spills to global).
+// Block-local (LDS) group-by table: the slot words of the global table
+// (key | count << 32, sum, min, max) stored word-major — word j of slot s at
+// tab[j * slots + s] — so a wave's 64 random slots spread over 32 bank pairs
+// instead of the 8 a 32-byte slot-major layout reaches (measured ~8 conflict
+// cycles per LDS instruction with it). Returns the slot index, or -1 when the
+// probe window is full (caller spills to global).
 GDB_DEV int gdb_lds_gb_update(u64* tab, u32 bits, u32 key, i64 v, u32 max_probe) {
-    const u32 mask = (1u << bits) - 1u;
+    const u32 n = 1u << bits, mask = n - 1u;
     u32 s = (u32)gdb_hash_slot(key, bits);
     for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
-        u64* w = tab + (u64)s * 4;
+        u64* w = tab + s;
         u64 cur = gdb_lds_load_u64(w);
         bool inserted = false;
         if ((u32)cur == GDB_EMPTY_KEY) {
@@ -429,25 +433,26 @@ GDB_DEV int gdb_lds_gb_update(u64* tab, u32 bits, u32 key, i64 v, u32 max_probe)
         }
         if (inserted || (u32)cur == key) {
             if (!inserted) gdb_lds_add_u64(w, 1ull << 32);
-            gdb_lds_add_u64(w + 1, (u64)v);
+            gdb_lds_add_u64(w + n, (u64)v);
             // min only falls and max only rises: a value no better than the
             // word as read (possibly stale, never better than current) cannot
             // change it, so the atomic is skipped (hot keys of a skewed column
             // otherwise serialise four same-address LDS atomics per row)
-            if (v < (i64)gdb_lds_load_u64(w + 2)) gdb_lds_min_i64((i64*)(w + 2), v);
-            if (v > (i64)gdb_lds_load_u64(w + 3)) gdb_lds_max_i64((i64*)(w + 3), v);
+            if (v < (i64)gdb_lds_load_u64(w + 2 * n)) gdb_lds_min_i64((i64*)(w + 2 * n), v);
+            if (v > (i64)gdb_lds_load_u64(w + 3 * n)) gdb_lds_max_i64((i64*)(w + 3 * n), v);
             return (int)s;
         }
     }
     return -1;
 }
 
-// Merge a partial group (cnt rows, sum/min/max) into an LDS table of
-// (mask + 1) slots starting at slot s. Returns the slot or -1 after
+// Merge a partial group (cnt rows, sum/min/max) into a word-major LDS table
+// of (mask + 1) slots starting at slot s. Returns the slot or -1 after
 // max_probe slots.
 GDB_DEV int gdb_lds_gb_merge_at(u64* tab, u32 mask, u32 s, u32 key, u32 cnt, i64 sum, i64 mn, i64 mx, u32 max_probe) {
+    const u32 n = mask + 1u;
     for (u32 p = 0; p < max_probe; ++p, s = (s + 1) & mask) {
-        u64* w = tab + (u64)s * 4;
+        u64* w = tab + s;
         u64 cur = gdb_lds_load_u64(w);
         bool inserted = false;
         if ((u32)cur == GDB_EMPTY_KEY) {
@@ -460,9 +465,9 @@ GDB_DEV int gdb_lds_gb_merge_at(u64* tab, u32 mask, u32 s, u32 key, u32 cnt, i64
         }
         if (inserted || (u32)cur == key) {
             if (!inserted) gdb_lds_add_u64(w, (u64)cnt << 32);
-            gdb_lds_add_u64(w + 1, (u64)sum);
-            if (mn < (i64)gdb_lds_load_u64(w + 2)) gdb_lds_min_i64((i64*)(w + 2), mn);
-            if (mx > (i64)gdb_lds_load_u64(w + 3)) gdb_lds_max_i64((i64*)(w + 3), mx);
+            gdb_lds_add_u64(w + n, (u64)sum);
+            if (mn < (i64)gdb_lds_load_u64(w + 2 * n)) gdb_lds_min_i64((i64*)(w + 2 * n), mn);
+            if (mx > (i64)gdb_lds_load_u64(w + 3 * n)) gdb_lds_max_i64((i64*)(w + 3 * n), mx);
             return (int)s;
         }
     }

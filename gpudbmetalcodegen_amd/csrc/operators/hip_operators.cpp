@@ -1695,9 +1695,11 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             LS = "_ls_" + T;
             LB = "_lb_" + T;
         }
+        // word-major LDS table (word j of slot s at j * LS + s)
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
-            cg.addLine("_gl_" + T + "[4 * _s] = GDB_EMPTY_WORD; _gl_" + T + "[4 * _s + 1] = 0;");
-            cg.addLine("_gl_" + T + "[4 * _s + 2] = (u64)GDB_I64_MAX; _gl_" + T + "[4 * _s + 3] = (u64)GDB_I64_MIN;");
+            cg.addLine("_gl_" + T + "[_s] = GDB_EMPTY_WORD; _gl_" + T + "[" + LS + " + _s] = 0;");
+            cg.addLine("_gl_" + T + "[2 * " + LS + " + _s] = (u64)GDB_I64_MAX; _gl_" + T + "[3 * " + LS +
+                       " + _s] = (u64)GDB_I64_MIN;");
         });
         cg.addLine("__syncthreads();");
         child_->produce(cg, [&]() {
@@ -1712,11 +1714,11 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         });
         cg.addLine("__syncthreads();");
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
-            cg.addLine("const u64 _w0 = _gl_" + T + "[4 * _s];");
+            cg.addLine("const u64 _w0 = _gl_" + T + "[_s];");
             cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
                 cg.addLine("const int _r = gdb_gb_merge(" + T + ", _bits_" + T + ", (u32)_w0, (u32)(_w0 >> 32), (i64)_gl_" +
-                           T + "[4 * _s + 1], (i64)_gl_" + T + "[4 * _s + 2], (i64)_gl_" + T + "[4 * _s + 3], _maxp_" +
-                           T + ");");
+                           T + "[" + LS + " + _s], (i64)_gl_" + T + "[2 * " + LS + " + _s], (i64)_gl_" + T + "[3 * " +
+                           LS + " + _s], _maxp_" + T + ");");
                 cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
             });
         });
@@ -1839,9 +1841,9 @@ void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addLine("const bool _fits = _P <= " + PMAX + "ull;");
     cg.addLine("u32 _ovf = (!_fits && blockIdx.x == 0 && threadIdx.x == 0) ? 1u : 0u, _spill = 0;");
     openLogicalBlocks(cg, nRows, "_fits");
-    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
-        cg.addLine("_gl[4 * _s] = GDB_EMPTY_WORD; _gl[4 * _s + 1] = 0;");
-        cg.addLine("_gl[4 * _s + 2] = (u64)GDB_I64_MAX; _gl[4 * _s + 3] = (u64)GDB_I64_MIN;");
+    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {  // word-major
+        cg.addLine("_gl[_s] = GDB_EMPTY_WORD; _gl[" + CAP + " + _s] = 0;");
+        cg.addLine("_gl[2 * " + CAP + " + _s] = (u64)GDB_I64_MAX; _gl[3 * " + CAP + " + _s] = (u64)GDB_I64_MIN;");
     });
     cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = 0;");
     cg.addLine("__syncthreads();");
@@ -1859,12 +1861,13 @@ void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addLine("__syncthreads();");
     cg.addComment("the block's final LDS table, and its cold-row counts per partition");
     cg.addLine("u64* _h = " + n.hot + " + _lb * " + CAP + " * 4;");
-    cg.addLine("for (int _s = (int)threadIdx.x; _s < " + CAP + " * 4; _s += (int)blockDim.x) _h[_s] = _gl[_s];");
+    cg.addLine("for (int _s = (int)threadIdx.x; _s < " + CAP + " * 4; _s += (int)blockDim.x) _h[_s] = _gl[(_s & 3) * " +
+               CAP + " + (_s >> 2)];  // slot-major in HBM");
     cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) { " + n.ccnt + "[_lb * _P + _q] = _pc[_q]; _pc[_q] = 0; }");
     cg.addLine("__syncthreads();");
     cg.addComment("partial-aggregate counts per partition");
     cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
-        cg.addLine("const u64 _w0 = _gl[4 * _s];");
+        cg.addLine("const u64 _w0 = _gl[_s];");
         cg.addLine("if ((u32)_w0 != GDB_EMPTY_KEY) gdb_lds_add_u32(&_pc[gdb_partition((u32)_w0, _pb)], 1u);");
     });
     cg.addLine("__syncthreads();");
@@ -2004,8 +2007,8 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.declareLds("u64", "_pl", PS + " * 4");
     cg.addBlock("for (u64 _p = blockIdx.x; _p < _P; _p += gridDim.x)", [&]() {
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + PS + "; _s += (int)blockDim.x)", [&]() {
-            cg.addLine("_pl[4 * _s] = GDB_EMPTY_WORD; _pl[4 * _s + 1] = 0;");
-            cg.addLine("_pl[4 * _s + 2] = (u64)GDB_I64_MAX; _pl[4 * _s + 3] = (u64)GDB_I64_MIN;");
+            cg.addLine("_pl[_s] = GDB_EMPTY_WORD; _pl[" + PS + " + _s] = 0;  // word-major");
+            cg.addLine("_pl[2 * " + PS + " + _s] = (u64)GDB_I64_MAX; _pl[3 * " + PS + " + _s] = (u64)GDB_I64_MIN;");
         });
         cg.addLine("__syncthreads();");
         cg.addLine("const u64 _cb = " + n.meta + "[2 * _P + _p], _cn = " + n.meta + "[_p];");
@@ -2027,8 +2030,9 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addLine("__syncthreads();");
         cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + PS + "; _s += (int)blockDim.x)", [&]() {
             cg.addLine("u64* _g = " + T + " + (_p * " + PS + " + (u64)_s) * 4;");
-            cg.addLine("const u64 _w0 = _pl[4 * _s];");
-            cg.addLine("_g[0] = _w0; _g[1] = _pl[4 * _s + 1]; _g[2] = _pl[4 * _s + 2]; _g[3] = _pl[4 * _s + 3];");
+            cg.addLine("const u64 _w0 = _pl[_s];");
+            cg.addLine("_g[0] = _w0; _g[1] = _pl[" + PS + " + _s]; _g[2] = _pl[2 * " + PS + " + _s]; _g[3] = _pl[3 * " + PS +
+                       " + _s];");
             cg.addLine("if ((u32)_w0 != GDB_EMPTY_KEY) ++_ins;");
         });
         cg.addLine("__syncthreads();");

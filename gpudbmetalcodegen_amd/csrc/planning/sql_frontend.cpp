@@ -1532,10 +1532,17 @@ HipQueryPlan buildSqlPlan(const std::string& sql, const PlanOptions& opts) {
                     "_keys\",\"column\":" + jstr(d.key) + "}";
     }
     if (!pl.grouped) {
+        // a scan of one-byte columns only (MB1's l_returnflag) takes 16 rows
+        // a thread, so each lane still loads 16 bytes at once
+        bool bytesOnly = !pl.cc.used.empty() && opts.find("items") == opts.end();
+        for (const auto& c : pl.cc.used) {
+            const infra::TpchColumn* tc = findColumn(pl.cc.cat, c);
+            bytesOnly = bytesOnly && tc && tc->type == infra::TpchType::CHAR1;
+        }
         auto agg = std::make_unique<HipKeyedAgg>(scan(pl.cc.used), "d_sql_agg", "0", 1);
         addAggs(*agg);
         // (launch bound 256: up to 256 VGPRs for the register accumulators)
-        appendPhase(plan, "SQL_reduce", std::move(agg), 256, items, {2, 4}).launchBound = 256;
+        appendPhase(plan, "SQL_reduce", std::move(agg), 256, bytesOnly ? 16 : items, {2, 4, 16}).launchBound = 256;
     } else {
         // Keys of at most 63 bits together (char(1) 8, dates and INT
         // columns 32) are packed into the hash key itself: exact, no check.
