@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <mutex>
 #include <cstring>
 #include <fstream>
 #include <numeric>
@@ -106,17 +107,36 @@ void sampleDeviceColumns(gdb_ctx ctx, gdb_stream s, const int32_t* dkeys, const 
                          uint64_t sampleSize, int32_t* hkeys, int64_t* hvals) {
     (void)ctx;
     if (sampleSize > n) sampleSize = n;
+    static std::mutex mu;
+    static uint8_t* stage = nullptr;  // pinned, kept until the process ends
+    static size_t stageBytes = 0;
+    std::lock_guard<std::mutex> lock(mu);
+    const size_t need = static_cast<size_t>(sampleSize) * 12 + 16;
+    if (need > stageBytes) {
+        check(gdb_stream_sync(s), "sample sync");
+        if (stage) gdb_host_free_pinned(stage);
+        stage = nullptr;
+        stageBytes = 0;
+        void* h = nullptr;
+        check(gdb_host_alloc_pinned(std::max<size_t>(need, 1 << 16), &h), "gdb_host_alloc_pinned");
+        stage = static_cast<uint8_t*>(h);
+        stageBytes = std::max<size_t>(need, 1 << 16);
+    }
+    int32_t* sk = reinterpret_cast<int32_t*>(stage);
+    int64_t* sv = reinterpret_cast<int64_t*>(stage + ((static_cast<size_t>(sampleSize) * 4 + 15) & ~static_cast<size_t>(15)));
     const uint64_t chunks = sampleSize >= 8 ? 8 : 1;
     const uint64_t per = sampleSize / chunks;
     uint64_t done = 0;
     for (uint64_t c = 0; c < chunks; ++c) {
         const uint64_t len = (c + 1 == chunks) ? sampleSize - done : per;
         const uint64_t start = (n - len) * c / std::max<uint64_t>(1, chunks - 1);
-        check(gdb_memcpy_d2h_async(hkeys + done, dkeys + start, len * 4, s), "sample keys");
-        if (dvals && hvals) check(gdb_memcpy_d2h_async(hvals + done, dvals + start, len * 8, s), "sample vals");
+        check(gdb_memcpy_d2h_async(sk + done, dkeys + start, len * 4, s), "sample keys");
+        if (dvals && hvals) check(gdb_memcpy_d2h_async(sv + done, dvals + start, len * 8, s), "sample vals");
         done += len;
     }
     check(gdb_stream_sync(s), "sample sync");
+    std::memcpy(hkeys, sk, static_cast<size_t>(sampleSize) * 4);
+    if (dvals && hvals) std::memcpy(hvals, sv, static_cast<size_t>(sampleSize) * 8);
 }
 
 // ---------------------------------------------------------------------------
@@ -128,7 +148,32 @@ CardinalityEstimator::CardinalityEstimator(gdb_ctx ctx) : ctx_(ctx), stream_(gdb
     initWeights(1);
 }
 
-CardinalityEstimator::~CardinalityEstimator() = default;
+CardinalityEstimator::~CardinalityEstimator() {
+    if (dP_) gdb_free(ctx_, dP_);
+    if (dX_) gdb_free(ctx_, dX_);
+    if (dY_) gdb_free(ctx_, dY_);
+    if (hStage_) gdb_host_free_pinned(hStage_);
+}
+
+void CardinalityEstimator::reserve(int n) {
+    if (n <= cap_ && dP_) return;
+    check(gdb_stream_sync(stream_), "sync");
+    const int cap = std::max(n, 64);
+    for (void** p : {&dP_, &dX_, &dY_})
+        if (*p) gdb_free(ctx_, *p);
+    if (hStage_) gdb_host_free_pinned(hStage_);
+    dP_ = dX_ = dY_ = nullptr;
+    hStage_ = nullptr;
+    cap_ = 0;
+    check(gdb_alloc(ctx_, GDB_MLP_NPARAMS * 4, &dP_), "gdb_alloc");
+    check(gdb_alloc(ctx_, static_cast<size_t>(cap) * kFeatures * 4, &dX_), "gdb_alloc");
+    check(gdb_alloc(ctx_, static_cast<size_t>(cap) * 4, &dY_), "gdb_alloc");
+    void* h = nullptr;
+    check(gdb_host_alloc_pinned((GDB_MLP_NPARAMS + static_cast<size_t>(cap) * (kFeatures + 1)) * 4, &h),
+          "gdb_host_alloc_pinned");
+    hStage_ = static_cast<float*>(h);
+    cap_ = cap;
+}
 
 void CardinalityEstimator::initWeights(uint64_t seed) {
     auto normal = [&](uint64_t i) {
@@ -221,12 +266,22 @@ void CardinalityEstimator::predict(const float* X, int n, float* out) {
     if (n <= 0) return;
     std::vector<float> Xs;
     standardise(X, n, Xs);
-    DevBuf dP(ctx_, GDB_MLP_NPARAMS * 4), dX(ctx_, Xs.size() * 4), dY(ctx_, static_cast<size_t>(n) * 4);
-    check(gdb_memcpy_h2d_async(dP.p, params_.data(), GDB_MLP_NPARAMS * 4, stream_), "upload params");
-    check(gdb_memcpy_h2d_async(dX.p, Xs.data(), Xs.size() * 4, stream_), "upload X");
-    check(gdb_mlp_forward(dP.as<float>(), dX.as<float>(), dY.as<float>(), n, stream_), "mlp forward");
-    check(gdb_memcpy_d2h_async(out, dY.p, static_cast<size_t>(n) * 4, stream_), "download");
+    reserve(n);
+    // weights, inputs and outputs staged in one pinned block: DMA copies,
+    // no allocation, one synchronisation
+    float* hP = hStage_;
+    float* hX = hStage_ + GDB_MLP_NPARAMS;
+    float* hY = hX + static_cast<size_t>(cap_) * kFeatures;
+    std::memcpy(hP, params_.data(), GDB_MLP_NPARAMS * 4);
+    std::memcpy(hX, Xs.data(), Xs.size() * 4);
+    check(gdb_memcpy_h2d_async(dP_, hP, GDB_MLP_NPARAMS * 4, stream_), "upload params");
+    check(gdb_memcpy_h2d_async(dX_, hX, Xs.size() * 4, stream_), "upload X");
+    check(gdb_mlp_forward(static_cast<const float*>(dP_), static_cast<const float*>(dX_), static_cast<float*>(dY_), n,
+                          stream_),
+          "mlp forward");
+    check(gdb_memcpy_d2h_async(hY, dY_, static_cast<size_t>(n) * 4, stream_), "download");
     check(gdb_stream_sync(stream_), "sync");
+    std::memcpy(out, hY, static_cast<size_t>(n) * 4);
 }
 
 double CardinalityEstimator::predictGroups(const FeatureOut& f) {

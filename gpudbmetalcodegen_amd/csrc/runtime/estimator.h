@@ -107,11 +107,22 @@ private:
     std::vector<float> params_;
     std::array<float, kFeatures> mean_{}, std_{};
     bool loaded_ = false;
+    // inference buffers kept across calls (no allocation per prediction):
+    // device weights, inputs and outputs for up to cap_ samples, and one
+    // pinned host block staging all three copies
+    void* dP_ = nullptr;
+    void* dX_ = nullptr;
+    void* dY_ = nullptr;
+    float* hStage_ = nullptr;
+    int cap_ = 0;
+    void reserve(int n);
     void standardise(const float* X, int n, std::vector<float>& out) const;
 };
 
 // Copy a strided sample of `sampleSize` rows (8 evenly spaced contiguous
-// chunks) of a device key column (and optional value column) to the host.
+// chunks) of a device key column (and optional value column) to the host,
+// through a process-wide pinned staging block (grown on demand and kept
+// until the process ends) so the copies are DMA and one synchronisation.
 void sampleDeviceColumns(gdb_ctx ctx, gdb_stream s, const int32_t* dkeys, const int64_t* dvals, uint64_t n,
                          uint64_t sampleSize, int32_t* hkeys, int64_t* hvals);
 
