@@ -327,7 +327,7 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     plain_ms = sum(rp["wall"]) / steps
     groups = run.result.rows()
     phases = rp["phases"]
-    out["plain"] = {"sizing": "exact distinct-count pre-pass (GBC_bitmap + GBC_popcount, host read of the count)",
+    out["plain"] = {"sizing": "exact distinct-count pre-pass (GBC_flags + GBC_count, host read of the count)",
                     "ms_per_step": plain_ms, "rows_per_s": n / (plain_ms / 1e3), "slots": run.slots,
                     "trials": {"wall_ms": ST.trial_stats(rp["wall"]), "gpu_ms": ST.trial_stats(rp["gpu"])},
                     "phase_ms": {k: round(v, 4) for k, v in phases.items()}}
