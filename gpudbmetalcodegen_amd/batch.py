@@ -95,12 +95,24 @@ def next_pow2(x: int) -> int:
 class Workload3:
     """Device-resident pipelines, each with its own runner."""
 
-    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096, window: int = 4):
+    def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096, window: int = 4,
+                 partials: int = 256):
+        """partials: in the batched form the aggregate pipelines store each
+        work item's LDS table as a partial table, merged into the group table
+        by a second phase (a second persistent kernel), up to this many per
+        pipeline; 0: every work item merges its LDS table into the group
+        table itself. The plain form always merges per block: its one launch
+        per pipeline has few blocks to merge, and a merge phase would add a
+        launch per pipeline."""
         self.ctx = ctx
         self.specs = specs
         self.lds_cap = lds_cap
-        self.plans = {k: E.build_hip_plan(plan_name(k), f"lds_cap={lds_cap},window={window}" if k == "agg" else "").compile(ctx)
-                      for k in {s.kind for s in specs}}
+        agg_opts = f"lds_cap={lds_cap},window={window}"
+        kinds = {s.kind for s in specs}
+        self.plans = {k: E.build_hip_plan(plan_name(k), agg_opts if k == "agg" else "").compile(ctx) for k in kinds}
+        self.batch_plans = dict(self.plans)
+        if partials > 0 and "agg" in kinds:
+            self.batch_plans["agg"] = E.build_hip_plan("PIPE_AGG", f"{agg_opts},partials={partials}").compile(ctx)
         self.tables, self.execs, self.bitmaps = [], [], []
         self.launch = {}   # pipe index -> (block, items, lds_bits)
         for s in specs:
@@ -126,9 +138,9 @@ class Workload3:
     def total_rows(self) -> int:
         return sum(s.n for s in self.specs)
 
-    def _apply(self, i: int) -> E.HipQueryPlan:
+    def _apply(self, i: int, batched: bool = False) -> E.HipQueryPlan:
         s = self.specs[i]
-        plan = self.plans[s.kind]
+        plan = (self.batch_plans if batched else self.plans)[s.kind]
         b, it, lb = self.launch[s.index]
         plan.set_launch(phase_name(s.kind), b, it)
         if s.kind == "agg":
@@ -152,8 +164,9 @@ class Workload3:
             bl = (C.c_int * len(blocks))(*blocks)
             it = (C.c_int * len(items))(*items)
             scal = "pipe_lds_bits" if s.kind == "agg" else None
-            sv = (C.c_int64 * len(lds_bits))(*lds_bits) if scal else None
-            nsv = len(lds_bits) if scal else 0
+            cand = [b for b in lds_bits if (1 << b) <= self.lds_cap]
+            sv = (C.c_int64 * len(cand))(*cand) if scal else None
+            nsv = len(cand) if scal else 0
             ncap = len(blocks) * len(items) * max(1, nsv)
             table = (C.c_double * (4 * ncap))()
             bb, bi, bs, ms, nc = C.c_int(), C.c_int(), C.c_int64(), C.c_float(), C.c_int()
@@ -204,7 +217,7 @@ class Batch:
         self._h = C.c_void_p()
         N.eng_check(_e().gdb_batch_create(w.ctx.handle, C.byref(self._h)))
         for i, s in enumerate(w.specs):
-            plan = w._apply(i)
+            plan = w._apply(i, batched=True)
             idx = C.c_int()
             N.eng_check(_e().gdb_batch_add(self._h, plan.handle, w.execs[i].handle, w.launch[s.index][1],
                                            C.byref(idx)))

@@ -10,7 +10,7 @@ namespace {
 // Recursive-descent evaluator over unsigned 64-bit integers:
 //   expr := term (('+'|'-') term)*
 //   term := atom (('*'|'/') atom)*
-//   atom := number | symbol | '(' expr ')'
+//   atom := number | symbol | '(' expr ')' | ('min'|'max') '(' expr ',' expr ')'
 class SizeParser {
 public:
     SizeParser(const std::string& s, const HipSizeResolver& r) : s_(s), r_(r) {}
@@ -85,7 +85,23 @@ private:
             while (pos_ < s_.size() &&
                    (std::isalnum(static_cast<unsigned char>(s_[pos_])) || s_[pos_] == '_'))
                 ++pos_;
-            return r_.getSymbol(s_.substr(b, pos_ - b));
+            const std::string name = s_.substr(b, pos_ - b);
+            skip();
+            if ((name == "min" || name == "max") && pos_ < s_.size() && s_[pos_] == '(') {
+                ++pos_;
+                const uint64_t a = expr();
+                skip();
+                if (pos_ >= s_.size() || s_[pos_] != ',')
+                    throw std::runtime_error("HipSizeResolver: expected ',' in '" + s_ + "'");
+                ++pos_;
+                const uint64_t c = expr();
+                skip();
+                if (pos_ >= s_.size() || s_[pos_] != ')')
+                    throw std::runtime_error("HipSizeResolver: unbalanced parentheses in '" + s_ + "'");
+                ++pos_;
+                return name == "min" ? (a < c ? a : c) : (a > c ? a : c);
+            }
+            return r_.getSymbol(name);
         }
         throw std::runtime_error("HipSizeResolver: unexpected '" + std::string(1, c) + "' in '" + s_ + "'");
     }

@@ -3,6 +3,8 @@
 #include <cstdlib>
 #include <sstream>
 #include <stdexcept>
+#include <tuple>
+#include <utility>
 
 namespace gdb {
 namespace codegen {
@@ -1668,44 +1670,89 @@ HipHashGroupBy::HipHashGroupBy(std::unique_ptr<HipOperator> child, const std::st
         throw std::runtime_error("HipHashGroupBy: LDS slots must be a power of two in [64, 4096]");
 }
 
-void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    cg.addBufferParam(table_, "u64", 8, slots_ + " * 4", InitKind::Pattern,
-                      wordsPattern({kEmptyWord, 0, kI64Max, kI64Min}), false, true);
-    cg.addSymbolParam("nslots_" + table_, slots_);
-    const std::string st = table_ + "_status";
+namespace {
+
+// Slots in use of a group-by LDS table: the compiled capacity, or 1 << scalar
+// (clamped to it) when the slot count comes from a run-time scalar. Returns
+// the (slots, bits) expressions.
+std::pair<std::string, std::string> ldsTableSize(HipCodegen& cg, const std::string& T, int cap,
+                                                 const std::string& bitsScalar) {
+    int lbits = 0;
+    while ((1 << lbits) < cap) ++lbits;
+    if (bitsScalar.empty()) return {s(cap), s(lbits) + "u"};
+    cg.addScalarParam(bitsScalar);
+    cg.addLine("const u32 _lb_" + T + " = (u32)(" + bitsScalar + " < 1 ? 1 : (" + bitsScalar + " > " + s(lbits) + " ? " +
+               s(lbits) + " : " + bitsScalar + "));");
+    cg.addLine("const int _ls_" + T + " = 1 << _lb_" + T + ";");
+    return {"_ls_" + T, "_lb_" + T};
+}
+
+// Word-major LDS group table (word j of slot s at j * LS + s): empty slots.
+void emitLdsGbInit(HipCodegen& cg, const std::string& tab, const std::string& LS) {
+    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
+        cg.addLine(tab + "[_s] = GDB_EMPTY_WORD; " + tab + "[" + LS + " + _s] = 0;");
+        cg.addLine(tab + "[2 * " + LS + " + _s] = (u64)GDB_I64_MAX; " + tab + "[3 * " + LS + " + _s] = (u64)GDB_I64_MIN;");
+    });
+}
+
+// Merge every occupied slot of an LDS group table into the global table T.
+void emitLdsGbFlush(HipCodegen& cg, const std::string& T, const std::string& tab, const std::string& LS) {
+    cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
+        cg.addLine("const u64 _w0 = " + tab + "[_s];");
+        cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+            cg.addLine("const int _r = gdb_gb_merge(" + T + ", _bits_" + T + ", (u32)_w0, (u32)(_w0 >> 32), (i64)" + tab +
+                       "[" + LS + " + _s], (i64)" + tab + "[2 * " + LS + " + _s], (i64)" + tab + "[3 * " + LS +
+                       " + _s], _maxp_" + T + ");");
+            cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
+        });
+    });
+}
+
+// The global group table T, its slot count symbol and status words, and the
+// per-thread locals every group-by phase on T uses.
+std::string bindGroupTable(HipCodegen& cg, const std::string& T, const std::string& slots) {
+    cg.addBufferParam(T, "u64", 8, slots + " * 4", InitKind::Pattern, wordsPattern({kEmptyWord, 0, kI64Max, kI64Min}),
+                      false, true);
+    cg.addSymbolParam("nslots_" + T, slots);
+    const std::string st = T + "_status";
     cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(st);
-    const std::string T = table_;
     cg.addLine("const u32 _bits_" + T + " = (u32)__builtin_ctzll(nslots_" + T + ");");
     cg.addLine("const u32 _maxp_" + T + " = (u32)(nslots_" + T + " < 65536ull ? nslots_" + T + " : 65536ull);");
     cg.addLine("u32 _ins_" + T + " = 0, _ovf_" + T + " = 0, _spill_" + T + " = 0;");
+    return st;
+}
+
+void setHashAggSchema(HipCodegen& cg, const std::string& T, const std::string& slots) {
+    auto& rs = cg.resultSchema();
+    rs.kind = HipResultSchema::HASH_AGG;
+    rs.hash.tableName = T;
+    rs.hash.slotsSymbol = slots;
+    rs.hash.slotWords = 4;
+    rs.hash.slots = {{"count", AggKind::Count, 0, 0},
+                     {"sum", AggKind::SumI64, 1, 0},
+                     {"min", AggKind::MinI64, 2, 0},
+                     {"max", AggKind::MaxI64, 3, 0}};
+}
+
+}  // namespace
+
+void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string T = table_;
+    const std::string st = bindGroupTable(cg, T, slots_);
     const std::string merge1 = "gdb_gb_merge(" + T + ", _bits_" + T + ", _gk, 1u, _gv, _gv, _gv, _maxp_" + T + ")";
     if (variant_ == LDS) {
-        int lbits = 0;
-        while ((1 << lbits) < ldsSlots_) ++lbits;
-        const std::string CAP = s(ldsSlots_);
-        std::string LS = CAP, LB = s(lbits) + "u";
+        const std::string CAP = s(ldsSlots_), tab = "_gl_" + T;
         cg.addComment("HashGroupBy " + T + ": LDS table of " + CAP + " slots absorbs hot keys; spills go global");
-        cg.declareLds("u64", "_gl_" + T, CAP + " * 4");
-        if (!ldsBitsScalar_.empty()) {
-            cg.addScalarParam(ldsBitsScalar_);
-            cg.addLine("const u32 _lb_" + T + " = (u32)(" + ldsBitsScalar_ + " < 1 ? 1 : (" + ldsBitsScalar_ + " > " +
-                       s(lbits) + " ? " + s(lbits) + " : " + ldsBitsScalar_ + "));");
-            cg.addLine("const int _ls_" + T + " = 1 << _lb_" + T + ";");
-            LS = "_ls_" + T;
-            LB = "_lb_" + T;
-        }
-        // word-major LDS table (word j of slot s at j * LS + s)
-        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
-            cg.addLine("_gl_" + T + "[_s] = GDB_EMPTY_WORD; _gl_" + T + "[" + LS + " + _s] = 0;");
-            cg.addLine("_gl_" + T + "[2 * " + LS + " + _s] = (u64)GDB_I64_MAX; _gl_" + T + "[3 * " + LS +
-                       " + _s] = (u64)GDB_I64_MIN;");
-        });
+        cg.declareLds("u64", tab, CAP + " * 4");
+        std::string LS, LB;
+        std::tie(LS, LB) = ldsTableSize(cg, T, ldsSlots_, ldsBitsScalar_);
+        emitLdsGbInit(cg, tab, LS);
         cg.addLine("__syncthreads();");
         child_->produce(cg, [&]() {
             cg.addLine("const u32 _gk = (u32)(" + key_ + ");");
             cg.addLine("const i64 _gv = (i64)(" + value_ + ");");
-            cg.addBlock("if (gdb_lds_gb_update(_gl_" + T + ", " + LB + ", _gk, _gv, " + s(window_) + "u) < 0)", [&]() {
+            cg.addBlock("if (gdb_lds_gb_update(" + tab + ", " + LB + ", _gk, _gv, " + s(window_) + "u) < 0)", [&]() {
                 cg.addLine("const int _r = " + merge1 + ";");
                 cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
                 cg.addLine("++_spill_" + T + ";");
@@ -1713,15 +1760,26 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             consume();
         });
         cg.addLine("__syncthreads();");
-        cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
-            cg.addLine("const u64 _w0 = _gl_" + T + "[_s];");
-            cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
-                cg.addLine("const int _r = gdb_gb_merge(" + T + ", _bits_" + T + ", (u32)_w0, (u32)(_w0 >> 32), (i64)_gl_" +
-                           T + "[" + LS + " + _s], (i64)_gl_" + T + "[2 * " + LS + " + _s], (i64)_gl_" + T + "[3 * " +
-                           LS + " + _s], _maxp_" + T + ");");
-                cg.addLine("if (_r < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_r;");
+        if (partialsMax_.empty()) {
+            emitLdsGbFlush(cg, T, tab, LS);
+        } else {
+            const std::string PT = T + "_pt", PN = T + "_ptn", NP = "nparts_" + T;
+            cg.addBufferParam(PT, "u64", 8, "(" + partialsMax_ + ") * " + CAP + " * 4", InitKind::None, {}, false, true);
+            cg.addBufferParam(PN, "u64", 8, "1", InitKind::Zero, {}, false, true);
+            cg.addSymbolParam(NP, partialsMax_);
+            cg.addComment("the LDS table becomes partial table _pi (word-major, " + CAP + " slots apart), if one is left");
+            cg.declareLds("u64", "_pti_" + T, "1");
+            cg.addLine("if (threadIdx.x == 0) _pti_" + T + "[0] = __hip_atomic_fetch_add(" + PN +
+                       ", 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);");
+            cg.addLine("__syncthreads();");
+            cg.addLine("const u64 _pi_" + T + " = _pti_" + T + "[0];");
+            cg.addBlock("if (_pi_" + T + " < " + NP + ")", [&]() {
+                cg.addLine("u64* const _pd = " + PT + " + _pi_" + T + " * " + CAP + "ull * 4ull;");
+                cg.addLine("for (int _s = (int)threadIdx.x; _s < 4 * " + LS + "; _s += (int)blockDim.x) _pd[_s] = " + tab +
+                           "[_s];");
             });
-        });
+            cg.addBlock("else", [&]() { emitLdsGbFlush(cg, T, tab, LS); });
+        }
     } else {
         cg.addComment("HashGroupBy " + T + ": every row merges into the global table");
         child_->produce(cg, [&]() {
@@ -1734,16 +1792,70 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     }
     cg.addComment("group-by status: [0] overflow flag, [2] keys inserted, [3] rows spilled from LDS");
     emitBlockStatus(cg, st, {{"_ins_" + T, 2, false}, {"_ovf_" + T, 0, true}, {"_spill_" + T, 3, false}});
+    setHashAggSchema(cg, T, slots_);
+}
 
-    auto& rs = cg.resultSchema();
-    rs.kind = HipResultSchema::HASH_AGG;
-    rs.hash.tableName = T;
-    rs.hash.slotsSymbol = slots_;
-    rs.hash.slotWords = 4;
-    rs.hash.slots = {{"count", AggKind::Count, 0, 0},
-                     {"sum", AggKind::SumI64, 1, 0},
-                     {"min", AggKind::MinI64, 2, 0},
-                     {"max", AggKind::MaxI64, 3, 0}};
+void HipGroupByMerge::produceImpl(HipCodegen& cg, ConsumerFn consume) {
+    const std::string T = table_, CAP = s(ldsSlots_), tab = "_gm_" + T;
+    if (ldsSlots_ < 64 || (ldsSlots_ & (ldsSlots_ - 1)))
+        throw std::runtime_error("HipGroupByMerge: LDS slots must be a power of two >= 64");
+    const std::string st = bindGroupTable(cg, T, slots_);
+    const std::string PT = T + "_pt", PN = T + "_ptn", NP = "nparts_" + T;
+    cg.addBufferParam(PT, "u64", 8, "(" + partialsMax_ + ") * " + CAP + " * 4", InitKind::None, {}, true, true);
+    cg.addBufferParam(PN, "u64", 8, "1", InitKind::Zero, {}, true, true);
+    cg.addSymbolParam(NP, partialsMax_);
+    cg.setPhaseRowsExpr("(" + partialsMax_ + ") * " + CAP);
+    cg.addComment("GroupByMerge " + T + ": partial LDS tables -> one LDS table per block -> the global table");
+    cg.declareLds("u64", tab, CAP + " * 4");
+    std::string LS, LB;
+    std::tie(LS, LB) = ldsTableSize(cg, T, ldsSlots_, ldsBitsScalar_);
+    emitLdsGbInit(cg, tab, LS);
+    cg.addLine("__syncthreads();");
+    cg.addLine("const u64 _np = " + PN + "[0] < " + NP + " ? " + PN + "[0] : " + NP + ";");
+    cg.addLine("const u64 _flat = _np * " + CAP + "ull;  // partial slots written");
+    auto body = [&]() {
+        cg.addLine("const u64 _s = _r & " + s(ldsSlots_ - 1) + "ull;");
+        cg.addBlock("if (_s < (u64)" + LS + ")", [&]() {
+            cg.addLine("const u64* const _p = " + PT + " + (_r - _s) * 4ull + _s;");
+            cg.addLine("const u64 _w0 = _p[0];");
+            cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+                cg.addLine("const u32 _k = (u32)_w0, _c = (u32)(_w0 >> 32);");
+                cg.addLine("const i64 _sm = (i64)_p[" + LS + "], _mn = (i64)_p[2 * " + LS + "], _mx = (i64)_p[3 * " + LS +
+                           "];");
+                // a bounded probe window: with more groups than LDS slots a
+                // full-table probe per missing key would cost thousands of
+                // LDS loads; such keys merge into the global table instead
+                cg.addBlock("if (gdb_lds_gb_merge_at(" + tab + ", (u32)" + LS + " - 1u, (u32)gdb_hash_slot(_k, " + LB +
+                                "), _k, _c, _sm, _mn, _mx, " + s(kMergeWindow) + "u) < 0)",
+                            [&]() {
+                                cg.addLine("const int _g = gdb_gb_merge(" + T + ", _bits_" + T +
+                                           ", _k, _c, _sm, _mn, _mx, _maxp_" + T + ");");
+                                cg.addLine("if (_g < 0) ++_ovf_" + T + "; else _ins_" + T + " += (u32)_g;");
+                            });
+            });
+        });
+    };
+    if (cg.rangeMode()) {
+        cg.addComment("work item: partial slots [_rb, _re)");
+        cg.addBlock("for (u64 _r = _rb + threadIdx.x; _r < (_re < _flat ? _re : _flat); _r += blockDim.x)", body);
+    } else {
+        cg.addComment("block: contiguous tiles of partial slots, tile-strided over the grid");
+        cg.addLine("const u64 _tile = (u64)blockDim.x * ITEMS;");
+        cg.addBlock("for (u64 _t0 = (u64)blockIdx.x * _tile; _t0 < _flat; _t0 += (u64)gridDim.x * _tile)", [&]() {
+            cg.addLine("const u64 _te = _t0 + _tile < _flat ? _t0 + _tile : _flat;");
+            cg.addBlock("for (u64 _r = _t0 + threadIdx.x; _r < _te; _r += blockDim.x)", body);
+        });
+    }
+    cg.addLine("__syncthreads();");
+    emitLdsGbFlush(cg, T, tab, LS);
+    cg.addComment("group-by status: [0] overflow flag, [2] keys inserted");
+    emitBlockStatus(cg, st, {{"_ins_" + T, 2, false}, {"_ovf_" + T, 0, true}});
+    consume();
+    setHashAggSchema(cg, T, slots_);
+}
+
+std::string HipGroupByMerge::describe() const {
+    return "GroupByMerge(" + table_ + ", partials<=" + partialsMax_ + ", lds" + s(ldsSlots_) + ")";
 }
 
 std::string HipHashGroupBy::describe() const {

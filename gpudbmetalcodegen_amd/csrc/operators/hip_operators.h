@@ -642,6 +642,13 @@ public:
     // global table (a long window absorbs more keys, a short one makes the
     // misses of high-cardinality keys cheap).
     void setWindow(int w) { window_ = w; }
+    // LDS variant: at the end of a block (or batched work item) store the
+    // LDS table whole as one of at most maxExpr partial tables instead of
+    // merging it into the global table slot by slot — with many blocks over
+    // few groups that merge is the same keys' atomics repeated once per
+    // block. HipGroupByMerge folds the partials into the table in the next
+    // phase; a block past the last partial merges directly as before.
+    void setPartials(const std::string& maxExpr) { partialsMax_ = maxExpr; }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
@@ -651,6 +658,29 @@ private:
     int ldsSlots_;
     std::string ldsBitsScalar_;
     int window_ = 16;
+    std::string partialsMax_;
+};
+
+// Second phase of HipHashGroupBy with partials: the partial LDS tables
+// (buffer <table>_pt, count in <table>_ptn) are folded into a block-local LDS
+// table — a block or work item covers a contiguous run of partials — which
+// is then merged into the global table, so a key costs one global merge per
+// block of this phase instead of one per block of the scan. Rows of the phase
+// = partial slots (maxExpr x ldsSlots); slots beyond the partials written, or
+// beyond the LDS slots in use, are skipped.
+class HipGroupByMerge : public HipOperator {
+public:
+    HipGroupByMerge(const std::string& table, const std::string& slotsExpr, int ldsSlots, const std::string& partialsMax)
+        : table_(table), slots_(slotsExpr), ldsSlots_(ldsSlots), partialsMax_(partialsMax) {}
+    void setRuntimeLdsBits(const std::string& scalarName) { ldsBitsScalar_ = scalarName; }
+    void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
+    std::string describe() const override;
+
+private:
+    static constexpr int kMergeWindow = 16;
+    std::string table_, slots_;
+    int ldsSlots_;
+    std::string partialsMax_, ldsBitsScalar_;
 };
 
 // Partitioned hash group-by: no counterpart in the reference, whose

@@ -483,13 +483,33 @@ HipQueryPlan buildPipeAgg(const PlanOptions& o) {
                                                HipHashGroupBy::LDS, optInt(o, "lds_cap", 2048));
     gb->setRuntimeLdsBits("pipe_lds_bits");
     gb->setWindow(optInt(o, "window", 4));  // measured best for the batched pipelines
+    // partials=N: each block (work item) stores its LDS table as one of at
+    // most min(N, rows / 16384 + 1) partial tables and PIPE_merge folds them
+    // into the table — one global merge per key and merge block instead of
+    // one per key and scan block
+    const int partials = optInt(o, "partials", 0);
+    const int cap = optInt(o, "lds_cap", 2048);
+    const std::string pmax = "min(" + std::to_string(partials) + ", n_P / 16384 + 1)";
+    if (partials > 0) gb->setPartials(pmax);
     auto& ph = appendPhase(plan, "PIPE_agg", std::move(gb), optInt(o, "block", 256), optInt(o, "items", 4),
                            {1, 2, 4, 8, 16});
     ph.launchBound = 1024;
     ph.scalarParams = {"pipe_date_lo", "pipe_date_hi", "pipe_flag"};
+    if (partials > 0) {
+        auto mg = std::make_unique<HipGroupByMerge>("d_pipe_ht", "pipe_slots", cap, pmax);
+        mg->setRuntimeLdsBits("pipe_lds_bits");
+        // a block folds ITEMS x block partial slots per tile (one 4096-slot
+        // partial at 256 x 16); at most 16 blocks, so a key is merged into
+        // the global table at most 16 times
+        auto& pm = appendPhase(plan, "PIPE_merge", std::move(mg), 256, 16, {16});
+        pm.launchBound = 1024;
+        pm.maxThreadgroups = 16;
+    }
     plan.requiredScalars = {"pipe_date_lo", "pipe_date_hi", "pipe_flag", "pipe_lds_bits"};
     plan.requiredSymbols = {"pipe_slots"};
-    plan.chunkable = true;
+    // partial tables are counted from zero once per run, so a chunked run
+    // (one scan + merge per chunk) would fold earlier chunks' partials again
+    plan.chunkable = partials == 0;
     return plan;
 }
 

@@ -1,17 +1,20 @@
 #pragma once
-// PipelineBatch — many independent pipelines in one persistent fused launch.
+// PipelineBatch — many independent pipelines in one persistent fused launch
+// per phase level.
 //
 // No counterpart in the reference, which dispatches one command buffer per
 // phase and query (codegen/execution/metal_generic_executor.cpp:338-390).
-// Each pipeline is a single-phase plan with its own runner (tables, scalars,
-// output buffers). The batch emits every distinct plan's phase body in range
-// mode (one block processes rows [_rb, _re) of one pipeline), packs each
-// pipeline's kernel arguments into a device table, splits every pipeline into
-// row-range work items, and launches one persistent kernel whose blocks take
-// work items from an atomic ticket until none are left. Launch shape per
-// pipeline (ITEMS) and the block size come from the autotuner; the fills and
-// the kernel can be captured in a graph and replayed.
-
+// Each pipeline is a plan of one or more phases with its own runner (tables,
+// scalars, output buffers). The batch emits every distinct plan phase's body
+// in range mode (one block processes rows [_rb, _re) of one pipeline's
+// phase), packs each pipeline's kernel arguments into a device table, splits
+// every pipeline phase into row-range work items, and launches one persistent
+// kernel per phase level (phase 0 of every pipeline, then phase 1 of those
+// that have one, ...) whose blocks take work items from an atomic ticket until
+// none are left; the kernel boundary orders a pipeline's phases. Launch shape
+// per pipeline (ITEMS of its first phase) and the block size come from the
+// autotuner; the fills and the kernels can be captured in a graph and
+// replayed.
 #include "../execution/hip_generic_executor.h"
 #include "../execution/runtime_compiler.h"
 #include "../planning/hip_plan_builder.h"
@@ -24,15 +27,22 @@
 namespace gdb {
 namespace runtime {
 
-// Source of the persistent batched kernel for a set of pipeline kinds (a
-// kind = one plan's phase body, instantiated for each ITEMS it is used with).
-// Needs no device, so the kernel can be emitted and compiled offline.
+// Source of the persistent batched kernels for a set of pipeline kinds (a
+// kind = one plan phase's body, instantiated for each ITEMS it is used with;
+// the phase index is the kind's level). Needs no device, so the kernels can
+// be emitted and compiled offline.
 class BatchKernelSource {
 public:
-    // Register a pipeline's plan and ITEMS; returns the kind index.
-    int add(const codegen::HipQueryPlan* plan, const codegen::HipCodegen* cg, int items);
+    // Register a pipeline's plan: phase 0 with the given ITEMS, later phases
+    // with their default ITEMS. Returns the kind index of every phase.
+    std::vector<int> add(const codegen::HipQueryPlan* plan, const codegen::HipCodegen* cg, int items);
+    // Every kind's body and one kernel per level: gdb_batch_kernel for level
+    // 0, gdb_batch_kernel_l<L> for level L > 0.
     std::string print() const;
+    static std::string kernelName(int level);
     size_t kinds() const { return kinds_.size(); }
+    int levels() const;
+    int level(int kind) const { return kinds_[static_cast<size_t>(kind)].level; }
     // Argument types of a kind's body, in binding order.
     const std::vector<std::string>& argTypes(int kind) const { return kinds_[static_cast<size_t>(kind)].argTypes; }
     const codegen::HipCodegen& kindCodegen(int kind) const { return *kindCg_[static_cast<size_t>(kind)]; }
@@ -44,8 +54,9 @@ public:
 
 private:
     struct Kind {
-        std::string key;   // plan source identity
+        std::string key;   // plan phase source identity
         std::string body;  // range-mode body name
+        int level = 0;     // phase index within its plan
         std::vector<std::string> argTypes;
         std::vector<int> items;
     };
@@ -60,16 +71,16 @@ public:
     PipelineBatch(const PipelineBatch&) = delete;
     PipelineBatch& operator=(const PipelineBatch&) = delete;
 
-    // Register a pipeline: its plan (one phase), the generated codegen of
-    // that plan, and the runner holding its registrations. `items` must be
-    // one of the phase's ITEMS variants. Returns the pipeline index.
+    // Register a pipeline: its plan, the generated codegen of that plan, and
+    // the runner holding its registrations. `items` (phase 0) must be one of
+    // the phase's ITEMS variants. Returns the pipeline index.
     int add(const codegen::HipQueryPlan* plan, const codegen::HipCodegen* cg, codegen::HipGenericExecutor* ex,
             int items);
     // Emit + compile the fused kernel, pack arguments, build work items.
     // chunkRows: target rows per work item (rounded to whole tiles); grid:
     // 0 = CUs x occupancy.
     void build(int block, uint64_t chunkRows, int grid);
-    // Fills of every pipeline's buffers + the fused kernel, on the stream.
+    // Fills of every pipeline's buffers + the fused kernels, on the stream.
     void enqueue();
     void capture();                         // record enqueue() into a graph
     // Run (graph replay when captured and useGraph), synchronise, check the
@@ -78,36 +89,42 @@ public:
     float run(bool useGraph);
 
     const std::string& source() const { return source_; }
-    size_t workItems() const { return itemPipe_.size(); }
-    unsigned grid() const { return grid_; }
+    // Work items over all levels; grid of the level-0 kernel.
+    size_t workItems() const;
+    unsigned grid() const { return levels_.empty() ? 0u : levels_[0].grid; }
 
 private:
     struct Pipe {
         const codegen::HipQueryPlan* plan;
         const codegen::HipCodegen* cg;
         codegen::HipGenericExecutor* ex;
-        int items;
-        int kind;
-        uint64_t rows;
+        std::vector<int> items;    // per phase
+        std::vector<int> kinds;    // per phase
+        std::vector<uint64_t> rows;  // per phase
+    };
+    // One persistent kernel: the work items of one phase level.
+    struct Level {
+        gdb_kernel kernel = nullptr;
+        unsigned grid = 0;
+        uint32_t argStride = 0;
+        std::vector<uint32_t> itemPipe;
+        std::vector<uint64_t> itemBegin, itemEnd;
+        void* dArgs = nullptr;
+        void* dKindItems = nullptr;
+        void* dItemPipe = nullptr;
+        void* dItemBegin = nullptr;
+        void* dItemEnd = nullptr;
     };
     gdb_ctx ctx_;
     gdb_stream stream_;
     codegen::RuntimeCompiler rc_;
     codegen::RuntimeCompiler::CompiledQuery cq_;
-    gdb_kernel kernel_ = nullptr;
+    bool built_ = false;
     std::vector<Pipe> pipes_;
     BatchKernelSource src_;
     std::string source_;
     int block_ = 256;
-    unsigned grid_ = 0;
-    uint32_t argStride_ = 0;
-    std::vector<uint32_t> itemPipe_;
-    std::vector<uint64_t> itemBegin_, itemEnd_;
-    void* dArgs_ = nullptr;
-    void* dKindItems_ = nullptr;
-    void* dItemPipe_ = nullptr;
-    void* dItemBegin_ = nullptr;
-    void* dItemEnd_ = nullptr;
+    std::vector<Level> levels_;
     void* dTicket_ = nullptr;
     // every pipeline's status and check words: gathered on the device after
     // the fused kernel and read back with one copy into pinned memory

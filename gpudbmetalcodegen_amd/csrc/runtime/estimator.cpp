@@ -13,7 +13,8 @@
 #include <fstream>
 #include <numeric>
 #include <stdexcept>
-#include <unordered_map>
+#include <climits>
+#include <vector>
 
 namespace gdb {
 namespace runtime {
@@ -51,42 +52,63 @@ FeatureOut computeFeatures(const FeatureInput& in) {
     FeatureOut out;
     const double n = static_cast<double>(in.nRows);
     const double range = static_cast<double>(std::max<int64_t>(1, in.keyMax - in.keyMin + 1));
-    std::unordered_map<int32_t, uint32_t> freq;
-    freq.reserve(static_cast<size_t>(in.sampleSize) * 2 + 1);
+    // Key frequencies of the (filtered) sample in an open-addressing table
+    // at most half full, then the frequencies' own histogram: the features
+    // need only counts of counts (singletons, pairs, the 8 largest, entropy),
+    // so each distinct count costs one log2 instead of each distinct key.
+    thread_local std::vector<int64_t> tkey;
+    thread_local std::vector<uint32_t> tcnt, mult;
+    size_t cap = 16;
+    while (cap < 2 * static_cast<size_t>(in.sampleSize) + 2) cap <<= 1;
+    tkey.assign(cap, INT64_MIN);
+    tcnt.assign(cap, 0u);
     uint64_t s = 0;
+    size_t d = 0;
     for (uint64_t i = 0; i < in.sampleSize; ++i) {
         if (in.sampleVals && in.sampleVals[i] >= in.valCut) continue;
-        ++freq[in.sampleKeys[i]];
+        const int64_t k = in.sampleKeys[i];
+        size_t h = (static_cast<uint64_t>(static_cast<uint32_t>(k)) * 0x9E3779B97F4A7C15ull) >> 20;
+        for (h &= cap - 1; tkey[h] != INT64_MIN && tkey[h] != k; h = (h + 1) & (cap - 1)) {
+        }
+        if (tkey[h] == INT64_MIN) {
+            tkey[h] = k;
+            ++d;
+        }
+        ++tcnt[h];
         ++s;
     }
     const double S = static_cast<double>(std::max<uint64_t>(1, in.sampleSize));
     const double sel = in.sampleSize ? static_cast<double>(s) / S : 1.0;
     const double nEst = std::max(1.0, n * sel);
     const double ub = std::max(1.0, std::min(nEst, range));
-    double d = static_cast<double>(freq.size()), f1 = 0, f2 = 0, ent = 0;
-    std::vector<uint32_t> counts;
-    counts.reserve(freq.size());
-    for (const auto& kv : freq) {
-        counts.push_back(kv.second);
-        if (kv.second == 1) f1 += 1;
-        if (kv.second == 2) f2 += 1;
-    }
-    std::sort(counts.begin(), counts.end(), std::greater<uint32_t>());
-    double top1 = counts.empty() ? 0 : counts[0], top8 = 0;
-    for (size_t i = 0; i < counts.size() && i < 8; ++i) top8 += counts[i];
+    mult.assign(static_cast<size_t>(s) + 1, 0u);  // mult[c] = keys seen c times
+    uint32_t maxc = 0;
+    for (size_t h = 0; h < cap; ++h)
+        if (tcnt[h]) {
+            ++mult[tcnt[h]];
+            maxc = std::max(maxc, tcnt[h]);
+        }
+    const double f1 = s >= 1 ? mult[1] : 0, f2 = s >= 2 ? mult[2] : 0;
     const double ss = std::max(1.0, static_cast<double>(s));
-    for (uint32_t c : counts) {
+    double top1 = maxc, top8 = 0, ent = 0;
+    uint32_t taken = 0;
+    for (uint32_t c = maxc; c >= 1; --c) {
+        if (!mult[c]) continue;
+        const uint32_t m = std::min<uint32_t>(mult[c], 8u - std::min(8u, taken));
+        top8 += static_cast<double>(m) * c;
+        taken += m;
         const double p = c / ss;
-        ent -= p * std::log2(p);
+        ent -= static_cast<double>(mult[c]) * p * std::log2(p);
     }
-    const double gee = std::sqrt(nEst / ss) * f1 + (d - f1);
-    const double chao = d + (f2 > 0 ? f1 * f1 / (2.0 * f2) : f1 * (f1 - 1.0) / 2.0);
+    const double dd = static_cast<double>(d);
+    const double gee = std::sqrt(nEst / ss) * f1 + (dd - f1);
+    const double chao = dd + (f2 > 0 ? f1 * f1 / (2.0 * f2) : f1 * (f1 - 1.0) / 2.0);
     Features& x = out.x;
     x[0] = static_cast<float>(lg2(n) / 32.0);
     x[1] = static_cast<float>(lg2(range) / 32.0);
     x[2] = static_cast<float>(sel);
     x[3] = static_cast<float>(lg2(nEst) / 32.0);
-    x[4] = static_cast<float>(d / ss);
+    x[4] = static_cast<float>(dd / ss);
     x[5] = static_cast<float>(f1 / ss);
     x[6] = static_cast<float>(f2 / ss);
     x[7] = static_cast<float>(top1 / ss);
@@ -94,7 +116,7 @@ FeatureOut computeFeatures(const FeatureInput& in) {
     x[9] = static_cast<float>(lg2(gee) / 32.0);
     x[10] = static_cast<float>(lg2(chao) / 32.0);
     x[11] = static_cast<float>(lg2(ub) / 32.0);
-    x[12] = static_cast<float>(lg2(d) / 32.0);
+    x[12] = static_cast<float>(lg2(dd) / 32.0);
     x[13] = static_cast<float>(ss > 1 ? ent / std::log2(ss) : 0.0);
     x[14] = clampf((lg2(nEst) - lg2(range)) / 10.0, -1.0, 1.0);
     x[15] = clampf((lg2(std::min(gee, ub)) - lg2(ub)) / 10.0, -2.0, 0.0);

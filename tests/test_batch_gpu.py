@@ -82,3 +82,32 @@ def test_autotune_then_batch(w3):
             assert same(got, exp)
     finally:
         b.close()
+
+
+@pytest.mark.parametrize("partials,bits", [(0, None), (256, None), (2, 6)])
+def test_partial_tables_and_direct_merges_agree(ctx, partials, bits):
+    """The aggregate pipelines' LDS tables reach the group table either as
+    partial tables folded by the merge phase (at most `partials` per
+    pipeline; the blocks past them merge directly) or merged by each block;
+    with 64-slot LDS tables most rows also spill to the group table. Every
+    combination equals the oracle, in the plain and the batched form."""
+    specs = [s for s in B.workload3(n_pipes=12, scale=0.01, seed=77) if s.kind == "agg"]
+    w = B.Workload3(ctx, specs, lds_cap=4096, partials=partials)
+    try:
+        if bits is not None:
+            for s in specs:
+                w.launch[s.index] = (256, 4, bits)
+        w.run_plain()
+        plain = w.results()
+        for s, got in zip(specs, plain):
+            assert same(got, expected(s)), s.index
+        b = w.build_batch(256, 4096)
+        try:
+            for _ in range(2):
+                b.run(graph=True)
+                for got, exp in zip(w.results(), plain):
+                    assert same(got, exp)
+        finally:
+            b.close()
+    finally:
+        w.close()

@@ -137,6 +137,27 @@ def test_batched_kernel_compiles_with_shared_lds(tmp_path):
     assert st == 0, hip.gdb_last_error().decode()[-2000:]
 
 
+def test_partial_table_plan_and_two_level_batch_kernel(tmp_path):
+    """PIPE_AGG with partials=N: the scan phase stores LDS tables as partial
+    tables and a PIPE_merge phase folds them (not chunkable: the partial
+    count restarts once per run); a batch of it emits one persistent kernel
+    per phase level, and the source compiles for gfx950."""
+    agg = E.build_hip_plan("PIPE_AGG", "lds_cap=4096,partials=256")
+    assert agg.phases() == ["PIPE_agg", "PIPE_merge"] and not agg.chunkable
+    src = agg.source()
+    assert "d_pipe_ht_pt" in src and "gdb_lds_gb_merge_at" in src
+    assert E.build_hip_plan("PIPE_AGG", "lds_cap=4096").phases() == ["PIPE_agg"]
+    semi = E.build_hip_plan("PIPE_SEMI")
+    bsrc = E.HipQueryPlan.batch_kernel_source([agg, agg, semi], [4, 16, 8])
+    assert "void __launch_bounds__(1024) gdb_batch_kernel(" in bsrc
+    assert "void __launch_bounds__(1024) gdb_batch_kernel_l1(" in bsrc
+    b = bsrc.encode()
+    hip, _ = N.load()
+    h, wc = C.c_uint64(), C.c_int()
+    st = hip.gdb_module_precompile(str(tmp_path).encode(), b"gfx950", b, len(b), None, 0, C.byref(h), C.byref(wc))
+    assert st == 0, hip.gdb_last_error().decode()[-2000:]
+
+
 def test_chunkable_plans():
     """Out-of-core eligibility follows the associative-output rule: plans
     whose stream-table phases only accumulate are chunkable; a plan whose

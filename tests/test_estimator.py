@@ -173,3 +173,43 @@ def test_gpu_training_reduces_loss_on_logged_runs(ctx):
     p, m, s = est.state()
     Xs = (runs["X"] - m) / s
     assert np.allclose(est.predict(runs["X"]), oracle.mlp_forward(p, Xs), rtol=1e-4, atol=1e-4)
+
+
+def test_features_equal_a_direct_restatement():
+    """The feature vector (counts of counts from an open-addressing table,
+    entropy summed per distinct count) equals a direct numpy restatement —
+    np.unique frequencies, per-key entropy — on random uniform and Zipf
+    samples with and without a value filter."""
+    import math
+
+    def direct(n, kmax, keys, vals=None, cut=(1 << 63) - 1):
+        S = max(1, len(keys))
+        if vals is not None:
+            keys = keys[vals < cut]
+        s = len(keys)
+        sel = s / S
+        n_est = max(1.0, n * sel)
+        ub = max(1.0, min(n_est, kmax))
+        u, c = np.unique(keys, return_counts=True)
+        d, f1, f2 = len(u), float((c == 1).sum()), float((c == 2).sum())
+        cs = np.sort(c)[::-1]
+        top1, top8 = (float(cs[0]) if d else 0.0), float(cs[:8].sum())
+        ss = max(1.0, s)
+        ent = float(-(c / ss * np.log2(c / ss)).sum()) if d else 0.0
+        lg2 = lambda x: math.log2(max(1.0, x))
+        gee = math.sqrt(n_est / ss) * f1 + (d - f1)
+        chao = d + (f1 * f1 / (2 * f2) if f2 > 0 else f1 * (f1 - 1) / 2)
+        x = [lg2(n) / 32, lg2(kmax) / 32, sel, lg2(n_est) / 32, d / ss, f1 / ss, f2 / ss, top1 / ss, top8 / ss,
+             lg2(gee) / 32, lg2(chao) / 32, lg2(ub) / 32, lg2(d) / 32, ent / math.log2(ss) if ss > 1 else 0.0,
+             min(1, max(-1, (lg2(n_est) - lg2(kmax)) / 10)), min(0, max(-2, (lg2(min(gee, ub)) - lg2(ub)) / 10))]
+        return np.array(x, np.float32), ub
+
+    r = np.random.default_rng(5)
+    for t in range(60):
+        m, K = int(r.integers(1, 9000)), int(r.integers(1, 100000))
+        keys = ((r.zipf(1.1 + r.random(), m) % K + 1) if t % 2 else r.integers(1, K + 1, m)).astype(np.int32)
+        vals = r.integers(0, 100, m).astype(np.int64) if t % 3 == 0 else None
+        cut = int(r.integers(1, 100)) if vals is not None else (1 << 63) - 1
+        x, ub = EST.features(10**7, 1, K, keys, vals, cut)
+        xr, ubr = direct(10**7, K, keys, vals, cut)
+        assert ub == ubr and np.array_equal(x, xr), t
