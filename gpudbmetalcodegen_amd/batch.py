@@ -96,7 +96,7 @@ class Workload3:
     """Device-resident pipelines, each with its own runner."""
 
     def __init__(self, ctx: E.Context, specs: list[PipeSpec], lds_cap: int = 4096, window: int = 4,
-                 partials: int = 256):
+                 partials: int = 256, lds_bitmap_words: int = 32768):
         """partials: in the batched form the aggregate pipelines store each
         work item's LDS table as a partial table, merged into the group table
         by a second phase (a second persistent kernel), up to this many per
@@ -109,7 +109,11 @@ class Workload3:
         self.lds_cap = lds_cap
         agg_opts = f"lds_cap={lds_cap},window={window}"
         kinds = {s.kind for s in specs}
-        self.plans = {k: E.build_hip_plan(plan_name(k), agg_opts if k == "agg" else "").compile(ctx) for k in kinds}
+        # batched join bodies keep the first 32768 bitmap words (1M keys) in
+        # the LDS the group-by bodies size the batched kernel for
+        join_opts = f"lds_bitmap_words={lds_bitmap_words}" if lds_bitmap_words else ""
+        self.plans = {k: E.build_hip_plan(plan_name(k), agg_opts if k == "agg" else join_opts).compile(ctx)
+                      for k in kinds}
         self.batch_plans = dict(self.plans)
         if partials > 0 and "agg" in kinds:
             self.batch_plans["agg"] = E.build_hip_plan("PIPE_AGG", f"{agg_opts},partials={partials}").compile(ctx)

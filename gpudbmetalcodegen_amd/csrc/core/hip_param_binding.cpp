@@ -11,6 +11,7 @@ namespace {
 //   expr := term (('+'|'-') term)*
 //   term := atom (('*'|'/') atom)*
 //   atom := number | symbol | '(' expr ')' | ('min'|'max') '(' expr ',' expr ')'
+//         | 'pow2' '(' expr ')'      (the smallest power of two >= expr, 1 for 0)
 class SizeParser {
 public:
     SizeParser(const std::string& s, const HipSizeResolver& r) : s_(s), r_(r) {}
@@ -87,6 +88,20 @@ private:
                 ++pos_;
             const std::string name = s_.substr(b, pos_ - b);
             skip();
+            if (name == "pow2" && pos_ < s_.size() && s_[pos_] == '(') {
+                ++pos_;
+                const uint64_t a = expr();
+                skip();
+                if (pos_ >= s_.size() || s_[pos_] != ')')
+                    throw std::runtime_error("HipSizeResolver: unbalanced parentheses in '" + s_ + "'");
+                ++pos_;
+                uint64_t p = 1;
+                while (p < a) {
+                    if (p >> 63) throw std::runtime_error("HipSizeResolver: pow2 overflow in '" + s_ + "'");
+                    p <<= 1;
+                }
+                return p;
+            }
             if ((name == "min" || name == "max") && pos_ < s_.size() && s_[pos_] == '(') {
                 ++pos_;
                 const uint64_t a = expr();

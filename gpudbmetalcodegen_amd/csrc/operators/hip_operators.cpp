@@ -492,6 +492,21 @@ HipBitmapProbe::HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::st
 
 void HipBitmapProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapReadParam(bitmap_, bits_);
+    const bool cache = ldsWords_ > 0 && cg.rangeMode();
+    const std::string C = "_bmc_" + bitmap_, CN = "_bmn_" + bitmap_;
+    if (cache) {
+        cg.addComment("the bitmap's first words in LDS for this work item");
+        cg.declareLds("u32", C, s(ldsWords_));
+        cg.addLine("const u64 " + CN + " = (nbits_" + bitmap_ + " + 31ull) >> 5 < " + s(ldsWords_) + "ull ? (nbits_" +
+                   bitmap_ + " + 31ull) >> 5 : " + s(ldsWords_) + "ull;");
+        cg.addLine("for (u64 _w = threadIdx.x; _w < " + CN + "; _w += blockDim.x) " + C + "[_w] = " + bitmap_ + "[_w];");
+        cg.addLine("__syncthreads();");
+    }
+    // the word holding bit k (k below nbits)
+    auto word = [&](const std::string& k) {
+        return cache ? "(" + k + " >> 5 < " + CN + " ? " + C + "[" + k + " >> 5] : " + bitmap_ + "[" + k + " >> 5])"
+                     : bitmap_ + "[" + k + " >> 5]";
+    };
     child_->produce(cg, [&]() {
         if (cg.staged()) {
             // all bitmap words of the batch first, then the bit tests
@@ -499,13 +514,20 @@ void HipBitmapProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             const std::string k = "((u64)(i64)(" + key_ + "))";
             cg.addLine("u32 " + A + "[ITEMS];");
             cg.rowLoop([&]() {
-                cg.addLine(A + "[_u] = " + k + " < nbits_" + bitmap_ + " ? " + bitmap_ + "[" + k + " >> 5] : 0u;");
+                cg.addLine(A + "[_u] = " + k + " < nbits_" + bitmap_ + " ? " + word(k) + " : 0u;");
             });
             cg.rowLoop([&]() {
                 cg.addLine("_act[_u] = ((" + A + "[_u] >> ((u32)" + k + " & 31u)) & 1u) " + (anti_ ? "==" : "!=") +
                            " 0u;");
             });
             consume();
+            return;
+        }
+        if (cache) {
+            const std::string k = "((u64)(i64)(" + key_ + "))";
+            cg.addIf("((" + k + " < nbits_" + bitmap_ + " ? " + word(k) + " : 0u) >> ((u32)" + k + " & 31u) & 1u) " +
+                         (anti_ ? "==" : "!=") + " 0u",
+                     [&]() { consume(); });
             return;
         }
         cg.addIf(std::string(anti_ ? "!" : "") + "gdb_bitmap_test(" + bitmap_ + ", nbits_" + bitmap_ + ", (i64)(" +
@@ -1774,9 +1796,16 @@ void HipHashGroupBy::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             cg.addLine("__syncthreads();");
             cg.addLine("const u64 _pi_" + T + " = _pti_" + T + "[0];");
             cg.addBlock("if (_pi_" + T + " < " + NP + ")", [&]() {
+                cg.addComment("key words of every slot; the other words of occupied slots only");
                 cg.addLine("u64* const _pd = " + PT + " + _pi_" + T + " * " + CAP + "ull * 4ull;");
-                cg.addLine("for (int _s = (int)threadIdx.x; _s < 4 * " + LS + "; _s += (int)blockDim.x) _pd[_s] = " + tab +
-                           "[_s];");
+                cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + LS + "; _s += (int)blockDim.x)", [&]() {
+                    cg.addLine("const u64 _w0 = " + tab + "[_s];");
+                    cg.addLine("_pd[_s] = _w0;");
+                    cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+                        cg.addLine("_pd[" + LS + " + _s] = " + tab + "[" + LS + " + _s]; _pd[2 * " + LS + " + _s] = " + tab +
+                                   "[2 * " + LS + " + _s]; _pd[3 * " + LS + " + _s] = " + tab + "[3 * " + LS + " + _s];");
+                    });
+                });
             });
             cg.addBlock("else", [&]() { emitLdsGbFlush(cg, T, tab, LS); });
         }
@@ -1879,33 +1908,54 @@ int PartitionConfig::maxParts() const {
 namespace {
 
 struct PartNames {
-    std::string P, NP, rows, hot, ccnt, pcnt, meta, rec, part;
+    std::string P, NP, PC, NPC, rows, hot, ccnt, pcnt, coff, poff, meta, rec, part;
 };
 
-// Bind the partition buffers and symbols of a phase. Sizes are expressions
-// of the group-table slots and the scanned row count, the same in every phase.
-PartNames bindPartition(HipCodegen& cg, const PartitionConfig& c, bool hotW, bool cntW, bool metaW, bool streamW) {
+// Which partition buffers a phase binds: 0 = none, 'r' = read, 'w' = written.
+struct PartUse {
+    char hot = 0, cnt = 0, off = 0, meta = 0, stream = 0;
+    bool np = false;   // binds the table's partition count (needs the table size)
+    bool npc = false;  // binds the first pass's count-partition count
+};
+
+// Bind the partition buffers and symbols a phase uses. Sizes are expressions
+// of the group-table slots, the count partitions and the scanned row count,
+// the same in every phase; the first pass binds nothing that depends on the
+// table size, so a pre-pass fused into it can size the table afterwards.
+PartNames bindPartition(HipCodegen& cg, const PartitionConfig& c, const PartUse& u) {
     PartNames n;
     const std::string px = c.prefix;
     const std::string LB = s(PartitionConfig::kLogicalBlocks), CAP = s(c.ldsSlots);
     n.P = "(" + c.slots + ") / " + s(PartitionConfig::kPartSlots);
+    n.PC = c.countParts.empty() ? n.P : c.countParts;
     n.NP = "nparts_" + px;
+    n.NPC = "ncparts_" + px;
     n.rows = "nrows_" + px;
     n.hot = px + "_hot";
     n.ccnt = px + "_ccnt";
     n.pcnt = px + "_pcnt";
+    n.coff = px + "_coff";
+    n.poff = px + "_poff";
     n.meta = px + "_meta";
     n.rec = px + "_rec";
     n.part = px + "_part";
-    cg.addSymbolParam(n.NP, n.P);
+    auto buf = [&](char use, const std::string& name, const char* type, size_t bytes, const std::string& size) {
+        if (use) cg.addBufferParam(name, type, bytes, size, InitKind::None, {}, use == 'r', true);
+    };
+    if (u.np) cg.addSymbolParam(n.NP, n.P);
+    if (u.npc) cg.addSymbolParam(n.NPC, n.PC);
     cg.addSymbolParam(n.rows, c.rowsSym);
-    cg.addBufferParam(n.hot, "u64", 8, LB + " * " + CAP + " * 4", InitKind::None, {}, !hotW, true);
-    cg.addBufferParam(n.ccnt, "u32", 4, LB + " * (" + n.P + ")", InitKind::None, {}, !cntW, true);
-    cg.addBufferParam(n.pcnt, "u32", 4, LB + " * (" + n.P + ")", InitKind::None, {}, !cntW, true);
-    cg.addBufferParam(n.meta, "u32", 4, "4 * (" + n.P + ")", InitKind::None, {}, !metaW, true);
+    buf(u.hot, n.hot, "u64", 8, LB + " * " + CAP + " * 4");
+    // cold-row and partial counts per (logical block, count partition); their
+    // exclusive offsets per (logical block, table partition)
+    buf(u.cnt, n.ccnt, "u32", 4, LB + " * (" + n.PC + ")");
+    buf(u.cnt, n.pcnt, "u32", 4, LB + " * (" + n.PC + ")");
+    buf(u.off, n.coff, "u32", 4, LB + " * (" + n.P + ")");
+    buf(u.off, n.poff, "u32", 4, LB + " * (" + n.P + ")");
+    buf(u.meta, n.meta, "u32", 4, "4 * (" + n.P + ")");
     // cold rows as 16-byte records (key, value): one store per row
-    cg.addBufferParam(n.rec, "u64", 8, "2 * (" + c.rowsSym + " + 1)", InitKind::None, {}, !streamW, true);
-    cg.addBufferParam(n.part, "u64", 8, LB + " * " + CAP + " * 4", InitKind::None, {}, !streamW, true);
+    buf(u.stream, n.rec, "u64", 8, "2 * (" + c.rowsSym + " + 1)");
+    buf(u.stream, n.part, "u64", 8, LB + " * " + CAP + " * 4");
     return n;
 }
 
@@ -1937,18 +1987,39 @@ HipPartitionCount::HipPartitionCount(std::unique_ptr<HipOperator> child, Partiti
 }
 
 void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const PartNames n = bindPartition(cg, c_, true, true, false, false);
+    PartUse u;
+    u.hot = 'w';
+    u.cnt = 'w';
+    u.npc = true;
+    const PartNames n = bindPartition(cg, c_, u);
     const std::string st = c_.table + "_status";
     cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(st);
     const std::string nRows = cg.addTableSizeParam(c_.scanTable);
     const std::string CAP = s(c_.ldsSlots), PMAX = s(c_.maxParts());
+    const bool flags = !c_.flags.empty();
+    std::string F, NF;
+    if (flags) {
+        // the exact distinct-count pre-pass, fused: one byte per key of the range
+        F = c_.flags;
+        NF = "nflags_" + F;
+        cg.addBufferParam(F, "u8", 1, c_.flagBits, InitKind::Zero);
+        cg.addSymbolParam(NF, c_.flagBits);
+        cg.addScalarParam(c_.flagMin);
+    }
+    // flag of u32 key k (a key of the signed 32-bit column): plain read, then
+    // a plain store (every writer stores the same byte)
+    auto setFlag = [&](const std::string& k) {
+        cg.addLine("{ const u64 _fk = (u64)((i64)(i32)(" + k + ") - " + c_.flagMin + "); if (_fk < " + NF + " && " + F +
+                   "[_fk] == 0) " + F + "[_fk] = 1; }");
+    };
     int lbits = 0;
     while ((1 << lbits) < c_.ldsSlots) ++lbits;
-    cg.addComment("pass 1: LDS table of " + CAP + " slots per logical block; cold rows counted per partition");
+    cg.addComment("pass 1: LDS table of " + CAP + " slots per logical block; cold rows counted per partition" +
+                  (flags ? "; key flags set (cold rows' keys here, the table's keys at the end)" : ""));
     cg.declareLds("u64", "_gl", CAP + " * 4");
     cg.declareLds("u32", "_pc", PMAX);
-    cg.addLine("const u64 _P = " + n.NP + ";");
+    cg.addLine("const u64 _P = " + n.NPC + ";  // count partitions");
     cg.addLine("const u32 _pb = (u32)__builtin_ctzll(_P);");
     cg.addLine("const bool _fits = _P <= " + PMAX + "ull;");
     cg.addLine("u32 _ovf = (!_fits && blockIdx.x == 0 && threadIdx.x == 0) ? 1u : 0u, _spill = 0;");
@@ -1966,6 +2037,7 @@ void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addBlock("if (gdb_lds_gb_update(_gl, " + s(lbits) + "u, _gk, _gv, " + s(c_.window) + "u) < 0)", [&]() {
             cg.addLine("gdb_lds_add_u32(&_pc[gdb_partition(_gk, _pb)], 1u);");
             cg.addLine("++_spill;");
+            if (flags) setFlag("_gk");
         });
         consume();
     });
@@ -1980,7 +2052,10 @@ void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addComment("partial-aggregate counts per partition");
     cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
         cg.addLine("const u64 _w0 = _gl[_s];");
-        cg.addLine("if ((u32)_w0 != GDB_EMPTY_KEY) gdb_lds_add_u32(&_pc[gdb_partition((u32)_w0, _pb)], 1u);");
+        cg.addBlock("if ((u32)_w0 != GDB_EMPTY_KEY)", [&]() {
+            cg.addLine("gdb_lds_add_u32(&_pc[gdb_partition((u32)_w0, _pb)], 1u);");
+            if (flags) setFlag("(u32)_w0");
+        });
     });
     cg.addLine("__syncthreads();");
     cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) " + n.pcnt + "[_lb * _P + _q] = _pc[_q];");
@@ -1992,29 +2067,73 @@ void HipPartitionCount::produceImpl(HipCodegen& cg, ConsumerFn consume) {
 
 std::string HipPartitionCount::describe() const {
     return "PartitionCount(" + c_.prefix + ", key=" + key_ + ", value=" + value_ + ", slots=" + c_.slots + ", lds" +
-           s(c_.ldsSlots) + ")";
+           s(c_.ldsSlots) + (c_.countParts.empty() ? "" : ", count parts " + c_.countParts) +
+           (c_.flags.empty() ? "" : ", flags " + c_.flags) + ")";
 }
 
 void HipPartitionOffsets::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const PartNames n = bindPartition(cg, c_, false, true, true, false);
-    cg.setPhaseRowsExpr(n.P);
-    cg.addComment("counts -> exclusive offsets over the logical blocks, per partition; totals");
-    cg.addLine("const u64 _P = " + n.NP + ";");
-    cg.addBlock("for (u64 _q = (u64)blockIdx.x * blockDim.x + threadIdx.x; _q < _P; _q += (u64)gridDim.x * blockDim.x)",
-                [&]() {
-                    cg.addLine("u32 _c = 0, _r = 0;");
-                    cg.addBlock("for (u64 _lb = 0; _lb < " + s(PartitionConfig::kLogicalBlocks) + "; ++_lb)", [&]() {
-                        cg.addLine("const u32 _a = " + n.ccnt + "[_lb * _P + _q], _b = " + n.pcnt + "[_lb * _P + _q];");
-                        cg.addLine(n.ccnt + "[_lb * _P + _q] = _c; " + n.pcnt + "[_lb * _P + _q] = _r;");
-                        cg.addLine("_c += _a; _r += _b;");
-                    });
-                    cg.addLine(n.meta + "[_q] = _c; " + n.meta + "[_P + _q] = _r;");
-                });
+    PartUse u;
+    u.cnt = 'r';
+    u.off = 'w';
+    u.meta = 'w';
+    u.np = true;
+    u.npc = true;
+    const PartNames n = bindPartition(cg, c_, u);
+    const std::string st = c_.table + "_status";
+    cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
+    cg.setPhaseStatusBuffer(st);
+    const std::string LB = s(PartitionConfig::kLogicalBlocks);
+    cg.setPhaseRowsExpr("(" + n.P + ") * " + LB);
+    cg.addComment("one block per partition: each logical block's counts (the sum of the partition's consecutive "
+                  "count partitions), exclusive offsets over the logical blocks by a block scan, and totals");
+    cg.declareLds("u32", "_sa", "1024");
+    cg.declareLds("u32", "_sb", "1024");
+    cg.addLine("const u64 _P = " + n.NP + ", _PC = " + n.NPC + ";");
+    cg.addLine("const bool _ok = _P <= _PC && _P <= " + s(c_.maxParts()) + "ull;");
+    cg.addLine("const u64 _f = _ok ? _PC / _P : 1;");
+    cg.addBlock("if (!_ok && blockIdx.x == 0 && threadIdx.x == 0)", [&]() {
+        cg.addComment("more table partitions than count partitions or LDS counters: [0]+[1]");
+        cg.addLine("gdb_atomic_max_u64(" + st + " + 0, 1ull); gdb_atomic_max_u64(" + st + " + 1, 1ull);");
+    });
+    cg.addLine("const u64 _k = (" + LB + " + blockDim.x - 1) / blockDim.x;  // logical blocks per thread");
+    cg.addBlock("for (u64 _q = blockIdx.x; _ok && _q < _P; _q += gridDim.x)", [&]() {
+        cg.addLine("const u64 _l0 = threadIdx.x * _k, _l1 = _l0 + _k < " + LB + " ? _l0 + _k : " + LB + ";");
+        cg.addLine("u32 _a = 0, _b = 0;");
+        cg.addBlock("for (u64 _lb = _l0; _lb < _l1; ++_lb)", [&]() {
+            cg.addLine("for (u64 _j = _q * _f; _j < (_q + 1) * _f; ++_j) { _a += " + n.ccnt + "[_lb * _PC + _j]; _b += " +
+                       n.pcnt + "[_lb * _PC + _j]; }");
+        });
+        cg.addLine("_sa[threadIdx.x] = _a; _sb[threadIdx.x] = _b;");
+        cg.addLine("__syncthreads();");
+        cg.addComment("inclusive Hillis-Steele scan of the per-thread sums");
+        cg.addBlock("for (u32 _d = 1; _d < blockDim.x; _d <<= 1)", [&]() {
+            cg.addLine("const u32 _xa = threadIdx.x >= _d ? _sa[threadIdx.x - _d] : 0u;");
+            cg.addLine("const u32 _xb = threadIdx.x >= _d ? _sb[threadIdx.x - _d] : 0u;");
+            cg.addLine("__syncthreads();");
+            cg.addLine("_sa[threadIdx.x] += _xa; _sb[threadIdx.x] += _xb;");
+            cg.addLine("__syncthreads();");
+        });
+        cg.addLine("u32 _c = _sa[threadIdx.x] - _a, _r = _sb[threadIdx.x] - _b;");
+        cg.addBlock("for (u64 _lb = _l0; _lb < _l1; ++_lb)", [&]() {
+            cg.addLine("u32 _x = 0, _y = 0;");
+            cg.addLine("for (u64 _j = _q * _f; _j < (_q + 1) * _f; ++_j) { _x += " + n.ccnt + "[_lb * _PC + _j]; _y += " +
+                       n.pcnt + "[_lb * _PC + _j]; }");
+            cg.addLine(n.coff + "[_lb * _P + _q] = _c; " + n.poff + "[_lb * _P + _q] = _r;");
+            cg.addLine("_c += _x; _r += _y;");
+        });
+        cg.addBlock("if (threadIdx.x == blockDim.x - 1)", [&]() {
+            cg.addLine(n.meta + "[_q] = _sa[threadIdx.x]; " + n.meta + "[_P + _q] = _sb[threadIdx.x];");
+        });
+        cg.addLine("__syncthreads();");
+    });
     consume();
 }
 
 void HipPartitionBases::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const PartNames n = bindPartition(cg, c_, false, false, true, false);
+    PartUse u;
+    u.meta = 'w';
+    u.np = true;
+    const PartNames n = bindPartition(cg, c_, u);
     cg.setPhaseRowsExpr("1");
     cg.setPhaseMaxThreadgroups(1);
     cg.addComment("one block: exclusive scans of the partition totals -> partition bases");
@@ -2051,7 +2170,13 @@ HipPartitionScatter::HipPartitionScatter(std::unique_ptr<HipOperator> child, Par
     : HipUnaryOperator(std::move(child)), c_(std::move(cfg)), key_(keyExpr), value_(valueExpr) {}
 
 void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const PartNames n = bindPartition(cg, c_, false, false, false, true);
+    PartUse u;
+    u.hot = 'r';
+    u.off = 'r';
+    u.meta = 'r';
+    u.stream = 'w';
+    u.np = true;
+    const PartNames n = bindPartition(cg, c_, u);
     const std::string nRows = cg.addTableSizeParam(c_.scanTable);
     const std::string CAP = s(c_.ldsSlots), PMAX = s(c_.maxParts());
     int lbits = 0;
@@ -2064,7 +2189,7 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     openLogicalBlocks(cg, nRows, "_P <= " + PMAX + "ull");
     cg.addLine("const u64* _h = " + n.hot + " + _lb * " + CAP + " * 4;");
     cg.addLine("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x) _hk[_s] = (u32)_h[4 * _s];");
-    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = " + n.meta + "[2 * _P + _q] + " + n.ccnt +
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = " + n.meta + "[2 * _P + _q] + " + n.coff +
                "[_lb * _P + _q];");
     cg.addLine("__syncthreads();");
     cg.setScanSegment(true);
@@ -2081,7 +2206,7 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.setScanSegment(false);
     cg.addLine("__syncthreads();");
     cg.addComment("the block's hot partial aggregates, packed by partition");
-    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = " + n.meta + "[3 * _P + _q] + " + n.pcnt +
+    cg.addLine("for (u64 _q = threadIdx.x; _q < _P; _q += blockDim.x) _pc[_q] = " + n.meta + "[3 * _P + _q] + " + n.poff +
                "[_lb * _P + _q];");
     cg.addLine("__syncthreads();");
     cg.addBlock("for (int _s = (int)threadIdx.x; _s < " + CAP + "; _s += (int)blockDim.x)", [&]() {
@@ -2102,7 +2227,11 @@ std::string HipPartitionScatter::describe() const {
 }
 
 void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
-    const PartNames n = bindPartition(cg, c_, false, false, false, false);
+    PartUse u;
+    u.meta = 'r';
+    u.stream = 'r';
+    u.np = true;
+    const PartNames n = bindPartition(cg, c_, u);
     const std::string T = c_.table;
     const std::string PS = s(PartitionConfig::kPartSlots), PM = s(PartitionConfig::kPartSlots - 1);
     cg.setPhaseRowsExpr(c_.slots);  // one tile (blockDim x ITEMS = 4096) of the slot range per partition

@@ -177,12 +177,19 @@ public:
     bool stageable() const override { return true; }
     HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
                    const std::string& bitsExpr, bool anti = false);
+    // Batched (range-mode) bodies only: copy the bitmap's first `words`
+    // words into LDS at the start of each work item and test keys below them
+    // there. A batched kernel's LDS is sized by its largest body (a group-by
+    // table), so a join item can use it without lowering occupancy; plain
+    // launches keep their small LDS footprint.
+    void setLdsCache(int words) { ldsWords_ = words; }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
 private:
     std::string bitmap_, key_, bits_;
     bool anti_;
+    int ldsWords_ = 0;
 };
 
 // Open-addressing hash build of unique keys with a 32-bit payload. Replaces
@@ -716,6 +723,17 @@ struct PartitionConfig {
     // whose window is full is cold. Short, because most rows of a
     // high-cardinality column are cold and pay the whole window in LDS loads.
     int window = 2;
+    // Partitions pass 1 counts cold rows and partials at (an expression;
+    // empty = the table's partitions). When it is larger, the offsets pass
+    // sums each table partition's consecutive count partitions (a key's
+    // partition at 2^b partitions is its partition at 2^c >= 2^b shifted
+    // right by c - b), so pass 1 need not know the table size.
+    std::string countParts;
+    // Pass 1 also sets a byte flag (buffer `flags`, index key - flagMin, a
+    // scalar; flagBits flags) for every key it sees — cold rows' keys as they
+    // come, the LDS table's keys at the end: the exact distinct-count pre-pass
+    // fused into pass 1. Empty = off.
+    std::string flags, flagMin, flagBits;
     static constexpr int kPartSlots = 4096;
     static constexpr int kLogicalBlocks = 256;
     // Largest partition count whose LDS counters fit beside the LDS table.

@@ -277,6 +277,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
 // HipPartitionCount); it needs gb_slots >= 4096, gb_slots / 4096 partitions
 // within PartitionConfig::maxParts() and the symbol gbp_rows (rows of G).
 void gbCountPhases(HipQueryPlan& plan, bool filter);
+void gbCountFlagsPhase(HipQueryPlan& plan);
 
 HipQueryPlan buildGroupBy(const PlanOptions& o) {
     HipQueryPlan plan;
@@ -285,10 +286,14 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
     const bool prepass = opt(o, "sizing", "given") == "prepass";
     const std::string variant = opt(o, "variant", "lds");
     const bool part = variant == "part";
-    if (prepass) {
+    // the partitioned form fuses the pre-pass's key flags into its first pass
+    // (fused=0: the separate byte-flag pass first, as for the other variants)
+    const bool fused = part && prepass && optInt(o, "fused", 1) != 0;
+    const HipQueryPlan::SizingRule sizing{"GBC_count", "gb_slots", "d_gb_distinct", 0, 2,
+                            part ? static_cast<uint64_t>(PartitionConfig::kPartSlots) : 64u, true};
+    if (prepass && !fused) {
         gbCountPhases(plan, filter);
-        plan.sizingRules.push_back(
-            {"GBC_count", "gb_slots", "d_gb_distinct", 0, 2, part ? static_cast<uint64_t>(PartitionConfig::kPartSlots) : 64u, true});
+        plan.sizingRules.push_back(sizing);
     }
     auto scan = makeScan("G", {"g_key", "g_val"});
     std::unique_ptr<HipOperator> cur = std::move(scan);
@@ -303,13 +308,28 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
         pc.scanTable = "G";
         pc.ldsSlots = ldsSlots;
         pc.window = optInt(o, "window", 2);
+        if (fused) {
+            // the table size (and so its partition count) is known only after
+            // the count: count at the most partitions the table could need —
+            // 2 x min(keys in range, rows) slots — within the LDS counters
+            pc.countParts = "min(" + std::to_string(pc.maxParts()) +
+                            ", max(1, pow2(2 * min(gb_key_bits, gbp_rows)) / " +
+                            std::to_string(PartitionConfig::kPartSlots) + "))";
+            pc.flags = "d_gb_key_flags";
+            pc.flagMin = "gb_key_min";
+            pc.flagBits = "gb_key_bits";
+        }
         const int block = optInt(o, "block", 1024);
         auto cnt = std::make_unique<HipPartitionCount>(std::move(cur), pc, "g_key", "g_val");
         auto& p1 = appendPhase(plan, "GBP_count", std::move(cnt), block, optInt(o, "items", 4), {4});
         p1.launchBound = 1024;
         if (filter) p1.scalarParams = {"gb_val_cut"};  // (phase references die at the next append)
+        if (fused) {
+            gbCountFlagsPhase(plan);
+            plan.sizingRules.push_back(sizing);
+        }
         auto& p2 = appendPhase(plan, "GBP_offsets", std::make_unique<HipPartitionOffsets>(pc), 256, 1, {1});
-        p2.launchBound = 256;
+        p2.launchBound = 256;  // one block per partition, one thread per logical block
         auto& p3 = appendPhase(plan, "GBP_bases", std::make_unique<HipPartitionBases>(pc), 1024, 1, {1});
         p3.launchBound = 1024;
         // pass 4 re-reads the rows (same scan, same filter)
@@ -379,12 +399,15 @@ void gbCountPhases(HipQueryPlan& plan, bool filter) {
         ph.scalarParams = {"gb_key_min"};
         if (filter) ph.scalarParams.push_back("gb_val_cut");
     }
-    {
-        auto scan = makeScan("GBFL", {"gbfl_flag"});
-        auto agg = std::make_unique<HipKeyedAgg>(std::move(scan), "d_gb_distinct", "0", 1);
-        agg->addAggregate("distinct_keys", AggKind::SumI64, "(i64)(gbfl_flag != 0)", 0);
-        appendPhase(plan, "GBC_count", std::move(agg), 256, 16, {16});
-    }
+    gbCountFlagsPhase(plan);
+}
+
+// GBC_count: the set key flags (d_gb_key_flags, scanned as table GBFL) counted.
+void gbCountFlagsPhase(HipQueryPlan& plan) {
+    auto scan = makeScan("GBFL", {"gbfl_flag"});
+    auto agg = std::make_unique<HipKeyedAgg>(std::move(scan), "d_gb_distinct", "0", 1);
+    agg->addAggregate("distinct_keys", AggKind::SumI64, "(i64)(gbfl_flag != 0)", 0);
+    appendPhase(plan, "GBC_count", std::move(agg), 256, 16, {16});
     plan.bufferTables.push_back({"GBFL", "gbfl_flag", "d_gb_key_flags", "gb_key_bits"});
 }
 
@@ -520,6 +543,8 @@ HipQueryPlan buildPipeJoin(const PlanOptions& o, bool anti) {
     plan.name = anti ? "PIPE_ANTI" : "PIPE_SEMI";
     auto scan = makeScan("P", {"p_key", "p_val"});
     auto probe = std::make_unique<HipBitmapProbe>(std::move(scan), "d_pipe_bitmap", "p_key", "pipe_bits", anti);
+    // lds_bitmap_words=N: batched bodies test keys below 32N bits in LDS
+    probe->setLdsCache(optInt(o, "lds_bitmap_words", 0));
     auto agg = std::make_unique<HipKeyedAgg>(std::move(probe), anti ? "d_pipe_anti" : "d_pipe_semi", "0", 1);
     agg->addAggregate("rows", AggKind::Count, "1", 0);
     agg->addAggregate("sum_val", AggKind::SumI64, "p_val", 0);

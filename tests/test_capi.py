@@ -158,6 +158,17 @@ def test_partial_table_plan_and_two_level_batch_kernel(tmp_path):
     assert st == 0, hip.gdb_last_error().decode()[-2000:]
 
 
+def test_join_bitmap_lds_cache_only_in_batched_bodies():
+    """lds_bitmap_words=N keeps a bitmap prefix in LDS only in the batched
+    (range-mode) body, whose LDS a group-by body sizes anyway; the plain
+    launch is unchanged."""
+    semi = E.build_hip_plan("PIPE_SEMI", "lds_bitmap_words=32768")
+    assert "_bmc_" not in semi.source()
+    agg = E.build_hip_plan("PIPE_AGG", "lds_cap=4096")
+    src = E.HipQueryPlan.batch_kernel_source([agg, semi], [4, 8])
+    assert "_bmc_d_pipe_bitmap[_w] = d_pipe_bitmap[_w]" in src
+
+
 def test_chunkable_plans():
     """Out-of-core eligibility follows the associative-output rule: plans
     whose stream-table phases only accumulate are chunkable; a plan whose
