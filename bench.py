@@ -327,7 +327,7 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     plain_ms = sum(rp["wall"]) / steps
     groups = run.result.rows()
     phases = rp["phases"]
-    out["plain"] = {"sizing": "exact distinct count fused into the first pass (key flags set by GBP_count, "
+    out["plain"] = {"sizing": "exact distinct count fused into the first pass (key flags set by GBP_count_flags, "
                               "counted by GBC_count, host read of the count)",
                     "ms_per_step": plain_ms, "rows_per_s": n / (plain_ms / 1e3), "slots": run.slots,
                     "trials": {"wall_ms": ST.trial_stats(rp["wall"]), "gpu_ms": ST.trial_stats(rp["gpu"])},
@@ -337,7 +337,7 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
     # hot partials and writes the table once
     st = t.status
     cold = int(st[-1][3])
-    pbytes = {"GBP_count": 12.0 * n, "GBP_scatter": 12.0 * n + 12.0 * cold,
+    pbytes = {"GBP_count_flags": 12.0 * n + cold, "GBP_scatter": 12.0 * n + 12.0 * cold,
               "GBP_agg": 12.0 * cold + 32.0 * run.slots}
     dom = max(pbytes, key=lambda k: phases[k])
     out["roofline"] = roofline(dom, phases[dom], pbytes[dom], "w2")

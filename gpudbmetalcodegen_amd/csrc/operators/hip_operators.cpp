@@ -2083,48 +2083,47 @@ void HipPartitionOffsets::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBufferParam(st, "u64", 8, "4", InitKind::Zero, {}, false, true);
     cg.setPhaseStatusBuffer(st);
     const std::string LB = s(PartitionConfig::kLogicalBlocks);
-    cg.setPhaseRowsExpr("(" + n.P + ") * " + LB);
-    cg.addComment("one block per partition: each logical block's counts (the sum of the partition's consecutive "
-                  "count partitions), exclusive offsets over the logical blocks by a block scan, and totals");
-    cg.declareLds("u32", "_sa", "1024");
-    cg.declareLds("u32", "_sb", "1024");
+    cg.setPhaseRowsExpr("(" + n.P + ") * 64");
+    cg.addComment("counts -> exclusive offsets over the logical blocks, per partition, and totals; a partition's "
+                  "count is the sum of its consecutive count partitions");
     cg.addLine("const u64 _P = " + n.NP + ", _PC = " + n.NPC + ";");
     cg.addLine("const bool _ok = _P <= _PC && _P <= " + s(c_.maxParts()) + "ull;");
     cg.addLine("const u64 _f = _ok ? _PC / _P : 1;");
-    cg.addBlock("if (!_ok && blockIdx.x == 0 && threadIdx.x == 0)", [&]() {
-        cg.addComment("more table partitions than count partitions or LDS counters: [0]+[1]");
-        cg.addLine("gdb_atomic_max_u64(" + st + " + 0, 1ull); gdb_atomic_max_u64(" + st + " + 1, 1ull);");
+    cg.addLine("const u64 _gt = (u64)blockIdx.x * blockDim.x + threadIdx.x, _nt = (u64)gridDim.x * blockDim.x;");
+    cg.addBlock("if (!_ok)", [&]() {
+        cg.addComment("more table partitions than count partitions or LDS counters: [0]+[1], and zero offsets "
+                      "and totals, so the later passes of this (failed) run stay within their buffers");
+        cg.addLine("if (_gt == 0) { gdb_atomic_max_u64(" + st + " + 0, 1ull); gdb_atomic_max_u64(" + st + " + 1, 1ull); }");
+        cg.addLine("for (u64 _i = _gt; _i < " + LB + " * _P; _i += _nt) { " + n.coff + "[_i] = 0; " + n.poff + "[_i] = 0; }");
+        cg.addLine("for (u64 _i = _gt; _i < 2 * _P; _i += _nt) " + n.meta + "[_i] = 0;");
     });
-    cg.addLine("const u64 _k = (" + LB + " + blockDim.x - 1) / blockDim.x;  // logical blocks per thread");
-    cg.addBlock("for (u64 _q = blockIdx.x; _ok && _q < _P; _q += gridDim.x)", [&]() {
-        cg.addLine("const u64 _l0 = threadIdx.x * _k, _l1 = _l0 + _k < " + LB + " ? _l0 + _k : " + LB + ";");
-        cg.addLine("u32 _a = 0, _b = 0;");
-        cg.addBlock("for (u64 _lb = _l0; _lb < _l1; ++_lb)", [&]() {
-            cg.addLine("for (u64 _j = _q * _f; _j < (_q + 1) * _f; ++_j) { _a += " + n.ccnt + "[_lb * _PC + _j]; _b += " +
-                       n.pcnt + "[_lb * _PC + _j]; }");
+    cg.addBlock("if (_f == 1)", [&]() {
+        cg.addComment("one thread per partition: consecutive lanes read consecutive counters");
+        cg.addBlock("for (u64 _q = _gt; _ok && _q < _P; _q += _nt)", [&]() {
+            cg.addLine("u32 _c = 0, _r = 0;");
+            cg.addBlock("for (u64 _lb = 0; _lb < " + LB + "; ++_lb)", [&]() {
+                cg.addLine("const u32 _a = " + n.ccnt + "[_lb * _PC + _q], _b = " + n.pcnt + "[_lb * _PC + _q];");
+                cg.addLine(n.coff + "[_lb * _P + _q] = _c; " + n.poff + "[_lb * _P + _q] = _r;");
+                cg.addLine("_c += _a; _r += _b;");
+            });
+            cg.addLine(n.meta + "[_q] = _c; " + n.meta + "[_P + _q] = _r;");
         });
-        cg.addLine("_sa[threadIdx.x] = _a; _sb[threadIdx.x] = _b;");
-        cg.addLine("__syncthreads();");
-        cg.addComment("inclusive Hillis-Steele scan of the per-thread sums");
-        cg.addBlock("for (u32 _d = 1; _d < blockDim.x; _d <<= 1)", [&]() {
-            cg.addLine("const u32 _xa = threadIdx.x >= _d ? _sa[threadIdx.x - _d] : 0u;");
-            cg.addLine("const u32 _xb = threadIdx.x >= _d ? _sb[threadIdx.x - _d] : 0u;");
-            cg.addLine("__syncthreads();");
-            cg.addLine("_sa[threadIdx.x] += _xa; _sb[threadIdx.x] += _xb;");
-            cg.addLine("__syncthreads();");
+    });
+    cg.addBlock("else", [&]() {
+        cg.addComment("one wave per partition: the lanes sum its count partitions");
+        cg.addLine("const u64 _lane = threadIdx.x & 63u;");
+        cg.addBlock("for (u64 _q = _gt >> 6; _ok && _q < _P; _q += _nt >> 6)", [&]() {
+            cg.addLine("u64 _c = 0, _r = 0;");
+            cg.addBlock("for (u64 _lb = 0; _lb < " + LB + "; ++_lb)", [&]() {
+                cg.addLine("u64 _a = 0, _b = 0;");
+                cg.addLine("for (u64 _j = _lane; _j < _f; _j += 64) { _a += " + n.ccnt + "[_lb * _PC + _q * _f + _j]; _b += " +
+                           n.pcnt + "[_lb * _PC + _q * _f + _j]; }");
+                cg.addLine("_a = gdb_wave_sum_u64(_a); _b = gdb_wave_sum_u64(_b);");
+                cg.addLine("if (_lane == 0) { " + n.coff + "[_lb * _P + _q] = (u32)_c; " + n.poff + "[_lb * _P + _q] = (u32)_r; }");
+                cg.addLine("_c += _a; _r += _b;");
+            });
+            cg.addLine("if (_lane == 0) { " + n.meta + "[_q] = (u32)_c; " + n.meta + "[_P + _q] = (u32)_r; }");
         });
-        cg.addLine("u32 _c = _sa[threadIdx.x] - _a, _r = _sb[threadIdx.x] - _b;");
-        cg.addBlock("for (u64 _lb = _l0; _lb < _l1; ++_lb)", [&]() {
-            cg.addLine("u32 _x = 0, _y = 0;");
-            cg.addLine("for (u64 _j = _q * _f; _j < (_q + 1) * _f; ++_j) { _x += " + n.ccnt + "[_lb * _PC + _j]; _y += " +
-                       n.pcnt + "[_lb * _PC + _j]; }");
-            cg.addLine(n.coff + "[_lb * _P + _q] = _c; " + n.poff + "[_lb * _P + _q] = _r;");
-            cg.addLine("_c += _x; _r += _y;");
-        });
-        cg.addBlock("if (threadIdx.x == blockDim.x - 1)", [&]() {
-            cg.addLine(n.meta + "[_q] = _sa[threadIdx.x]; " + n.meta + "[_P + _q] = _sb[threadIdx.x];");
-        });
-        cg.addLine("__syncthreads();");
     });
     consume();
 }
@@ -2252,7 +2251,9 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             cg.addLine("_pl[2 * " + PS + " + _s] = (u64)GDB_I64_MAX; _pl[3 * " + PS + " + _s] = (u64)GDB_I64_MIN;");
         });
         cg.addLine("__syncthreads();");
-        cg.addLine("const u64 _cb = " + n.meta + "[2 * _P + _p], _cn = " + n.meta + "[_p];");
+        cg.addLine("u64 _cb = " + n.meta + "[2 * _P + _p], _cn = " + n.meta + "[_p];");
+        cg.addComment("records beyond the buffers can only come from a failed earlier pass: reported, not read");
+        cg.addLine("if (_cb + _cn > " + n.rows + ") { _cn = 0; if (threadIdx.x == 0) ++_ovf; }");
         cg.addBlock("for (u64 _i = threadIdx.x; _i < _cn; _i += blockDim.x)", [&]() {
             cg.addLine("const u64x2 _rv = *reinterpret_cast<const u64x2*>(" + n.rec + " + 2 * (_cb + _i));");
             cg.addLine("const u32 _k = (u32)_rv.x;");
@@ -2260,7 +2261,9 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
             cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
                        "u, _k, 1u, _v, _v, _v, " + PS + "u) < 0) ++_ovf;");
         });
-        cg.addLine("const u64 _pbase = " + n.meta + "[3 * _P + _p], _pn = " + n.meta + "[_P + _p];");
+        cg.addLine("u64 _pbase = " + n.meta + "[3 * _P + _p], _pn = " + n.meta + "[_P + _p];");
+        cg.addLine("if (_pbase + _pn > " + s(PartitionConfig::kLogicalBlocks) + "ull * " + s(c_.ldsSlots) +
+                   ") { _pn = 0; if (threadIdx.x == 0) ++_ovf; }");
         cg.addBlock("for (u64 _j = threadIdx.x; _j < _pn; _j += blockDim.x)", [&]() {
             cg.addLine("const u64* _d = " + n.part + " + (_pbase + _j) * 4;");
             cg.addLine("const u64 _w0 = _d[0];");

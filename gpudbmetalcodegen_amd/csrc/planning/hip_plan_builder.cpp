@@ -321,7 +321,8 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
         }
         const int block = optInt(o, "block", 1024);
         auto cnt = std::make_unique<HipPartitionCount>(std::move(cur), pc, "g_key", "g_val");
-        auto& p1 = appendPhase(plan, "GBP_count", std::move(cnt), block, optInt(o, "items", 4), {4});
+        auto& p1 = appendPhase(plan, fused ? "GBP_count_flags" : "GBP_count", std::move(cnt), block,
+                               optInt(o, "items", 4), {4});
         p1.launchBound = 1024;
         if (filter) p1.scalarParams = {"gb_val_cut"};  // (phase references die at the next append)
         if (fused) {
@@ -329,7 +330,7 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
             plan.sizingRules.push_back(sizing);
         }
         auto& p2 = appendPhase(plan, "GBP_offsets", std::make_unique<HipPartitionOffsets>(pc), 256, 1, {1});
-        p2.launchBound = 256;  // one block per partition, one thread per logical block
+        p2.launchBound = 256;
         auto& p3 = appendPhase(plan, "GBP_bases", std::make_unique<HipPartitionBases>(pc), 1024, 1, {1});
         p3.launchBound = 1024;
         // pass 4 re-reads the rows (same scan, same filter)

@@ -222,10 +222,12 @@ gdb_status gdb_autotune_phase(gdb_executor ex, gdb_plan p, const char* phase, co
 /* ---- Batched persistent launch of many pipelines ----
  * No counterpart in the reference, which runs one command buffer per phase
  * (codegen/execution/metal_generic_executor.cpp:338-390); callers are the
- * batched workload and the request loop. Each pipeline is a single-phase
- * plan with its own runner holding its registrations; build() emits one
- * persistent kernel over all of them (work items from an atomic ticket),
- * capture() records fills + kernel in a graph, run() launches (graph replay
+ * batched workload and the request loop. Each pipeline is a plan (one or
+ * more phases) with its own runner holding its registrations; build() emits
+ * one persistent kernel per phase level over all of them (work items from an
+ * atomic ticket; `items` is the first phase's ITEMS, later phases use their
+ * defaults; work_items counts every level, grid_out is the first level's),
+ * capture() records fills + kernels in a graph, run() launches (graph replay
  * when use_graph) and checks every pipeline's status. Results are collected
  * per pipeline with gdb_executor_collect on that pipeline's runner. */
 typedef struct gdb_batch_s* gdb_batch;

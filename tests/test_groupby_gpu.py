@@ -127,6 +127,23 @@ def test_groupby_lds_slot_sizes(ctx):
         dev["G"].free(ctx)
 
 
+def test_groupby_fused_prepass_beyond_lds_counters_recovers_exactly(ctx):
+    """Plain form, partitioned: 20M uniform rows over 20M keys give ~12.6M
+    groups, a 2^25-slot table and 8192 partitions — more than the LDS
+    counters beside a 4096-slot pass-1 table. The offsets pass reports it,
+    the run is rebuilt with a smaller LDS table (more counters) and the
+    result stays exact."""
+    G = _gen(20_000_000, 0.0, 20_000_000, 21)
+    exp = oracle.groupby(G, threads=16)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, prepass=True, key_min=1, key_max=20_000_000, variant="part", lds_slots=4096)
+        assert run.resizes >= 1 and run.slots == 1 << 25
+        _cmp(run.result, exp)
+    finally:
+        dev["G"].free(ctx)
+
+
 @pytest.mark.slow
 def test_groupby_full_size(ctx):
     G = W.workload2()["G"]
