@@ -2178,9 +2178,16 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     const PartNames n = bindPartition(cg, c_, u);
     const std::string nRows = cg.addTableSizeParam(c_.scanTable);
     const std::string CAP = s(c_.ldsSlots), PMAX = s(c_.maxParts());
+    const std::string rst = c_.prefix + "_rec_status";
+    if (c_.narrow) {
+        cg.addBufferParam(rst, "u64", 8, "4", InitKind::Zero, {}, false, true);
+        cg.setPhaseStatusBuffer(rst);
+        cg.addLine("u32 _wide = 0;  // values that do not fit an 8-byte record");
+    }
     int lbits = 0;
     while ((1 << lbits) < c_.ldsSlots) ++lbits;
-    cg.addComment("pass 4: rows whose key is not in the block's final LDS table are written, packed by partition");
+    cg.addComment(std::string("pass 4: rows whose key is not in the block's final LDS table are written, packed by "
+                              "partition, as ") + (c_.narrow ? "8-byte records (key | value << 32)" : "16-byte records"));
     cg.declareLds("u32", "_hk", CAP);
     cg.declareLds("u32", "_pc", PMAX);
     cg.addLine("const u64 _P = " + n.NP + ";");
@@ -2197,8 +2204,14 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addBlock("if (!gdb_lds_key_member(_hk, " + s(lbits) + "u, _gk, " + s(c_.window) + "u))", [&]() {
             cg.addLine("const u32 _i = __hip_atomic_fetch_add(&_pc[gdb_partition(_gk, _pb)], 1u, __ATOMIC_RELAXED, "
                        "__HIP_MEMORY_SCOPE_WORKGROUP);");
-            cg.addLine("u64x2 _rv; _rv.x = (u64)_gk; _rv.y = (u64)(i64)(" + value_ + ");");
-            cg.addLine("*reinterpret_cast<u64x2*>(" + n.rec + " + 2 * (u64)_i) = _rv;");
+            if (c_.narrow) {
+                cg.addLine("const i64 _v = (i64)(" + value_ + ");");
+                cg.addLine("if (_v != (i64)(i32)_v) ++_wide;");
+                cg.addLine(n.rec + "[_i] = (u64)_gk | ((u64)(u32)(i32)_v << 32);");
+            } else {
+                cg.addLine("u64x2 _rv; _rv.x = (u64)_gk; _rv.y = (u64)(i64)(" + value_ + ");");
+                cg.addLine("*reinterpret_cast<u64x2*>(" + n.rec + " + 2 * (u64)_i) = _rv;");
+            }
         });
         consume();
     });
@@ -2219,10 +2232,15 @@ void HipPartitionScatter::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     });
     cg.addLine("__syncthreads();");
     closeLogicalBlocks(cg);
+    if (c_.narrow) {
+        cg.addComment("status: [0] a value needs a 16-byte record");
+        emitBlockStatus(cg, rst, {{"_wide", 0, true}});
+    }
 }
 
 std::string HipPartitionScatter::describe() const {
-    return "PartitionScatter(" + c_.prefix + ", key=" + key_ + ", lds" + s(c_.ldsSlots) + ")";
+    return "PartitionScatter(" + c_.prefix + ", key=" + key_ + ", lds" + s(c_.ldsSlots) +
+           (c_.narrow ? ", 8-byte records" : "") + ")";
 }
 
 void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
@@ -2255,9 +2273,15 @@ void HipPartitionAgg::produceImpl(HipCodegen& cg, ConsumerFn consume) {
         cg.addComment("records beyond the buffers can only come from a failed earlier pass: reported, not read");
         cg.addLine("if (_cb + _cn > " + n.rows + ") { _cn = 0; if (threadIdx.x == 0) ++_ovf; }");
         cg.addBlock("for (u64 _i = threadIdx.x; _i < _cn; _i += blockDim.x)", [&]() {
-            cg.addLine("const u64x2 _rv = *reinterpret_cast<const u64x2*>(" + n.rec + " + 2 * (_cb + _i));");
-            cg.addLine("const u32 _k = (u32)_rv.x;");
-            cg.addLine("const i64 _v = (i64)_rv.y;");
+            if (c_.narrow) {
+                cg.addLine("const u64 _w = " + n.rec + "[_cb + _i];");
+                cg.addLine("const u32 _k = (u32)_w;");
+                cg.addLine("const i64 _v = (i64)(i32)(u32)(_w >> 32);");
+            } else {
+                cg.addLine("const u64x2 _rv = *reinterpret_cast<const u64x2*>(" + n.rec + " + 2 * (_cb + _i));");
+                cg.addLine("const u32 _k = (u32)_rv.x;");
+                cg.addLine("const i64 _v = (i64)_rv.y;");
+            }
             cg.addLine("if (gdb_lds_gb_merge_at(_pl, " + PM + "u, ((_k * GDB_KNUTH) >> _sh) & " + PM +
                        "u, _k, 1u, _v, _v, _v, " + PS + "u) < 0) ++_ovf;");
         });

@@ -734,6 +734,12 @@ struct PartitionConfig {
     // come, the LDS table's keys at the end: the exact distinct-count pre-pass
     // fused into pass 1. Empty = off.
     std::string flags, flagMin, flagBits;
+    // Cold rows as 8-byte records (key | value << 32) instead of 16-byte
+    // (key, value) ones: half the scatter's writes and the fold's reads. Exact
+    // only while every value fits in 32 bits, so the scatter checks each one
+    // and reports a value that does not in <prefix>_rec_status[0]; the caller
+    // then reruns with wide records.
+    bool narrow = false;
     static constexpr int kPartSlots = 4096;
     static constexpr int kLogicalBlocks = 256;
     // Largest partition count whose LDS counters fit beside the LDS table.

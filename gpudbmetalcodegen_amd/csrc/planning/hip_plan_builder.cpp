@@ -308,6 +308,7 @@ HipQueryPlan buildGroupBy(const PlanOptions& o) {
         pc.scanTable = "G";
         pc.ldsSlots = ldsSlots;
         pc.window = optInt(o, "window", 2);
+        pc.narrow = optInt(o, "rec8", 0) != 0;  // 8-byte cold records, checked per value
         if (fused) {
             // the table size (and so its partition count) is known only after
             // the count: count at the most partitions the table could need —

@@ -254,11 +254,14 @@ class GroupByRun(QueryRun):
     resizes: int = 0
     distinct: int = 0
     times_ms: list = None
+    narrow: bool = False      # the partitioned form's cold rows as 8-byte records
 
 
 def groupby_options(variant: str = "lds", lds_slots: int = 2048, filtered: bool = False, prepass: bool = False,
-                    block: int | None = None, window: int | None = None) -> str:
+                    block: int | None = None, window: int | None = None, narrow: bool = False) -> str:
     o = [f"variant={variant}", f"lds_slots={lds_slots}"]
+    if narrow and variant == "part":
+        o.append("rec8=1")
     if window:
         o.append(f"window={window}")
     if filtered:
@@ -295,15 +298,20 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
                 variant: str = "lds", lds_slots: int = 2048, val_cut: int | None = None, key_min: int = 1,
                 key_max: int | None = None, max_resizes: int = 8, block: int | None = None,
                 warmup: int = 0, measured: int = 1, executor: E.HipGenericExecutor | None = None,
-                collect: bool = True, stream: Stream | None = None, window: int | None = None) -> GroupByRun:
+                collect: bool = True, stream: Stream | None = None, window: int | None = None,
+                narrow: bool = True) -> GroupByRun:
     """prepass=True: the plan counts distinct keys exactly first (key-range
     bitmap + popcount) and sizes the table at 2x (plain form). Otherwise
     `slots` (e.g. from the learned estimator) sizes it, and a reported
     overflow doubles the table and reruns (resize-on-overflow).
     variant="part" is the partitioned form without global atomics: slots >=
     4096, and its pass-1 LDS table is shrunk when slots / 4096 partitions
-    would not leave room for the per-partition LDS counters."""
+    would not leave room for the per-partition LDS counters. With `narrow`
+    its cold rows travel as 8-byte records while every value fits in 32 bits;
+    a value that does not is reported and the run repeated with 16-byte
+    records."""
     part = variant == "part"
+    narrow = narrow and part
     ex = _executor(ctx, executor)
     _register(ex, dev, ["G"], stream)
     if val_cut is not None:
@@ -325,7 +333,8 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
         if part:
             lds_slots = part_lds_for(slots, lds_slots)
     while True:
-        plan = get_plan(ctx, "GROUPBY", groupby_options(variant, lds_slots, val_cut is not None, prepass, block, window))
+        plan = get_plan(ctx, "GROUPBY", groupby_options(variant, lds_slots, val_cut is not None, prepass, block, window,
+                                                        narrow))
         if not prepass:
             ex.register_symbol("gb_slots", slots)
         try:
@@ -338,7 +347,9 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
             resizes += 1
             if resizes > max_resizes:
                 raise
-            if part and status[1]:
+            if narrow and ex.read_buffer("d_gbp_rec_status", 32, np.uint64)[0]:
+                narrow = False                  # a value needs 64 bits: 16-byte records
+            elif part and status[1]:
                 if lds_slots <= 64:
                     raise
                 lds_slots //= 2                 # more partitions than LDS counters beside the table
@@ -352,7 +363,7 @@ def run_groupby(ctx: E.Context, dev: dict, *, slots: int | None = None, prepass:
         slots = ex.get_symbol("gb_slots")
     res = ex.collect(plan) if collect else None
     return GroupByRun(res, t, plan, ex, sst, slots=slots, resizes=resizes, distinct=res.rows() if res else 0,
-                      times_ms=times)
+                      times_ms=times, narrow=narrow)
 
 
 # ---------------------------------------------------------------------------

@@ -127,6 +127,33 @@ def test_groupby_lds_slot_sizes(ctx):
         dev["G"].free(ctx)
 
 
+@pytest.mark.parametrize("big", [False, True])
+def test_groupby_partitioned_narrow_records_fall_back_exactly(ctx, big):
+    """Cold rows travel as 8-byte records (key | value << 32) while every
+    value fits in 32 bits. With 50 values beyond that range among the rows
+    (and a 64-slot pass-1 table, so nearly every row is cold) the scatter
+    reports them and the run repeats with 16-byte records. Exact either way,
+    and equal to a run that asks for 16-byte records from the start."""
+    G = dict(_gen(500_000, 0.0, 400_000, 31))
+    if big:
+        v = G["g_val"].copy()
+        idx = np.arange(50) * 9_973 + 11
+        v[idx] = np.where(idx % 2 == 0, -(1 << 40) - idx, (1 << 40) + idx)
+        v[idx[:3]] = [-(1 << 31) - 1, 1 << 31, (1 << 62) + 5]
+        G["g_val"] = v
+    exp = oracle.groupby(G)
+    dev = Q.upload_tables(ctx, {"G": G})
+    try:
+        run = Q.run_groupby(ctx, dev, slots=1 << 20, variant="part", lds_slots=64)
+        assert run.narrow == (not big) and run.resizes == (1 if big else 0)
+        _cmp(run.result, exp)
+        wide = Q.run_groupby(ctx, dev, slots=1 << 20, variant="part", lds_slots=64, narrow=False)
+        assert not wide.narrow and wide.resizes == 0
+        _cmp(wide.result, exp)
+    finally:
+        dev["G"].free(ctx)
+
+
 def test_groupby_fused_prepass_beyond_lds_counters_recovers_exactly(ctx):
     """Plain form, partitioned: 20M uniform rows over 20M keys give ~12.6M
     groups, a 2^25-slot table and 8192 partitions — more than the LDS
