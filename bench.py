@@ -475,7 +475,7 @@ def bench_serve(ctx, cpu_check: bool = True) -> dict:
                        "F 120M, B 30M, A 3M, S 4M (resident %.2f GB)" % (db.resident_bytes / 1e9),
            "rates": {}}
     streams = []
-    for rate, dur in ((50, 20.0), (200, 5.0), (500, 2.0)):
+    for rate, dur in ((50, 22.0), (200, 5.5), (500, 2.2)):   # ~1,100 arrivals each
         a = S.make_stream(db, rate, dur, seed=rate, menu=menu)
         b = S.make_stream(db, rate, dur, seed=rate, menu=menu)
         plain = S.serve(db, a, workers=1, budget_bytes=1 << 40, pool=pool)
