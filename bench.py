@@ -144,14 +144,22 @@ def q3_phase_bytes(T: dict, seg: int, d: int, b_slots: int) -> dict:
 
 def run_steps(ex, plan, steps: int) -> dict:
     """Times `steps` single runs of a plan: the host wall time of each step
-    (launch to result in HBM, host synchronised) and its device time, with the
-    per-phase device times averaged over the steps."""
+    (launch to result in HBM, host synchronised) and its device time (one
+    event at each end). The per-phase device times come from as many further
+    runs with an event pair around every phase — instrumentation kept out of
+    the timed steps."""
     wall, gpu, acc, last = [], [], {}, None
+    ex.set_phase_timing(False)
+    try:
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            last = ex.execute(plan, 0, 1)
+            wall.append((time.perf_counter() - t0) * 1e3)
+            gpu.append(last.total_ms)
+    finally:
+        ex.set_phase_timing(True)
     for _ in range(steps):
-        t0 = time.perf_counter()
         last = ex.execute(plan, 0, 1)
-        wall.append((time.perf_counter() - t0) * 1e3)
-        gpu.append(last.total_ms)
         for name, ms, *_ in last.phases:
             acc[name] = acc.get(name, 0.0) + ms / steps
     return {"wall": wall, "gpu": gpu, "phases": acc, "last": last}
@@ -363,15 +371,22 @@ def bench_groupby(ctx, steps: int, warmup: int, cpu: bool) -> dict:
             r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
                                executor=r2.executor, collect=False)
         ctx.sync()
-        # timed like the plain form: sizing + kernels, results left in HBM
+        # timed like the plain form: sizing + kernels, results left in HBM, no
+        # per-phase event records in the timed steps
         resizes, lwall = 0, []
-        for _ in range(steps):
-            t0 = time.perf_counter()
-            sz = learned_step()
-            r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
-                               executor=r2.executor, collect=False)
-            lwall.append((time.perf_counter() - t0) * 1e3)
-            resizes += r2.resizes
+        r2.executor.set_phase_timing(False)
+        try:
+            for _ in range(steps):
+                t0 = time.perf_counter()
+                sz = learned_step()
+                r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
+                                   executor=r2.executor, collect=False)
+                lwall.append((time.perf_counter() - t0) * 1e3)
+                resizes += r2.resizes
+        finally:
+            r2.executor.set_phase_timing(True)
+        r2 = Q.run_groupby(ctx, dev, slots=sz.slots, variant=sz.variant, lds_slots=sz.lds_slots,
+                           executor=r2.executor, collect=False)     # one instrumented run: per-phase times
         learned_ms = sum(lwall) / steps
         learned_res = r2.executor.collect(r2.plan)
         same = all(np.array_equal(plain_res.columns[c], learned_res.columns[c]) for c in plain_res.columns)

@@ -162,6 +162,7 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_executor_get_symbol", [vp, cstr, P(u64), P(i32)])
     _sig(eng, "gdb_executor_register_scalar", [vp, cstr, i64])
     _sig(eng, "gdb_executor_set_skip_init", [vp, i32])
+    _sig(eng, "gdb_executor_set_phase_timing", [vp, i32])
     _sig(eng, "gdb_executor_execute", [vp, vp, i32, i32, P(ExecTiming)])
     _sig(eng, "gdb_executor_execute_range", [vp, vp, i32, i32, i32, i32, P(ExecTiming)])
     _sig(eng, "gdb_executor_prepare", [vp, vp])

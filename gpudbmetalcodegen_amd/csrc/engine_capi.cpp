@@ -346,6 +346,11 @@ gdb_status gdb_executor_set_skip_init(gdb_executor ex, int skip) {
     return GDB_OK;
 }
 
+gdb_status gdb_executor_set_phase_timing(gdb_executor ex, int on) {
+    ex->ex->setPhaseTiming(on != 0);
+    return GDB_OK;
+}
+
 static void registerPlanTables(gdb_executor ex, gdb_plan p) {
     for (const auto& bt : p->plan.bufferTables) ex->ex->registerBufferTable(bt.table, bt.column, bt.buffer, bt.rowsExpr);
 }

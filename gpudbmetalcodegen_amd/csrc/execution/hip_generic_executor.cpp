@@ -445,8 +445,8 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
             const auto& p = phases[static_cast<size_t>(pi)];
             unsigned grid = 0;
             const size_t e = static_cast<size_t>(2 * (pi - firstPhase));
-            launchPhase(rc, cq, p, initDone, spec, &grid, events_[e + 1]);
-            check(gdb_event_record(events_[e + 2], stream_), "gdb_event_record");
+            launchPhase(rc, cq, p, initDone, spec, &grid, phaseTiming_ ? events_[e + 1] : nullptr);
+            if (phaseTiming_ || pi + 1 == last) check(gdb_event_record(events_[e + 2], stream_), "gdb_event_record");
             res.phaseNames.push_back(p.name);
             res.phaseGrid.push_back(grid);
             res.phaseBlock.push_back(static_cast<unsigned>(p.isSingleThread ? 64 : p.threadgroupSize));
@@ -470,7 +470,7 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
         check(gdb_event_sync(events_[static_cast<size_t>(2 * np)]), "gdb_event_sync");
         if (measured) {
             const float w = 1.0f / static_cast<float>(std::max(1, measuredRuns));
-            for (int i = 0; i < np; ++i) {
+            for (int i = 0; phaseTiming_ && i < np; ++i) {
                 float ms = 0.0f, fill = 0.0f;
                 const size_t e = static_cast<size_t>(2 * i);
                 check(gdb_event_elapsed_ms(events_[e + 1], events_[e + 2], &ms), "gdb_event_elapsed_ms");

@@ -77,6 +77,10 @@ public:
     const void* hookBytes() const { return pinned_; }
     // When true, buffers are not re-initialised (accumulate across runs).
     void setSkipInit(bool skip) { skipInit_ = skip; }
+    // Per-phase device times (an event pair around every phase) or only the
+    // whole run's (one event at each end): a query that needs no breakdown
+    // keeps the stream free of the per-phase event records.
+    void setPhaseTiming(bool on) { phaseTiming_ = on; }
     // Symbols a phase hook sets during a run: buffers sized by them are
     // initialised at their phase, the others all at the start of a run in
     // one multi-buffer fill.
@@ -173,6 +177,7 @@ private:
     uint64_t* ensurePinned(size_t bytes);
     std::map<std::pair<gdb_kernel, int>, int> occupancy_;
     bool skipInit_ = false;
+    bool phaseTiming_ = true;
     std::vector<std::string> lateSymbols_;
     gdb_event evA_ = nullptr;
     std::vector<gdb_event> events_;

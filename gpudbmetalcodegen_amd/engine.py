@@ -351,6 +351,11 @@ class HipGenericExecutor:
     def set_skip_init(self, skip: bool) -> None:
         N.eng_check(_eng().gdb_executor_set_skip_init(self._h, 1 if skip else 0))
 
+    def set_phase_timing(self, on: bool) -> None:
+        """Per-phase device times (default) or only the whole run's, without
+        the per-phase event records on the stream."""
+        N.eng_check(_eng().gdb_executor_set_phase_timing(self._h, 1 if on else 0))
+
     def _timing(self, plan: HipQueryPlan, t: N.ExecTiming) -> Timing:
         names = plan.phases()
         ph = [(names[i] if i < len(names) else f"phase{i}", t.phase_ms[i], t.grid[i], t.block[i], t.items[i])

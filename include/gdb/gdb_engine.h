@@ -101,6 +101,12 @@ gdb_status gdb_executor_register_symbol(gdb_executor ex, const char* name, uint6
 gdb_status gdb_executor_get_symbol(gdb_executor ex, const char* name, uint64_t* value, int* found);
 gdb_status gdb_executor_register_scalar(gdb_executor ex, const char* name, int64_t value);
 gdb_status gdb_executor_set_skip_init(gdb_executor ex, int skip);
+// on = 1 (default): gdb_exec_timing holds every phase's device time (an event
+// pair around each phase); on = 0: only the whole run's (one event at each
+// end), for runs that need no breakdown. No counterpart in the reference,
+// whose timings come from command-buffer timestamps
+// (codegen/execution/metal_generic_executor.cpp:338-390).
+gdb_status gdb_executor_set_phase_timing(gdb_executor ex, int on);
 
 #define GDB_MAX_PHASES 16
 typedef struct {

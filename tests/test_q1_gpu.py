@@ -97,3 +97,22 @@ def test_scalar_atomic_ablation_is_bit_exact(ctx, monkeypatch):
         assert as_lists(got) == exp
     finally:
         dev["F"].free(ctx)
+
+
+def test_phase_timing_off_keeps_results_and_total(ctx):
+    """Without per-phase event records a run still reports its whole device
+    time and the same exact result; per-phase times come back with them on."""
+    F = W.fact_f(200_000, W.seed_for(0) + 11)
+    dev = Q.upload_tables(ctx, {"F": F})
+    try:
+        run = Q.run_q1(ctx, dev, W.q1_cutoff())
+        ex, plan = run.executor, run.plan
+        ex.set_phase_timing(False)
+        t = ex.execute(plan, 0, 1)
+        assert t.total_ms > 0 and all(ms == 0.0 for _, ms, *_ in t.phases)
+        assert as_lists(ex.collect(plan).columns) == q1_expected(oracle.q1(F, W.q1_cutoff()))
+        ex.set_phase_timing(True)
+        t = ex.execute(plan, 0, 1)
+        assert t.total_ms > 0 and all(ms > 0.0 for _, ms, *_ in t.phases)
+    finally:
+        dev["F"].free(ctx)
