@@ -430,6 +430,7 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
     const int runs = warmupRuns + std::max(1, measuredRuns);
     measuredPhase_.assign(static_cast<size_t>(np), 0.0f);
     measuredFill_.assign(static_cast<size_t>(np), 0.0f);
+    PendingStatus pending;
     for (int r = 0; r < runs; ++r) {
         const bool measured = r >= warmupRuns;
         std::unordered_map<std::string, bool> initDone;
@@ -467,7 +468,13 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
                 h->second.fn(*this);
             }
         }
-        check(gdb_event_sync(events_[static_cast<size_t>(2 * np)]), "gdb_event_sync");
+        if (r + 1 == runs) {
+            // the final run: status words gathered and read behind it, one wait for both
+            pending = enqueueStatus(cg, firstPhase, last);
+            check(gdb_stream_sync(stream_), "gdb_stream_sync");
+        } else {
+            check(gdb_event_sync(events_[static_cast<size_t>(2 * np)]), "gdb_event_sync");
+        }
         if (measured) {
             const float w = 1.0f / static_cast<float>(std::max(1, measuredRuns));
             for (int i = 0; phaseTiming_ && i < np; ++i) {
@@ -485,34 +492,34 @@ HipExecutionResult HipGenericExecutor::execute(RuntimeCompiler& rc, const Runtim
     }
     res.phaseTimesMs = measuredPhase_;
     res.phaseFillMs = measuredFill_;
-    res.phaseStatus = checkStatus(cg, firstPhase, last);
+    res.phaseStatus = judgeStatuses(pending);
     return res;
 }
 
-std::vector<std::array<uint64_t, 4>> HipGenericExecutor::checkStatus(const HipCodegen& cg, int firstPhase,
-                                                                     int lastPhase) {
+HipGenericExecutor::PendingStatus HipGenericExecutor::enqueueStatus(const HipCodegen& cg, int firstPhase,
+                                                                    int lastPhase) {
     // Every status and check word of the range is gathered on the device and
-    // read back with one copy and one synchronisation.
+    // read back with one copy; the caller's next synchronisation covers it.
     const auto& phases = cg.getPhases();
     const int last = lastPhase < 0 ? static_cast<int>(phases.size()) : lastPhase;
-    std::vector<std::array<uint64_t, 4>> out(static_cast<size_t>(std::max(0, last - firstPhase)), {0, 0, 0, 0});
+    PendingStatus ps;
+    ps.phases = static_cast<size_t>(std::max(0, last - firstPhase));
     std::vector<const void*> ptrs;
-    std::vector<std::pair<int, StatusRef>> refs;  // (index in out, reference)
     for (int pi = firstPhase; pi < last; ++pi) {
         const auto& p = phases[static_cast<size_t>(pi)];
         auto st = buffers_.find(p.statusBuffer);
         if (!p.statusBuffer.empty() && st != buffers_.end()) {
-            refs.push_back({pi - firstPhase, {st->second.ptr, p.name, false, ""}});
+            ps.refs.push_back({pi - firstPhase, {st->second.ptr, p.name, false, ""}});
             ptrs.push_back(st->second.ptr);
         }
         auto ck = buffers_.find(p.checkBuffer);
         if (!p.checkBuffer.empty() && ck != buffers_.end()) {
-            refs.push_back({pi - firstPhase, {ck->second.ptr, p.name, true, p.checkMessage}});
+            ps.refs.push_back({pi - firstPhase, {ck->second.ptr, p.name, true, p.checkMessage}});
             ptrs.push_back(ck->second.ptr);
         }
     }
-    if (refs.empty()) return out;
-    const size_t n = refs.size();
+    if (ps.refs.empty()) return ps;
+    const size_t n = ps.refs.size();
     uint64_t* host = ensurePinned(n * 32 + n * sizeof(void*));
     if (n > statusCap_) {
         check(gdb_stream_sync(stream_), "gdb_stream_sync");
@@ -533,14 +540,26 @@ std::vector<std::array<uint64_t, 4>> HipGenericExecutor::checkStatus(const HipCo
                            static_cast<uint64_t*>(dStatusOut_), stream_),
           "gather status words");
     check(gdb_memcpy_d2h_async(host, dStatusOut_, n * 32, stream_), "read status words");
-    check(gdb_stream_sync(stream_), "gdb_stream_sync");
-    for (size_t i = 0; i < n; ++i) {
-        const uint64_t* w = host + 4 * i;
-        judgeStatus(refs[i].second, w);
-        if (!refs[i].second.check)
-            for (int k = 0; k < 4; ++k) out[static_cast<size_t>(refs[i].first)][static_cast<size_t>(k)] = w[k];
+    ps.host = host;
+    return ps;
+}
+
+std::vector<std::array<uint64_t, 4>> HipGenericExecutor::judgeStatuses(const PendingStatus& ps) const {
+    std::vector<std::array<uint64_t, 4>> out(ps.phases, {0, 0, 0, 0});
+    for (size_t i = 0; i < ps.refs.size(); ++i) {
+        const uint64_t* w = ps.host + 4 * i;
+        judgeStatus(ps.refs[i].second, w);
+        if (!ps.refs[i].second.check)
+            for (int k = 0; k < 4; ++k) out[static_cast<size_t>(ps.refs[i].first)][static_cast<size_t>(k)] = w[k];
     }
     return out;
+}
+
+std::vector<std::array<uint64_t, 4>> HipGenericExecutor::checkStatus(const HipCodegen& cg, int firstPhase,
+                                                                     int lastPhase) {
+    const PendingStatus ps = enqueueStatus(cg, firstPhase, lastPhase);
+    if (!ps.refs.empty()) check(gdb_stream_sync(stream_), "gdb_stream_sync");
+    return judgeStatuses(ps);
 }
 
 GenericResult HipGenericExecutor::collectResult(const HipCodegen& cg) const {

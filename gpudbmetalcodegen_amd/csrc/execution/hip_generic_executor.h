@@ -125,6 +125,16 @@ public:
     static void judgeStatus(const StatusRef& r, const uint64_t* words);
     // Read the status words of the plan's phases; throws GdbOverflowError.
     std::vector<std::array<uint64_t, 4>> checkStatus(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
+    // checkStatus in two halves: the gather and the copy into pinned memory
+    // enqueued on the stream, then (after the caller's synchronisation) the
+    // words judged.
+    struct PendingStatus {
+        std::vector<std::pair<int, StatusRef>> refs;  // (phase index in the range, reference)
+        const uint64_t* host = nullptr;
+        size_t phases = 0;
+    };
+    PendingStatus enqueueStatus(const HipCodegen& cg, int firstPhase = 0, int lastPhase = -1);
+    std::vector<std::array<uint64_t, 4>> judgeStatuses(const PendingStatus& ps) const;
 
     GenericResult collectResult(const HipCodegen& cg) const;
     void releaseAllocatedBuffers();
