@@ -484,6 +484,8 @@ def bench_serve(ctx, cpu_check: bool = True) -> dict:
     db = S.ResidentDB(ctx, T)
     menu = S.param_menu(db)
     pool = [S.Worker(db) for _ in range(4)]
+    for w in pool:                      # every worker has met every query before it serves
+        w.warm(menu)
     warm = S.make_stream(db, rate=400, duration=0.05, seed=1, backlog=True, menu=menu)
     S.serve(db, warm, workers=4, budget_bytes=8 << 30, pool=pool)
     out = {"workload": "served stream: scan-aggregate / 3-way join / anti-join / LIKE+group-by over "

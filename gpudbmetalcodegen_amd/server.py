@@ -189,6 +189,17 @@ class Worker:
         q.result = run.result.canonical
         q.columns = _plain_columns(run.result)
 
+    def warm(self, menu: dict) -> None:
+        """Run every query of the menu once on this worker, so that its plans
+        are loaded and its buffers have their largest sizes before it serves:
+        a worker that first meets a query while serving would allocate device
+        memory inside that query's latency."""
+        for kind, bindings in menu.items():
+            for i, params in enumerate(bindings):
+                self.run(Query(-1 - i, kind, dict(params)))
+        hip, _ = N.load()
+        N.hip_check(hip.gdb_stream_sync(self.stream))
+
     def close(self):
         hip, _ = N.load()
         for e in self.ex.values():
