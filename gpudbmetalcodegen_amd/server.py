@@ -239,7 +239,8 @@ class _Runner(threading.Thread):
 
 
 def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: int = 2 << 30,
-          pool: list[Worker] | None = None, max_heavy: int = 1, heavy_rows: int | None = None) -> dict:
+          pool: list[Worker] | None = None, max_heavy: int = 1, heavy_rows: int | None = None,
+          min_overlap_ms: float = 0.0, service_ms: dict | None = None) -> dict:
     """Run a query stream. workers=1 with an unlimited budget is the plain
     one-at-a-time form; workers>1 is the served (continuous batching) form.
     Served admission: the oldest waiting query that fits starts as soon as a
@@ -247,7 +248,12 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
     heavy_rows rows scanned; default half the largest plan type's) run at
     once: heavy scans are bandwidth-bound, so running two at once only
     stretches both, while a light query (anti-join, LIKE) overlaps a heavy
-    one almost for free and may pass a heavy query that waits."""
+    one almost for free and may pass a heavy query that waits.
+    min_overlap_ms > 0: a query does not start beside running ones when one
+    of them is expected to finish within that time (a running query's
+    expected service time per plan type: service_ms, then a running mean of
+    the completed queries' device-side times) — sharing the device stretches
+    both, which costs more than a short wait."""
     own = pool is None
     pool = (pool or [Worker(db) for _ in range(workers)])[:workers]
     arrivals = sorted(stream, key=lambda q: (q.arrival, q.qid))
@@ -256,6 +262,8 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
     cv = threading.Condition()
     state = {"used": 0, "done": 0, "inflight": 0, "max_inflight": 0, "heavy": 0}
     idle: deque = deque()
+    running: dict = {}                          # runner -> its query
+    est = dict(service_ms or {})               # plan type -> expected service ms
 
     def on_done(runner: _Runner, q: Query) -> None:
         with cv:
@@ -263,8 +271,27 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             state["done"] += 1
             state["inflight"] -= 1
             state["heavy"] -= q.rows >= heavy_rows
+            running.pop(runner, None)
+            ms = (q.done - q.start) * 1e3
+            est[q.kind] = ms if q.kind not in est else 0.8 * est[q.kind] + 0.2 * ms
             idle.append(runner)
             cv.notify()
+
+    def frees_soon(now: float) -> float | None:
+        """When a running query is expected to finish within min_overlap_ms
+        (or is overdue by less than that), the seconds until admission should
+        be looked at again (its completion usually comes first); else None —
+        a query overdue by more no longer holds others back."""
+        if min_overlap_ms <= 0 or not running:
+            return None
+        w = min_overlap_ms / 1e3
+        soon = None
+        for r in running.values():
+            left = est.get(r.kind, 0.0) / 1e3 - (now - r.start)
+            if -w <= left < w:
+                recheck = left + w
+                soon = recheck if soon is None else min(soon, recheck)
+        return soon
 
     t0 = time.perf_counter()
     runners = [_Runner(w, t0, on_done) for w in pool]
@@ -283,7 +310,12 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             # worker is idle (an empty device always admits one); a heavy
             # query waits while max_heavy heavy ones run, and light queries
             # behind it may pass
+            wake = None
             while pending and idle:
+                if state["inflight"] > 0:
+                    wake = frees_soon(time.perf_counter() - t0)
+                    if wake is not None:
+                        break               # a running query ends within min_overlap_ms: wait for it
                 pick = None
                 for j, c in enumerate(pending):
                     fits = state["used"] + c.footprint <= budget_bytes or state["inflight"] == 0
@@ -305,7 +337,9 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
                 # the most recently idle worker takes it (LIFO): at low load
                 # one worker, its thread and its stream serve every query, as
                 # in the plain form, instead of rotating through cold ones
-                idle.pop().submit(q)
+                runner = idle.pop()
+                running[runner] = q
+                runner.submit(q)
             if state["done"] >= len(stream):
                 break
             # sleep until the next arrival or a completion (a waiting query
@@ -314,6 +348,9 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             timeout = None
             if ai < len(arrivals):
                 timeout = max(0.0, arrivals[ai].arrival - (time.perf_counter() - t0))
+            if wake is not None:            # re-check when the expected completion is due
+                timeout = wake if timeout is None else min(timeout, wake)
+                timeout += 20e-6
             cv.wait(timeout)
     for r in runners:
         r.submit(None)
@@ -330,6 +367,7 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
         "errors": sum(1 for q in stream if q.error),
         "per_kind_p99_ms": {k: percentile([q.latency_ms for q in stream if q.kind == k], 0.99) for k in KINDS},
         "latency_ms": ST.trial_stats(lat),
+        "service_ms": {k: round(v, 4) for k, v in est.items()},
     }
     if own:
         for w in pool:

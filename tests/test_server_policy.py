@@ -83,3 +83,37 @@ def test_budget_keeps_memory_order():
     rep, qs, log = _run(["anti", "like", "anti", "like"], budget=10, footprint=10)
     assert rep["max_inflight"] == 1
     assert [qid for ev, qid, *_ in log if ev == "start"] == [0, 1, 2, 3]
+
+
+def _run_with(kinds, min_overlap_ms, service_ms):
+    log, lock = [], threading.Lock()
+    dur = {"scan": 0.03, "join": 0.04, "anti": 0.005, "like": 0.005}
+    pool = [FakeWorker(log, lock, dur) for _ in range(4)]
+    qs = _stream(kinds)
+    rep = S.serve(FakeDB(), qs, workers=4, budget_bytes=1 << 40, pool=pool, min_overlap_ms=min_overlap_ms,
+                  service_ms=service_ms)
+    start = {qid: t for ev, qid, _, t in log if ev == "start"}
+    end = {qid: t for ev, qid, _, t in log if ev == "end"}
+    return rep, start, end
+
+
+def test_no_overlap_with_a_query_about_to_finish():
+    """With min_overlap_ms, a query that arrives while the running one is
+    expected to end within that time waits for it instead of sharing the
+    device; one that would wait long still starts beside it."""
+    est = {"scan": 30.0, "join": 40.0, "anti": 5.0, "like": 5.0}
+    rep, start, end = _run_with(["anti", "join"], 10.0, est)
+    assert rep["errors"] == 0 and start[1] >= end[0]          # the join waited ~5 ms for the anti-join
+    rep, start, end = _run_with(["anti", "join"], 0.0, est)
+    assert start[1] < end[0]                                  # without the rule it starts beside it
+    rep, start, end = _run_with(["join", "anti"], 10.0, est)
+    assert start[1] < end[0]                                  # 40 ms left: the anti-join overlaps
+    assert rep["service_ms"]["join"] > 30.0 and rep["service_ms"]["anti"] >= 4.0
+
+
+def test_overdue_query_does_not_block_admission():
+    """A running query past its expected time no longer holds the next one
+    back (the estimate was wrong, the wait would be unbounded)."""
+    est = {"scan": 30.0, "join": 1.0, "anti": 5.0, "like": 5.0}   # the join really takes 40 ms
+    rep, start, end = _run_with(["join", "anti"], 10.0, est)
+    assert rep["errors"] == 0 and start[1] < end[0]
