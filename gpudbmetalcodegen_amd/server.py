@@ -250,10 +250,11 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
     stretches both, while a light query (anti-join, LIKE) overlaps a heavy
     one almost for free and may pass a heavy query that waits.
     min_overlap_ms > 0: a query does not start beside running ones when one
-    of them is expected to finish within that time (a running query's
-    expected service time per plan type: service_ms, then a running mean of
-    the completed queries' device-side times) — sharing the device stretches
-    both, which costs more than a short wait."""
+    of them is expected to finish within that time; min_overlap_ms < 0: within
+    the waiting query's own expected service time (waiting is then shorter
+    than running it stretched beside the other). Expected service time per
+    plan type: service_ms, then a running mean over completed queries. Sharing
+    the device stretches both queries, which costs more than a short wait."""
     own = pool is None
     pool = (pool or [Worker(db) for _ in range(workers)])[:workers]
     arrivals = sorted(stream, key=lambda q: (q.arrival, q.qid))
@@ -277,14 +278,17 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             idle.append(runner)
             cv.notify()
 
-    def frees_soon(now: float) -> float | None:
-        """When a running query is expected to finish within min_overlap_ms
-        (or is overdue by less than that), the seconds until admission should
-        be looked at again (its completion usually comes first); else None —
-        a query overdue by more no longer holds others back."""
-        if min_overlap_ms <= 0 or not running:
+    def frees_soon(now: float, kind: str) -> float | None:
+        """When a running query is expected to finish within the overlap
+        window of a waiting query of this kind (or is overdue by less than
+        that), the seconds until admission should be looked at again (the
+        completion usually comes first); else None — a query overdue by more
+        no longer holds others back."""
+        if min_overlap_ms == 0 or not running:
             return None
-        w = min_overlap_ms / 1e3
+        w = min_overlap_ms / 1e3 if min_overlap_ms > 0 else est.get(kind, 0.0) / 1e3
+        if w <= 0:
+            return None
         soon = None
         for r in running.values():
             left = est.get(r.kind, 0.0) / 1e3 - (now - r.start)
@@ -312,15 +316,16 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
             # behind it may pass
             wake = None
             while pending and idle:
-                if state["inflight"] > 0:
-                    wake = frees_soon(time.perf_counter() - t0)
-                    if wake is not None:
-                        break               # a running query ends within min_overlap_ms: wait for it
                 pick = None
+                now = time.perf_counter() - t0
                 for j, c in enumerate(pending):
                     fits = state["used"] + c.footprint <= budget_bytes or state["inflight"] == 0
                     heavy_ok = c.rows < heavy_rows or state["heavy"] < max_heavy or state["inflight"] == 0
-                    if fits and heavy_ok:
+                    # a running query about to finish: this one waits for it
+                    soon = frees_soon(now, c.kind) if state["inflight"] > 0 else None
+                    if soon is not None:
+                        wake = soon if wake is None else min(wake, soon)
+                    if fits and heavy_ok and soon is None:
                         pick = j
                         break
                     if not fits:

@@ -117,3 +117,14 @@ def test_overdue_query_does_not_block_admission():
     est = {"scan": 30.0, "join": 1.0, "anti": 5.0, "like": 5.0}   # the join really takes 40 ms
     rep, start, end = _run_with(["join", "anti"], 10.0, est)
     assert rep["errors"] == 0 and start[1] < end[0]
+
+
+def test_overlap_window_of_the_waiting_query():
+    """min_overlap_ms < 0: a query waits for a running one that ends within
+    its own expected service time — a join behind an anti-join about to end
+    waits, an anti-join beside a join with 40 ms to go starts at once."""
+    est = {"scan": 30.0, "join": 40.0, "anti": 5.0, "like": 5.0}
+    rep, start, end = _run_with(["anti", "join"], -1.0, est)
+    assert rep["errors"] == 0 and start[1] >= end[0]
+    rep, start, end = _run_with(["join", "anti"], -1.0, est)
+    assert start[1] < end[0]
