@@ -496,7 +496,10 @@ def bench_serve(ctx, cpu_check: bool = True) -> dict:
         a = S.make_stream(db, rate, dur, seed=rate, menu=menu)
         b = S.make_stream(db, rate, dur, seed=rate, menu=menu)
         plain = S.serve(db, a, workers=1, budget_bytes=1 << 40, pool=pool)
-        served = S.serve(db, b, workers=4, budget_bytes=8 << 30, pool=pool)
+        # a query waits for a running one that ends within its own expected
+        # service time (seeded from the plain run's per-type service times)
+        served = S.serve(db, b, workers=4, budget_bytes=8 << 30, pool=pool, min_overlap_ms=-1.0,
+                         service_ms=plain["service_ms"])
         streams += [a, b]
         cmp = ST.compare([q.latency_ms for q in b], [q.latency_ms for q in a])
         out["rates"][str(rate)] = {"plain": plain, "served": served, "served_vs_plain_latency": cmp}
@@ -504,7 +507,8 @@ def bench_serve(ctx, cpu_check: bool = True) -> dict:
     a = S.make_stream(db, 400, 2.5, seed=99, backlog=True, menu=menu)
     b = S.make_stream(db, 400, 2.5, seed=99, backlog=True, menu=menu)
     plain = S.serve(db, a, workers=1, budget_bytes=1 << 40, pool=pool)
-    served = S.serve(db, b, workers=4, budget_bytes=8 << 30, pool=pool)
+    served = S.serve(db, b, workers=4, budget_bytes=8 << 30, pool=pool, min_overlap_ms=-1.0,
+                     service_ms=plain["service_ms"])
     streams += [a, b]
     out["saturated"] = {"plain": plain, "served": served}
     for w in pool:
