@@ -426,6 +426,9 @@ def bench_batch(ctx, steps: int) -> dict:
     t0 = time.perf_counter()
     tuned = w.autotune(trials=3)
     tune_s = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    batch_tuned = w.tune_batch(1024, 32768)   # the batched bodies' own settings, on the batched replay
+    batch_tune_s = time.perf_counter() - t0
     w.run_plain()
     plain_ms = [w.run_plain() for _ in range(steps)]
     plain = w.results()
@@ -459,6 +462,7 @@ def bench_batch(ctx, steps: int) -> dict:
                        "vs_plain": ST.compare(best[7], plain_ms)},
            "speedup": pm / best[1], "bit_exact_vs_plain": bool(best[6]), "autotune_s": round(tune_s, 2),
            "tuned": {k: {kk: vv for kk, vv in v.items() if kk != "candidates"} for k, v in tuned.items()},
+           "batch_tuned": dict(batch_tuned, seconds=round(batch_tune_s, 2)),
            "rows_per_s": rows / (best[1] / 1e3),
            # device time of the replay (fills, both batched kernels, status gather)
            "roofline": roofline("gdb_batch_kernel", ST.lower_median(best[8]), col_bytes, "w3")}

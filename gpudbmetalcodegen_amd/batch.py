@@ -119,6 +119,7 @@ class Workload3:
             self.batch_plans["agg"] = E.build_hip_plan("PIPE_AGG", f"{agg_opts},partials={partials}").compile(ctx)
         self.tables, self.execs, self.bitmaps = [], [], []
         self.launch = {}   # pipe index -> (block, items, lds_bits)
+        self.batch_launch = {}   # the batched form's own choices, when tune_batch has run
         for s in specs:
             t = E.DeviceTable.from_host(ctx, "P", s.cols)
             ex = E.HipGenericExecutor(ctx)
@@ -142,10 +143,14 @@ class Workload3:
     def total_rows(self) -> int:
         return sum(s.n for s in self.specs)
 
+    def launch_for(self, i: int, batched: bool = False) -> tuple[int, int, int]:
+        s = self.specs[i]
+        return self.batch_launch.get(s.index, self.launch[s.index]) if batched else self.launch[s.index]
+
     def _apply(self, i: int, batched: bool = False) -> E.HipQueryPlan:
         s = self.specs[i]
         plan = (self.batch_plans if batched else self.plans)[s.kind]
-        b, it, lb = self.launch[s.index]
+        b, it, lb = self.launch_for(i, batched)
         plan.set_launch(phase_name(s.kind), b, it)
         if s.kind == "agg":
             self.execs[i].register_scalar("pipe_lds_bits", lb)
@@ -186,6 +191,54 @@ class Workload3:
             self.launch[s.index] = (c["block"], c["items"], c["lds_bits"] if c["lds_bits"] is not None else 10)
         return chosen
 
+    def tune_batch(self, block: int = 1024, chunk_rows: int = 32768, items=(1, 2, 4, 8), lds_bits=(10, 12),
+                   runs: int = 7) -> dict:
+        """Per-kind body settings for the batched form, chosen on the batched
+        replay itself (device time, median of `runs` graph replays): first
+        ITEMS x LDS table bits shared by every aggregate body, then ITEMS
+        shared by every join body. Settings tuned on one pipeline launched
+        alone do not carry over: in the shared persistent kernel the LDS
+        table size also sets how many partial tables the merge level folds
+        and how many blocks share a CU. Starts from the per-pipeline choices
+        of autotune() (or the defaults); the plain form keeps those."""
+        self.batch_launch = {}
+        kinds = {s.kind for s in self.specs}
+        cand_bits = [b for b in lds_bits if (1 << b) <= self.lds_cap]
+        trace = []
+
+        def measure(tag: str) -> float:
+            b = self.build_batch(block, chunk_rows)
+            try:
+                b.run(graph=True)
+                ms = sorted(b.run(graph=True) for _ in range(runs))[runs // 2]
+            finally:
+                b.close()
+            trace.append({"setting": tag, "device_ms": round(ms, 4)})
+            return ms
+
+        def assign(kind_test, it: int, lb: int | None) -> None:
+            for i, s in enumerate(self.specs):
+                if kind_test(s.kind):
+                    bb, _, old_lb = self.launch_for(i, True)
+                    self.batch_launch[s.index] = (bb, it, old_lb if lb is None else lb)
+
+        chosen = {}
+        groups = [("agg", lambda k: k == "agg", cand_bits or [None]),
+                  ("join", lambda k: k != "agg", [None])]
+        for name, test, bits in groups:
+            if not any(test(k) for k in kinds):
+                continue
+            best = None
+            for lb in bits:
+                for it in items:
+                    assign(test, it, lb)
+                    ms = measure(f"{name} items={it}" + (f" lds_bits={lb}" if lb is not None else ""))
+                    if best is None or ms < best[0]:
+                        best = (ms, it, lb)
+            assign(test, best[1], best[2])
+            chosen[name] = {"items": best[1], "lds_bits": best[2], "device_ms": round(best[0], 4)}
+        return {"block": block, "chunk_rows": chunk_rows, "chosen": chosen, "trace": trace}
+
     # --- plain form: one query at a time ----------------------------------
     def run_plain(self) -> float:
         self.ctx.sync()
@@ -223,7 +276,7 @@ class Batch:
         for i, s in enumerate(w.specs):
             plan = w._apply(i, batched=True)
             idx = C.c_int()
-            N.eng_check(_e().gdb_batch_add(self._h, plan.handle, w.execs[i].handle, w.launch[s.index][1],
+            N.eng_check(_e().gdb_batch_add(self._h, plan.handle, w.execs[i].handle, w.launch_for(i, True)[1],
                                            C.byref(idx)))
         g, nitems = C.c_uint(), C.c_uint64()
         N.eng_check(_e().gdb_batch_build(self._h, block, chunk_rows, grid, C.byref(g), C.byref(nitems)))

@@ -745,48 +745,46 @@ GDB_DEV void gdb_topk_pop(GdbTopK<K>& t) {
     t.h[K - 1] = 0;
     t.l[K - 1] = 0;
 }
-// Wave-wide maximum of (h, l) pairs under gdb_rank_gt, in every lane: DPP
-// steps within each row of 16 lanes, then the four row maxima by readlane.
-// All 64 lanes must be active.
-GDB_DEV void gdb_wave_max_pair(u64& h, u64& l) {
-#define GDB_MAXPAIR_STEP(C)                                         \
-    {                                                               \
-        const u64 oh = gdb_dpp64<C>(h), ol = gdb_dpp64<C>(l);      \
-        if (gdb_rank_gt(oh, ol, h, l)) {                            \
-            h = oh;                                                 \
-            l = ol;                                                 \
-        }                                                           \
+// Wave-wide maximum of a u64, in every lane (the row steps of the sums
+// above, then the four row maxima by readlane). All 64 lanes must be active.
+GDB_DEV u64 gdb_wave_max_u64(u64 v) {
+#define GDB_MAXU64_STEP(C)                       \
+    {                                            \
+        const u64 o = gdb_dpp64<C>(v);           \
+        v = o > v ? o : v;                       \
     }
-    GDB_MAXPAIR_STEP(0xB1)
-    GDB_MAXPAIR_STEP(0x4E)
-    GDB_MAXPAIR_STEP(0x124)
-    GDB_MAXPAIR_STEP(0x128)
-#undef GDB_MAXPAIR_STEP
-    u64 bh = gdb_readlane64(h, 0), bl = gdb_readlane64(l, 0);
-#pragma unroll
-    for (int r = 16; r < 64; r += 16) {
-        const u64 xh = gdb_readlane64(h, r), xl = gdb_readlane64(l, r);
-        if (gdb_rank_gt(xh, xl, bh, bl)) {
-            bh = xh;
-            bl = xl;
-        }
-    }
-    h = bh;
-    l = bl;
+    GDB_MAXU64_STEP(0xB1)
+    GDB_MAXU64_STEP(0x4E)
+    GDB_MAXU64_STEP(0x124)
+    GDB_MAXU64_STEP(0x128)
+#undef GDB_MAXU64_STEP
+    const u64 a = gdb_readlane64(v, 0), b = gdb_readlane64(v, 16);
+    const u64 c = gdb_readlane64(v, 32), d = gdb_readlane64(v, 48);
+    const u64 ab = a > b ? a : b, cd = c > d ? c : d;
+    return ab > cd ? ab : cd;
 }
 // The wave's top K over every lane's sorted list: K rounds of a wave
 // argmax over the list heads; the lane owning a round's maximum pops it
-// (entries are unique: the low word carries the group key). Lane 0 writes
-// round r's entry to out[2r], out[2r + 1].
+// (entries are unique: the low word carries the group key). A round
+// compares high words only; the low words decide only between lanes whose
+// high words tie with the maximum (read from the one such lane, or a second
+// wave maximum over the tied lanes). Lane 0 writes round r's entry to
+// out[2r], out[2r + 1].
 template <int K>
 GDB_DEV void gdb_topk_wave_select(GdbTopK<K>& t, u64* out) {
     for (int r = 0; r < K; ++r) {
-        u64 h = t.h[0], l = t.l[0];
-        gdb_wave_max_pair(h, l);
-        if ((h | l) != 0 && t.h[0] == h && t.l[0] == l) gdb_topk_pop<K>(t);
+        const u64 h = t.h[0], l = t.l[0];
+        const u64 mh = gdb_wave_max_u64(h);
+        const u64 tied = __ballot(h == mh);
+        u64 ml;
+        if (__builtin_popcountll(tied) == 1)
+            ml = gdb_readlane64(l, __builtin_ctzll(tied));
+        else
+            ml = gdb_wave_max_u64(h == mh ? l : 0ull);
+        if ((mh | ml) != 0 && h == mh && l == ml) gdb_topk_pop<K>(t);
         if (gdb_lane() == 0) {
-            out[2 * r] = h;
-            out[2 * r + 1] = l;
+            out[2 * r] = mh;
+            out[2 * r + 1] = ml;
         }
     }
 }

@@ -84,6 +84,33 @@ def test_autotune_then_batch(w3):
         b.close()
 
 
+def test_batch_tuning_keeps_plain_settings_and_results(w3):
+    """tune_batch picks the batched bodies' settings on the batched replay;
+    the plain form keeps its own, and both forms still equal the oracle."""
+    plain_launch = dict(w3.launch)
+    tb = w3.tune_batch(block=256, chunk_rows=8192, items=(2, 4), lds_bits=(8, 12), runs=1)
+    assert w3.launch == plain_launch
+    assert tb["chosen"]["agg"]["items"] in (2, 4) and tb["chosen"]["agg"]["lds_bits"] in (8, 12)
+    assert tb["chosen"]["join"]["items"] in (2, 4)
+    assert len(tb["trace"]) == 4 + 2
+    for i, s in enumerate(w3.specs):
+        kind = "agg" if s.kind == "agg" else "join"
+        assert w3.launch_for(i, True)[1] == tb["chosen"][kind]["items"]
+    w3.run_plain()
+    plain = w3.results()
+    for s, got in zip(w3.specs, plain):
+        assert same(got, expected(s))
+    b = w3.build_batch(256, 8192)
+    try:
+        for _ in range(2):
+            b.run(graph=True)
+            for got, exp in zip(w3.results(), plain):
+                assert same(got, exp)
+    finally:
+        b.close()
+        w3.batch_launch = {}
+
+
 @pytest.mark.parametrize("partials,bits", [(0, None), (256, None), (2, 6)])
 def test_partial_tables_and_direct_merges_agree(ctx, partials, bits):
     """The aggregate pipelines' LDS tables reach the group table either as
