@@ -492,10 +492,11 @@ HipBitmapProbe::HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::st
 
 void HipBitmapProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapReadParam(bitmap_, bits_);
-    const bool cache = ldsWords_ > 0 && cg.rangeMode();
+    const bool cache = ldsWords_ > 0;
     const std::string C = "_bmc_" + bitmap_, CN = "_bmn_" + bitmap_;
     if (cache) {
-        cg.addComment("the bitmap's first words in LDS for this work item");
+        cg.addComment(cg.rangeMode() ? "the bitmap's first words in LDS for this work item"
+                                     : "the bitmap's first words in LDS, once per (persistent) block");
         cg.declareLds("u32", C, s(ldsWords_));
         cg.addLine("const u64 " + CN + " = (nbits_" + bitmap_ + " + 31ull) >> 5 < " + s(ldsWords_) + "ull ? (nbits_" +
                    bitmap_ + " + 31ull) >> 5 : " + s(ldsWords_) + "ull;");

@@ -177,11 +177,14 @@ public:
     bool stageable() const override { return true; }
     HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::string& bitmapName, const std::string& keyExpr,
                    const std::string& bitsExpr, bool anti = false);
-    // Batched (range-mode) bodies only: copy the bitmap's first `words`
-    // words into LDS at the start of each work item and test keys below them
-    // there. A batched kernel's LDS is sized by its largest body (a group-by
-    // table), so a join item can use it without lowering occupancy; plain
-    // launches keep their small LDS footprint.
+    // Copy the bitmap's first `words` words into LDS and test keys below
+    // them there (keys above read the global words): in a batched (range-
+    // mode) body at the start of each work item, in a plain launch once per
+    // block before its grid-stride loop (plain grids are capped at the
+    // resident blocks, so each block copies once). A batched kernel's LDS is
+    // sized by its largest body (a group-by table), so a join item uses it
+    // without lowering occupancy; a plain launch with a large prefix runs
+    // one block per CU and wants large blocks. 0 (the default): no copy.
     void setLdsCache(int words) { ldsWords_ = words; }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;

@@ -147,9 +147,11 @@ void q3CountPhase(HipQueryPlan& plan, const PlanOptions& o) {
     auto scan = makeScan("B", {"b_fkey", "b_date"});
     auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)b_date < q3_date");
     auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
+    const int aWords = optInt(o, "a_lds_words", 0);
+    probe->setLdsCache(aWords);
     auto cnt = std::make_unique<HipKeyedAgg>(std::move(probe), "d_q3_bcount", "0", 1);
     cnt->addAggregate("b_rows", AggKind::Count, "1", 0);
-    auto& ph = appendPhase(plan, "Q3_count_b", std::move(cnt), 256, 8);
+    auto& ph = appendPhase(plan, "Q3_count_b", std::move(cnt), optInt(o, "block_count", aWords > 0 ? 1024 : 256), 8);
     ph.scalarParams = {"q3_date"};
     ph.staged = optInt(o, "staged", 1) != 0;  // measured: staged bitmap loads help this phase
 }
@@ -179,6 +181,10 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         auto scan = makeScan("B", {"b_key", "b_fkey", "b_date"});
         auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)b_date < q3_date");
         auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
+        // a_lds_words=N: the A bitmap's first N words are tested in LDS (one
+        // block of 1024 per CU then, unless block_b / block_count say otherwise)
+        const int aWords = optInt(o, "a_lds_words", 0);
+        probe->setLdsCache(aWords);
         std::unique_ptr<HipOperator> top =
             std::make_unique<HipHashBuild>(std::move(probe), "d_q3_b_ht", "b_key", "b_date", "b_slots", 2);
         // bflags=1: the B-key bitmap is built as byte flags (plain stores, no
@@ -187,7 +193,8 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
             top = std::make_unique<HipByteFlagBuild>(std::move(top), "d_q3_b_flags", "b_key", "b_bits");
         else if (probeBitmap)
             top = std::make_unique<HipBitmapBuild>(std::move(top), "d_q3_b_bitmap", "b_key", "b_bits");
-        auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), 256, optInt(o, "items_b", 4));
+        auto& ph = appendPhase(plan, "Q3_build_b_table", std::move(top), optInt(o, "block_b", aWords > 0 ? 1024 : 256),
+                               optInt(o, "items_b", 4));
         ph.scalarParams = {"q3_date"};
         // staged=1 puts the home-slot CAS of a thread's ITEMS rows in flight
         // together; per-row emission measures faster here (0.166 vs 0.177 ms

@@ -48,7 +48,9 @@ def test_q3_small(ctx, seed, seg, d):
                                      "late=0,prefetch=1", "late=0,prefetch=1,items=8", "late=0,compact=1",
                                      "late=0,compact=1,items=8", "late=0,compact=1,probe_bitmap=0", "late=1,items=8",
                                      "late=1,probe_bitmap=0", "late=1,sizing=given,block=128", "late=1,compact=0",
-                                     "late=1,compact=0,items=8", "bflags=1", "bflags=1,sizing=given"])
+                                     "late=1,compact=0,items=8", "bflags=1", "bflags=1,sizing=given",
+                                     "a_lds_words=32", "a_lds_words=32768", "a_lds_words=64,block_b=256,block_count=512",
+                                     "a_lds_words=32,sizing=given"])
 def test_q3_plan_variants(ctx, options):
     A, B, F = small_join(11, n_a=3000, n_b=40_000)
     _check(ctx, A, B, F, 2, 60, options)
