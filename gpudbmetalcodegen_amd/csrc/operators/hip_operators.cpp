@@ -492,7 +492,7 @@ HipBitmapProbe::HipBitmapProbe(std::unique_ptr<HipOperator> child, const std::st
 
 void HipBitmapProbe::produceImpl(HipCodegen& cg, ConsumerFn consume) {
     cg.addBitmapReadParam(bitmap_, bits_);
-    const bool cache = ldsWords_ > 0;
+    const bool cache = ldsWords_ > 0 && (ldsPlain_ || cg.rangeMode());
     const std::string C = "_bmc_" + bitmap_, CN = "_bmn_" + bitmap_;
     if (cache) {
         cg.addComment(cg.rangeMode() ? "the bitmap's first words in LDS for this work item"

@@ -179,13 +179,17 @@ public:
                    const std::string& bitsExpr, bool anti = false);
     // Copy the bitmap's first `words` words into LDS and test keys below
     // them there (keys above read the global words): in a batched (range-
-    // mode) body at the start of each work item, in a plain launch once per
-    // block before its grid-stride loop (plain grids are capped at the
-    // resident blocks, so each block copies once). A batched kernel's LDS is
-    // sized by its largest body (a group-by table), so a join item uses it
-    // without lowering occupancy; a plain launch with a large prefix runs
-    // one block per CU and wants large blocks. 0 (the default): no copy.
-    void setLdsCache(int words) { ldsWords_ = words; }
+    // mode) body at the start of each work item, and with `plain` also in a
+    // plain launch, once per block before its grid-stride loop (plain grids
+    // are capped at the resident blocks, so each block copies once). A
+    // batched kernel's LDS is sized by its largest body (a group-by table),
+    // so a join item uses it without lowering occupancy; a plain launch with
+    // a large prefix runs one block per CU and wants large blocks. 0 (the
+    // default): no copy.
+    void setLdsCache(int words, bool plain = false) {
+        ldsWords_ = words;
+        ldsPlain_ = plain;
+    }
     void produceImpl(HipCodegen& cg, ConsumerFn consume) override;
     std::string describe() const override;
 
@@ -193,6 +197,7 @@ private:
     std::string bitmap_, key_, bits_;
     bool anti_;
     int ldsWords_ = 0;
+    bool ldsPlain_ = false;
 };
 
 // Open-addressing hash build of unique keys with a 32-bit payload. Replaces

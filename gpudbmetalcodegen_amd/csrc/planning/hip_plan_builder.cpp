@@ -148,7 +148,7 @@ void q3CountPhase(HipQueryPlan& plan, const PlanOptions& o) {
     auto sel = std::make_unique<HipSelection>(std::move(scan), "(i64)b_date < q3_date");
     auto probe = std::make_unique<HipBitmapProbe>(std::move(sel), "d_q3_a_bitmap", "b_fkey", "a_bits");
     const int aWords = optInt(o, "a_lds_words", 0);
-    probe->setLdsCache(aWords);
+    probe->setLdsCache(aWords, true);
     auto cnt = std::make_unique<HipKeyedAgg>(std::move(probe), "d_q3_bcount", "0", 1);
     cnt->addAggregate("b_rows", AggKind::Count, "1", 0);
     auto& ph = appendPhase(plan, "Q3_count_b", std::move(cnt), optInt(o, "block_count", aWords > 0 ? 1024 : 256), 8);
@@ -184,7 +184,7 @@ HipQueryPlan buildQ3(const PlanOptions& o) {
         // a_lds_words=N: the A bitmap's first N words are tested in LDS (one
         // block of 1024 per CU then, unless block_b / block_count say otherwise)
         const int aWords = optInt(o, "a_lds_words", 0);
-        probe->setLdsCache(aWords);
+        probe->setLdsCache(aWords, true);
         std::unique_ptr<HipOperator> top =
             std::make_unique<HipHashBuild>(std::move(probe), "d_q3_b_ht", "b_key", "b_date", "b_slots", 2);
         // bflags=1: the B-key bitmap is built as byte flags (plain stores, no
