@@ -810,8 +810,8 @@ private:
 
 // Top-k by a 128-bit rank (hi, lo), larger first, rank (0,0) reserved.
 // Per-thread register lists, wave then block selection (DPP argmax rounds),
-// one candidate list per block, and a final merge by the last block to
-// finish (one ticket atomic per block).
+// one candidate list per block; a following one-block phase (HipTopKMerge,
+// from makeMerge) merges the lists.
 // Replaces the reference's CPU full sort (codegen/codegen_main.cpp:1211-1248).
 class HipTopK : public HipUnaryOperator {
 public:
