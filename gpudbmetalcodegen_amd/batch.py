@@ -303,3 +303,51 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def main(argv=None) -> int:
+    """Run the 64-pipeline workload's batched form on device 0: per-shape
+    autotune, the batched bodies' own tuning, one batch build, then
+    `--replays` graph replays; prints one JSON line (settings and device
+    milliseconds per replay). A fixed command for profilers."""
+    import argparse
+    import json
+    ap = argparse.ArgumentParser(prog="python -m gpudbmetalcodegen_amd.batch")
+    ap.add_argument("--replays", type=int, default=20)
+    ap.add_argument("--block", type=int, default=1024)
+    ap.add_argument("--chunk", type=int, default=65536)
+    ap.add_argument("--no-tune", action="store_true", help="keep the default body settings")
+    ap.add_argument("--set", default="", metavar="AGG_ITEMS,LDS_BITS,JOIN_ITEMS",
+                    help="batched body settings given instead of tuned (e.g. 4,12,2)")
+    args = ap.parse_args(argv)
+    from pathlib import Path
+    ctx = E.Context(0)
+    ctx.set_cache_dir(str(Path(__file__).resolve().parent / "_cocache"))
+    w = Workload3(ctx, workload3(), lds_cap=4096)
+    try:
+        tuned = None
+        if args.set:
+            ai, lb, ji = (int(x) for x in args.set.split(","))
+            for i, s in enumerate(w.specs):
+                w.batch_launch[s.index] = (256, ai, lb) if s.kind == "agg" else (256, ji, 10)
+            tuned = {"agg": {"items": ai, "lds_bits": lb}, "join": {"items": ji}, "given": True}
+        elif not args.no_tune:
+            w.autotune(trials=3)
+            tuned = w.tune_batch(args.block, 32768)["chosen"]
+        b = w.build_batch(args.block, args.chunk)
+        try:
+            b.run(graph=True)
+            ms = [b.run(graph=True) for _ in range(args.replays)]
+        finally:
+            b.close()
+        print(json.dumps({"rows": w.total_rows, "block": args.block, "chunk_rows": args.chunk, "grid": b.grid,
+                          "work_items": b.work_items, "batch_tuned": tuned,
+                          "device_ms_median": sorted(ms)[len(ms) // 2], "device_ms": [round(x, 4) for x in ms]}),
+              flush=True)
+    finally:
+        w.close()
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())
