@@ -434,7 +434,7 @@ def bench_batch(ctx, steps: int) -> dict:
     plain = w.results()
     best = None
     for block in (256, 512, 1024):
-        for chunk in (32768, 65536):
+        for chunk in (32768, 65536, 131072):
             b = w.build_batch(block, chunk)
             b.run(graph=True)
             ms_each, dev_each = [], []
