@@ -78,6 +78,11 @@ def _sig(lib, name, args, res=C.c_int):
     f.restype = res
 
 
+class FillDesc(C.Structure):
+    """gdb_fill_desc (gdb_hip.h): one buffer of a device-resident fill table."""
+    _fields_ = [("dst", C.c_void_p), ("words", C.c_uint64), ("period", C.c_uint64), ("pattern", C.c_uint64 * 4)]
+
+
 def load() -> tuple[C.CDLL, C.CDLL]:
     """Load libgdbhip.so and libgdbengine.so (raises if they are not built)."""
     global _hip, _eng
@@ -104,6 +109,8 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(hip, "gdb_memcpy_d2h_async", [vp, vp, sz, vp])
     _sig(hip, "gdb_memcpy_d2d_async", [vp, vp, sz, vp])
     _sig(hip, "gdb_fill_async", [vp, sz, vp, i32, vp])
+    _sig(hip, "gdb_fill_multi_async", [i32, P(vp), P(sz), P(vp), P(i32), vp])
+    _sig(hip, "gdb_fill_table_async", [vp, i32, u64, vp])
     _sig(hip, "gdb_memset_async", [vp, i32, sz, vp])
     _sig(hip, "gdb_module_load", [vp, cstr, sz, P(cstr), i32, P(u64), P(vp)])
     _sig(hip, "gdb_module_precompile", [cstr, cstr, cstr, sz, P(cstr), i32, P(u64), P(i32)])

@@ -136,6 +136,29 @@ std::unordered_map<std::string, const HipParamBinding*> HipGenericExecutor::init
 
 namespace {
 
+// A size and pattern the multi-buffer fill takes (gdb_fill_multi_async):
+// whole 8-byte words, a pattern of 1, 2 or 4 bytes or 1..4 words dividing it.
+bool multiFillable(size_t bytes, int pl) {
+    if (bytes % 8 != 0 || pl < 1) return false;
+    if (pl == 1 || pl == 2 || pl == 4) return true;
+    return pl % 8 == 0 && pl <= 32 && bytes % static_cast<size_t>(pl) == 0;
+}
+
+// The shortest prefix of a fill pattern that repeats to the whole pattern,
+// among the lengths the multi-buffer fill takes (1, 2, 4 bytes, 1..4 words):
+// a 16-word all-zero bucket pattern fills as one zero word. The pattern
+// itself when none does.
+std::vector<uint8_t> shortestPeriod(const std::vector<uint8_t>& pat) {
+    for (size_t p : {1, 2, 4, 8, 16, 24, 32}) {
+        if (p >= pat.size()) break;
+        if (pat.size() % p != 0) continue;
+        bool periodic = true;
+        for (size_t i = p; i < pat.size() && periodic; ++i) periodic = pat[i] == pat[i % p];
+        if (periodic) return std::vector<uint8_t>(pat.begin(), pat.begin() + static_cast<long>(p));
+    }
+    return pat;
+}
+
 bool mentionsSymbol(const std::string& expr, const std::string& sym) {
     size_t pos = 0;
     while ((pos = expr.find(sym, pos)) != std::string::npos) {
@@ -158,7 +181,7 @@ void HipGenericExecutor::hoistInits(const HipCodegen& cg, int first, int last,
     std::vector<size_t> bytes;
     std::vector<const void*> pats;
     std::vector<int> plen;
-    std::vector<uint8_t> zero(8, 0);
+    std::vector<std::vector<uint8_t>> periods;  // each buffer's pattern at its shortest period
     const auto& phases = cg.getPhases();
     for (int pi = first; pi < last; ++pi) {
         const auto& p = phases[static_cast<size_t>(pi)];
@@ -180,17 +203,20 @@ void HipGenericExecutor::hoistInits(const HipCodegen& cg, int first, int last,
             // zero fills may cover the allocation's padding (allocations are
             // whole 256-byte multiples), so byte arrays of any length join
             if (ib->init == InitKind::Zero) nb = (nb + 7) & ~static_cast<size_t>(7);
-            const int pl = ib->init == InitKind::Zero ? 8 : static_cast<int>(ib->pattern.size());
-            if (nb == 0 || nb % 8 != 0 || pl > 32 || (32 % pl) != 0 || nb % static_cast<size_t>(pl) != 0) continue;
+            periods.push_back(ib->init == InitKind::Zero ? std::vector<uint8_t>(8, 0) : shortestPeriod(ib->pattern));
+            const int pl = static_cast<int>(periods.back().size());
+            if (nb == 0 || !multiFillable(nb, pl)) {
+                periods.pop_back();
+                continue;
+            }
             Buffer& buf = ensureBuffer(*ib);
             dst.push_back(buf.ptr);
             bytes.push_back(nb);
-            pats.push_back(ib->init == InitKind::Zero ? static_cast<const void*>(zero.data())
-                                                      : static_cast<const void*>(ib->pattern.data()));
             plen.push_back(pl);
             initDone[b.name] = true;
         }
     }
+    for (const auto& pp : periods) pats.push_back(pp.data());
     if (!dst.empty())
         check(gdb_fill_multi_async(static_cast<int>(dst.size()), dst.data(), bytes.data(), pats.data(), plen.data(),
                                    stream_),
@@ -348,14 +374,13 @@ void HipGenericExecutor::collectInits(const HipCodegen& cg, FillList& out) {
         Buffer& buf = ensureBuffer(*ib);
         size_t bytes = static_cast<size_t>(sizes_.resolve(ib->sizeExpr)) * ib->elemBytes;
         if (!bytes) continue;
-        std::vector<uint8_t> pat = ib->init == InitKind::Zero ? std::vector<uint8_t>(8, 0) : ib->pattern;
+        std::vector<uint8_t> pat = ib->init == InitKind::Zero ? std::vector<uint8_t>(8, 0) : shortestPeriod(ib->pattern);
         // zero fills may cover the allocation's padding (whole 256-byte multiples)
         if (ib->init == InitKind::Zero) bytes = (bytes + 7) & ~static_cast<size_t>(7);
         const int pl = static_cast<int>(pat.size());
-        const bool multi = pl >= 1 && pl <= 32 && 32 % pl == 0 && bytes % 8 == 0 && bytes % static_cast<size_t>(pl) == 0 &&
-                           reinterpret_cast<uintptr_t>(buf.ptr) % 8 == 0 && bytes <= buf.bytes;
+        const bool multi = multiFillable(bytes, pl) && reinterpret_cast<uintptr_t>(buf.ptr) % 8 == 0 && bytes <= buf.bytes;
         if (!multi) {
-            check(gdb_fill_async(buf.ptr, bytes, pat.data(), pl, stream_), "fill " + ib->name);
+            out.singles.push_back({buf.ptr, bytes, pat});
             continue;
         }
         std::array<uint8_t, 32> w{};

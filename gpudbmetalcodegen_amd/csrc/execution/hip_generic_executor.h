@@ -102,15 +102,22 @@ public:
     // the initialisation fills of every buffer of the plan.
     std::vector<uint64_t> packPhaseArgs(const HipCodegen& cg, int phase);
     void enqueueInits(const HipCodegen& cg);
-    // The same fills gathered for one multi-buffer fill launch
-    // (gdb_fill_multi_async) by a caller that batches many runners; a fill
-    // whose pattern or size does not fit that form is enqueued here.
+    // The same fills gathered by a caller that batches many runners.
     struct FillList {
         std::vector<void*> dst;
         std::vector<size_t> bytes;
         std::vector<std::array<uint8_t, 32>> pattern;  // the pattern repeated to 32 bytes
-        std::vector<int> plen;                          // its length (divides 32 and the size)
+        std::vector<int> plen;                          // its length (1, 2, 4 bytes or 1..4 words)
+        struct Single {                                 // fills the multi-buffer fill cannot take
+            void* dst;
+            size_t bytes;
+            std::vector<uint8_t> pattern;
+        };
+        std::vector<Single> singles;
     };
+    // Every initialisation of cg's buffers (allocating them); the caller
+    // issues them (gdb_fill_multi_async / gdb_fill_table_async for the
+    // lists, gdb_fill_async for each single).
     void collectInits(const HipCodegen& cg, FillList& out);
     // Device addresses of the phases' status and check words (4 x u64 each)
     // for a caller that reads them in bulk, and how to judge them.

@@ -184,23 +184,59 @@ __global__ void __launch_bounds__(256) gdb_fill_kernel(unsigned long long* __res
 }
 
 // Multi-buffer fill: one launch initialises up to kFillMax buffers, each with
-// a pattern of up to 32 bytes (4 words, repeated); blockIdx.y selects the
-// buffer. Used to issue all of a run's buffer initialisations at once (48
-// descriptors of 48 bytes: 2.3 KB of kernel arguments).
+// a pattern of 1..4 words repeated; blockIdx.y selects the buffer. Used to
+// issue all of a run's buffer initialisations at once (48 descriptors of 48
+// bytes: 2.3 KB of kernel arguments; the pattern period, in words, rides in
+// the top two bits of the word count).
 constexpr int kFillMax = 48;
 struct FillDesc {
     unsigned long long* dst;
-    unsigned long long nwords;
+    unsigned long long nwords;  // words | (period - 1) << 62
     unsigned long long w[4];
 };
 struct FillDescs {
     FillDesc d[kFillMax];
 };
 
+// Words i = start, start + stride, ... of one buffer set to w[i % period]:
+// the phase advances by stride % period per step (one conditional subtract).
+__device__ __forceinline__ void gdb_fill_words(unsigned long long* __restrict__ dst, size_t n, unsigned period,
+                                               const unsigned long long* w) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    if (period == 1) {
+        const unsigned long long v = w[0];
+        for (; i < n; i += stride) dst[i] = v;
+        return;
+    }
+    unsigned ph = (unsigned)(i % period);
+    const unsigned step = (unsigned)(stride % period);
+    for (; i < n; i += stride) {
+        dst[i] = w[ph];
+        ph += step;
+        if (ph >= period) ph -= period;
+    }
+}
+
 __global__ void __launch_bounds__(256) gdb_fill_multi_kernel(FillDescs ds) {
     const FillDesc& f = ds.d[blockIdx.y];
-    const size_t stride = (size_t)gridDim.x * blockDim.x;
-    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < f.nwords; i += stride) f.dst[i] = f.w[i & 3];
+    unsigned long long w[4] = {f.w[0], f.w[1], f.w[2], f.w[3]};
+    gdb_fill_words(f.dst, (size_t)(f.nwords & ((1ull << 62) - 1ull)), (unsigned)(f.nwords >> 62) + 1u, w);
+}
+
+__global__ void __launch_bounds__(256) gdb_fill_table_kernel(const gdb_fill_desc* __restrict__ t) {
+    const gdb_fill_desc f = t[blockIdx.y];
+    unsigned long long w[4] = {f.pattern[0], f.pattern[1], f.pattern[2], f.pattern[3]};
+    gdb_fill_words(static_cast<unsigned long long*>(f.dst), (size_t)f.words, (unsigned)f.period, w);
+}
+
+// A fill pattern of 1, 2 or 4 bytes widened to one word, or 1..4 whole words:
+// its period in words, or 0 when the buffer cannot take it.
+int fillPeriod(size_t bytes, int pl, const void* dst) {
+    if (bytes % 8 != 0 || reinterpret_cast<uintptr_t>(dst) % 8 != 0 || pl < 1) return 0;
+    if (pl == 1 || pl == 2 || pl == 4) return 1;
+    if (pl % 8 == 0 && pl <= 32 && bytes % static_cast<size_t>(pl) == 0) return pl / 8;
+    return 0;
 }
 
 }  // namespace
@@ -367,11 +403,10 @@ gdb_status gdb_fill_multi_async(int n, void* const* dst, const size_t* bytes, co
     if (n <= 0) return GDB_OK;
     if (!dst || !bytes || !patterns || !pattern_len) return fail(GDB_ERR_INVALID, "gdb_fill_multi_async: null argument");
     for (int i = 0; i < n; ++i) {
-        const int pl = pattern_len[i];
-        if (!dst[i] || !patterns[i] || pl < 1 || pl > 32 || (32 % pl) != 0 || bytes[i] % 8 != 0 ||
-            bytes[i] % static_cast<size_t>(pl) != 0 || reinterpret_cast<uintptr_t>(dst[i]) % 8 != 0)
+        if (!dst[i] || !patterns[i] || fillPeriod(bytes[i], pattern_len[i], dst[i]) == 0)
             return fail(GDB_ERR_INVALID, "gdb_fill_multi_async: buffer " + std::to_string(i) +
-                                             " needs 8-byte alignment and size and a pattern dividing 32 bytes");
+                                             " needs 8-byte alignment and size and a pattern of 1, 2, 4 bytes or "
+                                             "1..4 words dividing the size");
     }
     for (int b = 0; b < n; b += kFillMax) {
         const int m = std::min(kFillMax, n - b);
@@ -379,11 +414,13 @@ gdb_status gdb_fill_multi_async(int n, void* const* dst, const size_t* bytes, co
         size_t maxWords = 0;
         for (int j = 0; j < m; ++j) {
             FillDesc& f = ds.d[j];
+            const int pl = pattern_len[b + j];
+            const int period = fillPeriod(bytes[b + j], pl, dst[b + j]);
             f.dst = static_cast<unsigned long long*>(dst[b + j]);
-            f.nwords = bytes[b + j] / 8;
+            f.nwords = (bytes[b + j] / 8) | (static_cast<unsigned long long>(period - 1) << 62);
             unsigned char* pb = reinterpret_cast<unsigned char*>(f.w);
-            for (int k = 0; k < 32; ++k) pb[k] = static_cast<const unsigned char*>(patterns[b + j])[k % pattern_len[b + j]];
-            maxWords = std::max<size_t>(maxWords, f.nwords);
+            for (int k = 0; k < 8 * period; ++k) pb[k] = static_cast<const unsigned char*>(patterns[b + j])[k % pl];
+            maxWords = std::max<size_t>(maxWords, bytes[b + j] / 8);
         }
         size_t blocks = (maxWords + 255) / 256;
         if (blocks > 4096) blocks = 4096;
@@ -392,6 +429,19 @@ gdb_status gdb_fill_multi_async(int n, void* const* dst, const size_t* bytes, co
                            0, static_cast<hipStream_t>(s), ds);
         GDB_HIP_TRY(hipGetLastError(), "gdb_fill_multi_kernel launch");
     }
+    return GDB_OK;
+}
+
+gdb_status gdb_fill_table_async(const gdb_fill_desc* device_table, int n, uint64_t max_words, gdb_stream s) {
+    if (n <= 0) return GDB_OK;
+    if (!device_table) return fail(GDB_ERR_INVALID, "gdb_fill_table_async: null table");
+    if (n > 65535) return fail(GDB_ERR_INVALID, "gdb_fill_table_async: more than 65535 buffers");
+    size_t blocks = (static_cast<size_t>(max_words) + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(gdb_fill_table_kernel, dim3(static_cast<unsigned>(blocks), static_cast<unsigned>(n)), dim3(256), 0,
+                       static_cast<hipStream_t>(s), device_table);
+    GDB_HIP_TRY(hipGetLastError(), "gdb_fill_table_kernel launch");
     return GDB_OK;
 }
 

@@ -4,6 +4,7 @@
 #include "gdb/gdb_kernels.h"
 
 #include <algorithm>
+#include <cstring>
 #include <memory>
 #include <set>
 #include <sstream>
@@ -48,7 +49,7 @@ void PipelineBatch::freeDevice() {
             if (*p) gdb_free(ctx_, *p);
             *p = nullptr;
         }
-    for (void** p : {&dTicket_, &dStatusSrc_, &dStatusOut_}) {
+    for (void** p : {&dTicket_, &dFill_, &dStatusSrc_, &dStatusOut_}) {
         if (*p) gdb_free(ctx_, *p);
         *p = nullptr;
     }
@@ -263,6 +264,26 @@ void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
     }
     const uint64_t zero = 0;
     up(dTicket_, &zero, 8);
+    HipGenericExecutor::FillList fl;
+    for (auto& p : pipes_) p.ex->collectInits(*p.cg, fl);
+    fl.dst.push_back(dTicket_);
+    fl.bytes.push_back(8);
+    fl.pattern.push_back({});
+    fl.plen.push_back(8);
+    std::vector<gdb_fill_desc> table(fl.dst.size());
+    fillMaxWords_ = 0;
+    for (size_t i = 0; i < fl.dst.size(); ++i) {
+        gdb_fill_desc& d = table[i];
+        const int pl = fl.plen[i];
+        d.dst = fl.dst[i];
+        d.words = fl.bytes[i] / 8;
+        d.period = (pl == 1 || pl == 2 || pl == 4) ? 1 : static_cast<uint64_t>(pl / 8);
+        std::memcpy(d.pattern, fl.pattern[i].data(), 32);
+        fillMaxWords_ = std::max<uint64_t>(fillMaxWords_, d.words);
+    }
+    fillN_ = static_cast<int>(table.size());
+    up(dFill_, table.data(), table.size() * sizeof(gdb_fill_desc));
+    fillSingles_ = std::move(fl.singles);
     statusRefs_.clear();
     for (const auto& p : pipes_)
         for (const auto& r : p.ex->statusRefs(*p.cg)) statusRefs_.push_back(r);
@@ -285,19 +306,13 @@ void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
 
 void PipelineBatch::enqueue() {
     if (!built_) throw std::runtime_error("PipelineBatch: build() first");
-    // every pipeline's buffer fills and the work ticket in one multi-buffer
-    // fill (a few launches for 64 pipelines instead of one per buffer)
-    HipGenericExecutor::FillList fl;
-    for (auto& p : pipes_) p.ex->collectInits(*p.cg, fl);
-    fl.dst.push_back(dTicket_);
-    fl.bytes.push_back(8);
-    fl.pattern.push_back({});
-    fl.plen.push_back(8);
-    std::vector<const void*> pats;
-    for (const auto& w : fl.pattern) pats.push_back(w.data());
-    check(gdb_fill_multi_async(static_cast<int>(fl.dst.size()), fl.dst.data(), fl.bytes.data(), pats.data(),
-                               fl.plen.data(), stream_),
+    // every pipeline's buffer fills and the work ticket: one launch over the
+    // fill table made at build (one kernel for 64 pipelines' ~130 buffers)
+    check(gdb_fill_table_async(static_cast<const gdb_fill_desc*>(dFill_), fillN_, fillMaxWords_, stream_),
           "fill pipeline buffers");
+    for (const auto& f : fillSingles_)
+        check(gdb_fill_async(f.dst, f.bytes, f.pattern.data(), static_cast<int>(f.pattern.size()), stream_),
+              "fill pipeline buffer");
     for (size_t lv = 0; lv < levels_.size(); ++lv) {
         const Level& L = levels_[lv];
         if (L.itemPipe.empty()) continue;

@@ -126,6 +126,13 @@ private:
     int block_ = 256;
     std::vector<Level> levels_;
     void* dTicket_ = nullptr;
+    // every pipeline's buffer initialisations and the work ticket, as a fill
+    // table in device memory (the buffers are fixed at build): one launch per
+    // replay; fills the table cannot express are issued one by one
+    void* dFill_ = nullptr;
+    int fillN_ = 0;
+    uint64_t fillMaxWords_ = 0;
+    std::vector<codegen::HipGenericExecutor::FillList::Single> fillSingles_;
     // every pipeline's status and check words: gathered on the device after
     // the fused kernel and read back with one copy into pinned memory
     std::vector<codegen::HipGenericExecutor::StatusRef> statusRefs_;

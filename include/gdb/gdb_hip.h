@@ -97,10 +97,25 @@ gdb_status gdb_fill_async(void* dst, size_t bytes, const void* pattern, int patt
                           gdb_stream s);
 /* Fill n buffers in one launch (no counterpart in the reference, whose buffer
  * initialisation is a host memset per buffer, metal_generic_executor.cpp:164-175):
- * buffer i gets patterns[i] (1..32 bytes, a divisor of 32) repeated;
- * sizes and addresses must be multiples of 8. Graph-safe like gdb_fill_async. */
+ * buffer i gets patterns[i] repeated, a pattern of 1, 2 or 4 bytes or of 1..4
+ * whole 8-byte words (8, 16, 24 or 32 bytes) that divides the size; sizes and
+ * addresses must be multiples of 8. Graph-safe like gdb_fill_async. */
 gdb_status gdb_fill_multi_async(int n, void* const* dst, const size_t* bytes, const void* const* patterns,
                                 const int* pattern_len, gdb_stream s);
+/* One buffer of a fill table: `words` 8-byte words at dst (8-byte aligned)
+ * set to pattern[i % period], period 1..4. */
+typedef struct gdb_fill_desc {
+    void* dst;
+    uint64_t words;
+    uint64_t period;
+    uint64_t pattern[4];
+} gdb_fill_desc;
+/* Fill every buffer of a table of n descriptors that is already in device
+ * memory, in one launch (no counterpart in the reference; for callers that
+ * initialise the same buffers on every replay, e.g. a batch of pipelines
+ * captured in a graph). The table must stay allocated until the launch has
+ * run; max_words (the largest `words`) sizes the grid. Graph-safe. */
+gdb_status gdb_fill_table_async(const gdb_fill_desc* device_table, int n, uint64_t max_words, gdb_stream s);
 gdb_status gdb_memset_async(void* dst, int value, size_t bytes, gdb_stream s);
 
 /* JIT. Compile HIP C++ for the context's architecture with hipRTC, or take the
