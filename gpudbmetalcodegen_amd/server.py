@@ -254,7 +254,10 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
     the waiting query's own expected service time (waiting is then shorter
     than running it stretched beside the other). Expected service time per
     plan type: service_ms, then a running mean over completed queries. Sharing
-    the device stretches both queries, which costs more than a short wait."""
+    the device stretches both queries, which costs more than a short wait.
+    The window holds only while at most `workers` queries wait: in a backlog
+    every query admits as soon as it fits (measured on the saturated four-type
+    stream: 91 G rows/s with the window held, 107-113 G without)."""
     own = pool is None
     pool = (pool or [Worker(db) for _ in range(workers)])[:workers]
     arrivals = sorted(stream, key=lambda q: (q.arrival, q.qid))
@@ -321,8 +324,11 @@ def serve(db: ResidentDB, stream: list[Query], workers: int = 4, budget_bytes: i
                 for j, c in enumerate(pending):
                     fits = state["used"] + c.footprint <= budget_bytes or state["inflight"] == 0
                     heavy_ok = c.rows < heavy_rows or state["heavy"] < max_heavy or state["inflight"] == 0
-                    # a running query about to finish: this one waits for it
-                    soon = frees_soon(now, c.kind) if state["inflight"] > 0 else None
+                    # a running query about to finish: this one waits for it,
+                    # unless more queries wait than there are workers (a
+                    # backlog: then throughput decides, and every wait adds up)
+                    backlog = len(pending) > len(pool)
+                    soon = frees_soon(now, c.kind) if state["inflight"] > 0 and not backlog else None
                     if soon is not None:
                         wake = soon if wake is None else min(wake, soon)
                     if fits and heavy_ok and soon is None:

@@ -128,3 +128,13 @@ def test_overlap_window_of_the_waiting_query():
     assert rep["errors"] == 0 and start[1] >= end[0]
     rep, start, end = _run_with(["join", "anti"], -1.0, est)
     assert start[1] < end[0]
+
+
+def test_overlap_window_gives_way_to_a_backlog():
+    """With more queries waiting than workers, the overlap window holds
+    nothing back: the join behind an anti-join about to end starts at once
+    when four more queries wait behind it."""
+    est = {"scan": 30.0, "join": 40.0, "anti": 5.0, "like": 5.0}
+    rep, start, end = _run_with(["anti", "join", "anti", "anti", "anti", "anti"], -1.0, est)
+    assert rep["errors"] == 0 and rep["queries"] == 6
+    assert start[1] < end[0]
