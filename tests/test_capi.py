@@ -53,6 +53,19 @@ def test_no_gpu_paths_report_status():
     assert b"no GPU" in hip.gdb_last_error()
 
 
+def test_batch_command_fails_loudly_without_a_gpu():
+    """The batched workload command refuses bad settings before touching the
+    device and, with no GPU visible, stops with the runtime's error instead
+    of falling back to anything."""
+    from gpudbmetalcodegen_amd import batch as B
+    with pytest.raises(SystemExit):
+        B.main(["--replays", "x"])
+    if E.device_count() > 0:
+        pytest.skip("a GPU is visible")
+    with pytest.raises(N.GdbError, match="no GPU"):
+        B.main(["--replays", "1", "--set", "4,12,2"])
+
+
 @pytest.mark.parametrize("name,opts", PLANS)
 def test_plan_build_describe_source(name, opts):
     p = E.build_hip_plan(name, opts)
@@ -156,6 +169,19 @@ def test_partial_table_plan_and_two_level_batch_kernel(tmp_path):
     h, wc = C.c_uint64(), C.c_int()
     st = hip.gdb_module_precompile(str(tmp_path).encode(), b"gfx950", b, len(b), None, 0, C.byref(h), C.byref(wc))
     assert st == 0, hip.gdb_last_error().decode()[-2000:]
+
+
+def test_q3_a_bitmap_prefix_in_plain_launches_only_when_asked():
+    """Q3's a_lds_words=N copies the A bitmap's first N words into LDS once
+    per block of the plain count and build launches (and raises their
+    default block to 1024); without it neither phase declares the copy."""
+    on = E.build_hip_plan("Q3", "a_lds_words=4096")
+    src = on.source()
+    assert src.count("__attribute__((aligned(16))) u32 _bmc_d_q3_a_bitmap[4096]") >= 2
+    assert on.get_launch("Q3_count_b")[0] == 1024 and on.get_launch("Q3_build_b_table")[0] == 1024
+    off = E.build_hip_plan("Q3")
+    assert "u32 _bmc_d_q3_a_bitmap[" not in off.source()    # (the A-bitmap build's own u64 cache stays)
+    assert off.get_launch("Q3_count_b")[0] == 256 and off.get_launch("Q3_build_b_table")[0] == 256
 
 
 def test_join_bitmap_lds_cache_only_in_batched_bodies():
