@@ -433,30 +433,32 @@ def bench_batch(ctx, steps: int) -> dict:
     plain_ms = [w.run_plain() for _ in range(steps)]
     plain = w.results()
     best = None
-    for block in (256, 512, 1024):
-        for chunk in (32768, 65536, 131072):
-            b = w.build_batch(block, chunk)
-            b.run(graph=True)
-            ms_each, dev_each = [], []
-            for _ in range(steps):      # host wall time, as the plain form's
-                t0 = time.perf_counter()
-                dev_each.append(b.run(graph=True))   # device: events around the graph's launches
-                ms_each.append((time.perf_counter() - t0) * 1e3)
-            ms = sorted(ms_each)
-            ok = all(all(np.array_equal(np.asarray(g[k]), np.asarray(e[k])) for k in e)
-                     for g, e in zip(w.results(), plain))
-            all_ok = ok and (best is None or best[6])
-            if best is None or ms[len(ms) // 2] < best[1]:
-                best = (block, ms[len(ms) // 2], ok, b.grid, b.work_items, chunk, all_ok, ms_each, dev_each)
-            else:
-                best = best[:6] + (all_ok,) + best[7:]
-            b.close()
+    shapes = [(block, chunk, fold) for block in (256, 512, 1024) for chunk in (32768, 65536, 131072)
+              for fold in (0, 32768) if fold < chunk]     # fold: rows per item of the partial-table fold level
+    for block, chunk, fold in shapes:
+        b = w.build_batch(block, chunk, fold_rows=fold)
+        b.run(graph=True)
+        ms_each, dev_each = [], []
+        for _ in range(steps):      # host wall time, as the plain form's
+            t0 = time.perf_counter()
+            dev_each.append(b.run(graph=True))   # device: events around the graph's launches
+            ms_each.append((time.perf_counter() - t0) * 1e3)
+        ms = sorted(ms_each)
+        ok = all(all(np.array_equal(np.asarray(g[k]), np.asarray(e[k])) for k in e)
+                 for g, e in zip(w.results(), plain))
+        all_ok = ok and (best is None or best[6])
+        if best is None or ms[len(ms) // 2] < best[1]:
+            best = (block, ms[len(ms) // 2], ok, b.grid, b.work_items, chunk, all_ok, ms_each, dev_each, fold)
+        else:
+            best = best[:6] + (all_ok,) + best[7:]
+        b.close()
     pm = ST.lower_median(plain_ms)
     col_bytes = sum(s.n * (17 if s.kind == "agg" else 16) for s in specs)   # join rows: 12 B + a 4 B bitmap word
     out = {"workload": f"64 pipelines (32 filter+group-by, 32 semi/anti-join), {rows} rows", "rows": rows,
            "plain": {"ms_per_step": pm, "rows_per_s": rows / (pm / 1e3), "trials": {"ms": ST.trial_stats(plain_ms)}},
            "batched": {"ms_per_step": best[1], "rows_per_s": rows / (best[1] / 1e3), "block": best[0],
-                       "grid": best[3], "work_items": best[4], "chunk_rows": best[5], "graph": True,
+                       "grid": best[3], "work_items": best[4], "chunk_rows": best[5],
+                       "fold_rows": best[9] or best[5], "graph": True,
                        "kernels": "gdb_batch_kernel (scans) + gdb_batch_kernel_l1 (group-by partial merge)",
                        "trials": {"ms": ST.trial_stats(best[7]), "device_ms": ST.trial_stats(best[8])},
                        "vs_plain": ST.compare(best[7], plain_ms)},

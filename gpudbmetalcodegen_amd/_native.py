@@ -216,6 +216,7 @@ def load() -> tuple[C.CDLL, C.CDLL]:
     _sig(eng, "gdb_batch_create", [vp, P(vp)])
     _sig(eng, "gdb_batch_add", [vp, vp, vp, i32, P(i32)])
     _sig(eng, "gdb_batch_build", [vp, i32, u64, i32, P(C.c_uint), P(u64)])
+    _sig(eng, "gdb_batch_set_level_chunk", [vp, i32, u64])
     _sig(eng, "gdb_batch_source", [vp, vp, sz, P(sz)])
     _sig(eng, "gdb_batch_kernel_source", [P(vp), P(i32), i32, vp, sz, P(sz)])
     _sig(eng, "gdb_batch_capture", [vp])

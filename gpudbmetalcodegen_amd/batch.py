@@ -255,8 +255,10 @@ class Workload3:
         return out
 
     # --- batched form ----------------------------------------------------
-    def build_batch(self, block: int, chunk_rows: int = 32768, grid: int = 0) -> "Batch":
-        return Batch(self, block, chunk_rows, grid)
+    def build_batch(self, block: int, chunk_rows: int = 32768, grid: int = 0, fold_rows: int = 0) -> "Batch":
+        """fold_rows > 0: rows per work item of the partial-table fold level
+        (phase level 1) instead of chunk_rows."""
+        return Batch(self, block, chunk_rows, grid, fold_rows)
 
     def close(self):
         for t in self.tables:
@@ -269,7 +271,7 @@ class Workload3:
 
 
 class Batch:
-    def __init__(self, w: Workload3, block: int, chunk_rows: int, grid: int):
+    def __init__(self, w: Workload3, block: int, chunk_rows: int, grid: int, fold_rows: int = 0):
         self.w = w
         self._h = C.c_void_p()
         N.eng_check(_e().gdb_batch_create(w.ctx.handle, C.byref(self._h)))
@@ -278,6 +280,8 @@ class Batch:
             idx = C.c_int()
             N.eng_check(_e().gdb_batch_add(self._h, plan.handle, w.execs[i].handle, w.launch_for(i, True)[1],
                                            C.byref(idx)))
+        if fold_rows:
+            N.eng_check(_e().gdb_batch_set_level_chunk(self._h, 1, fold_rows))
         g, nitems = C.c_uint(), C.c_uint64()
         N.eng_check(_e().gdb_batch_build(self._h, block, chunk_rows, grid, C.byref(g), C.byref(nitems)))
         self.grid, self.work_items, self.block = g.value, nitems.value, block
@@ -316,6 +320,7 @@ def main(argv=None) -> int:
     ap.add_argument("--replays", type=int, default=20)
     ap.add_argument("--block", type=int, default=1024)
     ap.add_argument("--chunk", type=int, default=65536)
+    ap.add_argument("--fold", type=int, default=32768, help="rows per work item of the fold level (0: --chunk)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default body settings")
     ap.add_argument("--set", default="", metavar="AGG_ITEMS,LDS_BITS,JOIN_ITEMS",
                     help="batched body settings given instead of tuned (e.g. 4,12,2)")
@@ -334,13 +339,14 @@ def main(argv=None) -> int:
         elif not args.no_tune:
             w.autotune(trials=3)
             tuned = w.tune_batch(args.block, 32768)["chosen"]
-        b = w.build_batch(args.block, args.chunk)
+        b = w.build_batch(args.block, args.chunk, fold_rows=args.fold)
         try:
             b.run(graph=True)
             ms = [b.run(graph=True) for _ in range(args.replays)]
         finally:
             b.close()
-        print(json.dumps({"rows": w.total_rows, "block": args.block, "chunk_rows": args.chunk, "grid": b.grid,
+        print(json.dumps({"rows": w.total_rows, "block": args.block, "chunk_rows": args.chunk, "fold_rows": args.fold,
+                          "grid": b.grid,
                           "work_items": b.work_items, "batch_tuned": tuned,
                           "device_ms_median": sorted(ms)[len(ms) // 2], "device_ms": [round(x, 4) for x in ms]}),
               flush=True)

@@ -730,6 +730,13 @@ gdb_status gdb_batch_build(gdb_batch b, int block, uint64_t chunk_rows, int grid
     });
 }
 
+gdb_status gdb_batch_set_level_chunk(gdb_batch b, int level, uint64_t rows) {
+    return guard([&]() -> gdb_status {
+        bat(b).setLevelChunk(level, rows);
+        return GDB_OK;
+    });
+}
+
 gdb_status gdb_batch_source(gdb_batch b, char* buf, size_t cap, size_t* need) {
     return guard([&]() -> gdb_status { return copyOut(bat(b).source(), buf, cap, need); });
 }

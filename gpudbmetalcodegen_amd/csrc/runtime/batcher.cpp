@@ -244,7 +244,11 @@ void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
             const auto& p = pipes_[i];
             const uint64_t rows = p.rows[static_cast<size_t>(lv)];
             const uint64_t tile = static_cast<uint64_t>(block) * static_cast<uint64_t>(p.items[static_cast<size_t>(lv)]);
-            const uint64_t chunk = tile * std::max<uint64_t>(1, (chunkRows + tile - 1) / tile);
+            const uint64_t want =
+                static_cast<size_t>(lv) < levelChunk_.size() && levelChunk_[static_cast<size_t>(lv)] > 0
+                    ? levelChunk_[static_cast<size_t>(lv)]
+                    : chunkRows;
+            const uint64_t chunk = tile * std::max<uint64_t>(1, (want + tile - 1) / tile);
             for (uint64_t b = 0; b < rows; b += chunk) {
                 L.itemPipe.push_back(static_cast<uint32_t>(i));
                 L.itemBegin.push_back(b);
@@ -302,6 +306,12 @@ void PipelineBatch::build(int block, uint64_t chunkRows, int grid) {
         gdb_graph_destroy(graph_);
         graph_ = nullptr;
     }
+}
+
+void PipelineBatch::setLevelChunk(int level, uint64_t rows) {
+    if (level < 0 || level > 64) throw std::runtime_error("PipelineBatch: bad level");
+    if (levelChunk_.size() <= static_cast<size_t>(level)) levelChunk_.resize(static_cast<size_t>(level) + 1, 0);
+    levelChunk_[static_cast<size_t>(level)] = rows;
 }
 
 void PipelineBatch::enqueue() {

@@ -80,6 +80,10 @@ public:
     // chunkRows: target rows per work item (rounded to whole tiles); grid:
     // 0 = CUs x occupancy.
     void build(int block, uint64_t chunkRows, int grid);
+    // Rows per work item of one phase level from the next build() on,
+    // instead of chunkRows (0: back to chunkRows). A fold level over partial
+    // tables has few rows per pipeline: smaller items keep more CUs busy.
+    void setLevelChunk(int level, uint64_t rows);
     // Fills of every pipeline's buffers + the fused kernels, on the stream.
     void enqueue();
     void capture();                         // record enqueue() into a graph
@@ -125,6 +129,7 @@ private:
     std::string source_;
     int block_ = 256;
     std::vector<Level> levels_;
+    std::vector<uint64_t> levelChunk_;      // per level; 0 = chunkRows
     void* dTicket_ = nullptr;
     // every pipeline's buffer initialisations and the work ticket, as a fill
     // table in device memory (the buffers are fixed at build): one launch per

@@ -241,6 +241,10 @@ gdb_status gdb_batch_create(gdb_ctx ctx, gdb_batch* out);
 gdb_status gdb_batch_add(gdb_batch b, gdb_plan p, gdb_executor ex, int items, int* index);
 gdb_status gdb_batch_build(gdb_batch b, int block, uint64_t chunk_rows, int grid, unsigned* grid_out,
                            uint64_t* work_items);
+// Rows per work item of phase level `level` for the next build (0: chunk_rows
+// as for the other levels), e.g. smaller items for a fold level over
+// partial tables, which has few rows per pipeline.
+gdb_status gdb_batch_set_level_chunk(gdb_batch b, int level, uint64_t rows);
 gdb_status gdb_batch_source(gdb_batch b, char* buf, size_t cap, size_t* need);
 // Source of the batched kernel for pipelines plans[i] at items[i], without a
 // device (offline compilation with gdb_module_precompile, inspection).

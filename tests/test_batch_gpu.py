@@ -84,6 +84,26 @@ def test_autotune_then_batch(w3):
         b.close()
 
 
+@pytest.mark.parametrize("fold_rows", [4096, 12288])
+def test_fold_level_work_item_size(w3, fold_rows):
+    """Rows per work item of the partial-table fold level set apart from the
+    scan level's: more (smaller) fold items, the same results."""
+    w3.run_plain()
+    plain = w3.results()
+    base = w3.build_batch(256, 8192)
+    n0 = base.work_items
+    base.close()
+    b = w3.build_batch(256, 8192, fold_rows=fold_rows)
+    try:
+        assert b.work_items >= n0 if fold_rows < 8192 else b.work_items <= n0
+        for _ in range(2):
+            b.run(graph=True)
+            for got, exp in zip(w3.results(), plain):
+                assert same(got, exp)
+    finally:
+        b.close()
+
+
 def test_batch_tuning_keeps_plain_settings_and_results(w3):
     """tune_batch picks the batched bodies' settings on the batched replay;
     the plain form keeps its own, and both forms still equal the oracle."""
