@@ -319,7 +319,7 @@ def main(argv=None) -> int:
     ap = argparse.ArgumentParser(prog="python -m gpudbmetalcodegen_amd.batch")
     ap.add_argument("--replays", type=int, default=20)
     ap.add_argument("--block", type=int, default=1024)
-    ap.add_argument("--chunk", type=int, default=65536)
+    ap.add_argument("--chunk", type=int, default=131072)
     ap.add_argument("--fold", type=int, default=32768, help="rows per work item of the fold level (0: --chunk)")
     ap.add_argument("--no-tune", action="store_true", help="keep the default body settings")
     ap.add_argument("--set", default="", metavar="AGG_ITEMS,LDS_BITS,JOIN_ITEMS",
