@@ -181,7 +181,7 @@ def test_q3_a_bitmap_prefix_in_plain_launches_only_when_asked():
     assert on.get_launch("Q3_count_b")[0] == 1024 and on.get_launch("Q3_build_b_table")[0] == 1024
     off = E.build_hip_plan("Q3")
     assert "u32 _bmc_d_q3_a_bitmap[" not in off.source()    # (the A-bitmap build's own u64 cache stays)
-    assert off.get_launch("Q3_count_b")[0] == 256 and off.get_launch("Q3_build_b_table")[0] == 256
+    assert off.get_launch("Q3_count_b")[0] == 256 and off.get_launch("Q3_build_b_table")[0] == 512
 
 
 def test_join_bitmap_lds_cache_only_in_batched_bodies():
